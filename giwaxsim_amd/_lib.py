@@ -1,0 +1,137 @@
+"""ctypes binding of the C ABI declared in ``include/giwaxsim_amd.h``.
+
+The library is ``giwaxsim_amd/lib/libgiwaxsim_amd.so`` (built by
+``giwaxsim_amd/build.py``).  There is no fallback: if the library is missing
+or fails to load, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_void_p
+
+from .build import LIB
+
+GX_OK = 0
+GX_FP_STRIDE = 24
+GX_MOMENTS_WORK = 774
+GX_FIN_MIRROR = 1
+GX_FIN_FLOOR_SCALE = 2
+
+# frame record offsets (enum in include/giwaxsim_amd.h)
+(FP_SIN, FP_COS, FP_HX_START, FP_HX_STEP, FP_HX_STOP, FP_HY_START, FP_HY_STEP, FP_HY_STOP,
+ FP_LEN_MODE, FP_HOR, FP_VER, FP_STOP1, FP_STOP2, FP_S12, FP_MID, FP_A, FP_TP, FP_VC, FP_TT,
+ FP_CP, FP_CT, FP_JLO, FP_JHI, FP_UNUSED) = range(24)
+
+
+class GxError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+
+class GxSlab(ctypes.Structure):
+    _fields_ = [
+        ("n_atoms", c_int64), ("n_elem", c_int64), ("N", c_int64), ("r", c_double),
+        ("zmin", c_double),
+        ("x", c_void_p), ("y", c_void_p), ("z", c_void_p), ("elem", c_void_p),
+        ("f_table", c_void_p),
+        ("sx", c_void_p), ("sy", c_void_p), ("sq", c_void_p), ("szrow", c_void_p),
+        ("row_off", c_void_p), ("row_exp", c_void_p), ("work", c_void_p),
+    ]
+
+
+class GxFrames(ctypes.Structure):
+    _fields_ = [
+        ("N", c_int64), ("r", c_double), ("n_atoms", c_int64),
+        ("ax", c_void_p), ("ay", c_void_p), ("aq", c_void_p), ("azrow", c_void_p),
+        ("row_off", c_void_p), ("row_exp", c_void_p),
+        ("fill_bkg", c_int64), ("smooth", c_int64), ("max_voxels", c_double),
+        ("avg_re", c_double), ("avg_im", c_double), ("gauss_w", c_void_p),
+        ("smooth_radius", c_int64),
+        ("qmin", c_double), ("dq", c_double), ("c_lo", c_int64), ("c_n", c_int64),
+        ("q_val", c_double), ("i_lo", c_int64), ("i_hi", c_int64), ("W", c_int64),
+        ("bin_scale", c_double), ("twiddle", c_void_p), ("n_frames", c_int64),
+        ("fparams", c_void_p), ("batch", c_int64),
+        ("ymin_key", c_void_p), ("idx_bounds", c_void_p), ("col_nm", c_void_p),
+        ("col_sy", c_void_p), ("col_sz", c_void_p), ("inter", c_void_p),
+        ("bin_sum", c_void_p), ("bin_cnt", c_void_p), ("status", c_void_p),
+    ]
+
+
+class GxDetector(ctypes.Structure):
+    _fields_ = [
+        ("n_pix", c_int64), ("n_init", c_int64), ("n_orient", c_int64),
+        ("h_axis", c_void_p), ("v_axis", c_void_p), ("init_R", c_void_p),
+        ("orient_R", c_void_p), ("orient_w", c_void_p), ("iq", c_void_p), ("iq_f64", c_int64),
+        ("cy", c_int64), ("cx", c_int64), ("cz", c_int64),
+        ("qx_min", c_double), ("qy_min", c_double), ("qz_min", c_double), ("dq", c_double),
+        ("det", c_void_p),
+    ]
+
+
+_P = c_void_p
+_D = c_double
+_I = c_int64
+# name -> argtypes (all return int status except the first three)
+SIGNATURES = {
+    "gx_slab_bounds": [POINTER(GxSlab), _P, _P, _P],
+    "gx_slab_sort_rows": [POINTER(GxSlab), _P],
+    "gx_frames_stats": [POINTER(GxFrames), _I, _I, _P],
+    "gx_frames_columns": [POINTER(GxFrames), _I, _I, _P],
+    "gx_frames_rowpass": [POINTER(GxFrames), _I, _I, _P],
+    "gx_frames_colpass": [POINTER(GxFrames), _I, _I, _P],
+    "gx_frames_forward": [POINTER(GxFrames), _I, _I, _P],
+    "gx_bins_clear": [_P, _P, _I, _P],
+    "gx_bins_finalize": [_P, _P, _I, _D, _I, _I, _D, _D, _D, POINTER(c_double), _D, _P, _P, _P],
+    "gx_detector_accumulate": [POINTER(GxDetector), _P],
+    "gx_detector_finalize": [_P, _I, _I, _I, _P, _P],
+    "gx_rotate_points": [POINTER(c_double), _P, _P, _P, _I, _P, _P, _P, _P],
+    "gx_intersect": [_P, _I, _I, _I, _D, _D, _D, _D, _P, _P, _P, _I, _D, _I, _P, _P],
+    "gx_bin_frame_f64": [_P, _I, _I, _P, _P, _P, POINTER(c_double), _D, _I, _P, _P, _P, _P],
+    "gx_collapse_lengths": [POINTER(c_double), _P, _I, _P, _P],
+    "gx_scale_offset_moments": [_P, _P, _P, _I, _P, POINTER(c_double), _P],
+    "gx_apply_f0": [_P, _I, _I, _I, _P, _P, _P, POINTER(c_double), _D, _P, _P],
+    "gx_bins_accumulate_f64": [_P, _P, _I, _D, _P, _P, _P],
+}
+EXPORTS = ["gx_last_error", "gx_abi_version", "gx_supported_grid"] + list(SIGNATURES)
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load (once) and return the ctypes library with argtypes applied."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB
+    if not os.path.exists(path):
+        raise GxError(f"{path} not found: build it with `python -m giwaxsim_amd.build` "
+                      "(or __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    lib.gx_last_error.restype = c_char_p
+    lib.gx_last_error.argtypes = []
+    lib.gx_abi_version.restype = c_int
+    lib.gx_abi_version.argtypes = []
+    lib.gx_supported_grid.restype = c_int
+    lib.gx_supported_grid.argtypes = [c_int64]
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = c_int
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def call(name: str, *args):
+    """Call an entry point and raise GxError with the library's message on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != GX_OK:
+        msg = lib.gx_last_error().decode(errors="replace")
+        raise GxError(f"{name} failed ({rc}): {msg}")
+    return rc
+
+
+def dbl_array(values):
+    arr = (c_double * len(values))(*[float(v) for v in values])
+    return arr

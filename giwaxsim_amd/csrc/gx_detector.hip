@@ -1,0 +1,408 @@
+// gfx950 kernels of the detector stage (reference: tools/detector.py and
+// tools/comparison.py detectormaker_fitting 790-870) and the small
+// compatibility kernels behind the reference's per-call helpers.
+#include "gx_device.h"
+#include "gx_host.h"
+
+#include <math.h>
+#include <mutex>
+#include <string>
+
+using namespace gx;
+
+// ------------------------------------------------------------------------------------
+// error state (one message per host thread)
+// ------------------------------------------------------------------------------------
+namespace gx {
+static thread_local std::string g_err;
+
+int set_error(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+void clear_error() { g_err.clear(); }
+}  // namespace gx
+
+extern "C" const char *gx_last_error(void) { return gx::g_err.c_str(); }
+
+namespace {
+
+constexpr int BLK = 256;
+
+// canonical R @ (x, y, z): row i = (R[i,0] x + R[i,1] y) + R[i,2] z, no FMA
+__device__ __forceinline__ void rot3(const double *R, double &x, double &y, double &z) {
+  const double xr = R[0] * x + R[1] * y + R[2] * z;
+  const double yr = R[3] * x + R[4] * y + R[5] * z;
+  const double zr = R[6] * x + R[7] * y + R[8] * z;
+  x = xr;
+  y = yr;
+  z = zr;
+}
+
+__device__ __forceinline__ long long clip_idx(long long i, long long hi) { return i < 0 ? 0 : (i > hi ? hi : i); }
+
+// Orientation-summed detector image.  One thread per pixel; the orientation
+// loop runs in (psi outer, phi, theta inner) order with an FP64 accumulator,
+// so the sum is deterministic (the reference's `det_ints += det_int` from
+// worker threads is racy, tools/detector.py:298).
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
+  const T *__restrict__ iq = reinterpret_cast<const T *>(d.iq);
+  const int64_t npx = d.n_pix * d.n_pix;
+  const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (p >= npx) return;
+  const int64_t row = p / d.n_pix, col = p - row * d.n_pix;
+  double x0 = 0.0, y0 = d.h_axis[col], z0 = d.v_axis[row];  // make_detector, detector.py:24-27
+  for (int k = 0; k < d.n_init; ++k) rot3(d.init_R + 9 * k, x0, y0, z0);
+  const double dqinv = 1.0 / d.dq;
+  const long long cxm = d.cx - 1, cym = d.cy - 1, czm = d.cz - 1;
+  double acc = 0.0;
+  for (int64_t o = 0; o < d.n_orient; ++o) {
+    const double *R = d.orient_R + 27 * o;
+    double x = x0, y = y0, z = z0;
+    rot3(R, x, y, z);        // rotate_about_normal(psi)
+    rot3(R + 9, x, y, z);    // rotate_about_vertical(phi)
+    rot3(R + 18, x, y, z);   // rotate_about_horizontal(theta)
+    const long long xi = clip_idx(to_int(floor_div(x - d.qx_min, d.dq, dqinv)), cxm);
+    const long long yi = clip_idx(to_int(floor_div(y - d.qy_min, d.dq, dqinv)), cym);
+    const long long zi = clip_idx(to_int(floor_div(z - d.qz_min, d.dq, dqinv)), czm);
+    const double v = (double)iq[(yi * d.cx + xi) * d.cz + zi];
+    acc = acc + v * d.orient_w[o];
+  }
+  d.det[p] = acc;
+}
+
+// mirror_vertical_horizontal + NaN/<=0 floor + 1e-6 scale
+__global__ __launch_bounds__(BLK) void k_detector_finalize(const double *__restrict__ q, int64_t nr, int64_t nc,
+                                                           int64_t flags, double *__restrict__ out) {
+  const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (p >= nr * nc) return;
+  const int64_t r = p / nc, c = p - r * nc;
+  const int64_t rr = nr - 1 - r, cc = nc - 1 - c;
+  double v = q[p];
+  if (flags & GX_FIN_MIRROR) {
+    v = ((q[r * nc + c] + q[r * nc + cc]) + q[rr * nc + c]) + q[rr * nc + cc];
+    if ((nr & 1) && r == nr / 2) v = (q[r * nc + c] + q[r * nc + cc]) * 2.0;
+    if ((nc & 1) && c == nc / 2) v = (q[r * nc + c] + q[rr * nc + c]) * 2.0;
+    if ((nr & 1) && (nc & 1) && r == nr / 2 && c == nc / 2) v = q[r * nc + c] * 4.0;
+  }
+  if (flags & GX_FIN_FLOOR_SCALE) {
+    if (v != v) v = 1e-6;
+    if (v <= 0.0) v = 1e-6;
+    v = v * 1e-6;
+  }
+  out[p] = v;
+}
+
+struct Mat9 {
+  double m[9];
+};
+
+__global__ __launch_bounds__(BLK) void k_rotate_points(Mat9 R, const double *x, const double *y, const double *z,
+                                                       int64_t n, double *ox, double *oy, double *oz) {
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    double a = x[i], b = y[i], c = z[i];
+    rot3(R.m, a, b, c);
+    ox[i] = a;
+    oy[i] = b;
+    oz[i] = c;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_intersect(const double *iq, int64_t cy, int64_t cx, int64_t cz,
+                                                   double qx_min, double qy_min, double qz_min, double dq,
+                                                   const double *x, const double *y, const double *z, int64_t n,
+                                                   double weight, int64_t accumulate, double *out) {
+  const double dqinv = 1.0 / dq;
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    const long long xi = clip_idx(to_int(floor_div(x[i] - qx_min, dq, dqinv)), cx - 1);
+    const long long yi = clip_idx(to_int(floor_div(y[i] - qy_min, dq, dqinv)), cy - 1);
+    const long long zi = clip_idx(to_int(floor_div(z[i] - qz_min, dq, dqinv)), cz - 1);
+    const double v = iq[(yi * cx + xi) * cz + zi] * weight;
+    out[i] = accumulate ? out[i] + v : v;
+  }
+}
+
+struct QLim {
+  double v[6];
+};
+
+__global__ __launch_bounds__(BLK) void k_bin_frame_f64(const double *iq2d, int64_t nv, int64_t nh, const double *hqx,
+                                                       const double *hqy, const double *vqz, QLim L, double dq,
+                                                       int64_t q_num, double *gsum, double *gcnt, int32_t *status) {
+  const double dqinv = 1.0 / dq;
+  for (int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x; p < nv * nh; p += (int64_t)gridDim.x * BLK) {
+    const int64_t i = p / nh, j = p - i * nh;
+    const double qx = hqx[j], qy = hqy[j], qz = vqz[i];
+    if (!(qx <= L.v[1] && qx >= L.v[0] && qy <= L.v[3] && qy >= L.v[2])) continue;
+    if (!(qz <= L.v[5] && qz >= L.v[4])) continue;
+    const long long xi = to_int(floor_div(qx - L.v[0], dq, dqinv));
+    const long long yi = to_int(floor_div(qy - L.v[2], dq, dqinv));
+    const long long zi = to_int(floor_div(qz - L.v[4], dq, dqinv));
+    if (xi < 0 || xi >= q_num || yi < 0 || yi >= q_num || zi < 0 || zi >= q_num) {
+      atomicAdd(&status[0], 1);
+      continue;
+    }
+    const int64_t b = (yi * q_num + xi) * q_num + zi;
+    atomicAdd(&gsum[b], iq2d[p]);
+    atomicAdd(&gcnt[b], 1.0);
+  }
+}
+
+struct FP {
+  double v[GX_FP_STRIDE];
+};
+
+__global__ __launch_bounds__(BLK) void k_collapse_lengths(FP fp, const double *x, int64_t n, double *out) {
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK)
+    out[i] = collapse_length(fp.v, x[i]);
+}
+
+struct Aff9 {
+  double a[9];
+};
+
+// iq * ((f0(|q|) / Z)^2) over a [ny][nx][nz] grid (add_f0_q_3d)
+__global__ __launch_bounds__(BLK) void k_apply_f0(const double *iq, int64_t ny, int64_t nx, int64_t nz, const double *qx,
+                                                  const double *qy, const double *qz, Aff9 aff, double z_elem,
+                                                  double c16pi2, double *out) {
+  const int64_t n = ny * nx * nz;
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    const int64_t iz = i % nz, ix = (i / nz) % nx, iy = i / (nz * nx);
+    const double a = qx[ix], b = qy[iy], c = qz[iz];
+    const double q2 = (a * a + b * b) + c * c;
+    double f0 = aff.a[0] * exp(-aff.a[1] * q2 / c16pi2);
+    f0 = f0 + aff.a[2] * exp(-aff.a[3] * q2 / c16pi2);
+    f0 = f0 + aff.a[4] * exp(-aff.a[5] * q2 / c16pi2);
+    f0 = f0 + aff.a[6] * exp(-aff.a[7] * q2 / c16pi2);
+    f0 = f0 + aff.a[8];
+    const double fn = f0 / z_elem;
+    out[i] = iq[i] * (fn * fn);
+  }
+}
+
+// grid_sum += bin_sum / scale, grid_cnt += bin_cnt
+__global__ __launch_bounds__(BLK) void k_bins_accumulate_f64(const unsigned long long *bs, const int32_t *bc, int64_t n,
+                                                             double inv_scale, double *gs, double *gc) {
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    gs[i] = gs[i] + (double)bs[i] * inv_scale;
+    gc[i] = gc[i] + (double)bc[i];
+  }
+}
+
+// deterministic two-level FP64 reductions for the scale/offset fit
+constexpr int MOM_BLOCKS = 256;
+
+__device__ double block_sum(double v, double *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, GX_WAVE);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < BLK / GX_WAVE; ++i) t += sh[i];
+  return t;
+}
+
+__global__ __launch_bounds__(BLK) void k_mom1(const double *s, const double *y, const uint8_t *mask, int64_t n,
+                                              double *part) {
+  __shared__ double sh[BLK / GX_WAVE];
+  double c = 0, ss = 0, sy = 0;
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)MOM_BLOCKS * BLK)
+    if (mask[i] == 0) {
+      c += 1.0;
+      ss += s[i];
+      sy += y[i];
+    }
+  double a = block_sum(c, sh), b = block_sum(ss, sh), e = block_sum(sy, sh);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = a;
+    part[3 * blockIdx.x + 1] = b;
+    part[3 * blockIdx.x + 2] = e;
+  }
+}
+
+__global__ void k_mom1_fin(double *part, double *res) {
+  if (threadIdx.x == 0) {
+    double c = 0, a = 0, b = 0;
+    for (int i = 0; i < MOM_BLOCKS; ++i) {
+      c += part[3 * i];
+      a += part[3 * i + 1];
+      b += part[3 * i + 2];
+    }
+    res[0] = c;
+    res[1] = c > 0 ? a / c : 0.0;
+    res[2] = c > 0 ? b / c : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_mom2(const double *s, const double *y, const uint8_t *mask, int64_t n,
+                                              const double *res, double *part) {
+  __shared__ double sh[BLK / GX_WAVE];
+  const double ms = res[1], my = res[2];
+  double a = 0, b = 0;
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)MOM_BLOCKS * BLK)
+    if (mask[i] == 0) {
+      const double ds = s[i] - ms, dy = y[i] - my;
+      a += ds * ds;
+      b += ds * dy;
+    }
+  double ta = block_sum(a, sh), tb = block_sum(b, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ta;
+    part[2 * blockIdx.x + 1] = tb;
+  }
+}
+
+__global__ void k_mom2_fin(const double *part, double *res) {
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int i = 0; i < MOM_BLOCKS; ++i) {
+      a += part[2 * i];
+      b += part[2 * i + 1];
+    }
+    res[3] = a;
+    res[4] = b;
+    res[5] = 0.0;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gx_detector_accumulate(const gx_detector *d, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(d != nullptr, "NULL detector");
+  GX_REQUIRE(d->n_pix >= 1 && d->n_pix <= 65536, "n_pix=%lld", (long long)d->n_pix);
+  GX_REQUIRE(d->n_init >= 0 && d->n_init <= 3, "n_init=%lld outside [0, 3]", (long long)d->n_init);
+  GX_REQUIRE(d->n_orient >= 0, "n_orient");
+  GX_REQUIRE(d->cy >= 1 && d->cx >= 1 && d->cz >= 1, "empty iq grid");
+  GX_REQUIRE(d->dq > 0.0, "dq must be positive");
+  GX_REQUIRE(d->h_axis && d->v_axis && d->iq && d->det && (d->n_init == 0 || d->init_R) &&
+                 (d->n_orient == 0 || (d->orient_R && d->orient_w)),
+             "a detector buffer is NULL");
+  const int64_t npx = d->n_pix * d->n_pix;
+  if (d->iq_f64)
+    k_detector<double><<<grid1d(npx, BLK), BLK, 0, as_stream(stream)>>>(*d);
+  else
+    k_detector<float><<<grid1d(npx, BLK), BLK, 0, as_stream(stream)>>>(*d);
+  return check_launch("gx_detector_accumulate");
+}
+
+int gx_detector_finalize(const double *det, int64_t n_rows, int64_t n_cols, int64_t flags, double *out,
+                         gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(det && out && det != out, "det/out must be distinct non-NULL buffers");
+  GX_REQUIRE(n_rows >= 1 && n_cols >= 1, "empty image");
+  k_detector_finalize<<<grid1d(n_rows * n_cols, BLK), BLK, 0, as_stream(stream)>>>(det, n_rows, n_cols, flags,
+                                                                                   out);
+  return check_launch("gx_detector_finalize");
+}
+
+int gx_rotate_points(const double *host_R, const double *x, const double *y, const double *z, int64_t n, double *ox,
+                     double *oy, double *oz, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(host_R && x && y && z && ox && oy && oz, "NULL argument");
+  GX_REQUIRE(n >= 0, "n");
+  if (n == 0) return GX_OK;
+  Mat9 R;
+  for (int i = 0; i < 9; ++i) R.m[i] = host_R[i];
+  unsigned g = grid1d(n, BLK);
+  if (g > 16384) g = 16384;
+  k_rotate_points<<<g, BLK, 0, as_stream(stream)>>>(R, x, y, z, n, ox, oy, oz);
+  return check_launch("gx_rotate_points");
+}
+
+int gx_intersect(const double *iq, int64_t cy, int64_t cx, int64_t cz, double qx_min, double qy_min, double qz_min,
+                 double dq, const double *x, const double *y, const double *z, int64_t n, double weight,
+                 int64_t accumulate, double *out, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(iq && x && y && z && out, "NULL argument");
+  GX_REQUIRE(cy >= 1 && cx >= 1 && cz >= 1 && dq > 0.0 && n >= 0, "bad sizes");
+  if (n == 0) return GX_OK;
+  unsigned g = grid1d(n, BLK);
+  if (g > 16384) g = 16384;
+  k_intersect<<<g, BLK, 0, as_stream(stream)>>>(iq, cy, cx, cz, qx_min, qy_min, qz_min, dq, x, y, z, n, weight,
+                                                 accumulate, out);
+  return check_launch("gx_intersect");
+}
+
+int gx_bin_frame_f64(const double *iq2d, int64_t nv, int64_t nh, const double *hqx, const double *hqy,
+                     const double *vqz, const double *host_qlim, double dq, int64_t q_num, double *grid_sum,
+                     double *grid_cnt, int32_t *status, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(iq2d && hqx && hqy && vqz && host_qlim && grid_sum && grid_cnt && status, "NULL argument");
+  GX_REQUIRE(nv >= 0 && nh >= 0 && q_num >= 1 && dq > 0.0, "bad sizes");
+  if (nv * nh == 0) return GX_OK;
+  QLim L;
+  for (int i = 0; i < 6; ++i) L.v[i] = host_qlim[i];
+  unsigned g = grid1d(nv * nh, BLK);
+  if (g > 16384) g = 16384;
+  k_bin_frame_f64<<<g, BLK, 0, as_stream(stream)>>>(iq2d, nv, nh, hqx, hqy, vqz, L, dq, q_num, grid_sum, grid_cnt,
+                                                     status);
+  return check_launch("gx_bin_frame_f64");
+}
+
+int gx_collapse_lengths(const double *host_fp, const double *x, int64_t n, double *out, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(host_fp && x && out && n >= 0, "bad argument");
+  if (n == 0) return GX_OK;
+  FP fp;
+  for (int i = 0; i < GX_FP_STRIDE; ++i) fp.v[i] = host_fp[i];
+  k_collapse_lengths<<<grid1d(n, BLK), BLK, 0, as_stream(stream)>>>(fp, x, n, out);
+  return check_launch("gx_collapse_lengths");
+}
+
+int gx_apply_f0(const double *iq, int64_t ny, int64_t nx, int64_t nz, const double *qx, const double *qy,
+                const double *qz, const double *host_aff, double z_elem, double *out, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(iq && qx && qy && qz && host_aff && out, "NULL argument");
+  GX_REQUIRE(ny >= 0 && nx >= 0 && nz >= 0 && z_elem != 0.0, "bad sizes");
+  const int64_t n = ny * nx * nz;
+  if (n == 0) return GX_OK;
+  Aff9 aff;
+  for (int i = 0; i < 9; ++i) aff.a[i] = host_aff[i];
+  unsigned g = grid1d(n, BLK);
+  if (g > 16384) g = 16384;
+  k_apply_f0<<<g, BLK, 0, as_stream(stream)>>>(iq, ny, nx, nz, qx, qy, qz, aff, z_elem, 16 * (M_PI * M_PI), out);
+  return check_launch("gx_apply_f0");
+}
+
+int gx_bins_accumulate_f64(const unsigned long long *bin_sum, const int32_t *bin_cnt, int64_t n, double bin_scale,
+                           double *grid_sum, double *grid_cnt, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(bin_sum && bin_cnt && grid_sum && grid_cnt && n >= 0 && bin_scale > 0.0, "bad argument");
+  if (n == 0) return GX_OK;
+  unsigned g = grid1d(n, BLK);
+  if (g > 16384) g = 16384;
+  k_bins_accumulate_f64<<<g, BLK, 0, as_stream(stream)>>>(bin_sum, bin_cnt, n, 1.0 / bin_scale, grid_sum, grid_cnt);
+  return check_launch("gx_bins_accumulate_f64");
+}
+
+int gx_scale_offset_moments(const double *sim, const double *exp_, const uint8_t *mask, int64_t n, double *work,
+                            double *host_out, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(sim && exp_ && mask && work && host_out && n >= 0, "bad argument");
+  hipStream_t st = as_stream(stream);
+  double *part = work;                   // [3 * MOM_BLOCKS]
+  double *res = work + 3 * MOM_BLOCKS;   // [6]
+  k_mom1<<<MOM_BLOCKS, BLK, 0, st>>>(sim, exp_, mask, n, part);
+  k_mom1_fin<<<1, 64, 0, st>>>(part, res);
+  k_mom2<<<MOM_BLOCKS, BLK, 0, st>>>(sim, exp_, mask, n, res, part);
+  k_mom2_fin<<<1, 64, 0, st>>>(part, res);
+  int rc = check_launch("gx_scale_offset_moments");
+  if (rc) return rc;
+  GX_HIP(hipMemcpyAsync(host_out, res, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+  GX_HIP(hipStreamSynchronize(st));
+  return GX_OK;
+}
+
+}  // extern "C"

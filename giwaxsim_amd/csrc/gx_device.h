@@ -1,0 +1,282 @@
+// Device-side building blocks shared by the gfx950 kernels of the
+// forward-scattering path: exact FP64 floor division (the reference's numpy
+// float `//`), order-preserving keys for FP64 min/max atomics, complex FP32
+// helpers and an in-wave Stockham FFT (64 lanes per transform).
+//
+// Everything here is compiled with -ffp-contract=off so that the FP64 index
+// arithmetic rounds after every operation exactly like numpy's elementwise
+// evaluation; fused multiply-adds are written out explicitly where wanted.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/giwaxsim_amd.h"
+
+#define GX_WAVE 64
+
+namespace gx {
+
+// ------------------------------------------------------------------------------------
+// exact floor(a / b) for doubles (b > 0)
+//
+// numpy's float floor-divide (used for every voxel and bin index in the
+// reference: tools/voxelgrids.py:328-329, 497-499; tools/detector.py:217-219)
+// is fmod based and returns the exact floor of the real quotient a/b of the
+// two doubles, e.g. 1.0 // 0.2 == 4.0 while floor(1.0/0.2) == 5.0.
+// Here: q0 = floor(a * binv) is within one of the exact floor; the sign of
+// the exact residual a - q*b is obtained from a single-rounding fma (rounding
+// never flips the sign of a nonzero real), and q is corrected by one step.
+// Valid while |a/b| < 2^50, far beyond any grid index here.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double floor_div(double a, double b, double binv) {
+  double q = floor(a * binv);
+  if (fma(-q, b, a) < 0.0) {
+    q -= 1.0;
+  } else if (fma(-(q + 1.0), b, a) >= 0.0) {
+    q += 1.0;
+  }
+  return q;
+}
+
+// numpy `.astype(int)` of a floor-div result (already integral): truncation.
+__device__ __forceinline__ long long to_int(double q) { return (long long)q; }
+
+// ------------------------------------------------------------------------------------
+// order-preserving uint64 keys for doubles (for atomicMin/atomicMax)
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long dkey(double v) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dkey_inv(unsigned long long k) {
+  unsigned long long u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)u);
+}
+
+// ------------------------------------------------------------------------------------
+// wave-level synchronisation for LDS exchanges between lanes of one wave
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, GX_WAVE));
+  return v;
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, GX_WAVE));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, GX_WAVE));
+  return v;
+}
+
+// ------------------------------------------------------------------------------------
+// complex FP32
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+// multiply by -i
+__device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// ------------------------------------------------------------------------------------
+// forward DFTs of small radix in registers, W_R = exp(-2 pi i / R), natural order
+// ------------------------------------------------------------------------------------
+template <int R> struct Dft;
+
+template <> struct Dft<2> {
+  static __device__ __forceinline__ void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <> struct Dft<4> {
+  static __device__ __forceinline__ void run(float2* v) {
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = cmul_mi(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <> struct Dft<8> {
+  static __device__ __forceinline__ void run(float2* v) {
+    const float c = 0.70710678118654752440f;
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4>::run(e);
+    Dft<4>::run(o);
+    float2 w1 = make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));   // * (c, -c)
+    float2 w2 = cmul_mi(o[2]);                                               // * -i
+    float2 w3 = make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));  // * (-c, -c)
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], w1);
+    v[5] = csub(e[1], w1);
+    v[2] = cadd(e[2], w2);
+    v[6] = csub(e[2], w2);
+    v[3] = cadd(e[3], w3);
+    v[7] = csub(e[3], w3);
+  }
+};
+
+template <> struct Dft<16> {
+  static __device__ __forceinline__ void run(float2* v) {
+    // W16^k = (cos(2 pi k/16), -sin(2 pi k/16))
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+    const float c2 = 0.70710678118654752440f;
+    float2 e[8], o[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      e[i] = v[2 * i];
+      o[i] = v[2 * i + 1];
+    }
+    Dft<8>::run(e);
+    Dft<8>::run(o);
+    const float2 w[8] = {make_float2(1.f, 0.f),  make_float2(c1, -s1), make_float2(c2, -c2),
+                         make_float2(s1, -c1),   make_float2(0.f, -1.f), make_float2(-s1, -c1),
+                         make_float2(-c2, -c2), make_float2(-c1, -s1)};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      float2 t = (k == 0) ? o[0] : (k == 4 ? cmul_mi(o[4]) : cmul(o[k], w[k]));
+      v[k] = cadd(e[k], t);
+      v[k + 8] = csub(e[k], t);
+    }
+  }
+};
+
+// ------------------------------------------------------------------------------------
+// In-wave Stockham autosort FFT.  One transform of N points per wave; each of
+// the 64 lanes holds P = N/64 points in registers.  A stage of radix R with
+// Ns = product of the previous radices works on butterflies j = lane + 64 t:
+//   inputs  x[j + r N/R]  (r < R), twiddled by W_{Ns R}^{r (j mod Ns)},
+//   outputs y[(j / Ns) Ns R + (j mod Ns) + r Ns].
+// Data moves between stages through a per-wave LDS buffer of N float2.
+// The first stage's inputs are supplied by the caller in registers
+// (v[t R + r] = x[j + r N/R]); the last stage leaves its outputs in
+// registers, v[t R + r] = X[j + r N/R] with j = lane + 64 t.
+// tw[m] = exp(-2 pi i m / N), m < N, precomputed in FP64 and rounded to FP32.
+// ------------------------------------------------------------------------------------
+template <int N, int R, int NS, int P>
+__device__ __forceinline__ void fft_butterflies(float2 (&v)[P], const float2* __restrict__ tw, int lane) {
+  constexpr int NB = P / R;
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    if constexpr (NS > 1) {
+      const int j = lane + GX_WAVE * t;
+      const int k = j & (NS - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const int m = (r * k) * (N / (NS * R));
+        v[t * R + r] = cmul(v[t * R + r], tw[m]);
+      }
+    }
+    Dft<R>::run(&v[t * R]);
+  }
+}
+
+template <int N, int R, int NS, int P>
+__device__ __forceinline__ void fft_store(float2 (&v)[P], float2* lds, int lane) {
+  constexpr int NB = P / R;
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const int j = lane + GX_WAVE * t;
+    const int base = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[base + r * NS] = v[t * R + r];
+  }
+}
+
+template <int N, int R, int P>
+__device__ __forceinline__ void fft_load(float2 (&v)[P], const float2* lds, int lane) {
+  constexpr int NB = P / R;
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const int j = lane + GX_WAVE * t;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[t * R + r] = lds[j + r * (N / R)];
+  }
+}
+
+// Radix plans (first stage radix is the first template argument).
+template <int N> struct FftPlan;
+template <> struct FftPlan<256>  { static constexpr int R0 = 4,  R1 = 4,  R2 = 4, R3 = 4; static constexpr int NST = 4; };
+template <> struct FftPlan<512>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 8, R3 = 1; static constexpr int NST = 3; };
+template <> struct FftPlan<1024> { static constexpr int R0 = 16, R1 = 16, R2 = 4, R3 = 1; static constexpr int NST = 3; };
+template <> struct FftPlan<2048> { static constexpr int R0 = 16, R1 = 16, R2 = 8, R3 = 1; static constexpr int NST = 3; };
+template <> struct FftPlan<4096> { static constexpr int R0 = 16, R1 = 16, R2 = 16, R3 = 1; static constexpr int NST = 3; };
+
+template <int N> struct FftFirst { static constexpr int R = FftPlan<N>::R0; };
+template <int N> struct FftLast {
+  static constexpr int R = FftPlan<N>::NST == 4 ? FftPlan<N>::R3 : FftPlan<N>::R2;
+};
+
+// Runs the whole transform.  On entry v holds the first-stage inputs
+// (v[t R0 + r] = x[lane + 64 t + r N/R0]); on exit v holds the outputs in
+// last-stage order (v[t RL + r] = X[lane + 64 t + r N/RL]).  `lds` is this
+// wave's private buffer of N float2; it is free again on return.
+template <int N>
+__device__ __forceinline__ void fft_wave(float2 (&v)[N / GX_WAVE], float2* lds,
+                                         const float2* __restrict__ tw, int lane) {
+  constexpr int P = N / GX_WAVE;
+  using PL = FftPlan<N>;
+  constexpr int R0 = PL::R0, R1 = PL::R1, R2 = PL::R2, R3 = PL::R3;
+  fft_butterflies<N, R0, 1, P>(v, tw, lane);
+  fft_store<N, R0, 1, P>(v, lds, lane);
+  wave_sync();
+  fft_load<N, R1, P>(v, lds, lane);
+  wave_sync();
+  fft_butterflies<N, R1, R0, P>(v, tw, lane);
+  fft_store<N, R1, R0, P>(v, lds, lane);
+  wave_sync();
+  fft_load<N, R2, P>(v, lds, lane);
+  wave_sync();
+  fft_butterflies<N, R2, R0 * R1, P>(v, tw, lane);
+  if constexpr (PL::NST == 4) {
+    fft_store<N, R2, R0 * R1, P>(v, lds, lane);
+    wave_sync();
+    fft_load<N, R3, P>(v, lds, lane);
+    wave_sync();
+    fft_butterflies<N, R3, R0 * R1 * R2, P>(v, tw, lane);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// rectangular_collapse_lengths (tools/voxelgrids.py:238-309) at one x value,
+// from the per-frame record prepared by the host planner (same branch order
+// and the same FP64 operations as the reference's per-x loop).
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double collapse_length(const double *fp, double x) {
+  const int mode = (int)fp[GX_FP_LEN_MODE];
+  if (mode == 1) return (x < fp[GX_FP_HOR]) ? fp[GX_FP_VER] : 0.0;
+  if (mode == 2) return (x < fp[GX_FP_VER]) ? fp[GX_FP_HOR] : 0.0;
+  if (x == 0.0) return 0.0;
+  if (x < fp[GX_FP_STOP1]) {
+    double tot = fp[GX_FP_A] + x * fp[GX_FP_TP];
+    double miss = (fp[GX_FP_VC] - x) * fp[GX_FP_TT];
+    return tot - miss;
+  }
+  if (x <= fp[GX_FP_STOP2]) return fp[GX_FP_MID];
+  if (x < fp[GX_FP_S12]) {
+    double rem = fp[GX_FP_HOR] - ((x - fp[GX_FP_VC]) / fp[GX_FP_CP]);
+    return rem / fp[GX_FP_CT];
+  }
+  return 0.0;
+}
+
+}  // namespace gx

@@ -1,0 +1,736 @@
+// gfx950 kernels of the phi-frame pipeline (reference: tools/voxelgrids.py
+// rotate_project_fft_coords 311-416 and process_file2 464-506, driven by
+// tools/comparison.py voxelgridmaker_fitting 673-788).
+//
+// Data flow per batch of frames (all device resident):
+//   k_frames_ymin     min_y'(phi) over all atoms              (voxelgrids.py:319-323)
+//   k_frames_bounds   y/z index bounds over valid atoms       (voxelgrids.py:328-349)
+//   k_frames_columns  num_missing_voxels and smoothed masks   (voxelgrids.py:350-374)
+//   k_rowpass<N>      per z row: LDS fixed-point scatter of the row's atoms,
+//                     fill + blend, FFT along y, keep the fftshifted column band
+//                     that reaches the cropped bins, LDS-staged transposed store
+//   k_colpass<N>      per kept column: FFT along z, |F|^2, bin indices in FP64,
+//                     deterministic fixed-point bin sums + int32 counts
+#include "gx_device.h"
+#include "gx_host.h"
+
+#include <limits.h>
+#include <math.h>
+
+using namespace gx;
+
+namespace {
+
+constexpr int BLK = 256;
+
+// ------------------------------------------------------------------------------------
+// slab preparation
+// ------------------------------------------------------------------------------------
+__global__ void k_slab_bounds_init(unsigned long long *keys, unsigned long long *counts, int64_t n_elem) {
+  int t = threadIdx.x;
+  if (t < 6) keys[t] = (t % 2 == 0) ? ~0ull : 0ull;
+  for (int e = t; e < n_elem; e += blockDim.x) counts[e] = 0;
+}
+
+__global__ __launch_bounds__(BLK) void k_slab_bounds(const double *__restrict__ x, const double *__restrict__ y,
+                                                     const double *__restrict__ z, const int32_t *__restrict__ el,
+                                                     int64_t n, int64_t n_elem, unsigned long long *keys,
+                                                     unsigned long long *counts) {
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t a = blockIdx.x * (int64_t)BLK + threadIdx.x; a < n; a += (int64_t)gridDim.x * BLK) {
+    double v[3] = {x[a], y[a], z[a]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = fmin(mn[k], v[k]);
+      mx[k] = fmax(mx[k], v[k]);
+    }
+    int e = el[a];
+    if (e >= 0 && e < n_elem) atomicAdd(&counts[e], 1ull);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double a = mn[k], b = mx[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      a = fmin(a, __shfl_xor(a, o, GX_WAVE));
+      b = fmax(b, __shfl_xor(b, o, GX_WAVE));
+    }
+    if ((threadIdx.x & (GX_WAVE - 1)) == 0) {
+      atomicMin(&keys[2 * k], dkey(a));
+      atomicMax(&keys[2 * k + 1], dkey(b));
+    }
+  }
+}
+
+__global__ void k_slab_bounds_fin(unsigned long long *keys) {
+  int t = threadIdx.x;
+  if (t < 6) {
+    double v = dkey_inv(keys[t]);
+    reinterpret_cast<double *>(keys)[t] = v;
+  }
+}
+
+// work layout: [N * n_elem] per-row element counts | [N + 1] row counts | [N + 1] cursors
+__global__ __launch_bounds__(BLK) void k_slab_rowcount(gx_slab s) {
+  const int N = (int)s.N;
+  const double rinv = 1.0 / s.r;
+  int32_t *ecnt = s.work;
+  int32_t *rcnt = s.work + (int64_t)N * s.n_elem;
+  for (int64_t a = blockIdx.x * (int64_t)BLK + threadIdx.x; a < s.n_atoms; a += (int64_t)gridDim.x * BLK) {
+    double zz = s.z[a] - s.zmin;
+    long long zi = to_int(floor_div(zz, s.r, rinv));
+    int row = (zi >= 0 && zi < N) ? (int)zi : N;
+    const int e = s.elem[a];
+    if (e < 0 || e >= s.n_elem) row = N;  // unknown element ids never reach a row
+    atomicAdd(&rcnt[row], 1);
+    if (row < N) atomicAdd(&ecnt[(int64_t)row * s.n_elem + e], 1);
+  }
+}
+
+// Single block: exclusive scan of the N+1 row counts, per-row fixed-point exponent.
+__global__ __launch_bounds__(1024) void k_slab_scan(gx_slab s) {
+  const int N = (int)s.N;
+  const int M = N + 1;
+  int32_t *ecnt = s.work;
+  int32_t *rcnt = s.work + (int64_t)N * s.n_elem;
+  int32_t *cursor = rcnt + M;
+  __shared__ long long part[1024];
+  const int T = blockDim.x, t = threadIdx.x;
+  const int chunk = (M + T - 1) / T;
+  const int b0 = t * chunk, b1 = min(M, b0 + chunk);
+  long long sum = 0;
+  for (int i = b0; i < b1; ++i) sum += rcnt[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < T; off <<= 1) {
+    long long v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long run = part[t] - sum;  // exclusive prefix of this chunk
+  for (int i = b0; i < b1; ++i) {
+    s.row_off[i] = (int32_t)run;
+    cursor[i] = (int32_t)run;
+    run += rcnt[i];
+  }
+  if (t == T - 1) s.row_off[M] = (int32_t)part[T - 1];
+  // per-row exponent: largest S with 2^S * sum|f| + cnt/2 <= 2^31 - 1 for re and im
+  for (int row = t; row < N; row += T) {
+    double bound = 0.0;
+    int cnt = 0;
+    for (int e = 0; e < s.n_elem; ++e) {
+      int c = ecnt[(int64_t)row * s.n_elem + e];
+      double fr = fabs(s.f_table[2 * e]), fi = fabs(s.f_table[2 * e + 1]);
+      bound += (double)c * fmax(fr, fi);
+      cnt += c;
+    }
+    int S = 0;
+    if (bound > 0.0) {
+      double room = 2147483647.0 - 0.5 * (double)cnt - 1.0;
+      S = ilogb(room / bound);  // floor(log2(room / bound))
+      if (S > 60) S = 60;
+    }
+    s.row_exp[row] = S;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_slab_scatter(gx_slab s) {
+  const int N = (int)s.N;
+  const double rinv = 1.0 / s.r;
+  int32_t *rcnt = s.work + (int64_t)N * s.n_elem;
+  int32_t *cursor = rcnt + (N + 1);
+  for (int64_t a = blockIdx.x * (int64_t)BLK + threadIdx.x; a < s.n_atoms; a += (int64_t)gridDim.x * BLK) {
+    double zz = s.z[a] - s.zmin;
+    long long zi = to_int(floor_div(zz, s.r, rinv));
+    int row = (zi >= 0 && zi < N) ? (int)zi : N;
+    const int e = s.elem[a];
+    if (e < 0 || e >= s.n_elem) row = N;
+    int pos = atomicAdd(&cursor[row], 1);
+    s.sx[pos] = s.x[a];
+    s.sy[pos] = s.y[a];
+    s.szrow[pos] = row;
+    int qr = 0, qi = 0;
+    if (row < N) {
+      double sc = ldexp(1.0, s.row_exp[row]);
+      qr = (int)rint(s.f_table[2 * e] * sc);
+      qi = (int)rint(s.f_table[2 * e + 1] * sc);
+    }
+    s.sq[2 * (int64_t)pos] = qr;
+    s.sq[2 * (int64_t)pos + 1] = qi;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// per-frame statistics
+// ------------------------------------------------------------------------------------
+__global__ void k_frames_init(unsigned long long *ymin_key, int32_t *bounds, int64_t nf) {
+  int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (f < nf) {
+    ymin_key[f] = ~0ull;
+    bounds[4 * f + 0] = INT_MAX;
+    bounds[4 * f + 1] = INT_MIN;
+    bounds[4 * f + 2] = INT_MAX;
+    bounds[4 * f + 3] = INT_MIN;
+  }
+}
+
+constexpr int APT = 4;  // atoms per thread in the statistics kernels
+constexpr int FPB = 8;  // frames per block (grid.y)
+
+// min over all atoms of y' = x sin(phi) + y cos(phi) (canonical rotate_coords_z)
+__global__ __launch_bounds__(BLK) void k_frames_ymin(gx_frames d, int64_t f0, int64_t nf) {
+  const int64_t base = (int64_t)blockIdx.x * BLK * APT + threadIdx.x;
+  double xs[APT], ys[APT];
+  bool ok[APT];
+#pragma unroll
+  for (int k = 0; k < APT; ++k) {
+    int64_t a = base + (int64_t)k * BLK;
+    ok[k] = a < d.n_atoms;
+    xs[k] = ok[k] ? d.ax[a] : 0.0;
+    ys[k] = ok[k] ? d.ay[a] : 0.0;
+  }
+  const int fb0 = blockIdx.y * FPB;
+  for (int q = 0; q < FPB; ++q) {
+    int fb = fb0 + q;
+    if (fb >= nf) break;
+    const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+    const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+    double m = INFINITY;
+#pragma unroll
+    for (int k = 0; k < APT; ++k)
+      if (ok[k]) m = fmin(m, xs[k] * s + ys[k] * c);
+    m = wave_min(m);
+    if ((threadIdx.x & (GX_WAVE - 1)) == 0 && m < INFINITY) atomicMin(&d.ymin_key[fb], dkey(m));
+  }
+}
+
+// y/z index bounds over valid atoms (voxelgrids.py:328-335, 346-349)
+__global__ __launch_bounds__(BLK) void k_frames_bounds(gx_frames d, int64_t f0, int64_t nf) {
+  const int N = (int)d.N;
+  const double rinv = 1.0 / d.r;
+  const int64_t n_rowed = d.row_off[N];
+  const int64_t base = (int64_t)blockIdx.x * BLK * APT + threadIdx.x;
+  double xs[APT], ys[APT];
+  int zr[APT];
+#pragma unroll
+  for (int k = 0; k < APT; ++k) {
+    int64_t a = base + (int64_t)k * BLK;
+    bool ok = a < n_rowed;
+    xs[k] = ok ? d.ax[a] : 0.0;
+    ys[k] = ok ? d.ay[a] : 0.0;
+    zr[k] = ok ? d.azrow[a] : -1;
+  }
+  const int fb0 = blockIdx.y * FPB;
+  for (int q = 0; q < FPB; ++q) {
+    int fb = fb0 + q;
+    if (fb >= nf) break;
+    const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+    const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+    const double ymin = dkey_inv(d.ymin_key[fb]);
+    int ylo = INT_MAX, yhi = INT_MIN, zlo = INT_MAX, zhi = INT_MIN;
+#pragma unroll
+    for (int k = 0; k < APT; ++k) {
+      if (zr[k] < 0) continue;
+      double yy = (xs[k] * s + ys[k] * c) - ymin;
+      long long yi = to_int(floor_div(yy, d.r, rinv));
+      if (yi >= 0 && yi < N) {
+        ylo = min(ylo, (int)yi);
+        yhi = max(yhi, (int)yi);
+        zlo = min(zlo, zr[k]);
+        zhi = max(zhi, zr[k]);
+      }
+    }
+    ylo = wave_min_i(ylo);
+    yhi = wave_max_i(yhi);
+    zlo = wave_min_i(zlo);
+    zhi = wave_max_i(zhi);
+    if ((threadIdx.x & (GX_WAVE - 1)) == 0 && ylo != INT_MAX) {
+      int32_t *b = d.idx_bounds + 4 * fb;
+      atomicMin(&b[0], ylo);
+      atomicMax(&b[1], yhi);
+      atomicMin(&b[2], zlo);
+      atomicMax(&b[3], zhi);
+    }
+  }
+}
+
+}  // namespace
+
+namespace {
+
+// python slice [a:b] of a length-n axis, normalised to [lo, hi)
+__device__ __forceinline__ void py_slice(long long a, long long b, long long n, long long &lo, long long &hi) {
+  if (a < 0) a += n;
+  if (a < 0) a = 0;
+  if (a > n) a = n;
+  if (b < 0) b += n;
+  if (b < 0) b = 0;
+  if (b > n) b = n;
+  lo = a;
+  hi = b > a ? b : a;
+}
+
+// gaussian_filter1d(mask, sigma, mode='wrap') of the indicator of [lo, hi),
+// in scipy's symmetric correlate order (centre first, then the outermost
+// pair inwards).
+__device__ double smooth_mask(const double *w, int radius, long long lo, long long hi, int i, int n) {
+  auto m = [&](long long k) -> double {
+    k %= n;
+    if (k < 0) k += n;
+    return (k >= lo && k < hi) ? 1.0 : 0.0;
+  };
+  double acc = m(i) * w[0];
+  for (int d = radius; d >= 1; --d) acc += (m((long long)i - d) + m((long long)i + d)) * w[d];
+  return acc;
+}
+
+__global__ __launch_bounds__(BLK) void k_frames_columns(gx_frames d, int64_t f0, int64_t nf) {
+  const int N = (int)d.N;
+  const int fb = blockIdx.y;
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (fb >= nf || i >= N) return;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int32_t *b = d.idx_bounds + 4 * fb;
+  if (i == 0 && b[0] == INT_MAX) atomicAdd(&d.status[0], 1);
+  if (d.fill_bkg) {
+    const double rinv = 1.0 / d.r;
+    double x = (double)i * d.r;
+    double len = collapse_length(fp, x);
+    double nm = d.max_voxels - floor_div(len, d.r, rinv);
+    d.col_nm[(int64_t)fb * N + i] = (int32_t)to_int(nm);
+  }
+  if (d.smooth) {
+    long long lo, hi;
+    py_slice((long long)b[0] + d.smooth, (long long)b[1] - d.smooth, N, lo, hi);
+    d.col_sy[(int64_t)fb * N + i] = smooth_mask(d.gauss_w, (int)d.smooth_radius, lo, hi, i, N);
+    py_slice((long long)b[2] + d.smooth, (long long)b[3] - d.smooth, N, lo, hi);
+    d.col_sz[(int64_t)fb * N + i] = smooth_mask(d.gauss_w, (int)d.smooth_radius, lo, hi, i, N);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// row pass
+// ------------------------------------------------------------------------------------
+template <int N, int RPW>
+__global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
+  constexpr int P = N / GX_WAVE;
+  constexpr int RB = (BLK / GX_WAVE) * RPW;  // rows per block
+  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  extern __shared__ float2 smem[];
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int wave = threadIdx.x / GX_WAVE;
+  float2 *wbuf = smem + wave * N;
+  float2 *stage = smem + (BLK / GX_WAVE) * N;  // [W][RB]
+  int32_t *acc_re = reinterpret_cast<int32_t *>(wbuf);
+  int32_t *acc_im = acc_re + N;
+
+  const int fb = blockIdx.y;
+  const int64_t f = f0 + fb;
+  const double *fp = d.fparams + f * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = jhi - jlo + 1;
+  if (wf <= 0) return;  // block-uniform
+  if (wf > d.W) {       // planner guarantees wf <= W; never overrun the staging tile
+    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
+    return;
+  }
+
+  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const double ymin = dkey_inv(d.ymin_key[fb]);
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  const double r = d.r, rinv = 1.0 / d.r;
+  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
+  const int32_t *nm_row = d.col_nm + (int64_t)fb * N;
+  const double *sy_row = d.col_sy + (int64_t)fb * N;
+  const double *sz_row = d.col_sz + (int64_t)fb * N;
+  const float2 *tw = reinterpret_cast<const float2 *>(d.twiddle);
+  const int z0 = blockIdx.x * RB;
+
+  for (int rr = 0; rr < RPW; ++rr) {
+    const int rb = wave * RPW + rr;
+    const int z = z0 + rb;
+    const int a0 = d.row_off[z], a1 = d.row_off[z + 1];
+    const bool outside_z = d.fill_bkg && (z < zlo || z >= zhi);
+    float2 v[P];
+    if (!d.fill_bkg && !d.smooth && a0 == a1) {
+      // empty row: transform is zero
+#pragma unroll
+      for (int q = 0; q < P; ++q) v[q] = make_float2(0.f, 0.f);
+    } else if (outside_z && !d.smooth) {
+      // constant row avg_voxel_f * max_voxels: only the k = 0 term survives
+#pragma unroll
+      for (int q = 0; q < P; ++q) v[q] = make_float2(0.f, 0.f);
+      if (lane == 0) v[0] = make_float2((float)(avv_re * (double)N), (float)(avv_im * (double)N));
+    } else {
+      // 1) zero the fixed-point accumulators
+      for (int y = lane; y < N; y += GX_WAVE) {
+        acc_re[y] = 0;
+        acc_im[y] = 0;
+      }
+      wave_sync();
+      // 2) scatter this row's atoms (np.add.at(grid, (z, y), f), voxelgrids.py:341)
+      for (int a = a0 + lane; a < a1; a += GX_WAVE) {
+        double yy = (d.ax[a] * s + d.ay[a] * c) - ymin;
+        long long yi = to_int(floor_div(yy, r, rinv));
+        if (yi >= 0 && yi < N) {
+          atomicAdd(&acc_re[yi], d.aq[2 * (int64_t)a]);
+          atomicAdd(&acc_im[yi], d.aq[2 * (int64_t)a + 1]);
+        }
+      }
+      wave_sync();
+      // 3) grid values of the first-stage inputs, FP64 then rounded to FP32
+      const double inv = ldexp(1.0, -d.row_exp[z]);
+      const double mz = d.smooth ? sz_row[z] : 0.0;
+#pragma unroll
+      for (int t = 0; t < P / R0; ++t) {
+#pragma unroll
+        for (int q = 0; q < R0; ++q) {
+          const int y = lane + GX_WAVE * t + q * (N / R0);
+          double gre = (double)acc_re[y] * inv, gim = (double)acc_im[y] * inv;
+          if (d.fill_bkg) {
+            if (outside_z || y <= ylo || y >= yhi) {
+              gre = avv_re;
+              gim = avv_im;
+            } else {
+              double nm = (double)nm_row[y];
+              gre = gre + nm * d.avg_re;
+              gim = gim + nm * d.avg_im;
+            }
+          }
+          if (d.smooth) {
+            double m = mz * sy_row[y];
+            double om = 1.0 - m;
+            gre = gre * m + avv_re * om;
+            gim = gim * m + avv_im * om;
+          }
+          v[t * R0 + q] = make_float2((float)gre, (float)gim);
+        }
+      }
+      wave_sync();
+      // 4) FFT along y
+      fft_wave<N>(v, wbuf, tw, lane);
+    }
+    // 5) keep the fftshifted columns [jlo, jhi] in the staging tile
+#pragma unroll
+    for (int t = 0; t < P / RL; ++t) {
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        const int k = lane + GX_WAVE * t + q * (N / RL);
+        const int col = (k + N / 2) & (N - 1);
+        const int w = col - jlo;
+        if (w >= 0 && w < wf) stage[w * RB + rb] = v[t * RL + q];
+      }
+    }
+    wave_sync();
+  }
+  __syncthreads();
+  // 6) transposed store: inter[fb][w][z0 .. z0+RB)
+  float2 *out = reinterpret_cast<float2 *>(d.inter) + (int64_t)fb * d.W * N;
+  for (int idx = threadIdx.x; idx < wf * RB; idx += BLK) {
+    const int w = idx / RB, q = idx - w * RB;
+    out[(int64_t)w * N + z0 + q] = stage[idx];
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// column pass
+// ------------------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
+  constexpr int P = N / GX_WAVE;
+  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  __shared__ float2 lds[BLK / GX_WAVE][N];
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int wave = threadIdx.x / GX_WAVE;
+  const int fb = blockIdx.y;
+  const int64_t f = f0 + fb;
+  const double *fp = d.fparams + f * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wcol = blockIdx.x * (BLK / GX_WAVE) + wave;
+  if (wcol > jhi - jlo || wcol >= d.W) return;  // wave-uniform; no block barrier below
+  const int j = jlo + wcol;
+  // det_h_qx / det_h_qy of this column (numpy linspace: j*step + start, last = stop)
+  const double hx = (j == N - 1) ? fp[GX_FP_HX_STOP] : (double)j * fp[GX_FP_HX_STEP] + fp[GX_FP_HX_START];
+  const double hy = (j == N - 1) ? fp[GX_FP_HY_STOP] : (double)j * fp[GX_FP_HY_STEP] + fp[GX_FP_HY_START];
+  const double dqinv = 1.0 / d.dq;
+  const long long bx = to_int(floor_div(hx - d.qmin, d.dq, dqinv)) - d.c_lo;
+  const long long by = to_int(floor_div(hy - d.qmin, d.dq, dqinv)) - d.c_lo;
+  if (bx < 0 || bx >= d.c_n || by < 0 || by >= d.c_n) return;
+
+  const float2 *col = reinterpret_cast<const float2 *>(d.inter) + ((int64_t)fb * d.W + wcol) * N;
+  float2 v[P];
+#pragma unroll
+  for (int t = 0; t < P / R0; ++t)
+#pragma unroll
+    for (int q = 0; q < R0; ++q) v[t * R0 + q] = col[lane + GX_WAVE * t + q * (N / R0)];
+  const float2 *tw = reinterpret_cast<const float2 *>(d.twiddle);
+  fft_wave<N>(v, lds[wave], tw, lane);
+
+  const int64_t cbase = (by * d.c_n + bx) * d.c_n;
+  const double val = d.q_val;
+#pragma unroll
+  for (int t = 0; t < P / RL; ++t) {
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int k = lane + GX_WAVE * t + q * (N / RL);
+      const int i = (k + N / 2) & (N - 1);
+      if (i < d.i_lo || i > d.i_hi) continue;
+      // det_v_qz[i] = fftshift(fftfreq(N, r) * 2 * pi)[i]
+      const int kint = (k < N / 2) ? k : k - N;
+      const double qz = (((double)kint * val) * 2.0) * M_PI;
+      const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
+      if (bz < 0 || bz >= d.c_n) continue;
+      const float2 F = v[t * RL + q];
+      const float I = fmaf(F.x, F.x, F.y * F.y);
+      const unsigned long long fx = (unsigned long long)llrint((double)I * d.bin_scale);
+      atomicAdd(&d.bin_sum[cbase + bz], fx);
+      atomicAdd(&d.bin_cnt[cbase + bz], 1);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// bins
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BLK) void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) p[i] = v;
+}
+
+__global__ __launch_bounds__(BLK) void k_bins_clear(unsigned long long *s, int32_t *c, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    s[i] = 0;
+    c[i] = 0;
+  }
+}
+
+struct AffArgs {
+  double a[9];
+};
+
+__device__ __forceinline__ double lin_axis(int64_t i, int64_t n, double start, double step, double stop) {
+  return (i == n - 1) ? stop : (double)i * step + start;
+}
+
+__global__ __launch_bounds__(BLK) void k_bins_finalize(const unsigned long long *__restrict__ bsum,
+                                                       const int32_t *__restrict__ bcnt, int64_t c_n,
+                                                       double inv_scale, int64_t c_lo, int64_t q_num,
+                                                       double qa_start, double qa_step, double qa_stop,
+                                                       AffArgs aff, double z_elem, double c16pi2,
+                                                       double *iq64, float *iq32) {
+  const int64_t n = c_n * c_n * c_n;
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    const int64_t iz = i % c_n, ix = (i / c_n) % c_n, iy = i / (c_n * c_n);
+    const int32_t cnt = bcnt[i];
+    double mean = 0.0;
+    if (cnt != 0) mean = ((double)bsum[i] * inv_scale) / (double)cnt;
+    const double qx = lin_axis(c_lo + ix, q_num, qa_start, qa_step, qa_stop);
+    const double qy = lin_axis(c_lo + iy, q_num, qa_start, qa_step, qa_stop);
+    const double qz = lin_axis(c_lo + iz, q_num, qa_start, qa_step, qa_stop);
+    const double q2 = (qx * qx + qy * qy) + qz * qz;
+    double f0 = aff.a[0] * exp(-aff.a[1] * q2 / c16pi2);
+    f0 = f0 + aff.a[2] * exp(-aff.a[3] * q2 / c16pi2);
+    f0 = f0 + aff.a[4] * exp(-aff.a[5] * q2 / c16pi2);
+    f0 = f0 + aff.a[6] * exp(-aff.a[7] * q2 / c16pi2);
+    f0 = f0 + aff.a[8];
+    const double fn = f0 / z_elem;
+    const double v = mean * (fn * fn);
+    if (iq64) iq64[i] = v;
+    if (iq32) iq32[i] = (float)v;
+  }
+}
+
+template <int N>
+int launch_rowpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  constexpr int RPW = (N <= 1024) ? 2 : 1;
+  constexpr int RB = (BLK / GX_WAVE) * RPW;
+  size_t lds = ((size_t)(BLK / GX_WAVE) * N + (size_t)d->W * RB) * sizeof(float2);
+  if (lds > 160 * 1024)
+    return set_error(GX_ERR_UNSUPPORTED, "row pass needs %zu bytes of LDS (W=%lld too wide)", lds,
+                     (long long)d->W);
+  static bool attr_set = false;
+  if (!attr_set) {
+    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass<N, RPW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               160 * 1024));
+    attr_set = true;
+  }
+  dim3 grid(N / RB, (unsigned)nf);
+  k_rowpass<N, RPW><<<grid, BLK, lds, st>>>(*d, f0);
+  return check_launch("k_rowpass");
+}
+
+template <int N>
+int launch_colpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  dim3 grid(grid1d(d->W, BLK / GX_WAVE), (unsigned)nf);
+  k_colpass<N><<<grid, BLK, 0, st>>>(*d, f0);
+  return check_launch("k_colpass");
+}
+
+int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
+  GX_REQUIRE(d != nullptr, "gx_frames descriptor is NULL");
+  GX_REQUIRE(gx_supported_grid(d->N), "grid edge N=%lld has no compiled FFT plan (supported: 256, 512, "
+             "1024, 2048, 4096)", (long long)d->N);
+  GX_REQUIRE(d->r > 0.0, "r_voxel_size must be positive");
+  GX_REQUIRE(nf >= 0 && nf <= d->batch, "batch of %lld frames exceeds workspace batch %lld", (long long)nf,
+             (long long)d->batch);
+  GX_REQUIRE(f0 >= 0 && f0 + nf <= d->n_frames, "frames [%lld, %lld) outside [0, %lld)", (long long)f0,
+             (long long)(f0 + nf), (long long)d->n_frames);
+  GX_REQUIRE(d->W >= 1 && d->W <= d->N, "intermediate width W=%lld outside [1, N]", (long long)d->W);
+  GX_REQUIRE(d->c_n >= 1, "empty cropped grid");
+  GX_REQUIRE(d->fparams && d->ymin_key && d->idx_bounds && d->inter && d->bin_sum && d->bin_cnt && d->twiddle &&
+                 d->status && d->row_off && d->row_exp,
+             "a required device buffer of gx_frames is NULL");
+  GX_REQUIRE(!d->fill_bkg || d->col_nm, "fill_bkg needs col_nm");
+  GX_REQUIRE(!d->smooth || (d->col_sy && d->col_sz && d->gauss_w), "smooth needs col_sy, col_sz, gauss_w");
+  GX_REQUIRE(d->n_atoms >= 0 && (d->n_atoms == 0 || (d->ax && d->ay && d->aq && d->azrow)), "atom arrays");
+  return GX_OK;
+}
+
+}  // namespace
+
+// =====================================================================================
+// C ABI
+// =====================================================================================
+extern "C" {
+
+int gx_abi_version(void) { return GX_ABI_VERSION; }
+
+int gx_supported_grid(int64_t N) {
+  return N == 256 || N == 512 || N == 1024 || N == 2048 || N == 4096;
+}
+
+int gx_slab_bounds(const gx_slab *s, double *out_minmax, int64_t *out_elem_count, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(s && out_minmax && out_elem_count, "NULL argument");
+  GX_REQUIRE(s->n_atoms > 0, "slab has no atoms");
+  GX_REQUIRE(s->x && s->y && s->z && s->elem, "atom arrays are NULL");
+  hipStream_t st = as_stream(stream);
+  auto *keys = reinterpret_cast<unsigned long long *>(out_minmax);
+  auto *cnts = reinterpret_cast<unsigned long long *>(out_elem_count);
+  k_slab_bounds_init<<<1, 64, 0, st>>>(keys, cnts, s->n_elem);
+  unsigned g = grid1d(s->n_atoms, BLK);
+  if (g > 4096) g = 4096;
+  k_slab_bounds<<<g, BLK, 0, st>>>(s->x, s->y, s->z, s->elem, s->n_atoms, s->n_elem, keys, cnts);
+  k_slab_bounds_fin<<<1, 64, 0, st>>>(keys);
+  return check_launch("gx_slab_bounds");
+}
+
+int gx_slab_sort_rows(const gx_slab *s, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(s != nullptr, "NULL slab");
+  GX_REQUIRE(s->n_atoms > 0 && s->n_atoms < (1ll << 31), "atom count %lld outside [1, 2^31)",
+             (long long)s->n_atoms);
+  GX_REQUIRE(s->N >= 1 && s->N <= (1 << 20), "grid edge N=%lld", (long long)s->N);
+  GX_REQUIRE(s->r > 0.0, "r_voxel_size must be positive");
+  GX_REQUIRE(s->n_elem >= 1 && s->n_elem <= 128, "n_elem=%lld outside [1, 128]", (long long)s->n_elem);
+  GX_REQUIRE(s->x && s->y && s->z && s->elem && s->f_table && s->sx && s->sy && s->sq && s->szrow && s->row_off &&
+                 s->row_exp && s->work,
+             "a slab buffer is NULL");
+  hipStream_t st = as_stream(stream);
+  const int64_t nwork = s->N * s->n_elem + 2 * (s->N + 1);
+  k_fill_i32<<<grid1d(nwork, BLK) > 4096 ? 4096 : grid1d(nwork, BLK), BLK, 0, st>>>(s->work, nwork, 0);
+  unsigned g = grid1d(s->n_atoms, BLK);
+  if (g > 8192) g = 8192;
+  k_slab_rowcount<<<g, BLK, 0, st>>>(*s);
+  k_slab_scan<<<1, 1024, 0, st>>>(*s);
+  k_slab_scatter<<<g, BLK, 0, st>>>(*s);
+  return check_launch("gx_slab_sort_rows");
+}
+
+int gx_frames_stats(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream) {
+  clear_error();
+  int rc = check_frames(d, f0, nf);
+  if (rc) return rc;
+  if (nf == 0) return GX_OK;
+  hipStream_t st = as_stream(stream);
+  k_frames_init<<<grid1d(nf, BLK), BLK, 0, st>>>(d->ymin_key, d->idx_bounds, nf);
+  if (d->n_atoms > 0) {
+    dim3 g(grid1d(d->n_atoms, BLK * APT), grid1d(nf, FPB));
+    k_frames_ymin<<<g, BLK, 0, st>>>(*d, f0, nf);
+    k_frames_bounds<<<g, BLK, 0, st>>>(*d, f0, nf);
+  }
+  return check_launch("gx_frames_stats");
+}
+
+int gx_frames_columns(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream) {
+  clear_error();
+  int rc = check_frames(d, f0, nf);
+  if (rc) return rc;
+  if (nf == 0) return GX_OK;
+  dim3 g(grid1d(d->N, BLK), (unsigned)nf);
+  k_frames_columns<<<g, BLK, 0, as_stream(stream)>>>(*d, f0, nf);
+  return check_launch("gx_frames_columns");
+}
+
+int gx_frames_rowpass(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream) {
+  clear_error();
+  int rc = check_frames(d, f0, nf);
+  if (rc) return rc;
+  if (nf == 0) return GX_OK;
+  hipStream_t st = as_stream(stream);
+  switch (d->N) {
+    case 256: return launch_rowpass<256>(d, f0, nf, st);
+    case 512: return launch_rowpass<512>(d, f0, nf, st);
+    case 1024: return launch_rowpass<1024>(d, f0, nf, st);
+    case 2048: return launch_rowpass<2048>(d, f0, nf, st);
+    case 4096: return launch_rowpass<4096>(d, f0, nf, st);
+  }
+  return set_error(GX_ERR_UNSUPPORTED, "N=%lld", (long long)d->N);
+}
+
+int gx_frames_colpass(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream) {
+  clear_error();
+  int rc = check_frames(d, f0, nf);
+  if (rc) return rc;
+  if (nf == 0) return GX_OK;
+  hipStream_t st = as_stream(stream);
+  switch (d->N) {
+    case 256: return launch_colpass<256>(d, f0, nf, st);
+    case 512: return launch_colpass<512>(d, f0, nf, st);
+    case 1024: return launch_colpass<1024>(d, f0, nf, st);
+    case 2048: return launch_colpass<2048>(d, f0, nf, st);
+    case 4096: return launch_colpass<4096>(d, f0, nf, st);
+  }
+  return set_error(GX_ERR_UNSUPPORTED, "N=%lld", (long long)d->N);
+}
+
+int gx_frames_forward(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream) {
+  int rc = gx_frames_stats(d, f0, nf, stream);
+  if (rc == GX_OK && (d->fill_bkg || d->smooth)) rc = gx_frames_columns(d, f0, nf, stream);
+  if (rc == GX_OK) rc = gx_frames_rowpass(d, f0, nf, stream);
+  if (rc == GX_OK) rc = gx_frames_colpass(d, f0, nf, stream);
+  return rc;
+}
+
+int gx_bins_clear(unsigned long long *bin_sum, int32_t *bin_cnt, int64_t n_bins, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(bin_sum && bin_cnt && n_bins >= 0, "bins");
+  if (n_bins == 0) return GX_OK;
+  unsigned g = grid1d(n_bins, BLK);
+  if (g > 8192) g = 8192;
+  k_bins_clear<<<g, BLK, 0, as_stream(stream)>>>(bin_sum, bin_cnt, n_bins);
+  return check_launch("gx_bins_clear");
+}
+
+int gx_bins_finalize(const unsigned long long *bin_sum, const int32_t *bin_cnt, int64_t c_n, double bin_scale,
+                     int64_t c_lo, int64_t q_num, double qa_start, double qa_step, double qa_stop,
+                     const double *host_aff, double z_elem, double *iq64, float *iq32, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(bin_sum && bin_cnt && host_aff, "NULL argument");
+  GX_REQUIRE(c_n >= 1 && c_lo >= 0 && c_lo + c_n <= q_num, "crop window [%lld, %lld) outside q axis of %lld",
+             (long long)c_lo, (long long)(c_lo + c_n), (long long)q_num);
+  GX_REQUIRE(bin_scale > 0.0 && z_elem != 0.0, "bin_scale / Z");
+  AffArgs aff;
+  for (int i = 0; i < 9; ++i) aff.a[i] = host_aff[i];
+  const double c16pi2 = 16 * (M_PI * M_PI);
+  const int64_t n = c_n * c_n * c_n;
+  unsigned g = grid1d(n, BLK);
+  if (g > 16384) g = 16384;
+  k_bins_finalize<<<g, BLK, 0, as_stream(stream)>>>(bin_sum, bin_cnt, c_n, 1.0 / bin_scale, c_lo, q_num,
+                                                     qa_start, qa_step, qa_stop, aff, z_elem, c16pi2, iq64,
+                                                     iq32);
+  return check_launch("gx_bins_finalize");
+}
+
+}  // extern "C"

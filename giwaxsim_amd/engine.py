@@ -1,0 +1,636 @@
+"""Host planner and device runner of the forward-scattering path.
+
+Division of labour:
+
+* host (numpy, FP64): geometry planning on 1-D axes and per-frame scalars,
+  exactly as the reference computes them -- grid size, q axis, phi list,
+  crop window (tools/comparison.py:705-731, tools/voxelgrids.py:16-48),
+  per-frame trig scalars and linspace parameters (tools/voxelgrids.py:381-402),
+  the branch constants of ``rectangular_collapse_lengths``
+  (tools/voxelgrids.py:238-309) and the detector rotation matrices
+  (tools/detector.py:33-162, tools/utilities.py:222-245).
+* device (gfx950 HIP through the C ABI): every per-atom, per-voxel, per-pixel
+  and per-bin operation.
+
+Plans depend only on geometry (grid, q grid, angles, slab bounds), never on
+atom positions, and are cached.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import functools
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import GxDetector, GxFrames, GxSlab, call
+from .tools.ptable_dict import aff_dict, ptable
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the platform
+    torch = None
+
+
+# --------------------------------------------------------------------------------------
+# device plumbing
+# --------------------------------------------------------------------------------------
+def device():
+    """The HIP device; raises if none is visible (there is no CPU path)."""
+    if torch is None or not torch.cuda.is_available():
+        raise RuntimeError("giwaxsim_amd runs on a ROCm GPU (HIP device) and none is visible")
+    return torch.device("cuda")
+
+
+def stream_handle():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def to_dev(a, dtype):
+    """numpy / torch -> contiguous device tensor of ``dtype``."""
+    dev = device()
+    if torch.is_tensor(a):
+        return a.to(device=dev, dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a)).to(device=dev, dtype=dtype).contiguous()
+
+
+def _empty(n, dtype):
+    return torch.empty(int(max(n, 1)), dtype=dtype, device=device())
+
+
+# --------------------------------------------------------------------------------------
+# element tables
+# --------------------------------------------------------------------------------------
+def element_table(elements, energy):
+    """Unique elements -> (ids int32, f_table complex128, counts).
+
+    f = Z + f1 + i f2 per element (tools/comparison.py:735-739); f1/f2 from
+    the offline provider (``giwaxsim_amd.tools.f1f2``)."""
+    from .tools.f1f2 import f1_f2
+    elements = np.asarray(elements)
+    uniq, inv, counts = np.unique(elements, return_inverse=True, return_counts=True)
+    ftab = np.empty(len(uniq), dtype=np.complex128)
+    for i, e in enumerate(uniq):
+        f1, f2 = f1_f2(str(e), energy)
+        ftab[i] = complex(f1, f2) + ptable[str(e)]
+    return inv.astype(np.int32), ftab, counts.astype(np.int64), uniq
+
+
+# --------------------------------------------------------------------------------------
+# q-space grid (tools/comparison.py:705-731) and crop (tools/voxelgrids.py:16-48)
+# --------------------------------------------------------------------------------------
+@dataclass
+class QGrid:
+    q_voxel: float
+    max_q: float
+    max_q_diag: float
+    q_num: int
+    q_axis: np.ndarray
+    phis: np.ndarray
+    dq: float
+    qmin: float
+    qmax: float
+    c_lo: int
+    c_n: int
+
+    @property
+    def crop_axis(self):
+        return self.q_axis[self.c_lo:self.c_lo + self.c_n]
+
+    @property
+    def linspace_params(self):
+        """(start, step, stop) with q_axis[i] = i*step + start, last = stop."""
+        start = self.q_axis[0]
+        stop = self.q_axis[-1]
+        step = (np.float64(self.max_q_diag) - np.float64(-self.max_q_diag)) / (self.q_num - 1)
+        return float(start), float(step), float(stop)
+
+
+@functools.lru_cache(maxsize=64)
+def qgrid(q_voxel_size: float, max_q: float) -> QGrid:
+    max_q_diag = np.sqrt(2) * max_q
+    max_q_diag = max_q_diag + max_q_diag % q_voxel_size
+    q_num = ((2 * max_q_diag / q_voxel_size) + 1).astype(int)
+    if q_num % 2 == 0:
+        q_num += 1
+    q_axis = np.linspace(-max_q_diag, max_q_diag, q_num)
+    delta_phi_rad = np.arctan(q_voxel_size / max_q_diag)
+    phi_num = np.ceil(2 * np.pi / delta_phi_rad).astype(int)
+    last_phi = 180 - (180 / phi_num)
+    phis = np.linspace(0, last_phi, num=phi_num)
+    voxel_size = np.abs(q_axis[1] - q_axis[0])
+    max_val = max_q + voxel_size
+    idx = np.where(np.abs(q_axis) < max_val)[0]
+    return QGrid(q_voxel=q_voxel_size, max_q=max_q, max_q_diag=float(max_q_diag), q_num=int(q_num),
+                 q_axis=q_axis, phis=phis, dq=float(np.diff(q_axis)[0]), qmin=float(np.min(q_axis)),
+                 qmax=float(np.max(q_axis)), c_lo=int(idx[0]), c_n=int(idx[-1] - idx[0] + 1))
+
+
+def check_physical(r_voxel_size, q_voxel_size, max_q, x_bound, y_bound, z_bound):
+    """The two checks of tools/comparison.py:705-717; returns grid_size."""
+    max_q_diag = np.sqrt(2) * max_q
+    if max_q_diag > 2 * np.pi / r_voxel_size:
+        raise Exception('Max_q is non-physical for given voxel size')
+    grid_size = int(np.ceil(2 * np.pi / (q_voxel_size * r_voxel_size)))
+    min_bound = np.min([x_bound, y_bound, z_bound])
+    if grid_size * r_voxel_size < min_bound:
+        raise Exception('Calculated real-space bounds smaller than simulation. '
+                        'Please lower delta_q value')
+    return grid_size
+
+
+# --------------------------------------------------------------------------------------
+# per-frame records
+# --------------------------------------------------------------------------------------
+def _linspace_params(start, stop, num):
+    """(start, step, stop) of np.linspace(start, stop, num) (endpoint=True):
+    y = arange(num)*step + start, y[-1] = stop."""
+    start = np.float64(start)
+    stop = np.float64(stop)
+    delta = np.subtract(stop, start, dtype=np.float64)
+    step = delta / (num - 1)
+    if step == 0 and delta != 0:
+        raise ValueError("linspace step underflows to zero")
+    return start, step, stop
+
+
+def collapse_record(phi, hor_length, ver_length):
+    """Branch constants of rectangular_collapse_lengths (tools/voxelgrids.py:253-285),
+    as (mode, hor, ver, stop1, stop2, s12, mid, A, tp, vc, tt, cp, ct)."""
+    hor_length, ver_length = np.float64(hor_length), np.float64(ver_length)
+    if phi > 90:
+        phi = phi - 90
+        hor_length, ver_length = ver_length, hor_length
+    theta = 90 - phi
+    theta_rad = np.deg2rad(theta)
+    phi_rad = np.deg2rad(phi)
+    if phi == 0:
+        return (1, hor_length, ver_length) + (0.0,) * 10
+    if phi == 90:
+        return (2, hor_length, ver_length) + (0.0,) * 10
+    stop1 = ver_length * np.cos(theta_rad)
+    stop2 = hor_length * np.cos(phi_rad)
+    if stop1 < stop2:
+        mid = ver_length / np.sin(theta_rad)
+    else:
+        stop1, stop2 = stop2, stop1
+        mid = hor_length / np.sin(phi_rad)
+    A = np.sqrt(ver_length ** 2 - (ver_length * np.cos(theta_rad)) ** 2)
+    return (0, hor_length, ver_length, stop1, stop2, stop1 + stop2, mid, A, np.tan(phi_rad),
+            ver_length * np.cos(theta_rad), np.tan(theta_rad), np.cos(phi_rad), np.cos(theta_rad))
+
+
+@dataclass
+class FramePlan:
+    """Atom-independent plan of the phi-frame loop of one slab."""
+    N: int
+    r: float
+    grid: QGrid
+    x_bound: float
+    y_bound: float
+    fparams: np.ndarray           # [F, 24]
+    W: int                        # widest kept column band
+    i_lo: int
+    i_hi: int
+    q_val: float
+    max_voxels: float
+    frame_cols: np.ndarray = field(repr=False, default=None)  # kept columns per frame
+
+    @property
+    def n_frames(self):
+        return len(self.grid.phis)
+
+
+def qgrid_from_axis(q_axis, phis, max_q=None):
+    """QGrid of an explicit (qx = qy = qz) axis and phi list.  With ``max_q``
+    the bins are cropped like downselect_voxelgrid, else the full grid is kept."""
+    q_axis = np.asarray(q_axis, dtype=np.float64)
+    q_num = len(q_axis)
+    if max_q is None:
+        c_lo, c_n = 0, q_num
+    else:
+        voxel_size = np.abs(q_axis[1] - q_axis[0])
+        idx = np.where(np.abs(q_axis) < max_q + voxel_size)[0]
+        c_lo, c_n = int(idx[0]), int(idx[-1] - idx[0] + 1)
+    return QGrid(q_voxel=float(np.diff(q_axis)[0]), max_q=max_q, max_q_diag=float(q_axis[-1]),
+                 q_num=q_num, q_axis=q_axis, phis=np.atleast_1d(np.asarray(phis, dtype=np.float64)),
+                 dq=float(np.diff(q_axis)[0]), qmin=float(np.min(q_axis)), qmax=float(np.max(q_axis)),
+                 c_lo=c_lo, c_n=c_n)
+
+
+@functools.lru_cache(maxsize=32)
+def frame_plan(r_voxel_size: float, q_voxel_size: float, max_q: float, x_bound: float,
+               y_bound: float, z_bound: float) -> FramePlan:
+    N = check_physical(r_voxel_size, q_voxel_size, max_q, x_bound, y_bound, z_bound)
+    return build_frame_plan(N, r_voxel_size, qgrid(q_voxel_size, max_q), x_bound, y_bound)
+
+
+def build_frame_plan(N, r, g: QGrid, x_bound, y_bound) -> FramePlan:
+    q_h = np.fft.fftfreq(N, d=r) * 2 * np.pi
+    q_h_shifted = np.fft.fftshift(q_h)
+    q_v_shifted = np.fft.fftshift(np.fft.fftfreq(N, d=r) * 2 * np.pi)
+    Rq = np.max(q_h_shifted)
+    c_lo, c_hi = g.c_lo, g.c_lo + g.c_n
+    # rows (tools/voxelgrids.py:476, 499): in range and inside the crop
+    vmask = (q_v_shifted <= g.qmax) & (q_v_shifted >= g.qmin)
+    bz = np.floor_divide(q_v_shifted - g.qmin, g.dq).astype(int)
+    rows = np.nonzero(vmask & (bz >= c_lo) & (bz < c_hi))[0]
+    if len(rows) == 0 or rows[-1] - rows[0] + 1 != len(rows):
+        raise RuntimeError("kept detector rows are not one contiguous band")
+    F = len(g.phis)
+    fp = np.zeros((F, _lib.GX_FP_STRIDE), dtype=np.float64)
+    max_voxels = np.sqrt(np.float64(x_bound) ** 2 + np.float64(y_bound) ** 2) // r
+    widths = np.zeros(F, dtype=np.int64)
+    for f, phi in enumerate(g.phis):
+        phi_rad = np.radians(phi)
+        fp[f, _lib.FP_SIN] = np.sin(phi_rad)
+        fp[f, _lib.FP_COS] = np.cos(phi_rad)
+        right_qy = Rq * np.cos(np.deg2rad(-phi))
+        left_qy = -right_qy
+        right_qx = -Rq * np.sin(np.deg2rad(-phi))
+        left_qx = -right_qx
+        fp[f, _lib.FP_HX_START:_lib.FP_HX_STOP + 1] = _linspace_params(left_qx, right_qx, N)
+        fp[f, _lib.FP_HY_START:_lib.FP_HY_STOP + 1] = _linspace_params(left_qy, right_qy, N)
+        rec = collapse_record(phi, y_bound, x_bound)
+        fp[f, _lib.FP_LEN_MODE:_lib.FP_CT + 1] = rec
+        # kept columns (tools/voxelgrids.py:475, 497-498) inside the crop
+        hqx = np.linspace(left_qx, right_qx, num=N)
+        hqy = np.linspace(left_qy, right_qy, num=N)
+        hmask = (hqx <= g.qmax) & (hqx >= g.qmin) & (hqy <= g.qmax) & (hqy >= g.qmin)
+        with np.errstate(invalid='ignore'):
+            bx = np.floor_divide(hqx - g.qmin, g.dq)
+            by = np.floor_divide(hqy - g.qmin, g.dq)
+        keep = hmask & (bx >= c_lo) & (bx < c_hi) & (by >= c_lo) & (by < c_hi)
+        js = np.nonzero(keep)[0]
+        if len(js):
+            if js[-1] - js[0] + 1 != len(js):
+                raise RuntimeError(f"kept columns of frame {f} are not one contiguous band")
+            fp[f, _lib.FP_JLO], fp[f, _lib.FP_JHI] = js[0], js[-1]
+            widths[f] = len(js)
+        else:
+            fp[f, _lib.FP_JLO], fp[f, _lib.FP_JHI] = 0, -1
+    return FramePlan(N=N, r=r, grid=g, x_bound=float(x_bound), y_bound=float(y_bound), fparams=fp,
+                     W=int(max(1, widths.max())), i_lo=int(rows[0]), i_hi=int(rows[-1]),
+                     q_val=1.0 / (N * r), max_voxels=float(max_voxels))
+
+
+@functools.lru_cache(maxsize=16)
+def twiddles(N: int):
+    m = np.arange(N, dtype=np.float64)
+    ang = -2.0 * np.pi * m / N
+    tw = np.empty(2 * N, dtype=np.float32)
+    tw[0::2] = np.cos(ang)
+    tw[1::2] = np.sin(ang)
+    return tw
+
+
+def gauss_half_kernel(sigma):
+    """scipy.ndimage gaussian kernel (truncate 4.0), centre outwards:
+    radius = int(4*sigma + 0.5); phi = exp(-0.5/sigma^2 x^2) / sum."""
+    sd = float(sigma)
+    radius = int(4.0 * sd + 0.5)
+    sigma2 = sigma * sigma
+    x = np.arange(-radius, radius + 1)
+    phi_x = np.exp(-0.5 / sigma2 * x ** 2)
+    phi_x = phi_x / phi_x.sum()
+    return phi_x[radius:].copy(), radius
+
+
+# --------------------------------------------------------------------------------------
+# fixed-point scale of the bin sums
+# --------------------------------------------------------------------------------------
+def intensity_bound(sum_abs_f, N, fill_bkg, smooth, avg, max_voxels, hor, ver, r):
+    """Upper bound B on |F(k)| = |sum G| <= sum |G| of one frame's grid."""
+    a = abs(avg)
+    per_voxel = 0.0
+    if fill_bkg:
+        nm_max = max_voxels + (hor + ver) / r + 2.0
+        per_voxel += (nm_max + max_voxels) * a
+    if smooth:
+        per_voxel += a * max_voxels
+    return 1.01 * (sum_abs_f + N * N * per_voxel) + 1.0
+
+
+def pixels_per_bin(plan: FramePlan):
+    """Max pixels of one frame that can share a bin."""
+    g = plan.grid
+    Rq = (plan.N // 2 - 1 if plan.N % 2 == 0 else (plan.N - 1) // 2) * plan.q_val * 2 * np.pi
+    dh = 2 * Rq / (plan.N - 1)
+    dv = plan.q_val * 2 * np.pi
+    return (math.ceil(math.sqrt(2) * g.dq / dh) + 1) * (math.ceil(g.dq / dv) + 1)
+
+
+def bin_scale_for(terms):
+    """terms: iterable of (frames, pixels_per_bin, B).  Power-of-two scale with
+    sum_frames k B^2 * scale <= 2^62."""
+    total = sum(float(F) * float(k) * float(B) ** 2 for F, k, B in terms)
+    e = math.floor(62 - math.log2(max(total, 1e-300)))
+    e = max(min(e, 200), -200)
+    return math.ldexp(1.0, e)
+
+
+# --------------------------------------------------------------------------------------
+# device runner of the phi-frame loop
+# --------------------------------------------------------------------------------------
+class Slab:
+    """Atoms of one slab on the device, regrouped by projected z row."""
+
+    def __init__(self, coords, elem_ids, f_table, N, r, zmin):
+        dev = device()
+        c = to_dev(coords, torch.float64)
+        self.n = int(c.shape[0])
+        self.x = c[:, 0].contiguous()
+        self.y = c[:, 1].contiguous()
+        self.z = c[:, 2].contiguous()
+        self.elem = to_dev(elem_ids, torch.int32)
+        ft = np.empty((len(f_table), 2), dtype=np.float64)
+        ft[:, 0] = np.real(f_table)
+        ft[:, 1] = np.imag(f_table)
+        self.f_table = torch.as_tensor(ft).to(dev).contiguous()
+        self.n_elem = len(f_table)
+        self.N, self.r, self.zmin = int(N), float(r), float(zmin)
+        n = self.n
+        self.sx = _empty(n, torch.float64)
+        self.sy = _empty(n, torch.float64)
+        self.sq = _empty(2 * n, torch.int32)
+        self.szrow = _empty(n, torch.int32)
+        self.row_off = _empty(N + 2, torch.int32)
+        self.row_exp = _empty(N, torch.int32)
+        self.work = _empty(N * self.n_elem + 2 * (N + 1), torch.int32)
+        self.desc = GxSlab(
+            n_atoms=n, n_elem=self.n_elem, N=self.N, r=self.r, zmin=self.zmin,
+            x=ptr(self.x).value, y=ptr(self.y).value, z=ptr(self.z).value,
+            elem=ptr(self.elem).value, f_table=ptr(self.f_table).value,
+            sx=ptr(self.sx).value, sy=ptr(self.sy).value, sq=ptr(self.sq).value,
+            szrow=ptr(self.szrow).value, row_off=ptr(self.row_off).value,
+            row_exp=ptr(self.row_exp).value, work=ptr(self.work).value)
+        call("gx_slab_sort_rows", ctypes.byref(self.desc), stream_handle())
+
+
+def slab_bounds_device(coords, elem_ids, n_elem):
+    """(x_bound, y_bound, z_bound, zmin, elem_counts) of device-resident atoms."""
+    c = to_dev(coords, torch.float64)
+    x, y, z = c[:, 0].contiguous(), c[:, 1].contiguous(), c[:, 2].contiguous()
+    el = to_dev(elem_ids, torch.int32)
+    out = _empty(6, torch.float64)
+    cnt = _empty(n_elem, torch.int64)
+    d = GxSlab(n_atoms=int(c.shape[0]), n_elem=int(n_elem), x=ptr(x).value, y=ptr(y).value,
+               z=ptr(z).value, elem=ptr(el).value)
+    call("gx_slab_bounds", ctypes.byref(d), ptr(out), ptr(cnt), stream_handle())
+    mm = out.cpu().numpy()
+    return (mm[1] - mm[0], mm[3] - mm[2], mm[5] - mm[4], mm[4], cnt.cpu().numpy())
+
+
+class Bins:
+    """Cropped fixed-point bin sums + counts on the device."""
+
+    def __init__(self, grid: QGrid, scale: float):
+        self.grid = grid
+        self.scale = float(scale)
+        n = grid.c_n ** 3
+        self.sum = _empty(n, torch.int64)
+        self.cnt = _empty(n, torch.int32)
+        self.clear()
+
+    def clear(self):
+        call("gx_bins_clear", ptr(self.sum), ptr(self.cnt), self.grid.c_n ** 3, stream_handle())
+
+    def finalize(self, want64=True, want32=True):
+        """(iq64, iq32) device tensors [c, c, c] indexed [qy, qx, qz]."""
+        g = self.grid
+        n = g.c_n ** 3
+        iq64 = _empty(n, torch.float64) if want64 else None
+        iq32 = _empty(n, torch.float32) if want32 else None
+        start, step, stop = g.linspace_params
+        aff = _lib.dbl_array(aff_dict['C'])
+        call("gx_bins_finalize", ptr(self.sum), ptr(self.cnt), g.c_n, self.scale, g.c_lo, g.q_num,
+             start, step, stop, aff, float(ptable['C']), ptr(iq64), ptr(iq32), stream_handle())
+        shp = (g.c_n,) * 3
+        return (iq64.view(shp) if iq64 is not None else None,
+                iq32.view(shp) if iq32 is not None else None)
+
+
+class FrameRunner:
+    """Runs every phi-frame of one slab into shared bins, in batches."""
+
+    def __init__(self, plan: FramePlan, slab: Slab, bins: Bins, fill_bkg, smooth, avg,
+                 batch=None):
+        dev = device()
+        N, W = plan.N, plan.W
+        self.plan, self.slab, self.bins = plan, slab, bins
+        F = plan.n_frames
+        if batch is None:
+            # keep the batch's intermediate (batch * W * N * 8 B) within ~160 MB
+            batch = max(1, min(F, int(160e6 // (W * N * 8))))
+        self.batch = int(batch)
+        self.fparams = torch.as_tensor(plan.fparams).to(dev).contiguous()
+        self.tw = torch.as_tensor(twiddles(N)).to(dev).contiguous()
+        B = self.batch
+        self.ymin_key = _empty(B, torch.int64)
+        self.idx_bounds = _empty(4 * B, torch.int32)
+        self.col_nm = _empty(B * N, torch.int32) if fill_bkg else None
+        self.col_sy = _empty(B * N, torch.float64) if smooth else None
+        self.col_sz = _empty(B * N, torch.float64) if smooth else None
+        self.inter = _empty(B * W * N * 2, torch.float32)
+        self.status = torch.zeros(4, dtype=torch.int32, device=dev)
+        if smooth:
+            w, rad = gauss_half_kernel(smooth)
+            self.gauss = torch.as_tensor(w).to(dev).contiguous()
+        else:
+            self.gauss, rad = None, 0
+        g = plan.grid
+        self.desc = GxFrames(
+            N=N, r=plan.r, n_atoms=slab.n,
+            ax=ptr(slab.sx).value, ay=ptr(slab.sy).value, aq=ptr(slab.sq).value,
+            azrow=ptr(slab.szrow).value, row_off=ptr(slab.row_off).value,
+            row_exp=ptr(slab.row_exp).value,
+            fill_bkg=int(bool(fill_bkg)), smooth=int(smooth), max_voxels=plan.max_voxels,
+            avg_re=float(np.real(avg)), avg_im=float(np.imag(avg)), gauss_w=ptr(self.gauss).value,
+            smooth_radius=rad, qmin=g.qmin, dq=g.dq, c_lo=g.c_lo, c_n=g.c_n, q_val=plan.q_val,
+            i_lo=plan.i_lo, i_hi=plan.i_hi, W=W, bin_scale=bins.scale, twiddle=ptr(self.tw).value,
+            n_frames=F, fparams=ptr(self.fparams).value, batch=self.batch,
+            ymin_key=ptr(self.ymin_key).value, idx_bounds=ptr(self.idx_bounds).value,
+            col_nm=ptr(self.col_nm).value, col_sy=ptr(self.col_sy).value,
+            col_sz=ptr(self.col_sz).value, inter=ptr(self.inter).value,
+            bin_sum=ptr(bins.sum).value, bin_cnt=ptr(bins.cnt).value, status=ptr(self.status).value)
+
+    def run(self, frames=None):
+        """Enqueue frames [lo, hi) (default: all) on the current stream."""
+        lo, hi = (0, self.plan.n_frames) if frames is None else frames
+        st = stream_handle()
+        for f0 in range(lo, hi, self.batch):
+            nf = min(self.batch, hi - f0)
+            call("gx_frames_forward", ctypes.byref(self.desc), f0, nf, st)
+
+    def check(self):
+        s = self.status.cpu().numpy()
+        if s[0]:
+            raise ValueError("zero-size array to reduction operation minimum which has no identity "
+                             f"({int(s[0])} phi-frame(s) without any atom inside the grid)")
+        if s[1]:
+            raise RuntimeError("internal: a frame's column band exceeded the planned width")
+
+
+def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, fill_bkg=False,
+                  smooth=0, f_table=None, elem_ids=None, batch=None, want64=True, frames=None):
+    """Full voxel-grid path of one slab.  Returns dict with device iq32/iq64,
+    host axes and the objects that produced them.  ``frames=(lo, hi)``
+    restricts the run to phi-frames [lo, hi) (bounded samples)."""
+    if f_table is None:
+        elem_ids, f_table, counts, uniq = element_table(elements, energy)
+    else:
+        counts = np.bincount(np.asarray(elem_ids), minlength=len(f_table))
+    if torch.is_tensor(coords):
+        xb, yb, zb, zmin, counts = slab_bounds_device(coords, elem_ids, len(f_table))
+    else:
+        coords = np.asarray(coords, dtype=np.float64)
+        xb = np.max(coords[:, 0]) - np.min(coords[:, 0])
+        yb = np.max(coords[:, 1]) - np.min(coords[:, 1])
+        zb = np.max(coords[:, 2]) - np.min(coords[:, 2])
+        zmin = np.min(coords[:, 2])
+    plan = frame_plan(float(r_voxel_size), float(q_voxel_size), float(max_q), float(xb), float(yb),
+                      float(zb))
+    if not _lib.load().gx_supported_grid(plan.N):
+        raise NotImplementedError(
+            f"grid_size N={plan.N} (from 2*pi/(q_voxel_size*r_voxel_size)) has no compiled FFT; "
+            "supported: 256, 512, 1024, 2048, 4096")
+    sum_f = np.sum(counts * f_table)
+    volume = xb * yb * zb
+    avg = (sum_f / volume) * r_voxel_size ** 3
+    sum_abs = float(np.sum(counts * np.abs(f_table)))
+    B = intensity_bound(sum_abs, plan.N, fill_bkg, smooth, avg, plan.max_voxels, yb, xb,
+                        r_voxel_size)
+    scale = bin_scale_for([(plan.n_frames, pixels_per_bin(plan), B)])
+    bins = Bins(plan.grid, scale)
+    slab = Slab(coords, elem_ids, f_table, plan.N, r_voxel_size, zmin)
+    runner = FrameRunner(plan, slab, bins, fill_bkg, smooth, avg, batch=batch)
+    runner.run(frames)
+    runner.check()
+    iq64, iq32 = bins.finalize(want64=want64, want32=True)
+    ax = plan.grid.crop_axis
+    return dict(iq64=iq64, iq32=iq32, qx=ax.copy(), qy=ax.copy(), qz=ax.copy(), plan=plan,
+                bins=bins, runner=runner, slab=slab, avg=avg)
+
+
+# --------------------------------------------------------------------------------------
+# detector planning (tools/detector.py, tools/comparison.py:790-853)
+# --------------------------------------------------------------------------------------
+def rotation_matrix(u, theta):
+    """Rodrigues matrix, tools/utilities.py:222-245."""
+    ux, uy, uz = u[0], u[1], u[2]
+    R = np.zeros((3, 3))
+    c = np.cos(theta)
+    s = np.sin(theta)
+    R[0, 0] = c + ux ** 2 * (1 - c)
+    R[0, 1] = ux * uy * (1 - c) - uz * s
+    R[0, 2] = ux * uz * (1 - c) + uy * s
+    R[1, 0] = uy * ux * (1 - c) + uz * s
+    R[1, 1] = c + uy ** 2 * (1 - c)
+    R[1, 2] = uy * uz * (1 - c) - ux * s
+    R[2, 0] = uz * ux * (1 - c) - uy * s
+    R[2, 1] = uz * uy * (1 - c) + ux * s
+    R[2, 2] = c + uz ** 2 * (1 - c)
+    return R
+
+
+def _apply_host(R, pts):
+    """Canonical R @ (x, y, z) on host arrays (same operations as the device)."""
+    x, y, z = pts
+    return (R[0, 0] * x + R[0, 1] * y + R[0, 2] * z,
+            R[1, 0] * x + R[1, 1] * y + R[1, 2] * z,
+            R[2, 0] * x + R[2, 1] * y + R[2, 2] * z)
+
+
+def _axis_matrix(kind, corners, deg):
+    """Rotation matrix of rotate_about_{normal,vertical,horizontal} from the
+    detector's current corner pixels [0,0], [0,-1], [-1,0]
+    (tools/detector.py:56-68, 104-110, 146-152)."""
+    x, y, z = corners
+    p1 = np.array([x[0], y[0], z[0]])
+    p2 = np.array([x[1], y[1], z[1]])
+    p3 = np.array([x[2], y[2], z[2]])
+    if kind == 'psi':
+        ax = np.cross(p2 - p1, p3 - p1)
+    elif kind == 'phi':
+        ax = p3 - p1
+    elif kind == 'theta':
+        ax = p2 - p1
+    else:
+        raise ValueError(kind)
+    ax /= np.linalg.norm(ax)
+    return rotation_matrix(ax, np.radians(deg))
+
+
+@dataclass
+class DetectorPlan:
+    n_pix: int
+    h: np.ndarray
+    v: np.ndarray
+    init_R: np.ndarray      # [n_init, 9]
+    orient_R: np.ndarray    # [O, 27]
+    orient_w: np.ndarray    # [O]
+
+
+def detector_plan(num_pixels, max_q, angle_init_vals, angle_init_axs, psis, psi_w, phis, phi_w,
+                  thetas, theta_w):
+    n = int(num_pixels)
+    h = np.linspace(-max_q, max_q, n)
+    v = np.linspace(-max_q, max_q, n)
+    corners = (np.zeros(3), np.array([h[0], h[-1], h[0]]), np.array([v[0], v[0], v[-1]]))
+    init = []
+    for val, ax in zip(angle_init_vals, angle_init_axs):
+        # each slot applies at most one of psi/phi/theta (tools/comparison.py:799-818)
+        if ax in ('psi', 'phi', 'theta'):
+            R = _axis_matrix(ax, corners, val)
+            corners = _apply_host(R, corners)
+            init.append(R.ravel())
+    mats, ws = [], []
+    for ps, pw in zip(psis, psi_w):
+        Rp = _axis_matrix('psi', corners, ps)
+        c1 = _apply_host(Rp, corners)
+        for ph, phw in zip(phis, phi_w):
+            Rv = _axis_matrix('phi', c1, ph)
+            c2 = _apply_host(Rv, c1)
+            for th, tw in zip(thetas, theta_w):
+                Rh = _axis_matrix('theta', c2, th)
+                mats.append(np.concatenate([Rp.ravel(), Rv.ravel(), Rh.ravel()]))
+                ws.append(pw * phw * tw)
+    return DetectorPlan(n_pix=n, h=h, v=v,
+                        init_R=np.array(init, dtype=np.float64).reshape(-1, 9),
+                        orient_R=np.array(mats, dtype=np.float64).reshape(-1, 27),
+                        orient_w=np.array(ws, dtype=np.float64))
+
+
+def run_detector(iq_dev, qx, qy, qz, plan: DetectorPlan, mirror):
+    """Orientation sum + finalize on the device.  ``iq_dev``: float32 or float64
+    device tensor [cy, cx, cz].  Returns the FP64 device image [n, n]."""
+    dev = device()
+    n = plan.n_pix
+    h = torch.as_tensor(plan.h).to(dev)
+    v = torch.as_tensor(plan.v).to(dev)
+    iR = torch.as_tensor(plan.init_R).to(dev).contiguous()
+    oR = torch.as_tensor(plan.orient_R).to(dev).contiguous()
+    ow = torch.as_tensor(plan.orient_w).to(dev).contiguous()
+    iq_dev = iq_dev.contiguous()
+    cy, cx, cz = iq_dev.shape
+    det = _empty(n * n, torch.float64)
+    dq = np.diff(qz)[0]
+    d = GxDetector(n_pix=n, n_init=len(plan.init_R), n_orient=len(plan.orient_w),
+                   h_axis=ptr(h).value, v_axis=ptr(v).value, init_R=ptr(iR).value,
+                   orient_R=ptr(oR).value, orient_w=ptr(ow).value, iq=ptr(iq_dev).value,
+                   iq_f64=int(iq_dev.dtype == torch.float64), cy=cy, cx=cx, cz=cz,
+                   qx_min=float(np.min(qx)), qy_min=float(np.min(qy)), qz_min=float(np.min(qz)),
+                   dq=float(dq), det=ptr(det).value)
+    st = stream_handle()
+    call("gx_detector_accumulate", ctypes.byref(d), st)
+    out = _empty(n * n, torch.float64)
+    flags = _lib.GX_FIN_FLOOR_SCALE | (_lib.GX_FIN_MIRROR if mirror else 0)
+    call("gx_detector_finalize", ptr(det), n, n, flags, ptr(out), st)
+    return out.view(n, n), det.view(n, n)

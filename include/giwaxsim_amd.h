@@ -1,0 +1,268 @@
+/*
+ * giwaxsim_amd — C ABI of the MI355X (gfx950) forward-scattering path.
+ *
+ * The reference (tchaney97/giwaxsim) is pure Python/numpy; it has no native
+ * FFI.  Each entry point below replaces the reference function(s) cited in
+ * its comment; the Python package `giwaxsim_amd.tools` binds them through
+ * ctypes behind the reference's own call signatures.
+ *
+ * Conventions
+ *   - Every function returns GX_OK (0) or an error code; gx_last_error()
+ *     returns a message for the calling thread's last failure.  No C++
+ *     exception crosses this boundary.
+ *   - All array pointers are DEVICE pointers unless the name says host_.
+ *     The library never allocates device memory: the caller passes every
+ *     buffer (sizes given by the comments), so calls are safe to enqueue
+ *     into a caller's stream and graph.
+ *   - `stream` is a hipStream_t (NULL = the null stream).  Work is only
+ *     enqueued; nothing synchronises the host.
+ *   - Structures contain only 8-byte members (int64_t, double, pointers).
+ */
+#ifndef GIWAXSIM_AMD_H
+#define GIWAXSIM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GX_ABI_VERSION 1
+
+#define GX_OK 0
+#define GX_ERR_ARG 1         /* invalid argument (message says which) */
+#define GX_ERR_UNSUPPORTED 2 /* e.g. a grid edge without a compiled FFT plan */
+#define GX_ERR_HIP 3         /* a HIP runtime call failed */
+
+typedef void *gx_stream_t;
+
+/* ------------------------------------------------------------------------------------
+ * Per-frame record: GX_FP_STRIDE doubles per phi-frame, prepared by the host
+ * planner from the frame angle (tools/voxelgrids.py:238-309, 381-402).
+ * ---------------------------------------------------------------------------------- */
+#define GX_FP_STRIDE 24
+enum {
+  GX_FP_SIN = 0,   /* sin(radians(phi))   rotate_coords_z, tools/utilities.py:305-312 */
+  GX_FP_COS,       /* cos(radians(phi)) */
+  GX_FP_HX_START,  /* det_h_qx = linspace(start, stop, N), tools/voxelgrids.py:398-401 */
+  GX_FP_HX_STEP,
+  GX_FP_HX_STOP,
+  GX_FP_HY_START,  /* det_h_qy = linspace(start, stop, N), tools/voxelgrids.py:396-400 */
+  GX_FP_HY_STEP,
+  GX_FP_HY_STOP,
+  GX_FP_LEN_MODE,  /* rectangular_collapse_lengths branch: 0 general, 1 phi==0, 2 phi==90 */
+  GX_FP_HOR,       /* hor_length / ver_length after the phi>90 swap (voxelgrids.py:253-255) */
+  GX_FP_VER,
+  GX_FP_STOP1,     /* voxelgrids.py:279-285 */
+  GX_FP_STOP2,
+  GX_FP_S12,       /* stop1 + stop2 */
+  GX_FP_MID,       /* mid_length */
+  GX_FP_A,         /* sqrt(ver^2 - (ver cos theta)^2) */
+  GX_FP_TP,        /* tan(phi) */
+  GX_FP_VC,        /* ver cos(theta) */
+  GX_FP_TT,        /* tan(theta) */
+  GX_FP_CP,        /* cos(phi) */
+  GX_FP_CT,        /* cos(theta) */
+  GX_FP_JLO,       /* first fftshifted column whose bin lies in the cropped grid */
+  GX_FP_JHI,       /* last such column (JHI < JLO: none) */
+  GX_FP_UNUSED
+};
+
+/* ------------------------------------------------------------------------------------
+ * A slab of atoms prepared for the phi-frame loop.
+ * ---------------------------------------------------------------------------------- */
+typedef struct gx_slab {
+  int64_t n_atoms;
+  int64_t n_elem;         /* entries of f_table */
+  int64_t N;              /* real-space grid edge (grid_size) */
+  double r;               /* r_voxel_size (A) */
+  double zmin;            /* min z over all atoms */
+  const double *x;        /* [n_atoms] */
+  const double *y;        /* [n_atoms] */
+  const double *z;        /* [n_atoms] */
+  const int32_t *elem;    /* [n_atoms] index into f_table */
+  const double *f_table;  /* [n_elem][2] f = Z + f1 + i f2 */
+  /* outputs, atoms regrouped by projected z row (rows 0..N-1, then rows >= N) */
+  double *sx;             /* [n_atoms] */
+  double *sy;             /* [n_atoms] */
+  int32_t *sq;            /* [2 n_atoms] fixed-point (re, im) of f, scale 2^row_exp[row] */
+  int32_t *szrow;         /* [n_atoms] z row of each regrouped atom (N for rows >= N) */
+  int32_t *row_off;       /* [N + 2] first atom of each row; row_off[N] = atoms with row < N */
+  int32_t *row_exp;       /* [N] per-row fixed-point exponent */
+  int32_t *work;          /* [N * n_elem + 2 * (N + 1)] scratch counters */
+} gx_slab;
+
+/* ------------------------------------------------------------------------------------
+ * The phi-frame pipeline of one prepared slab into a cropped 3D bin grid.
+ * ---------------------------------------------------------------------------------- */
+typedef struct gx_frames {
+  int64_t N;               /* real-space grid edge: 256, 512, 1024, 2048 or 4096 */
+  double r;                /* r_voxel_size */
+  int64_t n_atoms;         /* atoms (ymin runs over all of them) */
+  const double *ax;        /* regrouped atoms (gx_slab.sx) */
+  const double *ay;        /* gx_slab.sy */
+  const int32_t *aq;       /* gx_slab.sq */
+  const int32_t *azrow;    /* gx_slab.szrow */
+  const int32_t *row_off;  /* gx_slab.row_off */
+  const int32_t *row_exp;  /* gx_slab.row_exp */
+  /* background fill and edge smoothing (tools/voxelgrids.py:343-379) */
+  int64_t fill_bkg;        /* 0/1 */
+  int64_t smooth;          /* sigma in voxels, 0 = off */
+  double max_voxels;       /* sqrt(x_b^2 + y_b^2) // r */
+  double avg_re, avg_im;   /* avg_voxel_f */
+  const double *gauss_w;   /* [smooth_radius + 1] normalised half kernel, w[0] = centre */
+  int64_t smooth_radius;   /* int(4 sigma + 0.5) */
+  /* q geometry */
+  double qmin;             /* np.min of the full q axis */
+  double dq;               /* np.diff(qz)[0] */
+  int64_t c_lo;            /* first index of the cropped window on every axis */
+  int64_t c_n;             /* cropped edge c */
+  double q_val;            /* 1.0 / (N * r), fftfreq */
+  int64_t i_lo, i_hi;      /* fftshifted rows whose bin lies in the crop */
+  int64_t W;               /* intermediate row width (>= JHI-JLO+1 of every frame) */
+  double bin_scale;        /* fixed-point scale of I in bin sums (a power of two) */
+  const float *twiddle;    /* [N] exp(-2 pi i m / N) as float2 */
+  int64_t n_frames;        /* frames described by fparams */
+  const double *fparams;   /* [n_frames][GX_FP_STRIDE] */
+  /* per-frame scratch for a batch of up to `batch` frames */
+  int64_t batch;
+  unsigned long long *ymin_key; /* [batch] */
+  int32_t *idx_bounds;     /* [batch][4] y_min, y_max, z_min, z_max */
+  int32_t *col_nm;         /* [batch][N] num_missing_voxels per column (fill_bkg) */
+  double *col_sy;          /* [batch][N] smoothed y mask (smooth) */
+  double *col_sz;          /* [batch][N] smoothed z mask (smooth) */
+  float *inter;            /* [batch][W][N] float2: row-transformed, column-pruned frame */
+  /* output: cropped bins [c][c][c] indexed [qy][qx][qz] */
+  unsigned long long *bin_sum; /* fixed-point sum of I (scale bin_scale) */
+  int32_t *bin_cnt;
+  int32_t *status;         /* [4] counters; [0] += frames without any valid atom */
+} gx_frames;
+
+/* ------------------------------------------------------------------------------------
+ * Detector orientation sum (tools/detector.py:5-300, tools/comparison.py:790-870).
+ * ---------------------------------------------------------------------------------- */
+typedef struct gx_detector {
+  int64_t n_pix;           /* pixels per edge; image is [n_pix (v)][n_pix (h)] */
+  int64_t n_init;          /* 0..3 initial rotations */
+  int64_t n_orient;        /* orientations (psi outer, phi, theta inner) */
+  const double *h_axis;    /* [n_pix] linspace(-max_q, max_q, n) */
+  const double *v_axis;    /* [n_pix] */
+  const double *init_R;    /* [n_init][9] row-major */
+  const double *orient_R;  /* [n_orient][27] R_psi, R_phi, R_theta row-major */
+  const double *orient_w;  /* [n_orient] psi_w * phi_w * theta_w */
+  const void *iq;          /* [cy][cx][cz] cropped I(q), float or double */
+  int64_t iq_f64;          /* 1: iq is double, 0: float */
+  int64_t cy, cx, cz;
+  double qx_min, qy_min, qz_min, dq;
+  double *det;             /* [n_pix][n_pix] FP64 accumulated image (overwritten) */
+} gx_detector;
+
+const char *gx_last_error(void);
+int gx_abi_version(void);
+/* 1 if the FFT of this grid edge is compiled in. */
+int gx_supported_grid(int64_t N);
+
+/* Slab bounds: min/max of x, y, z (device out[6] = xmin,xmax,ymin,ymax,zmin,zmax)
+ * and per-element atom counts (device counts[n_elem], int64).
+ * Replaces the reductions of tools/comparison.py:712-714, 742. */
+int gx_slab_bounds(const gx_slab *s, double *out_minmax, int64_t *out_elem_count, gx_stream_t stream);
+
+/* Groups atoms by their phi-invariant projected z row
+ * (z_indices of tools/voxelgrids.py:329; the rotation leaves z unchanged) and
+ * quantises each atom's f to the row's fixed-point scale. */
+int gx_slab_sort_rows(const gx_slab *s, gx_stream_t stream);
+
+/* Per-frame statistics for frames [f0, f0+nf) of d->fparams: min rotated y
+ * (tools/voxelgrids.py:319-323) and the index bounds y_min, y_max, z_min,
+ * z_max over valid atoms (voxelgrids.py:328-349). */
+int gx_frames_stats(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream);
+/* Per-column tables: num_missing_voxels (voxelgrids.py:350-354, 238-309) and
+ * the smoothed edge masks (voxelgrids.py:363-374). */
+int gx_frames_columns(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream);
+/* Atom scatter (voxelgrids.py:337-341) + fill (355-361) + blend (377-379) +
+ * FFT along y of every z row; writes only the column band that reaches the
+ * cropped bins (d->inter). */
+int gx_frames_rowpass(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream);
+/* FFT along z of every kept column + |F|^2 (voxelgrids.py:387-392) + binning
+ * into the cropped grid with counts (process_file2, voxelgrids.py:464-503). */
+int gx_frames_colpass(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream);
+/* stats + columns + rowpass + colpass. */
+int gx_frames_forward(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream);
+
+/* Zero the cropped bins (c^3 sums and counts). */
+int gx_bins_clear(unsigned long long *bin_sum, int32_t *bin_cnt, int64_t n_bins, gx_stream_t stream);
+/* iq = (count ? sum / count : 0) * ((f0_C(|q|) / Z_C)^2) over the cropped grid
+ * (tools/comparison.py:765-788; tools/voxelgrids.py:16-48, 828-857).
+ * q axis value of crop index i: qa_step * (c_lo + i) + qa_start, or qa_stop for
+ * the last full-axis index (numpy linspace).  aff = 9 Cromer-Mann coefficients.
+ * Writes FP64 iq64 and/or FP32 iq32 (either may be NULL). */
+int gx_bins_finalize(const unsigned long long *bin_sum, const int32_t *bin_cnt, int64_t c_n,
+                     double bin_scale, int64_t c_lo, int64_t q_num, double qa_start,
+                     double qa_step, double qa_stop, const double *host_aff, double z_elem,
+                     double *iq64, float *iq32, gx_stream_t stream);
+
+/* Orientation-summed detector image (generate_detector_ints over all
+ * orientations, tools/detector.py:234-244, 278-300; intersect_detector
+ * 194-232), accumulated in FP64 in (psi, phi, theta) order. */
+int gx_detector_accumulate(const gx_detector *d, gx_stream_t stream);
+/* flags & GX_FIN_MIRROR: mirror_vertical_horizontal (tools/detector.py:246-275);
+ * then flags & GX_FIN_FLOOR_SCALE: NaN -> 1e-6, <= 0 -> 1e-6, * 1e-6
+ * (tools/comparison.py:857-868).  det and out must not alias. */
+#define GX_FIN_MIRROR 1
+#define GX_FIN_FLOOR_SCALE 2
+int gx_detector_finalize(const double *det, int64_t n_rows, int64_t n_cols, int64_t flags,
+                         double *out, gx_stream_t stream);
+
+/* (R @ vstack([x, y, z])) of n points, rows evaluated left to right without
+ * FMA (tools/detector.py:71, 113, 155; tools/utilities.py:315). */
+int gx_rotate_points(const double *host_R, const double *x, const double *y, const double *z,
+                     int64_t n, double *ox, double *oy, double *oz, gx_stream_t stream);
+
+/* One detector gather (intersect_detector, tools/detector.py:194-232) of
+ * n points into out (FP64), scaled by `weight` and added to out if
+ * accumulate != 0, else stored. */
+int gx_intersect(const double *iq, int64_t cy, int64_t cx, int64_t cz, double qx_min, double qy_min,
+                 double qz_min, double dq, const double *x, const double *y, const double *z,
+                 int64_t n, double weight, int64_t accumulate, double *out, gx_stream_t stream);
+
+/* process_file2 (tools/voxelgrids.py:464-503) on one FP64 frame image
+ * iq2d [nv][nh] with axes det_h_qx/det_h_qy [nh] and det_v_qz [nv]:
+ * masks against host_qlim = {qx_min, qx_max, qy_min, qy_max, qz_min, qz_max},
+ * bins (q - q_min) // dq and adds I and 1 into FP64 grids
+ * [q_num][q_num][q_num] indexed [qy][qx][qz].  Pixels whose bin falls
+ * outside the grid are skipped and counted in status[0] (the reference would
+ * raise IndexError). */
+int gx_bin_frame_f64(const double *iq2d, int64_t nv, int64_t nh, const double *hqx,
+                     const double *hqy, const double *vqz, const double *host_qlim, double dq,
+                     int64_t q_num, double *grid_sum, double *grid_cnt, int32_t *status,
+                     gx_stream_t stream);
+
+/* rectangular_collapse_lengths (tools/voxelgrids.py:238-309) for n x values
+ * using a frame record (host_fp, GX_FP_STRIDE doubles) from the planner. */
+int gx_collapse_lengths(const double *host_fp, const double *x, int64_t n, double *out,
+                        gx_stream_t stream);
+
+/* add_f0_q_3d (tools/voxelgrids.py:828-857) on an FP64 grid iq [ny][nx][nz]
+ * with axes qx [nx], qy [ny], qz [nz]: out = iq * ((f0(|q|)/z_elem)^2),
+ * host_aff = 9 Cromer-Mann coefficients.  out may alias iq. */
+int gx_apply_f0(const double *iq, int64_t ny, int64_t nx, int64_t nz, const double *qx, const double *qy,
+                const double *qz, const double *host_aff, double z_elem, double *out, gx_stream_t stream);
+
+/* Fold fixed-point bins into FP64 sum/count grids of the same shape:
+ * grid_sum += bin_sum / bin_scale, grid_cnt += bin_cnt (the shared-memory
+ * accumulators of tools/voxelgrids.py:471-472, 502-503). */
+int gx_bins_accumulate_f64(const unsigned long long *bin_sum, const int32_t *bin_cnt, int64_t n, double bin_scale,
+                           double *grid_sum, double *grid_cnt, gx_stream_t stream);
+
+/* Reductions for optimize_scale_offset (tools/comparison.py:873-882): over
+ * pixels with mask == 0, host_out[0..5] = n, mean_s, mean_y, Sss, Ssy, (FP64,
+ * centred second moments).  Blocks until done (host output). */
+int gx_scale_offset_moments(const double *sim, const double *exp_, const uint8_t *mask, int64_t n,
+                            double *work /* [GX_MOMENTS_WORK] */, double *host_out, gx_stream_t stream);
+#define GX_MOMENTS_WORK 774
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GIWAXSIM_AMD_H */

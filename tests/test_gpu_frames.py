@@ -1,0 +1,131 @@
+"""GPU parity of the phi-frame path against the FP64 oracle.
+
+Bar: per-voxel bin COUNTS bit-exact (they are fixed entirely by the FP64
+index arithmetic); per-frame atom index bounds bit-exact; I(q) (mean, cropped,
+f0-scaled) within 1e-5 relative error wherever the oracle value is above
+1e-6 * max; bin sums bit-identical run to run.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from giwaxsim_amd import engine
+from giwaxsim_amd.synth import graphite_slab, polymer_slab
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5       # max relative error of I(q) above the floor
+FLOOR = 1e-6      # ... relative to max I(q)
+
+
+def _case(name):
+    if name == 'graphite256':
+        c, e = graphite_slab(size=(16.0, 16.0, 16.0), seed=1)
+        return c, e, dict(r=0.25, qv=2 * np.pi / (0.25 * 256) * (1 + 1e-9), mq=2.5, fill=False, smooth=0)
+    if name == 'polymer256_fill_smooth':
+        c, e = polymer_slab(size=(24.0, 24.0, 24.0), seed=2)
+        return c, e, dict(r=0.2, qv=2 * np.pi / (0.2 * 256) * (1 + 1e-9), mq=2.0, fill=True, smooth=3)
+    if name == 'graphite512_fill':
+        c, e = graphite_slab(size=(30.0, 30.0, 20.0), seed=3)
+        return c, e, dict(r=0.25, qv=2 * np.pi / (0.25 * 512) * (1 + 1e-9), mq=2.5, fill=True, smooth=0)
+    if name == 'polymer1024_smooth':
+        c, e = polymer_slab(size=(40.0, 40.0, 30.0), seed=4)
+        return c, e, dict(r=0.2, qv=2 * np.pi / (0.2 * 1024) * (1 + 1e-9), mq=2.0, fill=True, smooth=10)
+    raise KeyError(name)
+
+
+def _run_both(name, frames=None):
+    print(f"[gpu-frames] case={name} frames={frames}", flush=True)
+    sys.stdout.flush()
+    coords, elements, p = _case(name)
+    ids, ftab, counts, _ = engine.element_table(elements, 12700.0)
+    fvals = ftab[ids]
+    g = engine.qgrid(p['qv'], p['mq'])
+    phis = g.phis if frames is None else g.phis[frames[0]:frames[1]]
+    ref = O.voxelgridmaker(coords, fvals, p['r'], p['qv'], p['mq'], fill_bkg=p['fill'],
+                           smooth=p['smooth'], phis=phis)
+    res = engine.run_voxelgrid(coords, elements, p['r'], p['qv'], p['mq'], 12700.0,
+                               fill_bkg=p['fill'], smooth=p['smooth'], f_table=ftab, elem_ids=ids,
+                               frames=frames)
+    return coords, p, ref, res
+
+
+def _crop(grid, g):
+    s = slice(g.c_lo, g.c_lo + g.c_n)
+    return grid[s, s, s]
+
+
+def _check(ref, res):
+    g = res['plan'].grid
+    cnt = res['bins'].cnt.view(g.c_n, g.c_n, g.c_n).cpu().numpy()
+    want_cnt = _crop(ref['count'], g)
+    assert np.array_equal(cnt, want_cnt.astype(np.int32)), \
+        f"{np.sum(cnt != want_cnt)} bins with different counts"
+    iq = res['iq64'].cpu().numpy()
+    want = ref['iq']
+    assert iq.shape == want.shape
+    assert np.array_equal(res['qx'], ref['qx'])
+    m = want > FLOOR * want.max()
+    rel = np.abs(iq[m] - want[m]) / want[m]
+    print(f"  max rel err {rel.max():.3e} over {m.sum()} voxels; mean {rel.mean():.3e}", flush=True)
+    assert rel.max() <= RTOL
+    # below the floor the absolute error is bounded by the floor itself
+    assert np.max(np.abs(iq[~m] - want[~m])) <= RTOL * FLOOR * want.max() * 10 + 1e-300
+    return rel.max()
+
+
+@pytest.mark.parametrize("name", ['graphite256', 'polymer256_fill_smooth'])
+def test_full_small(name):
+    _, _, ref, res = _run_both(name)
+    _check(ref, res)
+
+
+@pytest.mark.parametrize("name,frames", [('graphite512_fill', (0, 48)),
+                                         ('polymer1024_smooth', (100, 116))])
+def test_frame_subsets(name, frames):
+    _, _, ref, res = _run_both(name, frames)
+    _check(ref, res)
+
+
+def test_frame_stats_bit_exact():
+    coords, elements, p = _case('polymer256_fill_smooth')
+    ids, ftab, counts, _ = engine.element_table(elements, 12700.0)
+    res = engine.run_voxelgrid(coords, elements, p['r'], p['qv'], p['mq'], 12700.0, fill_bkg=True,
+                               smooth=3, f_table=ftab, elem_ids=ids, frames=(0, 8), batch=8)
+    runner = res['runner']
+    b = runner.idx_bounds.cpu().numpy().reshape(-1, 4)[:8]
+    g = res['plan'].grid
+    fvals = ftab[ids]
+    for f in range(8):
+        _, st = O.frame_grid(coords, fvals, g.phis[f], res['plan'].N, p['r'], 0.0,
+                             res['plan'].x_bound, res['plan'].y_bound, False, 0, return_stats=True)
+        assert tuple(b[f]) == (st['y_min'], st['y_max'], st['z_min'], st['z_max'])
+
+
+def test_row_grouping():
+    coords, elements, p = _case('graphite256')
+    ids, ftab, counts, _ = engine.element_table(elements, 12700.0)
+    res = engine.run_voxelgrid(coords, elements, p['r'], p['qv'], p['mq'], 12700.0, f_table=ftab,
+                               elem_ids=ids, frames=(0, 1))
+    slab = res['slab']
+    N = res['plan'].N
+    off = slab.row_off.cpu().numpy()
+    zi = O.floor_div(coords[:, 2] - coords[:, 2].min(), p['r']).astype(int)
+    hist = np.bincount(np.minimum(zi, N), minlength=N + 1)
+    assert np.array_equal(np.diff(off[:N + 2]), hist)
+    rows = slab.szrow.cpu().numpy()
+    assert np.array_equal(np.sort(rows), np.sort(np.minimum(zi, N)))
+
+
+def test_deterministic():
+    coords, elements, p = _case('polymer256_fill_smooth')
+    ids, ftab, counts, _ = engine.element_table(elements, 12700.0)
+    sums = []
+    for _ in range(2):
+        res = engine.run_voxelgrid(coords, elements, p['r'], p['qv'], p['mq'], 12700.0,
+                                   fill_bkg=True, smooth=3, f_table=ftab, elem_ids=ids)
+        sums.append(res['bins'].sum.cpu().numpy().copy())
+    assert np.array_equal(sums[0], sums[1])
