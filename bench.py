@@ -1,0 +1,174 @@
+"""Benchmark of the phi-frame forward path (project + FFT + 3D bin) on one GPU.
+
+Workload (the production-size synthetic config): a semicrystalline polymer
+slab of 100 x 100 x 100 A (~110k C/H/N/O/S/F atoms), r_voxel_size 0.2 A, real
+grid 1024 x 1024 (q_voxel = 2 pi / (0.2 * 1024)), energy 12700 eV, fill_bkg,
+smooth 10 voxels (2 A), max_q 2.0 -> 581 phi-frames into a 185^3 bin grid
+cropped to 131^3.  One step = every phi-frame of the slab, starting from
+atoms resident in HBM, through the cropped FP64 I(q) on the device.  The
+detector stage (512 x 512 px, 0.15 deg psi tilt, psi 0..180 in 0.5 deg steps,
+361 orientations) is timed separately and reported as images/s.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: the path runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-frames", type=int, default=48,
+                    help="phi-frames of the same slab timed on the host (CPU baseline sample)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-iters", type=int, default=20)
+    return ap.parse_args()
+
+
+def workload():
+    from giwaxsim_amd.synth import polymer_slab
+    coords, elements = polymer_slab(size=(100.0, 100.0, 100.0), seed=0)
+    r = 0.2
+    return dict(coords=coords, elements=elements, r=r, qv=2 * np.pi / (r * 1024) * (1 + 1e-9),
+                max_q=2.0, energy=12700.0, fill=True, smooth=10)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+    import torch
+    from giwaxsim_amd import engine, _lib
+    from giwaxsim_amd._lib import call
+
+    W = workload()
+    dev = engine.device()
+    ids, ftab, counts, _ = engine.element_table(W['elements'], W['energy'])
+    coords_d = torch.as_tensor(W['coords']).to(dev)
+    ids_d = torch.as_tensor(ids).to(dev)
+
+    def step():
+        return engine.run_voxelgrid(coords_d, None, W['r'], W['qv'], W['max_q'], W['energy'],
+                                    fill_bkg=W['fill'], smooth=W['smooth'], f_table=ftab,
+                                    elem_ids=ids_d, want64=True)
+
+    for _ in range(args.warmup):
+        res = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    plan = res['plan']
+    F = plan.n_frames
+    N = plan.N
+    n_atoms = len(W['coords'])
+
+    # ---- dominant kernel, timed live with HIP events on the stream it runs on
+    runner = res['runner']
+    st = torch.cuda.current_stream()
+    nb = runner.batch
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    call("gx_frames_stats", ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
+    call("gx_frames_columns", ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
+    times = {}
+    for kname in ("gx_frames_rowpass", "gx_frames_colpass"):
+        call(kname, ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
+        ev[0].record(st)
+        for _ in range(args.kernel_iters):
+            call(kname, ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
+        ev[1].record(st)
+        torch.cuda.synchronize()
+        times[kname] = ev[0].elapsed_time(ev[1]) / args.kernel_iters / 1e3  # s per launch
+    runner.bins.clear()
+    dom = max(times, key=times.get)
+    # algorithmic bytes per frame (fixed contract): 24 n_atoms + 16 N^2 + 16 P.
+    P_mean = float(res['bins'].cnt.sum().item()) / F
+    B_frame = 24 * n_atoms + 16 * N * N + 16 * P_mean
+    # the row pass is charged the atom reads and the frame write (24 n_a + 8 N^2),
+    # the column pass the frame read and the bin read-modify-writes (8 N^2 + 16 P)
+    alg = {"gx_frames_rowpass": nb * (24 * n_atoms + 8 * N * N),
+           "gx_frames_colpass": nb * (8 * N * N + 16 * P_mean)}
+    achieved = alg[dom] / times[dom] / 1e9
+
+    # ---- detector images/s
+    from giwaxsim_amd.tools.comparison import _weights
+    psis = np.linspace(0, 180, 361)
+    one = np.array([0.0])
+    dplan = engine.detector_plan(512, W['max_q'], (0.15, 0, 0), ('psi', 'None', 'None'), psis,
+                                 _weights(None, psis), one, _weights(None, one), one,
+                                 _weights(None, one))
+    iq = res['iq64']
+    engine.run_detector(iq, res['qx'], res['qy'], res['qz'], dplan, True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nd = max(3, args.steps // 4)
+    for _ in range(nd):
+        engine.run_detector(iq, res['qx'], res['qy'], res['qz'], dplan, True)
+    torch.cuda.synchronize()
+    det_s = (time.perf_counter() - t1) / nd
+
+    # ---- CPU baseline: the oracle in reference-faithful threaded mode on a bounded sample
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        phis = plan.grid.phis[:args.cpu_frames]
+        workers = O.default_threads()
+        tc = time.perf_counter()
+        O.voxelgridmaker(W['coords'], ftab[ids], W['r'], W['qv'], W['max_q'], fill_bkg=W['fill'],
+                         smooth=W['smooth'], phis=phis, threads=None)
+        tcpu = time.perf_counter() - tc
+        cpu = {"value": len(phis) / tcpu, "unit": "phi-frames/s", "cores": workers, "kind": "port",
+               "sample": f"{len(phis)} of {F} phi-frames of the same slab, FP64 numpy oracle with "
+                         f"ThreadPoolExecutor() default workers ({workers}) on {os.cpu_count()} "
+                         f"host CPUs; {tcpu:.1f} s"}
+
+    out = {
+        "metric": "phi-frames/s (project+FFT+3D bin)",
+        "value": F / dt,
+        "unit": "phi-frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 FFT/|F|^2, f64 indices, int64 fixed-point bin sums",
+        "data": "synthetic (seeded polymer slab, 100^3 A)",
+        "config": {"workload": "polymer slab 100^3 A, 110k atoms, N=1024, 581 phi-frames, fill_bkg, "
+                               "smooth 10 vox, max_q 2.0, bins 185^3 cropped to 131^3",
+                   "n_atoms": n_atoms, "grid": N, "frames": F, "W": plan.W, "batch": runner.batch},
+        "hbm_gbs_algorithmic_step": B_frame * F / dt / 1e9,
+        "detector_images_per_s": 1.0 / det_s,
+        "detector_ms_per_image": det_s * 1e3,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "launch_ms": times[dom] * 1e3, "frames_per_launch": nb,
+                     "kernel_ms": {k: v * 1e3 for k, v in times.items()}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -80,33 +80,46 @@ __device__ __forceinline__ int wave_max_i(int v) {
 }
 
 // ------------------------------------------------------------------------------------
-// complex FP32
+// complex arithmetic on float2 (FP32 transforms) and double2 (FP64 transforms)
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+template <typename C> struct Real;
+template <> struct Real<float2> { using T = float; };
+template <> struct Real<double2> { using T = double; };
+
+__device__ __forceinline__ float2 mk(float a, float b, float2) { return make_float2(a, b); }
+__device__ __forceinline__ double2 mk(double a, double b, double2) { return make_double2(a, b); }
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
 // multiply by -i
-__device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+template <typename C> __device__ __forceinline__ C cmul_mi(C a) { return C{a.y, -a.x}; }
+// |a|^2
+__device__ __forceinline__ float cnorm(float2 a) { return fmaf(a.x, a.x, a.y * a.y); }
+__device__ __forceinline__ double cnorm(double2 a) { return fma(a.x, a.x, a.y * a.y); }
 
 // ------------------------------------------------------------------------------------
 // forward DFTs of small radix in registers, W_R = exp(-2 pi i / R), natural order
 // ------------------------------------------------------------------------------------
-template <int R> struct Dft;
+template <int R, typename C> struct Dft;
 
-template <> struct Dft<2> {
-  static __device__ __forceinline__ void run(float2* v) {
-    float2 a = v[0], b = v[1];
+template <typename C> struct Dft<2, C> {
+  static __device__ __forceinline__ void run(C* v) {
+    C a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
   }
 };
 
-template <> struct Dft<4> {
-  static __device__ __forceinline__ void run(float2* v) {
-    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-    float2 t2 = cadd(v[1], v[3]), t3 = cmul_mi(csub(v[1], v[3]));
+template <typename C> struct Dft<4, C> {
+  static __device__ __forceinline__ void run(C* v) {
+    C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    C t2 = cadd(v[1], v[3]), t3 = cmul_mi(csub(v[1], v[3]));
     v[0] = cadd(t0, t2);
     v[2] = csub(t0, t2);
     v[1] = cadd(t1, t3);
@@ -114,16 +127,17 @@ template <> struct Dft<4> {
   }
 };
 
-template <> struct Dft<8> {
-  static __device__ __forceinline__ void run(float2* v) {
-    const float c = 0.70710678118654752440f;
-    float2 e[4] = {v[0], v[2], v[4], v[6]};
-    float2 o[4] = {v[1], v[3], v[5], v[7]};
-    Dft<4>::run(e);
-    Dft<4>::run(o);
-    float2 w1 = make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));   // * (c, -c)
-    float2 w2 = cmul_mi(o[2]);                                               // * -i
-    float2 w3 = make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));  // * (-c, -c)
+template <typename C> struct Dft<8, C> {
+  static __device__ __forceinline__ void run(C* v) {
+    using T = typename Real<C>::T;
+    const T c = (T)0.70710678118654752440;
+    C e[4] = {v[0], v[2], v[4], v[6]};
+    C o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, C>::run(e);
+    Dft<4, C>::run(o);
+    C w1 = C{c * (o[1].x + o[1].y), c * (o[1].y - o[1].x)};   // * (c, -c)
+    C w2 = cmul_mi(o[2]);                                     // * -i
+    C w3 = C{c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y)};  // * (-c, -c)
     v[0] = cadd(e[0], o[0]);
     v[4] = csub(e[0], o[0]);
     v[1] = cadd(e[1], w1);
@@ -135,25 +149,25 @@ template <> struct Dft<8> {
   }
 };
 
-template <> struct Dft<16> {
-  static __device__ __forceinline__ void run(float2* v) {
+template <typename C> struct Dft<16, C> {
+  static __device__ __forceinline__ void run(C* v) {
+    using T = typename Real<C>::T;
     // W16^k = (cos(2 pi k/16), -sin(2 pi k/16))
-    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
-    const float c2 = 0.70710678118654752440f;
-    float2 e[8], o[8];
+    const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173;
+    const T c2 = (T)0.70710678118654752440;
+    C e[8], o[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       e[i] = v[2 * i];
       o[i] = v[2 * i + 1];
     }
-    Dft<8>::run(e);
-    Dft<8>::run(o);
-    const float2 w[8] = {make_float2(1.f, 0.f),  make_float2(c1, -s1), make_float2(c2, -c2),
-                         make_float2(s1, -c1),   make_float2(0.f, -1.f), make_float2(-s1, -c1),
-                         make_float2(-c2, -c2), make_float2(-c1, -s1)};
+    Dft<8, C>::run(e);
+    Dft<8, C>::run(o);
+    const C w[8] = {C{(T)1, (T)0}, C{c1, -s1}, C{c2, -c2}, C{s1, -c1},
+                    C{(T)0, (T)-1}, C{-s1, -c1}, C{-c2, -c2}, C{-c1, -s1}};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      float2 t = (k == 0) ? o[0] : (k == 4 ? cmul_mi(o[4]) : cmul(o[k], w[k]));
+      C t = (k == 0) ? o[0] : (k == 4 ? cmul_mi(o[4]) : cmul(o[k], w[k]));
       v[k] = cadd(e[k], t);
       v[k + 8] = csub(e[k], t);
     }
@@ -166,14 +180,14 @@ template <> struct Dft<16> {
 // Ns = product of the previous radices works on butterflies j = lane + 64 t:
 //   inputs  x[j + r N/R]  (r < R), twiddled by W_{Ns R}^{r (j mod Ns)},
 //   outputs y[(j / Ns) Ns R + (j mod Ns) + r Ns].
-// Data moves between stages through a per-wave LDS buffer of N float2.
+// Data moves between stages through a per-wave LDS buffer of N complex values.
 // The first stage's inputs are supplied by the caller in registers
 // (v[t R + r] = x[j + r N/R]); the last stage leaves its outputs in
 // registers, v[t R + r] = X[j + r N/R] with j = lane + 64 t.
-// tw[m] = exp(-2 pi i m / N), m < N, precomputed in FP64 and rounded to FP32.
+// tw[m] = exp(-2 pi i m / N), m < N, precomputed in FP64 (rounded for FP32).
 // ------------------------------------------------------------------------------------
-template <int N, int R, int NS, int P>
-__device__ __forceinline__ void fft_butterflies(float2 (&v)[P], const float2* __restrict__ tw, int lane) {
+template <int N, int R, int NS, int P, typename C>
+__device__ __forceinline__ void fft_butterflies(C (&v)[P], const C* __restrict__ tw, int lane) {
   constexpr int NB = P / R;
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
@@ -186,12 +200,12 @@ __device__ __forceinline__ void fft_butterflies(float2 (&v)[P], const float2* __
         v[t * R + r] = cmul(v[t * R + r], tw[m]);
       }
     }
-    Dft<R>::run(&v[t * R]);
+    Dft<R, C>::run(&v[t * R]);
   }
 }
 
-template <int N, int R, int NS, int P>
-__device__ __forceinline__ void fft_store(float2 (&v)[P], float2* lds, int lane) {
+template <int N, int R, int NS, int P, typename C>
+__device__ __forceinline__ void fft_store(C (&v)[P], C* lds, int lane) {
   constexpr int NB = P / R;
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
@@ -202,8 +216,8 @@ __device__ __forceinline__ void fft_store(float2 (&v)[P], float2* lds, int lane)
   }
 }
 
-template <int N, int R, int P>
-__device__ __forceinline__ void fft_load(float2 (&v)[P], const float2* lds, int lane) {
+template <int N, int R, int P, typename C>
+__device__ __forceinline__ void fft_load(C (&v)[P], const C* lds, int lane) {
   constexpr int NB = P / R;
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
@@ -213,7 +227,7 @@ __device__ __forceinline__ void fft_load(float2 (&v)[P], const float2* lds, int 
   }
 }
 
-// Radix plans (first stage radix is the first template argument).
+// Radix plans (first stage radix is R0).
 template <int N> struct FftPlan;
 template <> struct FftPlan<256>  { static constexpr int R0 = 4,  R1 = 4,  R2 = 4, R3 = 4; static constexpr int NST = 4; };
 template <> struct FftPlan<512>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 8, R3 = 1; static constexpr int NST = 3; };
@@ -229,30 +243,29 @@ template <int N> struct FftLast {
 // Runs the whole transform.  On entry v holds the first-stage inputs
 // (v[t R0 + r] = x[lane + 64 t + r N/R0]); on exit v holds the outputs in
 // last-stage order (v[t RL + r] = X[lane + 64 t + r N/RL]).  `lds` is this
-// wave's private buffer of N float2; it is free again on return.
-template <int N>
-__device__ __forceinline__ void fft_wave(float2 (&v)[N / GX_WAVE], float2* lds,
-                                         const float2* __restrict__ tw, int lane) {
+// wave's private buffer of N complex values; it is free again on return.
+template <int N, typename C>
+__device__ __forceinline__ void fft_wave(C (&v)[N / GX_WAVE], C* lds, const C* __restrict__ tw, int lane) {
   constexpr int P = N / GX_WAVE;
   using PL = FftPlan<N>;
   constexpr int R0 = PL::R0, R1 = PL::R1, R2 = PL::R2, R3 = PL::R3;
-  fft_butterflies<N, R0, 1, P>(v, tw, lane);
-  fft_store<N, R0, 1, P>(v, lds, lane);
+  fft_butterflies<N, R0, 1, P, C>(v, tw, lane);
+  fft_store<N, R0, 1, P, C>(v, lds, lane);
   wave_sync();
-  fft_load<N, R1, P>(v, lds, lane);
+  fft_load<N, R1, P, C>(v, lds, lane);
   wave_sync();
-  fft_butterflies<N, R1, R0, P>(v, tw, lane);
-  fft_store<N, R1, R0, P>(v, lds, lane);
+  fft_butterflies<N, R1, R0, P, C>(v, tw, lane);
+  fft_store<N, R1, R0, P, C>(v, lds, lane);
   wave_sync();
-  fft_load<N, R2, P>(v, lds, lane);
+  fft_load<N, R2, P, C>(v, lds, lane);
   wave_sync();
-  fft_butterflies<N, R2, R0 * R1, P>(v, tw, lane);
+  fft_butterflies<N, R2, R0 * R1, P, C>(v, tw, lane);
   if constexpr (PL::NST == 4) {
-    fft_store<N, R2, R0 * R1, P>(v, lds, lane);
+    fft_store<N, R2, R0 * R1, P, C>(v, lds, lane);
     wave_sync();
-    fft_load<N, R3, P>(v, lds, lane);
+    fft_load<N, R3, P, C>(v, lds, lane);
     wave_sync();
-    fft_butterflies<N, R3, R0 * R1 * R2, P>(v, tw, lane);
+    fft_butterflies<N, R3, R0 * R1 * R2, P, C>(v, tw, lane);
   }
 }
 

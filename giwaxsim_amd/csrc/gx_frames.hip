@@ -312,16 +312,18 @@ __global__ __launch_bounds__(BLK) void k_frames_columns(gx_frames d, int64_t f0,
 // ------------------------------------------------------------------------------------
 // row pass
 // ------------------------------------------------------------------------------------
-template <int N, int RPW>
+template <int N, int RPW, typename C>
 __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
+  using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
   constexpr int RB = (BLK / GX_WAVE) * RPW;  // rows per block
   constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
-  extern __shared__ float2 smem[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *smem = reinterpret_cast<C *>(smem_raw);
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
-  float2 *wbuf = smem + wave * N;
-  float2 *stage = smem + (BLK / GX_WAVE) * N;  // [W][RB]
+  C *wbuf = smem + wave * N;
+  C *stage = smem + (BLK / GX_WAVE) * N;  // [W][RB]
   int32_t *acc_re = reinterpret_cast<int32_t *>(wbuf);
   int32_t *acc_im = acc_re + N;
 
@@ -345,7 +347,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
   const int32_t *nm_row = d.col_nm + (int64_t)fb * N;
   const double *sy_row = d.col_sy + (int64_t)fb * N;
   const double *sz_row = d.col_sz + (int64_t)fb * N;
-  const float2 *tw = reinterpret_cast<const float2 *>(d.twiddle);
+  const C *tw = reinterpret_cast<const C *>(d.twiddle);
   const int z0 = blockIdx.x * RB;
 
   for (int rr = 0; rr < RPW; ++rr) {
@@ -353,16 +355,16 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
     const int z = z0 + rb;
     const int a0 = d.row_off[z], a1 = d.row_off[z + 1];
     const bool outside_z = d.fill_bkg && (z < zlo || z >= zhi);
-    float2 v[P];
+    C v[P];
     if (!d.fill_bkg && !d.smooth && a0 == a1) {
       // empty row: transform is zero
 #pragma unroll
-      for (int q = 0; q < P; ++q) v[q] = make_float2(0.f, 0.f);
+      for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
     } else if (outside_z && !d.smooth) {
       // constant row avg_voxel_f * max_voxels: only the k = 0 term survives
 #pragma unroll
-      for (int q = 0; q < P; ++q) v[q] = make_float2(0.f, 0.f);
-      if (lane == 0) v[0] = make_float2((float)(avv_re * (double)N), (float)(avv_im * (double)N));
+      for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
+      if (lane == 0) v[0] = C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)};
     } else {
       // 1) zero the fixed-point accumulators
       for (int y = lane; y < N; y += GX_WAVE) {
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
         }
       }
       wave_sync();
-      // 3) grid values of the first-stage inputs, FP64 then rounded to FP32
+      // 3) grid values of the first-stage inputs in FP64, rounded once to T
       const double inv = ldexp(1.0, -d.row_exp[z]);
       const double mz = d.smooth ? sz_row[z] : 0.0;
 #pragma unroll
@@ -405,12 +407,12 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
             gre = gre * m + avv_re * om;
             gim = gim * m + avv_im * om;
           }
-          v[t * R0 + q] = make_float2((float)gre, (float)gim);
+          v[t * R0 + q] = C{(T)gre, (T)gim};
         }
       }
       wave_sync();
       // 4) FFT along y
-      fft_wave<N>(v, wbuf, tw, lane);
+      fft_wave<N, C>(v, wbuf, tw, lane);
     }
     // 5) keep the fftshifted columns [jlo, jhi] in the staging tile
 #pragma unroll
@@ -427,7 +429,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
   }
   __syncthreads();
   // 6) transposed store: inter[fb][w][z0 .. z0+RB)
-  float2 *out = reinterpret_cast<float2 *>(d.inter) + (int64_t)fb * d.W * N;
+  C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * N;
   for (int idx = threadIdx.x; idx < wf * RB; idx += BLK) {
     const int w = idx / RB, q = idx - w * RB;
     out[(int64_t)w * N + z0 + q] = stage[idx];
@@ -437,11 +439,11 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
 // ------------------------------------------------------------------------------------
 // column pass
 // ------------------------------------------------------------------------------------
-template <int N>
+template <int N, typename C>
 __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
   constexpr int P = N / GX_WAVE;
   constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
-  __shared__ float2 lds[BLK / GX_WAVE][N];
+  __shared__ C lds[BLK / GX_WAVE][N];
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
   const int fb = blockIdx.y;
@@ -459,14 +461,14 @@ __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
   const long long by = to_int(floor_div(hy - d.qmin, d.dq, dqinv)) - d.c_lo;
   if (bx < 0 || bx >= d.c_n || by < 0 || by >= d.c_n) return;
 
-  const float2 *col = reinterpret_cast<const float2 *>(d.inter) + ((int64_t)fb * d.W + wcol) * N;
-  float2 v[P];
+  const C *col = reinterpret_cast<const C *>(d.inter) + ((int64_t)fb * d.W + wcol) * N;
+  C v[P];
 #pragma unroll
   for (int t = 0; t < P / R0; ++t)
 #pragma unroll
     for (int q = 0; q < R0; ++q) v[t * R0 + q] = col[lane + GX_WAVE * t + q * (N / R0)];
-  const float2 *tw = reinterpret_cast<const float2 *>(d.twiddle);
-  fft_wave<N>(v, lds[wave], tw, lane);
+  const C *tw = reinterpret_cast<const C *>(d.twiddle);
+  fft_wave<N, C>(v, lds[wave], tw, lane);
 
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;
   const double val = d.q_val;
@@ -482,9 +484,8 @@ __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
       const double qz = (((double)kint * val) * 2.0) * M_PI;
       const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
       if (bz < 0 || bz >= d.c_n) continue;
-      const float2 F = v[t * RL + q];
-      const float I = fmaf(F.x, F.x, F.y * F.y);
-      const unsigned long long fx = (unsigned long long)llrint((double)I * d.bin_scale);
+      const double I = (double)cnorm(v[t * RL + q]);
+      const unsigned long long fx = (unsigned long long)llrint(I * d.bin_scale);
       atomicAdd(&d.bin_sum[cbase + bz], fx);
       atomicAdd(&d.bin_cnt[cbase + bz], 1);
     }
@@ -541,29 +542,45 @@ __global__ __launch_bounds__(BLK) void k_bins_finalize(const unsigned long long 
   }
 }
 
-template <int N>
-int launch_rowpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+template <int N, typename C>
+int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int RPW = (N <= 1024) ? 2 : 1;
   constexpr int RB = (BLK / GX_WAVE) * RPW;
-  size_t lds = ((size_t)(BLK / GX_WAVE) * N + (size_t)d->W * RB) * sizeof(float2);
+  size_t lds = ((size_t)(BLK / GX_WAVE) * N + (size_t)d->W * RB) * sizeof(C);
   if (lds > 160 * 1024)
-    return set_error(GX_ERR_UNSUPPORTED, "row pass needs %zu bytes of LDS (W=%lld too wide)", lds,
-                     (long long)d->W);
+    return set_error(GX_ERR_UNSUPPORTED, "row pass needs %zu bytes of LDS (N=%d, W=%lld, %s)", lds, N,
+                     (long long)d->W, sizeof(C) == 16 ? "FP64" : "FP32");
   static bool attr_set = false;
   if (!attr_set) {
-    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass<N, RPW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass<N, RPW, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                160 * 1024));
     attr_set = true;
   }
   dim3 grid(N / RB, (unsigned)nf);
-  k_rowpass<N, RPW><<<grid, BLK, lds, st>>>(*d, f0);
+  k_rowpass<N, RPW, C><<<grid, BLK, lds, st>>>(*d, f0);
   return check_launch("k_rowpass");
+}
+
+template <int N>
+int launch_rowpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  if (d->fft_f64) {
+    if constexpr (N <= 2048) return launch_rowpass_t<N, double2>(d, f0, nf, st);
+    return set_error(GX_ERR_UNSUPPORTED, "FP64 transforms support N <= 2048");
+  }
+  return launch_rowpass_t<N, float2>(d, f0, nf, st);
 }
 
 template <int N>
 int launch_colpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   dim3 grid(grid1d(d->W, BLK / GX_WAVE), (unsigned)nf);
-  k_colpass<N><<<grid, BLK, 0, st>>>(*d, f0);
+  if (d->fft_f64) {
+    if constexpr (N <= 2048) {
+      k_colpass<N, double2><<<grid, BLK, 0, st>>>(*d, f0);
+      return check_launch("k_colpass");
+    }
+    return set_error(GX_ERR_UNSUPPORTED, "FP64 transforms support N <= 2048");
+  }
+  k_colpass<N, float2><<<grid, BLK, 0, st>>>(*d, f0);
   return check_launch("k_colpass");
 }
 

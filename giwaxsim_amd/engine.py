@@ -282,13 +282,33 @@ def build_frame_plan(N, r, g: QGrid, x_bound, y_bound) -> FramePlan:
 
 
 @functools.lru_cache(maxsize=16)
-def twiddles(N: int):
+def twiddles(N: int, f64: bool = False):
+    """exp(-2 pi i m / N), m < N, interleaved (re, im), computed in FP64."""
     m = np.arange(N, dtype=np.float64)
     ang = -2.0 * np.pi * m / N
-    tw = np.empty(2 * N, dtype=np.float32)
+    tw = np.empty(2 * N, dtype=np.float64 if f64 else np.float32)
     tw[0::2] = np.cos(ang)
     tw[1::2] = np.sin(ang)
     return tw
+
+
+def fft_precision(fill_bkg, smooth, precision='auto'):
+    """True for FP64 transforms.
+
+    'auto': FP32 for dense grids (fill_bkg or smooth) and FP64 for sparse
+    ones.  The FP32 transform error of a pixel is ~eps * sqrt(log2 N^2) *
+    ||G||_2, while the smallest intensities that must be accurate are 1e-6 of
+    |sum G|^2; for a sparse grid of n atoms |sum G| / ||G||_2 ~ sqrt(n), which
+    leaves FP32 at ~1e-5 relative error at that floor for n ~ 1e4, whereas a
+    background-filled grid has |sum G| / ||G||_2 ~ N and FP32 stays at a few
+    1e-6 or below."""
+    if precision == 'fp64':
+        return True
+    if precision == 'fp32':
+        return False
+    if precision != 'auto':
+        raise ValueError(f"precision must be 'auto', 'fp32' or 'fp64', not {precision!r}")
+    return not (fill_bkg or smooth)
 
 
 def gauss_half_kernel(sigma):
@@ -421,24 +441,26 @@ class FrameRunner:
     """Runs every phi-frame of one slab into shared bins, in batches."""
 
     def __init__(self, plan: FramePlan, slab: Slab, bins: Bins, fill_bkg, smooth, avg,
-                 batch=None):
+                 batch=None, precision='auto'):
         dev = device()
         N, W = plan.N, plan.W
         self.plan, self.slab, self.bins = plan, slab, bins
+        self.f64 = fft_precision(fill_bkg, smooth, precision)
+        esz = 16 if self.f64 else 8
         F = plan.n_frames
         if batch is None:
-            # keep the batch's intermediate (batch * W * N * 8 B) within ~160 MB
-            batch = max(1, min(F, int(160e6 // (W * N * 8))))
+            # keep the batch's intermediate (batch * W * N complex) within ~160 MB
+            batch = max(1, min(F, int(160e6 // (W * N * esz))))
         self.batch = int(batch)
         self.fparams = torch.as_tensor(plan.fparams).to(dev).contiguous()
-        self.tw = torch.as_tensor(twiddles(N)).to(dev).contiguous()
+        self.tw = torch.as_tensor(twiddles(N, self.f64)).to(dev).contiguous()
         B = self.batch
         self.ymin_key = _empty(B, torch.int64)
         self.idx_bounds = _empty(4 * B, torch.int32)
         self.col_nm = _empty(B * N, torch.int32) if fill_bkg else None
         self.col_sy = _empty(B * N, torch.float64) if smooth else None
         self.col_sz = _empty(B * N, torch.float64) if smooth else None
-        self.inter = _empty(B * W * N * 2, torch.float32)
+        self.inter = _empty(B * W * N * 2, torch.float64 if self.f64 else torch.float32)
         self.status = torch.zeros(4, dtype=torch.int32, device=dev)
         if smooth:
             w, rad = gauss_half_kernel(smooth)
@@ -454,7 +476,8 @@ class FrameRunner:
             fill_bkg=int(bool(fill_bkg)), smooth=int(smooth), max_voxels=plan.max_voxels,
             avg_re=float(np.real(avg)), avg_im=float(np.imag(avg)), gauss_w=ptr(self.gauss).value,
             smooth_radius=rad, qmin=g.qmin, dq=g.dq, c_lo=g.c_lo, c_n=g.c_n, q_val=plan.q_val,
-            i_lo=plan.i_lo, i_hi=plan.i_hi, W=W, bin_scale=bins.scale, twiddle=ptr(self.tw).value,
+            i_lo=plan.i_lo, i_hi=plan.i_hi, W=W, bin_scale=bins.scale, fft_f64=int(self.f64),
+            twiddle=ptr(self.tw).value,
             n_frames=F, fparams=ptr(self.fparams).value, batch=self.batch,
             ymin_key=ptr(self.ymin_key).value, idx_bounds=ptr(self.idx_bounds).value,
             col_nm=ptr(self.col_nm).value, col_sy=ptr(self.col_sy).value,
@@ -479,13 +502,14 @@ class FrameRunner:
 
 
 def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, fill_bkg=False,
-                  smooth=0, f_table=None, elem_ids=None, batch=None, want64=True, frames=None):
+                  smooth=0, f_table=None, elem_ids=None, batch=None, want64=True, frames=None,
+                  precision='auto'):
     """Full voxel-grid path of one slab.  Returns dict with device iq32/iq64,
     host axes and the objects that produced them.  ``frames=(lo, hi)``
     restricts the run to phi-frames [lo, hi) (bounded samples)."""
     if f_table is None:
         elem_ids, f_table, counts, uniq = element_table(elements, energy)
-    else:
+    elif not torch.is_tensor(elem_ids):
         counts = np.bincount(np.asarray(elem_ids), minlength=len(f_table))
     if torch.is_tensor(coords):
         xb, yb, zb, zmin, counts = slab_bounds_device(coords, elem_ids, len(f_table))
@@ -510,7 +534,7 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
     scale = bin_scale_for([(plan.n_frames, pixels_per_bin(plan), B)])
     bins = Bins(plan.grid, scale)
     slab = Slab(coords, elem_ids, f_table, plan.N, r_voxel_size, zmin)
-    runner = FrameRunner(plan, slab, bins, fill_bkg, smooth, avg, batch=batch)
+    runner = FrameRunner(plan, slab, bins, fill_bkg, smooth, avg, batch=batch, precision=precision)
     runner.run(frames)
     runner.check()
     iq64, iq32 = bins.finalize(want64=want64, want32=True)

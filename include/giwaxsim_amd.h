@@ -96,7 +96,7 @@ typedef struct gx_slab {
  * The phi-frame pipeline of one prepared slab into a cropped 3D bin grid.
  * ---------------------------------------------------------------------------------- */
 typedef struct gx_frames {
-  int64_t N;               /* real-space grid edge: 256, 512, 1024, 2048 or 4096 */
+  int64_t N;               /* real-space grid edge: 256, 512, 1024, 2048 or 4096 (FP64: <= 2048) */
   double r;                /* r_voxel_size */
   int64_t n_atoms;         /* atoms (ymin runs over all of them) */
   const double *ax;        /* regrouped atoms (gx_slab.sx) */
@@ -121,7 +121,8 @@ typedef struct gx_frames {
   int64_t i_lo, i_hi;      /* fftshifted rows whose bin lies in the crop */
   int64_t W;               /* intermediate row width (>= JHI-JLO+1 of every frame) */
   double bin_scale;        /* fixed-point scale of I in bin sums (a power of two) */
-  const float *twiddle;    /* [N] exp(-2 pi i m / N) as float2 */
+  int64_t fft_f64;         /* 0: FP32 transforms (float2), 1: FP64 transforms (double2) */
+  const void *twiddle;     /* [N] exp(-2 pi i m / N) as float2 or double2 (fft_f64) */
   int64_t n_frames;        /* frames described by fparams */
   const double *fparams;   /* [n_frames][GX_FP_STRIDE] */
   /* per-frame scratch for a batch of up to `batch` frames */
@@ -131,7 +132,7 @@ typedef struct gx_frames {
   int32_t *col_nm;         /* [batch][N] num_missing_voxels per column (fill_bkg) */
   double *col_sy;          /* [batch][N] smoothed y mask (smooth) */
   double *col_sz;          /* [batch][N] smoothed z mask (smooth) */
-  float *inter;            /* [batch][W][N] float2: row-transformed, column-pruned frame */
+  void *inter;             /* [batch][W][N] float2 or double2: row-transformed, column-pruned frame */
   /* output: cropped bins [c][c][c] indexed [qy][qx][qz] */
   unsigned long long *bin_sum; /* fixed-point sum of I (scale bin_scale) */
   int32_t *bin_cnt;

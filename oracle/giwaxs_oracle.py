@@ -25,7 +25,7 @@ __all__ = [
     "add_f0_q_3d", "voxelgridmaker", "make_detector", "rotate_about_normal",
     "rotate_about_horizontal", "rotate_about_vertical", "intersect_detector",
     "rotate_psi_phi_theta", "mirror_vertical_horizontal", "detectormaker",
-    "optimize_scale_offset", "avg_voxel_f_of",
+    "optimize_scale_offset", "avg_voxel_f_of", "default_threads",
 ]
 
 # Cromer-Mann coefficients of carbon (a1,b1,a2,b2,a3,b3,a4,b4,c), International

@@ -1,0 +1,155 @@
+"""GPU parity of the reference-compatible helpers (tools/voxelgrids.py,
+tools/utilities.py, tools/comparison.py) and of the config-file entry point
+against the FP64 oracle."""
+
+import os
+from multiprocessing import shared_memory
+
+import numpy as np
+import pytest
+
+import oracle as O
+from giwaxsim_amd import engine
+from giwaxsim_amd.tools import comparison as C
+from giwaxsim_amd.tools import utilities as U
+from giwaxsim_amd.tools import voxelgrids as V
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+def test_collapse_lengths_bit_exact():
+    x = np.arange(1024) * 0.2
+    for phi in (0.0, 0.31, 45.0, 89.99, 90.0, 133.7):
+        got = V.rectangular_collapse_lengths(x, 100.0, 37.5, phi)
+        assert np.array_equal(got, O.rectangular_collapse_lengths(x, 100.0, 37.5, phi))
+
+
+def test_add_f0():
+    rng = np.random.default_rng(0)
+    q = np.linspace(-2.1, 2.1, 23)
+    iq = rng.uniform(size=(23, 23, 23))
+    assert np.allclose(V.add_f0_q_3d(iq, q, q, q, 'C'), O.add_f0_q_3d(iq, q, q, q), rtol=1e-14,
+                       atol=0)
+
+
+def _shm(shape):
+    shm = shared_memory.SharedMemory(create=True, size=int(np.prod(shape)) * 8)
+    a = np.ndarray(shape, dtype=np.float64, buffer=shm.buf)
+    a[...] = 0
+    return shm, a
+
+
+def test_process_file2():
+    rng = np.random.default_rng(1)
+    N = 64
+    q = np.linspace(-1.5, 1.5, 31)
+    iq = rng.uniform(0, 100, size=(N, N))
+    hx, hy, vz, _ = O.frame_axes(N, 0.5, 23.0)
+    s1, g = _shm((31,) * 3)
+    s2, c = _shm((31,) * 3)
+    try:
+        V.process_file2(iq, hx, hy, vz, q, q, q, s1.name, s2.name)
+        wg = np.zeros((31,) * 3)
+        wc = np.zeros((31,) * 3)
+        O.bin_frame(iq, hx, hy, vz, q, q, q, wg, wc)
+        assert np.array_equal(c, wc)
+        assert np.allclose(g, wg, rtol=1e-14, atol=0)
+    finally:
+        del g, c
+        for s in (s1, s2):
+            s.close()
+            s.unlink()
+
+
+@pytest.mark.parametrize("fill,smooth", [(False, 0), (True, 2)])
+def test_rotate_project_fft_coords(fill, smooth):
+    rng = np.random.default_rng(2)
+    coords = rng.uniform(0, 20, size=(600, 3))
+    f = np.where(rng.uniform(size=600) < 0.7, 6.003 + 0.002j, 8.01 + 0.006j)
+    r, N = 0.25, 256
+    q = engine.qgrid(2 * np.pi / (r * N) * (1 + 1e-9), 2.0).q_axis
+    xb, yb, zb = [np.ptp(coords[:, i]) for i in range(3)]
+    avg = O.avg_voxel_f_of(f, xb, yb, zb, r)
+    phi = 31.4
+    s1, g = _shm((len(q),) * 3)
+    s2, c = _shm((len(q),) * 3)
+    try:
+        V.rotate_project_fft_coords((coords, f, phi, N, r, avg, xb, yb, zb, fill, smooth, q, q, q,
+                                     s1.name, s2.name))
+        wg = np.zeros_like(g)
+        wc = np.zeros_like(c)
+        grid = O.frame_grid(coords, f, phi, N, r, avg, xb, yb, fill, smooth)
+        hx, hy, vz, _ = O.frame_axes(N, r, phi)
+        O.bin_frame(O.frame_iq(grid), hx, hy, vz, q, q, q, wg, wc)
+        assert np.array_equal(c, wc)
+        m = wg > 1e-6 * wg.max()
+        assert np.max(np.abs(g[m] - wg[m]) / wg[m]) < 1e-5
+    finally:
+        del g, c
+        for s in (s1, s2):
+            s.close()
+            s.unlink()
+
+
+def test_rotate_coords_z():
+    rng = np.random.default_rng(3)
+    c = rng.uniform(-40, 40, size=(5000, 3))
+    for phi in (0.0, 17.3, 90.0):
+        got = U.rotate_coords_z(c, phi)
+        assert np.array_equal(got[:, 1], O.rotate_coords_z_y(c, phi))
+        assert np.array_equal(got[:, 2], c[:, 2])
+
+
+def test_optimize_scale_offset():
+    rng = np.random.default_rng(4)
+    s = rng.uniform(0, 5, size=(64, 64))
+    y = 1.7 * s - 0.2 + rng.normal(0, 0.05, s.shape)
+    m = (rng.uniform(size=s.shape) < 0.3).astype(int)
+    a, b = C.optimize_scale_offset(s, y, m)
+    a2, b2 = O.optimize_scale_offset(s, y, m)
+    assert a == pytest.approx(a2, rel=1e-12) and b == pytest.approx(b2, rel=1e-10, abs=1e-12)
+
+
+def test_voxelgridmaker_fitting_api():
+    coords, elements = U.load_xyz(f'{GOLDEN}/graphite_medium.xyz')
+    r, qv, mq, E = 0.25, 2 * np.pi / (0.25 * 256) * (1 + 1e-9), 2.0, 12700.0
+    iq, qx, qy, qz = C.voxelgridmaker_fitting(coords, elements, r, qv, mq, E)
+    ids, ftab, _, _ = engine.element_table(elements, E)
+    ref = O.voxelgridmaker(coords, ftab[ids], r, qv, mq)
+    assert np.array_equal(qx, ref['qx']) and iq.shape == ref['iq'].shape
+    m = ref['iq'] > 1e-6 * ref['iq'].max()
+    assert np.max(np.abs(iq[m] - ref['iq'][m]) / ref['iq'][m]) <= 1e-5
+
+
+def test_simulate_giwaxs_config(tmp_path):
+    from giwaxsim_amd import simulate_GIWAXS as S
+    cfg = tmp_path / 'cfg.txt'
+    out = tmp_path / 'out'
+    cfg.write_text(
+        f"input_filepath={GOLDEN}/graphite_medium.xyz\nx_size=20\ny_size=20\nz_size=14\n"
+        "a=2.456\nb=2.456\nc=6.696\nalpha=90\nbeta=90\ngamma=120\n"
+        f"r_voxel_size=0.25\nq_voxel_size={2 * np.pi / (0.25 * 256) * (1 + 1e-9)!r}\nmax_q=2\n"
+        "energy=12700\nfill_bkg=True\nsmooth=2\nnum_pixels=64\n"
+        "angle_init_val1=0.15\nangle_init_ax1=psi\n"
+        "psi_start=0\npsi_end=180\npsi_num=19\nphi_start=0\nphi_end=0\nphi_num=1\n"
+        "theta_start=0\ntheta_end=0\ntheta_num=1\nmirror=True\n"
+        f"save_folder={out}\n")
+    config = U.parse_config_file(str(cfg))
+    res = S.main(config)
+    det = np.load(out / 'graphite_medium' / 'det_sum.npy', allow_pickle=False)
+    dh = np.load(out / 'graphite_medium' / 'det_h.npy', allow_pickle=False)
+    assert det.shape == (32, 32) and np.all(dh >= 0)
+    # the same pipeline through the oracle
+    coords, el = C.slabmaker_fitting(f'{GOLDEN}/graphite_medium.xyz', 20.0, 20.0, 14.0, 2.456, 2.456,
+                                     6.696, 90.0, 90.0, 120.0)
+    ids, ftab, _, _ = engine.element_table(el, 12700.0)
+    ref = O.voxelgridmaker(coords, ftab[ids], 0.25, float(config['q_voxel_size']), 2.0,
+                           fill_bkg=True, smooth=2)
+    psis = np.linspace(0, 180, 19)
+    one = np.array([0.0])
+    d, h, v = O.detectormaker(ref['iq'], ref['qx'], ref['qy'], ref['qz'], 64, 2.0, (0.15, 0, 0),
+                              ('psi', 'None', 'None'), psis, None, one, None, one, None, mirror=True)
+    d = d[np.ix_(np.where(v >= 0)[0], np.where(h >= 0)[0])]
+    m = d > 1e-6 * d.max()
+    assert np.max(np.abs(det[m] - d[m]) / d[m]) <= 1e-5
