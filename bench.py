@@ -34,10 +34,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-frames", type=int, default=48,
+    ap.add_argument("--cpu-frames", type=int, default=581,
                     help="phi-frames of the same slab timed on the host (CPU baseline sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=20)
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="one step, then one 256 MiB device copy (counter calibration) and one "
+                         "launch of each frame kernel; for rocprofv3 --pmc passes")
     return ap.parse_args()
 
 
@@ -69,6 +72,20 @@ def main():
         return engine.run_voxelgrid(coords_d, None, W['r'], W['qv'], W['max_q'], W['energy'],
                                     fill_bkg=W['fill'], smooth=W['smooth'], f_table=ftab,
                                     elem_ids=ids_d, want64=True)
+
+    if args.profile_kernels:
+        res = step()
+        torch.cuda.synchronize()
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)                      # 256 MiB read + 256 MiB write
+        runner = res['runner']
+        nb = runner.batch
+        for kname in ("gx_frames_stats", "gx_frames_columns", "gx_frames_rowpass", "gx_frames_colpass"):
+            call(kname, ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_kernels": True, "frames_per_launch": nb, "copy_bytes": 2 * 4 * src.numel()}))
+        return
 
     for _ in range(args.warmup):
         res = step()

@@ -32,10 +32,15 @@ __global__ void k_slab_bounds_init(unsigned long long *keys, unsigned long long 
   for (int e = t; e < n_elem; e += blockDim.x) counts[e] = 0;
 }
 
+constexpr int MAX_ELEM = 128;
+
 __global__ __launch_bounds__(BLK) void k_slab_bounds(const double *__restrict__ x, const double *__restrict__ y,
                                                      const double *__restrict__ z, const int32_t *__restrict__ el,
                                                      int64_t n, int64_t n_elem, unsigned long long *keys,
                                                      unsigned long long *counts) {
+  __shared__ unsigned int hist[MAX_ELEM];
+  for (int e = threadIdx.x; e < MAX_ELEM; e += BLK) hist[e] = 0;
+  __syncthreads();
   double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int64_t a = blockIdx.x * (int64_t)BLK + threadIdx.x; a < n; a += (int64_t)gridDim.x * BLK) {
     double v[3] = {x[a], y[a], z[a]};
@@ -45,8 +50,11 @@ __global__ __launch_bounds__(BLK) void k_slab_bounds(const double *__restrict__ 
       mx[k] = fmax(mx[k], v[k]);
     }
     int e = el[a];
-    if (e >= 0 && e < n_elem) atomicAdd(&counts[e], 1ull);
+    if (e >= 0 && e < n_elem) atomicAdd(&hist[e], 1u);
   }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n_elem; e += BLK)
+    if (hist[e]) atomicAdd(&counts[e], (unsigned long long)hist[e]);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     double a = mn[k], b = mx[k];
@@ -175,83 +183,81 @@ __global__ void k_frames_init(unsigned long long *ymin_key, int32_t *bounds, int
   }
 }
 
-constexpr int APT = 4;  // atoms per thread in the statistics kernels
-constexpr int FPB = 8;  // frames per block (grid.y)
+constexpr int STAT_CHUNK = 16384;  // atoms per block in the statistics kernels
 
-// min over all atoms of y' = x sin(phi) + y cos(phi) (canonical rotate_coords_z)
+__device__ __forceinline__ double block_min_d(double v, double *sh) {
+  v = wave_min(v);
+  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = sh[0];
+  for (int i = 1; i < BLK / GX_WAVE; ++i) r = fmin(r, sh[i]);
+  return r;
+}
+
+__device__ __forceinline__ void block_minmax4(int (&v)[4], int *sh) {
+  v[0] = wave_min_i(v[0]);
+  v[1] = wave_max_i(v[1]);
+  v[2] = wave_min_i(v[2]);
+  v[3] = wave_max_i(v[3]);
+  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
+  if (lane == 0)
+    for (int k = 0; k < 4; ++k) sh[4 * w + k] = v[k];
+  __syncthreads();
+  for (int i = 1; i < BLK / GX_WAVE; ++i) {
+    v[0] = min(v[0], sh[4 * i + 0]);
+    v[1] = max(v[1], sh[4 * i + 1]);
+    v[2] = min(v[2], sh[4 * i + 2]);
+    v[3] = max(v[3], sh[4 * i + 3]);
+  }
+}
+
+// min over all atoms of y' = x sin(phi) + y cos(phi) (canonical rotate_coords_z);
+// grid (atom chunks, frames), one atomic per block.
 __global__ __launch_bounds__(BLK) void k_frames_ymin(gx_frames d, int64_t f0, int64_t nf) {
-  const int64_t base = (int64_t)blockIdx.x * BLK * APT + threadIdx.x;
-  double xs[APT], ys[APT];
-  bool ok[APT];
-#pragma unroll
-  for (int k = 0; k < APT; ++k) {
-    int64_t a = base + (int64_t)k * BLK;
-    ok[k] = a < d.n_atoms;
-    xs[k] = ok[k] ? d.ax[a] : 0.0;
-    ys[k] = ok[k] ? d.ay[a] : 0.0;
-  }
-  const int fb0 = blockIdx.y * FPB;
-  for (int q = 0; q < FPB; ++q) {
-    int fb = fb0 + q;
-    if (fb >= nf) break;
-    const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
-    const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
-    double m = INFINITY;
-#pragma unroll
-    for (int k = 0; k < APT; ++k)
-      if (ok[k]) m = fmin(m, xs[k] * s + ys[k] * c);
-    m = wave_min(m);
-    if ((threadIdx.x & (GX_WAVE - 1)) == 0 && m < INFINITY) atomicMin(&d.ymin_key[fb], dkey(m));
-  }
+  __shared__ double sh[BLK / GX_WAVE];
+  const int fb = blockIdx.y;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const int64_t a0 = (int64_t)blockIdx.x * STAT_CHUNK;
+  const int64_t a1 = min((int64_t)d.n_atoms, a0 + STAT_CHUNK);
+  double m = INFINITY;
+  for (int64_t a = a0 + threadIdx.x; a < a1; a += BLK) m = fmin(m, d.ax[a] * s + d.ay[a] * c);
+  m = block_min_d(m, sh);
+  if (threadIdx.x == 0 && m < INFINITY) atomicMin(&d.ymin_key[fb], dkey(m));
 }
 
 // y/z index bounds over valid atoms (voxelgrids.py:328-335, 346-349)
 __global__ __launch_bounds__(BLK) void k_frames_bounds(gx_frames d, int64_t f0, int64_t nf) {
+  __shared__ int sh[4 * (BLK / GX_WAVE)];
   const int N = (int)d.N;
   const double rinv = 1.0 / d.r;
+  const int fb = blockIdx.y;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const double ymin = dkey_inv(d.ymin_key[fb]);
   const int64_t n_rowed = d.row_off[N];
-  const int64_t base = (int64_t)blockIdx.x * BLK * APT + threadIdx.x;
-  double xs[APT], ys[APT];
-  int zr[APT];
-#pragma unroll
-  for (int k = 0; k < APT; ++k) {
-    int64_t a = base + (int64_t)k * BLK;
-    bool ok = a < n_rowed;
-    xs[k] = ok ? d.ax[a] : 0.0;
-    ys[k] = ok ? d.ay[a] : 0.0;
-    zr[k] = ok ? d.azrow[a] : -1;
+  const int64_t a0 = (int64_t)blockIdx.x * STAT_CHUNK;
+  const int64_t a1 = min(n_rowed, a0 + STAT_CHUNK);
+  int v[4] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+  for (int64_t a = a0 + threadIdx.x; a < a1; a += BLK) {
+    const double yy = (d.ax[a] * s + d.ay[a] * c) - ymin;
+    const long long yi = to_int(floor_div(yy, d.r, rinv));
+    if (yi >= 0 && yi < N) {
+      const int zr = d.azrow[a];
+      v[0] = min(v[0], (int)yi);
+      v[1] = max(v[1], (int)yi);
+      v[2] = min(v[2], zr);
+      v[3] = max(v[3], zr);
+    }
   }
-  const int fb0 = blockIdx.y * FPB;
-  for (int q = 0; q < FPB; ++q) {
-    int fb = fb0 + q;
-    if (fb >= nf) break;
-    const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
-    const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
-    const double ymin = dkey_inv(d.ymin_key[fb]);
-    int ylo = INT_MAX, yhi = INT_MIN, zlo = INT_MAX, zhi = INT_MIN;
-#pragma unroll
-    for (int k = 0; k < APT; ++k) {
-      if (zr[k] < 0) continue;
-      double yy = (xs[k] * s + ys[k] * c) - ymin;
-      long long yi = to_int(floor_div(yy, d.r, rinv));
-      if (yi >= 0 && yi < N) {
-        ylo = min(ylo, (int)yi);
-        yhi = max(yhi, (int)yi);
-        zlo = min(zlo, zr[k]);
-        zhi = max(zhi, zr[k]);
-      }
-    }
-    ylo = wave_min_i(ylo);
-    yhi = wave_max_i(yhi);
-    zlo = wave_min_i(zlo);
-    zhi = wave_max_i(zhi);
-    if ((threadIdx.x & (GX_WAVE - 1)) == 0 && ylo != INT_MAX) {
-      int32_t *b = d.idx_bounds + 4 * fb;
-      atomicMin(&b[0], ylo);
-      atomicMax(&b[1], yhi);
-      atomicMin(&b[2], zlo);
-      atomicMax(&b[3], zhi);
-    }
+  block_minmax4(v, sh);
+  if (threadIdx.x == 0 && v[0] != INT_MAX) {
+    int32_t *b = d.idx_bounds + 4 * fb;
+    atomicMin(&b[0], v[0]);
+    atomicMax(&b[1], v[1]);
+    atomicMin(&b[2], v[2]);
+    atomicMax(&b[3], v[3]);
   }
 }
 
@@ -275,9 +281,13 @@ __device__ __forceinline__ void py_slice(long long a, long long b, long long n, 
 // in scipy's symmetric correlate order (centre first, then the outermost
 // pair inwards).
 __device__ double smooth_mask(const double *w, int radius, long long lo, long long hi, int i, int n) {
+  // indices i +- d with d <= radius; radius may exceed n, so wrap with a loop-free
+  // double modulo only when needed
   auto m = [&](long long k) -> double {
-    k %= n;
-    if (k < 0) k += n;
+    if (k < 0 || k >= n) {
+      k %= n;
+      if (k < 0) k += n;
+    }
     return (k >= lo && k < hi) ? 1.0 : 0.0;
   };
   double acc = m(i) * w[0];
@@ -626,8 +636,10 @@ int gx_slab_bounds(const gx_slab *s, double *out_minmax, int64_t *out_elem_count
   auto *keys = reinterpret_cast<unsigned long long *>(out_minmax);
   auto *cnts = reinterpret_cast<unsigned long long *>(out_elem_count);
   k_slab_bounds_init<<<1, 64, 0, st>>>(keys, cnts, s->n_elem);
-  unsigned g = grid1d(s->n_atoms, BLK);
-  if (g > 4096) g = 4096;
+  GX_REQUIRE(s->n_elem >= 1 && s->n_elem <= MAX_ELEM, "n_elem=%lld outside [1, %d]", (long long)s->n_elem,
+             MAX_ELEM);
+  unsigned g = grid1d(s->n_atoms, BLK * 16);
+  if (g > 2048) g = 2048;
   k_slab_bounds<<<g, BLK, 0, st>>>(s->x, s->y, s->z, s->elem, s->n_atoms, s->n_elem, keys, cnts);
   k_slab_bounds_fin<<<1, 64, 0, st>>>(keys);
   return check_launch("gx_slab_bounds");
@@ -663,7 +675,7 @@ int gx_frames_stats(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stre
   hipStream_t st = as_stream(stream);
   k_frames_init<<<grid1d(nf, BLK), BLK, 0, st>>>(d->ymin_key, d->idx_bounds, nf);
   if (d->n_atoms > 0) {
-    dim3 g(grid1d(d->n_atoms, BLK * APT), grid1d(nf, FPB));
+    dim3 g(grid1d(d->n_atoms, STAT_CHUNK), (unsigned)nf);
     k_frames_ymin<<<g, BLK, 0, st>>>(*d, f0, nf);
     k_frames_bounds<<<g, BLK, 0, st>>>(*d, f0, nf);
   }
