@@ -51,10 +51,11 @@ class GxFrames(ctypes.Structure):
         ("qmin", c_double), ("dq", c_double), ("c_lo", c_int64), ("c_n", c_int64),
         ("q_val", c_double), ("i_lo", c_int64), ("i_hi", c_int64), ("W", c_int64),
         ("bin_scale", c_double), ("fft_f64", c_int64), ("twiddle", c_void_p),
+        ("twiddle64", c_void_p),
         ("n_frames", c_int64),
         ("fparams", c_void_p), ("batch", c_int64),
         ("ymin_key", c_void_p), ("idx_bounds", c_void_p), ("col_nm", c_void_p),
-        ("col_sy", c_void_p), ("col_sz", c_void_p), ("inter", c_void_p),
+        ("col_sy", c_void_p), ("col_sz", c_void_p), ("inter", c_void_p), ("col_spec", c_void_p),
         ("bin_sum", c_void_p), ("bin_cnt", c_void_p), ("status", c_void_p),
     ]
 

@@ -233,7 +233,6 @@ template <> struct FftPlan<256>  { static constexpr int R0 = 4,  R1 = 4,  R2 = 4
 template <> struct FftPlan<512>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 8, R3 = 1; static constexpr int NST = 3; };
 template <> struct FftPlan<1024> { static constexpr int R0 = 16, R1 = 16, R2 = 4, R3 = 1; static constexpr int NST = 3; };
 template <> struct FftPlan<2048> { static constexpr int R0 = 16, R1 = 16, R2 = 8, R3 = 1; static constexpr int NST = 3; };
-template <> struct FftPlan<4096> { static constexpr int R0 = 16, R1 = 16, R2 = 16, R3 = 1; static constexpr int NST = 3; };
 
 template <int N> struct FftFirst { static constexpr int R = FftPlan<N>::R0; };
 template <int N> struct FftLast {

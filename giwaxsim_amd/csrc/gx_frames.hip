@@ -320,8 +320,79 @@ __global__ __launch_bounds__(BLK) void k_frames_columns(gx_frames d, int64_t f0,
 }
 
 // ------------------------------------------------------------------------------------
+// per-frame background spectrum
+//
+// With my[y] = smooth ? s_y[y] : 1, mz = smooth ? s_z[z] : 1, A = avg_voxel_f *
+// max_voxels and b[y] = (y_min < y < y_max ? num_missing[y] * avg_voxel_f : A),
+// the grid of a row inside the filled region is
+//   G[z, y] = (S[z, y] + b[y]) my[y] mz + A (1 - mz my[y])        (smooth)
+//   G[z, y] =  S[z, y] + b[y]                                     (no smooth)
+// (without fill b = 0, with S = atoms), so its transform along y is
+//   F[z, k] = mz (FFT(S my)[k] + U[k]) + A N delta_k0 [smooth],
+//   U = FFT((fill ? b : 0) my - (smooth ? A my : 0)),
+// and rows outside the filled z range are the constant A.  U is one FP64 FFT
+// per frame; only rows that hold atoms need a transform of their own.
+// ------------------------------------------------------------------------------------
+template <int N, typename C>
+__global__ __launch_bounds__(GX_WAVE) void k_frames_spectra(gx_frames d, int64_t f0) {
+  using T = typename Real<C>::T;
+  constexpr int P = N / GX_WAVE;
+  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  __shared__ double2 lds[N];
+  const int lane = threadIdx.x;
+  const int fb = blockIdx.x;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = jhi - jlo + 1;
+  if (wf <= 0 || wf > d.W) return;
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1];
+  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
+  const int32_t *nm_row = d.col_nm + (int64_t)fb * N;
+  const double *sy_row = d.col_sy + (int64_t)fb * N;
+  double2 v[P];
+#pragma unroll
+  for (int t = 0; t < P / R0; ++t) {
+#pragma unroll
+    for (int q = 0; q < R0; ++q) {
+      const int y = lane + GX_WAVE * t + q * (N / R0);
+      const double my = d.smooth ? sy_row[y] : 1.0;
+      double ure = 0.0, uim = 0.0;
+      if (d.fill_bkg) {
+        double bre = avv_re, bim = avv_im;
+        if (y > ylo && y < yhi) {
+          const double nm = (double)nm_row[y];
+          bre = nm * d.avg_re;
+          bim = nm * d.avg_im;
+        }
+        ure = bre * my;
+        uim = bim * my;
+      }
+      if (d.smooth) {
+        ure = ure - avv_re * my;
+        uim = uim - avv_im * my;
+      }
+      v[t * R0 + q] = make_double2(ure, uim);
+    }
+  }
+  fft_wave<N, double2>(v, lds, reinterpret_cast<const double2 *>(d.twiddle64), lane);
+  C *spec = reinterpret_cast<C *>(d.col_spec) + (int64_t)fb * d.W;
+#pragma unroll
+  for (int t = 0; t < P / RL; ++t) {
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int k = lane + GX_WAVE * t + q * (N / RL);
+      const int w = ((k + N / 2) & (N - 1)) - jlo;
+      if (w >= 0 && w < wf) spec[w] = C{(T)v[t * RL + q].x, (T)v[t * RL + q].y};
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // row pass
 // ------------------------------------------------------------------------------------
+constexpr int SCATTER_UNROLL = 4;
+
 template <int N, int RPW, typename C>
 __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
   using T = typename Real<C>::T;
@@ -334,6 +405,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
   const int wave = threadIdx.x / GX_WAVE;
   C *wbuf = smem + wave * N;
   C *stage = smem + (BLK / GX_WAVE) * N;  // [W][RB]
+  C *tspec = stage + d.W * RB;            // [W] background spectrum of this frame
   int32_t *acc_re = reinterpret_cast<int32_t *>(wbuf);
   int32_t *acc_im = acc_re + N;
 
@@ -347,92 +419,114 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
     if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
     return;
   }
+  const bool dense = d.fill_bkg || d.smooth;
+  if (dense) {
+    const C *spec = reinterpret_cast<const C *>(d.col_spec) + (int64_t)fb * d.W;
+    for (int w = threadIdx.x; w < wf; w += BLK) tspec[w] = spec[w];
+  }
+  __syncthreads();
 
   const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
   const double ymin = dkey_inv(d.ymin_key[fb]);
   const int32_t *bnd = d.idx_bounds + 4 * fb;
   const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  // atoms that land in y columns overwritten by the fill never contribute
+  const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;   // keep ya <= y < yb
+  const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
   const double r = d.r, rinv = 1.0 / d.r;
   const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
-  const int32_t *nm_row = d.col_nm + (int64_t)fb * N;
-  const double *sy_row = d.col_sy + (int64_t)fb * N;
-  const double *sz_row = d.col_sz + (int64_t)fb * N;
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
   const int z0 = blockIdx.x * RB;
+
+  // y-mask of this lane's first-stage inputs (same for every row of the frame)
+  T my[P];
+#pragma unroll
+  for (int t = 0; t < P / R0; ++t)
+#pragma unroll
+    for (int q = 0; q < R0; ++q)
+      my[t * R0 + q] = d.smooth ? (T)d.col_sy[(int64_t)fb * N + lane + GX_WAVE * t + q * (N / R0)] : (T)1;
 
   for (int rr = 0; rr < RPW; ++rr) {
     const int rb = wave * RPW + rr;
     const int z = z0 + rb;
+    const bool inside = !d.fill_bkg || (z >= zlo && z < zhi);
     const int a0 = d.row_off[z], a1 = d.row_off[z + 1];
-    const bool outside_z = d.fill_bkg && (z < zlo || z >= zhi);
+    const bool atoms = inside && a1 > a0;
     C v[P];
-    if (!d.fill_bkg && !d.smooth && a0 == a1) {
-      // empty row: transform is zero
-#pragma unroll
-      for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
-    } else if (outside_z && !d.smooth) {
-      // constant row avg_voxel_f * max_voxels: only the k = 0 term survives
-#pragma unroll
-      for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
-      if (lane == 0) v[0] = C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)};
-    } else {
+    if (atoms) {
       // 1) zero the fixed-point accumulators
       for (int y = lane; y < N; y += GX_WAVE) {
         acc_re[y] = 0;
         acc_im[y] = 0;
       }
       wave_sync();
-      // 2) scatter this row's atoms (np.add.at(grid, (z, y), f), voxelgrids.py:341)
-      for (int a = a0 + lane; a < a1; a += GX_WAVE) {
-        double yy = (d.ax[a] * s + d.ay[a] * c) - ymin;
-        long long yi = to_int(floor_div(yy, r, rinv));
-        if (yi >= 0 && yi < N) {
-          atomicAdd(&acc_re[yi], d.aq[2 * (int64_t)a]);
-          atomicAdd(&acc_im[yi], d.aq[2 * (int64_t)a + 1]);
+      // 2) scatter this row's atoms (np.add.at(grid, (z, y), f), voxelgrids.py:341);
+      //    loads for SCATTER_UNROLL atoms are issued before any of them is used
+      for (int a = a0 + lane; a < a1; a += GX_WAVE * SCATTER_UNROLL) {
+        double xs[SCATTER_UNROLL], ys[SCATTER_UNROLL];
+        int2 qs[SCATTER_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SCATTER_UNROLL; ++u) {
+          const int au = a + u * GX_WAVE;
+          const bool ok = au < a1;
+          xs[u] = ok ? d.ax[au] : 0.0;
+          ys[u] = ok ? d.ay[au] : 0.0;
+          qs[u] = ok ? reinterpret_cast<const int2 *>(d.aq)[au] : make_int2(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < SCATTER_UNROLL; ++u) {
+          if (a + u * GX_WAVE >= a1) break;
+          const double yy = (xs[u] * s + ys[u] * c) - ymin;
+          const long long yi = to_int(floor_div(yy, r, rinv));
+          if (yi >= ya && yi < yb) {
+            atomicAdd(&acc_re[yi], qs[u].x);
+            atomicAdd(&acc_im[yi], qs[u].y);
+          }
         }
       }
       wave_sync();
-      // 3) grid values of the first-stage inputs in FP64, rounded once to T
-      const double inv = ldexp(1.0, -d.row_exp[z]);
-      const double mz = d.smooth ? sz_row[z] : 0.0;
+      // 3) first-stage inputs S[y] my[y], exact fixed point -> T
+      const T inv = (T)ldexp(1.0, -d.row_exp[z]);
 #pragma unroll
       for (int t = 0; t < P / R0; ++t) {
 #pragma unroll
         for (int q = 0; q < R0; ++q) {
           const int y = lane + GX_WAVE * t + q * (N / R0);
-          double gre = (double)acc_re[y] * inv, gim = (double)acc_im[y] * inv;
-          if (d.fill_bkg) {
-            if (outside_z || y <= ylo || y >= yhi) {
-              gre = avv_re;
-              gim = avv_im;
-            } else {
-              double nm = (double)nm_row[y];
-              gre = gre + nm * d.avg_re;
-              gim = gim + nm * d.avg_im;
-            }
-          }
-          if (d.smooth) {
-            double m = mz * sy_row[y];
-            double om = 1.0 - m;
-            gre = gre * m + avv_re * om;
-            gim = gim * m + avv_im * om;
-          }
-          v[t * R0 + q] = C{(T)gre, (T)gim};
+          const T m = my[t * R0 + q];
+          v[t * R0 + q] = C{((T)acc_re[y] * inv) * m, ((T)acc_im[y] * inv) * m};
         }
       }
       wave_sync();
       // 4) FFT along y
       fft_wave<N, C>(v, wbuf, tw, lane);
+    } else {
+#pragma unroll
+      for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
     }
-    // 5) keep the fftshifted columns [jlo, jhi] in the staging tile
+    // 5) kept columns [jlo, jhi]: add the background spectrum, stage
+    const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + z] : 1.0;
 #pragma unroll
     for (int t = 0; t < P / RL; ++t) {
 #pragma unroll
       for (int q = 0; q < RL; ++q) {
         const int k = lane + GX_WAVE * t + q * (N / RL);
-        const int col = (k + N / 2) & (N - 1);
-        const int w = col - jlo;
-        if (w >= 0 && w < wf) stage[w * RB + rb] = v[t * RL + q];
+        const int w = ((k + N / 2) & (N - 1)) - jlo;
+        if (w < 0 || w >= wf) continue;
+        C o = v[t * RL + q];
+        if (!inside) {
+          // filled row: the constant avg_voxel_f * max_voxels
+          o = (k == 0) ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
+        } else if (dense) {
+          const C u = tspec[w];
+          double ore = mz * ((double)o.x + (double)u.x);
+          double oim = mz * ((double)o.y + (double)u.y);
+          if (d.smooth && k == 0) {
+            ore = ore + avv_re * (double)N;
+            oim = oim + avv_im * (double)N;
+          }
+          o = C{(T)ore, (T)oim};
+        }
+        stage[w * RB + rb] = o;
       }
     }
     wave_sync();
@@ -556,7 +650,7 @@ template <int N, typename C>
 int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int RPW = (N <= 1024) ? 2 : 1;
   constexpr int RB = (BLK / GX_WAVE) * RPW;
-  size_t lds = ((size_t)(BLK / GX_WAVE) * N + (size_t)d->W * RB) * sizeof(C);
+  size_t lds = ((size_t)(BLK / GX_WAVE) * N + (size_t)d->W * (RB + 1)) * sizeof(C);
   if (lds > 160 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "row pass needs %zu bytes of LDS (N=%d, W=%lld, %s)", lds, N,
                      (long long)d->W, sizeof(C) == 16 ? "FP64" : "FP32");
@@ -573,31 +667,33 @@ int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st)
 
 template <int N>
 int launch_rowpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
-  if (d->fft_f64) {
-    if constexpr (N <= 2048) return launch_rowpass_t<N, double2>(d, f0, nf, st);
-    return set_error(GX_ERR_UNSUPPORTED, "FP64 transforms support N <= 2048");
-  }
+  if (d->fft_f64) return launch_rowpass_t<N, double2>(d, f0, nf, st);
   return launch_rowpass_t<N, float2>(d, f0, nf, st);
 }
 
 template <int N>
 int launch_colpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   dim3 grid(grid1d(d->W, BLK / GX_WAVE), (unsigned)nf);
-  if (d->fft_f64) {
-    if constexpr (N <= 2048) {
-      k_colpass<N, double2><<<grid, BLK, 0, st>>>(*d, f0);
-      return check_launch("k_colpass");
-    }
-    return set_error(GX_ERR_UNSUPPORTED, "FP64 transforms support N <= 2048");
-  }
-  k_colpass<N, float2><<<grid, BLK, 0, st>>>(*d, f0);
+  if (d->fft_f64)
+    k_colpass<N, double2><<<grid, BLK, 0, st>>>(*d, f0);
+  else
+    k_colpass<N, float2><<<grid, BLK, 0, st>>>(*d, f0);
   return check_launch("k_colpass");
+}
+
+template <int N>
+int launch_spectra(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  if (d->fft_f64)
+    k_frames_spectra<N, double2><<<(unsigned)nf, GX_WAVE, 0, st>>>(*d, f0);
+  else
+    k_frames_spectra<N, float2><<<(unsigned)nf, GX_WAVE, 0, st>>>(*d, f0);
+  return check_launch("k_frames_spectra");
 }
 
 int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
   GX_REQUIRE(d != nullptr, "gx_frames descriptor is NULL");
   GX_REQUIRE(gx_supported_grid(d->N), "grid edge N=%lld has no compiled FFT plan (supported: 256, 512, "
-             "1024, 2048, 4096)", (long long)d->N);
+             "1024, 2048)", (long long)d->N);
   GX_REQUIRE(d->r > 0.0, "r_voxel_size must be positive");
   GX_REQUIRE(nf >= 0 && nf <= d->batch, "batch of %lld frames exceeds workspace batch %lld", (long long)nf,
              (long long)d->batch);
@@ -610,6 +706,8 @@ int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
              "a required device buffer of gx_frames is NULL");
   GX_REQUIRE(!d->fill_bkg || d->col_nm, "fill_bkg needs col_nm");
   GX_REQUIRE(!d->smooth || (d->col_sy && d->col_sz && d->gauss_w), "smooth needs col_sy, col_sz, gauss_w");
+  GX_REQUIRE(!(d->fill_bkg || d->smooth) || (d->col_spec && d->twiddle64),
+             "fill_bkg / smooth need col_spec and twiddle64");
   GX_REQUIRE(d->n_atoms >= 0 && (d->n_atoms == 0 || (d->ax && d->ay && d->aq && d->azrow)), "atom arrays");
   return GX_OK;
 }
@@ -624,7 +722,7 @@ extern "C" {
 int gx_abi_version(void) { return GX_ABI_VERSION; }
 
 int gx_supported_grid(int64_t N) {
-  return N == 256 || N == 512 || N == 1024 || N == 2048 || N == 4096;
+  return N == 256 || N == 512 || N == 1024 || N == 2048;
 }
 
 int gx_slab_bounds(const gx_slab *s, double *out_minmax, int64_t *out_elem_count, gx_stream_t stream) {
@@ -687,9 +785,18 @@ int gx_frames_columns(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t st
   int rc = check_frames(d, f0, nf);
   if (rc) return rc;
   if (nf == 0) return GX_OK;
+  hipStream_t st = as_stream(stream);
   dim3 g(grid1d(d->N, BLK), (unsigned)nf);
-  k_frames_columns<<<g, BLK, 0, as_stream(stream)>>>(*d, f0, nf);
-  return check_launch("gx_frames_columns");
+  k_frames_columns<<<g, BLK, 0, st>>>(*d, f0, nf);
+  rc = check_launch("gx_frames_columns");
+  if (rc || !(d->fill_bkg || d->smooth)) return rc;
+  switch (d->N) {
+    case 256: return launch_spectra<256>(d, f0, nf, st);
+    case 512: return launch_spectra<512>(d, f0, nf, st);
+    case 1024: return launch_spectra<1024>(d, f0, nf, st);
+    case 2048: return launch_spectra<2048>(d, f0, nf, st);
+  }
+  return set_error(GX_ERR_UNSUPPORTED, "N=%lld", (long long)d->N);
 }
 
 int gx_frames_rowpass(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream) {
@@ -703,7 +810,6 @@ int gx_frames_rowpass(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t st
     case 512: return launch_rowpass<512>(d, f0, nf, st);
     case 1024: return launch_rowpass<1024>(d, f0, nf, st);
     case 2048: return launch_rowpass<2048>(d, f0, nf, st);
-    case 4096: return launch_rowpass<4096>(d, f0, nf, st);
   }
   return set_error(GX_ERR_UNSUPPORTED, "N=%lld", (long long)d->N);
 }
@@ -719,7 +825,6 @@ int gx_frames_colpass(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t st
     case 512: return launch_colpass<512>(d, f0, nf, st);
     case 1024: return launch_colpass<1024>(d, f0, nf, st);
     case 2048: return launch_colpass<2048>(d, f0, nf, st);
-    case 4096: return launch_colpass<4096>(d, f0, nf, st);
   }
   return set_error(GX_ERR_UNSUPPORTED, "N=%lld", (long long)d->N);
 }

@@ -454,13 +454,16 @@ class FrameRunner:
         self.batch = int(batch)
         self.fparams = torch.as_tensor(plan.fparams).to(dev).contiguous()
         self.tw = torch.as_tensor(twiddles(N, self.f64)).to(dev).contiguous()
+        self.tw64 = torch.as_tensor(twiddles(N, True)).to(dev).contiguous()
         B = self.batch
         self.ymin_key = _empty(B, torch.int64)
         self.idx_bounds = _empty(4 * B, torch.int32)
         self.col_nm = _empty(B * N, torch.int32) if fill_bkg else None
         self.col_sy = _empty(B * N, torch.float64) if smooth else None
         self.col_sz = _empty(B * N, torch.float64) if smooth else None
-        self.inter = _empty(B * W * N * 2, torch.float64 if self.f64 else torch.float32)
+        cdt = torch.float64 if self.f64 else torch.float32
+        self.inter = _empty(B * W * N * 2, cdt)
+        self.col_spec = _empty(B * W * 2, cdt) if (fill_bkg or smooth) else None
         self.status = torch.zeros(4, dtype=torch.int32, device=dev)
         if smooth:
             w, rad = gauss_half_kernel(smooth)
@@ -477,11 +480,12 @@ class FrameRunner:
             avg_re=float(np.real(avg)), avg_im=float(np.imag(avg)), gauss_w=ptr(self.gauss).value,
             smooth_radius=rad, qmin=g.qmin, dq=g.dq, c_lo=g.c_lo, c_n=g.c_n, q_val=plan.q_val,
             i_lo=plan.i_lo, i_hi=plan.i_hi, W=W, bin_scale=bins.scale, fft_f64=int(self.f64),
-            twiddle=ptr(self.tw).value,
+            twiddle=ptr(self.tw).value, twiddle64=ptr(self.tw64).value,
             n_frames=F, fparams=ptr(self.fparams).value, batch=self.batch,
             ymin_key=ptr(self.ymin_key).value, idx_bounds=ptr(self.idx_bounds).value,
             col_nm=ptr(self.col_nm).value, col_sy=ptr(self.col_sy).value,
             col_sz=ptr(self.col_sz).value, inter=ptr(self.inter).value,
+            col_spec=ptr(self.col_spec).value,
             bin_sum=ptr(bins.sum).value, bin_cnt=ptr(bins.cnt).value, status=ptr(self.status).value)
 
     def run(self, frames=None):
@@ -524,7 +528,7 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
     if not _lib.load().gx_supported_grid(plan.N):
         raise NotImplementedError(
             f"grid_size N={plan.N} (from 2*pi/(q_voxel_size*r_voxel_size)) has no compiled FFT; "
-            "supported: 256, 512, 1024, 2048, 4096")
+            "supported: 256, 512, 1024, 2048")
     sum_f = np.sum(counts * f_table)
     volume = xb * yb * zb
     avg = (sum_f / volume) * r_voxel_size ** 3

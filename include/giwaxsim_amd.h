@@ -96,7 +96,7 @@ typedef struct gx_slab {
  * The phi-frame pipeline of one prepared slab into a cropped 3D bin grid.
  * ---------------------------------------------------------------------------------- */
 typedef struct gx_frames {
-  int64_t N;               /* real-space grid edge: 256, 512, 1024, 2048 or 4096 (FP64: <= 2048) */
+  int64_t N;               /* real-space grid edge: 256, 512, 1024 or 2048 */
   double r;                /* r_voxel_size */
   int64_t n_atoms;         /* atoms (ymin runs over all of them) */
   const double *ax;        /* regrouped atoms (gx_slab.sx) */
@@ -123,6 +123,7 @@ typedef struct gx_frames {
   double bin_scale;        /* fixed-point scale of I in bin sums (a power of two) */
   int64_t fft_f64;         /* 0: FP32 transforms (float2), 1: FP64 transforms (double2) */
   const void *twiddle;     /* [N] exp(-2 pi i m / N) as float2 or double2 (fft_f64) */
+  const double *twiddle64; /* [N] the same as double2 (per-frame background spectra) */
   int64_t n_frames;        /* frames described by fparams */
   const double *fparams;   /* [n_frames][GX_FP_STRIDE] */
   /* per-frame scratch for a batch of up to `batch` frames */
@@ -133,6 +134,8 @@ typedef struct gx_frames {
   double *col_sy;          /* [batch][N] smoothed y mask (smooth) */
   double *col_sz;          /* [batch][N] smoothed z mask (smooth) */
   void *inter;             /* [batch][W][N] float2 or double2: row-transformed, column-pruned frame */
+  void *col_spec;          /* [batch][W] float2 or double2: spectrum of the frame's background /
+                              smoothing terms on the kept columns (fill_bkg or smooth) */
   /* output: cropped bins [c][c][c] indexed [qy][qx][qz] */
   unsigned long long *bin_sum; /* fixed-point sum of I (scale bin_scale) */
   int32_t *bin_cnt;
@@ -178,11 +181,14 @@ int gx_slab_sort_rows(const gx_slab *s, gx_stream_t stream);
  * z_max over valid atoms (voxelgrids.py:328-349). */
 int gx_frames_stats(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream);
 /* Per-column tables: num_missing_voxels (voxelgrids.py:350-354, 238-309) and
- * the smoothed edge masks (voxelgrids.py:363-374). */
+ * the smoothed edge masks (voxelgrids.py:363-374); then, for fill_bkg or
+ * smooth, the FP64 spectrum along y of the frame's background and blend
+ * terms on the kept columns (d->col_spec). */
 int gx_frames_columns(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream);
 /* Atom scatter (voxelgrids.py:337-341) + fill (355-361) + blend (377-379) +
  * FFT along y of every z row; writes only the column band that reaches the
- * cropped bins (d->inter). */
+ * cropped bins (d->inter).  By linearity only rows holding atoms are
+ * transformed; the fill/blend terms enter through d->col_spec. */
 int gx_frames_rowpass(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream);
 /* FFT along z of every kept column + |F|^2 (voxelgrids.py:387-392) + binning
  * into the cropped grid with counts (process_file2, voxelgrids.py:464-503). */
