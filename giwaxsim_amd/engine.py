@@ -662,3 +662,136 @@ def run_detector(iq_dev, qx, qy, qz, plan: DetectorPlan, mirror):
     flags = _lib.GX_FIN_FLOOR_SCALE | (_lib.GX_FIN_MIRROR if mirror else 0)
     call("gx_detector_finalize", ptr(det), n, n, flags, ptr(out), st)
     return out.view(n, n), det.view(n, n)
+
+
+# --------------------------------------------------------------------------------------
+# ensembles: many slabs, uneven atom counts and grid sizes, one shared I(q) grid
+# --------------------------------------------------------------------------------------
+class Ensemble:
+    """Slabs of one q grid (same q_voxel_size and max_q, hence the same q axis and
+    phi-frames) binned into one shared cropped grid: pooled sums and counts over
+    every frame of every slab (the input_folder ensemble of simulate_GIWAXS.py:84-118,
+    averaged by pooling rather than per-file means).
+
+    ``slabs`` is a list of (coords (n,3), elements (n,), r_voxel_size).  All
+    geometry planning and the atom grouping happen at construction; ``run()``
+    enqueues the device work of one full pass (bins cleared, every frame of
+    every slab, finalize) and can be captured once in a HIP graph
+    (``capture()``) and replayed (``replay()``).
+    """
+
+    def __init__(self, slabs, q_voxel_size, max_q, energy, fill_bkg=False, smooth=0,
+                 precision='auto', batch=None):
+        self.grid = qgrid(float(q_voxel_size), float(max_q))
+        self.fill_bkg, self.smooth = fill_bkg, int(smooth)
+        all_el = np.concatenate([np.asarray(e) for _, e, _ in slabs])
+        _, ftab, _, uniq = element_table(all_el, energy)
+        index = {str(u): i for i, u in enumerate(uniq)}
+        self.f_table = ftab
+        prepared, terms = [], []
+        for coords, elements, r in slabs:
+            coords = np.asarray(coords, dtype=np.float64)
+            ids = np.array([index[str(e)] for e in elements], dtype=np.int32)
+            counts = np.bincount(ids, minlength=len(ftab))
+            xb, yb, zb = (np.max(coords[:, i]) - np.min(coords[:, i]) for i in range(3))
+            plan = frame_plan(float(r), float(q_voxel_size), float(max_q), float(xb), float(yb),
+                              float(zb))
+            if not _lib.load().gx_supported_grid(plan.N):
+                raise NotImplementedError(f"grid_size N={plan.N} has no compiled FFT")
+            avg = (np.sum(counts * ftab) / (xb * yb * zb)) * r ** 3
+            B = intensity_bound(float(np.sum(counts * np.abs(ftab))), plan.N, fill_bkg, smooth, avg,
+                                plan.max_voxels, yb, xb, r)
+            terms.append((plan.n_frames, pixels_per_bin(plan), B))
+            prepared.append((coords, ids, plan, avg, np.min(coords[:, 2])))
+        self.bins = Bins(self.grid, bin_scale_for(terms))
+        self.runners = []
+        for coords, ids, plan, avg, zmin in prepared:
+            slab = Slab(coords, ids, ftab, plan.N, plan.r, zmin)
+            f64 = fft_precision(fill_bkg, smooth, precision)
+            self.runners.append(FrameRunner(plan, slab, self.bins, fill_bkg, smooth, avg,
+                                            batch=batch if batch is not None else
+                                            tuned_batch(plan.N, plan.W, plan.n_frames, f64),
+                                            precision=precision))
+        self.n_frames = sum(rn.plan.n_frames for rn in self.runners)
+        self.graph = None
+        self.iq64 = self.iq32 = None
+
+    def run(self, regroup=True):
+        """Enqueue one full pass; returns (iq64, iq32) device tensors."""
+        self.bins.clear()
+        for rn in self.runners:
+            if regroup:
+                call("gx_slab_sort_rows", ctypes.byref(rn.slab.desc), stream_handle())
+            rn.run()
+        self.iq64, self.iq32 = self.bins.finalize()
+        return self.iq64, self.iq32
+
+    def capture(self):
+        """Capture run() in a HIP graph (after one eager warm-up pass)."""
+        self.run()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        self.graph = g
+        return g
+
+    def replay(self):
+        self.graph.replay()
+        return self.iq64, self.iq32
+
+    def check(self):
+        for rn in self.runners:
+            rn.check()
+
+
+# --------------------------------------------------------------------------------------
+# autotuner: frames per launch, per (grid edge, band width, device)
+# --------------------------------------------------------------------------------------
+_TUNED = {}
+
+
+def tuned_batch(N, W, n_frames, f64=False):
+    """Frames per launch for a grid edge: the best of a few candidates timed once on
+    a synthetic slab, cached per (N, W bucket, device name).  Falls back to the
+    memory rule (intermediate <= ~160 MB) when tuning is disabled
+    (GIWAXSIM_AUTOTUNE=0)."""
+    import os
+    esz = 16 if f64 else 8
+    cap = max(1, int(160e6 // (W * N * esz)))
+    if os.environ.get("GIWAXSIM_AUTOTUNE", "1") == "0":
+        return min(n_frames, cap)
+    key = (N, (W + 63) // 64, bool(f64), torch.cuda.get_device_name())
+    if key not in _TUNED:
+        _TUNED[key] = _tune(N, cap, f64)
+    return max(1, min(n_frames, _TUNED[key]))
+
+
+def _tune(N, cap, f64):
+    from .synth import polymer_slab
+    r = 0.2 * 1024 / N
+    qv = 2 * np.pi / (r * N) * (1 + 1e-9)
+    coords, elements = polymer_slab(size=(40.0, 40.0, 40.0), seed=123)
+    cands = sorted({c for c in (16, 32, 64, 128, 256) if c <= cap} | {min(cap, 256)})
+    ids, ftab, counts, _ = element_table(elements, 12700.0)
+    xb, yb, zb = (np.max(coords[:, i]) - np.min(coords[:, i]) for i in range(3))
+    plan = frame_plan(r, qv, 2.0, float(xb), float(yb), float(zb))
+    avg = (np.sum(counts * ftab) / (xb * yb * zb)) * r ** 3
+    bins = Bins(plan.grid, 1.0)
+    slab = Slab(coords, ids, ftab, plan.N, r, np.min(coords[:, 2]))
+    best, best_t = cands[0], float("inf")
+    for b in cands:
+        rn = FrameRunner(plan, slab, bins, True, 10, avg, batch=b,
+                         precision='fp64' if f64 else 'fp32')
+        nfr = min(plan.n_frames, 4 * b)
+        rn.run((0, nfr))
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rn.run((0, nfr))
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / nfr
+        if t < best_t:
+            best, best_t = b, t
+    return best

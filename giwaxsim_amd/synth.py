@@ -134,3 +134,20 @@ def polymer_slab(size=(100.0, 100.0, 100.0), seed=0, density=0.11, crystal_fract
     ee = np.concatenate([ee, am_el])
     cc = cc - cc.min(axis=0)
     return cc.astype(np.float64), POLYMER_ELEMENTS[ee]
+
+
+def ensemble_slabs(n_slabs=64, seed=0, size_range=(40.0, 160.0), grids=(256, 512, 1024, 2048),
+                   q_voxel=2 * np.pi / (0.2 * 1024) * (1 + 1e-9)):
+    """Slabs of the polymer generator with seeded random edge lengths per axis
+    and a per-slab grid edge N drawn from ``grids``; r_voxel_size = 2 pi /
+    (q_voxel N) so every slab shares one q grid.  Returns [(coords, elements, r)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_slabs):
+        size = tuple(rng.uniform(size_range[0], size_range[1], 3))
+        N = int(rng.choice(grids))
+        # slightly above 2 pi / (q_voxel N) so ceil(2 pi / (q_voxel r)) is N, not N + 1
+        r = 2 * np.pi / (q_voxel * N) * (1 + 1e-9)
+        c, e = polymer_slab(size=size, seed=seed * 1000 + i)
+        out.append((c, e, r))
+    return out
