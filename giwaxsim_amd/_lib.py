@@ -46,10 +46,11 @@ class GxFrames(ctypes.Structure):
         ("ax", c_void_p), ("ay", c_void_p), ("aq", c_void_p), ("azrow", c_void_p),
         ("row_off", c_void_p), ("row_exp", c_void_p),
         ("fill_bkg", c_int64), ("smooth", c_int64), ("max_voxels", c_double),
-        ("avg_re", c_double), ("avg_im", c_double), ("gauss_w", c_void_p),
-        ("smooth_radius", c_int64),
+        ("avg_re", c_double), ("avg_im", c_double), ("gauss_cw", c_void_p),
+        ("smooth_radius", c_int64), ("zrow_max", c_int64),
         ("qmin", c_double), ("dq", c_double), ("c_lo", c_int64), ("c_n", c_int64),
         ("q_val", c_double), ("i_lo", c_int64), ("i_hi", c_int64), ("W", c_int64),
+        ("Z", c_int64),
         ("bin_scale", c_double), ("fft_f64", c_int64), ("twiddle", c_void_p),
         ("twiddle64", c_void_p),
         ("n_frames", c_int64),

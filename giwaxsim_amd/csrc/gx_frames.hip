@@ -175,7 +175,9 @@ __global__ __launch_bounds__(BLK) void k_slab_scatter(gx_slab s) {
 __global__ void k_frames_init(unsigned long long *ymin_key, int32_t *bounds, int64_t nf) {
   int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (f < nf) {
-    ymin_key[f] = ~0ull;
+    ymin_key[3 * f] = ~0ull;
+    ymin_key[3 * f + 1] = ~0ull;
+    ymin_key[3 * f + 2] = 0ull;
     bounds[4 * f + 0] = INT_MAX;
     bounds[4 * f + 1] = INT_MIN;
     bounds[4 * f + 2] = INT_MAX;
@@ -212,30 +214,71 @@ __device__ __forceinline__ void block_minmax4(int (&v)[4], int *sh) {
   }
 }
 
-// min over all atoms of y' = x sin(phi) + y cos(phi) (canonical rotate_coords_z);
-// grid (atom chunks, frames), one atomic per block.
+// One pass per frame over the atoms: min y' = x sin(phi) + y cos(phi) over all
+// atoms (canonical rotate_coords_z, voxelgrids.py:319-323) and min / max y' over
+// the atoms whose z row is inside the grid.  Grid (atom chunks, frames), one
+// atomic per block and key; min/max of doubles are exact, so order-free.
 __global__ __launch_bounds__(BLK) void k_frames_ymin(gx_frames d, int64_t f0, int64_t nf) {
   __shared__ double sh[BLK / GX_WAVE];
   const int fb = blockIdx.y;
   const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
   const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const int64_t n_rowed = d.row_off[d.N];
   const int64_t a0 = (int64_t)blockIdx.x * STAT_CHUNK;
   const int64_t a1 = min((int64_t)d.n_atoms, a0 + STAT_CHUNK);
-  double m = INFINITY;
-  for (int64_t a = a0 + threadIdx.x; a < a1; a += BLK) m = fmin(m, d.ax[a] * s + d.ay[a] * c);
+  double m = INFINITY, mz = INFINITY, Mz = -INFINITY;
+  for (int64_t a = a0 + threadIdx.x; a < a1; a += BLK) {
+    const double y = d.ax[a] * s + d.ay[a] * c;
+    m = fmin(m, y);
+    if (a < n_rowed) {
+      mz = fmin(mz, y);
+      Mz = fmax(Mz, y);
+    }
+  }
   m = block_min_d(m, sh);
-  if (threadIdx.x == 0 && m < INFINITY) atomicMin(&d.ymin_key[fb], dkey(m));
+  __syncthreads();
+  mz = block_min_d(mz, sh);
+  __syncthreads();
+  Mz = -block_min_d(-Mz, sh);
+  if (threadIdx.x == 0) {
+    unsigned long long *k = d.ymin_key + 3 * fb;
+    if (m < INFINITY) atomicMin(&k[0], dkey(m));
+    if (mz < INFINITY) atomicMin(&k[1], dkey(mz));
+    if (Mz > -INFINITY) atomicMax(&k[2], dkey(Mz));
+  }
 }
 
-// y/z index bounds over valid atoms (voxelgrids.py:328-335, 346-349)
+// Index bounds from the extremes (floor division is monotone): valid when every
+// atom with z row < N also has y row < N and every atom's z row is < N; frames
+// where that fails are left at their initial values for k_frames_bounds.
+__global__ void k_frames_bounds_fast(gx_frames d, int64_t nf) {
+  const int fb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fb >= nf) return;
+  const unsigned long long *k = d.ymin_key + 3 * fb;
+  int32_t *b = d.idx_bounds + 4 * fb;
+  if (k[1] == ~0ull || d.zrow_max < 0) return;
+  const double ymin = dkey_inv(k[0]);
+  const double rinv = 1.0 / d.r;
+  const long long lo = to_int(floor_div(dkey_inv(k[1]) - ymin, d.r, rinv));
+  const long long hi = to_int(floor_div(dkey_inv(k[2]) - ymin, d.r, rinv));
+  if (hi >= d.N) return;
+  b[0] = (int32_t)lo;
+  b[1] = (int32_t)hi;
+  b[2] = 0;
+  b[3] = (int32_t)d.zrow_max;
+}
+
+// y/z index bounds over valid atoms (voxelgrids.py:328-335, 346-349), per atom;
+// only for frames the fast path above could not settle.
 __global__ __launch_bounds__(BLK) void k_frames_bounds(gx_frames d, int64_t f0, int64_t nf) {
   __shared__ int sh[4 * (BLK / GX_WAVE)];
   const int N = (int)d.N;
   const double rinv = 1.0 / d.r;
   const int fb = blockIdx.y;
+  if (d.idx_bounds[4 * fb] != INT_MAX || d.ymin_key[3 * fb + 1] == ~0ull) return;  // block-uniform
   const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
   const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
-  const double ymin = dkey_inv(d.ymin_key[fb]);
+  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
   const int64_t n_rowed = d.row_off[N];
   const int64_t a0 = (int64_t)blockIdx.x * STAT_CHUNK;
   const int64_t a1 = min(n_rowed, a0 + STAT_CHUNK);
@@ -277,21 +320,22 @@ __device__ __forceinline__ void py_slice(long long a, long long b, long long n, 
   hi = b > a ? b : a;
 }
 
-// gaussian_filter1d(mask, sigma, mode='wrap') of the indicator of [lo, hi),
-// in scipy's symmetric correlate order (centre first, then the outermost
-// pair inwards).
-__device__ double smooth_mask(const double *w, int radius, long long lo, long long hi, int i, int n) {
-  // indices i +- d with d <= radius; radius may exceed n, so wrap with a loop-free
-  // double modulo only when needed
-  auto m = [&](long long k) -> double {
-    if (k < 0 || k >= n) {
-      k %= n;
-      if (k < 0) k += n;
-    }
-    return (k >= lo && k < hi) ? 1.0 : 0.0;
-  };
-  double acc = m(i) * w[0];
-  for (int d = radius; d >= 1; --d) acc += (m((long long)i - d) + m((long long)i + d)) * w[d];
+// gaussian_filter1d(mask, sigma, mode='wrap') of the indicator of [lo, hi):
+// s[i] = sum_{d=-R..R} w[d] [lo <= (i + d) mod n < hi], evaluated from prefix
+// sums of w over the (at most a few) runs of d whose wrapped index is inside.
+__device__ double smooth_mask(const double *cw, int R, long long lo, long long hi, int i, int n) {
+  if (hi <= lo) return 0.0;
+  double acc = 0.0;
+  const long long x0 = (long long)i - R, x1 = (long long)i + R;  // window [x0, x1]
+  // copies [lo + m n, hi + m n) of the interval that meet the window
+  long long m0 = (x0 - hi) / n - 1, m1 = (x1 - lo) / n + 1;
+  for (long long m = m0; m <= m1; ++m) {
+    long long a = lo + m * n, b = hi + m * n - 1;  // inclusive
+    if (a < x0) a = x0;
+    if (b > x1) b = x1;
+    if (a > b) continue;
+    acc += cw[b - i + R + 1] - cw[a - i + R];
+  }
   return acc;
 }
 
@@ -313,9 +357,9 @@ __global__ __launch_bounds__(BLK) void k_frames_columns(gx_frames d, int64_t f0,
   if (d.smooth) {
     long long lo, hi;
     py_slice((long long)b[0] + d.smooth, (long long)b[1] - d.smooth, N, lo, hi);
-    d.col_sy[(int64_t)fb * N + i] = smooth_mask(d.gauss_w, (int)d.smooth_radius, lo, hi, i, N);
+    d.col_sy[(int64_t)fb * N + i] = smooth_mask(d.gauss_cw, (int)d.smooth_radius, lo, hi, i, N);
     py_slice((long long)b[2] + d.smooth, (long long)b[3] - d.smooth, N, lo, hi);
-    d.col_sz[(int64_t)fb * N + i] = smooth_mask(d.gauss_w, (int)d.smooth_radius, lo, hi, i, N);
+    d.col_sz[(int64_t)fb * N + i] = smooth_mask(d.gauss_cw, (int)d.smooth_radius, lo, hi, i, N);
   }
 }
 
@@ -427,7 +471,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
   __syncthreads();
 
   const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
-  const double ymin = dkey_inv(d.ymin_key[fb]);
+  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
   const int32_t *bnd = d.idx_bounds + 4 * fb;
   const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
   // atoms that land in y columns overwritten by the fill never contribute
@@ -533,10 +577,10 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
   }
   __syncthreads();
   // 6) transposed store: inter[fb][w][z0 .. z0+RB)
-  C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * N;
+  C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * d.Z;
   for (int idx = threadIdx.x; idx < wf * RB; idx += BLK) {
     const int w = idx / RB, q = idx - w * RB;
-    out[(int64_t)w * N + z0 + q] = stage[idx];
+    out[(int64_t)w * d.Z + z0 + q] = stage[idx];
   }
 }
 
@@ -565,12 +609,40 @@ __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
   const long long by = to_int(floor_div(hy - d.qmin, d.dq, dqinv)) - d.c_lo;
   if (bx < 0 || bx >= d.c_n || by < 0 || by >= d.c_n) return;
 
-  const C *col = reinterpret_cast<const C *>(d.inter) + ((int64_t)fb * d.W + wcol) * N;
+  using T = typename Real<C>::T;
+  const C *col = reinterpret_cast<const C *>(d.inter) + ((int64_t)fb * d.W + wcol) * d.Z;
+  // rows >= Z hold no atoms: with fill they are the constant avg_voxel_f * max_voxels
+  // (only the k = 0 column), otherwise mz * background spectrum (+ A N at k = 0)
+  const bool k0 = (j == N / 2);
+  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
+  C u = C{(T)0, (T)0};
+  if (!d.fill_bkg && d.smooth) u = reinterpret_cast<const C *>(d.col_spec)[(int64_t)fb * d.W + wcol];
+  const double *sz_row = d.col_sz + (int64_t)fb * N;
   C v[P];
 #pragma unroll
-  for (int t = 0; t < P / R0; ++t)
+  for (int t = 0; t < P / R0; ++t) {
 #pragma unroll
-    for (int q = 0; q < R0; ++q) v[t * R0 + q] = col[lane + GX_WAVE * t + q * (N / R0)];
+    for (int q = 0; q < R0; ++q) {
+      const int z = lane + GX_WAVE * t + q * (N / R0);
+      C x;
+      if (z < d.Z) {
+        x = col[z];
+      } else if (d.fill_bkg) {
+        x = k0 ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
+      } else if (d.smooth) {
+        const double mz = sz_row[z];
+        double xre = mz * (double)u.x, xim = mz * (double)u.y;
+        if (k0) {
+          xre = xre + avv_re * (double)N;
+          xim = xim + avv_im * (double)N;
+        }
+        x = C{(T)xre, (T)xim};
+      } else {
+        x = C{(T)0, (T)0};
+      }
+      v[t * R0 + q] = x;
+    }
+  }
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
   fft_wave<N, C>(v, lds[wave], tw, lane);
 
@@ -660,7 +732,7 @@ int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st)
                                160 * 1024));
     attr_set = true;
   }
-  dim3 grid(N / RB, (unsigned)nf);
+  dim3 grid((unsigned)(d->Z / RB), (unsigned)nf);
   k_rowpass<N, RPW, C><<<grid, BLK, lds, st>>>(*d, f0);
   return check_launch("k_rowpass");
 }
@@ -700,12 +772,14 @@ int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
   GX_REQUIRE(f0 >= 0 && f0 + nf <= d->n_frames, "frames [%lld, %lld) outside [0, %lld)", (long long)f0,
              (long long)(f0 + nf), (long long)d->n_frames);
   GX_REQUIRE(d->W >= 1 && d->W <= d->N, "intermediate width W=%lld outside [1, N]", (long long)d->W);
+  GX_REQUIRE(d->Z >= 8 && d->Z <= d->N && d->Z % 8 == 0, "row range Z=%lld must be a multiple of 8 in [8, N]",
+             (long long)d->Z);
   GX_REQUIRE(d->c_n >= 1, "empty cropped grid");
   GX_REQUIRE(d->fparams && d->ymin_key && d->idx_bounds && d->inter && d->bin_sum && d->bin_cnt && d->twiddle &&
                  d->status && d->row_off && d->row_exp,
              "a required device buffer of gx_frames is NULL");
   GX_REQUIRE(!d->fill_bkg || d->col_nm, "fill_bkg needs col_nm");
-  GX_REQUIRE(!d->smooth || (d->col_sy && d->col_sz && d->gauss_w), "smooth needs col_sy, col_sz, gauss_w");
+  GX_REQUIRE(!d->smooth || (d->col_sy && d->col_sz && d->gauss_cw), "smooth needs col_sy, col_sz, gauss_cw");
   GX_REQUIRE(!(d->fill_bkg || d->smooth) || (d->col_spec && d->twiddle64),
              "fill_bkg / smooth need col_spec and twiddle64");
   GX_REQUIRE(d->n_atoms >= 0 && (d->n_atoms == 0 || (d->ax && d->ay && d->aq && d->azrow)), "atom arrays");
@@ -775,6 +849,7 @@ int gx_frames_stats(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stre
   if (d->n_atoms > 0) {
     dim3 g(grid1d(d->n_atoms, STAT_CHUNK), (unsigned)nf);
     k_frames_ymin<<<g, BLK, 0, st>>>(*d, f0, nf);
+    k_frames_bounds_fast<<<grid1d(nf, 64), 64, 0, st>>>(*d, nf);
     k_frames_bounds<<<g, BLK, 0, st>>>(*d, f0, nf);
   }
   return check_launch("gx_frames_stats");

@@ -201,6 +201,8 @@ class FramePlan:
     i_hi: int
     q_val: float
     max_voxels: float
+    Z: int = 0                    # rows [0, Z) can hold atoms (multiple of 8)
+    z_bound: float = None
     frame_cols: np.ndarray = field(repr=False, default=None)  # kept columns per frame
 
     @property
@@ -229,10 +231,19 @@ def qgrid_from_axis(q_axis, phis, max_q=None):
 def frame_plan(r_voxel_size: float, q_voxel_size: float, max_q: float, x_bound: float,
                y_bound: float, z_bound: float) -> FramePlan:
     N = check_physical(r_voxel_size, q_voxel_size, max_q, x_bound, y_bound, z_bound)
-    return build_frame_plan(N, r_voxel_size, qgrid(q_voxel_size, max_q), x_bound, y_bound)
+    return build_frame_plan(N, r_voxel_size, qgrid(q_voxel_size, max_q), x_bound, y_bound, z_bound)
 
 
-def build_frame_plan(N, r, g: QGrid, x_bound, y_bound) -> FramePlan:
+def row_range(N, r, z_bound):
+    """Rows [0, Z) that can hold atoms: the z row of an atom is
+    (z - zmin) // r <= z_bound // r (floor division is monotone), rounded up to 8."""
+    if z_bound is None:
+        return N
+    zr = int(np.floor_divide(np.float64(z_bound), np.float64(r))) + 1
+    return int(min(N, max(8, -(-zr // 8) * 8)))
+
+
+def build_frame_plan(N, r, g: QGrid, x_bound, y_bound, z_bound=None) -> FramePlan:
     q_h = np.fft.fftfreq(N, d=r) * 2 * np.pi
     q_h_shifted = np.fft.fftshift(q_h)
     q_v_shifted = np.fft.fftshift(np.fft.fftfreq(N, d=r) * 2 * np.pi)
@@ -278,7 +289,8 @@ def build_frame_plan(N, r, g: QGrid, x_bound, y_bound) -> FramePlan:
             fp[f, _lib.FP_JLO], fp[f, _lib.FP_JHI] = 0, -1
     return FramePlan(N=N, r=r, grid=g, x_bound=float(x_bound), y_bound=float(y_bound), fparams=fp,
                      W=int(max(1, widths.max())), i_lo=int(rows[0]), i_hi=int(rows[-1]),
-                     q_val=1.0 / (N * r), max_voxels=float(max_voxels))
+                     q_val=1.0 / (N * r), max_voxels=float(max_voxels), Z=row_range(N, r, z_bound),
+                     z_bound=None if z_bound is None else float(z_bound))
 
 
 @functools.lru_cache(maxsize=16)
@@ -309,6 +321,14 @@ def fft_precision(fill_bkg, smooth, precision='auto'):
     if precision != 'auto':
         raise ValueError(f"precision must be 'auto', 'fp32' or 'fp64', not {precision!r}")
     return not (fill_bkg or smooth)
+
+
+def gauss_prefix(sigma):
+    """Prefix sums cw[t] = sum_{d < t - R} w[d], t = 0..2R+1, of scipy's
+    normalised Gaussian kernel w[-R..R] (truncate 4.0)."""
+    half, R = gauss_half_kernel(sigma)
+    w = np.concatenate([half[:0:-1], half])
+    return np.concatenate([[0.0], np.cumsum(w)]), R
 
 
 def gauss_half_kernel(sigma):
@@ -443,33 +463,36 @@ class FrameRunner:
     def __init__(self, plan: FramePlan, slab: Slab, bins: Bins, fill_bkg, smooth, avg,
                  batch=None, precision='auto'):
         dev = device()
-        N, W = plan.N, plan.W
+        N, W, Z = plan.N, plan.W, plan.Z
         self.plan, self.slab, self.bins = plan, slab, bins
         self.f64 = fft_precision(fill_bkg, smooth, precision)
         esz = 16 if self.f64 else 8
         F = plan.n_frames
         if batch is None:
-            # keep the batch's intermediate (batch * W * N complex) within ~160 MB
-            batch = max(1, min(F, int(160e6 // (W * N * esz))))
+            # keep the batch's intermediate (batch * W * Z complex) within ~160 MB
+            batch = max(1, min(F, int(160e6 // (W * Z * esz))))
         self.batch = int(batch)
         self.fparams = torch.as_tensor(plan.fparams).to(dev).contiguous()
         self.tw = torch.as_tensor(twiddles(N, self.f64)).to(dev).contiguous()
         self.tw64 = torch.as_tensor(twiddles(N, True)).to(dev).contiguous()
         B = self.batch
-        self.ymin_key = _empty(B, torch.int64)
+        self.ymin_key = _empty(3 * B, torch.int64)
         self.idx_bounds = _empty(4 * B, torch.int32)
         self.col_nm = _empty(B * N, torch.int32) if fill_bkg else None
         self.col_sy = _empty(B * N, torch.float64) if smooth else None
         self.col_sz = _empty(B * N, torch.float64) if smooth else None
         cdt = torch.float64 if self.f64 else torch.float32
-        self.inter = _empty(B * W * N * 2, cdt)
+        self.inter = _empty(B * W * Z * 2, cdt)
         self.col_spec = _empty(B * W * 2, cdt) if (fill_bkg or smooth) else None
         self.status = torch.zeros(4, dtype=torch.int32, device=dev)
         if smooth:
-            w, rad = gauss_half_kernel(smooth)
-            self.gauss = torch.as_tensor(w).to(dev).contiguous()
+            cw, rad = gauss_prefix(smooth)
+            self.gauss = torch.as_tensor(cw).to(dev).contiguous()
         else:
             self.gauss, rad = None, 0
+        zr = int(np.floor_divide(np.float64(plan.z_bound), np.float64(plan.r))) \
+            if plan.z_bound is not None else N
+        zrow_max = zr if zr < N else -1
         g = plan.grid
         self.desc = GxFrames(
             N=N, r=plan.r, n_atoms=slab.n,
@@ -477,9 +500,9 @@ class FrameRunner:
             azrow=ptr(slab.szrow).value, row_off=ptr(slab.row_off).value,
             row_exp=ptr(slab.row_exp).value,
             fill_bkg=int(bool(fill_bkg)), smooth=int(smooth), max_voxels=plan.max_voxels,
-            avg_re=float(np.real(avg)), avg_im=float(np.imag(avg)), gauss_w=ptr(self.gauss).value,
-            smooth_radius=rad, qmin=g.qmin, dq=g.dq, c_lo=g.c_lo, c_n=g.c_n, q_val=plan.q_val,
-            i_lo=plan.i_lo, i_hi=plan.i_hi, W=W, bin_scale=bins.scale, fft_f64=int(self.f64),
+            avg_re=float(np.real(avg)), avg_im=float(np.imag(avg)), gauss_cw=ptr(self.gauss).value,
+            smooth_radius=rad, zrow_max=zrow_max, qmin=g.qmin, dq=g.dq, c_lo=g.c_lo, c_n=g.c_n, q_val=plan.q_val,
+            i_lo=plan.i_lo, i_hi=plan.i_hi, W=W, Z=Z, bin_scale=bins.scale, fft_f64=int(self.f64),
             twiddle=ptr(self.tw).value, twiddle64=ptr(self.tw64).value,
             n_frames=F, fparams=ptr(self.fparams).value, batch=self.batch,
             ymin_key=ptr(self.ymin_key).value, idx_bounds=ptr(self.idx_bounds).value,
@@ -710,7 +733,7 @@ class Ensemble:
             f64 = fft_precision(fill_bkg, smooth, precision)
             self.runners.append(FrameRunner(plan, slab, self.bins, fill_bkg, smooth, avg,
                                             batch=batch if batch is not None else
-                                            tuned_batch(plan.N, plan.W, plan.n_frames, f64),
+                                            tuned_batch(plan.N, plan.W, plan.n_frames, f64, plan.Z),
                                             precision=precision))
         self.n_frames = sum(rn.plan.n_frames for rn in self.runners)
         self.graph = None
@@ -751,14 +774,14 @@ class Ensemble:
 _TUNED = {}
 
 
-def tuned_batch(N, W, n_frames, f64=False):
+def tuned_batch(N, W, n_frames, f64=False, Z=None):
     """Frames per launch for a grid edge: the best of a few candidates timed once on
     a synthetic slab, cached per (N, W bucket, device name).  Falls back to the
     memory rule (intermediate <= ~160 MB) when tuning is disabled
     (GIWAXSIM_AUTOTUNE=0)."""
     import os
     esz = 16 if f64 else 8
-    cap = max(1, int(160e6 // (W * N * esz)))
+    cap = max(1, int(160e6 // (W * (Z or N) * esz)))
     if os.environ.get("GIWAXSIM_AUTOTUNE", "1") == "0":
         return min(n_frames, cap)
     key = (N, (W + 63) // 64, bool(f64), torch.cuda.get_device_name())

@@ -122,7 +122,8 @@ def rotate_project_fft_coords(args):
         raise NotImplementedError(f"grid_size {grid_size} has no compiled FFT "
                                   "(supported: 256, 512, 1024, 2048)")
     g = engine.qgrid_from_axis(qz, [phi])
-    plan = engine.build_frame_plan(int(grid_size), float(r_voxel_size), g, x_bound, y_bound)
+    plan = engine.build_frame_plan(int(grid_size), float(r_voxel_size), g, x_bound, y_bound,
+                                   np.max(coords[:, 2]) - np.min(coords[:, 2]))
     f_table, elem_ids = np.unique(np.asarray(f_values, dtype=np.complex128), return_inverse=True)
     counts = np.bincount(elem_ids, minlength=len(f_table))
     B = engine.intensity_bound(float(np.sum(counts * np.abs(f_table))), plan.N, fill_bkg, smooth,

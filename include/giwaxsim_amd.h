@@ -110,8 +110,10 @@ typedef struct gx_frames {
   int64_t smooth;          /* sigma in voxels, 0 = off */
   double max_voxels;       /* sqrt(x_b^2 + y_b^2) // r */
   double avg_re, avg_im;   /* avg_voxel_f */
-  const double *gauss_w;   /* [smooth_radius + 1] normalised half kernel, w[0] = centre */
-  int64_t smooth_radius;   /* int(4 sigma + 0.5) */
+  const double *gauss_cw;  /* [2 smooth_radius + 2] prefix sums of the normalised Gaussian
+                              kernel w[-R..R]: gauss_cw[t] = sum_{d < t - R} w[d] */
+  int64_t smooth_radius;   /* R = int(4 sigma + 0.5) */
+  int64_t zrow_max;        /* largest projected z row of any atom if every row is < N, else -1 */
   /* q geometry */
   double qmin;             /* np.min of the full q axis */
   double dq;               /* np.diff(qz)[0] */
@@ -120,6 +122,8 @@ typedef struct gx_frames {
   double q_val;            /* 1.0 / (N * r), fftfreq */
   int64_t i_lo, i_hi;      /* fftshifted rows whose bin lies in the crop */
   int64_t W;               /* intermediate row width (>= JHI-JLO+1 of every frame) */
+  int64_t Z;               /* z rows that can hold atoms: [0, Z), a multiple of 8, <= N;
+                              rows >= Z are synthesised by the column pass */
   double bin_scale;        /* fixed-point scale of I in bin sums (a power of two) */
   int64_t fft_f64;         /* 0: FP32 transforms (float2), 1: FP64 transforms (double2) */
   const void *twiddle;     /* [N] exp(-2 pi i m / N) as float2 or double2 (fft_f64) */
@@ -128,12 +132,13 @@ typedef struct gx_frames {
   const double *fparams;   /* [n_frames][GX_FP_STRIDE] */
   /* per-frame scratch for a batch of up to `batch` frames */
   int64_t batch;
-  unsigned long long *ymin_key; /* [batch] */
+  unsigned long long *ymin_key; /* [batch][3] order keys of min y' (all atoms), min and
+                                   max y' (atoms with z row < N) */
   int32_t *idx_bounds;     /* [batch][4] y_min, y_max, z_min, z_max */
   int32_t *col_nm;         /* [batch][N] num_missing_voxels per column (fill_bkg) */
   double *col_sy;          /* [batch][N] smoothed y mask (smooth) */
   double *col_sz;          /* [batch][N] smoothed z mask (smooth) */
-  void *inter;             /* [batch][W][N] float2 or double2: row-transformed, column-pruned frame */
+  void *inter;             /* [batch][W][Z] float2 or double2: row-transformed, column-pruned frame */
   void *col_spec;          /* [batch][W] float2 or double2: spectrum of the frame's background /
                               smoothing terms on the kept columns (fill_bkg or smooth) */
   /* output: cropped bins [c][c][c] indexed [qy][qx][qz] */

@@ -1,0 +1,107 @@
+"""CPU checks of the arithmetic choices the kernels make where they deviate
+from a literal restatement (the device code is mirrored here line by line in
+numpy and compared against the oracle / library):
+
+* exact floor division via floor(a * (1/b)) plus one fma-signed correction
+  step (gx::floor_div) vs numpy's fmod-based ``//``;
+* smoothed masks from prefix sums over runs of the wrapped window
+  (smooth_mask) vs scipy.ndimage.gaussian_filter1d(mode='wrap');
+* the linear split of the fill / blend terms into a per-frame background
+  spectrum (k_frames_spectra + k_rowpass step 5) vs the FFT of the full grid.
+"""
+
+from fractions import Fraction
+
+import numpy as np
+import pytest
+from scipy.ndimage import gaussian_filter1d
+
+import oracle as O
+from giwaxsim_amd import engine
+
+
+def fma_sign(q, b, a):
+    """sign of the exact a - q*b (what the device obtains from one fma)."""
+    return np.sign(float(Fraction(a) - Fraction(q) * Fraction(b)))
+
+
+def dev_floor_div(a, b):
+    q = np.floor(a * (1.0 / b))
+    if fma_sign(q, b, a) < 0:
+        q -= 1.0
+    elif fma_sign(q + 1.0, b, a) >= 0:
+        q += 1.0
+    return q
+
+
+def test_device_floor_div():
+    rng = np.random.default_rng(0)
+    for b in (0.2, 0.15, 0.3, 0.0307875, 0.0204399):
+        k = rng.integers(0, 2500, 300)
+        a = k * b
+        a = np.concatenate([a, np.nextafter(a, np.inf), np.nextafter(a, -np.inf),
+                            rng.uniform(0, 500, 300)])
+        got = np.array([dev_floor_div(x, b) for x in a])
+        assert np.array_equal(got, O.floor_div(a, b))
+
+
+def dev_smooth_mask(cw, R, lo, hi, i, n):
+    if hi <= lo:
+        return 0.0
+    acc = 0.0
+    x0, x1 = i - R, i + R
+    m0 = int((x0 - hi) / n) - 1
+    m1 = int((x1 - lo) / n) + 1
+    for m in range(m0, m1 + 1):
+        a, b = lo + m * n, hi + m * n - 1
+        a, b = max(a, x0), min(b, x1)
+        if a > b:
+            continue
+        acc += cw[b - i + R + 1] - cw[a - i + R]
+    return acc
+
+
+@pytest.mark.parametrize("n,sigma,lo,hi", [(256, 3, 40, 200), (256, 3, 0, 256), (128, 10, 5, 20),
+                                           (64, 25, 10, 50), (512, 20, 100, 480), (256, 2, 7, 7)])
+def test_smooth_mask_prefix_sums(n, sigma, lo, hi):
+    mask = np.zeros(n)
+    mask[lo:hi] = 1
+    want = gaussian_filter1d(mask, sigma=sigma, mode='wrap')
+    cw, R = engine.gauss_prefix(sigma)
+    got = np.array([dev_smooth_mask(cw, R, lo, hi, i, n) for i in range(n)])
+    assert np.allclose(got, want, rtol=0, atol=4e-16 * max(1, R / 10))
+
+
+def test_background_spectrum_split():
+    """F_row = mz (FFT(S my) + U) + A N delta_0 with U = FFT(b my - A my) for a row
+    inside the fill, equal to the FFT of the blended row of the oracle."""
+    rng = np.random.default_rng(3)
+    coords = rng.uniform(0, 20, size=(500, 3))
+    f = np.full(500, 6.003 + 0.0016j)
+    N, r, sig, phi = 128, 0.5, 3, 23.0
+    xb, yb, zb = (np.ptp(coords[:, i]) for i in range(3))
+    avg = O.avg_voxel_f_of(f, xb, yb, zb, r)
+    g, st = O.frame_grid(coords, f, phi, N, r, avg, xb, yb, True, sig, return_stats=True)
+    S = O.frame_grid(coords, f, phi, N, r, avg, xb, yb, False, 0)
+    mv = st['max_voxels']
+    A = avg * mv
+    L = O.rectangular_collapse_lengths(np.arange(N) * r, yb, xb, phi)
+    nm = (mv - np.floor_divide(L, r)).astype(int)
+    y = np.arange(N)
+    inside_y = (y > st['y_min']) & (y < st['y_max'])
+    b = np.where(inside_y, nm * avg, A)
+    my = np.zeros(N); my[st['y_min'] + sig:st['y_max'] - sig] = 1
+    mz = np.zeros(N); mz[st['z_min'] + sig:st['z_max'] - sig] = 1
+    sy = gaussian_filter1d(my, sig, mode='wrap')
+    sz = gaussian_filter1d(mz, sig, mode='wrap')
+    U = np.fft.fft(b * sy - A * sy)
+    for z in range(N):
+        want = np.fft.fft(g[z])
+        if st['z_min'] <= z < st['z_max']:
+            Sz = np.where(inside_y, S[z], 0)
+            got = sz[z] * (np.fft.fft(Sz * sy) + U)
+            got[0] += A * N
+        else:
+            got = np.zeros(N, complex)
+            got[0] = A * N
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-9 * abs(want[0]))
