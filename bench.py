@@ -38,10 +38,26 @@ def parse():
                     help="phi-frames of the same slab timed on the host (CPU baseline sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=20)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the ensemble (64 slabs) and large-slab (N=2048) figures")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="one step, then one 256 MiB device copy (counter calibration) and one "
                          "launch of each frame kernel; for rocprofv3 --pmc passes")
     return ap.parse_args()
+
+
+def measured_traffic_per_frame():
+    """Calibrated HBM bytes per frame of the frame kernels, measured with two
+    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) on this workload and kept in
+    profiles/frames_polymer100_N1024_counters.md."""
+    path = os.path.join(ROOT, "profiles", "frames_polymer100_N1024_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_frame:"):
+                return {k: float(v) for k, v in (t.split("=") for t in line.split(":", 1)[1].split())}
+    except OSError:
+        pass
+    return {}
 
 
 def workload():
@@ -126,6 +142,7 @@ def main():
     alg = {"gx_frames_rowpass": nb * (24 * n_atoms + 8 * N * N),
            "gx_frames_colpass": nb * (8 * N * N + 16 * P_mean)}
     achieved = alg[dom] / times[dom] / 1e9
+    traffic = measured_traffic_per_frame()
 
     # ---- detector images/s
     from giwaxsim_amd.tools.comparison import _weights
@@ -143,6 +160,11 @@ def main():
         engine.run_detector(iq, res['qx'], res['qy'], res['qz'], dplan, True)
     torch.cuda.synchronize()
     det_s = (time.perf_counter() - t1) / nd
+
+    # ---- other synthetic configurations, reported beside the headline figure
+    extra = {}
+    if not args.no_extra:
+        extra = extra_configs(args, engine, torch)
 
     # ---- CPU baseline: the oracle in reference-faithful threaded mode on a bounded sample
     cpu = None
@@ -179,12 +201,61 @@ def main():
         "detector_images_per_s": 1.0 / det_s,
         "detector_ms_per_image": det_s * 1e3,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": (traffic[dom.split("_")[-1]] * nb) if dom.split("_")[-1] in traffic
+                     else None,
+                     "traffic_note": "calibrated rocprofv3 FETCH_SIZE+WRITE_SIZE bytes per launch, "
+                                     "profiles/frames_polymer100_N1024_counters.md",
+                     "algorithmic_bytes_per_launch": alg[dom],
                      "launch_ms": times[dom] * 1e3, "frames_per_launch": nb,
                      "kernel_ms": {k: v * 1e3 for k, v in times.items()}},
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
+
+
+def extra_configs(args, engine, torch):
+    """Ensemble of 64 uneven slabs into one shared grid (HIP-graph replay) and one
+    large slab (200 x 200 x 60 A at ~1.3M atoms, N=2048, 1307 frames)."""
+    from giwaxsim_amd.synth import ensemble_slabs, polymer_slab
+    out = {}
+    qv = 2 * np.pi / (0.2 * 1024) * (1 + 1e-9)
+    slabs = ensemble_slabs(n_slabs=64, seed=0, q_voxel=qv)
+    ens = engine.Ensemble(slabs, qv, 2.0, 12700.0, fill_bkg=True, smooth=10)
+    ens.capture()
+    ens.replay()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        ens.replay()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    out["ensemble"] = {"workload": "64 polymer slabs, edges U[40,160] A, N in {256,512,1024,2048}, "
+                                   "one shared 185^3 grid (131^3 crop), fill_bkg, smooth 10",
+                       "n_atoms": int(sum(len(c) for c, _, _ in slabs)), "frames": ens.n_frames,
+                       "ms_per_pass": dt * 1e3, "phi_frames_per_s": ens.n_frames / dt}
+    del ens
+    c, e = polymer_slab(size=(200.0, 200.0, 60.0), seed=5, density=0.54)
+    r = 0.15
+    ids, ftab, _, _ = engine.element_table(e, 12700.0)
+    cd = torch.as_tensor(c).cuda()
+    idd = torch.as_tensor(ids).cuda()
+    run = lambda: engine.run_voxelgrid(cd, None, r, 2 * np.pi / (r * 2048) * (1 + 1e-9), 3.0, 12700.0,
+                                       fill_bkg=True, smooth=20, f_table=ftab, elem_ids=idd)
+    res = run()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        res = run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    out["large_slab"] = {"workload": "polymer slab 200x200x60 A, 1.3M atoms, N=2048, fill_bkg, smooth 20, "
+                                     "max_q 3.0, 417^3 grid (295^3 crop)",
+                         "n_atoms": len(c), "frames": res['plan'].n_frames, "ms_per_slab": dt * 1e3,
+                         "phi_frames_per_s": res['plan'].n_frames / dt}
+    return out
 
 
 if __name__ == "__main__":
