@@ -52,7 +52,8 @@ class GxFrames(ctypes.Structure):
         ("q_val", c_double), ("i_lo", c_int64), ("i_hi", c_int64), ("W", c_int64),
         ("Z", c_int64),
         ("bin_scale", c_double), ("fft_f64", c_int64), ("twiddle", c_void_p),
-        ("twiddle64", c_void_p),
+        ("twiddle64", c_void_p), ("bs_M", c_int64), ("bs_chirp", c_void_p),
+        ("bs_kernel", c_void_p), ("bs_twiddle", c_void_p),
         ("n_frames", c_int64),
         ("fparams", c_void_p), ("batch", c_int64),
         ("ymin_key", c_void_p), ("idx_bounds", c_void_p), ("col_nm", c_void_p),

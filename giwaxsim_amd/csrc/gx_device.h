@@ -186,13 +186,13 @@ template <typename C> struct Dft<16, C> {
 // registers, v[t R + r] = X[j + r N/R] with j = lane + 64 t.
 // tw[m] = exp(-2 pi i m / N), m < N, precomputed in FP64 (rounded for FP32).
 // ------------------------------------------------------------------------------------
-template <int N, int R, int NS, int P, typename C>
+template <int N, int R, int NS, int P, typename C, int L = GX_WAVE>
 __device__ __forceinline__ void fft_butterflies(C (&v)[P], const C* __restrict__ tw, int lane) {
   constexpr int NB = P / R;
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
     if constexpr (NS > 1) {
-      const int j = lane + GX_WAVE * t;
+      const int j = lane + L * t;
       const int k = j & (NS - 1);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
@@ -204,30 +204,30 @@ __device__ __forceinline__ void fft_butterflies(C (&v)[P], const C* __restrict__
   }
 }
 
-template <int N, int R, int NS, int P, typename C>
+template <int N, int R, int NS, int P, typename C, int L = GX_WAVE>
 __device__ __forceinline__ void fft_store(C (&v)[P], C* lds, int lane) {
   constexpr int NB = P / R;
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
-    const int j = lane + GX_WAVE * t;
+    const int j = lane + L * t;
     const int base = (j / NS) * NS * R + (j & (NS - 1));
 #pragma unroll
     for (int r = 0; r < R; ++r) lds[base + r * NS] = v[t * R + r];
   }
 }
 
-template <int N, int R, int P, typename C>
+template <int N, int R, int P, typename C, int L = GX_WAVE>
 __device__ __forceinline__ void fft_load(C (&v)[P], const C* lds, int lane) {
   constexpr int NB = P / R;
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
-    const int j = lane + GX_WAVE * t;
+    const int j = lane + L * t;
 #pragma unroll
     for (int r = 0; r < R; ++r) v[t * R + r] = lds[j + r * (N / R)];
   }
 }
 
-// Radix plans (first stage radix is R0).
+// Radix plans (first stage radix is R0) of the in-wave transforms (64 lanes).
 template <int N> struct FftPlan;
 template <> struct FftPlan<256>  { static constexpr int R0 = 4,  R1 = 4,  R2 = 4, R3 = 4; static constexpr int NST = 4; };
 template <> struct FftPlan<512>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 8, R3 = 1; static constexpr int NST = 3; };
@@ -265,6 +265,61 @@ __device__ __forceinline__ void fft_wave(C (&v)[N / GX_WAVE], C* lds, const C* _
     fft_load<N, R3, P, C>(v, lds, lane);
     wave_sync();
     fft_butterflies<N, R3, R0 * R1 * R2, P, C>(v, tw, lane);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Block-level Stockham transforms of length M over GX_BLK_FFT = 256 threads
+// (P = M / 256 points per thread), used by the chirp-z (Bluestein) path for
+// grid edges that are not powers of two.  The plans have equal first and last
+// radix, so a transform's output registers are already laid out as the next
+// transform's inputs (index j + r M/R with j = tid + 256 t).  Block barriers
+// separate the stages; the caller must reach this with the whole block.
+// ------------------------------------------------------------------------------------
+#define GX_BLK_FFT 256
+template <int M> struct BlkPlan;
+template <> struct BlkPlan<1024> { static constexpr int R = 4; static constexpr int NST = 5; };   // 4^5
+template <> struct BlkPlan<2048> { static constexpr int R = 8; static constexpr int NST = 4; };   // 8 4 8 8
+template <> struct BlkPlan<4096> { static constexpr int R = 16; static constexpr int NST = 3; };  // 16^3
+
+template <int M, typename C>
+__device__ __forceinline__ void fft_block(C (&v)[M / GX_BLK_FFT], C* lds, const C* __restrict__ tw, int tid) {
+  constexpr int P = M / GX_BLK_FFT;
+  constexpr int L = GX_BLK_FFT;
+  if constexpr (M == 1024) {
+    fft_butterflies<M, 4, 1, P, C, L>(v, tw, tid);
+    fft_store<M, 4, 1, P, C, L>(v, lds, tid); __syncthreads();
+    fft_load<M, 4, P, C, L>(v, lds, tid); __syncthreads();
+    fft_butterflies<M, 4, 4, P, C, L>(v, tw, tid);
+    fft_store<M, 4, 4, P, C, L>(v, lds, tid); __syncthreads();
+    fft_load<M, 4, P, C, L>(v, lds, tid); __syncthreads();
+    fft_butterflies<M, 4, 16, P, C, L>(v, tw, tid);
+    fft_store<M, 4, 16, P, C, L>(v, lds, tid); __syncthreads();
+    fft_load<M, 4, P, C, L>(v, lds, tid); __syncthreads();
+    fft_butterflies<M, 4, 64, P, C, L>(v, tw, tid);
+    fft_store<M, 4, 64, P, C, L>(v, lds, tid); __syncthreads();
+    fft_load<M, 4, P, C, L>(v, lds, tid); __syncthreads();
+    fft_butterflies<M, 4, 256, P, C, L>(v, tw, tid);
+  } else if constexpr (M == 2048) {
+    fft_butterflies<M, 8, 1, P, C, L>(v, tw, tid);
+    fft_store<M, 8, 1, P, C, L>(v, lds, tid); __syncthreads();
+    fft_load<M, 4, P, C, L>(v, lds, tid); __syncthreads();
+    fft_butterflies<M, 4, 8, P, C, L>(v, tw, tid);
+    fft_store<M, 4, 8, P, C, L>(v, lds, tid); __syncthreads();
+    fft_load<M, 8, P, C, L>(v, lds, tid); __syncthreads();
+    fft_butterflies<M, 8, 32, P, C, L>(v, tw, tid);
+    fft_store<M, 8, 32, P, C, L>(v, lds, tid); __syncthreads();
+    fft_load<M, 8, P, C, L>(v, lds, tid); __syncthreads();
+    fft_butterflies<M, 8, 256, P, C, L>(v, tw, tid);
+  } else {
+    static_assert(M == 4096, "block transform lengths: 1024, 2048, 4096");
+    fft_butterflies<M, 16, 1, P, C, L>(v, tw, tid);
+    fft_store<M, 16, 1, P, C, L>(v, lds, tid); __syncthreads();
+    fft_load<M, 16, P, C, L>(v, lds, tid); __syncthreads();
+    fft_butterflies<M, 16, 16, P, C, L>(v, tw, tid);
+    fft_store<M, 16, 16, P, C, L>(v, lds, tid); __syncthreads();
+    fft_load<M, 16, P, C, L>(v, lds, tid); __syncthreads();
+    fft_butterflies<M, 16, 256, P, C, L>(v, tw, tid);
   }
 }
 

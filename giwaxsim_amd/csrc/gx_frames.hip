@@ -669,6 +669,308 @@ __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
 }
 
 // ------------------------------------------------------------------------------------
+// grid edges that are not powers of two: chirp-z (Bluestein) transforms
+//
+// X[k] = c[k] sum_n (x[n] c[n]) conj(c[k - n]),  c[n] = exp(-i pi n^2 / N),
+// a linear convolution evaluated as IFFT_M(FFT_M(x c) K) with K = FFT_M of the
+// zero-padded conj(c) (host, FP64) and M >= 2N - 1 a power of two; the inverse
+// transform is conj(FFT(conj(.))) and K already carries the 1/M.
+// ------------------------------------------------------------------------------------
+template <int M, typename C>
+__device__ __forceinline__ void chirpz_block(C (&v)[M / GX_BLK_FFT], C *lds, const gx_frames &d, int tid) {
+  constexpr int P = M / GX_BLK_FFT;
+  constexpr int R = BlkPlan<M>::R;
+  const C *tw = reinterpret_cast<const C *>(d.bs_twiddle);
+  const C *K = reinterpret_cast<const C *>(d.bs_kernel);
+  fft_block<M, C>(v, lds, tw, tid);
+#pragma unroll
+  for (int t = 0; t < P / R; ++t)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = tid + GX_BLK_FFT * t + r * (M / R);
+      const C x = cmul(v[t * R + r], K[k]);
+      v[t * R + r] = C{x.x, -x.y};
+    }
+  __syncthreads();
+  fft_block<M, C>(v, lds, tw, tid);
+#pragma unroll
+  for (int q = 0; q < P; ++q) v[q] = C{v[q].x, -v[q].y};
+}
+
+// per-frame background spectrum on the kept columns by a direct FP64 DFT
+__global__ __launch_bounds__(BLK) void k_frames_spectra_dft(gx_frames d, int64_t f0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2 *u = reinterpret_cast<double2 *>(smem_raw);  // [N]
+  const int N = (int)d.N;
+  const int fb = blockIdx.x;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = jhi - jlo + 1;
+  if (wf <= 0 || wf > d.W) return;
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1];
+  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
+  for (int y = threadIdx.x; y < N; y += BLK) {
+    const double my = d.smooth ? d.col_sy[(int64_t)fb * N + y] : 1.0;
+    double ure = 0.0, uim = 0.0;
+    if (d.fill_bkg) {
+      double bre = avv_re, bim = avv_im;
+      if (y > ylo && y < yhi) {
+        const double nm = (double)d.col_nm[(int64_t)fb * N + y];
+        bre = nm * d.avg_re;
+        bim = nm * d.avg_im;
+      }
+      ure = bre * my;
+      uim = bim * my;
+    }
+    if (d.smooth) {
+      ure = ure - avv_re * my;
+      uim = uim - avv_im * my;
+    }
+    u[y] = make_double2(ure, uim);
+  }
+  __syncthreads();
+  const double2 *tw = reinterpret_cast<const double2 *>(d.twiddle64);
+  for (int w = threadIdx.x; w < wf; w += BLK) {
+    const int k = (jlo + w + N - N / 2) % N;  // unshifted frequency of column jlo + w
+    double re = 0.0, im = 0.0;
+    int m = 0;
+    for (int y = 0; y < N; ++y) {
+      const double2 t = tw[m], x = u[y];
+      re = fma(x.x, t.x, fma(-x.y, t.y, re));
+      im = fma(x.x, t.y, fma(x.y, t.x, im));
+      m += k;
+      if (m >= N) m -= N;
+    }
+    if (d.fft_f64)
+      reinterpret_cast<double2 *>(d.col_spec)[(int64_t)fb * d.W + w] = make_double2(re, im);
+    else
+      reinterpret_cast<float2 *>(d.col_spec)[(int64_t)fb * d.W + w] = make_float2((float)re, (float)im);
+  }
+}
+
+template <int M, typename C>
+__global__ __launch_bounds__(GX_BLK_FFT) void k_rowpass_bs(gx_frames d, int64_t f0) {
+  using T = typename Real<C>::T;
+  constexpr int P = M / GX_BLK_FFT;
+  constexpr int R = BlkPlan<M>::R;
+  constexpr int RB = 8;  // rows per block, transformed one after another by the whole block
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *fbuf = reinterpret_cast<C *>(smem_raw);  // [M] transform exchange; aliased by the accumulators
+  int32_t *acc_re = reinterpret_cast<int32_t *>(fbuf);
+  const int N = (int)d.N;
+  int32_t *acc_im = acc_re + N;
+  C *stage = fbuf + M;               // [W][RB]
+  C *tspec = stage + d.W * RB;       // [W]
+  const int tid = threadIdx.x;
+  const int fb = blockIdx.y;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = jhi - jlo + 1;
+  if (wf <= 0) return;
+  if (wf > d.W) {
+    if (tid == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
+    return;
+  }
+  const bool dense = d.fill_bkg || d.smooth;
+  if (dense) {
+    const C *spec = reinterpret_cast<const C *>(d.col_spec) + (int64_t)fb * d.W;
+    for (int w = tid; w < wf; w += GX_BLK_FFT) tspec[w] = spec[w];
+  }
+  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
+  const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
+  const double r = d.r, rinv = 1.0 / d.r;
+  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
+  const C *chirp = reinterpret_cast<const C *>(d.bs_chirp);
+  const int z0 = blockIdx.x * RB;
+  __syncthreads();
+  for (int rb = 0; rb < RB; ++rb) {
+    const int z = z0 + rb;
+    const bool inside = !d.fill_bkg || (z >= zlo && z < zhi);
+    const int a0 = d.row_off[z], a1 = d.row_off[z + 1];
+    const bool atoms = inside && a1 > a0;  // block-uniform
+    C v[P];
+    if (atoms) {
+      for (int y = tid; y < N; y += GX_BLK_FFT) {
+        acc_re[y] = 0;
+        acc_im[y] = 0;
+      }
+      __syncthreads();
+      for (int a = a0 + tid; a < a1; a += GX_BLK_FFT) {
+        const double yy = (d.ax[a] * s + d.ay[a] * c) - ymin;
+        const long long yi = to_int(floor_div(yy, r, rinv));
+        if (yi >= ya && yi < yb) {
+          atomicAdd(&acc_re[yi], d.aq[2 * (int64_t)a]);
+          atomicAdd(&acc_im[yi], d.aq[2 * (int64_t)a + 1]);
+        }
+      }
+      __syncthreads();
+      const T inv = (T)ldexp(1.0, -d.row_exp[z]);
+#pragma unroll
+      for (int t = 0; t < P / R; ++t)
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int n = tid + GX_BLK_FFT * t + q * (M / R);
+          C x = C{(T)0, (T)0};
+          if (n < N) {
+            const T m = d.smooth ? (T)d.col_sy[(int64_t)fb * N + n] : (T)1;
+            x = cmul(C{((T)acc_re[n] * inv) * m, ((T)acc_im[n] * inv) * m}, chirp[n]);
+          }
+          v[t * R + q] = x;
+        }
+      __syncthreads();
+      chirpz_block<M, C>(v, fbuf, d, tid);
+    } else {
+#pragma unroll
+      for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
+    }
+    const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + z] : 1.0;
+#pragma unroll
+    for (int t = 0; t < P / R; ++t)
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int k = tid + GX_BLK_FFT * t + q * (M / R);
+        if (k >= N) continue;
+        const int w = (k + N / 2) % N - jlo;
+        if (w < 0 || w >= wf) continue;
+        C o = atoms ? cmul(v[t * R + q], chirp[k]) : C{(T)0, (T)0};
+        if (!inside) {
+          o = (k == 0) ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
+        } else if (dense) {
+          const C u = tspec[w];
+          double ore = mz * ((double)o.x + (double)u.x);
+          double oim = mz * ((double)o.y + (double)u.y);
+          if (d.smooth && k == 0) {
+            ore = ore + avv_re * (double)N;
+            oim = oim + avv_im * (double)N;
+          }
+          o = C{(T)ore, (T)oim};
+        }
+        stage[w * RB + rb] = o;
+      }
+    __syncthreads();
+  }
+  C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * d.Z;
+  for (int idx = tid; idx < wf * RB; idx += GX_BLK_FFT) {
+    const int w = idx / RB, q = idx - w * RB;
+    out[(int64_t)w * d.Z + z0 + q] = stage[idx];
+  }
+}
+
+template <int M, typename C>
+__global__ __launch_bounds__(GX_BLK_FFT) void k_colpass_bs(gx_frames d, int64_t f0) {
+  using T = typename Real<C>::T;
+  constexpr int P = M / GX_BLK_FFT;
+  constexpr int R = BlkPlan<M>::R;
+  __shared__ C fbuf[M];
+  const int N = (int)d.N;
+  const int tid = threadIdx.x;
+  const int fb = blockIdx.y;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wcol = blockIdx.x;
+  if (wcol > jhi - jlo || wcol >= d.W) return;  // block-uniform
+  const int j = jlo + wcol;
+  const double hx = (j == N - 1) ? fp[GX_FP_HX_STOP] : (double)j * fp[GX_FP_HX_STEP] + fp[GX_FP_HX_START];
+  const double hy = (j == N - 1) ? fp[GX_FP_HY_STOP] : (double)j * fp[GX_FP_HY_STEP] + fp[GX_FP_HY_START];
+  const double dqinv = 1.0 / d.dq;
+  const long long bx = to_int(floor_div(hx - d.qmin, d.dq, dqinv)) - d.c_lo;
+  const long long by = to_int(floor_div(hy - d.qmin, d.dq, dqinv)) - d.c_lo;
+  if (bx < 0 || bx >= d.c_n || by < 0 || by >= d.c_n) return;
+  const C *col = reinterpret_cast<const C *>(d.inter) + ((int64_t)fb * d.W + wcol) * d.Z;
+  const bool k0 = (j == N / 2);
+  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
+  C u = C{(T)0, (T)0};
+  if (!d.fill_bkg && d.smooth) u = reinterpret_cast<const C *>(d.col_spec)[(int64_t)fb * d.W + wcol];
+  const C *chirp = reinterpret_cast<const C *>(d.bs_chirp);
+  C v[P];
+#pragma unroll
+  for (int t = 0; t < P / R; ++t)
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int z = tid + GX_BLK_FFT * t + q * (M / R);
+      C x = C{(T)0, (T)0};
+      if (z < d.Z) {
+        x = col[z];
+      } else if (z < N) {
+        if (d.fill_bkg) {
+          x = k0 ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
+        } else if (d.smooth) {
+          const double mz = d.col_sz[(int64_t)fb * N + z];
+          double xre = mz * (double)u.x, xim = mz * (double)u.y;
+          if (k0) {
+            xre = xre + avv_re * (double)N;
+            xim = xim + avv_im * (double)N;
+          }
+          x = C{(T)xre, (T)xim};
+        }
+      }
+      v[t * R + q] = (z < N) ? cmul(x, chirp[z]) : C{(T)0, (T)0};
+    }
+  chirpz_block<M, C>(v, fbuf, d, tid);
+  const int64_t cbase = (by * d.c_n + bx) * d.c_n;
+  const double val = d.q_val;
+  const int nn = (N - 1) / 2 + 1;  // fftfreq: k < nn are the non-negative frequencies
+#pragma unroll
+  for (int t = 0; t < P / R; ++t)
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int k = tid + GX_BLK_FFT * t + q * (M / R);
+      if (k >= N) continue;
+      const int i = (k + N / 2) % N;
+      if (i < d.i_lo || i > d.i_hi) continue;
+      const int kint = (k < nn) ? k : k - N;
+      const double qz = (((double)kint * val) * 2.0) * M_PI;
+      const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
+      if (bz < 0 || bz >= d.c_n) continue;
+      const double I = (double)cnorm(cmul(v[t * R + q], chirp[k]));
+      const unsigned long long fx = (unsigned long long)llrint(I * d.bin_scale);
+      atomicAdd(&d.bin_sum[cbase + bz], fx);
+      atomicAdd(&d.bin_cnt[cbase + bz], 1);
+    }
+}
+
+template <int M, typename C>
+int launch_rowpass_bs_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  constexpr int RB = 8;
+  size_t lds = ((size_t)M + (size_t)d->W * (RB + 1)) * sizeof(C);
+  if (lds > 160 * 1024)
+    return set_error(GX_ERR_UNSUPPORTED, "chirp-z row pass needs %zu bytes of LDS (W=%lld)", lds, (long long)d->W);
+  static bool attr_set = false;
+  if (!attr_set) {
+    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass_bs<M, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               160 * 1024));
+    attr_set = true;
+  }
+  dim3 grid((unsigned)(d->Z / RB), (unsigned)nf);
+  k_rowpass_bs<M, C><<<grid, GX_BLK_FFT, lds, st>>>(*d, f0);
+  return check_launch("k_rowpass_bs");
+}
+
+template <typename C>
+int launch_bs(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st, bool row) {
+  if (row) {
+    switch (d->bs_M) {
+      case 1024: return launch_rowpass_bs_t<1024, C>(d, f0, nf, st);
+      case 2048: return launch_rowpass_bs_t<2048, C>(d, f0, nf, st);
+      case 4096: return launch_rowpass_bs_t<4096, C>(d, f0, nf, st);
+    }
+  } else {
+    dim3 grid((unsigned)d->W, (unsigned)nf);
+    switch (d->bs_M) {
+      case 1024: k_colpass_bs<1024, C><<<grid, GX_BLK_FFT, 0, st>>>(*d, f0); return check_launch("k_colpass_bs");
+      case 2048: k_colpass_bs<2048, C><<<grid, GX_BLK_FFT, 0, st>>>(*d, f0); return check_launch("k_colpass_bs");
+      case 4096: k_colpass_bs<4096, C><<<grid, GX_BLK_FFT, 0, st>>>(*d, f0); return check_launch("k_colpass_bs");
+    }
+  }
+  return set_error(GX_ERR_UNSUPPORTED, "chirp-z length M=%lld", (long long)d->bs_M);
+}
+
+// ------------------------------------------------------------------------------------
 // bins
 // ------------------------------------------------------------------------------------
 __global__ __launch_bounds__(BLK) void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -762,10 +1064,16 @@ int launch_spectra(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   return check_launch("k_frames_spectra");
 }
 
+bool pow2_grid(int64_t N) { return N == 256 || N == 512 || N == 1024 || N == 2048; }
+
 int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
   GX_REQUIRE(d != nullptr, "gx_frames descriptor is NULL");
-  GX_REQUIRE(gx_supported_grid(d->N), "grid edge N=%lld has no compiled FFT plan (supported: 256, 512, "
-             "1024, 2048)", (long long)d->N);
+  GX_REQUIRE(gx_supported_grid(d->N), "grid edge N=%lld outside [16, 2048]", (long long)d->N);
+  GX_REQUIRE(pow2_grid(d->N) ? d->bs_M == 0 : (d->bs_M == 1024 || d->bs_M == 2048 || d->bs_M == 4096),
+             "grid edge N=%lld needs bs_M = 0 for 256..2048 powers of two, else a chirp-z length "
+             "1024/2048/4096 (got %lld)", (long long)d->N, (long long)d->bs_M);
+  GX_REQUIRE(!d->bs_M || (d->bs_M >= 2 * d->N - 1 && d->bs_chirp && d->bs_kernel && d->bs_twiddle),
+             "chirp-z tables missing or M < 2N-1");
   GX_REQUIRE(d->r > 0.0, "r_voxel_size must be positive");
   GX_REQUIRE(nf >= 0 && nf <= d->batch, "batch of %lld frames exceeds workspace batch %lld", (long long)nf,
              (long long)d->batch);
@@ -795,9 +1103,7 @@ extern "C" {
 
 int gx_abi_version(void) { return GX_ABI_VERSION; }
 
-int gx_supported_grid(int64_t N) {
-  return N == 256 || N == 512 || N == 1024 || N == 2048;
-}
+int gx_supported_grid(int64_t N) { return N >= 16 && N <= 2048; }
 
 int gx_slab_bounds(const gx_slab *s, double *out_minmax, int64_t *out_elem_count, gx_stream_t stream) {
   clear_error();
@@ -865,6 +1171,10 @@ int gx_frames_columns(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t st
   k_frames_columns<<<g, BLK, 0, st>>>(*d, f0, nf);
   rc = check_launch("gx_frames_columns");
   if (rc || !(d->fill_bkg || d->smooth)) return rc;
+  if (d->bs_M) {
+    k_frames_spectra_dft<<<(unsigned)nf, BLK, (size_t)d->N * sizeof(double2), st>>>(*d, f0);
+    return check_launch("k_frames_spectra_dft");
+  }
   switch (d->N) {
     case 256: return launch_spectra<256>(d, f0, nf, st);
     case 512: return launch_spectra<512>(d, f0, nf, st);
@@ -880,6 +1190,7 @@ int gx_frames_rowpass(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t st
   if (rc) return rc;
   if (nf == 0) return GX_OK;
   hipStream_t st = as_stream(stream);
+  if (d->bs_M) return d->fft_f64 ? launch_bs<double2>(d, f0, nf, st, true) : launch_bs<float2>(d, f0, nf, st, true);
   switch (d->N) {
     case 256: return launch_rowpass<256>(d, f0, nf, st);
     case 512: return launch_rowpass<512>(d, f0, nf, st);
@@ -895,6 +1206,7 @@ int gx_frames_colpass(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t st
   if (rc) return rc;
   if (nf == 0) return GX_OK;
   hipStream_t st = as_stream(stream);
+  if (d->bs_M) return d->fft_f64 ? launch_bs<double2>(d, f0, nf, st, false) : launch_bs<float2>(d, f0, nf, st, false);
   switch (d->N) {
     case 256: return launch_colpass<256>(d, f0, nf, st);
     case 512: return launch_colpass<512>(d, f0, nf, st);

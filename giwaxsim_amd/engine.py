@@ -304,6 +304,34 @@ def twiddles(N: int, f64: bool = False):
     return tw
 
 
+def is_pow2_grid(N):
+    return N in (256, 512, 1024, 2048)
+
+
+@functools.lru_cache(maxsize=16)
+def chirpz_tables(N: int, f64: bool = False):
+    """Chirp-z (Bluestein) tables of a length-N DFT through power-of-two
+    transforms of length M >= 2N - 1: chirp c[n] = exp(-i pi (n^2 mod 2N) / N),
+    kernel K = FFT_M(zero-padded conj(c) wrapped to negative lags) / M, and the
+    length-M twiddles.  All computed in FP64 (n^2 mod 2N in exact integers)."""
+    M = next(m for m in (1024, 2048, 4096) if m >= 2 * N - 1)
+    n = np.arange(N, dtype=np.int64)
+    ph = np.pi * ((n * n) % (2 * N)).astype(np.float64) / N
+    c = np.cos(ph) - 1j * np.sin(ph)
+    b = np.zeros(M, dtype=np.complex128)
+    b[:N] = np.conj(c)
+    b[M - N + 1:] = np.conj(c[1:])[::-1]
+    K = np.fft.fft(b) / M
+    dt = np.float64 if f64 else np.float32
+
+    def inter(z):
+        out = np.empty(2 * len(z), dtype=dt)
+        out[0::2] = z.real
+        out[1::2] = z.imag
+        return out
+    return M, inter(c), inter(K), twiddles(M, f64)
+
+
 def fft_precision(fill_bkg, smooth, precision='auto'):
     """True for FP64 transforms.
 
@@ -475,6 +503,12 @@ class FrameRunner:
         self.fparams = torch.as_tensor(plan.fparams).to(dev).contiguous()
         self.tw = torch.as_tensor(twiddles(N, self.f64)).to(dev).contiguous()
         self.tw64 = torch.as_tensor(twiddles(N, True)).to(dev).contiguous()
+        if is_pow2_grid(N):
+            self.bs_M, self.bs = 0, (None, None, None)
+        else:
+            M, ch, K, twm = chirpz_tables(N, self.f64)
+            self.bs_M = M
+            self.bs = tuple(torch.as_tensor(a).to(dev).contiguous() for a in (ch, K, twm))
         B = self.batch
         self.ymin_key = _empty(3 * B, torch.int64)
         self.idx_bounds = _empty(4 * B, torch.int32)
@@ -503,7 +537,9 @@ class FrameRunner:
             avg_re=float(np.real(avg)), avg_im=float(np.imag(avg)), gauss_cw=ptr(self.gauss).value,
             smooth_radius=rad, zrow_max=zrow_max, qmin=g.qmin, dq=g.dq, c_lo=g.c_lo, c_n=g.c_n, q_val=plan.q_val,
             i_lo=plan.i_lo, i_hi=plan.i_hi, W=W, Z=Z, bin_scale=bins.scale, fft_f64=int(self.f64),
-            twiddle=ptr(self.tw).value, twiddle64=ptr(self.tw64).value,
+            twiddle=ptr(self.tw).value, twiddle64=ptr(self.tw64).value, bs_M=self.bs_M,
+            bs_chirp=ptr(self.bs[0]).value, bs_kernel=ptr(self.bs[1]).value,
+            bs_twiddle=ptr(self.bs[2]).value,
             n_frames=F, fparams=ptr(self.fparams).value, batch=self.batch,
             ymin_key=ptr(self.ymin_key).value, idx_bounds=ptr(self.idx_bounds).value,
             col_nm=ptr(self.col_nm).value, col_sy=ptr(self.col_sy).value,
@@ -550,8 +586,8 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
                       float(zb))
     if not _lib.load().gx_supported_grid(plan.N):
         raise NotImplementedError(
-            f"grid_size N={plan.N} (from 2*pi/(q_voxel_size*r_voxel_size)) has no compiled FFT; "
-            "supported: 256, 512, 1024, 2048")
+            f"grid_size N={plan.N} (from 2*pi/(q_voxel_size*r_voxel_size)) is outside the "
+            "supported range 16..2048")
     sum_f = np.sum(counts * f_table)
     volume = xb * yb * zb
     avg = (sum_f / volume) * r_voxel_size ** 3

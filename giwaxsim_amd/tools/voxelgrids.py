@@ -119,8 +119,7 @@ def rotate_project_fft_coords(args):
     _same_axes(qx, qy, qz)
     coords = np.asarray(coords, dtype=np.float64)
     if not engine._lib.load().gx_supported_grid(int(grid_size)):
-        raise NotImplementedError(f"grid_size {grid_size} has no compiled FFT "
-                                  "(supported: 256, 512, 1024, 2048)")
+        raise NotImplementedError(f"grid_size {grid_size} is outside the supported range 16..2048")
     g = engine.qgrid_from_axis(qz, [phi])
     plan = engine.build_frame_plan(int(grid_size), float(r_voxel_size), g, x_bound, y_bound,
                                    np.max(coords[:, 2]) - np.min(coords[:, 2]))

@@ -96,7 +96,8 @@ typedef struct gx_slab {
  * The phi-frame pipeline of one prepared slab into a cropped 3D bin grid.
  * ---------------------------------------------------------------------------------- */
 typedef struct gx_frames {
-  int64_t N;               /* real-space grid edge: 256, 512, 1024 or 2048 */
+  int64_t N;               /* real-space grid edge: 16..2048 (powers of two 256..2048 use
+                              in-wave transforms, other edges chirp-z transforms) */
   double r;                /* r_voxel_size */
   int64_t n_atoms;         /* atoms (ymin runs over all of them) */
   const double *ax;        /* regrouped atoms (gx_slab.sx) */
@@ -128,6 +129,11 @@ typedef struct gx_frames {
   int64_t fft_f64;         /* 0: FP32 transforms (float2), 1: FP64 transforms (double2) */
   const void *twiddle;     /* [N] exp(-2 pi i m / N) as float2 or double2 (fft_f64) */
   const double *twiddle64; /* [N] the same as double2 (per-frame background spectra) */
+  /* grid edges that are not powers of two: chirp-z (Bluestein) transforms of length M */
+  int64_t bs_M;            /* 0: N is a power of two (in-wave transforms); else 1024, 2048 or 4096 >= 2N-1 */
+  const void *bs_chirp;    /* [N] exp(-i pi (n^2 mod 2N) / N) as float2 / double2 */
+  const void *bs_kernel;   /* [M] FFT_M of the zero-padded conjugate chirp, divided by M */
+  const void *bs_twiddle;  /* [M] exp(-2 pi i m / M) */
   int64_t n_frames;        /* frames described by fparams */
   const double *fparams;   /* [n_frames][GX_FP_STRIDE] */
   /* per-frame scratch for a batch of up to `batch` frames */
@@ -168,7 +174,7 @@ typedef struct gx_detector {
 
 const char *gx_last_error(void);
 int gx_abi_version(void);
-/* 1 if the FFT of this grid edge is compiled in. */
+/* 1 if grid edge N is supported (16 <= N <= 2048). */
 int gx_supported_grid(int64_t N);
 
 /* Slab bounds: min/max of x, y, z (device out[6] = xmin,xmax,ymin,ymax,zmin,zmax)

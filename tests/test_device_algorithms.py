@@ -105,3 +105,18 @@ def test_background_spectrum_split():
             got = np.zeros(N, complex)
             got[0] = A * N
         assert np.allclose(got, want, rtol=1e-12, atol=1e-9 * abs(want[0]))
+
+
+@pytest.mark.parametrize("N", [131, 300, 1048, 2000])
+def test_chirpz_tables(N):
+    """The chirp-z identity with the product's tables reproduces a length-N DFT."""
+    M, ch, K, _ = engine.chirpz_tables(N, True)
+    assert M >= 2 * N - 1 and M in (1024, 2048, 4096)
+    c = ch[0::2] + 1j * ch[1::2]
+    Kc = K[0::2] + 1j * K[1::2]
+    x = np.random.default_rng(N).normal(size=N) + 1j * np.random.default_rng(N + 1).normal(size=N)
+    a = np.zeros(M, complex)
+    a[:N] = x * c
+    conv = np.conj(np.fft.fft(np.conj(np.fft.fft(a) * Kc)))   # IFFT via conj(FFT(conj)); 1/M in K
+    X = c * conv[:N]
+    assert np.allclose(X, np.fft.fft(x), rtol=1e-12, atol=1e-10)

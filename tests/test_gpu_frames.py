@@ -34,6 +34,15 @@ def _case(name):
     if name == 'polymer1024_smooth':
         c, e = polymer_slab(size=(40.0, 40.0, 30.0), seed=4)
         return c, e, dict(r=0.2, qv=2 * np.pi / (0.2 * 1024) * (1 + 1e-9), mq=2.0, fill=True, smooth=10)
+    if name == 'graphite300':          # not a power of two: chirp-z transforms, FP64 (sparse)
+        c, e = graphite_slab(size=(16.0, 16.0, 16.0), seed=5)
+        return c, e, dict(r=0.25, qv=2 * np.pi / (0.25 * 300) * (1 + 1e-9), mq=2.5, fill=False, smooth=0)
+    if name == 'polymer700_fill':      # chirp-z, FP32 (dense)
+        c, e = polymer_slab(size=(30.0, 30.0, 24.0), seed=6)
+        return c, e, dict(r=0.25, qv=2 * np.pi / (0.25 * 700) * (1 + 1e-9), mq=2.0, fill=True, smooth=0)
+    if name == 'polymer1048_fill_smooth':  # the reference's example settings (N = 1048 = 8 x 131)
+        c, e = polymer_slab(size=(40.0, 30.0, 40.0), seed=7)
+        return c, e, dict(r=0.3, qv=0.02, mq=2.0, fill=True, smooth=5)
     raise KeyError(name)
 
 
@@ -84,7 +93,10 @@ def test_full_small(name):
 
 
 @pytest.mark.parametrize("name,frames", [('graphite512_fill', (0, 48)),
-                                         ('polymer1024_smooth', (100, 116))])
+                                         ('polymer1024_smooth', (100, 116)),
+                                         ('graphite300', (0, 40)),
+                                         ('polymer700_fill', (30, 50)),
+                                         ('polymer1048_fill_smooth', (200, 210))])
 def test_frame_subsets(name, frames):
     _, _, ref, res = _run_both(name, frames)
     _check(ref, res)
