@@ -14,6 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_void_p
 from .build import LIB
 
 GX_OK = 0
+GX_ABI_VERSION = 2   # structure layouts below match this version of the header
 GX_FP_STRIDE = 24
 GX_MOMENTS_WORK = 774
 GX_FIN_MIRROR = 1
@@ -37,6 +38,7 @@ class GxSlab(ctypes.Structure):
         ("f_table", c_void_p),
         ("sx", c_void_p), ("sy", c_void_p), ("sq", c_void_p), ("szrow", c_void_p),
         ("row_off", c_void_p), ("row_exp", c_void_p), ("work", c_void_p),
+        ("szf", c_void_p), ("sperm", c_void_p), ("selem", c_void_p),
     ]
 
 
@@ -73,6 +75,19 @@ class GxDetector(ctypes.Structure):
     ]
 
 
+GX_ADJ_RED_BLOCKS = 64
+
+
+class GxAdjoint(ctypes.Structure):
+    _fields_ = [
+        ("n_elem", c_int64), ("f_table", c_void_p), ("afz", c_void_p), ("aperm", c_void_p),
+        ("aelem", c_void_p), ("ecnt", c_void_p), ("z", c_void_p), ("zmin", c_double),
+        ("row_exp", c_void_p), ("ainv", c_void_p),
+        ("g_bins", c_void_p), ("g_part", c_void_p), ("red", c_void_p), ("amin", c_void_p),
+        ("g_atom", c_void_p),
+    ]
+
+
 _P = c_void_p
 _D = c_double
 _I = c_int64
@@ -96,6 +111,16 @@ SIGNATURES = {
     "gx_scale_offset_moments": [_P, _P, _P, _I, _P, POINTER(c_double), _P],
     "gx_apply_f0": [_P, _I, _I, _I, _P, _P, _P, POINTER(c_double), _D, _P, _P],
     "gx_bins_accumulate_f64": [_P, _P, _I, _D, _P, _P, _P],
+    "gx_adj_prepare": [POINTER(GxFrames), POINTER(GxAdjoint), _P],
+    "gx_adj_rowpass": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
+    "gx_adj_forward": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
+    "gx_adj_colpass": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
+    "gx_adj_rowgrad": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
+    "gx_adj_reduce": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
+    "gx_adj_backward": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
+    "gx_adj_finish": [POINTER(GxFrames), POINTER(GxAdjoint), _P, _P, _P],
+    "gx_bins_adjoint": [_P, _P, _P, _I, _I, _I, _D, _D, _D, POINTER(c_double), _D, _P, _P],
+    "gx_detector_adjoint": [POINTER(GxDetector), _P, _I, _P, _P, _P, _P],
 }
 EXPORTS = ["gx_last_error", "gx_abi_version", "gx_supported_grid"] + list(SIGNATURES)
 
@@ -122,6 +147,9 @@ def load(path: str | None = None):
         fn = getattr(lib, name)
         fn.restype = c_int
         fn.argtypes = args
+    if lib.gx_abi_version() != GX_ABI_VERSION:
+        raise GxError(f"{path} has ABI version {lib.gx_abi_version()}, the binding expects "
+                      f"{GX_ABI_VERSION}: rebuild it")
     _lib = lib
     return lib
 

@@ -1,0 +1,335 @@
+"""Differentiable variant of the phi-frame path, with a hand-written adjoint.
+
+The reference has no gradients; nothing here replaces a reference function.
+Its nearest-grid-point deposit (tools/voxelgrids.py:328-341) is piecewise
+constant in the atom positions, so this variant deposits every atom with
+cloud-in-cell (bilinear) weights on the y-z grid; the FFT, |F|^2, binning with
+counts, mean, crop, f0 weighting and the detector gather / mirror / floor are
+the forward path's (tools/voxelgrids.py:381-503, tools/comparison.py:765-870).
+
+``DifferentiableSlab`` maps (atom coordinates, per-element complex f) to the
+finalized detector image through ``torch.autograd.Function``; its backward is
+the HIP adjoint (include/giwaxsim_amd.h, gx_adj_* / gx_bins_adjoint /
+gx_detector_adjoint), which recomputes every phi-frame's transform instead of
+storing it.  Gradients: dL/d coords [n, 3], dL/d f_re and dL/d f_im per
+element.  A scale / flat background on the image is ordinary torch algebra
+(``image_loss``).
+
+Limits: power-of-two grid edges (256..2048), fill_bkg off, smooth 0, every
+atom at least one voxel inside the grid edge.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import GxAdjoint, GxDetector, GxFrames, GxSlab, call
+from .engine import (_empty, bin_scale_for, detector_plan, device, element_table, frame_plan,
+                     intensity_bound, is_pow2_grid, pixels_per_bin, ptr, stream_handle, twiddles)
+from .tools.ptable_dict import aff_dict, ptable
+
+
+def _uniform(a):
+    a = np.atleast_1d(np.asarray(a, dtype=np.float64))
+    return a, np.ones_like(a) / len(a)
+
+
+class _DetectorImage(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, coords, f_re, f_im, model):
+        out = model._forward(coords, f_re, f_im)
+        ctx.model = model
+        ctx.generation = model._generation
+        ctx.coords_dtype = coords.dtype
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        model = ctx.model
+        if model._generation != ctx.generation:
+            raise RuntimeError("DifferentiableSlab ran another forward before this backward; "
+                               "keep one forward/backward pair in flight per model")
+        gc, gf = model._backward(g)
+        return gc.to(ctx.coords_dtype), gf[:, 0].clone(), gf[:, 1].clone(), None
+
+
+class DifferentiableSlab:
+    """Detector image of one slab as a differentiable function of its atoms.
+
+    ``elements``: element symbol per atom (fixed); ``coords0``: (n, 3)
+    coordinates that fix the plan (grid edge N = ceil(2 pi / (q_voxel r)), q
+    axis, crop and phi frames exactly as voxelgridmaker_fitting,
+    tools/comparison.py:705-731); ``detector``: detectormaker_fitting's
+    arguments as a dict (num_pixels, max_q, psis, phis, thetas, optional
+    *_weights, angle_init_vals, angle_init_axs, mirror).  ``frames=(lo, hi)``
+    restricts the phi-frames.  ``precision``: 'auto' (FP64 transforms, the
+    grid is sparse), 'fp32' or 'fp64'.
+
+    ``model(coords, f_re, f_im)`` -> finalized image [n_pix, n_pix] (FP64
+    device tensor).  f_re / f_im (per element, order of ``self.elements``)
+    default to the tabulated f = Z + f1 + i f2 at ``energy``.
+    """
+
+    def __init__(self, elements, coords0, r_voxel_size, q_voxel_size, max_q, detector,
+                 energy=12700.0, frames=None, precision='auto', batch=None):
+        dev = device()
+        ids, ftab, _, uniq = element_table(elements, energy)
+        self.elements = [str(u) for u in uniq]
+        self.f_re0 = torch.as_tensor(ftab.real.copy()).to(dev)
+        self.f_im0 = torch.as_tensor(ftab.imag.copy()).to(dev)
+        self.n_elem = len(ftab)
+        c0 = np.asarray(coords0, dtype=np.float64)
+        self.n = n = int(c0.shape[0])
+        xb, yb, zb = (float(np.ptp(c0[:, i])) for i in range(3))
+        self.r = r = float(r_voxel_size)
+        plan = frame_plan(r, float(q_voxel_size), float(max_q), xb, yb, zb)
+        if not is_pow2_grid(plan.N):
+            raise NotImplementedError(f"the differentiable path needs a power-of-two grid edge "
+                                      f"256..2048, got N={plan.N}")
+        self.plan = plan
+        N, W = plan.N, plan.W
+        F = plan.n_frames
+        self.frames = (0, F) if frames is None else (int(frames[0]), int(frames[1]))
+        self.f64 = {'auto': True, 'fp64': True, 'fp32': False}[precision]
+        esz = 16 if self.f64 else 8
+        nf = self.frames[1] - self.frames[0]
+        if batch is None:
+            cap_inter = 160e6 // (W * N * esz)
+            cap_part = 512e6 // (32 * max(n, 1))
+            batch = max(1, min(nf, int(cap_inter), int(cap_part)))
+        self.batch = B = int(batch)
+        self._pixels_per_bin = pixels_per_bin(plan)
+
+        # atoms and the regrouped slab
+        self.x, self.y, self.z = (_empty(n, torch.float64) for _ in range(3))
+        self.elem = torch.as_tensor(ids).to(dev).to(torch.int32).contiguous()
+        self.ftab = _empty(2 * self.n_elem, torch.float64)
+        self.sx, self.sy = _empty(n, torch.float64), _empty(n, torch.float64)
+        self.sq = _empty(2 * n, torch.int32)
+        self.szrow = _empty(n, torch.int32)
+        self.row_off = _empty(N + 2, torch.int32)
+        self.row_exp = _empty(N, torch.int32)
+        self.work = _empty(N * self.n_elem + 2 * (N + 1), torch.int32)
+        self.szf = _empty(n, torch.float64)
+        self.sperm = _empty(n, torch.int32)
+        self.selem = _empty(n, torch.int32)
+        self.minmax = _empty(6, torch.float64)
+        self.ecount = _empty(self.n_elem, torch.int64)
+
+        # frames, bins
+        g = plan.grid
+        self.fparams = torch.as_tensor(plan.fparams).to(dev).contiguous()
+        self.tw = torch.as_tensor(twiddles(N, self.f64)).to(dev).contiguous()
+        self.tw64 = torch.as_tensor(twiddles(N, True)).to(dev).contiguous()
+        self.ymin_key = _empty(3 * B, torch.int64)
+        self.idx_bounds = _empty(4 * B, torch.int32)
+        cdt = torch.float64 if self.f64 else torch.float32
+        self.inter = _empty(B * W * N * 2, cdt)
+        self.status = torch.zeros(4, dtype=torch.int32, device=dev)
+        nb = g.c_n ** 3
+        self.bin_sum = _empty(nb, torch.int64)
+        self.bin_cnt = _empty(nb, torch.int32)
+        self.iq64 = _empty(nb, torch.float64)
+
+        # adjoint
+        self.adj_row_exp = _empty(N, torch.int32)
+        self.ainv = _empty(n, torch.int32)
+        self.g_bins = _empty(nb, torch.float64)
+        self.g_part = _empty(B * 4 * n, torch.float64)
+        self.red = _empty(max(B, 2 * (1 + self.n_elem)) * _lib.GX_ADJ_RED_BLOCKS, torch.float64)
+        self.amin = _empty(B + 1, torch.int32)
+        self.g_atom = _empty(5 * n, torch.float64)
+        self.g_coords = _empty(3 * n, torch.float64)
+        self.g_f = _empty(2 * self.n_elem, torch.float64)
+
+        # detector
+        d = dict(detector)
+        psis, pw = _uniform(d['psis'])
+        phis, phw = _uniform(d.get('phis', [0.0]))
+        ths, thw = _uniform(d.get('thetas', [0.0]))
+        pw = pw if d.get('psi_weights') is None else np.asarray(d['psi_weights'], dtype=np.float64)
+        phw = phw if d.get('phi_weights') is None else np.asarray(d['phi_weights'], dtype=np.float64)
+        thw = thw if d.get('theta_weights') is None else np.asarray(d['theta_weights'], dtype=np.float64)
+        dp = detector_plan(int(d['num_pixels']), float(d['max_q']), d.get('angle_init_vals', ()),
+                           d.get('angle_init_axs', ()), psis, pw, phis, phw, ths, thw)
+        self.n_pix = npx = dp.n_pix
+        self.mirror = bool(d.get('mirror', True))
+        self.det_h = torch.as_tensor(dp.h).to(dev)
+        self.det_v = torch.as_tensor(dp.v).to(dev)
+        self.det_iR = torch.as_tensor(dp.init_R).to(dev).contiguous()
+        self.det_oR = torch.as_tensor(dp.orient_R).to(dev).contiguous()
+        self.det_ow = torch.as_tensor(dp.orient_w).to(dev).contiguous()
+        self.det_raw = _empty(npx * npx, torch.float64)
+        self.det_out = _empty(npx * npx, torch.float64)
+        self.det_work = _empty(2 * npx * npx, torch.float64)
+        self.g_fx = _empty(nb, torch.int64)
+        self.fx_scale = _empty(1, torch.float64)
+        ax = g.crop_axis
+        self.detector_desc = GxDetector(
+            n_pix=npx, n_init=len(dp.init_R), n_orient=len(dp.orient_w),
+            h_axis=ptr(self.det_h).value, v_axis=ptr(self.det_v).value,
+            init_R=ptr(self.det_iR).value, orient_R=ptr(self.det_oR).value,
+            orient_w=ptr(self.det_ow).value, iq=ptr(self.iq64).value, iq_f64=1,
+            cy=g.c_n, cx=g.c_n, cz=g.c_n, qx_min=float(np.min(ax)), qy_min=float(np.min(ax)),
+            qz_min=float(np.min(ax)), dq=float(np.diff(ax)[0]), det=ptr(self.det_raw).value)
+        self._generation = 0
+        self._prepared = None
+
+    # ----------------------------------------------------------------------------
+    def __call__(self, coords, f_re=None, f_im=None):
+        f_re = self.f_re0 if f_re is None else f_re
+        f_im = self.f_im0 if f_im is None else f_im
+        return _DetectorImage.apply(coords, f_re, f_im, self)
+
+    def descriptors(self, coords, f_re, f_im):
+        """Upload atoms and f, regroup the slab and return (gx_frames, gx_adjoint)."""
+        dev = device()
+        c = torch.as_tensor(coords).detach().to(device=dev, dtype=torch.float64)
+        if c.shape != (self.n, 3):
+            raise ValueError(f"coords must be ({self.n}, 3), got {tuple(c.shape)}")
+        self.x.copy_(c[:, 0])
+        self.y.copy_(c[:, 1])
+        self.z.copy_(c[:, 2])
+        ft = self.ftab.view(self.n_elem, 2)
+        ft[:, 0].copy_(torch.as_tensor(f_re).detach().to(dev, torch.float64))
+        ft[:, 1].copy_(torch.as_tensor(f_im).detach().to(dev, torch.float64))
+        st = stream_handle()
+        sb = GxSlab(n_atoms=self.n, n_elem=self.n_elem, x=ptr(self.x).value, y=ptr(self.y).value,
+                    z=ptr(self.z).value, elem=ptr(self.elem).value)
+        call("gx_slab_bounds", ctypes.byref(sb), ptr(self.minmax), ptr(self.ecount), st)
+        mm = self.minmax.cpu().numpy()
+        counts = self.ecount.cpu().numpy()
+        xb, yb, zb, zmin = mm[1] - mm[0], mm[3] - mm[2], mm[5] - mm[4], float(mm[4])
+        N, r = self.plan.N, self.r
+        if math.hypot(xb, yb) / r + 2.0 >= N or zb / r + 2.0 >= N:
+            raise ValueError(f"slab extent ({xb:.3f}, {yb:.3f}, {zb:.3f}) A does not fit the "
+                             f"{N} x {N} grid of {r} A voxels with a one-voxel margin")
+        Z = int(min(N, max(8, -(-(int(np.floor_divide(zb, r)) + 2) // 8) * 8)))
+        slab = GxSlab(n_atoms=self.n, n_elem=self.n_elem, N=N, r=r, zmin=zmin,
+                      x=ptr(self.x).value, y=ptr(self.y).value, z=ptr(self.z).value,
+                      elem=ptr(self.elem).value, f_table=ptr(self.ftab).value,
+                      sx=ptr(self.sx).value, sy=ptr(self.sy).value, sq=ptr(self.sq).value,
+                      szrow=ptr(self.szrow).value, row_off=ptr(self.row_off).value,
+                      row_exp=ptr(self.row_exp).value, work=ptr(self.work).value,
+                      szf=ptr(self.szf).value, sperm=ptr(self.sperm).value,
+                      selem=ptr(self.selem).value)
+        call("gx_slab_sort_rows", ctypes.byref(slab), st)
+        ftab = ft.cpu().numpy()
+        sum_abs = float(np.sum(counts * np.abs(ftab[:, 0] + 1j * ftab[:, 1])))
+        nf = self.frames[1] - self.frames[0]
+        B = intensity_bound(sum_abs, N, False, 0, 0.0, 0.0, 0.0, 0.0, r)
+        self.bin_scale = bin_scale_for([(nf, self._pixels_per_bin, B)])
+        p, g = self.plan, self.plan.grid
+        fr = GxFrames(
+            N=N, r=r, n_atoms=self.n, ax=ptr(self.sx).value, ay=ptr(self.sy).value,
+            aq=ptr(self.sq).value, azrow=ptr(self.szrow).value, row_off=ptr(self.row_off).value,
+            row_exp=ptr(self.row_exp).value, fill_bkg=0, smooth=0, max_voxels=p.max_voxels,
+            zrow_max=int(np.floor_divide(zb, r)), qmin=g.qmin, dq=g.dq, c_lo=g.c_lo, c_n=g.c_n,
+            q_val=p.q_val, i_lo=p.i_lo, i_hi=p.i_hi, W=p.W, Z=Z, bin_scale=self.bin_scale,
+            fft_f64=int(self.f64), twiddle=ptr(self.tw).value, twiddle64=ptr(self.tw64).value,
+            bs_M=0, n_frames=p.n_frames, fparams=ptr(self.fparams).value, batch=self.batch,
+            ymin_key=ptr(self.ymin_key).value, idx_bounds=ptr(self.idx_bounds).value,
+            inter=ptr(self.inter).value, bin_sum=ptr(self.bin_sum).value,
+            bin_cnt=ptr(self.bin_cnt).value, status=ptr(self.status).value)
+        adj = GxAdjoint(
+            n_elem=self.n_elem, f_table=ptr(self.ftab).value, afz=ptr(self.szf).value,
+            aperm=ptr(self.sperm).value, aelem=ptr(self.selem).value, ecnt=ptr(self.work).value,
+            z=ptr(self.z).value, zmin=zmin, row_exp=ptr(self.adj_row_exp).value,
+            ainv=ptr(self.ainv).value, g_bins=ptr(self.g_bins).value,
+            g_part=ptr(self.g_part).value, red=ptr(self.red).value, amin=ptr(self.amin).value,
+            g_atom=ptr(self.g_atom).value)
+        return fr, adj
+
+    def _check_status(self):
+        s = self.status.cpu().numpy()
+        if s[1] or s[2]:
+            raise RuntimeError(f"internal: differentiable path status {s.tolist()}")
+
+    def _forward(self, coords, f_re, f_im):
+        fr, adj = self.descriptors(coords, f_re, f_im)
+        self.status.zero_()
+        st = stream_handle()
+        call("gx_adj_prepare", ctypes.byref(fr), ctypes.byref(adj), st)
+        g = self.plan.grid
+        call("gx_bins_clear", ptr(self.bin_sum), ptr(self.bin_cnt), g.c_n ** 3, st)
+        lo, hi = self.frames
+        for f0 in range(lo, hi, self.batch):
+            call("gx_adj_forward", ctypes.byref(fr), ctypes.byref(adj), f0, min(self.batch, hi - f0), st)
+        self.finalize_image(st)
+        self._check_status()
+        self._prepared = (fr, adj)
+        self._generation += 1
+        return self.det_out.view(self.n_pix, self.n_pix).clone()
+
+    def finalize_image(self, st):
+        g = self.plan.grid
+        start, step, stop = g.linspace_params
+        aff = _lib.dbl_array(aff_dict['C'])
+        call("gx_bins_finalize", ptr(self.bin_sum), ptr(self.bin_cnt), g.c_n, self.bin_scale, g.c_lo,
+             g.q_num, start, step, stop, aff, float(ptable['C']), ptr(self.iq64), ptr(None), st)
+        call("gx_detector_accumulate", ctypes.byref(self.detector_desc), st)
+        call("gx_detector_finalize", ptr(self.det_raw), self.n_pix, self.n_pix, self._flags(),
+             ptr(self.det_out), st)
+
+    def _flags(self):
+        return _lib.GX_FIN_FLOOR_SCALE | (_lib.GX_FIN_MIRROR if self.mirror else 0)
+
+    def _backward(self, g_out):
+        fr, adj = self._prepared
+        st = stream_handle()
+        go = g_out.detach().to(device=device(), dtype=torch.float64).contiguous()
+        self.backward_enqueue(fr, adj, go, st)
+        self._check_status()
+        return (self.g_coords.view(self.n, 3).clone(), self.g_f.view(self.n_elem, 2).clone())
+
+    def backward_enqueue(self, fr, adj, g_out, st):
+        """Enqueue the whole adjoint for the image cotangent ``g_out`` [n_pix^2]."""
+        g = self.plan.grid
+        call("gx_detector_adjoint", ctypes.byref(self.detector_desc), ptr(g_out), self._flags(),
+             ptr(self.det_work), ptr(self.g_fx), ptr(self.fx_scale), st)
+        start, step, stop = g.linspace_params
+        aff = _lib.dbl_array(aff_dict['C'])
+        call("gx_bins_adjoint", ptr(self.g_fx), ptr(self.fx_scale), ptr(self.bin_cnt), g.c_n, g.c_lo,
+             g.q_num, start, step, stop, aff, float(ptable['C']), ptr(self.g_bins), st)
+        call("gx_adj_prepare", ctypes.byref(fr), ctypes.byref(adj), st)
+        lo, hi = self.frames
+        for f0 in range(lo, hi, self.batch):
+            call("gx_adj_backward", ctypes.byref(fr), ctypes.byref(adj), f0, min(self.batch, hi - f0), st)
+        call("gx_adj_finish", ctypes.byref(fr), ctypes.byref(adj), ptr(self.g_coords), ptr(self.g_f), st)
+
+    def forward_enqueue(self, fr, adj, st):
+        """Enqueue the forward (bins cleared, every frame, image) for prepared descriptors."""
+        g = self.plan.grid
+        call("gx_adj_prepare", ctypes.byref(fr), ctypes.byref(adj), st)
+        call("gx_bins_clear", ptr(self.bin_sum), ptr(self.bin_cnt), g.c_n ** 3, st)
+        lo, hi = self.frames
+        for f0 in range(lo, hi, self.batch):
+            call("gx_adj_forward", ctypes.byref(fr), ctypes.byref(adj), f0, min(self.batch, hi - f0), st)
+        self.finalize_image(st)
+
+
+def image_loss(image, target, scale, background):
+    """sum (scale * image + background - target)^2 (torch; differentiable in
+    every argument)."""
+    return torch.sum((scale * image + background - target) ** 2)
+
+
+def fit_scale_background(image, target):
+    """Least-squares scale and flat background of ``image`` onto ``target``
+    (optimize_scale_offset, tools/comparison.py:873-882, without a mask)."""
+    dev = device()
+    s = image.detach().to(device=dev, dtype=torch.float64).contiguous().reshape(-1)
+    y = target.detach().to(device=dev, dtype=torch.float64).contiguous().reshape(-1)
+    m = torch.zeros(s.numel(), dtype=torch.uint8, device=dev)
+    work = torch.empty(_lib.GX_MOMENTS_WORK, dtype=torch.float64, device=dev)
+    res = (ctypes.c_double * 6)()
+    call("gx_scale_offset_moments", ptr(s), ptr(y), ptr(m), int(s.numel()), ptr(work), res,
+         stream_handle())
+    ms, my, sss, ssy = res[1], res[2], res[3], res[4]
+    scale = ssy / sss if sss != 0 else 0.0
+    return scale, my - scale * ms

@@ -1,0 +1,783 @@
+// gfx950 kernels of the differentiable phi-frame path and its hand-written adjoint.
+//
+// The reference (tools/voxelgrids.py rotate_project_fft_coords 311-416 and
+// process_file2 464-506) has no gradients.  Its nearest-grid-point deposit is
+// piecewise constant in the atom positions, so this variant deposits with
+// cloud-in-cell (bilinear) weights; everything after the deposit is the forward
+// path's (k_colpass binning, k_bins_finalize, the detector gather).
+//
+// Forward per batch of frames:
+//   gx_frames_stats   min y' per frame (origin of u)
+//   k_adj_rowpass<N>  per z row v: atoms of rows v-1 (weight tv) and v (weight 1-tv),
+//                     bilinear in u, exact int64 fixed-point LDS sums, FFT along y,
+//                     kept column band staged and stored transposed (d.inter)
+//   k_colpass<N>      (forward path) FFT along z, |F|^2, bins + counts
+// Backward per batch (frames recomputed, never stored):
+//   k_adj_rowpass<N>  as above
+//   k_adj_colpass<N>  per kept column: FFT along z, cotangent 2 g_bins[bin] F on the
+//                     binned pixels, adjoint DFT (conj FFT conj) back to z, in place
+//   k_adj_rowgrad<N>  per block of rows (+1 halo row): adjoint DFT along y of the kept
+//                     band, then every atom of the rows gathers its four nodes:
+//                     dL/du, dL/dv, dL/df per (frame, atom) into g_part
+//   k_adj_frame_sums  per frame: sum_a dL/dy'_a in original atom order, argmin y'
+//   k_adj_reduce      g_atom += sum_f (sin, cos) dL/dy', dL/dz, dL/df over the batch
+//   k_adj_frame_apply origin term: the argmin atom of min y' takes -sum_a dL/dy'_a
+// Sums over atoms run in original atom order with fixed block partitions, and
+// the row deposit is exact, so every result is bitwise reproducible.
+#include "gx_device.h"
+#include "gx_host.h"
+
+#include <limits.h>
+#include <math.h>
+
+using namespace gx;
+
+namespace {
+
+constexpr int BLK = 256;
+constexpr int NWAVE = BLK / GX_WAVE;
+constexpr int RED = GX_ADJ_RED_BLOCKS;
+
+// fractional grid coordinate along y of an atom in a frame: u = (y' - ymin) / r,
+// u0 = exact floor (the same index arithmetic as the forward path), tu in [0, 1]
+__device__ __forceinline__ void cic_u(double x, double y, double s, double c, double ymin, double r, double rinv,
+                                      long long &u0, double &tu) {
+  const double yy = (x * s + y * c) - ymin;
+  u0 = to_int(floor_div(yy, r, rinv));
+  const double t = yy * rinv - (double)u0;
+  tu = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+}
+
+template <typename C>
+__device__ __forceinline__ C cconj(C a) {
+  return C{a.x, -a.y};
+}
+
+// ------------------------------------------------------------------------------------
+// preparation
+// ------------------------------------------------------------------------------------
+// Deposit exponent of row v: the row receives f w from atoms of rows v-1 and v with
+// w <= 1, so 2^S sum |f| + (rounding) stays below 2^62.
+__global__ __launch_bounds__(BLK) void k_adj_rowexp(gx_frames d, gx_adjoint g) {
+  const int N = (int)d.N;
+  for (int v = blockIdx.x * BLK + threadIdx.x; v < N; v += gridDim.x * BLK) {
+    double bound = 0.0;
+    double cnt = 0.0;
+    for (int e = 0; e < g.n_elem; ++e) {
+      const double fm = fmax(fabs(g.f_table[2 * e]), fabs(g.f_table[2 * e + 1]));
+      double c = (double)g.ecnt[(int64_t)v * g.n_elem + e];
+      if (v > 0) c += (double)g.ecnt[(int64_t)(v - 1) * g.n_elem + e];
+      bound += c * fm;
+      cnt += c;
+    }
+    int S = 0;
+    if (bound > 0.0) {
+      S = ilogb((4611686018427387904.0 - 2.0 * cnt - 2.0) / bound);
+      if (S > 200) S = 200;
+      if (S < -200) S = -200;
+    }
+    g.row_exp[v] = S;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_adj_init(gx_frames d, gx_adjoint g) {
+  const int64_t n = d.n_atoms;
+  for (int64_t a = blockIdx.x * (int64_t)BLK + threadIdx.x; a < n; a += (int64_t)gridDim.x * BLK) {
+    g.ainv[g.aperm[a]] = (int32_t)a;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) g.g_atom[k * n + a] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) p[i] = v;
+}
+
+// ------------------------------------------------------------------------------------
+// forward row pass with the cloud-in-cell deposit
+// ------------------------------------------------------------------------------------
+template <int N, int RPW, typename C>
+__global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, int64_t f0) {
+  using T = typename Real<C>::T;
+  constexpr int P = N / GX_WAVE;
+  constexpr int RB = NWAVE * RPW;
+  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  unsigned long long *accs = reinterpret_cast<unsigned long long *>(smem_raw);  // [NWAVE][2][N]
+  C *stage = reinterpret_cast<C *>(accs + (size_t)NWAVE * 2 * N);            // [W][RB]
+  double *ftab = reinterpret_cast<double *>(stage + d.W * RB);               // [n_elem][2]
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int wave = threadIdx.x / GX_WAVE;
+  unsigned long long *acc_re = accs + (size_t)wave * 2 * N;
+  unsigned long long *acc_im = acc_re + N;
+  C *wbuf = reinterpret_cast<C *>(acc_re);  // FFT exchange buffer (N sizeof(C) <= 16 N bytes)
+
+  const int fb = blockIdx.y;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = jhi - jlo + 1;
+  if (wf <= 0) return;  // block-uniform
+  if (wf > d.W) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
+    return;
+  }
+  for (int i = threadIdx.x; i < 2 * g.n_elem; i += BLK) ftab[i] = g.f_table[i];
+  __syncthreads();
+
+  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
+  const double r = d.r, rinv = 1.0 / d.r;
+  const C *tw = reinterpret_cast<const C *>(d.twiddle);
+  const int z0 = blockIdx.x * RB;
+
+  for (int rr = 0; rr < RPW; ++rr) {
+    const int rb = wave * RPW + rr;
+    const int v = z0 + rb;
+    const int a_lo = d.row_off[v > 0 ? v - 1 : 0], a_mid = d.row_off[v], a_hi = d.row_off[v + 1];
+    C x[P];
+    if (a_hi > a_lo) {
+      for (int y = lane; y < N; y += GX_WAVE) {
+        acc_re[y] = 0;
+        acc_im[y] = 0;
+      }
+      wave_sync();
+      const double sc = ldexp(1.0, g.row_exp[v]);
+      for (int a = a_lo + lane; a < a_hi; a += GX_WAVE) {
+        const double tv = g.afz[a];
+        const double wv = (a < a_mid) ? tv : 1.0 - tv;  // atoms of row v-1 reach v with tv
+        long long u0;
+        double tu;
+        cic_u(d.ax[a], d.ay[a], s, c, ymin, r, rinv, u0, tu);
+        const int e = g.aelem[a];
+        const double fr = ftab[2 * e], fi = ftab[2 * e + 1];
+        const double w0 = wv * (1.0 - tu), w1 = wv * tu;
+        if (u0 >= 0 && u0 + 1 < N) {
+          atomicAdd(&acc_re[u0], (unsigned long long)llrint((fr * w0) * sc));
+          atomicAdd(&acc_im[u0], (unsigned long long)llrint((fi * w0) * sc));
+          atomicAdd(&acc_re[u0 + 1], (unsigned long long)llrint((fr * w1) * sc));
+          atomicAdd(&acc_im[u0 + 1], (unsigned long long)llrint((fi * w1) * sc));
+        } else {
+          atomicAdd(&d.status[2], 1);
+        }
+      }
+      wave_sync();
+      const double inv = ldexp(1.0, -g.row_exp[v]);
+#pragma unroll
+      for (int t = 0; t < P / R0; ++t) {
+#pragma unroll
+        for (int q = 0; q < R0; ++q) {
+          const int y = lane + GX_WAVE * t + q * (N / R0);
+          x[t * R0 + q] = C{(T)((double)(long long)acc_re[y] * inv), (T)((double)(long long)acc_im[y] * inv)};
+        }
+      }
+      wave_sync();
+      fft_wave<N, C>(x, wbuf, tw, lane);
+    } else {
+#pragma unroll
+      for (int q = 0; q < P; ++q) x[q] = C{(T)0, (T)0};
+    }
+#pragma unroll
+    for (int t = 0; t < P / RL; ++t) {
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        const int k = lane + GX_WAVE * t + q * (N / RL);
+        const int w = ((k + N / 2) & (N - 1)) - jlo;
+        if (w >= 0 && w < wf) stage[w * RB + rb] = x[t * RL + q];
+      }
+    }
+    wave_sync();
+  }
+  __syncthreads();
+  C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * d.Z;
+  for (int idx = threadIdx.x; idx < wf * RB; idx += BLK) {
+    const int w = idx / RB, q = idx - w * RB;
+    out[(int64_t)w * d.Z + z0 + q] = stage[idx];
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// adjoint of the column pass (in place on d.inter)
+// ------------------------------------------------------------------------------------
+template <int N, typename C>
+__global__ __launch_bounds__(BLK) void k_adj_colpass(gx_frames d, gx_adjoint g, int64_t f0) {
+  using T = typename Real<C>::T;
+  constexpr int P = N / GX_WAVE;
+  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  __shared__ C lds[NWAVE][N];
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int wave = threadIdx.x / GX_WAVE;
+  const int fb = blockIdx.y;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wcol = blockIdx.x * NWAVE + wave;
+  if (wcol > jhi - jlo || wcol >= d.W) return;  // wave-uniform; no block barrier below
+  C *col = reinterpret_cast<C *>(d.inter) + ((int64_t)fb * d.W + wcol) * d.Z;
+  const int j = jlo + wcol;
+  // the forward's column -> bin arithmetic (k_colpass)
+  const double hx = (j == N - 1) ? fp[GX_FP_HX_STOP] : (double)j * fp[GX_FP_HX_STEP] + fp[GX_FP_HX_START];
+  const double hy = (j == N - 1) ? fp[GX_FP_HY_STOP] : (double)j * fp[GX_FP_HY_STEP] + fp[GX_FP_HY_START];
+  const double dqinv = 1.0 / d.dq;
+  const long long bx = to_int(floor_div(hx - d.qmin, d.dq, dqinv)) - d.c_lo;
+  const long long by = to_int(floor_div(hy - d.qmin, d.dq, dqinv)) - d.c_lo;
+  if (bx < 0 || bx >= d.c_n || by < 0 || by >= d.c_n) {  // column never binned: zero cotangent
+    for (int z = lane; z < d.Z; z += GX_WAVE) col[z] = C{(T)0, (T)0};
+    return;
+  }
+  C v[P];
+#pragma unroll
+  for (int t = 0; t < P / R0; ++t) {
+#pragma unroll
+    for (int q = 0; q < R0; ++q) {
+      const int z = lane + GX_WAVE * t + q * (N / R0);
+      v[t * R0 + q] = (z < d.Z) ? col[z] : C{(T)0, (T)0};
+    }
+  }
+  const C *tw = reinterpret_cast<const C *>(d.twiddle);
+  C *buf = lds[wave];
+  fft_wave<N, C>(v, buf, tw, lane);
+  wave_sync();
+  // conj of the cotangent dL/dF = 2 (dL/dI) F at its natural index k
+  const int64_t cbase = (by * d.c_n + bx) * d.c_n;
+  const double val = d.q_val;
+#pragma unroll
+  for (int t = 0; t < P / RL; ++t) {
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int k = lane + GX_WAVE * t + q * (N / RL);
+      const int i = (k + N / 2) & (N - 1);
+      C gf = C{(T)0, (T)0};
+      if (i >= d.i_lo && i <= d.i_hi) {
+        const int kint = (k < N / 2) ? k : k - N;
+        const double qz = (((double)kint * val) * 2.0) * M_PI;
+        const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
+        if (bz >= 0 && bz < d.c_n) {
+          const double gi = 2.0 * g.g_bins[cbase + bz];
+          const C F = v[t * RL + q];
+          gf = C{(T)(gi * (double)F.x), (T)(-(gi * (double)F.y))};
+        }
+      }
+      buf[k] = gf;
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int t = 0; t < P / R0; ++t)
+#pragma unroll
+    for (int q = 0; q < R0; ++q) v[t * R0 + q] = buf[lane + GX_WAVE * t + q * (N / R0)];
+  wave_sync();
+  fft_wave<N, C>(v, buf, tw, lane);
+  // dL/dx[z] = conj(FFT(conj(dL/dF)))[z]
+#pragma unroll
+  for (int t = 0; t < P / RL; ++t) {
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int z = lane + GX_WAVE * t + q * (N / RL);
+      if (z < d.Z) col[z] = cconj(v[t * RL + q]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// adjoint of the row pass: rows of the grid's cotangent, gathered at the atoms
+// ------------------------------------------------------------------------------------
+template <int N, typename C>
+struct RowGradShape {
+  // H rows per block: RB = H - 1 rows own atoms, the last one is the halo row v0 + 1
+  static constexpr int H = (N * (int)sizeof(C) <= 8192) ? 8 : ((N * (int)sizeof(C) <= 16384) ? 4 : 2);
+};
+
+template <int N, typename C>
+__global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, int64_t f0) {
+  using T = typename Real<C>::T;
+  constexpr int P = N / GX_WAVE;
+  constexpr int H = RowGradShape<N, C>::H, RB = H - 1;
+  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *rows = reinterpret_cast<C *>(smem_raw);  // [H][N] cotangent of the grid rows
+  C *stage = rows + H * N;                    // [W][H]
+  double *ftab = reinterpret_cast<double *>(stage + d.W * H);
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int wave = threadIdx.x / GX_WAVE;
+  const int fb = blockIdx.y;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = max(0, min(jhi - jlo + 1, (int)d.W));
+  const int z0 = blockIdx.x * RB;
+  const int64_t n = d.n_atoms;
+
+  for (int i = threadIdx.x; i < 2 * g.n_elem; i += BLK) ftab[i] = g.f_table[i];
+  const C *src = reinterpret_cast<const C *>(d.inter) + (int64_t)fb * d.W * d.Z;
+  for (int idx = threadIdx.x; idx < wf * H; idx += BLK) {
+    const int w = idx / H, q = idx - w * H;
+    const int v = z0 + q;
+    stage[idx] = (v < d.Z) ? src[(int64_t)w * d.Z + v] : C{(T)0, (T)0};
+  }
+  __syncthreads();
+  const C *tw = reinterpret_cast<const C *>(d.twiddle);
+  for (int q = wave; q < H; q += NWAVE) {
+    C x[P];
+#pragma unroll
+    for (int t = 0; t < P / R0; ++t) {
+#pragma unroll
+      for (int qq = 0; qq < R0; ++qq) {
+        const int k = lane + GX_WAVE * t + qq * (N / R0);
+        const int w = ((k + N / 2) & (N - 1)) - jlo;
+        x[t * R0 + qq] = (w >= 0 && w < wf) ? cconj(stage[w * H + q]) : C{(T)0, (T)0};
+      }
+    }
+    C *buf = rows + q * N;
+    fft_wave<N, C>(x, buf, tw, lane);
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < P / RL; ++t)
+#pragma unroll
+      for (int qq = 0; qq < RL; ++qq) buf[lane + GX_WAVE * t + qq * (N / RL)] = cconj(x[t * RL + qq]);
+  }
+  __syncthreads();
+
+  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
+  const double r = d.r, rinv = 1.0 / d.r;
+  const int a_lo = d.row_off[z0], a_hi = d.row_off[min(z0 + RB, (int)d.Z)];
+  double *gp = g.g_part + (int64_t)fb * 4 * n;
+  for (int a = a_lo + threadIdx.x; a < a_hi; a += BLK) {
+    const int vr = d.azrow[a] - z0;  // 0 .. RB-1
+    long long u0;
+    double tu;
+    cic_u(d.ax[a], d.ay[a], s, c, ymin, r, rinv, u0, tu);
+    const double tv = g.afz[a];
+    const int e = g.aelem[a];
+    const double fr = ftab[2 * e], fi = ftab[2 * e + 1];
+    double gu = 0.0, gv = 0.0, gfr = 0.0, gfi = 0.0;
+    if (u0 >= 0 && u0 + 1 < N && vr >= 0 && vr < RB) {
+      const C *r0 = rows + vr * N + u0;
+      const C *r1 = r0 + N;
+      const double g00r = (double)r0[0].x, g00i = (double)r0[0].y;
+      const double g01r = (double)r0[1].x, g01i = (double)r0[1].y;
+      const double g10r = (double)r1[0].x, g10i = (double)r1[0].y;
+      const double g11r = (double)r1[1].x, g11i = (double)r1[1].y;
+      // h = Re(conj(dL/dG) f) at the four nodes
+      const double h00 = g00r * fr + g00i * fi, h01 = g01r * fr + g01i * fi;
+      const double h10 = g10r * fr + g10i * fi, h11 = g11r * fr + g11i * fi;
+      const double w00 = (1.0 - tv) * (1.0 - tu), w01 = (1.0 - tv) * tu;
+      const double w10 = tv * (1.0 - tu), w11 = tv * tu;
+      gu = (1.0 - tv) * (h01 - h00) + tv * (h11 - h10);
+      gv = (1.0 - tu) * (h10 - h00) + tu * (h11 - h01);
+      gfr = ((w00 * g00r + w01 * g01r) + w10 * g10r) + w11 * g11r;
+      gfi = ((w00 * g00i + w01 * g01i) + w10 * g10i) + w11 * g11i;
+    } else {
+      atomicAdd(&d.status[2], 1);
+    }
+    gp[a] = gu * rinv;
+    gp[n + a] = gv * rinv;
+    gp[2 * n + a] = gfr;
+    gp[3 * n + a] = gfi;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// reductions (fixed partitions of the original atom order: bitwise reproducible)
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, GX_WAVE);
+  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int i = 0; i < NWAVE; ++i) r += sh[i];
+  return r;
+}
+
+__device__ __forceinline__ int block_min_int(int v, int *sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, GX_WAVE));
+  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  int r = INT_MAX;
+  for (int i = 0; i < NWAVE; ++i) r = min(r, sh[i]);
+  return r;
+}
+
+// grid (RED, nf): per frame, partial sums of dL/dy' over original atoms and the
+// smallest original index whose y' equals the frame's min y'
+__global__ __launch_bounds__(BLK) void k_adj_frame_sums(gx_frames d, gx_adjoint g, int64_t f0) {
+  __shared__ double shd[NWAVE];
+  __shared__ int shi[NWAVE];
+  const int fb = blockIdx.y, b = blockIdx.x;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
+  const int64_t n = d.n_atoms, n_rowed = d.row_off[d.N];
+  const int64_t chunk = (n + RED - 1) / RED;
+  const int64_t i0 = b * chunk, i1 = min(n, i0 + chunk);
+  const double *gy = g.g_part + (int64_t)fb * 4 * n;
+  double acc = 0.0;
+  int am = INT_MAX;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += BLK) {
+    const int a = g.ainv[i];
+    if (a < n_rowed) acc += gy[a];
+    const double yp = d.ax[a] * s + d.ay[a] * c;
+    if (yp == ymin) am = min(am, (int)i);
+  }
+  acc = block_sum(acc, shd);
+  am = block_min_int(am, shi);
+  if (threadIdx.x == 0) {
+    g.red[(int64_t)fb * RED + b] = acc;
+    if (am != INT_MAX) atomicMin(&g.amin[fb], am);
+  }
+}
+
+// g_atom += over the batch's frames in order: (sin, cos) dL/dy', dL/dz, dL/df
+__global__ __launch_bounds__(BLK) void k_adj_reduce(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf) {
+  const int64_t n = d.n_atoms, n_rowed = d.row_off[d.N];
+  for (int64_t a = blockIdx.x * (int64_t)BLK + threadIdx.x; a < n_rowed; a += (int64_t)gridDim.x * BLK) {
+    double gx = 0.0, gy = 0.0, gz = 0.0, gr = 0.0, gi = 0.0;
+    for (int64_t fb = 0; fb < nf; ++fb) {
+      const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+      const double *gp = g.g_part + fb * 4 * n;
+      const double u = gp[a];
+      gx += fp[GX_FP_SIN] * u;
+      gy += fp[GX_FP_COS] * u;
+      gz += gp[n + a];
+      gr += gp[2 * n + a];
+      gi += gp[3 * n + a];
+    }
+    g.g_atom[a] += gx;
+    g.g_atom[n + a] += gy;
+    g.g_atom[2 * n + a] += gz;
+    g.g_atom[3 * n + a] += gr;
+    g.g_atom[4 * n + a] += gi;
+  }
+}
+
+// One block: S_f = sum of the frame's partials (in block order), then the origin
+// term d(min y')/d(argmin atom) = 1 applied frame by frame by one thread.
+__global__ __launch_bounds__(BLK) void k_adj_frame_apply(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf) {
+  __shared__ double S[BLK];
+  const int64_t n = d.n_atoms;
+  for (int64_t base = 0; base < nf; base += BLK) {
+    const int64_t fb = base + threadIdx.x;
+    if (fb < nf) {
+      double acc = 0.0;
+      for (int b = 0; b < RED; ++b) acc += g.red[fb * RED + b];
+      S[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int64_t k = 0; k < BLK && base + k < nf; ++k) {
+        const int64_t f = base + k;
+        const int am = g.amin[f];
+        if (am == INT_MAX) continue;
+        const double *fp = d.fparams + (f0 + f) * GX_FP_STRIDE;
+        const int a = g.ainv[am];
+        g.g_atom[a] -= fp[GX_FP_SIN] * S[k];
+        g.g_atom[n + a] -= fp[GX_FP_COS] * S[k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// grid (RED, 1 + n_elem): y = 0 sums dL/dz over atoms (original order) and finds
+// the smallest original index with z == zmin; y = 1 + e sums dL/df of element e
+__global__ __launch_bounds__(BLK) void k_adj_fin_sums(gx_frames d, gx_adjoint g, int64_t slot) {
+  __shared__ double shd[NWAVE];
+  __shared__ int shi[NWAVE];
+  const int b = blockIdx.x, y = blockIdx.y;
+  const int64_t n = d.n_atoms;
+  const int64_t chunk = (n + RED - 1) / RED;
+  const int64_t i0 = b * chunk, i1 = min(n, i0 + chunk);
+  double s0 = 0.0, s1 = 0.0;
+  int am = INT_MAX;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += BLK) {
+    const int a = g.ainv[i];
+    if (y == 0) {
+      s0 += g.g_atom[2 * n + a];
+      if (g.z[i] == g.zmin) am = min(am, (int)i);
+    } else if (g.aelem[a] == y - 1) {
+      s0 += g.g_atom[3 * n + a];
+      s1 += g.g_atom[4 * n + a];
+    }
+  }
+  s0 = block_sum(s0, shd);
+  s1 = block_sum(s1, shd);
+  if (y == 0) am = block_min_int(am, shi);
+  if (threadIdx.x == 0) {
+    double *o = g.red + ((int64_t)y * RED + b) * 2;
+    o[0] = s0;
+    o[1] = s1;
+    if (y == 0 && am != INT_MAX) atomicMin(&g.amin[slot], am);
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_adj_fin_apply(gx_frames d, gx_adjoint g, int64_t slot, double *g_f) {
+  const int y = threadIdx.x;
+  if (y > g.n_elem) return;
+  double s0 = 0.0, s1 = 0.0;
+  for (int b = 0; b < RED; ++b) {
+    s0 += g.red[((int64_t)y * RED + b) * 2];
+    s1 += g.red[((int64_t)y * RED + b) * 2 + 1];
+  }
+  if (y == 0) {
+    const int am = g.amin[slot];
+    if (am != INT_MAX) g.g_atom[2 * d.n_atoms + g.ainv[am]] -= s0;  // origin term of min z
+  } else {
+    g_f[2 * (y - 1)] = s0;
+    g_f[2 * (y - 1) + 1] = s1;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_adj_fin_out(gx_frames d, gx_adjoint g, double *g_coords) {
+  const int64_t n = d.n_atoms, n_rowed = d.row_off[d.N];
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    const int a = g.ainv[i];
+    const bool in = a < n_rowed;
+    g_coords[3 * i] = in ? g.g_atom[a] : 0.0;
+    g_coords[3 * i + 1] = in ? g.g_atom[n + a] : 0.0;
+    g_coords[3 * i + 2] = in ? g.g_atom[2 * n + a] : 0.0;
+  }
+}
+
+struct AffArgs {
+  double a[9];
+};
+
+__global__ __launch_bounds__(BLK) void k_bins_adjoint(const unsigned long long *__restrict__ g_fx,
+                                                      const double *fx_scale, const int32_t *__restrict__ bcnt,
+                                                      int64_t c_n, int64_t c_lo, int64_t q_num, double qa_start,
+                                                      double qa_step, double qa_stop, AffArgs aff, double z_elem,
+                                                      double c16pi2, double *g_bins) {
+  const int64_t n = c_n * c_n * c_n;
+  const double inv = 1.0 / fx_scale[0];
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    const int32_t cnt = bcnt[i];
+    double gb = 0.0;
+    if (cnt != 0) {
+      const int64_t iz = i % c_n, ix = (i / c_n) % c_n, iy = i / (c_n * c_n);
+      const double qx = lin_axis(c_lo + ix, q_num, qa_start, qa_step, qa_stop);
+      const double qy = lin_axis(c_lo + iy, q_num, qa_start, qa_step, qa_stop);
+      const double qz = lin_axis(c_lo + iz, q_num, qa_start, qa_step, qa_stop);
+      const double q2 = (qx * qx + qy * qy) + qz * qz;
+      const double giq = (double)(long long)g_fx[i] * inv;
+      gb = (giq * f0_weight(aff.a, q2, z_elem, c16pi2)) / (double)cnt;
+    }
+    g_bins[i] = gb;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------
+template <int N, typename C>
+int launch_adj_rowpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
+  constexpr int RPW = (N <= 1024) ? 2 : 1;
+  constexpr int RB = NWAVE * RPW;
+  const size_t lds = (size_t)NWAVE * 2 * N * 8 + (size_t)d->W * RB * sizeof(C) + (size_t)g->n_elem * 16;
+  if (lds > 160 * 1024)
+    return set_error(GX_ERR_UNSUPPORTED, "differentiable row pass needs %zu bytes of LDS (N=%d, W=%lld)", lds, N,
+                     (long long)d->W);
+  static bool attr_set = false;
+  if (!attr_set) {
+    GX_HIP(hipFuncSetAttribute((const void *)k_adj_rowpass<N, RPW, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               160 * 1024));
+    attr_set = true;
+  }
+  if (d->Z % RB) return set_error(GX_ERR_ARG, "Z=%lld must be a multiple of %d", (long long)d->Z, RB);
+  dim3 grid((unsigned)(d->Z / RB), (unsigned)nf);
+  k_adj_rowpass<N, RPW, C><<<grid, BLK, lds, st>>>(*d, *g, f0);
+  return check_launch("k_adj_rowpass");
+}
+
+template <int N, typename C>
+int launch_adj_colpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
+  dim3 grid(grid1d(d->W, NWAVE), (unsigned)nf);
+  k_adj_colpass<N, C><<<grid, BLK, 0, st>>>(*d, *g, f0);
+  return check_launch("k_adj_colpass");
+}
+
+template <int N, typename C>
+int launch_adj_rowgrad_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
+  constexpr int H = RowGradShape<N, C>::H, RB = H - 1;
+  const size_t lds = ((size_t)H * N + (size_t)d->W * H) * sizeof(C) + (size_t)g->n_elem * 16;
+  if (lds > 160 * 1024)
+    return set_error(GX_ERR_UNSUPPORTED, "adjoint row pass needs %zu bytes of LDS (N=%d, W=%lld)", lds, N,
+                     (long long)d->W);
+  static bool attr_set = false;
+  if (!attr_set) {
+    GX_HIP(hipFuncSetAttribute((const void *)k_adj_rowgrad<N, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               160 * 1024));
+    attr_set = true;
+  }
+  dim3 grid((unsigned)((d->Z + RB - 1) / RB), (unsigned)nf);
+  k_adj_rowgrad<N, C><<<grid, BLK, lds, st>>>(*d, *g, f0);
+  return check_launch("k_adj_rowgrad");
+}
+
+enum Op { OP_ROWPASS, OP_COLPASS, OP_ROWGRAD };
+
+template <int N>
+int dispatch_n(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
+  if (d->fft_f64) {
+    if (op == OP_ROWPASS) return launch_adj_rowpass_t<N, double2>(d, g, f0, nf, st);
+    if (op == OP_COLPASS) return launch_adj_colpass_t<N, double2>(d, g, f0, nf, st);
+    return launch_adj_rowgrad_t<N, double2>(d, g, f0, nf, st);
+  }
+  if (op == OP_ROWPASS) return launch_adj_rowpass_t<N, float2>(d, g, f0, nf, st);
+  if (op == OP_COLPASS) return launch_adj_colpass_t<N, float2>(d, g, f0, nf, st);
+  return launch_adj_rowgrad_t<N, float2>(d, g, f0, nf, st);
+}
+
+int dispatch(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
+  switch (d->N) {
+    case 256: return dispatch_n<256>(op, d, g, f0, nf, st);
+    case 512: return dispatch_n<512>(op, d, g, f0, nf, st);
+    case 1024: return dispatch_n<1024>(op, d, g, f0, nf, st);
+    case 2048: return dispatch_n<2048>(op, d, g, f0, nf, st);
+    default: return set_error(GX_ERR_UNSUPPORTED, "differentiable path: grid edge N=%lld is not 256/512/1024/2048",
+                              (long long)d->N);
+  }
+}
+
+int check_adj(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, bool backward) {
+  GX_REQUIRE(d != nullptr && g != nullptr, "NULL descriptor");
+  GX_REQUIRE(d->N == 256 || d->N == 512 || d->N == 1024 || d->N == 2048,
+             "differentiable path: grid edge N=%lld is not 256/512/1024/2048", (long long)d->N);
+  GX_REQUIRE(!d->fill_bkg && !d->smooth, "differentiable path: fill_bkg and smooth must be off");
+  GX_REQUIRE(d->bs_M == 0, "differentiable path: chirp-z tables given for a power-of-two grid");
+  GX_REQUIRE(d->r > 0.0, "r_voxel_size must be positive");
+  GX_REQUIRE(nf >= 0 && nf <= d->batch, "batch of %lld frames exceeds workspace batch %lld", (long long)nf,
+             (long long)d->batch);
+  GX_REQUIRE(f0 >= 0 && f0 + nf <= d->n_frames, "frames [%lld, %lld) outside [0, %lld)", (long long)f0,
+             (long long)(f0 + nf), (long long)d->n_frames);
+  GX_REQUIRE(d->W >= 1 && d->W <= d->N, "intermediate width W=%lld outside [1, N]", (long long)d->W);
+  GX_REQUIRE(d->Z >= 8 && d->Z <= d->N && d->Z % 8 == 0, "row range Z=%lld must be a multiple of 8 in [8, N]",
+             (long long)d->Z);
+  GX_REQUIRE(d->c_n >= 1, "empty cropped grid");
+  GX_REQUIRE(d->fparams && d->ymin_key && d->idx_bounds && d->inter && d->twiddle && d->status && d->row_off &&
+                 d->ax && d->ay && d->azrow,
+             "a required device buffer of gx_frames is NULL");
+  GX_REQUIRE(g->n_elem >= 1 && g->n_elem <= 128, "n_elem=%lld outside [1, 128]", (long long)g->n_elem);
+  GX_REQUIRE(g->f_table && g->afz && g->aperm && g->aelem && g->ecnt && g->row_exp && g->ainv,
+             "a required buffer of gx_adjoint is NULL");
+  if (backward)
+    GX_REQUIRE(g->g_bins && g->g_part && g->red && g->amin && g->g_atom, "a backward buffer of gx_adjoint is NULL");
+  return GX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gx_adj_prepare(const gx_frames *d, const gx_adjoint *g, gx_stream_t stream) {
+  clear_error();
+  int rc = check_adj(d, g, 0, 0, false);
+  if (rc) return rc;
+  GX_REQUIRE(g->g_atom, "g_atom is NULL");
+  hipStream_t st = as_stream(stream);
+  k_adj_rowexp<<<grid1d(d->N, BLK), BLK, 0, st>>>(*d, *g);
+  if (d->n_atoms > 0) {
+    unsigned gr = grid1d(d->n_atoms, BLK);
+    k_adj_init<<<gr > 4096 ? 4096 : gr, BLK, 0, st>>>(*d, *g);
+  }
+  return check_launch("gx_adj_prepare");
+}
+
+int gx_adj_rowpass(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream) {
+  clear_error();
+  int rc = check_adj(d, g, f0, nf, false);
+  if (rc || nf == 0) return rc;
+  return dispatch(OP_ROWPASS, d, g, f0, nf, as_stream(stream));
+}
+
+int gx_adj_forward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream) {
+  clear_error();
+  int rc = check_adj(d, g, f0, nf, false);
+  if (rc || nf == 0) return rc;
+  if ((rc = gx_frames_stats(d, f0, nf, stream))) return rc;
+  if ((rc = dispatch(OP_ROWPASS, d, g, f0, nf, as_stream(stream)))) return rc;
+  return gx_frames_colpass(d, f0, nf, stream);
+}
+
+int gx_adj_colpass(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream) {
+  clear_error();
+  int rc = check_adj(d, g, f0, nf, true);
+  if (rc || nf == 0) return rc;
+  return dispatch(OP_COLPASS, d, g, f0, nf, as_stream(stream));
+}
+
+int gx_adj_rowgrad(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream) {
+  clear_error();
+  int rc = check_adj(d, g, f0, nf, true);
+  if (rc || nf == 0) return rc;
+  return dispatch(OP_ROWGRAD, d, g, f0, nf, as_stream(stream));
+}
+
+int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream) {
+  clear_error();
+  int rc = check_adj(d, g, f0, nf, true);
+  if (rc || nf == 0) return rc;
+  hipStream_t st = as_stream(stream);
+  k_fill_i32<<<grid1d(nf, BLK), BLK, 0, st>>>(g->amin, nf, INT_MAX);
+  k_adj_frame_sums<<<dim3(RED, (unsigned)nf), BLK, 0, st>>>(*d, *g, f0);
+  if (d->n_atoms > 0) {
+    unsigned gr = grid1d(d->n_atoms, BLK);
+    k_adj_reduce<<<gr > 8192 ? 8192 : gr, BLK, 0, st>>>(*d, *g, f0, nf);
+  }
+  k_adj_frame_apply<<<1, BLK, 0, st>>>(*d, *g, f0, nf);
+  return check_launch("gx_adj_reduce");
+}
+
+int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream) {
+  clear_error();
+  int rc = check_adj(d, g, f0, nf, true);
+  if (rc || nf == 0) return rc;
+  hipStream_t st = as_stream(stream);
+  if ((rc = gx_frames_stats(d, f0, nf, stream))) return rc;
+  if ((rc = dispatch(OP_ROWPASS, d, g, f0, nf, st))) return rc;
+  if ((rc = dispatch(OP_COLPASS, d, g, f0, nf, st))) return rc;
+  if ((rc = dispatch(OP_ROWGRAD, d, g, f0, nf, st))) return rc;
+  return gx_adj_reduce(d, g, f0, nf, stream);
+}
+
+int gx_adj_finish(const gx_frames *d, const gx_adjoint *g, double *g_coords, double *g_f, gx_stream_t stream) {
+  clear_error();
+  int rc = check_adj(d, g, 0, 0, true);
+  if (rc) return rc;
+  GX_REQUIRE(g_coords && g_f, "NULL output");
+  GX_REQUIRE(g->z, "gx_adjoint.z (original z) is NULL");
+  GX_REQUIRE(g->n_elem + 1 <= BLK, "n_elem");
+  hipStream_t st = as_stream(stream);
+  const int64_t slot = d->batch;
+  k_fill_i32<<<1, BLK, 0, st>>>(g->amin + slot, 1, INT_MAX);
+  k_adj_fin_sums<<<dim3(RED, (unsigned)(1 + g->n_elem)), BLK, 0, st>>>(*d, *g, slot);
+  k_adj_fin_apply<<<1, BLK, 0, st>>>(*d, *g, slot, g_f);
+  if (d->n_atoms > 0) {
+    unsigned gr = grid1d(d->n_atoms, BLK);
+    k_adj_fin_out<<<gr > 4096 ? 4096 : gr, BLK, 0, st>>>(*d, *g, g_coords);
+  }
+  return check_launch("gx_adj_finish");
+}
+
+int gx_bins_adjoint(const unsigned long long *g_fx, const double *fx_scale, const int32_t *bin_cnt, int64_t c_n,
+                    int64_t c_lo, int64_t q_num, double qa_start, double qa_step, double qa_stop,
+                    const double *host_aff, double z_elem, double *g_bins, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(g_fx && fx_scale && bin_cnt && host_aff && g_bins, "NULL argument");
+  GX_REQUIRE(c_n >= 1 && c_lo >= 0 && c_lo + c_n <= q_num, "crop [%lld, %lld) outside the q axis of %lld",
+             (long long)c_lo, (long long)(c_lo + c_n), (long long)q_num);
+  AffArgs aff;
+  for (int i = 0; i < 9; ++i) aff.a[i] = host_aff[i];
+  const double c16pi2 = 16 * (M_PI * M_PI);
+  const int64_t n = c_n * c_n * c_n;
+  unsigned gr = grid1d(n, BLK);
+  k_bins_adjoint<<<gr > 8192 ? 8192 : gr, BLK, 0, as_stream(stream)>>>(g_fx, fx_scale, bin_cnt, c_n, c_lo, q_num,
+                                                                       qa_start, qa_step, qa_stop, aff, z_elem,
+                                                                       c16pi2, g_bins);
+  return check_launch("gx_bins_adjoint");
+}
+
+}  // extern "C"

@@ -50,31 +50,123 @@ __device__ __forceinline__ long long clip_idx(long long i, long long hi) { retur
 // loop runs in (psi outer, phi, theta inner) order with an FP64 accumulator,
 // so the sum is deterministic (the reference's `det_ints += det_int` from
 // worker threads is racy, tools/detector.py:298).
+// q point of pixel p after the initial rotations (make_detector, detector.py:24-27)
+__device__ __forceinline__ void det_pixel(const gx_detector &d, int64_t p, double &x0, double &y0, double &z0) {
+  const int64_t row = p / d.n_pix, col = p - row * d.n_pix;
+  x0 = 0.0;
+  y0 = d.h_axis[col];
+  z0 = d.v_axis[row];
+  for (int k = 0; k < d.n_init; ++k) rot3(d.init_R + 9 * k, x0, y0, z0);
+}
+
+// flat [cy][cx][cz] voxel of the pixel's q point under orientation o: the psi,
+// phi, theta rotations, then the edge-clipped floor indices of intersect_detector
+__device__ __forceinline__ int64_t det_voxel(const gx_detector &d, double x0, double y0, double z0, int64_t o,
+                                             double dqinv) {
+  const double *R = d.orient_R + 27 * o;
+  double x = x0, y = y0, z = z0;
+  rot3(R, x, y, z);        // rotate_about_normal(psi)
+  rot3(R + 9, x, y, z);    // rotate_about_vertical(phi)
+  rot3(R + 18, x, y, z);   // rotate_about_horizontal(theta)
+  const long long xi = clip_idx(to_int(floor_div(x - d.qx_min, d.dq, dqinv)), d.cx - 1);
+  const long long yi = clip_idx(to_int(floor_div(y - d.qy_min, d.dq, dqinv)), d.cy - 1);
+  const long long zi = clip_idx(to_int(floor_div(z - d.qz_min, d.dq, dqinv)), d.cz - 1);
+  return (yi * d.cx + xi) * d.cz + zi;
+}
+
 template <typename T>
 __global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
   const T *__restrict__ iq = reinterpret_cast<const T *>(d.iq);
-  const int64_t npx = d.n_pix * d.n_pix;
   const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (p >= npx) return;
-  const int64_t row = p / d.n_pix, col = p - row * d.n_pix;
-  double x0 = 0.0, y0 = d.h_axis[col], z0 = d.v_axis[row];  // make_detector, detector.py:24-27
-  for (int k = 0; k < d.n_init; ++k) rot3(d.init_R + 9 * k, x0, y0, z0);
+  if (p >= d.n_pix * d.n_pix) return;
+  double x0, y0, z0;
+  det_pixel(d, p, x0, y0, z0);
   const double dqinv = 1.0 / d.dq;
-  const long long cxm = d.cx - 1, cym = d.cy - 1, czm = d.cz - 1;
   double acc = 0.0;
   for (int64_t o = 0; o < d.n_orient; ++o) {
-    const double *R = d.orient_R + 27 * o;
-    double x = x0, y = y0, z = z0;
-    rot3(R, x, y, z);        // rotate_about_normal(psi)
-    rot3(R + 9, x, y, z);    // rotate_about_vertical(phi)
-    rot3(R + 18, x, y, z);   // rotate_about_horizontal(theta)
-    const long long xi = clip_idx(to_int(floor_div(x - d.qx_min, d.dq, dqinv)), cxm);
-    const long long yi = clip_idx(to_int(floor_div(y - d.qy_min, d.dq, dqinv)), cym);
-    const long long zi = clip_idx(to_int(floor_div(z - d.qz_min, d.dq, dqinv)), czm);
-    const double v = (double)iq[(yi * d.cx + xi) * d.cz + zi];
+    const double v = (double)iq[det_voxel(d, x0, y0, z0, o, dqinv)];
     acc = acc + v * d.orient_w[o];
   }
   d.det[p] = acc;
+}
+
+// ------------------------------------------------------------------------------------
+// adjoint of the detector stage
+// ------------------------------------------------------------------------------------
+// t = g_out * d(finalize)/dv elementwise: the floor replaces v (NaN or <= 0) by a
+// constant, else v * 1e-6; v is the forward's mirrored value, recomputed bitwise.
+__global__ __launch_bounds__(BLK) void k_det_fin_adj(const double *__restrict__ q, const double *__restrict__ g,
+                                                     int64_t nr, int64_t nc, int64_t flags, double *__restrict__ t) {
+  const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (p >= nr * nc) return;
+  if (!(flags & GX_FIN_FLOOR_SCALE)) {
+    t[p] = g[p];
+    return;
+  }
+  const int64_t r = p / nc, c = p - r * nc;
+  const int64_t rr = nr - 1 - r, cc = nc - 1 - c;
+  double v = q[p];
+  if (flags & GX_FIN_MIRROR) {
+    v = ((q[r * nc + c] + q[r * nc + cc]) + q[rr * nc + c]) + q[rr * nc + cc];
+    if ((nr & 1) && r == nr / 2) v = (q[r * nc + c] + q[r * nc + cc]) * 2.0;
+    if ((nc & 1) && c == nc / 2) v = (q[r * nc + c] + q[rr * nc + c]) * 2.0;
+    if ((nr & 1) && (nc & 1) && r == nr / 2 && c == nc / 2) v = q[r * nc + c] * 4.0;
+  }
+  t[p] = (v == v && v > 0.0) ? g[p] * 1e-6 : 0.0;
+}
+
+// One block: fx_scale = 2^floor(62 - log2(B)) with B = sum_p |g_p| * sum_o |w_o|,
+// a bound on every |dL/d iq| and on every partial sum of the scatter below.
+__global__ __launch_bounds__(BLK) void k_det_scale(const double *__restrict__ g, int64_t n, const double *w,
+                                                   int64_t n_orient, double *fx_scale) {
+  __shared__ double sh[2][BLK / GX_WAVE];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += BLK) a += fabs(g[i]);
+  for (int64_t o = threadIdx.x; o < n_orient; o += BLK) b += fabs(w[o]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, GX_WAVE);
+    b += __shfl_xor(b, off, GX_WAVE);
+  }
+  const int lane = threadIdx.x & (GX_WAVE - 1), wv = threadIdx.x / GX_WAVE;
+  if (lane == 0) {
+    sh[0][wv] = a;
+    sh[1][wv] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double A = 0.0, B = 0.0;
+    for (int i = 0; i < BLK / GX_WAVE; ++i) {
+      A += sh[0][i];
+      B += sh[1][i];
+    }
+    const double bound = (A * B) * (1.0 + 1e-9);
+    double sc = 1.0;
+    if (bound > 0.0 && bound < INFINITY) sc = ldexp(1.0, 62 - 1 - ilogb(bound));
+    fx_scale[0] = sc;
+  }
+}
+
+// dL/d iq: the transpose of the orientation gather, as exact fixed-point sums
+__global__ __launch_bounds__(BLK) void k_det_adj(gx_detector d, const double *__restrict__ g, const double *fx_scale,
+                                                 unsigned long long *g_fx) {
+  const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (p >= d.n_pix * d.n_pix) return;
+  const double gp = g[p];
+  if (gp == 0.0) return;
+  const double sc = fx_scale[0];
+  double x0, y0, z0;
+  det_pixel(d, p, x0, y0, z0);
+  const double dqinv = 1.0 / d.dq;
+  for (int64_t o = 0; o < d.n_orient; ++o) {
+    const int64_t v = det_voxel(d, x0, y0, z0, o, dqinv);
+    const long long q = llrint((gp * d.orient_w[o]) * sc);
+    if (q != 0) atomicAdd(&g_fx[v], (unsigned long long)q);
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_zero_u64(unsigned long long *p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) p[i] = 0;
 }
 
 // mirror_vertical_horizontal + NaN/<=0 floor + 1e-6 scale
@@ -176,13 +268,7 @@ __global__ __launch_bounds__(BLK) void k_apply_f0(const double *iq, int64_t ny, 
     const int64_t iz = i % nz, ix = (i / nz) % nx, iy = i / (nz * nx);
     const double a = qx[ix], b = qy[iy], c = qz[iz];
     const double q2 = (a * a + b * b) + c * c;
-    double f0 = aff.a[0] * exp(-aff.a[1] * q2 / c16pi2);
-    f0 = f0 + aff.a[2] * exp(-aff.a[3] * q2 / c16pi2);
-    f0 = f0 + aff.a[4] * exp(-aff.a[5] * q2 / c16pi2);
-    f0 = f0 + aff.a[6] * exp(-aff.a[7] * q2 / c16pi2);
-    f0 = f0 + aff.a[8];
-    const double fn = f0 / z_elem;
-    out[i] = iq[i] * (fn * fn);
+    out[i] = iq[i] * f0_weight(aff.a, q2, z_elem, c16pi2);
   }
 }
 
@@ -295,6 +381,35 @@ int gx_detector_accumulate(const gx_detector *d, gx_stream_t stream) {
   else
     k_detector<float><<<grid1d(npx, BLK), BLK, 0, as_stream(stream)>>>(*d);
   return check_launch("gx_detector_accumulate");
+}
+
+int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags, double *work,
+                        unsigned long long *g_fx, double *fx_scale, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(d != nullptr, "NULL detector");
+  GX_REQUIRE(d->n_pix >= 1 && d->n_pix <= 65536, "n_pix=%lld", (long long)d->n_pix);
+  GX_REQUIRE(d->n_init >= 0 && d->n_init <= 3, "n_init=%lld outside [0, 3]", (long long)d->n_init);
+  GX_REQUIRE(d->n_orient >= 0, "n_orient");
+  GX_REQUIRE(d->cy >= 1 && d->cx >= 1 && d->cz >= 1, "empty iq grid");
+  GX_REQUIRE(d->dq > 0.0, "dq must be positive");
+  GX_REQUIRE(d->h_axis && d->v_axis && d->det && (d->n_init == 0 || d->init_R) &&
+                 (d->n_orient == 0 || (d->orient_R && d->orient_w)),
+             "a detector buffer is NULL");
+  GX_REQUIRE(g_out && work && g_fx && fx_scale, "NULL adjoint buffer");
+  hipStream_t st = as_stream(stream);
+  const int64_t n = d->n_pix, npx = n * n;
+  double *t = work, *g_raw = work + npx;
+  k_det_fin_adj<<<grid1d(npx, BLK), BLK, 0, st>>>(d->det, g_out, n, n, flags, t);
+  if (flags & GX_FIN_MIRROR)  // the mirror sum is symmetric: its transpose is itself
+    k_detector_finalize<<<grid1d(npx, BLK), BLK, 0, st>>>(t, n, n, GX_FIN_MIRROR, g_raw);
+  else
+    g_raw = t;
+  const int64_t nv = d->cy * d->cx * d->cz;
+  unsigned gz = grid1d(nv, BLK);
+  k_zero_u64<<<gz > 4096 ? 4096 : gz, BLK, 0, st>>>(g_fx, nv);
+  k_det_scale<<<1, BLK, 0, st>>>(g_raw, npx, d->orient_w, d->n_orient, fx_scale);
+  k_det_adj<<<grid1d(npx, BLK), BLK, 0, st>>>(*d, g_raw, fx_scale, g_fx);
+  return check_launch("gx_detector_adjoint");
 }
 
 int gx_detector_finalize(const double *det, int64_t n_rows, int64_t n_cols, int64_t flags, double *out,

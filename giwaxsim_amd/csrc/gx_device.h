@@ -346,4 +346,21 @@ __device__ __forceinline__ double collapse_length(const double *fp, double x) {
   return 0.0;
 }
 
+// numpy linspace element i of n: i * step + start, the last one exactly stop.
+__device__ __forceinline__ double lin_axis(int64_t i, int64_t n, double start, double step, double stop) {
+  return (i == n - 1) ? stop : (double)i * step + start;
+}
+
+// (f0(|q|) / z_elem)^2 from the nine Cromer-Mann coefficients a1 b1 .. a4 b4 c
+// (add_f0_q_3d, tools/voxelgrids.py:828-857; f0 of tools/utilities.py:320-340).
+__device__ __forceinline__ double f0_weight(const double *aff, double q2, double z_elem, double c16pi2) {
+  double f0 = aff[0] * exp(-aff[1] * q2 / c16pi2);
+  f0 = f0 + aff[2] * exp(-aff[3] * q2 / c16pi2);
+  f0 = f0 + aff[4] * exp(-aff[5] * q2 / c16pi2);
+  f0 = f0 + aff[6] * exp(-aff[7] * q2 / c16pi2);
+  f0 = f0 + aff[8];
+  const double fn = f0 / z_elem;
+  return fn * fn;
+}
+
 }  // namespace gx

@@ -166,6 +166,16 @@ __global__ __launch_bounds__(BLK) void k_slab_scatter(gx_slab s) {
     }
     s.sq[2 * (int64_t)pos] = qr;
     s.sq[2 * (int64_t)pos + 1] = qi;
+    if (s.szf) {
+      double t = 0.0;
+      if (row < N) {
+        t = zz * rinv - (double)zi;
+        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+      }
+      s.szf[pos] = t;
+    }
+    if (s.sperm) s.sperm[pos] = (int32_t)a;
+    if (s.selem) s.selem[pos] = e;
   }
 }
 
@@ -988,10 +998,6 @@ struct AffArgs {
   double a[9];
 };
 
-__device__ __forceinline__ double lin_axis(int64_t i, int64_t n, double start, double step, double stop) {
-  return (i == n - 1) ? stop : (double)i * step + start;
-}
-
 __global__ __launch_bounds__(BLK) void k_bins_finalize(const unsigned long long *__restrict__ bsum,
                                                        const int32_t *__restrict__ bcnt, int64_t c_n,
                                                        double inv_scale, int64_t c_lo, int64_t q_num,
@@ -1008,13 +1014,7 @@ __global__ __launch_bounds__(BLK) void k_bins_finalize(const unsigned long long 
     const double qy = lin_axis(c_lo + iy, q_num, qa_start, qa_step, qa_stop);
     const double qz = lin_axis(c_lo + iz, q_num, qa_start, qa_step, qa_stop);
     const double q2 = (qx * qx + qy * qy) + qz * qz;
-    double f0 = aff.a[0] * exp(-aff.a[1] * q2 / c16pi2);
-    f0 = f0 + aff.a[2] * exp(-aff.a[3] * q2 / c16pi2);
-    f0 = f0 + aff.a[4] * exp(-aff.a[5] * q2 / c16pi2);
-    f0 = f0 + aff.a[6] * exp(-aff.a[7] * q2 / c16pi2);
-    f0 = f0 + aff.a[8];
-    const double fn = f0 / z_elem;
-    const double v = mean * (fn * fn);
+    const double v = mean * f0_weight(aff.a, q2, z_elem, c16pi2);
     if (iq64) iq64[i] = v;
     if (iq32) iq32[i] = (float)v;
   }

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define GX_ABI_VERSION 1
+#define GX_ABI_VERSION 2
 
 #define GX_OK 0
 #define GX_ERR_ARG 1         /* invalid argument (message says which) */
@@ -89,7 +89,12 @@ typedef struct gx_slab {
   int32_t *szrow;         /* [n_atoms] z row of each regrouped atom (N for rows >= N) */
   int32_t *row_off;       /* [N + 2] first atom of each row; row_off[N] = atoms with row < N */
   int32_t *row_exp;       /* [N] per-row fixed-point exponent */
-  int32_t *work;          /* [N * n_elem + 2 * (N + 1)] scratch counters */
+  int32_t *work;          /* [N * n_elem + 2 * (N + 1)] scratch counters; the first N * n_elem
+                             entries keep the per-row element counts after the call */
+  /* optional outputs (NULL = not written), used by the differentiable variant */
+  double *szf;            /* [n_atoms] fractional z offset (z - zmin) / r - row, in [0, 1] */
+  int32_t *sperm;         /* [n_atoms] original index of each regrouped atom */
+  int32_t *selem;         /* [n_atoms] element id of each regrouped atom */
 } gx_slab;
 
 /* ------------------------------------------------------------------------------------
@@ -278,6 +283,78 @@ int gx_bins_accumulate_f64(const unsigned long long *bin_sum, const int32_t *bin
 int gx_scale_offset_moments(const double *sim, const double *exp_, const uint8_t *mask, int64_t n,
                             double *work /* [GX_MOMENTS_WORK] */, double *host_out, gx_stream_t stream);
 #define GX_MOMENTS_WORK 774
+
+/* ------------------------------------------------------------------------------------
+ * Differentiable variant of the phi-frame path and its hand-written adjoint.
+ *
+ * The reference has no gradients; these entry points have no counterpart in it.
+ * Its nearest-grid-point deposit (np.add.at at (z // r, y' // r),
+ * tools/voxelgrids.py:328-341) is piecewise constant in the atom positions, so
+ * this variant deposits with cloud-in-cell weights instead: an atom at grid
+ * coordinates u = (y' - min y') / r, v = (z - min z) / r adds
+ *   f (1 - tv)(1 - tu), f (1 - tv) tu, f tv (1 - tu), f tv tu
+ * to nodes (v0, u0), (v0, u0 + 1), (v0 + 1, u0), (v0 + 1, u0 + 1) with
+ * u0 = u // 1, tu = u - u0 (likewise v).  Everything after the deposit (FFT,
+ * |F|^2, binning with counts, mean, f0, detector gather, mirror, floor) is the
+ * forward path's, and every buffer of gx_frames keeps its meaning.
+ * Requirements: power-of-two N, fill_bkg = 0, smooth = 0, every atom's
+ * u0 + 1 and v0 + 1 inside the grid, and d->Z > the largest v0 + 1.
+ * ---------------------------------------------------------------------------------- */
+#define GX_ADJ_RED_BLOCKS 64
+typedef struct gx_adjoint {
+  int64_t n_elem;
+  const double *f_table;   /* [n_elem][2] per-element f (re, im) */
+  const double *afz;       /* gx_slab.szf */
+  const int32_t *aperm;    /* gx_slab.sperm */
+  const int32_t *aelem;    /* gx_slab.selem */
+  const int32_t *ecnt;     /* gx_slab.work: [N][n_elem] per-row element counts */
+  const double *z;         /* [n_atoms] original z (for the argmin of z) */
+  double zmin;             /* gx_slab.zmin */
+  int32_t *row_exp;        /* [N] deposit exponents (written by gx_adj_prepare) */
+  int32_t *ainv;           /* [n_atoms] regrouped index of each original atom (gx_adj_prepare) */
+  /* backward */
+  const double *g_bins;    /* [c^3] dL / d(bin sum of I) (gx_bins_adjoint) */
+  double *g_part;          /* [batch][4][n_atoms] per-frame partials: dL/dy', dL/dz, dL/df re, im */
+  double *red;             /* [max(batch, 2 (1 + n_elem))][GX_ADJ_RED_BLOCKS] reduction scratch */
+  int32_t *amin;           /* [batch + 1] argmin atoms (original index) */
+  double *g_atom;          /* [5][n_atoms] accumulated dL/dx, dL/dy, dL/dz, dL/df re, im per
+                              regrouped atom (zeroed by gx_adj_prepare) */
+} gx_adjoint;
+
+/* Deposit exponents for the current f_table, inverse permutation, zero g_atom. */
+int gx_adj_prepare(const gx_frames *d, const gx_adjoint *g, gx_stream_t stream);
+/* Cloud-in-cell deposit + FFT along y of frames [f0, f0+nf) into d->inter (needs
+ * gx_frames_stats of the same frames first). */
+int gx_adj_rowpass(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
+/* stats + gx_adj_rowpass + gx_frames_colpass: the variant's forward. */
+int gx_adj_forward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
+/* Adjoint of the column pass, in place on d->inter: FFT along z of the recomputed
+ * column, cotangent 2 g_bins[bin] F on binned pixels, adjoint DFT back to z. */
+int gx_adj_colpass(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
+/* Adjoint DFT along y of every row, gathered at the atoms' four nodes into g_part. */
+int gx_adj_rowgrad(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
+/* Fold g_part of frames [f0, f0+nf) into g_atom (chain rule through
+ * y' = x sin phi + y cos phi and the per-frame origin min y'). */
+int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
+/* stats + rowpass (recomputed) + colpass adjoint + rowgrad + reduce. */
+int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
+/* Origin min z term, per-element f gradients and the coordinate gradients in the
+ * original atom order: g_coords [n_atoms][3], g_f [n_elem][2]. */
+int gx_adj_finish(const gx_frames *d, const gx_adjoint *g, double *g_coords, double *g_f, gx_stream_t stream);
+
+/* Adjoint of gx_bins_finalize for the FP64 iq: g_bins = g_iq * fn^2 / count
+ * (0 where count = 0), with g_iq = g_fx / fx_scale[0] (gx_detector_adjoint's
+ * fixed-point output).  Same axis / Cromer-Mann arguments as gx_bins_finalize. */
+int gx_bins_adjoint(const unsigned long long *g_fx, const double *fx_scale, const int32_t *bin_cnt,
+                    int64_t c_n, int64_t c_lo, int64_t q_num, double qa_start, double qa_step,
+                    double qa_stop, const double *host_aff, double z_elem, double *g_bins,
+                    gx_stream_t stream);
+/* Adjoint of gx_detector_accumulate + gx_detector_finalize(flags): g_out [n][n] is
+ * dL/d(finalized image); d->det must hold the accumulated image of the forward.
+ * Writes dL/d(iq) as fixed point g_fx [cy][cx][cz] with the power-of-two scale
+ * fx_scale[0] (sums are exact, hence deterministic).  work: [2 n^2] doubles. */
+int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags, double *work,
+                        unsigned long long *g_fx, double *fx_scale, gx_stream_t stream);
 
 #ifdef __cplusplus
 }

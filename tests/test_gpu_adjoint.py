@@ -1,0 +1,187 @@
+"""GPU parity of the differentiable variant and its hand-written adjoint.
+
+Checks, through the C ABI (giwaxsim_amd.adjoint):
+* forward image vs the FP64 autograd reference model (oracle/cic_model.py) on
+  the same inputs: max |diff| / max |image| <= 1e-10 with FP64 transforms,
+  <= 2e-5 with FP32;
+* gradients dL/d coords, dL/d f_re, dL/d f_im of L = <G, image> vs the
+  reference's torch.autograd: max |diff| / max |grad| <= 1e-8 (FP64),
+  <= 2e-3 (FP32);
+* size-independent properties: rigid translations leave the image unchanged
+  (sum of the coordinate gradient over atoms ~ 0), the image is homogeneous of
+  degree 2 in f (Euler: sum f dL/df = 2 L on unclipped pixels), and two
+  backward passes are bitwise identical;
+* the differentiation workload (graphite 40 x 40 x 30 A, N = 512, 453 frames,
+  256^2 detector, psi 0..90 step 2, target with c x 1.02, fitted scale and
+  background, taken 5% away from the least-squares optimum so that dL/ds and
+  dL/db are not zero): gradients vs FP64 central differences (h = 1e-4 A) of
+  the GPU forward on 64 random coordinates, and of f, scale and background.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from giwaxsim_amd.adjoint import DifferentiableSlab, fit_scale_background, image_loss
+from giwaxsim_amd.synth import graphite_slab
+from oracle.cic_model import CicModel
+
+pytestmark = pytest.mark.gpu
+
+R = 0.25
+MAXQ = 2.5
+
+
+def _small(precision):
+    coords, el = graphite_slab(size=(16.0, 16.0, 12.0), seed=21)
+    # a second element so per-element gradients are exercised
+    el = el.copy()
+    el[::7] = 'N'
+    N = 256
+    qv = 2 * np.pi / (R * N) * (1 + 1e-9)
+    det = dict(num_pixels=48, max_q=MAXQ, psis=list(np.arange(0.0, 91.0, 10.0)), phis=[0.0],
+               thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
+    frames = (0, 10)
+    model = DifferentiableSlab(el, coords, R, qv, MAXQ, det, frames=frames, precision=precision)
+    ids = np.array([model.elements.index(e) for e in el])
+    ref = CicModel(coords, ids, R, qv, MAXQ, det, frames=frames)
+    return coords, model, ref
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("precision,tol_img,tol_grad", [('fp64', 1e-10, 1e-8), ('fp32', 2e-5, 2e-3)])
+def test_forward_and_gradients_match_autograd(precision, tol_img, tol_grad):
+    _say(f"[gpu-adjoint] small graphite N=256 frames 0..10 precision={precision}")
+    coords, model, ref = _small(precision)
+    dev = torch.device('cuda')
+    G = np.random.default_rng(1).normal(size=(48, 48))
+    # HIP
+    c = torch.tensor(coords, dtype=torch.float64, device=dev, requires_grad=True)
+    fr = model.f_re0.clone().requires_grad_(True)
+    fi = model.f_im0.clone().requires_grad_(True)
+    img = model(c, fr, fi)
+    (img * torch.as_tensor(G, device=dev)).sum().backward()
+    # FP64 autograd reference
+    ct = torch.tensor(coords, dtype=torch.float64, requires_grad=True)
+    frt = model.f_re0.cpu().clone().requires_grad_(True)
+    fit = model.f_im0.cpu().clone().requires_grad_(True)
+    want = ref(ct, frt, fit)
+    (want * torch.as_tensor(G)).sum().backward()
+    w = want.detach().numpy()
+    e_img = np.max(np.abs(img.detach().cpu().numpy() - w)) / np.max(np.abs(w))
+    gc, gw = c.grad.cpu().numpy(), ct.grad.numpy()
+    e_c = np.max(np.abs(gc - gw)) / np.max(np.abs(gw))
+    e_fr = np.max(np.abs(fr.grad.cpu().numpy() - frt.grad.numpy())) / np.max(np.abs(frt.grad.numpy()))
+    e_fi = np.max(np.abs(fi.grad.cpu().numpy() - fit.grad.numpy())) / np.max(np.abs(fit.grad.numpy()))
+    _say(f"  image {e_img:.2e}  d/dcoords {e_c:.2e}  d/df_re {e_fr:.2e}  d/df_im {e_fi:.2e}")
+    assert e_img <= tol_img
+    assert e_c <= tol_grad and e_fr <= tol_grad and e_fi <= tol_grad
+
+
+def test_invariances_and_determinism():
+    _say("[gpu-adjoint] translation invariance, Euler identity in f, determinism")
+    coords, model, _ = _small('fp64')
+    dev = torch.device('cuda')
+    G = torch.as_tensor(np.random.default_rng(2).normal(size=(48, 48)), device=dev)
+    grads = []
+    for _ in range(2):
+        c = torch.tensor(coords, dtype=torch.float64, device=dev, requires_grad=True)
+        fr = model.f_re0.clone().requires_grad_(True)
+        fi = model.f_im0.clone().requires_grad_(True)
+        img = model(c, fr, fi)
+        L = (img * G).sum()
+        L.backward()
+        grads.append((c.grad.clone(), fr.grad.clone(), fi.grad.clone(), img.detach().clone(), L.item()))
+    assert all(torch.equal(a, b) for a, b in zip(grads[0][:4], grads[1][:4]))
+    gc, gfr, gfi, img, _ = grads[0]
+    # rigid translation: the deposit follows the atoms, the image does not change
+    scale = gc.abs().sum(0)
+    assert torch.all(gc.sum(0).abs() <= 1e-10 * scale), (gc.sum(0), scale)
+    # Euler: image(lambda f) = lambda^2 image(f) on unclipped pixels (clipped ones are 1e-12)
+    live = img != 1e-12
+    lhs = float((model.f_re0 * gfr).sum() + (model.f_im0 * gfi).sum())
+    rhs = 2.0 * float((img * G)[live].sum())
+    assert abs(lhs - rhs) <= 1e-10 * float((img * G).abs().sum())
+
+
+def _workload():
+    """Graphite 40 x 40 x 30 A (a = 2.46, c = 6.71, jitter 0.02 A), N = 512; the
+    target is the same lattice with c scaled by 1.02."""
+    coords, el = graphite_slab(size=(40.0, 40.0, 30.0), seed=4)
+    target, tel = graphite_slab(size=(40.0, 40.0, 30.0), seed=4, c=6.71 * 1.02)
+    N = 512
+    qv = 2 * np.pi / (R * N) * (1 + 1e-9)
+    det = dict(num_pixels=256, max_q=MAXQ, psis=list(np.arange(0.0, 91.0, 2.0)), phis=[0.0],
+               thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
+    return coords, el, target, tel, qv, det
+
+
+def test_workload_gradients_vs_central_differences():
+    coords, el, target, tel, qv, det = _workload()
+    _say(f"[gpu-adjoint] graphite 40x40x30 ({len(coords)} atoms) N=512 all frames, FD h=1e-4 on 64 coords")
+    dev = torch.device('cuda')
+    tmodel = DifferentiableSlab(tel, target, R, qv, MAXQ, det)
+    with torch.no_grad():
+        T = tmodel(torch.tensor(target, device=dev))
+    model = DifferentiableSlab(el, coords, R, qv, MAXQ, det)
+    with torch.no_grad():
+        img0 = model(torch.tensor(coords, device=dev))
+    s_fit, b_fit = fit_scale_background(img0, T)
+    # gradients are taken away from the least-squares optimum (where dL/ds = dL/db = 0)
+    s0, b0 = 1.05 * s_fit, b_fit + 0.05 * float(T.mean())
+    c = torch.tensor(coords, dtype=torch.float64, device=dev, requires_grad=True)
+    fr = model.f_re0.clone().requires_grad_(True)
+    fi = model.f_im0.clone().requires_grad_(True)
+    s = torch.tensor(s0, dtype=torch.float64, device=dev, requires_grad=True)
+    b = torch.tensor(b0, dtype=torch.float64, device=dev, requires_grad=True)
+    loss = image_loss(model(c, fr, fi), T, s, b)
+    loss.backward()
+    gc = c.grad.cpu().numpy()
+    h = 1e-4
+
+    def L(cc, a=None, bi=None, ss=s0, bb=b0):
+        with torch.no_grad():
+            img = model(torch.as_tensor(cc, device=dev), model.f_re0 if a is None else a,
+                        model.f_im0 if bi is None else bi)
+            return float(image_loss(img, T, ss, bb))
+
+    rng = np.random.default_rng(64)
+    picks = rng.choice(coords.size, size=64, replace=False)
+    errs = []
+    gmax = np.abs(gc).max()
+    for p in picks:
+        i, k = divmod(int(p), 3)
+        cp, cm = coords.copy(), coords.copy()
+        cp[i, k] += h
+        cm[i, k] -= h
+        fd = (L(cp) - L(cm)) / (2 * h)
+        errs.append(abs(fd - gc[i, k]) / gmax)
+    errs = np.array(errs)
+    _say(f"  coords: max|g| {gmax:.3e}; |fd - g| / max|g|: median {np.median(errs):.2e} "
+         f"95% {np.quantile(errs, 0.95):.2e} max {errs.max():.2e}")
+    # f, scale, background
+    ef = []
+    for e in range(len(model.elements)):
+        for which, grad in ((0, fr.grad), (1, fi.grad)):
+            d = torch.zeros_like(model.f_re0)
+            d[e] = h
+            if which == 0:
+                fd = (L(coords, a=model.f_re0 + d) - L(coords, a=model.f_re0 - d)) / (2 * h)
+            else:
+                fd = (L(coords, bi=model.f_im0 + d) - L(coords, bi=model.f_im0 - d)) / (2 * h)
+            ef.append(abs(fd - grad[e].item()) / max(abs(fd), 1e-300))
+    fds = (L(coords, ss=s0 + h * abs(s0)) - L(coords, ss=s0 - h * abs(s0))) / (2 * h * abs(s0))
+    fdb = (L(coords, bb=b0 + h) - L(coords, bb=b0 - h)) / (2 * h)
+    _say(f"  f: max rel err {max(ef):.2e}; scale grad {s.grad.item():.3e} (fd {fds:.3e}); "
+         f"background grad {b.grad.item():.3e} (fd {fdb:.3e})")
+    assert np.median(errs) <= 1e-5
+    assert np.quantile(errs, 0.95) <= 1e-3
+    assert max(ef) <= 1e-6
+    assert abs(fds - s.grad.item()) <= 1e-7 * abs(fds)
+    assert abs(fdb - b.grad.item()) <= 1e-7 * abs(fdb)

@@ -33,7 +33,7 @@ def test_exports_every_declared_symbol():
 
 def test_no_gpu_calls_needed_for_metadata():
     lib = _lib.load()
-    assert lib.gx_abi_version() == 1
+    assert lib.gx_abi_version() == _lib.GX_ABI_VERSION == 2
     assert [lib.gx_supported_grid(n) for n in (8, 128, 256, 1048, 2048, 2049, 4096)] == \
         [0, 1, 1, 1, 1, 0, 0]
 
@@ -47,6 +47,12 @@ def test_argument_errors_are_reported():
     d = _lib.GxFrames(N=1000, r=0.2)
     with pytest.raises(_lib.GxError, match='chirp-z length'):
         _lib.call('gx_frames_rowpass', ctypes.byref(d), 0, 1, None)
+    a = _lib.GxAdjoint()
+    with pytest.raises(_lib.GxError, match='not 256/512/1024/2048'):
+        _lib.call('gx_adj_rowpass', ctypes.byref(d), ctypes.byref(a), 0, 1, None)
+    d = _lib.GxFrames(N=512, r=0.2, fill_bkg=1)
+    with pytest.raises(_lib.GxError, match='fill_bkg and smooth must be off'):
+        _lib.call('gx_adj_backward', ctypes.byref(d), ctypes.byref(a), 0, 1, None)
     with pytest.raises(_lib.GxError, match='crop window'):
         _lib.call('gx_bins_finalize', ctypes.c_void_p(8), ctypes.c_void_p(8), 10, 1.0, 5, 12, 0.0,
                   0.0, 0.0, _lib.dbl_array([0] * 9), 6.0, None, None, None)
