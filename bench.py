@@ -215,11 +215,77 @@ def main():
     print(json.dumps(out))
 
 
+def adjoint_config(engine, torch, reps=10, precision='fp32'):
+    """Differentiation workload: graphite 40 x 40 x 30 A (5.6k atoms), r 0.25 A,
+    N = 512, 453 phi-frames, max_q 2.5, 256^2 detector with psi 0..90 step 2
+    (46 orientations) and a 0.15 deg psi tilt.  Forward = cloud-in-cell frames
+    + bins + iq + detector image; backward = the whole hand-written adjoint
+    (detector, bins, every frame recomputed, coordinate and f gradients).
+    Both timed from device-resident inputs with HIP events on the stream they
+    are enqueued on."""
+    from giwaxsim_amd.adjoint import DifferentiableSlab
+    from giwaxsim_amd.synth import graphite_slab
+    cg, eg = graphite_slab(size=(40.0, 40.0, 30.0), seed=4)
+    r, mq = 0.25, 2.5
+    qv = 2 * np.pi / (r * 512) * (1 + 1e-9)
+    det = dict(num_pixels=256, max_q=mq, psis=list(np.arange(0.0, 91.0, 2.0)), phis=[0.0],
+               thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
+    model = DifferentiableSlab(eg, cg, r, qv, mq, det, precision=precision)
+    ct = torch.as_tensor(cg).cuda()
+    fr, adj = model.descriptors(ct, model.f_re0, model.f_im0)
+    st = engine.stream_handle()
+    G = torch.randn(256 * 256, dtype=torch.float64, device='cuda')
+    model.forward_enqueue(fr, adj, st)
+    model.backward_enqueue(fr, adj, G, st)
+    torch.cuda.synchronize()
+    cs = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(cs)
+    for _ in range(reps):
+        model.forward_enqueue(fr, adj, st)
+    ev[1].record(cs)
+    for _ in range(reps):
+        model.forward_enqueue(fr, adj, st)
+        model.backward_enqueue(fr, adj, G, st)
+    ev[2].record(cs)
+    torch.cuda.synchronize()
+    fwd = ev[0].elapsed_time(ev[1]) / reps
+    fb = ev[1].elapsed_time(ev[2]) / reps
+    # the reference-parity (nearest-grid-point) forward of the same slab and detector
+    ids, ftab, _, _ = engine.element_table(eg, 12700.0)
+    idd = torch.as_tensor(ids).cuda()
+    dplan = engine.detector_plan(256, mq, (0.15,), ('psi',), np.arange(0.0, 91.0, 2.0),
+                                 np.full(46, 1.0 / 46), [0.0], [1.0], [0.0], [1.0])
+
+    def ngp():
+        res = engine.run_voxelgrid(ct, None, r, qv, mq, 12700.0, f_table=ftab, elem_ids=idd)
+        engine.run_detector(res['iq64'], res['qx'], res['qy'], res['qz'], dplan, True)
+        return res
+    res = ngp()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        ngp()
+    torch.cuda.synchronize()
+    ngp_ms = (time.perf_counter() - t) / reps * 1e3
+    F = model.plan.n_frames
+    return {"workload": "graphite 40x40x30 A, 5.6k atoms, N=512, 453 phi-frames, 256^2 detector, "
+                        f"psi 0..90 step 2; cloud-in-cell deposit, {precision.upper()} transforms; "
+                        "gradients of the image w.r.t. all coordinates and per-element f",
+            "precision": precision,
+            "n_atoms": len(cg), "frames": F, "fwd_ms": fwd, "fwd_bwd_ms": fb, "bwd_ms": fb - fwd,
+            "fwd_bwd_over_fwd": fb / fwd, "reference_parity_fwd_ms": ngp_ms,
+            "fwd_bwd_over_reference_parity_fwd": fb / ngp_ms,
+            "fwd_phi_frames_per_s": F / (fwd * 1e-3), "fwd_bwd_phi_frames_per_s": F / (fb * 1e-3)}
+
+
 def extra_configs(args, engine, torch):
-    """Ensemble of 64 uneven slabs into one shared grid (HIP-graph replay) and one
-    large slab (200 x 200 x 60 A at ~1.3M atoms, N=2048, 1307 frames)."""
+    """Ensemble of 64 uneven slabs into one shared grid (HIP-graph replay), one
+    large slab (200 x 200 x 60 A at ~1.3M atoms, N=2048, 1307 frames) and the
+    differentiation workload (forward + hand-written adjoint)."""
     from giwaxsim_amd.synth import ensemble_slabs, polymer_slab
-    out = {}
+    out = {"adjoint": adjoint_config(engine, torch, precision='fp32'),
+           "adjoint_fp64": adjoint_config(engine, torch, precision='fp64')}
     qv = 2 * np.pi / (0.2 * 1024) * (1 + 1e-9)
     slabs = ensemble_slabs(n_slabs=64, seed=0, q_voxel=qv)
     ens = engine.Ensemble(slabs, qv, 2.0, 12700.0, fill_bkg=True, smooth=10)

@@ -75,7 +75,7 @@ class GxDetector(ctypes.Structure):
     ]
 
 
-GX_ADJ_RED_BLOCKS = 64
+GX_ADJ_RED_BLOCKS = 16
 
 
 class GxAdjoint(ctypes.Structure):

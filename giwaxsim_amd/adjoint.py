@@ -96,13 +96,7 @@ class DifferentiableSlab:
         F = plan.n_frames
         self.frames = (0, F) if frames is None else (int(frames[0]), int(frames[1]))
         self.f64 = {'auto': True, 'fp64': True, 'fp32': False}[precision]
-        esz = 16 if self.f64 else 8
-        nf = self.frames[1] - self.frames[0]
-        if batch is None:
-            cap_inter = 160e6 // (W * N * esz)
-            cap_part = 512e6 // (32 * max(n, 1))
-            batch = max(1, min(nf, int(cap_inter), int(cap_part)))
-        self.batch = B = int(batch)
+        self._batch_arg = batch
         self._pixels_per_bin = pixels_per_bin(plan)
 
         # atoms and the regrouped slab
@@ -126,10 +120,11 @@ class DifferentiableSlab:
         self.fparams = torch.as_tensor(plan.fparams).to(dev).contiguous()
         self.tw = torch.as_tensor(twiddles(N, self.f64)).to(dev).contiguous()
         self.tw64 = torch.as_tensor(twiddles(N, True)).to(dev).contiguous()
-        self.ymin_key = _empty(3 * B, torch.int64)
-        self.idx_bounds = _empty(4 * B, torch.int32)
-        cdt = torch.float64 if self.f64 else torch.float32
-        self.inter = _empty(B * W * N * 2, cdt)
+        # per-frame statistics of every frame (the backward reuses the forward's)
+        self.ymin_key = _empty(3 * F, torch.int64)
+        self.idx_bounds = _empty(4 * F, torch.int32)
+        self._size_batch(self._rows_for(zb))
+        B = self.batch
         self.status = torch.zeros(4, dtype=torch.int32, device=dev)
         nb = g.c_n ** 3
         self.bin_sum = _empty(nb, torch.int64)
@@ -140,7 +135,6 @@ class DifferentiableSlab:
         self.adj_row_exp = _empty(N, torch.int32)
         self.ainv = _empty(n, torch.int32)
         self.g_bins = _empty(nb, torch.float64)
-        self.g_part = _empty(B * 4 * n, torch.float64)
         self.red = _empty(max(B, 2 * (1 + self.n_elem)) * _lib.GX_ADJ_RED_BLOCKS, torch.float64)
         self.amin = _empty(B + 1, torch.int32)
         self.g_atom = _empty(5 * n, torch.float64)
@@ -166,7 +160,7 @@ class DifferentiableSlab:
         self.det_ow = torch.as_tensor(dp.orient_w).to(dev).contiguous()
         self.det_raw = _empty(npx * npx, torch.float64)
         self.det_out = _empty(npx * npx, torch.float64)
-        self.det_work = _empty(2 * npx * npx, torch.float64)
+        self.det_work = _empty(2 * npx * npx + (npx * npx + 255) // 256, torch.float64)
         self.g_fx = _empty(nb, torch.int64)
         self.fx_scale = _empty(1, torch.float64)
         ax = g.crop_axis
@@ -181,6 +175,42 @@ class DifferentiableSlab:
         self._prepared = None
 
     # ----------------------------------------------------------------------------
+    def _rows_for(self, zb):
+        """Row range [0, Z) of a slab of z extent zb: rows v0 and v0 + 1 of every atom."""
+        N, r = self.plan.N, self.r
+        return int(min(N, max(8, -(-(int(np.floor_divide(zb, r)) + 2) // 8) * 8)))
+
+    def _size_batch(self, Z):
+        """(Re)size the per-batch buffers for row range Z (with 16 rows of slack)."""
+        N, W, n = self.plan.N, self.plan.W, self.n
+        Zc = int(min(N, Z + 16))
+        nf = self.frames[1] - self.frames[0]
+        esz = 16 if self.f64 else 8
+        B = self._batch_arg
+        if B is None:
+            cap_inter = 512e6 // (W * Zc * esz)
+            cap_part = 2e9 // (32 * max(n, 1))
+            B = max(1, min(nf, int(cap_inter), int(cap_part)))
+        if getattr(self, 'batch', None) not in (None, B):
+            raise RuntimeError("internal: batch size changed")
+        self.batch = int(B)
+        self.Z_cap = Zc
+        cdt = torch.float64 if self.f64 else torch.float32
+        self.inter = _empty(self.batch * W * Zc * 2, cdt)
+        self.g_part = _empty(self.batch * 4 * n, torch.float64)
+
+    def batches(self, fr):
+        """(f0, nf, descriptor) per batch: the descriptor's statistics pointers
+        address frame f0's slots of the all-frame arrays."""
+        lo, hi = self.frames
+        out = []
+        for f0 in range(lo, hi, self.batch):
+            d = GxFrames.from_buffer_copy(fr)
+            d.ymin_key = self.ymin_key.data_ptr() + 8 * 3 * f0
+            d.idx_bounds = self.idx_bounds.data_ptr() + 4 * 4 * f0
+            out.append((f0, min(self.batch, hi - f0), d))
+        return out
+
     def __call__(self, coords, f_re=None, f_im=None):
         f_re = self.f_re0 if f_re is None else f_re
         f_im = self.f_im0 if f_im is None else f_im
@@ -209,7 +239,10 @@ class DifferentiableSlab:
         if math.hypot(xb, yb) / r + 2.0 >= N or zb / r + 2.0 >= N:
             raise ValueError(f"slab extent ({xb:.3f}, {yb:.3f}, {zb:.3f}) A does not fit the "
                              f"{N} x {N} grid of {r} A voxels with a one-voxel margin")
-        Z = int(min(N, max(8, -(-(int(np.floor_divide(zb, r)) + 2) // 8) * 8)))
+        Z = self._rows_for(zb)
+        if Z > self.Z_cap:
+            self._batch_arg = self.batch
+            self._size_batch(Z)
         slab = GxSlab(n_atoms=self.n, n_elem=self.n_elem, N=N, r=r, zmin=zmin,
                       x=ptr(self.x).value, y=ptr(self.y).value, z=ptr(self.z).value,
                       elem=ptr(self.elem).value, f_table=ptr(self.ftab).value,
@@ -257,9 +290,8 @@ class DifferentiableSlab:
         call("gx_adj_prepare", ctypes.byref(fr), ctypes.byref(adj), st)
         g = self.plan.grid
         call("gx_bins_clear", ptr(self.bin_sum), ptr(self.bin_cnt), g.c_n ** 3, st)
-        lo, hi = self.frames
-        for f0 in range(lo, hi, self.batch):
-            call("gx_adj_forward", ctypes.byref(fr), ctypes.byref(adj), f0, min(self.batch, hi - f0), st)
+        for f0, nf, d in self.batches(fr):
+            call("gx_adj_forward", ctypes.byref(d), ctypes.byref(adj), f0, nf, st)
         self.finalize_image(st)
         self._check_status()
         self._prepared = (fr, adj)
@@ -297,9 +329,8 @@ class DifferentiableSlab:
         call("gx_bins_adjoint", ptr(self.g_fx), ptr(self.fx_scale), ptr(self.bin_cnt), g.c_n, g.c_lo,
              g.q_num, start, step, stop, aff, float(ptable['C']), ptr(self.g_bins), st)
         call("gx_adj_prepare", ctypes.byref(fr), ctypes.byref(adj), st)
-        lo, hi = self.frames
-        for f0 in range(lo, hi, self.batch):
-            call("gx_adj_backward", ctypes.byref(fr), ctypes.byref(adj), f0, min(self.batch, hi - f0), st)
+        for f0, nf, d in self.batches(fr):
+            call("gx_adj_backward", ctypes.byref(d), ctypes.byref(adj), f0, nf, st)
         call("gx_adj_finish", ctypes.byref(fr), ctypes.byref(adj), ptr(self.g_coords), ptr(self.g_f), st)
 
     def forward_enqueue(self, fr, adj, st):
@@ -307,9 +338,8 @@ class DifferentiableSlab:
         g = self.plan.grid
         call("gx_adj_prepare", ctypes.byref(fr), ctypes.byref(adj), st)
         call("gx_bins_clear", ptr(self.bin_sum), ptr(self.bin_cnt), g.c_n ** 3, st)
-        lo, hi = self.frames
-        for f0 in range(lo, hi, self.batch):
-            call("gx_adj_forward", ctypes.byref(fr), ctypes.byref(adj), f0, min(self.batch, hi - f0), st)
+        for f0, nf, d in self.batches(fr):
+            call("gx_adj_forward", ctypes.byref(d), ctypes.byref(adj), f0, nf, st)
         self.finalize_image(st)
 
 

@@ -12,7 +12,8 @@
 //                     bilinear in u, exact int64 fixed-point LDS sums, FFT along y,
 //                     kept column band staged and stored transposed (d.inter)
 //   k_colpass<N>      (forward path) FFT along z, |F|^2, bins + counts
-// Backward per batch (frames recomputed, never stored):
+// Backward per batch (frames recomputed, never stored; the forward's per-frame
+// min y' in d.ymin_key is reused):
 //   k_adj_rowpass<N>  as above
 //   k_adj_colpass<N>  per kept column: FFT along z, cotangent 2 g_bins[bin] F on the
 //                     binned pixels, adjoint DFT (conj FFT conj) back to z, in place
@@ -20,8 +21,8 @@
 //                     band, then every atom of the rows gathers its four nodes:
 //                     dL/du, dL/dv, dL/df per (frame, atom) into g_part
 //   k_adj_frame_sums  per frame: sum_a dL/dy'_a in original atom order, argmin y'
+//   k_adj_frame_origin origin term: the argmin atom of min y' takes -sum_a dL/dy'_a
 //   k_adj_reduce      g_atom += sum_f (sin, cos) dL/dy', dL/dz, dL/df over the batch
-//   k_adj_frame_apply origin term: the argmin atom of min y' takes -sum_a dL/dy'_a
 // Sums over atoms run in original atom order with fixed block partitions, and
 // the row deposit is exact, so every result is bitwise reproducible.
 #include "gx_device.h"
@@ -291,11 +292,12 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
   constexpr int H = RowGradShape<N, C>::H, RB = H - 1;
+  constexpr int HW = (H + NWAVE - 1) / NWAVE;  // rows per wave
   constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *rows = reinterpret_cast<C *>(smem_raw);  // [H][N] cotangent of the grid rows
-  C *stage = rows + H * N;                    // [W][H]
-  double *ftab = reinterpret_cast<double *>(stage + d.W * H);
+  C *stage = rows;                            // [W][H], consumed before rows is written
+  double *ftab = reinterpret_cast<double *>(rows + H * N);
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
   const int fb = blockIdx.y;
@@ -313,25 +315,34 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
     stage[idx] = (v < d.Z) ? src[(int64_t)w * d.Z + v] : C{(T)0, (T)0};
   }
   __syncthreads();
-  const C *tw = reinterpret_cast<const C *>(d.twiddle);
-  for (int q = wave; q < H; q += NWAVE) {
-    C x[P];
+  // this wave's rows q = wave + NWAVE h: the kept band, conjugated, into registers
+  C x[HW][P];
+#pragma unroll
+  for (int h = 0; h < HW; ++h) {
+    const int q = wave + NWAVE * h;
 #pragma unroll
     for (int t = 0; t < P / R0; ++t) {
 #pragma unroll
       for (int qq = 0; qq < R0; ++qq) {
         const int k = lane + GX_WAVE * t + qq * (N / R0);
         const int w = ((k + N / 2) & (N - 1)) - jlo;
-        x[t * R0 + qq] = (w >= 0 && w < wf) ? cconj(stage[w * H + q]) : C{(T)0, (T)0};
+        x[h][t * R0 + qq] = (q < H && w >= 0 && w < wf) ? cconj(stage[w * H + q]) : C{(T)0, (T)0};
       }
     }
+  }
+  __syncthreads();
+  const C *tw = reinterpret_cast<const C *>(d.twiddle);
+#pragma unroll
+  for (int h = 0; h < HW; ++h) {
+    const int q = wave + NWAVE * h;
+    if (q >= H) break;  // wave-uniform
     C *buf = rows + q * N;
-    fft_wave<N, C>(x, buf, tw, lane);
+    fft_wave<N, C>(x[h], buf, tw, lane);
     wave_sync();
 #pragma unroll
     for (int t = 0; t < P / RL; ++t)
 #pragma unroll
-      for (int qq = 0; qq < RL; ++qq) buf[lane + GX_WAVE * t + qq * (N / RL)] = cconj(x[t * RL + qq]);
+      for (int qq = 0; qq < RL; ++qq) buf[lane + GX_WAVE * t + qq * (N / RL)] = cconj(x[h][t * RL + qq]);
   }
   __syncthreads();
 
@@ -431,55 +442,54 @@ __global__ __launch_bounds__(BLK) void k_adj_frame_sums(gx_frames d, gx_adjoint 
   }
 }
 
-// g_atom += over the batch's frames in order: (sin, cos) dL/dy', dL/dz, dL/df
+// One thread per frame: S_f = sum of the frame's RED partials (in block order);
+// the argmin atom of min y' carries the origin term, d(min y')/dy'_a = [a = argmin],
+// so its dL/dy' in this frame's row of g_part becomes dL/dy' - S_f.  Frames own
+// distinct rows, so the frames run in parallel.
+__global__ __launch_bounds__(BLK) void k_adj_frame_origin(gx_frames d, gx_adjoint g, int64_t nf) {
+  const int64_t fb = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (fb >= nf) return;
+  double S = 0.0;
+  for (int b = 0; b < RED; ++b) S += g.red[fb * RED + b];
+  const int am = g.amin[fb];
+  if (am == INT_MAX) return;
+  g.g_part[fb * 4 * d.n_atoms + g.ainv[am]] -= S;
+}
+
+// g_atom += sum over the batch's frames of (sin, cos) dL/dy', dL/dz, dL/df.
+// Block = 16 atoms x 16 frame lanes; lane j sums frames j, j + 16, ... in order,
+// then the 16 lane sums are added in a fixed tree order.
+constexpr int RA = 16, RF = BLK / RA;
 __global__ __launch_bounds__(BLK) void k_adj_reduce(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf) {
+  __shared__ double sh[5][RF][RA + 1];
   const int64_t n = d.n_atoms, n_rowed = d.row_off[d.N];
-  for (int64_t a = blockIdx.x * (int64_t)BLK + threadIdx.x; a < n_rowed; a += (int64_t)gridDim.x * BLK) {
-    double gx = 0.0, gy = 0.0, gz = 0.0, gr = 0.0, gi = 0.0;
-    for (int64_t fb = 0; fb < nf; ++fb) {
+  const int al = threadIdx.x % RA, j = threadIdx.x / RA;
+  const int64_t a = blockIdx.x * (int64_t)RA + al;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (a < n_rowed) {
+    for (int64_t fb = j; fb < nf; fb += RF) {
       const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
       const double *gp = g.g_part + fb * 4 * n;
       const double u = gp[a];
-      gx += fp[GX_FP_SIN] * u;
-      gy += fp[GX_FP_COS] * u;
-      gz += gp[n + a];
-      gr += gp[2 * n + a];
-      gi += gp[3 * n + a];
+      acc[0] += fp[GX_FP_SIN] * u;
+      acc[1] += fp[GX_FP_COS] * u;
+      acc[2] += gp[n + a];
+      acc[3] += gp[2 * n + a];
+      acc[4] += gp[3 * n + a];
     }
-    g.g_atom[a] += gx;
-    g.g_atom[n + a] += gy;
-    g.g_atom[2 * n + a] += gz;
-    g.g_atom[3 * n + a] += gr;
-    g.g_atom[4 * n + a] += gi;
   }
-}
-
-// One block: S_f = sum of the frame's partials (in block order), then the origin
-// term d(min y')/d(argmin atom) = 1 applied frame by frame by one thread.
-__global__ __launch_bounds__(BLK) void k_adj_frame_apply(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf) {
-  __shared__ double S[BLK];
-  const int64_t n = d.n_atoms;
-  for (int64_t base = 0; base < nf; base += BLK) {
-    const int64_t fb = base + threadIdx.x;
-    if (fb < nf) {
-      double acc = 0.0;
-      for (int b = 0; b < RED; ++b) acc += g.red[fb * RED + b];
-      S[threadIdx.x] = acc;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int64_t k = 0; k < BLK && base + k < nf; ++k) {
-        const int64_t f = base + k;
-        const int am = g.amin[f];
-        if (am == INT_MAX) continue;
-        const double *fp = d.fparams + (f0 + f) * GX_FP_STRIDE;
-        const int a = g.ainv[am];
-        g.g_atom[a] -= fp[GX_FP_SIN] * S[k];
-        g.g_atom[n + a] -= fp[GX_FP_COS] * S[k];
-      }
-    }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) sh[k][j][al] = acc[k];
+  __syncthreads();
+  for (int w = RF / 2; w > 0; w >>= 1) {
+    if (j < w)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) sh[k][j][al] += sh[k][j + w][al];
     __syncthreads();
   }
+  if (j == 0 && a < n_rowed)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) g.g_atom[k * n + a] += sh[k][0][al];
 }
 
 // grid (RED, 1 + n_elem): y = 0 sums dL/dz over atoms (original order) and finds
@@ -602,7 +612,7 @@ int launch_adj_colpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, in
 template <int N, typename C>
 int launch_adj_rowgrad_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int H = RowGradShape<N, C>::H, RB = H - 1;
-  const size_t lds = ((size_t)H * N + (size_t)d->W * H) * sizeof(C) + (size_t)g->n_elem * 16;
+  const size_t lds = (size_t)H * N * sizeof(C) + (size_t)g->n_elem * 16;  // staging aliases the rows (W <= N)
   if (lds > 160 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "adjoint row pass needs %zu bytes of LDS (N=%d, W=%lld)", lds, N,
                      (long long)d->W);
@@ -723,11 +733,8 @@ int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t n
   hipStream_t st = as_stream(stream);
   k_fill_i32<<<grid1d(nf, BLK), BLK, 0, st>>>(g->amin, nf, INT_MAX);
   k_adj_frame_sums<<<dim3(RED, (unsigned)nf), BLK, 0, st>>>(*d, *g, f0);
-  if (d->n_atoms > 0) {
-    unsigned gr = grid1d(d->n_atoms, BLK);
-    k_adj_reduce<<<gr > 8192 ? 8192 : gr, BLK, 0, st>>>(*d, *g, f0, nf);
-  }
-  k_adj_frame_apply<<<1, BLK, 0, st>>>(*d, *g, f0, nf);
+  k_adj_frame_origin<<<grid1d(nf, BLK), BLK, 0, st>>>(*d, *g, nf);
+  if (d->n_atoms > 0) k_adj_reduce<<<grid1d(d->n_atoms, RA), BLK, 0, st>>>(*d, *g, f0, nf);
   return check_launch("gx_adj_reduce");
 }
 
@@ -736,7 +743,6 @@ int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t
   int rc = check_adj(d, g, f0, nf, true);
   if (rc || nf == 0) return rc;
   hipStream_t st = as_stream(stream);
-  if ((rc = gx_frames_stats(d, f0, nf, stream))) return rc;
   if ((rc = dispatch(OP_ROWPASS, d, g, f0, nf, st))) return rc;
   if ((rc = dispatch(OP_COLPASS, d, g, f0, nf, st))) return rc;
   if ((rc = dispatch(OP_ROWGRAD, d, g, f0, nf, st))) return rc;

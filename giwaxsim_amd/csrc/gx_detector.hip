@@ -95,33 +95,50 @@ __global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
 // ------------------------------------------------------------------------------------
 // t = g_out * d(finalize)/dv elementwise: the floor replaces v (NaN or <= 0) by a
 // constant, else v * 1e-6; v is the forward's mirrored value, recomputed bitwise.
+// Each block also writes sum |t| over its pixels (fixed order) to part[blockIdx.x].
 __global__ __launch_bounds__(BLK) void k_det_fin_adj(const double *__restrict__ q, const double *__restrict__ g,
-                                                     int64_t nr, int64_t nc, int64_t flags, double *__restrict__ t) {
+                                                     int64_t nr, int64_t nc, int64_t flags, double *__restrict__ t,
+                                                     double *__restrict__ part) {
+  __shared__ double sh[BLK / GX_WAVE];
   const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (p >= nr * nc) return;
-  if (!(flags & GX_FIN_FLOOR_SCALE)) {
-    t[p] = g[p];
-    return;
+  double tv = 0.0;
+  if (p < nr * nc) {
+    if (!(flags & GX_FIN_FLOOR_SCALE)) {
+      tv = g[p];
+    } else {
+      const int64_t r = p / nc, c = p - r * nc;
+      const int64_t rr = nr - 1 - r, cc = nc - 1 - c;
+      double v = q[p];
+      if (flags & GX_FIN_MIRROR) {
+        v = ((q[r * nc + c] + q[r * nc + cc]) + q[rr * nc + c]) + q[rr * nc + cc];
+        if ((nr & 1) && r == nr / 2) v = (q[r * nc + c] + q[r * nc + cc]) * 2.0;
+        if ((nc & 1) && c == nc / 2) v = (q[r * nc + c] + q[rr * nc + c]) * 2.0;
+        if ((nr & 1) && (nc & 1) && r == nr / 2 && c == nc / 2) v = q[r * nc + c] * 4.0;
+      }
+      tv = (v == v && v > 0.0) ? g[p] * 1e-6 : 0.0;
+    }
+    t[p] = tv;
   }
-  const int64_t r = p / nc, c = p - r * nc;
-  const int64_t rr = nr - 1 - r, cc = nc - 1 - c;
-  double v = q[p];
-  if (flags & GX_FIN_MIRROR) {
-    v = ((q[r * nc + c] + q[r * nc + cc]) + q[rr * nc + c]) + q[rr * nc + cc];
-    if ((nr & 1) && r == nr / 2) v = (q[r * nc + c] + q[r * nc + cc]) * 2.0;
-    if ((nc & 1) && c == nc / 2) v = (q[r * nc + c] + q[rr * nc + c]) * 2.0;
-    if ((nr & 1) && (nc & 1) && r == nr / 2 && c == nc / 2) v = q[r * nc + c] * 4.0;
+  double a = fabs(tv);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, GX_WAVE);
+  if ((threadIdx.x & (GX_WAVE - 1)) == 0) sh[threadIdx.x / GX_WAVE] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < BLK / GX_WAVE; ++i) s += sh[i];
+    part[blockIdx.x] = s;
   }
-  t[p] = (v == v && v > 0.0) ? g[p] * 1e-6 : 0.0;
 }
 
-// One block: fx_scale = 2^floor(62 - log2(B)) with B = sum_p |g_p| * sum_o |w_o|,
-// a bound on every |dL/d iq| and on every partial sum of the scatter below.
-__global__ __launch_bounds__(BLK) void k_det_scale(const double *__restrict__ g, int64_t n, const double *w,
-                                                   int64_t n_orient, double *fx_scale) {
+// One block: fx_scale = 2^(61 - floor(log2 B)) with B = mult * sum(part) * sum_o |w_o|,
+// a bound on every |dL/d iq| and on every partial sum of the scatter below (mult
+// = 4 covers the mirror, whose outputs add at most four inputs).
+__global__ __launch_bounds__(BLK) void k_det_scale(const double *__restrict__ part, int64_t n_part, double mult,
+                                                   const double *w, int64_t n_orient, double *fx_scale) {
   __shared__ double sh[2][BLK / GX_WAVE];
   double a = 0.0, b = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += BLK) a += fabs(g[i]);
+  for (int64_t i = threadIdx.x; i < n_part; i += BLK) a += part[i];
   for (int64_t o = threadIdx.x; o < n_orient; o += BLK) b += fabs(w[o]);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -140,28 +157,50 @@ __global__ __launch_bounds__(BLK) void k_det_scale(const double *__restrict__ g,
       A += sh[0][i];
       B += sh[1][i];
     }
-    const double bound = (A * B) * (1.0 + 1e-9);
+    const double bound = ((mult * A) * B) * (1.0 + 1e-9);
     double sc = 1.0;
     if (bound > 0.0 && bound < INFINITY) sc = ldexp(1.0, 62 - 1 - ilogb(bound));
     fx_scale[0] = sc;
   }
 }
 
-// dL/d iq: the transpose of the orientation gather, as exact fixed-point sums
+// dL/d iq: the transpose of the orientation gather, as exact fixed-point sums.
+// One lane per pixel; neighbouring pixels of a row mostly land in the same voxel,
+// so each wave first sums runs of equal voxel index (segmented scan over lanes)
+// and only the last lane of a run issues the atomic.  Integer sums are exact, so
+// the result does not depend on the grouping.
 __global__ __launch_bounds__(BLK) void k_det_adj(gx_detector d, const double *__restrict__ g, const double *fx_scale,
                                                  unsigned long long *g_fx) {
+  const int lane = threadIdx.x & (GX_WAVE - 1);
   const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (p >= d.n_pix * d.n_pix) return;
-  const double gp = g[p];
-  if (gp == 0.0) return;
+  const bool live = p < d.n_pix * d.n_pix;
+  const double gp = live ? g[p] : 0.0;
+  if (__all(gp == 0.0)) return;  // wave-uniform
   const double sc = fx_scale[0];
-  double x0, y0, z0;
-  det_pixel(d, p, x0, y0, z0);
+  double x0 = 0.0, y0 = 0.0, z0 = 0.0;
+  if (live) det_pixel(d, p, x0, y0, z0);
   const double dqinv = 1.0 / d.dq;
   for (int64_t o = 0; o < d.n_orient; ++o) {
-    const int64_t v = det_voxel(d, x0, y0, z0, o, dqinv);
-    const long long q = llrint((gp * d.orient_w[o]) * sc);
-    if (q != 0) atomicAdd(&g_fx[v], (unsigned long long)q);
+    int key = -1 - lane;  // unique: never merges with a neighbour
+    long long val = 0;
+    if (live) {
+      key = (int)det_voxel(d, x0, y0, z0, o, dqinv);
+      val = llrint((gp * d.orient_w[o]) * sc);
+    }
+    const int up = __shfl_up(key, 1, GX_WAVE);
+    int f = (lane == 0 || up != key) ? 1 : 0;
+#pragma unroll
+    for (int s = 1; s < GX_WAVE; s <<= 1) {
+      const long long vv = __shfl_up(val, s, GX_WAVE);
+      const int ff = __shfl_up(f, s, GX_WAVE);
+      if (lane >= s) {
+        if (!f) val += vv;
+        f |= ff;
+      }
+    }
+    const int dn = __shfl_down(key, 1, GX_WAVE);
+    const bool tail = (lane == GX_WAVE - 1) || dn != key;
+    if (tail && key >= 0 && val != 0) atomicAdd(&g_fx[key], (unsigned long long)val);
   }
 }
 
@@ -398,17 +437,20 @@ int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags
   GX_REQUIRE(g_out && work && g_fx && fx_scale, "NULL adjoint buffer");
   hipStream_t st = as_stream(stream);
   const int64_t n = d->n_pix, npx = n * n;
-  double *t = work, *g_raw = work + npx;
-  k_det_fin_adj<<<grid1d(npx, BLK), BLK, 0, st>>>(d->det, g_out, n, n, flags, t);
+  const unsigned nblk = grid1d(npx, BLK);
+  double *t = work, *g_raw = work + npx, *part = work + 2 * npx;
+  k_det_fin_adj<<<nblk, BLK, 0, st>>>(d->det, g_out, n, n, flags, t, part);
   if (flags & GX_FIN_MIRROR)  // the mirror sum is symmetric: its transpose is itself
-    k_detector_finalize<<<grid1d(npx, BLK), BLK, 0, st>>>(t, n, n, GX_FIN_MIRROR, g_raw);
+    k_detector_finalize<<<nblk, BLK, 0, st>>>(t, n, n, GX_FIN_MIRROR, g_raw);
   else
     g_raw = t;
   const int64_t nv = d->cy * d->cx * d->cz;
+  GX_REQUIRE(nv < 2147483647LL, "iq grid of %lld voxels exceeds int32 indexing", (long long)nv);
   unsigned gz = grid1d(nv, BLK);
   k_zero_u64<<<gz > 4096 ? 4096 : gz, BLK, 0, st>>>(g_fx, nv);
-  k_det_scale<<<1, BLK, 0, st>>>(g_raw, npx, d->orient_w, d->n_orient, fx_scale);
-  k_det_adj<<<grid1d(npx, BLK), BLK, 0, st>>>(*d, g_raw, fx_scale, g_fx);
+  k_det_scale<<<1, BLK, 0, st>>>(part, nblk, (flags & GX_FIN_MIRROR) ? 4.0 : 1.0, d->orient_w, d->n_orient,
+                                 fx_scale);
+  k_det_adj<<<nblk, BLK, 0, st>>>(*d, g_raw, fx_scale, g_fx);
   return check_launch("gx_detector_adjoint");
 }
 

@@ -300,7 +300,7 @@ int gx_scale_offset_moments(const double *sim, const double *exp_, const uint8_t
  * Requirements: power-of-two N, fill_bkg = 0, smooth = 0, every atom's
  * u0 + 1 and v0 + 1 inside the grid, and d->Z > the largest v0 + 1.
  * ---------------------------------------------------------------------------------- */
-#define GX_ADJ_RED_BLOCKS 64
+#define GX_ADJ_RED_BLOCKS 16
 typedef struct gx_adjoint {
   int64_t n_elem;
   const double *f_table;   /* [n_elem][2] per-element f (re, im) */
@@ -336,7 +336,8 @@ int gx_adj_rowgrad(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t 
 /* Fold g_part of frames [f0, f0+nf) into g_atom (chain rule through
  * y' = x sin phi + y cos phi and the per-frame origin min y'). */
 int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
-/* stats + rowpass (recomputed) + colpass adjoint + rowgrad + reduce. */
+/* rowpass (recomputed) + colpass adjoint + rowgrad + reduce.  d->ymin_key must hold
+ * the statistics of the same frames and atoms (gx_frames_stats, run by gx_adj_forward). */
 int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
 /* Origin min z term, per-element f gradients and the coordinate gradients in the
  * original atom order: g_coords [n_atoms][3], g_f [n_elem][2]. */
@@ -352,7 +353,8 @@ int gx_bins_adjoint(const unsigned long long *g_fx, const double *fx_scale, cons
 /* Adjoint of gx_detector_accumulate + gx_detector_finalize(flags): g_out [n][n] is
  * dL/d(finalized image); d->det must hold the accumulated image of the forward.
  * Writes dL/d(iq) as fixed point g_fx [cy][cx][cz] with the power-of-two scale
- * fx_scale[0] (sums are exact, hence deterministic).  work: [2 n^2] doubles. */
+ * fx_scale[0] (sums are exact, hence deterministic).
+ * work: [2 n^2 + ceil(n^2 / 256)] doubles. */
 int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags, double *work,
                         unsigned long long *g_fx, double *fx_scale, gx_stream_t stream);
 

@@ -2,11 +2,11 @@
 
 Checks, through the C ABI (giwaxsim_amd.adjoint):
 * forward image vs the FP64 autograd reference model (oracle/cic_model.py) on
-  the same inputs: max |diff| / max |image| <= 1e-10 with FP64 transforms,
-  <= 2e-5 with FP32;
+  the same inputs: max |diff| / max |image| <= 1e-12 with FP64 transforms,
+  <= 2e-6 with FP32;
 * gradients dL/d coords, dL/d f_re, dL/d f_im of L = <G, image> vs the
-  reference's torch.autograd: max |diff| / max |grad| <= 1e-8 (FP64),
-  <= 2e-3 (FP32);
+  reference's torch.autograd: max |diff| / max |grad| <= 1e-11 (FP64),
+  <= 1e-4 (FP32);
 * size-independent properties: rigid translations leave the image unchanged
   (sum of the coordinate gradient over atoms ~ 0), the image is homogeneous of
   degree 2 in f (Euler: sum f dL/df = 2 L on unclipped pixels), and two
@@ -55,7 +55,7 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("precision,tol_img,tol_grad", [('fp64', 1e-10, 1e-8), ('fp32', 2e-5, 2e-3)])
+@pytest.mark.parametrize("precision,tol_img,tol_grad", [('fp64', 1e-12, 1e-11), ('fp32', 2e-6, 1e-4)])
 def test_forward_and_gradients_match_autograd(precision, tol_img, tol_grad):
     _say(f"[gpu-adjoint] small graphite N=256 frames 0..10 precision={precision}")
     coords, model, ref = _small(precision)
@@ -180,8 +180,11 @@ def test_workload_gradients_vs_central_differences():
     fdb = (L(coords, bb=b0 + h) - L(coords, bb=b0 - h)) / (2 * h)
     _say(f"  f: max rel err {max(ef):.2e}; scale grad {s.grad.item():.3e} (fd {fds:.3e}); "
          f"background grad {b.grad.item():.3e} (fd {fdb:.3e})")
-    assert np.median(errs) <= 1e-5
-    assert np.quantile(errs, 0.95) <= 1e-3
-    assert max(ef) <= 1e-6
+    # the loss is piecewise quartic in one coordinate; a cell crossing inside
+    # [x - h, x + h] in one of the 453 frames costs the difference its exactness
+    assert np.median(errs) <= 1e-7
+    assert np.quantile(errs, 0.95) <= 1e-5
+    assert errs.max() <= 1e-4
+    assert max(ef) <= 1e-7
     assert abs(fds - s.grad.item()) <= 1e-7 * abs(fds)
     assert abs(fdb - b.grad.item()) <= 1e-7 * abs(fdb)
