@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--kernel-iters", type=int, default=20)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble (64 slabs) and large-slab (N=2048) figures")
+    ap.add_argument("--only", choices=["adjoint", "adjoint_fp64", "ensemble", "large_slab"],
+                    help="run one of the extra workloads alone and print its JSON (for profiling)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="one step, then one 256 MiB device copy (counter calibration) and one "
                          "launch of each frame kernel; for rocprofv3 --pmc passes")
@@ -77,6 +79,14 @@ def main():
     import torch
     from giwaxsim_amd import engine, _lib
     from giwaxsim_amd._lib import call
+
+    if args.only:
+        if args.only.startswith("adjoint"):
+            prec = "fp64" if args.only == "adjoint_fp64" else "fp32"
+            print(json.dumps(adjoint_config(engine, torch, reps=args.steps, precision=prec)))
+        else:
+            print(json.dumps(extra_configs(args, engine, torch, which=args.only)[args.only]))
+        return
 
     W = workload()
     dev = engine.device()
@@ -279,13 +289,25 @@ def adjoint_config(engine, torch, reps=10, precision='fp32'):
             "fwd_phi_frames_per_s": F / (fwd * 1e-3), "fwd_bwd_phi_frames_per_s": F / (fb * 1e-3)}
 
 
-def extra_configs(args, engine, torch):
+def extra_configs(args, engine, torch, which=None):
     """Ensemble of 64 uneven slabs into one shared grid (HIP-graph replay), one
     large slab (200 x 200 x 60 A at ~1.3M atoms, N=2048, 1307 frames) and the
-    differentiation workload (forward + hand-written adjoint)."""
-    from giwaxsim_amd.synth import ensemble_slabs, polymer_slab
-    out = {"adjoint": adjoint_config(engine, torch, precision='fp32'),
-           "adjoint_fp64": adjoint_config(engine, torch, precision='fp64')}
+    differentiation workload (forward + hand-written adjoint).  ``which`` runs
+    just one of the last two."""
+    out = {}
+    if which is None:
+        out["adjoint"] = adjoint_config(engine, torch, precision='fp32')
+        out["adjoint_fp64"] = adjoint_config(engine, torch, precision='fp64')
+    reps = 3
+    if which in (None, "ensemble"):
+        out["ensemble"] = ensemble_config(engine, torch, reps)
+    if which in (None, "large_slab"):
+        out["large_slab"] = large_slab_config(engine, torch, reps)
+    return out
+
+
+def ensemble_config(engine, torch, reps):
+    from giwaxsim_amd.synth import ensemble_slabs
     qv = 2 * np.pi / (0.2 * 1024) * (1 + 1e-9)
     slabs = ensemble_slabs(n_slabs=64, seed=0, q_voxel=qv)
     ens = engine.Ensemble(slabs, qv, 2.0, 12700.0, fill_bkg=True, smooth=10)
@@ -293,16 +315,18 @@ def extra_configs(args, engine, torch):
     ens.replay()
     torch.cuda.synchronize()
     t = time.perf_counter()
-    reps = 3
     for _ in range(reps):
         ens.replay()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t) / reps
-    out["ensemble"] = {"workload": "64 polymer slabs, edges U[40,160] A, N in {256,512,1024,2048}, "
-                                   "one shared 185^3 grid (131^3 crop), fill_bkg, smooth 10",
-                       "n_atoms": int(sum(len(c) for c, _, _ in slabs)), "frames": ens.n_frames,
-                       "ms_per_pass": dt * 1e3, "phi_frames_per_s": ens.n_frames / dt}
-    del ens
+    return {"workload": "64 polymer slabs, edges U[40,160] A, N in {256,512,1024,2048}, "
+                        "one shared 185^3 grid (131^3 crop), fill_bkg, smooth 10",
+            "n_atoms": int(sum(len(c) for c, _, _ in slabs)), "frames": ens.n_frames,
+            "ms_per_pass": dt * 1e3, "phi_frames_per_s": ens.n_frames / dt}
+
+
+def large_slab_config(engine, torch, reps):
+    from giwaxsim_amd.synth import polymer_slab
     c, e = polymer_slab(size=(200.0, 200.0, 60.0), seed=5, density=0.54)
     r = 0.15
     ids, ftab, _, _ = engine.element_table(e, 12700.0)
@@ -317,11 +341,24 @@ def extra_configs(args, engine, torch):
         res = run()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t) / reps
-    out["large_slab"] = {"workload": "polymer slab 200x200x60 A, 1.3M atoms, N=2048, fill_bkg, smooth 20, "
-                                     "max_q 3.0, 417^3 grid (295^3 crop)",
-                         "n_atoms": len(c), "frames": res['plan'].n_frames, "ms_per_slab": dt * 1e3,
-                         "phi_frames_per_s": res['plan'].n_frames / dt}
-    return out
+    # its detector: 2048 x 2048 px, psi 0..180 in 0.25 deg steps (721 orientations)
+    from giwaxsim_amd.tools.comparison import _weights
+    psis = np.linspace(0, 180, 721)
+    one = np.array([0.0])
+    dplan = engine.detector_plan(2048, 3.0, (0.15, 0, 0), ('psi', 'None', 'None'), psis,
+                                 _weights(None, psis), one, _weights(None, one), one,
+                                 _weights(None, one))
+    engine.run_detector(res['iq32'], res['qx'], res['qy'], res['qz'], dplan, True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    engine.run_detector(res['iq32'], res['qx'], res['qy'], res['qz'], dplan, True)
+    torch.cuda.synchronize()
+    det_ms = (time.perf_counter() - t) * 1e3
+    return {"workload": "polymer slab 200x200x60 A, 1.3M atoms, N=2048, fill_bkg, smooth 20, "
+                        "max_q 3.0, 417^3 grid (295^3 crop); detector 2048^2, psi step 0.25",
+            "n_atoms": len(c), "frames": res['plan'].n_frames, "ms_per_slab": dt * 1e3,
+            "phi_frames_per_s": res['plan'].n_frames / dt, "detector_ms_per_image": det_ms,
+            "detector_images_per_s": 1e3 / det_ms}
 
 
 if __name__ == "__main__":
