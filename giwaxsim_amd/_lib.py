@@ -85,6 +85,8 @@ class GxAdjoint(ctypes.Structure):
         ("row_exp", c_void_p), ("ainv", c_void_p),
         ("g_bins", c_void_p), ("g_part", c_void_p), ("red", c_void_p), ("amin", c_void_p),
         ("g_atom", c_void_p),
+        ("x", c_void_p), ("y", c_void_p), ("ext", c_void_p), ("ecount", c_void_p),
+        ("vol_r3", c_double), ("avg_spec", c_void_p), ("avg_part", c_void_p), ("g_avg", c_void_p),
     ]
 
 

@@ -15,8 +15,14 @@ storing it.  Gradients: dL/d coords [n, 3], dL/d f_re and dL/d f_im per
 element.  A scale / flat background on the image is ordinary torch algebra
 (``image_loss``).
 
-Limits: power-of-two grid edges (256..2048), fill_bkg off, smooth 0, every
-atom at least one voxel inside the grid edge.
+fill_bkg and smooth act on the deposited grid as in the forward path: almost
+everywhere in the positions the frame grid is M S + avg_voxel_f p with fixed
+masks M and a fixed pattern p (the filled region and smoothed edges follow the
+integer index bounds, which are piecewise constant), and avg_voxel_f = sum f r^3
+/ (x_b y_b z_b) carries gradient to f and to the atoms that fix the bounding box.
+
+Limits: power-of-two grid edges (256..2048), every atom at least one voxel
+inside the grid edge.
 """
 
 from __future__ import annotations
@@ -29,8 +35,9 @@ import torch
 
 from . import _lib
 from ._lib import GxAdjoint, GxDetector, GxFrames, GxSlab, call
-from .engine import (_empty, bin_scale_for, detector_plan, device, element_table, frame_plan,
-                     intensity_bound, is_pow2_grid, pixels_per_bin, ptr, stream_handle, twiddles)
+from .engine import (_empty, bin_scale_for, collapse_record, detector_plan, device, element_table,
+                     fft_precision, frame_plan, gauss_prefix, intensity_bound, is_pow2_grid,
+                     pixels_per_bin, ptr, stream_handle, twiddles)
 from .tools.ptable_dict import aff_dict, ptable
 
 
@@ -67,8 +74,10 @@ class DifferentiableSlab:
     tools/comparison.py:705-731); ``detector``: detectormaker_fitting's
     arguments as a dict (num_pixels, max_q, psis, phis, thetas, optional
     *_weights, angle_init_vals, angle_init_axs, mirror).  ``frames=(lo, hi)``
-    restricts the phi-frames.  ``precision``: 'auto' (FP64 transforms, the
-    grid is sparse), 'fp32' or 'fp64'.
+    restricts the phi-frames.  ``fill_bkg`` / ``smooth`` as in
+    voxelgridmaker_fitting.  ``precision``: 'auto' (FP32 transforms for filled
+    or smoothed grids, FP64 for sparse ones, as the forward path), 'fp32' or
+    'fp64'.
 
     ``model(coords, f_re, f_im)`` -> finalized image [n_pix, n_pix] (FP64
     device tensor).  f_re / f_im (per element, order of ``self.elements``)
@@ -76,7 +85,7 @@ class DifferentiableSlab:
     """
 
     def __init__(self, elements, coords0, r_voxel_size, q_voxel_size, max_q, detector,
-                 energy=12700.0, frames=None, precision='auto', batch=None):
+                 energy=12700.0, frames=None, precision='auto', batch=None, fill_bkg=False, smooth=0):
         dev = device()
         ids, ftab, _, uniq = element_table(elements, energy)
         self.elements = [str(u) for u in uniq]
@@ -95,7 +104,9 @@ class DifferentiableSlab:
         N, W = plan.N, plan.W
         F = plan.n_frames
         self.frames = (0, F) if frames is None else (int(frames[0]), int(frames[1]))
-        self.f64 = {'auto': True, 'fp64': True, 'fp32': False}[precision]
+        self.fill, self.smooth = bool(fill_bkg), int(smooth)
+        self.dense = self.fill or self.smooth > 0
+        self.f64 = fft_precision(self.fill, self.smooth, precision)
         self._batch_arg = batch
         self._pixels_per_bin = pixels_per_bin(plan)
 
@@ -120,6 +131,7 @@ class DifferentiableSlab:
         self.fparams = torch.as_tensor(plan.fparams).to(dev).contiguous()
         self.tw = torch.as_tensor(twiddles(N, self.f64)).to(dev).contiguous()
         self.tw64 = torch.as_tensor(twiddles(N, True)).to(dev).contiguous()
+        cdt = torch.float64 if self.f64 else torch.float32
         # per-frame statistics of every frame (the backward reuses the forward's)
         self.ymin_key = _empty(3 * F, torch.int64)
         self.idx_bounds = _empty(4 * F, torch.int32)
@@ -136,7 +148,21 @@ class DifferentiableSlab:
         self.ainv = _empty(n, torch.int32)
         self.g_bins = _empty(nb, torch.float64)
         self.red = _empty(max(B, 2 * (1 + self.n_elem)) * _lib.GX_ADJ_RED_BLOCKS, torch.float64)
-        self.amin = _empty(B + 1, torch.int32)
+        self.amin = _empty(B + 7, torch.int32)
+        # fill_bkg / smooth tables (per batch) and the background-level adjoint
+        self.col_nm = _empty(B * N, torch.int32) if self.fill else None
+        self.col_sy = _empty(B * N, torch.float64) if self.smooth else None
+        self.col_sz = _empty(B * N, torch.float64) if self.smooth else None
+        self.col_spec = _empty(B * W * 2, cdt) if self.dense else None
+        if self.smooth:
+            cw, self.smooth_radius = gauss_prefix(self.smooth)
+            self.gauss = torch.as_tensor(cw).to(dev).contiguous()
+        else:
+            self.gauss, self.smooth_radius = None, 0
+        self.avg_spec = _empty(B * (N + W) * 2, torch.float64) if self.dense else None
+        self.avg_part = _empty(B * W * 2, torch.float64) if self.dense else None
+        self.g_avg = _empty(2, torch.float64)
+        self._records_for = (xb, yb)
         self.g_atom = _empty(5 * n, torch.float64)
         self.g_coords = _empty(3 * n, torch.float64)
         self.g_f = _empty(2 * self.n_elem, torch.float64)
@@ -253,15 +279,32 @@ class DifferentiableSlab:
                       selem=ptr(self.selem).value)
         call("gx_slab_sort_rows", ctypes.byref(slab), st)
         ftab = ft.cpu().numpy()
-        sum_abs = float(np.sum(counts * np.abs(ftab[:, 0] + 1j * ftab[:, 1])))
+        fc = ftab[:, 0] + 1j * ftab[:, 1]
+        sum_abs = float(np.sum(counts * np.abs(fc)))
         nf = self.frames[1] - self.frames[0]
-        B = intensity_bound(sum_abs, N, False, 0, 0.0, 0.0, 0.0, 0.0, r)
+        # background level and grid constants of the current bounding box
+        # (tools/comparison.py:741-744, tools/voxelgrids.py:343)
+        vol = xb * yb * zb
+        self.avg = (np.sum(counts * fc) / vol) * r ** 3 if self.dense else 0.0
+        max_voxels = np.sqrt(np.float64(xb) ** 2 + np.float64(yb) ** 2) // r
+        if self.fill and self._records_for != (xb, yb):
+            fpr = self.plan.fparams.copy()
+            for f, phi in enumerate(self.plan.grid.phis):
+                fpr[f, _lib.FP_LEN_MODE:_lib.FP_CT + 1] = collapse_record(phi, yb, xb)
+            self.fparams.copy_(torch.as_tensor(fpr))
+            self._records_for = (xb, yb)
+        B = intensity_bound(sum_abs, N, self.fill, self.smooth, self.avg, max_voxels, yb, xb, r)
         self.bin_scale = bin_scale_for([(nf, self._pixels_per_bin, B)])
         p, g = self.plan, self.plan.grid
         fr = GxFrames(
             N=N, r=r, n_atoms=self.n, ax=ptr(self.sx).value, ay=ptr(self.sy).value,
             aq=ptr(self.sq).value, azrow=ptr(self.szrow).value, row_off=ptr(self.row_off).value,
-            row_exp=ptr(self.row_exp).value, fill_bkg=0, smooth=0, max_voxels=p.max_voxels,
+            row_exp=ptr(self.row_exp).value, fill_bkg=int(self.fill), smooth=self.smooth,
+            max_voxels=float(max_voxels), avg_re=float(np.real(self.avg)),
+            avg_im=float(np.imag(self.avg)), gauss_cw=ptr(self.gauss).value,
+            smooth_radius=self.smooth_radius, col_nm=ptr(self.col_nm).value,
+            col_sy=ptr(self.col_sy).value, col_sz=ptr(self.col_sz).value,
+            col_spec=ptr(self.col_spec).value,
             zrow_max=int(np.floor_divide(zb, r)), qmin=g.qmin, dq=g.dq, c_lo=g.c_lo, c_n=g.c_n,
             q_val=p.q_val, i_lo=p.i_lo, i_hi=p.i_hi, W=p.W, Z=Z, bin_scale=self.bin_scale,
             fft_f64=int(self.f64), twiddle=ptr(self.tw).value, twiddle64=ptr(self.tw64).value,
@@ -275,7 +318,10 @@ class DifferentiableSlab:
             z=ptr(self.z).value, zmin=zmin, row_exp=ptr(self.adj_row_exp).value,
             ainv=ptr(self.ainv).value, g_bins=ptr(self.g_bins).value,
             g_part=ptr(self.g_part).value, red=ptr(self.red).value, amin=ptr(self.amin).value,
-            g_atom=ptr(self.g_atom).value)
+            g_atom=ptr(self.g_atom).value, x=ptr(self.x).value, y=ptr(self.y).value,
+            ext=ptr(self.minmax).value, ecount=ptr(self.ecount).value, vol_r3=r ** 3 / vol,
+            avg_spec=ptr(self.avg_spec).value, avg_part=ptr(self.avg_part).value,
+            g_avg=ptr(self.g_avg).value)
         return fr, adj
 
     def _check_status(self):

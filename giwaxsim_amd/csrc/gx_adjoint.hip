@@ -90,6 +90,10 @@ __global__ __launch_bounds__(BLK) void k_adj_init(gx_frames d, gx_adjoint g) {
   }
 }
 
+__global__ void k_zero_f64(double *p, int64_t n) {
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
 __global__ __launch_bounds__(BLK) void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
   for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) p[i] = v;
 }
@@ -102,11 +106,12 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
   constexpr int RB = NWAVE * RPW;
-  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  constexpr int R0 = FftFirst<N>::R;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   unsigned long long *accs = reinterpret_cast<unsigned long long *>(smem_raw);  // [NWAVE][2][N]
   C *stage = reinterpret_cast<C *>(accs + (size_t)NWAVE * 2 * N);            // [W][RB]
-  double *ftab = reinterpret_cast<double *>(stage + d.W * RB);               // [n_elem][2]
+  C *tspec = stage + d.W * RB;                                               // [W] background spectrum
+  double *ftab = reinterpret_cast<double *>(tspec + d.W);                    // [n_elem][2]
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
   unsigned long long *acc_re = accs + (size_t)wave * 2 * N;
@@ -123,20 +128,41 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
     return;
   }
   for (int i = threadIdx.x; i < 2 * g.n_elem; i += BLK) ftab[i] = g.f_table[i];
+  if (d.fill_bkg || d.smooth) {
+    const C *spec = reinterpret_cast<const C *>(d.col_spec) + (int64_t)fb * d.W;
+    for (int w = threadIdx.x; w < wf; w += BLK) tspec[w] = spec[w];
+  }
   __syncthreads();
 
   const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
   const double ymin = dkey_inv(d.ymin_key[3 * fb]);
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  // nodes in columns the fill overwrites keep no deposit: ya <= y < yb survive
+  const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
+  const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
   const double r = d.r, rinv = 1.0 / d.r;
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
   const int z0 = blockIdx.x * RB;
 
+  // node mask of this lane's first-stage inputs (smoothed edge x filled region)
+  T my[P];
+#pragma unroll
+  for (int t = 0; t < P / R0; ++t)
+#pragma unroll
+    for (int q = 0; q < R0; ++q) {
+      const int y = lane + GX_WAVE * t + q * (N / R0);
+      const T m = d.smooth ? (T)d.col_sy[(int64_t)fb * N + y] : (T)1;
+      my[t * R0 + q] = (y >= ya && y < yb) ? m : (T)0;
+    }
+
   for (int rr = 0; rr < RPW; ++rr) {
     const int rb = wave * RPW + rr;
     const int v = z0 + rb;
+    const bool inside = !d.fill_bkg || (v >= zlo && v < zhi);
     const int a_lo = d.row_off[v > 0 ? v - 1 : 0], a_mid = d.row_off[v], a_hi = d.row_off[v + 1];
     C x[P];
-    if (a_hi > a_lo) {
+    if (inside && a_hi > a_lo) {
       for (int y = lane; y < N; y += GX_WAVE) {
         acc_re[y] = 0;
         acc_im[y] = 0;
@@ -168,7 +194,8 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
 #pragma unroll
         for (int q = 0; q < R0; ++q) {
           const int y = lane + GX_WAVE * t + q * (N / R0);
-          x[t * R0 + q] = C{(T)((double)(long long)acc_re[y] * inv), (T)((double)(long long)acc_im[y] * inv)};
+          const T m = my[t * R0 + q];
+          x[t * R0 + q] = C{(T)((double)(long long)acc_re[y] * inv) * m, (T)((double)(long long)acc_im[y] * inv) * m};
         }
       }
       wave_sync();
@@ -177,15 +204,8 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
 #pragma unroll
       for (int q = 0; q < P; ++q) x[q] = C{(T)0, (T)0};
     }
-#pragma unroll
-    for (int t = 0; t < P / RL; ++t) {
-#pragma unroll
-      for (int q = 0; q < RL; ++q) {
-        const int k = lane + GX_WAVE * t + q * (N / RL);
-        const int w = ((k + N / 2) & (N - 1)) - jlo;
-        if (w >= 0 && w < wf) stage[w * RB + rb] = x[t * RL + q];
-      }
-    }
+    const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + v] : 1.0;
+    stage_row_band<N, C>(x, lane, jlo, wf, inside, d, mz, tspec, stage, RB, rb);
     wave_sync();
   }
   __syncthreads();
@@ -212,6 +232,8 @@ __global__ __launch_bounds__(BLK) void k_adj_colpass(gx_frames d, gx_adjoint g, 
   const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
   const int wcol = blockIdx.x * NWAVE + wave;
   if (wcol > jhi - jlo || wcol >= d.W) return;  // wave-uniform; no block barrier below
+  const bool dense = d.fill_bkg || d.smooth;
+  double2 *apart = reinterpret_cast<double2 *>(g.avg_part);
   C *col = reinterpret_cast<C *>(d.inter) + ((int64_t)fb * d.W + wcol) * d.Z;
   const int j = jlo + wcol;
   // the forward's column -> bin arithmetic (k_colpass)
@@ -222,42 +244,73 @@ __global__ __launch_bounds__(BLK) void k_adj_colpass(gx_frames d, gx_adjoint g, 
   const long long by = to_int(floor_div(hy - d.qmin, d.dq, dqinv)) - d.c_lo;
   if (bx < 0 || bx >= d.c_n || by < 0 || by >= d.c_n) {  // column never binned: zero cotangent
     for (int z = lane; z < d.Z; z += GX_WAVE) col[z] = C{(T)0, (T)0};
+    if (dense && lane == 0) apart[(int64_t)fb * d.W + wcol] = make_double2(0.0, 0.0);
     return;
   }
+  const bool k0 = (j == N / 2);
+  C u = C{(T)0, (T)0};
+  if (!d.fill_bkg && d.smooth) u = reinterpret_cast<const C *>(d.col_spec)[(int64_t)fb * d.W + wcol];
+  const double *sz_row = d.col_sz + (int64_t)fb * N;
   C v[P];
 #pragma unroll
-  for (int t = 0; t < P / R0; ++t) {
+  for (int t = 0; t < P / R0; ++t)
 #pragma unroll
-    for (int q = 0; q < R0; ++q) {
-      const int z = lane + GX_WAVE * t + q * (N / R0);
-      v[t * R0 + q] = (z < d.Z) ? col[z] : C{(T)0, (T)0};
-    }
-  }
+    for (int q = 0; q < R0; ++q) v[t * R0 + q] = col_input(d, (const C *)col, lane + GX_WAVE * t + q * (N / R0), k0, u, sz_row);
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
   C *buf = lds[wave];
   fft_wave<N, C>(v, buf, tw, lane);
   wave_sync();
-  // conj of the cotangent dL/dF = 2 (dL/dI) F at its natural index k
+  // conj of the cotangent dL/dF = 2 (dL/dI) F at its natural index k; with fill or
+  // smoothing also this column's share sum_k dL/dF_k conj(Pz^[k]) conj(Py^[j]) of dL/d avg
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;
   const double val = d.q_val;
+  const double2 *zhat = reinterpret_cast<const double2 *>(g.avg_spec) + (int64_t)fb * (N + d.W);
+  double are = 0.0, aim = 0.0, dc_re = 0.0, dc_im = 0.0;
 #pragma unroll
   for (int t = 0; t < P / RL; ++t) {
 #pragma unroll
     for (int q = 0; q < RL; ++q) {
       const int k = lane + GX_WAVE * t + q * (N / RL);
       const int i = (k + N / 2) & (N - 1);
-      C gf = C{(T)0, (T)0};
+      double gr = 0.0, gi = 0.0;
       if (i >= d.i_lo && i <= d.i_hi) {
         const int kint = (k < N / 2) ? k : k - N;
         const double qz = (((double)kint * val) * 2.0) * M_PI;
         const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
         if (bz >= 0 && bz < d.c_n) {
-          const double gi = 2.0 * g.g_bins[cbase + bz];
+          const double gI = 2.0 * g.g_bins[cbase + bz];
           const C F = v[t * RL + q];
-          gf = C{(T)(gi * (double)F.x), (T)(-(gi * (double)F.y))};
+          gr = gI * (double)F.x;
+          gi = gI * (double)F.y;
         }
       }
-      buf[k] = gf;
+      if (dense) {
+        const double2 zh = zhat[k];  // dL/dF conj(Pz^)
+        are += gr * zh.x + gi * zh.y;
+        aim += gi * zh.x - gr * zh.y;
+        if (k == 0) {
+          dc_re = gr;
+          dc_im = gi;
+        }
+      }
+      buf[k] = C{(T)gr, (T)(-gi)};
+    }
+  }
+  if (dense) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      are += __shfl_xor(are, off, GX_WAVE);
+      aim += __shfl_xor(aim, off, GX_WAVE);
+    }
+    if (lane == 0) {
+      const double2 yh = zhat[N + wcol];  // times conj(Py^[j])
+      double pre = are * yh.x + aim * yh.y, pim = aim * yh.x - are * yh.y;
+      if (k0) {  // the constant max_voxels of the pattern: max_voxels N^2 dL/dF at k = 0
+        const double c0 = d.max_voxels * (double)N * (double)N;
+        pre = pre + c0 * dc_re;
+        pim = pim + c0 * dc_im;
+      }
+      apart[(int64_t)fb * d.W + wcol] = make_double2(pre, pim);
     }
   }
   wave_sync();
@@ -275,6 +328,93 @@ __global__ __launch_bounds__(BLK) void k_adj_colpass(gx_frames d, gx_adjoint g, 
       const int z = lane + GX_WAVE * t + q * (N / RL);
       if (z < d.Z) col[z] = cconj(v[t * RL + q]);
     }
+  }
+}
+
+// Per frame: Pz^ = FFT of Pz (all k) and Py^ = FFT of Py on the kept columns, FP64,
+// one wave each (grid (nf, 2)):  fill: Pz = inz (sz), Py = iny (nm - max_voxels) (sy);
+// smooth only: Pz = sz, Py = -max_voxels sy  (sz, sy the smoothed masks when smoothing).
+template <int N>
+__global__ __launch_bounds__(GX_WAVE) void k_adj_avg_spectra(gx_frames d, gx_adjoint g, int64_t f0) {
+  constexpr int P = N / GX_WAVE;
+  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  __shared__ double2 lds[N];
+  const int lane = threadIdx.x;
+  const int fb = blockIdx.x;
+  const bool along_y = blockIdx.y == 1;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = max(0, min(jhi - jlo + 1, (int)d.W));
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  const int ya = ylo == INT_MAX ? N : ylo + 1, yb = yhi == INT_MIN ? 0 : yhi;
+  double2 v[P];
+#pragma unroll
+  for (int t = 0; t < P / R0; ++t) {
+#pragma unroll
+    for (int q = 0; q < R0; ++q) {
+      const int i = lane + GX_WAVE * t + q * (N / R0);
+      double p;
+      if (!along_y) {
+        p = (!d.fill_bkg || (i >= zlo && i < zhi)) ? 1.0 : 0.0;
+        if (d.smooth) p = p * d.col_sz[(int64_t)fb * N + i];
+      } else {
+        if (d.fill_bkg)
+          p = (i >= ya && i < yb) ? (double)d.col_nm[(int64_t)fb * N + i] - d.max_voxels : 0.0;
+        else
+          p = -d.max_voxels;
+        if (d.smooth) p = p * d.col_sy[(int64_t)fb * N + i];
+      }
+      v[t * R0 + q] = make_double2(p, 0.0);
+    }
+  }
+  fft_wave<N, double2>(v, lds, reinterpret_cast<const double2 *>(d.twiddle64), lane);
+  double2 *out = reinterpret_cast<double2 *>(g.avg_spec) + (int64_t)fb * (N + d.W);
+#pragma unroll
+  for (int t = 0; t < P / RL; ++t) {
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int k = lane + GX_WAVE * t + q * (N / RL);
+      if (!along_y) {
+        out[k] = v[t * RL + q];
+      } else {
+        const int w = ((k + N / 2) & (N - 1)) - jlo;
+        if (w >= 0 && w < wf) out[N + w] = v[t * RL + q];
+      }
+    }
+  }
+}
+
+// One block: g_avg += sum over the batch's frames (in order) of the sum over each
+// frame's kept columns (in order) of avg_part.
+__global__ __launch_bounds__(BLK) void k_adj_avg_reduce(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf) {
+  __shared__ double2 S[BLK];
+  const double2 *apart = reinterpret_cast<const double2 *>(g.avg_part);
+  double tre = 0.0, tim = 0.0;
+  for (int64_t base = 0; base < nf; base += BLK) {
+    const int64_t fb = base + threadIdx.x;
+    if (fb < nf) {
+      const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+      const int wf = max(0, min((int)fp[GX_FP_JHI] - (int)fp[GX_FP_JLO] + 1, (int)d.W));
+      double re = 0.0, im = 0.0;
+      for (int w = 0; w < wf; ++w) {
+        const double2 a = apart[fb * d.W + w];
+        re += a.x;
+        im += a.y;
+      }
+      S[threadIdx.x] = make_double2(re, im);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int64_t k = 0; k < BLK && base + k < nf; ++k) {
+        tre += S[k].x;
+        tim += S[k].y;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    g.g_avg[0] += tre;
+    g.g_avg[1] += tim;
   }
 }
 
@@ -332,17 +472,36 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   }
   __syncthreads();
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
+  // node mask M = (mz inz) x (my iny) of the deposit (1 without fill and smoothing)
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
+  const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
+  const bool masked = d.fill_bkg || d.smooth;
 #pragma unroll
   for (int h = 0; h < HW; ++h) {
     const int q = wave + NWAVE * h;
     if (q >= H) break;  // wave-uniform
+    const int v = z0 + q;
+    double mzv = 1.0;
+    if (d.fill_bkg && !(v >= zlo && v < zhi)) mzv = 0.0;
+    if (d.smooth && v < N) mzv = mzv * d.col_sz[(int64_t)fb * N + v];
     C *buf = rows + q * N;
     fft_wave<N, C>(x[h], buf, tw, lane);
     wave_sync();
 #pragma unroll
     for (int t = 0; t < P / RL; ++t)
 #pragma unroll
-      for (int qq = 0; qq < RL; ++qq) buf[lane + GX_WAVE * t + qq * (N / RL)] = cconj(x[h][t * RL + qq]);
+      for (int qq = 0; qq < RL; ++qq) {
+        const int y = lane + GX_WAVE * t + qq * (N / RL);
+        C o = cconj(x[h][t * RL + qq]);
+        if (masked) {
+          double m = (y >= ya && y < yb) ? mzv : 0.0;
+          if (d.smooth) m = m * d.col_sy[(int64_t)fb * N + y];
+          o = C{(T)((double)o.x * m), (T)((double)o.y * m)};
+        }
+        buf[y] = o;
+      }
   }
   __syncthreads();
 
@@ -492,8 +651,10 @@ __global__ __launch_bounds__(BLK) void k_adj_reduce(gx_frames d, gx_adjoint g, i
     for (int k = 0; k < 5; ++k) g.g_atom[k * n + a] += sh[k][0][al];
 }
 
-// grid (RED, 1 + n_elem): y = 0 sums dL/dz over atoms (original order) and finds
-// the smallest original index with z == zmin; y = 1 + e sums dL/df of element e
+// grid (RED, 1 + n_elem): y = 0 sums dL/dz over atoms (original order) and finds the
+// smallest original index attaining each bounding-box extreme (slots: x min, x max,
+// y min, y max, z min, z max; only z min without fill / smoothing); y = 1 + e sums
+// dL/df of element e
 __global__ __launch_bounds__(BLK) void k_adj_fin_sums(gx_frames d, gx_adjoint g, int64_t slot) {
   __shared__ double shd[NWAVE];
   __shared__ int shi[NWAVE];
@@ -501,13 +662,21 @@ __global__ __launch_bounds__(BLK) void k_adj_fin_sums(gx_frames d, gx_adjoint g,
   const int64_t n = d.n_atoms;
   const int64_t chunk = (n + RED - 1) / RED;
   const int64_t i0 = b * chunk, i1 = min(n, i0 + chunk);
+  const bool box = d.fill_bkg || d.smooth;
   double s0 = 0.0, s1 = 0.0;
-  int am = INT_MAX;
+  int am[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX};
   for (int64_t i = i0 + threadIdx.x; i < i1; i += BLK) {
     const int a = g.ainv[i];
     if (y == 0) {
       s0 += g.g_atom[2 * n + a];
-      if (g.z[i] == g.zmin) am = min(am, (int)i);
+      if (g.z[i] == g.zmin) am[4] = min(am[4], (int)i);
+      if (box) {
+        if (g.x[i] == g.ext[0]) am[0] = min(am[0], (int)i);
+        if (g.x[i] == g.ext[1]) am[1] = min(am[1], (int)i);
+        if (g.y[i] == g.ext[2]) am[2] = min(am[2], (int)i);
+        if (g.y[i] == g.ext[3]) am[3] = min(am[3], (int)i);
+        if (g.z[i] == g.ext[5]) am[5] = min(am[5], (int)i);
+      }
     } else if (g.aelem[a] == y - 1) {
       s0 += g.g_atom[3 * n + a];
       s1 += g.g_atom[4 * n + a];
@@ -515,27 +684,51 @@ __global__ __launch_bounds__(BLK) void k_adj_fin_sums(gx_frames d, gx_adjoint g,
   }
   s0 = block_sum(s0, shd);
   s1 = block_sum(s1, shd);
-  if (y == 0) am = block_min_int(am, shi);
+  if (y == 0)
+    for (int k = 0; k < 6; ++k) am[k] = block_min_int(am[k], shi);
   if (threadIdx.x == 0) {
     double *o = g.red + ((int64_t)y * RED + b) * 2;
     o[0] = s0;
     o[1] = s1;
-    if (y == 0 && am != INT_MAX) atomicMin(&g.amin[slot], am);
+    if (y == 0)
+      for (int k = 0; k < 6; ++k)
+        if (am[k] != INT_MAX) atomicMin(&g.amin[slot + k], am[k]);
   }
 }
 
+// y = 0: the origin term of min z; with fill / smoothing also the bounding-box
+// terms of avg_voxel_f = sum f r^3 / (x_b y_b z_b): dL/dx_b = -Re(conj(dL/d avg) avg) / x_b
+// (likewise y, z) on the extreme atoms.  y = 1 + e: dL/df_e, plus count_e r^3 / V dL/d avg.
 __global__ __launch_bounds__(BLK) void k_adj_fin_apply(gx_frames d, gx_adjoint g, int64_t slot, double *g_f) {
   const int y = threadIdx.x;
   if (y > g.n_elem) return;
+  const bool box = d.fill_bkg || d.smooth;
   double s0 = 0.0, s1 = 0.0;
   for (int b = 0; b < RED; ++b) {
     s0 += g.red[((int64_t)y * RED + b) * 2];
     s1 += g.red[((int64_t)y * RED + b) * 2 + 1];
   }
+  const int64_t n = d.n_atoms;
   if (y == 0) {
-    const int am = g.amin[slot];
-    if (am != INT_MAX) g.g_atom[2 * d.n_atoms + g.ainv[am]] -= s0;  // origin term of min z
+    const int am = g.amin[slot + 4];
+    if (am != INT_MAX) g.g_atom[2 * n + g.ainv[am]] -= s0;  // origin term of min z
+    if (box) {
+      const double t = g.g_avg[0] * d.avg_re + g.g_avg[1] * d.avg_im;
+      for (int k = 0; k < 3; ++k) {
+        const double ext = g.ext[2 * k + 1] - g.ext[2 * k];
+        if (!(ext > 0.0)) continue;
+        const double dl = -t / ext;
+        const int lo = g.amin[slot + 2 * k], hi = g.amin[slot + 2 * k + 1];
+        if (hi != INT_MAX) g.g_atom[k * n + g.ainv[hi]] += dl;
+        if (lo != INT_MAX) g.g_atom[k * n + g.ainv[lo]] -= dl;
+      }
+    }
   } else {
+    if (box) {
+      const double ce = (double)g.ecount[y - 1] * g.vol_r3;
+      s0 = s0 + ce * g.g_avg[0];
+      s1 = s1 + ce * g.g_avg[1];
+    }
     g_f[2 * (y - 1)] = s0;
     g_f[2 * (y - 1) + 1] = s1;
   }
@@ -586,7 +779,7 @@ template <int N, typename C>
 int launch_adj_rowpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int RPW = (N <= 1024) ? 2 : 1;
   constexpr int RB = NWAVE * RPW;
-  const size_t lds = (size_t)NWAVE * 2 * N * 8 + (size_t)d->W * RB * sizeof(C) + (size_t)g->n_elem * 16;
+  const size_t lds = (size_t)NWAVE * 2 * N * 8 + (size_t)d->W * (RB + 1) * sizeof(C) + (size_t)g->n_elem * 16;
   if (lds > 160 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "differentiable row pass needs %zu bytes of LDS (N=%d, W=%lld)", lds, N,
                      (long long)d->W);
@@ -627,10 +820,14 @@ int launch_adj_rowgrad_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, in
   return check_launch("k_adj_rowgrad");
 }
 
-enum Op { OP_ROWPASS, OP_COLPASS, OP_ROWGRAD };
+enum Op { OP_ROWPASS, OP_COLPASS, OP_ROWGRAD, OP_AVGSPEC };
 
 template <int N>
 int dispatch_n(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
+  if (op == OP_AVGSPEC) {
+    k_adj_avg_spectra<N><<<dim3((unsigned)nf, 2), GX_WAVE, 0, st>>>(*d, *g, f0);
+    return check_launch("k_adj_avg_spectra");
+  }
   if (d->fft_f64) {
     if (op == OP_ROWPASS) return launch_adj_rowpass_t<N, double2>(d, g, f0, nf, st);
     if (op == OP_COLPASS) return launch_adj_colpass_t<N, double2>(d, g, f0, nf, st);
@@ -656,7 +853,10 @@ int check_adj(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, b
   GX_REQUIRE(d != nullptr && g != nullptr, "NULL descriptor");
   GX_REQUIRE(d->N == 256 || d->N == 512 || d->N == 1024 || d->N == 2048,
              "differentiable path: grid edge N=%lld is not 256/512/1024/2048", (long long)d->N);
-  GX_REQUIRE(!d->fill_bkg && !d->smooth, "differentiable path: fill_bkg and smooth must be off");
+  const bool dense = d->fill_bkg || d->smooth;
+  GX_REQUIRE(!d->fill_bkg || d->col_nm, "fill_bkg needs col_nm");
+  GX_REQUIRE(!d->smooth || (d->col_sy && d->col_sz && d->gauss_cw), "smooth needs col_sy, col_sz, gauss_cw");
+  GX_REQUIRE(!dense || (d->col_spec && d->twiddle64), "fill_bkg / smooth need col_spec and twiddle64");
   GX_REQUIRE(d->bs_M == 0, "differentiable path: chirp-z tables given for a power-of-two grid");
   GX_REQUIRE(d->r > 0.0, "r_voxel_size must be positive");
   GX_REQUIRE(nf >= 0 && nf <= d->batch, "batch of %lld frames exceeds workspace batch %lld", (long long)nf,
@@ -673,8 +873,11 @@ int check_adj(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, b
   GX_REQUIRE(g->n_elem >= 1 && g->n_elem <= 128, "n_elem=%lld outside [1, 128]", (long long)g->n_elem);
   GX_REQUIRE(g->f_table && g->afz && g->aperm && g->aelem && g->ecnt && g->row_exp && g->ainv,
              "a required buffer of gx_adjoint is NULL");
-  if (backward)
+  if (backward) {
     GX_REQUIRE(g->g_bins && g->g_part && g->red && g->amin && g->g_atom, "a backward buffer of gx_adjoint is NULL");
+    GX_REQUIRE(!dense || (g->x && g->y && g->ext && g->ecount && g->avg_spec && g->avg_part && g->g_avg),
+               "fill_bkg / smooth need gx_adjoint.x, y, ext, ecount, avg_spec, avg_part, g_avg");
+  }
   return GX_OK;
 }
 
@@ -689,6 +892,7 @@ int gx_adj_prepare(const gx_frames *d, const gx_adjoint *g, gx_stream_t stream) 
   GX_REQUIRE(g->g_atom, "g_atom is NULL");
   hipStream_t st = as_stream(stream);
   k_adj_rowexp<<<grid1d(d->N, BLK), BLK, 0, st>>>(*d, *g);
+  if (g->g_avg) k_zero_f64<<<1, 2, 0, st>>>(g->g_avg, 2);
   if (d->n_atoms > 0) {
     unsigned gr = grid1d(d->n_atoms, BLK);
     k_adj_init<<<gr > 4096 ? 4096 : gr, BLK, 0, st>>>(*d, *g);
@@ -708,6 +912,7 @@ int gx_adj_forward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t 
   int rc = check_adj(d, g, f0, nf, false);
   if (rc || nf == 0) return rc;
   if ((rc = gx_frames_stats(d, f0, nf, stream))) return rc;
+  if ((d->fill_bkg || d->smooth) && (rc = gx_frames_columns(d, f0, nf, stream))) return rc;
   if ((rc = dispatch(OP_ROWPASS, d, g, f0, nf, as_stream(stream)))) return rc;
   return gx_frames_colpass(d, f0, nf, stream);
 }
@@ -743,8 +948,17 @@ int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t
   int rc = check_adj(d, g, f0, nf, true);
   if (rc || nf == 0) return rc;
   hipStream_t st = as_stream(stream);
+  const bool dense = d->fill_bkg || d->smooth;
+  if (dense) {
+    if ((rc = gx_frames_columns(d, f0, nf, stream))) return rc;
+    if ((rc = dispatch(OP_AVGSPEC, d, g, f0, nf, st))) return rc;
+  }
   if ((rc = dispatch(OP_ROWPASS, d, g, f0, nf, st))) return rc;
   if ((rc = dispatch(OP_COLPASS, d, g, f0, nf, st))) return rc;
+  if (dense) {
+    k_adj_avg_reduce<<<1, BLK, 0, st>>>(*d, *g, f0, nf);
+    if ((rc = check_launch("k_adj_avg_reduce"))) return rc;
+  }
   if ((rc = dispatch(OP_ROWGRAD, d, g, f0, nf, st))) return rc;
   return gx_adj_reduce(d, g, f0, nf, stream);
 }
@@ -758,7 +972,7 @@ int gx_adj_finish(const gx_frames *d, const gx_adjoint *g, double *g_coords, dou
   GX_REQUIRE(g->n_elem + 1 <= BLK, "n_elem");
   hipStream_t st = as_stream(stream);
   const int64_t slot = d->batch;
-  k_fill_i32<<<1, BLK, 0, st>>>(g->amin + slot, 1, INT_MAX);
+  k_fill_i32<<<1, BLK, 0, st>>>(g->amin + slot, 6, INT_MAX);
   k_adj_fin_sums<<<dim3(RED, (unsigned)(1 + g->n_elem)), BLK, 0, st>>>(*d, *g, slot);
   k_adj_fin_apply<<<1, BLK, 0, st>>>(*d, *g, slot, g_f);
   if (d->n_atoms > 0) {

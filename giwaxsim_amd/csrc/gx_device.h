@@ -346,6 +346,71 @@ __device__ __forceinline__ double collapse_length(const double *fp, double x) {
   return 0.0;
 }
 
+// ------------------------------------------------------------------------------------
+// Pieces of the row and column passes shared by the forward and the
+// differentiable variant (same arithmetic in both, by construction).
+//
+// stage_row_band: the kept fftshifted column band [jlo, jlo + wf) of one row
+// transform v (last-stage register layout) into stage[w * RB + rb].  A row
+// outside the filled z range is the constant A = avg_voxel_f * max_voxels
+// (only k = 0 is non-zero); inside, with fill or smoothing, the frame's
+// background spectrum U is added: o = mz (v + U) (+ A N at k = 0 when smoothing).
+// ------------------------------------------------------------------------------------
+template <int N, typename C>
+__device__ __forceinline__ void stage_row_band(const C (&v)[N / GX_WAVE], int lane, int jlo, int wf, bool inside,
+                                               const gx_frames &d, double mz, const C *tspec, C *stage, int RB,
+                                               int rb) {
+  using T = typename Real<C>::T;
+  constexpr int P = N / GX_WAVE, RL = FftLast<N>::R;
+  const bool dense = d.fill_bkg || d.smooth;
+  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
+#pragma unroll
+  for (int t = 0; t < P / RL; ++t) {
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int k = lane + GX_WAVE * t + q * (N / RL);
+      const int w = ((k + N / 2) & (N - 1)) - jlo;
+      if (w < 0 || w >= wf) continue;
+      C o = v[t * RL + q];
+      if (!inside) {
+        o = (k == 0) ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
+      } else if (dense) {
+        const C u = tspec[w];
+        double ore = mz * ((double)o.x + (double)u.x);
+        double oim = mz * ((double)o.y + (double)u.y);
+        if (d.smooth && k == 0) {
+          ore = ore + avv_re * (double)N;
+          oim = oim + avv_im * (double)N;
+        }
+        o = C{(T)ore, (T)oim};
+      }
+      stage[w * RB + rb] = o;
+    }
+  }
+}
+
+// col_input: element z of a kept column (fftshifted column j = jlo + wcol) for the
+// transform along z.  Rows z < Z come from the row pass; rows >= Z hold no atoms:
+// with fill they are the constant A (only the k = 0 column, k0), otherwise
+// mz[z] U (+ A N at k = 0) when smoothing, else zero.
+template <typename C>
+__device__ __forceinline__ C col_input(const gx_frames &d, const C *col, int z, bool k0, C u, const double *sz_row) {
+  using T = typename Real<C>::T;
+  if (z < d.Z) return col[z];
+  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
+  if (d.fill_bkg) return k0 ? C{(T)(avv_re * (double)d.N), (T)(avv_im * (double)d.N)} : C{(T)0, (T)0};
+  if (d.smooth) {
+    const double mz = sz_row[z];
+    double xre = mz * (double)u.x, xim = mz * (double)u.y;
+    if (k0) {
+      xre = xre + avv_re * (double)d.N;
+      xim = xim + avv_im * (double)d.N;
+    }
+    return C{(T)xre, (T)xim};
+  }
+  return C{(T)0, (T)0};
+}
+
 // numpy linspace element i of n: i * step + start, the last one exactly stop.
 __device__ __forceinline__ double lin_axis(int64_t i, int64_t n, double start, double step, double stop) {
   return (i == n - 1) ? stop : (double)i * step + start;

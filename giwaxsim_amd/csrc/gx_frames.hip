@@ -488,7 +488,6 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
   const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;   // keep ya <= y < yb
   const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
   const double r = d.r, rinv = 1.0 / d.r;
-  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
   const int z0 = blockIdx.x * RB;
 
@@ -559,30 +558,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
     }
     // 5) kept columns [jlo, jhi]: add the background spectrum, stage
     const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + z] : 1.0;
-#pragma unroll
-    for (int t = 0; t < P / RL; ++t) {
-#pragma unroll
-      for (int q = 0; q < RL; ++q) {
-        const int k = lane + GX_WAVE * t + q * (N / RL);
-        const int w = ((k + N / 2) & (N - 1)) - jlo;
-        if (w < 0 || w >= wf) continue;
-        C o = v[t * RL + q];
-        if (!inside) {
-          // filled row: the constant avg_voxel_f * max_voxels
-          o = (k == 0) ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
-        } else if (dense) {
-          const C u = tspec[w];
-          double ore = mz * ((double)o.x + (double)u.x);
-          double oim = mz * ((double)o.y + (double)u.y);
-          if (d.smooth && k == 0) {
-            ore = ore + avv_re * (double)N;
-            oim = oim + avv_im * (double)N;
-          }
-          o = C{(T)ore, (T)oim};
-        }
-        stage[w * RB + rb] = o;
-      }
-    }
+    stage_row_band<N, C>(v, lane, jlo, wf, inside, d, mz, tspec, stage, RB, rb);
     wave_sync();
   }
   __syncthreads();
@@ -621,38 +597,15 @@ __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
 
   using T = typename Real<C>::T;
   const C *col = reinterpret_cast<const C *>(d.inter) + ((int64_t)fb * d.W + wcol) * d.Z;
-  // rows >= Z hold no atoms: with fill they are the constant avg_voxel_f * max_voxels
-  // (only the k = 0 column), otherwise mz * background spectrum (+ A N at k = 0)
   const bool k0 = (j == N / 2);
-  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
   C u = C{(T)0, (T)0};
   if (!d.fill_bkg && d.smooth) u = reinterpret_cast<const C *>(d.col_spec)[(int64_t)fb * d.W + wcol];
   const double *sz_row = d.col_sz + (int64_t)fb * N;
   C v[P];
 #pragma unroll
-  for (int t = 0; t < P / R0; ++t) {
+  for (int t = 0; t < P / R0; ++t)
 #pragma unroll
-    for (int q = 0; q < R0; ++q) {
-      const int z = lane + GX_WAVE * t + q * (N / R0);
-      C x;
-      if (z < d.Z) {
-        x = col[z];
-      } else if (d.fill_bkg) {
-        x = k0 ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
-      } else if (d.smooth) {
-        const double mz = sz_row[z];
-        double xre = mz * (double)u.x, xim = mz * (double)u.y;
-        if (k0) {
-          xre = xre + avv_re * (double)N;
-          xim = xim + avv_im * (double)N;
-        }
-        x = C{(T)xre, (T)xim};
-      } else {
-        x = C{(T)0, (T)0};
-      }
-      v[t * R0 + q] = x;
-    }
-  }
+    for (int q = 0; q < R0; ++q) v[t * R0 + q] = col_input(d, col, lane + GX_WAVE * t + q * (N / R0), k0, u, sz_row);
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
   fft_wave<N, C>(v, lds[wave], tw, lane);
 
@@ -893,9 +846,9 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_colpass_bs(gx_frames d, int64_t 
   if (bx < 0 || bx >= d.c_n || by < 0 || by >= d.c_n) return;
   const C *col = reinterpret_cast<const C *>(d.inter) + ((int64_t)fb * d.W + wcol) * d.Z;
   const bool k0 = (j == N / 2);
-  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
   C u = C{(T)0, (T)0};
   if (!d.fill_bkg && d.smooth) u = reinterpret_cast<const C *>(d.col_spec)[(int64_t)fb * d.W + wcol];
+  const double *sz_row = d.col_sz + (int64_t)fb * N;
   const C *chirp = reinterpret_cast<const C *>(d.bs_chirp);
   C v[P];
 #pragma unroll
@@ -903,23 +856,7 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_colpass_bs(gx_frames d, int64_t 
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int z = tid + GX_BLK_FFT * t + q * (M / R);
-      C x = C{(T)0, (T)0};
-      if (z < d.Z) {
-        x = col[z];
-      } else if (z < N) {
-        if (d.fill_bkg) {
-          x = k0 ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
-        } else if (d.smooth) {
-          const double mz = d.col_sz[(int64_t)fb * N + z];
-          double xre = mz * (double)u.x, xim = mz * (double)u.y;
-          if (k0) {
-            xre = xre + avv_re * (double)N;
-            xim = xim + avv_im * (double)N;
-          }
-          x = C{(T)xre, (T)xim};
-        }
-      }
-      v[t * R + q] = (z < N) ? cmul(x, chirp[z]) : C{(T)0, (T)0};
+      v[t * R + q] = (z < N) ? cmul(col_input(d, col, z, k0, u, sz_row), chirp[z]) : C{(T)0, (T)0};
     }
   chirpz_block<M, C>(v, fbuf, d, tid);
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;

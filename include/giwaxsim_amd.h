@@ -297,8 +297,15 @@ int gx_scale_offset_moments(const double *sim, const double *exp_, const uint8_t
  * u0 = u // 1, tu = u - u0 (likewise v).  Everything after the deposit (FFT,
  * |F|^2, binning with counts, mean, f0, detector gather, mirror, floor) is the
  * forward path's, and every buffer of gx_frames keeps its meaning.
- * Requirements: power-of-two N, fill_bkg = 0, smooth = 0, every atom's
- * u0 + 1 and v0 + 1 inside the grid, and d->Z > the largest v0 + 1.
+ * fill_bkg / smooth act on the deposited grid S exactly as in the forward path;
+ * almost everywhere in the positions the frame grid is then
+ *   G = M S + avg_voxel_f p,  M = (mz inz) x (my iny),  p = Pz x Py + max_voxels
+ * with real masks (smoothed edges my, mz; filled region iny, inz) and a real
+ * pattern p, so dL/dS = M dL/dG and dL/d avg_voxel_f = sum_k dL/dF_k conj(FFT2(p)_k);
+ * avg_voxel_f = sum f r^3 / (x_bound y_bound z_bound) passes that on to f and to
+ * the six atoms that fix the bounding box.
+ * Requirements: power-of-two N, every atom's u0 + 1 and v0 + 1 inside the grid,
+ * and d->Z > the largest v0 + 1.
  * ---------------------------------------------------------------------------------- */
 #define GX_ADJ_RED_BLOCKS 16
 typedef struct gx_adjoint {
@@ -316,9 +323,18 @@ typedef struct gx_adjoint {
   const double *g_bins;    /* [c^3] dL / d(bin sum of I) (gx_bins_adjoint) */
   double *g_part;          /* [batch][4][n_atoms] per-frame partials: dL/dy', dL/dz, dL/df re, im */
   double *red;             /* [max(batch, 2 (1 + n_elem))][GX_ADJ_RED_BLOCKS] reduction scratch */
-  int32_t *amin;           /* [batch + 1] argmin atoms (original index) */
+  int32_t *amin;           /* [batch + 7] argmin / argmax atoms (original index) */
   double *g_atom;          /* [5][n_atoms] accumulated dL/dx, dL/dy, dL/dz, dL/df re, im per
                               regrouped atom (zeroed by gx_adj_prepare) */
+  /* fill_bkg / smooth only (may be NULL otherwise) */
+  const double *x;         /* [n_atoms] original x */
+  const double *y;         /* [n_atoms] original y */
+  const double *ext;       /* [6] xmin, xmax, ymin, ymax, zmin, zmax (gx_slab_bounds) */
+  const int64_t *ecount;   /* [n_elem] atoms per element (gx_slab_bounds) */
+  double vol_r3;           /* r^3 / (x_bound y_bound z_bound) */
+  void *avg_spec;          /* [batch][N + W] double2: FFT of Pz (all k), FFT of Py (kept columns) */
+  double *avg_part;        /* [batch][W][2] per-column share of dL/d avg_voxel_f */
+  double *g_avg;           /* [2] dL/d avg_voxel_f (re, im), accumulated (zeroed by gx_adj_prepare) */
 } gx_adjoint;
 
 /* Deposit exponents for the current f_table, inverse permutation, zero g_atom. */
@@ -326,7 +342,7 @@ int gx_adj_prepare(const gx_frames *d, const gx_adjoint *g, gx_stream_t stream);
 /* Cloud-in-cell deposit + FFT along y of frames [f0, f0+nf) into d->inter (needs
  * gx_frames_stats of the same frames first). */
 int gx_adj_rowpass(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
-/* stats + gx_adj_rowpass + gx_frames_colpass: the variant's forward. */
+/* stats + columns + gx_adj_rowpass + gx_frames_colpass: the variant's forward. */
 int gx_adj_forward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
 /* Adjoint of the column pass, in place on d->inter: FFT along z of the recomputed
  * column, cotangent 2 g_bins[bin] F on binned pixels, adjoint DFT back to z. */
@@ -336,11 +352,13 @@ int gx_adj_rowgrad(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t 
 /* Fold g_part of frames [f0, f0+nf) into g_atom (chain rule through
  * y' = x sin phi + y cos phi and the per-frame origin min y'). */
 int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
-/* rowpass (recomputed) + colpass adjoint + rowgrad + reduce.  d->ymin_key must hold
- * the statistics of the same frames and atoms (gx_frames_stats, run by gx_adj_forward). */
+/* columns + rowpass (recomputed) + colpass adjoint + rowgrad + reduce.  d->ymin_key and
+ * d->idx_bounds must hold the statistics of the same frames and atoms (gx_frames_stats,
+ * run by gx_adj_forward). */
 int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
-/* Origin min z term, per-element f gradients and the coordinate gradients in the
- * original atom order: g_coords [n_atoms][3], g_f [n_elem][2]. */
+/* Origin min z term, the avg_voxel_f terms (fill_bkg / smooth), per-element f
+ * gradients and the coordinate gradients in the original atom order:
+ * g_coords [n_atoms][3], g_f [n_elem][2]. */
 int gx_adj_finish(const gx_frames *d, const gx_adjoint *g, double *g_coords, double *g_f, gx_stream_t stream);
 
 /* Adjoint of gx_bins_finalize for the FP64 iq: g_bins = g_iq * fn^2 / count

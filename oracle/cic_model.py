@@ -103,11 +103,12 @@ class CicModel:
     thetas, theta_weights, mirror)."""
 
     def __init__(self, coords0, elem_ids, r_voxel_size, q_voxel_size, max_q, detector,
-                 frames=None, deposit='cic'):
+                 frames=None, deposit='cic', fill_bkg=False, smooth=0):
         plan = O.voxelgrid_plan(np.asarray(coords0), r_voxel_size, q_voxel_size, max_q)
         self.N = N = plan['grid_size']
         self.r = float(r_voxel_size)
         self.deposit = deposit
+        self.fill, self.smooth = bool(fill_bkg), int(smooth)
         q_axis = plan['q_axis']
         voxel = np.abs(q_axis[1] - q_axis[0])
         keep = np.where(np.abs(q_axis) < max_q + voxel)[0]
@@ -124,7 +125,7 @@ class CicModel:
             rad = np.radians(phi)
             pix, b = _frame_bin_map(N, r_voxel_size, phi, q_axis, self.c_lo, self.c_n)
             np.add.at(count, b, 1.0)
-            self.maps.append((np.sin(rad), np.cos(rad), torch.as_tensor(pix), torch.as_tensor(b)))
+            self.maps.append((phi, np.sin(rad), np.cos(rad), torch.as_tensor(pix), torch.as_tensor(b)))
         self.count = torch.as_tensor(count)
         ones = np.ones((self.c_n,) * 3)
         qc = self.q_crop
@@ -148,8 +149,9 @@ class CicModel:
         self.mirror = bool(d.get('mirror', True))
 
     # ----------------------------------------------------------------------------
-    def grid(self, coords, f_re, f_im, s, c):
-        """One frame's deposited grid (real, imag), each [N*N] indexed z*N + y."""
+    def grid(self, coords, f_re, f_im, s, c, phi=0.0):
+        """One frame's grid (real, imag), each [N*N] indexed z*N + y: the deposit,
+        then fill_bkg and smoothing as frame_grid (tools/voxelgrids.py:343-379)."""
         N, r = self.N, self.r
         x, y, z = coords[:, 0], coords[:, 1], coords[:, 2]
         yp = x * s + y * c
@@ -175,19 +177,53 @@ class CicModel:
                 w = wv * wu
                 gr = gr.index_add(0, node, fr * w)
                 gi = gi.index_add(0, node, fi * w)
-        return gr, gi
+        if not (self.fill or self.smooth):
+            return gr, gi
+        # avg_voxel_f over the bounding box (tools/comparison.py:741-744), differentiable
+        xb = torch.max(x) - torch.min(x)
+        yb = torch.max(y) - torch.min(y)
+        zb = torch.max(z) - torch.min(z)
+        vol = xb * yb * zb
+        avg = torch.complex(torch.sum(fr) / vol * r ** 3, torch.sum(fi) / vol * r ** 3)
+        xbn, ybn = float(xb.detach()), float(yb.detach())
+        mv = np.sqrt(np.float64(xbn) ** 2 + np.float64(ybn) ** 2) // r
+        y_min, y_max = int(u0.min()), int(u0.max())
+        z_min, z_max = int(v0.min()), int(v0.max())
+        G = torch.complex(gr, gi).reshape(N, N)
+        A = avg * mv
+        if self.fill:
+            L = O.rectangular_collapse_lengths(np.arange(N) * r, ybn, xbn, phi)
+            nm = torch.as_tensor((mv - O.floor_div(L, r)).astype(int).astype(np.float64))
+            in_z = np.zeros(N)
+            in_z[z_min:z_max] = 1
+            in_y = np.zeros(N)
+            in_y[y_min + 1:y_max] = 1
+            inside = torch.as_tensor(np.outer(in_z, in_y))
+            G = inside * (G + nm[None, :] * avg) + (1 - inside) * A
+        if self.smooth:
+            from scipy.ndimage import gaussian_filter1d
+            sg = self.smooth
+            mask_y = np.zeros(N)
+            mask_z = np.zeros(N)
+            mask_y[y_min + sg:y_max - sg] = 1
+            mask_z[z_min + sg:z_max - sg] = 1
+            m = torch.as_tensor(np.outer(gaussian_filter1d(mask_z, sigma=sg, mode='wrap'),
+                                         gaussian_filter1d(mask_y, sigma=sg, mode='wrap')))
+            G = G * m + A * (1 - m)
+        G = G.reshape(-1)
+        return G.real, G.imag
 
-    def frame_intensity(self, coords, f_re, f_im, s, c):
+    def frame_intensity(self, coords, f_re, f_im, s, c, phi=0.0):
         """|FFT2|^2 of one frame, [N*N] unshifted [kz, ky]."""
-        gr, gi = self.grid(coords, f_re, f_im, s, c)
+        gr, gi = self.grid(coords, f_re, f_im, s, c, phi)
         F = torch.fft.fft2(torch.complex(gr, gi).reshape(self.N, self.N))
         return (F.real ** 2 + F.imag ** 2).reshape(-1)
 
     def iq(self, coords, f_re, f_im):
         """Cropped, f0-weighted mean I(q), flat [c^3] ([qy, qx, qz])."""
         total = torch.zeros(self.c_n ** 3, dtype=torch.float64)
-        for s, c, pix, b in self.maps:
-            I = self.frame_intensity(coords, f_re, f_im, s, c)
+        for phi, s, c, pix, b in self.maps:
+            I = self.frame_intensity(coords, f_re, f_im, s, c, phi)
             total = total.index_add(0, b, I[pix])
         cnt = self.count
         mean = torch.where(cnt > 0, total / torch.clamp(cnt, min=1.0), torch.zeros_like(total))

@@ -2,7 +2,7 @@
 
 Checks, through the C ABI (giwaxsim_amd.adjoint):
 * forward image vs the FP64 autograd reference model (oracle/cic_model.py) on
-  the same inputs: max |diff| / max |image| <= 1e-12 with FP64 transforms,
+  the same inputs, plain and with fill_bkg and/or smoothing: max |diff| / max |image| <= 1e-12 with FP64 transforms,
   <= 2e-6 with FP32;
 * gradients dL/d coords, dL/d f_re, dL/d f_im of L = <G, image> vs the
   reference's torch.autograd: max |diff| / max |grad| <= 1e-11 (FP64),
@@ -34,7 +34,7 @@ R = 0.25
 MAXQ = 2.5
 
 
-def _small(precision):
+def _small(precision, fill=False, smooth=0):
     coords, el = graphite_slab(size=(16.0, 16.0, 12.0), seed=21)
     # a second element so per-element gradients are exercised
     el = el.copy()
@@ -44,9 +44,10 @@ def _small(precision):
     det = dict(num_pixels=48, max_q=MAXQ, psis=list(np.arange(0.0, 91.0, 10.0)), phis=[0.0],
                thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
     frames = (0, 10)
-    model = DifferentiableSlab(el, coords, R, qv, MAXQ, det, frames=frames, precision=precision)
+    model = DifferentiableSlab(el, coords, R, qv, MAXQ, det, frames=frames, precision=precision,
+                               fill_bkg=fill, smooth=smooth)
     ids = np.array([model.elements.index(e) for e in el])
-    ref = CicModel(coords, ids, R, qv, MAXQ, det, frames=frames)
+    ref = CicModel(coords, ids, R, qv, MAXQ, det, frames=frames, fill_bkg=fill, smooth=smooth)
     return coords, model, ref
 
 
@@ -55,10 +56,14 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("precision,tol_img,tol_grad", [('fp64', 1e-12, 1e-11), ('fp32', 2e-6, 1e-4)])
-def test_forward_and_gradients_match_autograd(precision, tol_img, tol_grad):
-    _say(f"[gpu-adjoint] small graphite N=256 frames 0..10 precision={precision}")
-    coords, model, ref = _small(precision)
+@pytest.mark.parametrize("precision,fill,smooth,tol_img,tol_grad",
+                         [('fp64', False, 0, 1e-12, 1e-11), ('fp32', False, 0, 2e-6, 1e-4),
+                          ('fp64', True, 0, 1e-12, 1e-11), ('fp64', False, 3, 1e-12, 1e-11),
+                          ('fp64', True, 3, 1e-12, 1e-11), ('fp32', True, 3, 2e-6, 1e-4)])
+def test_forward_and_gradients_match_autograd(precision, fill, smooth, tol_img, tol_grad):
+    _say(f"[gpu-adjoint] small graphite N=256 frames 0..10 precision={precision} fill={fill} "
+         f"smooth={smooth}")
+    coords, model, ref = _small(precision, fill, smooth)
     dev = torch.device('cuda')
     G = np.random.default_rng(1).normal(size=(48, 48))
     # HIP
@@ -84,9 +89,11 @@ def test_forward_and_gradients_match_autograd(precision, tol_img, tol_grad):
     assert e_c <= tol_grad and e_fr <= tol_grad and e_fi <= tol_grad
 
 
-def test_invariances_and_determinism():
-    _say("[gpu-adjoint] translation invariance, Euler identity in f, determinism")
-    coords, model, _ = _small('fp64')
+@pytest.mark.parametrize("fill,smooth", [(False, 0), (True, 3)])
+def test_invariances_and_determinism(fill, smooth):
+    _say(f"[gpu-adjoint] translation invariance, Euler identity in f, determinism fill={fill} "
+         f"smooth={smooth}")
+    coords, model, _ = _small('fp64', fill, smooth)
     dev = torch.device('cuda')
     G = torch.as_tensor(np.random.default_rng(2).normal(size=(48, 48)), device=dev)
     grads = []

@@ -10,7 +10,9 @@
   |fA + fB Wv(kz) Wu(ky)|^2 with W(k) = (1 - t) e^{-2 pi i k n0 / N} + t e^{-2 pi i k (n0 + 1) / N},
   and its derivative in B's fractional position is checked against the
   model's autograd;
-* torch.autograd of the full model against FP64 central differences.
+* torch.autograd of the full model (also with fill_bkg and smoothing, including
+  the bounding-box atoms that carry the avg_voxel_f term) against FP64 central
+  differences.
 """
 
 import numpy as np
@@ -55,6 +57,23 @@ def test_ngp_mode_reproduces_oracle():
     assert np.allclose(det, want, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("fill,smooth", [(True, 0), (False, 2), (True, 2)])
+def test_ngp_mode_reproduces_oracle_fill_smooth(fill, smooth):
+    coords, ids = _slab()
+    f = np.array([6.0 + 0.0159, 0.0093])
+    frames = (2, 6)
+    model = CicModel(coords, ids, R, QV, MAXQ, DET, frames=frames, deposit='ngp', fill_bkg=fill,
+                     smooth=smooth)
+    fr = torch.tensor([f[0]], dtype=torch.float64)
+    fi = torch.tensor([f[1]], dtype=torch.float64)
+    iq = model.iq(torch.as_tensor(coords), fr, fi).numpy().reshape((model.c_n,) * 3)
+    plan = O.voxelgrid_plan(coords, R, QV, MAXQ)
+    fvals = np.full(len(coords), f[0] + 1j * f[1])
+    ref = O.voxelgridmaker(coords, fvals, R, QV, MAXQ, fill_bkg=fill, smooth=smooth,
+                           phis=plan['phis'][frames[0]:frames[1]])
+    assert np.allclose(iq, ref['iq'], rtol=1e-10, atol=1e-12 * ref['iq'].max())
+
+
 def _two_atoms(uB, vB):
     """Atom A at the origin (it is min y' and min z), B at grid coordinates
     (uB, vB) in the phi = 0 frame (y' = y)."""
@@ -95,9 +114,10 @@ def test_two_atom_closed_form(uB, vB):
     assert abs(got - ref) <= 1e-9 * abs(ref)
 
 
-def test_autograd_matches_central_differences():
+@pytest.mark.parametrize("fill,smooth", [(False, 0), (True, 2)])
+def test_autograd_matches_central_differences(fill, smooth):
     coords, ids = _slab()
-    model = CicModel(coords, ids, R, QV, MAXQ, DET, frames=(3, 6))
+    model = CicModel(coords, ids, R, QV, MAXQ, DET, frames=(3, 6), fill_bkg=fill, smooth=smooth)
     rng = np.random.default_rng(5)
     G = torch.as_tensor(rng.normal(size=(DET['num_pixels'],) * 2))
     fr = torch.tensor([6.0159], dtype=torch.float64, requires_grad=True)
@@ -112,7 +132,10 @@ def test_autograd_matches_central_differences():
             return float((model(torch.as_tensor(c), torch.tensor([a], dtype=torch.float64),
                                 torch.tensor([b], dtype=torch.float64)) * G).sum())
 
-    picks = rng.choice(len(coords) * 3, size=12, replace=False)
+    picks = list(rng.choice(len(coords) * 3, size=12, replace=False))
+    if fill:  # the atoms that fix the bounding box carry the avg_voxel_f term
+        picks += [3 * int(np.argmax(coords[:, 0])), 3 * int(np.argmin(coords[:, 1])) + 1,
+                  3 * int(np.argmax(coords[:, 2])) + 2]
     gmax = t.grad.abs().max().item()
     for p in picks:
         i, k = divmod(int(p), 3)
