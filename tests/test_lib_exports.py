@@ -51,7 +51,7 @@ def test_argument_errors_are_reported():
     with pytest.raises(_lib.GxError, match='not 256/512/1024/2048'):
         _lib.call('gx_adj_rowpass', ctypes.byref(d), ctypes.byref(a), 0, 1, None)
     d = _lib.GxFrames(N=512, r=0.2, fill_bkg=1)
-    with pytest.raises(_lib.GxError, match='fill_bkg and smooth must be off'):
+    with pytest.raises(_lib.GxError, match='fill_bkg needs col_nm'):
         _lib.call('gx_adj_backward', ctypes.byref(d), ctypes.byref(a), 0, 1, None)
     with pytest.raises(_lib.GxError, match='crop window'):
         _lib.call('gx_bins_finalize', ctypes.c_void_p(8), ctypes.c_void_p(8), 10, 1.0, 5, 12, 0.0,
