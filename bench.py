@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--kernel-iters", type=int, default=20)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble (64 slabs) and large-slab (N=2048) figures")
-    ap.add_argument("--only", choices=["adjoint", "adjoint_fp64", "ensemble", "large_slab"],
+    ap.add_argument("--only", choices=["adjoint", "adjoint_fp64", "adjoint_polymer", "adjoint_large",
+                                       "ensemble", "large_slab"],
                     help="run one of the extra workloads alone and print its JSON (for profiling)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="one step, then one 256 MiB device copy (counter calibration) and one "
@@ -81,7 +82,11 @@ def main():
     from giwaxsim_amd._lib import call
 
     if args.only:
-        if args.only.startswith("adjoint"):
+        if args.only == "adjoint_polymer":
+            print(json.dumps(adjoint_polymer_config(engine, torch, reps=args.steps)))
+        elif args.only == "adjoint_large":
+            print(json.dumps(adjoint_large_config(engine, torch, n_slabs=args.steps)))
+        elif args.only.startswith("adjoint"):
             prec = "fp64" if args.only == "adjoint_fp64" else "fp32"
             print(json.dumps(adjoint_config(engine, torch, reps=args.steps, precision=prec)))
         else:
@@ -225,6 +230,30 @@ def main():
     print(json.dumps(out))
 
 
+def _time_adjoint(model, coords, engine, torch, reps):
+    """(forward ms, forward + backward ms) of a DifferentiableSlab from device-resident
+    inputs, HIP events on the stream the work is enqueued on."""
+    ct = torch.as_tensor(coords).cuda()
+    fr, adj = model.descriptors(ct, model.f_re0, model.f_im0)
+    st = engine.stream_handle()
+    G = torch.randn(model.n_pix ** 2, dtype=torch.float64, device='cuda')
+    model.forward_enqueue(fr, adj, st)
+    model.backward_enqueue(fr, adj, G, st)
+    torch.cuda.synchronize()
+    cs = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(cs)
+    for _ in range(reps):
+        model.forward_enqueue(fr, adj, st)
+    ev[1].record(cs)
+    for _ in range(reps):
+        model.forward_enqueue(fr, adj, st)
+        model.backward_enqueue(fr, adj, G, st)
+    ev[2].record(cs)
+    torch.cuda.synchronize()
+    return ev[0].elapsed_time(ev[1]) / reps, ev[1].elapsed_time(ev[2]) / reps
+
+
 def adjoint_config(engine, torch, reps=10, precision='fp32'):
     """Differentiation workload: graphite 40 x 40 x 30 A (5.6k atoms), r 0.25 A,
     N = 512, 453 phi-frames, max_q 2.5, 256^2 detector with psi 0..90 step 2
@@ -241,26 +270,8 @@ def adjoint_config(engine, torch, reps=10, precision='fp32'):
     det = dict(num_pixels=256, max_q=mq, psis=list(np.arange(0.0, 91.0, 2.0)), phis=[0.0],
                thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
     model = DifferentiableSlab(eg, cg, r, qv, mq, det, precision=precision)
+    fwd, fb = _time_adjoint(model, cg, engine, torch, reps)
     ct = torch.as_tensor(cg).cuda()
-    fr, adj = model.descriptors(ct, model.f_re0, model.f_im0)
-    st = engine.stream_handle()
-    G = torch.randn(256 * 256, dtype=torch.float64, device='cuda')
-    model.forward_enqueue(fr, adj, st)
-    model.backward_enqueue(fr, adj, G, st)
-    torch.cuda.synchronize()
-    cs = torch.cuda.current_stream()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    ev[0].record(cs)
-    for _ in range(reps):
-        model.forward_enqueue(fr, adj, st)
-    ev[1].record(cs)
-    for _ in range(reps):
-        model.forward_enqueue(fr, adj, st)
-        model.backward_enqueue(fr, adj, G, st)
-    ev[2].record(cs)
-    torch.cuda.synchronize()
-    fwd = ev[0].elapsed_time(ev[1]) / reps
-    fb = ev[1].elapsed_time(ev[2]) / reps
     # the reference-parity (nearest-grid-point) forward of the same slab and detector
     ids, ftab, _, _ = engine.element_table(eg, 12700.0)
     idd = torch.as_tensor(ids).cuda()
@@ -287,6 +298,63 @@ def adjoint_config(engine, torch, reps=10, precision='fp32'):
             "fwd_bwd_over_fwd": fb / fwd, "reference_parity_fwd_ms": ngp_ms,
             "fwd_bwd_over_reference_parity_fwd": fb / ngp_ms,
             "fwd_phi_frames_per_s": F / (fwd * 1e-3), "fwd_bwd_phi_frames_per_s": F / (fb * 1e-3)}
+
+
+def adjoint_polymer_config(engine, torch, reps=3):
+    """The production-size workload through the differentiable variant: polymer
+    slab 100^3 A (110k atoms), r 0.2 A, N = 1024, fill_bkg, smooth 10 voxels,
+    max_q 2.0 (581 frames), 512^2 detector, psi 0..180 step 0.5 (361
+    orientations), 0.15 deg psi tilt; FP32 transforms."""
+    from giwaxsim_amd.adjoint import DifferentiableSlab
+    W = workload()
+    det = dict(num_pixels=512, max_q=W['max_q'], psis=list(np.linspace(0, 180, 361)), phis=[0.0],
+               thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
+    model = DifferentiableSlab(W['elements'], W['coords'], W['r'], W['qv'], W['max_q'], det,
+                               fill_bkg=W['fill'], smooth=W['smooth'])
+    fwd, fb = _time_adjoint(model, W['coords'], engine, torch, reps)
+    F = model.plan.n_frames
+    return {"workload": "polymer slab 100^3 A, 110k atoms, N=1024, 581 phi-frames, fill_bkg, smooth 10, "
+                        "512^2 detector, psi 0..180 step 0.5; cloud-in-cell deposit, FP32 transforms; "
+                        "gradients w.r.t. all coordinates and per-element f",
+            "n_atoms": len(W['coords']), "frames": F, "batch": model.batch, "fwd_ms": fwd,
+            "fwd_bwd_ms": fb, "bwd_ms": fb - fwd, "fwd_bwd_over_fwd": fb / fwd,
+            "fwd_phi_frames_per_s": F / (fwd * 1e-3), "fwd_bwd_phi_frames_per_s": F / (fb * 1e-3)}
+
+
+def adjoint_large_config(engine, torch, n_slabs=16):
+    """Large slabs through the differentiable variant: n_slabs polymer slabs of
+    200 x 200 x 60 A (~1.3M atoms each, seeds 0..n-1: seeded crystallite
+    orientations), r 0.15 A, N = 2048, fill_bkg, smooth 20 voxels (3 A), max_q
+    3.0 (1307 frames), 2048^2 detector, psi 0..180 step 0.25 (721 orientations).
+    Per slab: forward, then backward w.r.t. the coordinates (frames recomputed in
+    batches, never stored).  Slabs run one after another on one model's
+    buffers' worth of memory."""
+    from giwaxsim_amd.adjoint import DifferentiableSlab
+    from giwaxsim_amd.synth import polymer_slab
+    r, mq = 0.15, 3.0
+    qv = 2 * np.pi / (r * 2048) * (1 + 1e-9)
+    det = dict(num_pixels=2048, max_q=mq, psis=list(np.linspace(0, 180, 721)), phis=[0.0],
+               thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
+    fwd_tot = fb_tot = 0.0
+    n_atoms = 0
+    F = 0
+    peak = 0
+    for i in range(n_slabs):
+        c, e = polymer_slab(size=(200.0, 200.0, 60.0), seed=i, density=0.54)
+        model = DifferentiableSlab(e, c, r, qv, mq, det, fill_bkg=True, smooth=20)
+        fwd, fb = _time_adjoint(model, c, engine, torch, 1)
+        fwd_tot += fwd
+        fb_tot += fb
+        n_atoms += len(c)
+        F = model.plan.n_frames
+        peak = max(peak, torch.cuda.max_memory_allocated())
+        del model
+    return {"workload": f"{n_slabs} polymer slabs 200x200x60 A (~1.3M atoms each), N=2048, fill_bkg, "
+                        "smooth 20, max_q 3.0, 2048^2 detector psi step 0.25; forward + backward "
+                        "w.r.t. coordinates, frames recomputed",
+            "n_slabs": n_slabs, "n_atoms": n_atoms, "frames_per_slab": F,
+            "fwd_ms_total": fwd_tot, "fwd_bwd_ms_total": fb_tot, "fwd_bwd_over_fwd": fb_tot / fwd_tot,
+            "fwd_bwd_ms_per_slab": fb_tot / n_slabs, "peak_hbm_gb": peak / 1e9}
 
 
 def extra_configs(args, engine, torch, which=None):

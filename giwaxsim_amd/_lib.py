@@ -38,7 +38,7 @@ class GxSlab(ctypes.Structure):
         ("f_table", c_void_p),
         ("sx", c_void_p), ("sy", c_void_p), ("sq", c_void_p), ("szrow", c_void_p),
         ("row_off", c_void_p), ("row_exp", c_void_p), ("work", c_void_p),
-        ("szf", c_void_p), ("sperm", c_void_p), ("selem", c_void_p),
+        ("szf", c_void_p), ("sperm", c_void_p), ("selem", c_void_p), ("stable_work", c_void_p),
     ]
 
 
@@ -76,6 +76,7 @@ class GxDetector(ctypes.Structure):
 
 
 GX_ADJ_RED_BLOCKS = 16
+GX_STABLE_CHUNK = 4096
 
 
 class GxAdjoint(ctypes.Structure):
@@ -121,8 +122,8 @@ SIGNATURES = {
     "gx_adj_reduce": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
     "gx_adj_backward": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
     "gx_adj_finish": [POINTER(GxFrames), POINTER(GxAdjoint), _P, _P, _P],
-    "gx_bins_adjoint": [_P, _P, _P, _I, _I, _I, _D, _D, _D, POINTER(c_double), _D, _P, _P],
-    "gx_detector_adjoint": [POINTER(GxDetector), _P, _I, _P, _P, _P, _P],
+    "gx_bins_adjoint": [_P, _I, _P, _P, _I, _I, _I, _D, _D, _D, POINTER(c_double), _D, _P, _P],
+    "gx_detector_adjoint": [POINTER(GxDetector), _P, _I, _P, _P, _I, _P, _P],
 }
 EXPORTS = ["gx_last_error", "gx_abi_version", "gx_supported_grid"] + list(SIGNATURES)
 

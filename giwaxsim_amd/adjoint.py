@@ -123,6 +123,7 @@ class DifferentiableSlab:
         self.szf = _empty(n, torch.float64)
         self.sperm = _empty(n, torch.int32)
         self.selem = _empty(n, torch.int32)
+        self.stable_work = _empty(-(-n // _lib.GX_STABLE_CHUNK) * (N + 1), torch.int32)
         self.minmax = _empty(6, torch.float64)
         self.ecount = _empty(self.n_elem, torch.int64)
 
@@ -147,7 +148,7 @@ class DifferentiableSlab:
         self.adj_row_exp = _empty(N, torch.int32)
         self.ainv = _empty(n, torch.int32)
         self.g_bins = _empty(nb, torch.float64)
-        self.red = _empty(max(B, 2 * (1 + self.n_elem)) * _lib.GX_ADJ_RED_BLOCKS, torch.float64)
+        self.red = _empty(max(B * (N + 1), 2 * (1 + self.n_elem) * _lib.GX_ADJ_RED_BLOCKS), torch.float64)
         self.amin = _empty(B + 7, torch.int32)
         # fill_bkg / smooth tables (per batch) and the background-level adjoint
         self.col_nm = _empty(B * N, torch.int32) if self.fill else None
@@ -187,7 +188,9 @@ class DifferentiableSlab:
         self.det_raw = _empty(npx * npx, torch.float64)
         self.det_out = _empty(npx * npx, torch.float64)
         self.det_work = _empty(2 * npx * npx + (npx * npx + 255) // 256, torch.float64)
-        self.g_fx = _empty(nb, torch.int64)
+        # copies of the detector adjoint's accumulator (<= 256 MB)
+        self.n_rep = int(max(1, min(8, 256e6 // (8 * nb))))
+        self.g_fx = _empty(self.n_rep * nb, torch.int64)
         self.fx_scale = _empty(1, torch.float64)
         ax = g.crop_axis
         self.detector_desc = GxDetector(
@@ -215,15 +218,15 @@ class DifferentiableSlab:
         B = self._batch_arg
         if B is None:
             cap_inter = 512e6 // (W * Zc * esz)
-            cap_part = 2e9 // (32 * max(n, 1))
-            B = max(1, min(nf, int(cap_inter), int(cap_part)))
+            cap_part = 2e9 // (4 * esz // 2 * max(n, 1))
+            B = max(1, min(nf, int(cap_inter), int(cap_part), 8192))
         if getattr(self, 'batch', None) not in (None, B):
             raise RuntimeError("internal: batch size changed")
         self.batch = int(B)
         self.Z_cap = Zc
         cdt = torch.float64 if self.f64 else torch.float32
         self.inter = _empty(self.batch * W * Zc * 2, cdt)
-        self.g_part = _empty(self.batch * 4 * n, torch.float64)
+        self.g_part = _empty(self.batch * 4 * n, torch.float64 if self.f64 else torch.float32)
 
     def batches(self, fr):
         """(f0, nf, descriptor) per batch: the descriptor's statistics pointers
@@ -276,7 +279,7 @@ class DifferentiableSlab:
                       szrow=ptr(self.szrow).value, row_off=ptr(self.row_off).value,
                       row_exp=ptr(self.row_exp).value, work=ptr(self.work).value,
                       szf=ptr(self.szf).value, sperm=ptr(self.sperm).value,
-                      selem=ptr(self.selem).value)
+                      selem=ptr(self.selem).value, stable_work=ptr(self.stable_work).value)
         call("gx_slab_sort_rows", ctypes.byref(slab), st)
         ftab = ft.cpu().numpy()
         fc = ftab[:, 0] + 1j * ftab[:, 1]
@@ -369,10 +372,10 @@ class DifferentiableSlab:
         """Enqueue the whole adjoint for the image cotangent ``g_out`` [n_pix^2]."""
         g = self.plan.grid
         call("gx_detector_adjoint", ctypes.byref(self.detector_desc), ptr(g_out), self._flags(),
-             ptr(self.det_work), ptr(self.g_fx), ptr(self.fx_scale), st)
+             ptr(self.det_work), ptr(self.g_fx), self.n_rep, ptr(self.fx_scale), st)
         start, step, stop = g.linspace_params
         aff = _lib.dbl_array(aff_dict['C'])
-        call("gx_bins_adjoint", ptr(self.g_fx), ptr(self.fx_scale), ptr(self.bin_cnt), g.c_n, g.c_lo,
+        call("gx_bins_adjoint", ptr(self.g_fx), self.n_rep, ptr(self.fx_scale), ptr(self.bin_cnt), g.c_n, g.c_lo,
              g.q_num, start, step, stop, aff, float(ptable['C']), ptr(self.g_bins), st)
         call("gx_adj_prepare", ctypes.byref(fr), ctypes.byref(adj), st)
         for f0, nf, d in self.batches(fr):

@@ -8,9 +8,9 @@
 //
 // Forward per batch of frames:
 //   gx_frames_stats   min y' per frame (origin of u)
-//   k_adj_rowpass<N>  per z row v: atoms of rows v-1 (weight tv) and v (weight 1-tv),
-//                     bilinear in u, exact int64 fixed-point LDS sums, FFT along y,
-//                     kept column band staged and stored transposed (d.inter)
+//   k_adj_rowpass<N>  per block of z rows: atoms of rows v-1 (weight tv) and v (weight 1-tv),
+//                     bilinear in u, exact fixed-point LDS sums, FFT along y, kept column
+//                     band staged and stored transposed (d.inter)
 //   k_colpass<N>      (forward path) FFT along z, |F|^2, bins + counts
 // Backward per batch (frames recomputed, never stored; the forward's per-frame
 // min y' in d.ymin_key is reused):
@@ -19,12 +19,13 @@
 //                     binned pixels, adjoint DFT (conj FFT conj) back to z, in place
 //   k_adj_rowgrad<N>  per block of rows (+1 halo row): adjoint DFT along y of the kept
 //                     band, then every atom of the rows gathers its four nodes:
-//                     dL/du, dL/dv, dL/df per (frame, atom) into g_part
-//   k_adj_frame_sums  per frame: sum_a dL/dy'_a in original atom order, argmin y'
-//   k_adj_frame_origin origin term: the argmin atom of min y' takes -sum_a dL/dy'_a
-//   k_adj_reduce      g_atom += sum_f (sin, cos) dL/dy', dL/dz, dL/df over the batch
-// Sums over atoms run in original atom order with fixed block partitions, and
-// the row deposit is exact, so every result is bitwise reproducible.
+//                     dL/dy', dL/dz, dL/df per (frame, atom) into g_part
+//                     and each block's share of sum_a dL/dy'_a and the argmin of y'
+//   k_adj_reduce      g_atom += sum_f (sin, cos) dL/dy', dL/dz, dL/df; the argmin atom
+//                     of min y' also takes -sum_a dL/dy'_a (origin term)
+// The regrouping is stable (original order within a row), sums over atoms run in
+// that order with fixed partitions, and the row deposit is exact, so every result
+// is bitwise reproducible.
 #include "gx_device.h"
 #include "gx_host.h"
 
@@ -54,11 +55,37 @@ __device__ __forceinline__ C cconj(C a) {
   return C{a.x, -a.y};
 }
 
+// block-wide reductions in a fixed order (wave shuffles, then the waves in order)
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, GX_WAVE);
+  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int i = 0; i < NWAVE; ++i) r += sh[i];
+  return r;
+}
+
+__device__ __forceinline__ int block_min_int(int v, int *sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, GX_WAVE));
+  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  int r = INT_MAX;
+  for (int i = 0; i < NWAVE; ++i) r = min(r, sh[i]);
+  return r;
+}
+
 // ------------------------------------------------------------------------------------
 // preparation
 // ------------------------------------------------------------------------------------
 // Deposit exponent of row v: the row receives f w from atoms of rows v-1 and v with
-// w <= 1, so 2^S sum |f| + (rounding) stays below 2^62.
+// w <= 1, so 2^S sum |f| + (rounding) stays below 2^62 (int64 accumulators, FP64
+// transforms) or 2^31 - 1 (int32, FP32 transforms).
 __global__ __launch_bounds__(BLK) void k_adj_rowexp(gx_frames d, gx_adjoint g) {
   const int N = (int)d.N;
   for (int v = blockIdx.x * BLK + threadIdx.x; v < N; v += gridDim.x * BLK) {
@@ -73,7 +100,8 @@ __global__ __launch_bounds__(BLK) void k_adj_rowexp(gx_frames d, gx_adjoint g) {
     }
     int S = 0;
     if (bound > 0.0) {
-      S = ilogb((4611686018427387904.0 - 2.0 * cnt - 2.0) / bound);
+      const double room = d.fft_f64 ? 4611686018427387904.0 : 2147483647.0;  // int64 / int32 accumulators
+      S = ilogb((room - 2.0 * cnt - 2.0) / bound);
       if (S > 200) S = 200;
       if (S < -200) S = -200;
     }
@@ -101,22 +129,40 @@ __global__ __launch_bounds__(BLK) void k_fill_i32(int32_t *p, int64_t n, int32_t
 // ------------------------------------------------------------------------------------
 // forward row pass with the cloud-in-cell deposit
 // ------------------------------------------------------------------------------------
-template <int N, int RPW, typename C>
+// Accumulator of the fixed-point deposit: int32 pairs under FP32 transforms (the
+// quantum, 2^-31 of the row bound, is below FP32 rounding), int64 under FP64.  A
+// row's accumulators take exactly N * sizeof(C) bytes, the FFT buffer of the row.
+template <typename C> struct DepositAcc;
+template <> struct DepositAcc<float2> { using T = int32_t; using U = unsigned int; static constexpr int BITS = 31; };
+template <> struct DepositAcc<double2> { using T = long long; using U = unsigned long long; static constexpr int BITS = 62; };
+
+// Rows per block of the differentiable row pass: RB rows of accumulators in 32 KB of
+// LDS (at most 8), so that three blocks share a CU.
+template <int N, typename C>
+struct RowPassShape {
+  static constexpr int FIT = 32768 / (N * (int)sizeof(C));
+  static constexpr int RB = FIT < 1 ? 1 : (FIT < 8 ? FIT : 8);
+};
+
+// One block = RB consecutive z rows of one frame.  All 256 threads deposit the atoms
+// of rows z0-1 .. z0+RB-1 once (an atom of row v0 reaches rows v0 and v0+1), then
+// each wave transforms its rows.
+template <int N, typename C>
 __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, int64_t f0) {
   using T = typename Real<C>::T;
+  using AT = typename DepositAcc<C>::T;
+  using AU = typename DepositAcc<C>::U;
   constexpr int P = N / GX_WAVE;
-  constexpr int RB = NWAVE * RPW;
+  constexpr int RB = RowPassShape<N, C>::RB;
   constexpr int R0 = FftFirst<N>::R;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  unsigned long long *accs = reinterpret_cast<unsigned long long *>(smem_raw);  // [NWAVE][2][N]
-  C *stage = reinterpret_cast<C *>(accs + (size_t)NWAVE * 2 * N);            // [W][RB]
-  C *tspec = stage + d.W * RB;                                               // [W] background spectrum
-  double *ftab = reinterpret_cast<double *>(tspec + d.W);                    // [n_elem][2]
+  AU *accs = reinterpret_cast<AU *>(smem_raw);                 // [RB][2][N]
+  C *stage = reinterpret_cast<C *>(accs + (size_t)RB * 2 * N);  // [W][RB]
+  C *tspec = stage + d.W * RB;                                  // [W] background spectrum
+  double *ftab = reinterpret_cast<double *>(tspec + d.W);       // [n_elem][2]
+  __shared__ double rsc[RB];
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
-  unsigned long long *acc_re = accs + (size_t)wave * 2 * N;
-  unsigned long long *acc_im = acc_re + N;
-  C *wbuf = reinterpret_cast<C *>(acc_re);  // FFT exchange buffer (N sizeof(C) <= 16 N bytes)
 
   const int fb = blockIdx.y;
   const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
@@ -127,11 +173,14 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
     if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
     return;
   }
+  const int z0 = blockIdx.x * RB;
   for (int i = threadIdx.x; i < 2 * g.n_elem; i += BLK) ftab[i] = g.f_table[i];
   if (d.fill_bkg || d.smooth) {
     const C *spec = reinterpret_cast<const C *>(d.col_spec) + (int64_t)fb * d.W;
     for (int w = threadIdx.x; w < wf; w += BLK) tspec[w] = spec[w];
   }
+  if (threadIdx.x < RB) rsc[threadIdx.x] = ldexp(1.0, g.row_exp[min(z0 + (int)threadIdx.x, (int)d.N - 1)]);
+  for (int i = threadIdx.x; i < RB * 2 * N; i += BLK) accs[i] = 0;
   __syncthreads();
 
   const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
@@ -142,10 +191,38 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
   const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
   const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
   const double r = d.r, rinv = 1.0 / d.r;
-  const C *tw = reinterpret_cast<const C *>(d.twiddle);
-  const int z0 = blockIdx.x * RB;
 
-  // node mask of this lane's first-stage inputs (smoothed edge x filled region)
+  // 1) deposit: atoms of rows z0-1 .. z0+RB-1 into the block's rows
+  const int a_lo = d.row_off[z0 > 0 ? z0 - 1 : 0], a_hi = d.row_off[min(z0 + RB, (int)d.Z)];
+  for (int a = a_lo + threadIdx.x; a < a_hi; a += BLK) {
+    const int lv = d.azrow[a] - z0;  // -1 .. RB-1
+    long long u0;
+    double tu;
+    cic_u(d.ax[a], d.ay[a], s, c, ymin, r, rinv, u0, tu);
+    if (u0 < 0 || u0 + 1 >= N) {
+      atomicAdd(&d.status[2], 1);
+      continue;
+    }
+    const double tv = g.afz[a];
+    const int e = g.aelem[a];
+    const double fr = ftab[2 * e], fi = ftab[2 * e + 1];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int row = lv + h;
+      if (row < 0 || row >= RB) continue;
+      const double wv = h ? tv : 1.0 - tv;
+      const double sc = rsc[row];
+      const double w0 = wv * (1.0 - tu), w1 = wv * tu;
+      AU *ar = accs + (size_t)row * 2 * N;
+      atomicAdd(&ar[u0], (AU)(AT)llrint((fr * w0) * sc));
+      atomicAdd(&ar[N + u0], (AU)(AT)llrint((fi * w0) * sc));
+      atomicAdd(&ar[u0 + 1], (AU)(AT)llrint((fr * w1) * sc));
+      atomicAdd(&ar[N + u0 + 1], (AU)(AT)llrint((fi * w1) * sc));
+    }
+  }
+  __syncthreads();
+
+  // 2) per row: node mask (smoothed edge x filled region), FFT along y, kept band
   T my[P];
 #pragma unroll
   for (int t = 0; t < P / R0; ++t)
@@ -155,64 +232,38 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
       const T m = d.smooth ? (T)d.col_sy[(int64_t)fb * N + y] : (T)1;
       my[t * R0 + q] = (y >= ya && y < yb) ? m : (T)0;
     }
-
-  for (int rr = 0; rr < RPW; ++rr) {
-    const int rb = wave * RPW + rr;
+  const C *tw = reinterpret_cast<const C *>(d.twiddle);
+  for (int rb = wave; rb < RB; rb += NWAVE) {
     const int v = z0 + rb;
     const bool inside = !d.fill_bkg || (v >= zlo && v < zhi);
-    const int a_lo = d.row_off[v > 0 ? v - 1 : 0], a_mid = d.row_off[v], a_hi = d.row_off[v + 1];
+    const bool atoms = inside && d.row_off[v + 1] > d.row_off[v > 0 ? v - 1 : 0];
+    AU *ar = accs + (size_t)rb * 2 * N;
     C x[P];
-    if (inside && a_hi > a_lo) {
-      for (int y = lane; y < N; y += GX_WAVE) {
-        acc_re[y] = 0;
-        acc_im[y] = 0;
-      }
-      wave_sync();
-      const double sc = ldexp(1.0, g.row_exp[v]);
-      for (int a = a_lo + lane; a < a_hi; a += GX_WAVE) {
-        const double tv = g.afz[a];
-        const double wv = (a < a_mid) ? tv : 1.0 - tv;  // atoms of row v-1 reach v with tv
-        long long u0;
-        double tu;
-        cic_u(d.ax[a], d.ay[a], s, c, ymin, r, rinv, u0, tu);
-        const int e = g.aelem[a];
-        const double fr = ftab[2 * e], fi = ftab[2 * e + 1];
-        const double w0 = wv * (1.0 - tu), w1 = wv * tu;
-        if (u0 >= 0 && u0 + 1 < N) {
-          atomicAdd(&acc_re[u0], (unsigned long long)llrint((fr * w0) * sc));
-          atomicAdd(&acc_im[u0], (unsigned long long)llrint((fi * w0) * sc));
-          atomicAdd(&acc_re[u0 + 1], (unsigned long long)llrint((fr * w1) * sc));
-          atomicAdd(&acc_im[u0 + 1], (unsigned long long)llrint((fi * w1) * sc));
-        } else {
-          atomicAdd(&d.status[2], 1);
-        }
-      }
-      wave_sync();
-      const double inv = ldexp(1.0, -g.row_exp[v]);
+    if (atoms) {
+      const double inv = 1.0 / rsc[rb];
 #pragma unroll
       for (int t = 0; t < P / R0; ++t) {
 #pragma unroll
         for (int q = 0; q < R0; ++q) {
           const int y = lane + GX_WAVE * t + q * (N / R0);
           const T m = my[t * R0 + q];
-          x[t * R0 + q] = C{(T)((double)(long long)acc_re[y] * inv) * m, (T)((double)(long long)acc_im[y] * inv) * m};
+          x[t * R0 + q] = C{(T)((double)(AT)ar[y] * inv) * m, (T)((double)(AT)ar[N + y] * inv) * m};
         }
       }
       wave_sync();
-      fft_wave<N, C>(x, wbuf, tw, lane);
+      fft_wave<N, C>(x, reinterpret_cast<C *>(ar), tw, lane);
     } else {
 #pragma unroll
       for (int q = 0; q < P; ++q) x[q] = C{(T)0, (T)0};
     }
     const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + v] : 1.0;
     stage_row_band<N, C>(x, lane, jlo, wf, inside, d, mz, tspec, stage, RB, rb);
-    wave_sync();
   }
   __syncthreads();
   C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * d.Z;
   for (int idx = threadIdx.x; idx < wf * RB; idx += BLK) {
     const int w = idx / RB, q = idx - w * RB;
-    out[(int64_t)w * d.Z + z0 + q] = stage[idx];
+    if (z0 + q < d.Z) out[(int64_t)w * d.Z + z0 + q] = stage[idx];
   }
 }
 
@@ -262,39 +313,44 @@ __global__ __launch_bounds__(BLK) void k_adj_colpass(gx_frames d, gx_adjoint g, 
   wave_sync();
   // conj of the cotangent dL/dF = 2 (dL/dI) F at its natural index k; with fill or
   // smoothing also this column's share sum_k dL/dF_k conj(Pz^[k]) conj(Py^[j]) of dL/d avg
+  // F at its natural index k into LDS, then, per k (rolled loop, each lane its own k):
+  // conj of the cotangent dL/dF = 2 (dL/dI) F on binned pixels, and with fill or
+  // smoothing this column's share sum_k dL/dF_k conj(Pz^[k]) conj(Py^[j]) of dL/d avg
+  // (+ max_voxels N^2 dL/dF_0 on the k = 0 column, the pattern's constant)
+#pragma unroll
+  for (int t = 0; t < P / RL; ++t)
+#pragma unroll
+    for (int q = 0; q < RL; ++q) buf[lane + GX_WAVE * t + q * (N / RL)] = v[t * RL + q];
+  wave_sync();
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;
   const double val = d.q_val;
   const double2 *zhat = reinterpret_cast<const double2 *>(g.avg_spec) + (int64_t)fb * (N + d.W);
   double are = 0.0, aim = 0.0, dc_re = 0.0, dc_im = 0.0;
-#pragma unroll
-  for (int t = 0; t < P / RL; ++t) {
-#pragma unroll
-    for (int q = 0; q < RL; ++q) {
-      const int k = lane + GX_WAVE * t + q * (N / RL);
-      const int i = (k + N / 2) & (N - 1);
-      double gr = 0.0, gi = 0.0;
-      if (i >= d.i_lo && i <= d.i_hi) {
-        const int kint = (k < N / 2) ? k : k - N;
-        const double qz = (((double)kint * val) * 2.0) * M_PI;
-        const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
-        if (bz >= 0 && bz < d.c_n) {
-          const double gI = 2.0 * g.g_bins[cbase + bz];
-          const C F = v[t * RL + q];
-          gr = gI * (double)F.x;
-          gi = gI * (double)F.y;
-        }
+#pragma unroll 2
+  for (int k = lane; k < N; k += GX_WAVE) {
+    const int i = (k + N / 2) & (N - 1);
+    double gr = 0.0, gi = 0.0;
+    if (i >= d.i_lo && i <= d.i_hi) {
+      const int kint = (k < N / 2) ? k : k - N;
+      const double qz = (((double)kint * val) * 2.0) * M_PI;
+      const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
+      if (bz >= 0 && bz < d.c_n) {
+        const double gI = 2.0 * g.g_bins[cbase + bz];
+        const C F = buf[k];
+        gr = gI * (double)F.x;
+        gi = gI * (double)F.y;
       }
-      if (dense) {
-        const double2 zh = zhat[k];  // dL/dF conj(Pz^)
-        are += gr * zh.x + gi * zh.y;
-        aim += gi * zh.x - gr * zh.y;
-        if (k == 0) {
-          dc_re = gr;
-          dc_im = gi;
-        }
-      }
-      buf[k] = C{(T)gr, (T)(-gi)};
     }
+    if (dense) {
+      const double2 zh = zhat[k];
+      are += gr * zh.x + gi * zh.y;
+      aim += gi * zh.x - gr * zh.y;
+      if (k == 0) {
+        dc_re = gr;
+        dc_im = gi;
+      }
+    }
+    buf[k] = C{(T)gr, (T)(-gi)};
   }
   if (dense) {
 #pragma unroll
@@ -303,9 +359,9 @@ __global__ __launch_bounds__(BLK) void k_adj_colpass(gx_frames d, gx_adjoint g, 
       aim += __shfl_xor(aim, off, GX_WAVE);
     }
     if (lane == 0) {
-      const double2 yh = zhat[N + wcol];  // times conj(Py^[j])
+      const double2 yh = zhat[N + wcol];
       double pre = are * yh.x + aim * yh.y, pim = aim * yh.x - are * yh.y;
-      if (k0) {  // the constant max_voxels of the pattern: max_voxels N^2 dL/dF at k = 0
+      if (k0) {
         const double c0 = d.max_voxels * (double)N * (double)N;
         pre = pre + c0 * dc_re;
         pim = pim + c0 * dc_im;
@@ -385,36 +441,41 @@ __global__ __launch_bounds__(GX_WAVE) void k_adj_avg_spectra(gx_frames d, gx_adj
   }
 }
 
-// One block: g_avg += sum over the batch's frames (in order) of the sum over each
-// frame's kept columns (in order) of avg_part.
-__global__ __launch_bounds__(BLK) void k_adj_avg_reduce(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf) {
-  __shared__ double2 S[BLK];
+// Block per frame: the frame's kept columns of avg_part summed in a fixed tree order,
+// the total left in the frame's column 0.
+__global__ __launch_bounds__(BLK) void k_adj_avg_frames(gx_frames d, gx_adjoint g, int64_t f0) {
+  __shared__ double sre[NWAVE], sim[NWAVE];
+  double2 *apart = reinterpret_cast<double2 *>(g.avg_part) + (int64_t)blockIdx.x * d.W;
+  const double *fp = d.fparams + (f0 + blockIdx.x) * GX_FP_STRIDE;
+  const int wf = max(0, min((int)fp[GX_FP_JHI] - (int)fp[GX_FP_JLO] + 1, (int)d.W));
+  double re = 0.0, im = 0.0;
+  for (int w = threadIdx.x; w < wf; w += BLK) {
+    re += apart[w].x;
+    im += apart[w].y;
+  }
+  re = block_sum(re, sre);
+  im = block_sum(im, sim);
+  if (threadIdx.x == 0) apart[0] = make_double2(wf > 0 ? re : 0.0, wf > 0 ? im : 0.0);
+}
+
+// One block: g_avg += the frame totals in frame order.
+__global__ __launch_bounds__(BLK) void k_adj_avg_reduce(gx_frames d, gx_adjoint g, int64_t nf) {
+  __shared__ double2 tot[BLK];
   const double2 *apart = reinterpret_cast<const double2 *>(g.avg_part);
-  double tre = 0.0, tim = 0.0;
+  double re = 0.0, im = 0.0;
   for (int64_t base = 0; base < nf; base += BLK) {
-    const int64_t fb = base + threadIdx.x;
-    if (fb < nf) {
-      const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
-      const int wf = max(0, min((int)fp[GX_FP_JHI] - (int)fp[GX_FP_JLO] + 1, (int)d.W));
-      double re = 0.0, im = 0.0;
-      for (int w = 0; w < wf; ++w) {
-        const double2 a = apart[fb * d.W + w];
-        re += a.x;
-        im += a.y;
-      }
-      S[threadIdx.x] = make_double2(re, im);
-    }
+    if (base + threadIdx.x < nf) tot[threadIdx.x] = apart[(base + threadIdx.x) * d.W];
     __syncthreads();
     if (threadIdx.x == 0)
       for (int64_t k = 0; k < BLK && base + k < nf; ++k) {
-        tre += S[k].x;
-        tim += S[k].y;
+        re += tot[k].x;
+        im += tot[k].y;
       }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    g.g_avg[0] += tre;
-    g.g_avg[1] += tim;
+    g.g_avg[0] += re;
+    g.g_avg[1] += im;
   }
 }
 
@@ -423,10 +484,18 @@ __global__ __launch_bounds__(BLK) void k_adj_avg_reduce(gx_frames d, gx_adjoint 
 // ------------------------------------------------------------------------------------
 template <int N, typename C>
 struct RowGradShape {
-  // H rows per block: RB = H - 1 rows own atoms, the last one is the halo row v0 + 1
-  static constexpr int H = (N * (int)sizeof(C) <= 8192) ? 8 : ((N * (int)sizeof(C) <= 16384) ? 4 : 2);
+  // H rows per block in 32 KB of LDS: RB = H - 1 rows own atoms, the last one is the
+  // halo row v0 + 1
+  static constexpr int FIT = 32768 / (N * (int)sizeof(C));
+  static constexpr int H = FIT < 2 ? 2 : (FIT < 8 ? FIT : 8);
 };
 
+// One block = RB rows (+ the halo row) of one frame: the kept band of each row,
+// conjugated, is transformed (the adjoint DFT along y) and masked; every atom of the
+// rows gathers its four nodes and writes its dL/dy', dL/dz, dL/df (re, im) for this
+// frame to g_part (precision of the transforms).  The block's share of
+// sum_a dL/dy'_a (regrouped order, fixed partition) goes to red, the argmin of y' to
+// amin.
 template <int N, typename C>
 __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, int64_t f0) {
   using T = typename Real<C>::T;
@@ -438,6 +507,8 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   C *rows = reinterpret_cast<C *>(smem_raw);  // [H][N] cotangent of the grid rows
   C *stage = rows;                            // [W][H], consumed before rows is written
   double *ftab = reinterpret_cast<double *>(rows + H * N);
+  __shared__ double shd[NWAVE];
+  __shared__ int shi[NWAVE];
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
   const int fb = blockIdx.y;
@@ -492,16 +563,16 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
 #pragma unroll
     for (int t = 0; t < P / RL; ++t)
 #pragma unroll
-      for (int qq = 0; qq < RL; ++qq) {
-        const int y = lane + GX_WAVE * t + qq * (N / RL);
-        C o = cconj(x[h][t * RL + qq]);
-        if (masked) {
-          double m = (y >= ya && y < yb) ? mzv : 0.0;
-          if (d.smooth) m = m * d.col_sy[(int64_t)fb * N + y];
-          o = C{(T)((double)o.x * m), (T)((double)o.y * m)};
-        }
-        buf[y] = o;
+      for (int qq = 0; qq < RL; ++qq) buf[lane + GX_WAVE * t + qq * (N / RL)] = cconj(x[h][t * RL + qq]);
+    if (masked) {  // each lane rescales the entries it wrote
+#pragma unroll 2
+      for (int y = lane; y < N; y += GX_WAVE) {
+        double m = (y >= ya && y < yb) ? mzv : 0.0;
+        if (d.smooth) m = m * d.col_sy[(int64_t)fb * N + y];
+        const C o = buf[y];
+        buf[y] = C{(T)((double)o.x * m), (T)((double)o.y * m)};
       }
+    }
   }
   __syncthreads();
 
@@ -509,8 +580,11 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   const double ymin = dkey_inv(d.ymin_key[3 * fb]);
   const double r = d.r, rinv = 1.0 / d.r;
   const int a_lo = d.row_off[z0], a_hi = d.row_off[min(z0 + RB, (int)d.Z)];
-  double *gp = g.g_part + (int64_t)fb * 4 * n;
+  T *gp = reinterpret_cast<T *>(g.g_part) + (int64_t)fb * 4 * n;
+  double su = 0.0;   // this block's share of sum_a dL/dy'_a
+  int am = INT_MAX;  // smallest regrouped index attaining min y'
   for (int a = a_lo + threadIdx.x; a < a_hi; a += BLK) {
+    if ((d.ax[a] * s + d.ay[a] * c) == ymin) am = min(am, a);
     const int vr = d.azrow[a] - z0;  // 0 .. RB-1
     long long u0;
     double tu;
@@ -531,110 +605,66 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
       const double h10 = g10r * fr + g10i * fi, h11 = g11r * fr + g11i * fi;
       const double w00 = (1.0 - tv) * (1.0 - tu), w01 = (1.0 - tv) * tu;
       const double w10 = tv * (1.0 - tu), w11 = tv * tu;
-      gu = (1.0 - tv) * (h01 - h00) + tv * (h11 - h10);
-      gv = (1.0 - tu) * (h10 - h00) + tu * (h11 - h01);
+      gu = ((1.0 - tv) * (h01 - h00) + tv * (h11 - h10)) * rinv;
+      gv = ((1.0 - tu) * (h10 - h00) + tu * (h11 - h01)) * rinv;
       gfr = ((w00 * g00r + w01 * g01r) + w10 * g10r) + w11 * g11r;
       gfi = ((w00 * g00i + w01 * g01i) + w10 * g10i) + w11 * g11i;
     } else {
       atomicAdd(&d.status[2], 1);
     }
-    gp[a] = gu * rinv;
-    gp[n + a] = gv * rinv;
-    gp[2 * n + a] = gfr;
-    gp[3 * n + a] = gfi;
+    gp[a] = (T)gu;
+    gp[n + a] = (T)gv;
+    gp[2 * n + a] = (T)gfr;
+    gp[3 * n + a] = (T)gfi;
+    su += gu;
   }
-}
-
-// ------------------------------------------------------------------------------------
-// reductions (fixed partitions of the original atom order: bitwise reproducible)
-// ------------------------------------------------------------------------------------
-__device__ __forceinline__ double block_sum(double v, double *sh) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, GX_WAVE);
-  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double r = 0.0;
-  for (int i = 0; i < NWAVE; ++i) r += sh[i];
-  return r;
-}
-
-__device__ __forceinline__ int block_min_int(int v, int *sh) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, GX_WAVE));
-  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  int r = INT_MAX;
-  for (int i = 0; i < NWAVE; ++i) r = min(r, sh[i]);
-  return r;
-}
-
-// grid (RED, nf): per frame, partial sums of dL/dy' over original atoms and the
-// smallest original index whose y' equals the frame's min y'
-__global__ __launch_bounds__(BLK) void k_adj_frame_sums(gx_frames d, gx_adjoint g, int64_t f0) {
-  __shared__ double shd[NWAVE];
-  __shared__ int shi[NWAVE];
-  const int fb = blockIdx.y, b = blockIdx.x;
-  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
-  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
-  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
-  const int64_t n = d.n_atoms, n_rowed = d.row_off[d.N];
-  const int64_t chunk = (n + RED - 1) / RED;
-  const int64_t i0 = b * chunk, i1 = min(n, i0 + chunk);
-  const double *gy = g.g_part + (int64_t)fb * 4 * n;
-  double acc = 0.0;
-  int am = INT_MAX;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += BLK) {
-    const int a = g.ainv[i];
-    if (a < n_rowed) acc += gy[a];
-    const double yp = d.ax[a] * s + d.ay[a] * c;
-    if (yp == ymin) am = min(am, (int)i);
-  }
-  acc = block_sum(acc, shd);
+  su = block_sum(su, shd);
   am = block_min_int(am, shi);
   if (threadIdx.x == 0) {
-    g.red[(int64_t)fb * RED + b] = acc;
+    g.red[(int64_t)fb * gridDim.x + blockIdx.x] = su;
     if (am != INT_MAX) atomicMin(&g.amin[fb], am);
   }
 }
 
-// One thread per frame: S_f = sum of the frame's RED partials (in block order);
-// the argmin atom of min y' carries the origin term, d(min y')/dy'_a = [a = argmin],
-// so its dL/dy' in this frame's row of g_part becomes dL/dy' - S_f.  Frames own
-// distinct rows, so the frames run in parallel.
-__global__ __launch_bounds__(BLK) void k_adj_frame_origin(gx_frames d, gx_adjoint g, int64_t nf) {
+// ------------------------------------------------------------------------------------
+// reductions (fixed partitions of the regrouped atom order: bitwise reproducible)
+// ------------------------------------------------------------------------------------
+// One thread per frame: S_f = sum of the frame's nb row-block partials (block order).
+__global__ __launch_bounds__(BLK) void k_adj_frame_sums(gx_adjoint g, int64_t nf, int nb, double *S) {
   const int64_t fb = blockIdx.x * (int64_t)BLK + threadIdx.x;
   if (fb >= nf) return;
-  double S = 0.0;
-  for (int b = 0; b < RED; ++b) S += g.red[fb * RED + b];
-  const int am = g.amin[fb];
-  if (am == INT_MAX) return;
-  g.g_part[fb * 4 * d.n_atoms + g.ainv[am]] -= S;
+  double acc = 0.0;
+  for (int b = 0; b < nb; ++b) acc += g.red[fb * nb + b];
+  S[fb] = acc;
 }
 
-// g_atom += sum over the batch's frames of (sin, cos) dL/dy', dL/dz, dL/df.
+// g_atom += sum over the batch's frames of (sin, cos) dL/dy', dL/dz, dL/df, and the
+// origin term: d(min y')/dy'_a = [a = argmin_f], so an atom that is the argmin of
+// frame f also takes -S_f (sin, cos), in frame order.
 // Block = 16 atoms x 16 frame lanes; lane j sums frames j, j + 16, ... in order,
 // then the 16 lane sums are added in a fixed tree order.
 constexpr int RA = 16, RF = BLK / RA;
-__global__ __launch_bounds__(BLK) void k_adj_reduce(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf) {
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_adj_reduce(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf,
+                                                    const double *S) {
   __shared__ double sh[5][RF][RA + 1];
+  extern __shared__ int amin_s[];  // [nf]
+  for (int64_t k = threadIdx.x; k < nf; k += BLK) amin_s[k] = g.amin[k];
   const int64_t n = d.n_atoms, n_rowed = d.row_off[d.N];
   const int al = threadIdx.x % RA, j = threadIdx.x / RA;
   const int64_t a = blockIdx.x * (int64_t)RA + al;
+  const T *gpart = reinterpret_cast<const T *>(g.g_part);
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (a < n_rowed) {
     for (int64_t fb = j; fb < nf; fb += RF) {
       const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
-      const double *gp = g.g_part + fb * 4 * n;
-      const double u = gp[a];
+      const T *gp = gpart + fb * 4 * n;
+      const double u = (double)gp[a];
       acc[0] += fp[GX_FP_SIN] * u;
       acc[1] += fp[GX_FP_COS] * u;
-      acc[2] += gp[n + a];
-      acc[3] += gp[2 * n + a];
-      acc[4] += gp[3 * n + a];
+      acc[2] += (double)gp[n + a];
+      acc[3] += (double)gp[2 * n + a];
+      acc[4] += (double)gp[3 * n + a];
     }
   }
 #pragma unroll
@@ -646,12 +676,22 @@ __global__ __launch_bounds__(BLK) void k_adj_reduce(gx_frames d, gx_adjoint g, i
       for (int k = 0; k < 5; ++k) sh[k][j][al] += sh[k][j + w][al];
     __syncthreads();
   }
-  if (j == 0 && a < n_rowed)
+  if (j == 0 && a < n_rowed) {
+    double gx = sh[0][0][al], gy = sh[1][0][al];
+    for (int64_t fb = 0; fb < nf; ++fb)
+      if (amin_s[fb] == (int)a) {
+        const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+        gx -= fp[GX_FP_SIN] * S[fb];
+        gy -= fp[GX_FP_COS] * S[fb];
+      }
+    g.g_atom[a] += gx;
+    g.g_atom[n + a] += gy;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) g.g_atom[k * n + a] += sh[k][0][al];
+    for (int k = 2; k < 5; ++k) g.g_atom[k * n + a] += sh[k][0][al];
+  }
 }
 
-// grid (RED, 1 + n_elem): y = 0 sums dL/dz over atoms (original order) and finds the
+// grid (RED, 1 + n_elem): y = 0 sums dL/dz over atoms (regrouped order) and finds the
 // smallest original index attaining each bounding-box extreme (slots: x min, x max,
 // y min, y max, z min, z max; only z min without fill / smoothing); y = 1 + e sums
 // dL/df of element e
@@ -666,9 +706,9 @@ __global__ __launch_bounds__(BLK) void k_adj_fin_sums(gx_frames d, gx_adjoint g,
   double s0 = 0.0, s1 = 0.0;
   int am[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX};
   for (int64_t i = i0 + threadIdx.x; i < i1; i += BLK) {
-    const int a = g.ainv[i];
+    // sums over regrouped atoms i; extremes over original atoms i
     if (y == 0) {
-      s0 += g.g_atom[2 * n + a];
+      s0 += g.g_atom[2 * n + i];
       if (g.z[i] == g.zmin) am[4] = min(am[4], (int)i);
       if (box) {
         if (g.x[i] == g.ext[0]) am[0] = min(am[0], (int)i);
@@ -677,9 +717,9 @@ __global__ __launch_bounds__(BLK) void k_adj_fin_sums(gx_frames d, gx_adjoint g,
         if (g.y[i] == g.ext[3]) am[3] = min(am[3], (int)i);
         if (g.z[i] == g.ext[5]) am[5] = min(am[5], (int)i);
       }
-    } else if (g.aelem[a] == y - 1) {
-      s0 += g.g_atom[3 * n + a];
-      s1 += g.g_atom[4 * n + a];
+    } else if (g.aelem[i] == y - 1) {
+      s0 += g.g_atom[3 * n + i];
+      s1 += g.g_atom[4 * n + i];
     }
   }
   s0 = block_sum(s0, shd);
@@ -749,7 +789,7 @@ struct AffArgs {
   double a[9];
 };
 
-__global__ __launch_bounds__(BLK) void k_bins_adjoint(const unsigned long long *__restrict__ g_fx,
+__global__ __launch_bounds__(BLK) void k_bins_adjoint(const unsigned long long *__restrict__ g_fx, int64_t n_rep,
                                                       const double *fx_scale, const int32_t *__restrict__ bcnt,
                                                       int64_t c_n, int64_t c_lo, int64_t q_num, double qa_start,
                                                       double qa_step, double qa_stop, AffArgs aff, double z_elem,
@@ -765,7 +805,9 @@ __global__ __launch_bounds__(BLK) void k_bins_adjoint(const unsigned long long *
       const double qy = lin_axis(c_lo + iy, q_num, qa_start, qa_step, qa_stop);
       const double qz = lin_axis(c_lo + iz, q_num, qa_start, qa_step, qa_stop);
       const double q2 = (qx * qx + qy * qy) + qz * qz;
-      const double giq = (double)(long long)g_fx[i] * inv;
+      unsigned long long acc = 0;  // exact (wrapping) integer sum of the copies
+      for (int64_t r = 0; r < n_rep; ++r) acc += g_fx[r * n + i];
+      const double giq = (double)(long long)acc * inv;
       gb = (giq * f0_weight(aff.a, q2, z_elem, c16pi2)) / (double)cnt;
     }
     g_bins[i] = gb;
@@ -777,21 +819,19 @@ __global__ __launch_bounds__(BLK) void k_bins_adjoint(const unsigned long long *
 // ------------------------------------------------------------------------------------
 template <int N, typename C>
 int launch_adj_rowpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
-  constexpr int RPW = (N <= 1024) ? 2 : 1;
-  constexpr int RB = NWAVE * RPW;
-  const size_t lds = (size_t)NWAVE * 2 * N * 8 + (size_t)d->W * (RB + 1) * sizeof(C) + (size_t)g->n_elem * 16;
-  if (lds > 160 * 1024)
+  constexpr int RB = RowPassShape<N, C>::RB;
+  const size_t lds = (size_t)RB * N * sizeof(C) + (size_t)d->W * (RB + 1) * sizeof(C) + (size_t)g->n_elem * 16;
+  if (lds > 150 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "differentiable row pass needs %zu bytes of LDS (N=%d, W=%lld)", lds, N,
                      (long long)d->W);
   static bool attr_set = false;
   if (!attr_set) {
-    GX_HIP(hipFuncSetAttribute((const void *)k_adj_rowpass<N, RPW, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               160 * 1024));
+    GX_HIP(hipFuncSetAttribute((const void *)k_adj_rowpass<N, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               150 * 1024));
     attr_set = true;
   }
-  if (d->Z % RB) return set_error(GX_ERR_ARG, "Z=%lld must be a multiple of %d", (long long)d->Z, RB);
-  dim3 grid((unsigned)(d->Z / RB), (unsigned)nf);
-  k_adj_rowpass<N, RPW, C><<<grid, BLK, lds, st>>>(*d, *g, f0);
+  dim3 grid((unsigned)((d->Z + RB - 1) / RB), (unsigned)nf);
+  k_adj_rowpass<N, C><<<grid, BLK, lds, st>>>(*d, *g, f0);
   return check_launch("k_adj_rowpass");
 }
 
@@ -805,19 +845,32 @@ int launch_adj_colpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, in
 template <int N, typename C>
 int launch_adj_rowgrad_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int H = RowGradShape<N, C>::H, RB = H - 1;
-  const size_t lds = (size_t)H * N * sizeof(C) + (size_t)g->n_elem * 16;  // staging aliases the rows (W <= N)
-  if (lds > 160 * 1024)
+  // rows (the staging tile aliases them, W <= N) + f table
+  const size_t lds = (size_t)H * N * sizeof(C) + (size_t)g->n_elem * 16;
+  if (lds > 150 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "adjoint row pass needs %zu bytes of LDS (N=%d, W=%lld)", lds, N,
                      (long long)d->W);
   static bool attr_set = false;
   if (!attr_set) {
     GX_HIP(hipFuncSetAttribute((const void *)k_adj_rowgrad<N, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               160 * 1024));
+                               150 * 1024));
     attr_set = true;
   }
   dim3 grid((unsigned)((d->Z + RB - 1) / RB), (unsigned)nf);
   k_adj_rowgrad<N, C><<<grid, BLK, lds, st>>>(*d, *g, f0);
   return check_launch("k_adj_rowgrad");
+}
+
+// row blocks per frame of k_adj_rowgrad (its partial sums per frame)
+int rowgrad_blocks(const gx_frames *d) {
+  int rb = 1;
+  switch (d->N) {
+    case 256: rb = d->fft_f64 ? RowGradShape<256, double2>::H - 1 : RowGradShape<256, float2>::H - 1; break;
+    case 512: rb = d->fft_f64 ? RowGradShape<512, double2>::H - 1 : RowGradShape<512, float2>::H - 1; break;
+    case 1024: rb = d->fft_f64 ? RowGradShape<1024, double2>::H - 1 : RowGradShape<1024, float2>::H - 1; break;
+    case 2048: rb = d->fft_f64 ? RowGradShape<2048, double2>::H - 1 : RowGradShape<2048, float2>::H - 1; break;
+  }
+  return (int)((d->Z + rb - 1) / rb);
 }
 
 enum Op { OP_ROWPASS, OP_COLPASS, OP_ROWGRAD, OP_AVGSPEC };
@@ -871,6 +924,7 @@ int check_adj(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, b
                  d->ax && d->ay && d->azrow,
              "a required device buffer of gx_frames is NULL");
   GX_REQUIRE(g->n_elem >= 1 && g->n_elem <= 128, "n_elem=%lld outside [1, 128]", (long long)g->n_elem);
+  GX_REQUIRE(d->batch <= 8192, "batch of %lld frames exceeds 8192", (long long)d->batch);
   GX_REQUIRE(g->f_table && g->afz && g->aperm && g->aelem && g->ecnt && g->row_exp && g->ainv,
              "a required buffer of gx_adjoint is NULL");
   if (backward) {
@@ -928,6 +982,7 @@ int gx_adj_rowgrad(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t 
   clear_error();
   int rc = check_adj(d, g, f0, nf, true);
   if (rc || nf == 0) return rc;
+  k_fill_i32<<<grid1d(nf, BLK), BLK, 0, as_stream(stream)>>>(g->amin, nf, INT_MAX);
   return dispatch(OP_ROWGRAD, d, g, f0, nf, as_stream(stream));
 }
 
@@ -936,10 +991,15 @@ int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t n
   int rc = check_adj(d, g, f0, nf, true);
   if (rc || nf == 0) return rc;
   hipStream_t st = as_stream(stream);
-  k_fill_i32<<<grid1d(nf, BLK), BLK, 0, st>>>(g->amin, nf, INT_MAX);
-  k_adj_frame_sums<<<dim3(RED, (unsigned)nf), BLK, 0, st>>>(*d, *g, f0);
-  k_adj_frame_origin<<<grid1d(nf, BLK), BLK, 0, st>>>(*d, *g, nf);
-  if (d->n_atoms > 0) k_adj_reduce<<<grid1d(d->n_atoms, RA), BLK, 0, st>>>(*d, *g, f0, nf);
+  double *S = g->red + d->batch * d->N;  // per-frame sums of dL/dy'
+  k_adj_frame_sums<<<grid1d(nf, BLK), BLK, 0, st>>>(*g, nf, rowgrad_blocks(d), S);
+  if (d->n_atoms > 0) {
+    const size_t lds = (size_t)nf * sizeof(int);
+    if (d->fft_f64)
+      k_adj_reduce<double><<<grid1d(d->n_atoms, RA), BLK, lds, st>>>(*d, *g, f0, nf, S);
+    else
+      k_adj_reduce<float><<<grid1d(d->n_atoms, RA), BLK, lds, st>>>(*d, *g, f0, nf, S);
+  }
   return check_launch("gx_adj_reduce");
 }
 
@@ -956,9 +1016,11 @@ int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t
   if ((rc = dispatch(OP_ROWPASS, d, g, f0, nf, st))) return rc;
   if ((rc = dispatch(OP_COLPASS, d, g, f0, nf, st))) return rc;
   if (dense) {
-    k_adj_avg_reduce<<<1, BLK, 0, st>>>(*d, *g, f0, nf);
+    k_adj_avg_frames<<<(unsigned)nf, BLK, 0, st>>>(*d, *g, f0);
+    k_adj_avg_reduce<<<1, BLK, 0, st>>>(*d, *g, nf);
     if ((rc = check_launch("k_adj_avg_reduce"))) return rc;
   }
+  k_fill_i32<<<grid1d(nf, BLK), BLK, 0, st>>>(g->amin, nf, INT_MAX);
   if ((rc = dispatch(OP_ROWGRAD, d, g, f0, nf, st))) return rc;
   return gx_adj_reduce(d, g, f0, nf, stream);
 }
@@ -982,11 +1044,13 @@ int gx_adj_finish(const gx_frames *d, const gx_adjoint *g, double *g_coords, dou
   return check_launch("gx_adj_finish");
 }
 
-int gx_bins_adjoint(const unsigned long long *g_fx, const double *fx_scale, const int32_t *bin_cnt, int64_t c_n,
+int gx_bins_adjoint(const unsigned long long *g_fx, int64_t n_rep, const double *fx_scale, const int32_t *bin_cnt,
+                    int64_t c_n,
                     int64_t c_lo, int64_t q_num, double qa_start, double qa_step, double qa_stop,
                     const double *host_aff, double z_elem, double *g_bins, gx_stream_t stream) {
   clear_error();
   GX_REQUIRE(g_fx && fx_scale && bin_cnt && host_aff && g_bins, "NULL argument");
+  GX_REQUIRE(n_rep >= 1 && n_rep <= 64, "n_rep=%lld outside [1, 64]", (long long)n_rep);
   GX_REQUIRE(c_n >= 1 && c_lo >= 0 && c_lo + c_n <= q_num, "crop [%lld, %lld) outside the q axis of %lld",
              (long long)c_lo, (long long)(c_lo + c_n), (long long)q_num);
   AffArgs aff;
@@ -994,7 +1058,7 @@ int gx_bins_adjoint(const unsigned long long *g_fx, const double *fx_scale, cons
   const double c16pi2 = 16 * (M_PI * M_PI);
   const int64_t n = c_n * c_n * c_n;
   unsigned gr = grid1d(n, BLK);
-  k_bins_adjoint<<<gr > 8192 ? 8192 : gr, BLK, 0, as_stream(stream)>>>(g_fx, fx_scale, bin_cnt, c_n, c_lo, q_num,
+  k_bins_adjoint<<<gr > 8192 ? 8192 : gr, BLK, 0, as_stream(stream)>>>(g_fx, n_rep, fx_scale, bin_cnt, c_n, c_lo, q_num,
                                                                        qa_start, qa_step, qa_stop, aff, z_elem,
                                                                        c16pi2, g_bins);
   return check_launch("gx_bins_adjoint");

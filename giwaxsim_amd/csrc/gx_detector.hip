@@ -170,13 +170,15 @@ __global__ __launch_bounds__(BLK) void k_det_scale(const double *__restrict__ pa
 // and only the last lane of a run issues the atomic.  Integer sums are exact, so
 // the result does not depend on the grouping.
 __global__ __launch_bounds__(BLK) void k_det_adj(gx_detector d, const double *__restrict__ g, const double *fx_scale,
-                                                 unsigned long long *g_fx) {
+                                                 unsigned long long *g_fx, int64_t n_rep) {
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
   const bool live = p < d.n_pix * d.n_pix;
   const double gp = live ? g[p] : 0.0;
   if (__all(gp == 0.0)) return;  // wave-uniform
   const double sc = fx_scale[0];
+  // blocks spread their atomics over n_rep copies of the grid (summed exactly later)
+  unsigned long long *dst = g_fx + (int64_t)(blockIdx.x % n_rep) * (d.cy * d.cx * d.cz);
   double x0 = 0.0, y0 = 0.0, z0 = 0.0;
   if (live) det_pixel(d, p, x0, y0, z0);
   const double dqinv = 1.0 / d.dq;
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(BLK) void k_det_adj(gx_detector d, const double *__
     }
     const int dn = __shfl_down(key, 1, GX_WAVE);
     const bool tail = (lane == GX_WAVE - 1) || dn != key;
-    if (tail && key >= 0 && val != 0) atomicAdd(&g_fx[key], (unsigned long long)val);
+    if (tail && key >= 0 && val != 0) atomicAdd(&dst[key], (unsigned long long)val);
   }
 }
 
@@ -423,7 +425,7 @@ int gx_detector_accumulate(const gx_detector *d, gx_stream_t stream) {
 }
 
 int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags, double *work,
-                        unsigned long long *g_fx, double *fx_scale, gx_stream_t stream) {
+                        unsigned long long *g_fx, int64_t n_rep, double *fx_scale, gx_stream_t stream) {
   clear_error();
   GX_REQUIRE(d != nullptr, "NULL detector");
   GX_REQUIRE(d->n_pix >= 1 && d->n_pix <= 65536, "n_pix=%lld", (long long)d->n_pix);
@@ -435,6 +437,7 @@ int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags
                  (d->n_orient == 0 || (d->orient_R && d->orient_w)),
              "a detector buffer is NULL");
   GX_REQUIRE(g_out && work && g_fx && fx_scale, "NULL adjoint buffer");
+  GX_REQUIRE(n_rep >= 1 && n_rep <= 64, "n_rep=%lld outside [1, 64]", (long long)n_rep);
   hipStream_t st = as_stream(stream);
   const int64_t n = d->n_pix, npx = n * n;
   const unsigned nblk = grid1d(npx, BLK);
@@ -446,11 +449,11 @@ int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags
     g_raw = t;
   const int64_t nv = d->cy * d->cx * d->cz;
   GX_REQUIRE(nv < 2147483647LL, "iq grid of %lld voxels exceeds int32 indexing", (long long)nv);
-  unsigned gz = grid1d(nv, BLK);
-  k_zero_u64<<<gz > 4096 ? 4096 : gz, BLK, 0, st>>>(g_fx, nv);
+  unsigned gz = grid1d(nv * n_rep, BLK);
+  k_zero_u64<<<gz > 4096 ? 4096 : gz, BLK, 0, st>>>(g_fx, nv * n_rep);
   k_det_scale<<<1, BLK, 0, st>>>(part, nblk, (flags & GX_FIN_MIRROR) ? 4.0 : 1.0, d->orient_w, d->n_orient,
                                  fx_scale);
-  k_det_adj<<<nblk, BLK, 0, st>>>(*d, g_raw, fx_scale, g_fx);
+  k_det_adj<<<nblk, BLK, 0, st>>>(*d, g_raw, fx_scale, g_fx, n_rep);
   return check_launch("gx_detector_adjoint");
 }
 

@@ -180,6 +180,111 @@ __global__ __launch_bounds__(BLK) void k_slab_scatter(gx_slab s) {
 }
 
 // ------------------------------------------------------------------------------------
+// stable regrouping: atoms keep their original order within a row.  Chunks of
+// GX_STABLE_CHUNK atoms (original order) count their rows; a per-row scan over the
+// chunks gives each chunk's first slot in every row; each chunk then places its
+// atoms in order, 256 at a time, ranking equal rows within a wave by ballots and
+// across waves in wave order.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ int slab_row_of(const gx_slab &s, int64_t a, double rinv) {
+  const int N = (int)s.N;
+  const double zz = s.z[a] - s.zmin;
+  const long long zi = to_int(floor_div(zz, s.r, rinv));
+  int row = (zi >= 0 && zi < N) ? (int)zi : N;
+  const int e = s.elem[a];
+  if (e < 0 || e >= s.n_elem) row = N;
+  return row;
+}
+
+__global__ __launch_bounds__(BLK) void k_slab_stable_count(gx_slab s) {
+  extern __shared__ int hist[];  // [N + 1]
+  const int M = (int)s.N + 1;
+  const double rinv = 1.0 / s.r;
+  for (int i = threadIdx.x; i < M; i += BLK) hist[i] = 0;
+  __syncthreads();
+  const int64_t a0 = (int64_t)blockIdx.x * GX_STABLE_CHUNK;
+  const int64_t a1 = min(s.n_atoms, a0 + GX_STABLE_CHUNK);
+  for (int64_t a = a0 + threadIdx.x; a < a1; a += BLK) atomicAdd(&hist[slab_row_of(s, a, rinv)], 1);
+  __syncthreads();
+  int32_t *tab = s.stable_work + (int64_t)blockIdx.x * M;
+  for (int i = threadIdx.x; i < M; i += BLK) tab[i] = hist[i];
+}
+
+// per row: exclusive scan over the chunks, offset by the row's first slot
+__global__ __launch_bounds__(BLK) void k_slab_stable_scan(gx_slab s, int64_t n_chunks) {
+  const int M = (int)s.N + 1;
+  const int row = blockIdx.x * BLK + threadIdx.x;
+  if (row >= M) return;
+  int32_t run = s.row_off[row];
+  for (int64_t c = 0; c < n_chunks; ++c) {
+    int32_t *t = s.stable_work + c * M + row;
+    const int32_t v = *t;
+    *t = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_slab_stable_scatter(gx_slab s) {
+  extern __shared__ int cur[];  // [N + 1] next free slot of each row for this chunk
+  const int N = (int)s.N, M = N + 1;
+  const double rinv = 1.0 / s.r;
+  const int lane = threadIdx.x & (GX_WAVE - 1), wave = threadIdx.x / GX_WAVE;
+  const int32_t *tab = s.stable_work + (int64_t)blockIdx.x * M;
+  for (int i = threadIdx.x; i < M; i += BLK) cur[i] = tab[i];
+  __syncthreads();
+  const int64_t a0 = (int64_t)blockIdx.x * GX_STABLE_CHUNK;
+  const int64_t a1 = min(s.n_atoms, a0 + GX_STABLE_CHUNK);
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (int64_t base = a0; base < a1; base += BLK) {
+    const int64_t a = base + threadIdx.x;
+    const bool valid = a < a1;
+    const int row = valid ? slab_row_of(s, a, rinv) : -1;
+    // rank among the lanes of this wave with the same row (lower lanes first)
+    int rank = 0, cnt = 0;
+    bool leader = false;
+    unsigned long long rem = __ballot(valid);
+    while (rem) {
+      const int l = __ffsll((long long)rem) - 1;
+      const int lr = __shfl(row, l, GX_WAVE);
+      const unsigned long long m = __ballot(valid && row == lr);
+      if (valid && row == lr) {
+        rank = __popcll(m & lt);
+        cnt = __popcll(m);
+        leader = (lane == l);
+      }
+      rem &= ~m;
+    }
+    for (int w = 0; w < BLK / GX_WAVE; ++w) {
+      if (wave == w && valid) {
+        const int pos = cur[row] + rank;
+        wave_sync();
+        if (leader) cur[row] += cnt;
+        s.sx[pos] = s.x[a];
+        s.sy[pos] = s.y[a];
+        s.szrow[pos] = row;
+        int qr = 0, qi = 0;
+        double t = 0.0;
+        const int e = s.elem[a];
+        if (row < N) {
+          const double sc = ldexp(1.0, s.row_exp[row]);
+          qr = (int)rint(s.f_table[2 * e] * sc);
+          qi = (int)rint(s.f_table[2 * e + 1] * sc);
+          const double zz = s.z[a] - s.zmin;
+          t = zz * rinv - (double)to_int(floor_div(zz, s.r, rinv));
+          t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        }
+        s.sq[2 * (int64_t)pos] = qr;
+        s.sq[2 * (int64_t)pos + 1] = qi;
+        if (s.szf) s.szf[pos] = t;
+        s.sperm[pos] = (int32_t)a;
+        if (s.selem) s.selem[pos] = e;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // per-frame statistics
 // ------------------------------------------------------------------------------------
 __global__ void k_frames_init(unsigned long long *ymin_key, int32_t *bounds, int64_t nf) {
@@ -1078,7 +1183,16 @@ int gx_slab_sort_rows(const gx_slab *s, gx_stream_t stream) {
   if (g > 8192) g = 8192;
   k_slab_rowcount<<<g, BLK, 0, st>>>(*s);
   k_slab_scan<<<1, 1024, 0, st>>>(*s);
-  k_slab_scatter<<<g, BLK, 0, st>>>(*s);
+  if (s->sperm && s->stable_work) {
+    GX_REQUIRE(s->N <= 4096, "stable regrouping needs N <= 4096 (got %lld)", (long long)s->N);
+    const int64_t n_chunks = (s->n_atoms + GX_STABLE_CHUNK - 1) / GX_STABLE_CHUNK;
+    const size_t lds = (size_t)(s->N + 1) * sizeof(int);
+    k_slab_stable_count<<<(unsigned)n_chunks, BLK, lds, st>>>(*s);
+    k_slab_stable_scan<<<grid1d(s->N + 1, BLK), BLK, 0, st>>>(*s, n_chunks);
+    k_slab_stable_scatter<<<(unsigned)n_chunks, BLK, lds, st>>>(*s);
+  } else {
+    k_slab_scatter<<<g, BLK, 0, st>>>(*s);
+  }
   return check_launch("gx_slab_sort_rows");
 }
 

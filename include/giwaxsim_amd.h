@@ -95,7 +95,11 @@ typedef struct gx_slab {
   double *szf;            /* [n_atoms] fractional z offset (z - zmin) / r - row, in [0, 1] */
   int32_t *sperm;         /* [n_atoms] original index of each regrouped atom */
   int32_t *selem;         /* [n_atoms] element id of each regrouped atom */
+  int32_t *stable_work;   /* optional [ceil(n_atoms / GX_STABLE_CHUNK)][N + 1] scratch: when given
+                             (N <= 4096), atoms keep their original order within each row, so
+                             the regrouped order is reproducible */
 } gx_slab;
+#define GX_STABLE_CHUNK 4096
 
 /* ------------------------------------------------------------------------------------
  * The phi-frame pipeline of one prepared slab into a cropped 3D bin grid.
@@ -321,8 +325,9 @@ typedef struct gx_adjoint {
   int32_t *ainv;           /* [n_atoms] regrouped index of each original atom (gx_adj_prepare) */
   /* backward */
   const double *g_bins;    /* [c^3] dL / d(bin sum of I) (gx_bins_adjoint) */
-  double *g_part;          /* [batch][4][n_atoms] per-frame partials: dL/dy', dL/dz, dL/df re, im */
-  double *red;             /* [max(batch, 2 (1 + n_elem))][GX_ADJ_RED_BLOCKS] reduction scratch */
+  void *g_part;            /* [batch][4][n_atoms] per-frame dL/dy', dL/dz, dL/df re, im; float
+                              with FP32 transforms (fft_f64 = 0), double with FP64 */
+  double *red;             /* [max(batch (N + 1), 2 (1 + n_elem) GX_ADJ_RED_BLOCKS)] reduction scratch */
   int32_t *amin;           /* [batch + 7] argmin / argmax atoms (original index) */
   double *g_atom;          /* [5][n_atoms] accumulated dL/dx, dL/dy, dL/dz, dL/df re, im per
                               regrouped atom (zeroed by gx_adj_prepare) */
@@ -347,10 +352,12 @@ int gx_adj_forward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t 
 /* Adjoint of the column pass, in place on d->inter: FFT along z of the recomputed
  * column, cotangent 2 g_bins[bin] F on binned pixels, adjoint DFT back to z. */
 int gx_adj_colpass(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
-/* Adjoint DFT along y of every row, gathered at the atoms' four nodes into g_part. */
+/* Adjoint DFT along y of every row, gathered at the atoms' four nodes into g_part;
+ * also each frame's sum of dL/dy' per row block and argmin of y' (for gx_adj_reduce). */
 int gx_adj_rowgrad(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
 /* Fold g_part of frames [f0, f0+nf) into g_atom (chain rule through
- * y' = x sin phi + y cos phi and the per-frame origin min y'). */
+ * y' = x sin phi + y cos phi and the per-frame origin min y'); follows gx_adj_rowgrad
+ * of the same frames. */
 int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
 /* columns + rowpass (recomputed) + colpass adjoint + rowgrad + reduce.  d->ymin_key and
  * d->idx_bounds must hold the statistics of the same frames and atoms (gx_frames_stats,
@@ -362,19 +369,21 @@ int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t
 int gx_adj_finish(const gx_frames *d, const gx_adjoint *g, double *g_coords, double *g_f, gx_stream_t stream);
 
 /* Adjoint of gx_bins_finalize for the FP64 iq: g_bins = g_iq * fn^2 / count
- * (0 where count = 0), with g_iq = g_fx / fx_scale[0] (gx_detector_adjoint's
- * fixed-point output).  Same axis / Cromer-Mann arguments as gx_bins_finalize. */
-int gx_bins_adjoint(const unsigned long long *g_fx, const double *fx_scale, const int32_t *bin_cnt,
-                    int64_t c_n, int64_t c_lo, int64_t q_num, double qa_start, double qa_step,
-                    double qa_stop, const double *host_aff, double z_elem, double *g_bins,
-                    gx_stream_t stream);
+ * (0 where count = 0), with g_iq = (sum of the n_rep copies of g_fx) / fx_scale[0]
+ * (gx_detector_adjoint's fixed-point output).  Same axis / Cromer-Mann arguments
+ * as gx_bins_finalize. */
+int gx_bins_adjoint(const unsigned long long *g_fx, int64_t n_rep, const double *fx_scale,
+                    const int32_t *bin_cnt, int64_t c_n, int64_t c_lo, int64_t q_num, double qa_start,
+                    double qa_step, double qa_stop, const double *host_aff, double z_elem,
+                    double *g_bins, gx_stream_t stream);
 /* Adjoint of gx_detector_accumulate + gx_detector_finalize(flags): g_out [n][n] is
  * dL/d(finalized image); d->det must hold the accumulated image of the forward.
- * Writes dL/d(iq) as fixed point g_fx [cy][cx][cz] with the power-of-two scale
- * fx_scale[0] (sums are exact, hence deterministic).
+ * Writes dL/d(iq) as fixed point with the power-of-two scale fx_scale[0], spread over
+ * n_rep (1..64) copies g_fx [n_rep][cy][cx][cz] whose sum is the result (fewer
+ * atomics on one address; integer sums are exact, hence deterministic).
  * work: [2 n^2 + ceil(n^2 / 256)] doubles. */
 int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags, double *work,
-                        unsigned long long *g_fx, double *fx_scale, gx_stream_t stream);
+                        unsigned long long *g_fx, int64_t n_rep, double *fx_scale, gx_stream_t stream);
 
 #ifdef __cplusplus
 }
