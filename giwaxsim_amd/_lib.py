@@ -124,6 +124,8 @@ SIGNATURES = {
     "gx_adj_finish": [POINTER(GxFrames), POINTER(GxAdjoint), _P, _P, _P],
     "gx_bins_adjoint": [_P, _I, _P, _P, _I, _I, _I, _D, _D, _D, POINTER(c_double), _D, _P, _P],
     "gx_detector_adjoint": [POINTER(GxDetector), _P, _I, _P, _P, _I, _P, _P],
+    "gx_detector_map_build": [POINTER(GxDetector), _P, _P, _P, _P, _P],
+    "gx_detector_adjoint_mapped": [POINTER(GxDetector), _P, _I, _P, _P, _P, _P, _P, _P],
 }
 EXPORTS = ["gx_last_error", "gx_abi_version", "gx_supported_grid"] + list(SIGNATURES)
 

@@ -188,9 +188,15 @@ class DifferentiableSlab:
         self.det_raw = _empty(npx * npx, torch.float64)
         self.det_out = _empty(npx * npx, torch.float64)
         self.det_work = _empty(2 * npx * npx + (npx * npx + 255) // 256, torch.float64)
-        # copies of the detector adjoint's accumulator (<= 256 MB)
-        self.n_rep = int(max(1, min(8, 256e6 // (8 * nb))))
+        # the detector adjoint gathers through a transpose map of the orientation
+        # gather when it fits (<= 8 GB of pair ids), else scatters into copies of its
+        # accumulator (<= 256 MB)
+        n_pairs = npx * npx * len(dp.orient_w)
+        pbits = int(np.ceil(np.log2(max(npx * npx, 1))))
+        self.mapped = n_pairs * 4 <= 8e9 and pbits < 32 and len(dp.orient_w) <= 2 ** (32 - pbits)
+        self.n_rep = 1 if self.mapped else int(max(1, min(8, 256e6 // (8 * nb))))
         self.g_fx = _empty(self.n_rep * nb, torch.int64)
+        self.det_map = None
         self.fx_scale = _empty(1, torch.float64)
         ax = g.crop_axis
         self.detector_desc = GxDetector(
@@ -200,6 +206,15 @@ class DifferentiableSlab:
             orient_w=ptr(self.det_ow).value, iq=ptr(self.iq64).value, iq_f64=1,
             cy=g.c_n, cx=g.c_n, cz=g.c_n, qx_min=float(np.min(ax)), qy_min=float(np.min(ax)),
             qz_min=float(np.min(ax)), dq=float(np.diff(ax)[0]), det=ptr(self.det_raw).value)
+        if self.mapped:
+            cnt = _empty(nb, torch.int32)
+            scan = _empty(-(-nb // 2048) + 1, torch.int64)
+            off = _empty(nb + 1, torch.int64)
+            pairs = _empty(n_pairs, torch.int32)
+            call("gx_detector_map_build", ctypes.byref(self.detector_desc), ptr(cnt), ptr(scan), ptr(off),
+                 ptr(pairs), stream_handle())
+            self.det_map = (off, pairs)
+            del cnt, scan
         self._generation = 0
         self._prepared = None
 
@@ -371,8 +386,13 @@ class DifferentiableSlab:
     def backward_enqueue(self, fr, adj, g_out, st):
         """Enqueue the whole adjoint for the image cotangent ``g_out`` [n_pix^2]."""
         g = self.plan.grid
-        call("gx_detector_adjoint", ctypes.byref(self.detector_desc), ptr(g_out), self._flags(),
-             ptr(self.det_work), ptr(self.g_fx), self.n_rep, ptr(self.fx_scale), st)
+        if self.mapped:
+            call("gx_detector_adjoint_mapped", ctypes.byref(self.detector_desc), ptr(g_out), self._flags(),
+                 ptr(self.det_work), ptr(self.det_map[0]), ptr(self.det_map[1]), ptr(self.g_fx),
+                 ptr(self.fx_scale), st)
+        else:
+            call("gx_detector_adjoint", ctypes.byref(self.detector_desc), ptr(g_out), self._flags(),
+                 ptr(self.det_work), ptr(self.g_fx), self.n_rep, ptr(self.fx_scale), st)
         start, step, stop = g.linspace_params
         aff = _lib.dbl_array(aff_dict['C'])
         call("gx_bins_adjoint", ptr(self.g_fx), self.n_rep, ptr(self.fx_scale), ptr(self.bin_cnt), g.c_n, g.c_lo,

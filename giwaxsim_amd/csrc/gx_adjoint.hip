@@ -21,8 +21,8 @@
 //                     band, then every atom of the rows gathers its four nodes:
 //                     dL/dy', dL/dz, dL/df per (frame, atom) into g_part
 //                     and each block's share of sum_a dL/dy'_a and the argmin of y'
-//   k_adj_reduce      g_atom += sum_f (sin, cos) dL/dy', dL/dz, dL/df; the argmin atom
-//                     of min y' also takes -sum_a dL/dy'_a (origin term)
+//   k_adj_frame_origin origin term: the argmin atom of min y' takes -sum_a dL/dy'_a
+//   k_adj_reduce      g_atom += sum_f (sin, cos) dL/dy', dL/dz, dL/df
 // The regrouping is stable (original order within a row), sums over atoms run in
 // that order with fixed partitions, and the row deposit is exact, so every result
 // is bitwise reproducible.
@@ -484,8 +484,8 @@ __global__ __launch_bounds__(BLK) void k_adj_avg_reduce(gx_frames d, gx_adjoint 
 // ------------------------------------------------------------------------------------
 template <int N, typename C>
 struct RowGradShape {
-  // H rows per block in 32 KB of LDS: RB = H - 1 rows own atoms, the last one is the
-  // halo row v0 + 1
+  // H rows per block in 32 KB of LDS (64 KB measured slower: fewer blocks per CU):
+  // RB = H - 1 rows own atoms, the last one is the halo row v0 + 1
   static constexpr int FIT = 32768 / (N * (int)sizeof(C));
   static constexpr int H = FIT < 2 ? 2 : (FIT < 8 ? FIT : 8);
 };
@@ -638,18 +638,25 @@ __global__ __launch_bounds__(BLK) void k_adj_frame_sums(gx_adjoint g, int64_t nf
   S[fb] = acc;
 }
 
-// g_atom += sum over the batch's frames of (sin, cos) dL/dy', dL/dz, dL/df, and the
-// origin term: d(min y')/dy'_a = [a = argmin_f], so an atom that is the argmin of
-// frame f also takes -S_f (sin, cos), in frame order.
+// Origin term, one thread per frame: d(min y')/dy'_a = [a = argmin_f], so the argmin
+// atom's dL/dy' of frame f becomes dL/dy' - S_f.  Frames own distinct rows of g_part.
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_adj_frame_origin(gx_frames d, gx_adjoint g, int64_t nf, const double *S) {
+  const int64_t fb = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (fb >= nf) return;
+  const int am = g.amin[fb];
+  if (am == INT_MAX) return;
+  T *u = reinterpret_cast<T *>(g.g_part) + fb * 4 * d.n_atoms + am;
+  *u = (T)((double)*u - S[fb]);
+}
+
+// g_atom += sum over the batch's frames of (sin, cos) dL/dy', dL/dz, dL/df.
 // Block = 16 atoms x 16 frame lanes; lane j sums frames j, j + 16, ... in order,
 // then the 16 lane sums are added in a fixed tree order.
 constexpr int RA = 16, RF = BLK / RA;
 template <typename T>
-__global__ __launch_bounds__(BLK) void k_adj_reduce(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf,
-                                                    const double *S) {
+__global__ __launch_bounds__(BLK) void k_adj_reduce(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf) {
   __shared__ double sh[5][RF][RA + 1];
-  extern __shared__ int amin_s[];  // [nf]
-  for (int64_t k = threadIdx.x; k < nf; k += BLK) amin_s[k] = g.amin[k];
   const int64_t n = d.n_atoms, n_rowed = d.row_off[d.N];
   const int al = threadIdx.x % RA, j = threadIdx.x / RA;
   const int64_t a = blockIdx.x * (int64_t)RA + al;
@@ -676,19 +683,9 @@ __global__ __launch_bounds__(BLK) void k_adj_reduce(gx_frames d, gx_adjoint g, i
       for (int k = 0; k < 5; ++k) sh[k][j][al] += sh[k][j + w][al];
     __syncthreads();
   }
-  if (j == 0 && a < n_rowed) {
-    double gx = sh[0][0][al], gy = sh[1][0][al];
-    for (int64_t fb = 0; fb < nf; ++fb)
-      if (amin_s[fb] == (int)a) {
-        const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
-        gx -= fp[GX_FP_SIN] * S[fb];
-        gy -= fp[GX_FP_COS] * S[fb];
-      }
-    g.g_atom[a] += gx;
-    g.g_atom[n + a] += gy;
+  if (j == 0 && a < n_rowed)
 #pragma unroll
-    for (int k = 2; k < 5; ++k) g.g_atom[k * n + a] += sh[k][0][al];
-  }
+    for (int k = 0; k < 5; ++k) g.g_atom[k * n + a] += sh[k][0][al];
 }
 
 // grid (RED, 1 + n_elem): y = 0 sums dL/dz over atoms (regrouped order) and finds the
@@ -993,12 +990,15 @@ int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t n
   hipStream_t st = as_stream(stream);
   double *S = g->red + d->batch * d->N;  // per-frame sums of dL/dy'
   k_adj_frame_sums<<<grid1d(nf, BLK), BLK, 0, st>>>(*g, nf, rowgrad_blocks(d), S);
+  if (d->fft_f64)
+    k_adj_frame_origin<double><<<grid1d(nf, BLK), BLK, 0, st>>>(*d, *g, nf, S);
+  else
+    k_adj_frame_origin<float><<<grid1d(nf, BLK), BLK, 0, st>>>(*d, *g, nf, S);
   if (d->n_atoms > 0) {
-    const size_t lds = (size_t)nf * sizeof(int);
     if (d->fft_f64)
-      k_adj_reduce<double><<<grid1d(d->n_atoms, RA), BLK, lds, st>>>(*d, *g, f0, nf, S);
+      k_adj_reduce<double><<<grid1d(d->n_atoms, RA), BLK, 0, st>>>(*d, *g, f0, nf);
     else
-      k_adj_reduce<float><<<grid1d(d->n_atoms, RA), BLK, lds, st>>>(*d, *g, f0, nf, S);
+      k_adj_reduce<float><<<grid1d(d->n_atoms, RA), BLK, 0, st>>>(*d, *g, f0, nf);
   }
   return check_launch("gx_adj_reduce");
 }

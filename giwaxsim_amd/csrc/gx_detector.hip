@@ -206,6 +206,114 @@ __global__ __launch_bounds__(BLK) void k_det_adj(gx_detector d, const double *__
   }
 }
 
+// ------------------------------------------------------------------------------------
+// transpose map of the orientation gather (fixed detector geometry): per voxel, the
+// (pixel, orientation) pairs that read it, as pair ids p * n_orient + o
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BLK) void k_det_map_count(gx_detector d, int32_t *cnt) {
+  const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (p >= d.n_pix * d.n_pix) return;
+  double x0, y0, z0;
+  det_pixel(d, p, x0, y0, z0);
+  const double dqinv = 1.0 / d.dq;
+  for (int64_t o = 0; o < d.n_orient; ++o) atomicAdd(&cnt[det_voxel(d, x0, y0, z0, o, dqinv)], 1);
+}
+
+// exclusive scan of int32 counts into int64 offsets, SCAN_B elements per block
+constexpr int SCAN_B = 2048;
+__global__ __launch_bounds__(BLK) void k_scan_local(const int32_t *cnt, int64_t n, int64_t *off, int64_t *bsum) {
+  __shared__ int64_t sh[BLK];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_B;
+  constexpr int PER = SCAN_B / BLK;
+  int64_t v[PER];
+  int64_t tot = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int64_t i = base + threadIdx.x * PER + k;
+    v[k] = i < n ? cnt[i] : 0;
+    tot += v[k];
+  }
+  sh[threadIdx.x] = tot;
+  __syncthreads();
+  for (int s = 1; s < BLK; s <<= 1) {
+    const int64_t t = threadIdx.x >= s ? sh[threadIdx.x - s] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int64_t run = sh[threadIdx.x] - tot;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int64_t i = base + threadIdx.x * PER + k;
+    if (i < n) off[i] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == BLK - 1) bsum[blockIdx.x] = sh[BLK - 1];
+}
+
+__global__ void k_scan_blocks(int64_t *bsum, int64_t nb) {
+  if (threadIdx.x != 0) return;
+  int64_t run = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t t = bsum[b];
+    bsum[b] = run;
+    run += t;
+  }
+  bsum[nb] = run;
+}
+
+__global__ __launch_bounds__(BLK) void k_scan_add(int64_t *off, int64_t n, const int64_t *bsum, int64_t nb,
+                                                  int32_t *cnt) {
+  const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (i < n) {
+    off[i] += bsum[i / SCAN_B];
+    cnt[i] = 0;  // becomes the fill cursor
+  }
+  if (i == 0) off[n] = bsum[nb];
+}
+
+__global__ __launch_bounds__(BLK) void k_det_map_fill(gx_detector d, const int64_t *off, int32_t *cur, uint32_t *pairs,
+                                                      int pbits) {
+  const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (p >= d.n_pix * d.n_pix) return;
+  double x0, y0, z0;
+  det_pixel(d, p, x0, y0, z0);
+  const double dqinv = 1.0 / d.dq;
+  for (int64_t o = 0; o < d.n_orient; ++o) {
+    const int64_t v = det_voxel(d, x0, y0, z0, o, dqinv);
+    const int64_t pos = off[v] + atomicAdd(&cur[v], 1);
+    pairs[pos] = ((uint32_t)o << pbits) | (uint32_t)p;
+  }
+}
+
+// dL/d iq per voxel: one wave per voxel sums its pairs' g[p] w[o] in fixed point
+// (exact, so independent of the pairs' order)
+__global__ __launch_bounds__(BLK) void k_det_adj_map(gx_detector d, const double *__restrict__ g,
+                                                     const double *fx_scale, const int64_t *__restrict__ off,
+                                                     const uint32_t *__restrict__ pairs, unsigned long long *g_fx,
+                                                     int pbits) {
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int64_t nv = d.cy * d.cx * d.cz;
+  const double sc = fx_scale[0];
+  const uint32_t pmask = (pbits >= 32) ? 0xffffffffu : ((1u << pbits) - 1u);
+  for (int64_t v = (blockIdx.x * (int64_t)BLK + threadIdx.x) / GX_WAVE; v < nv;
+       v += (int64_t)gridDim.x * (BLK / GX_WAVE)) {
+    const int64_t a = off[v], b = off[v + 1];
+    long long acc = 0;
+    for (int64_t k = a + lane; k < b; k += GX_WAVE) {
+      const uint32_t id = pairs[k];
+      acc += llrint((g[id & pmask] * d.orient_w[id >> pbits]) * sc);
+    }
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) acc += __shfl_xor(acc, s, GX_WAVE);
+    if (lane == 0) g_fx[v] = (unsigned long long)acc;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_fill_zero_i32(int32_t *p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) p[i] = 0;
+}
+
 __global__ __launch_bounds__(BLK) void k_zero_u64(unsigned long long *p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) p[i] = 0;
 }
@@ -422,6 +530,66 @@ int gx_detector_accumulate(const gx_detector *d, gx_stream_t stream) {
   else
     k_detector<float><<<grid1d(npx, BLK), BLK, 0, as_stream(stream)>>>(*d);
   return check_launch("gx_detector_accumulate");
+}
+
+// bits of a pair id that hold the pixel (the orientation takes the rest)
+static int map_pixel_bits(int64_t npx) {
+  int b = 0;
+  while ((1ll << b) < npx) ++b;
+  return b;
+}
+
+int gx_detector_map_build(const gx_detector *d, int32_t *counts, int64_t *scan_work, int64_t *voxel_off,
+                          uint32_t *pairs, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(d != nullptr, "NULL detector");
+  GX_REQUIRE(d->n_pix >= 1 && d->n_pix <= 65536 && d->n_init >= 0 && d->n_init <= 3 && d->n_orient >= 1,
+             "detector sizes");
+  GX_REQUIRE(d->cy >= 1 && d->cx >= 1 && d->cz >= 1 && d->dq > 0.0, "iq grid");
+  GX_REQUIRE(d->h_axis && d->v_axis && (d->n_init == 0 || d->init_R) && d->orient_R && d->orient_w,
+             "a detector buffer is NULL");
+  GX_REQUIRE(counts && scan_work && voxel_off && pairs, "NULL map buffer");
+  const int64_t npx = d->n_pix * d->n_pix, nv = d->cy * d->cx * d->cz;
+  const int pbits = map_pixel_bits(npx);
+  GX_REQUIRE(pbits < 32 && d->n_orient <= (1ll << (32 - pbits)),
+             "%lld orientations x %lld pixels do not pack into 32-bit pair ids", (long long)d->n_orient,
+             (long long)npx);
+  hipStream_t st = as_stream(stream);
+  const int64_t nb = (nv + SCAN_B - 1) / SCAN_B;
+  k_fill_zero_i32<<<grid1d(nv, BLK) > 8192 ? 8192 : grid1d(nv, BLK), BLK, 0, st>>>(counts, nv);
+  k_det_map_count<<<grid1d(npx, BLK), BLK, 0, st>>>(*d, counts);
+  k_scan_local<<<(unsigned)nb, BLK, 0, st>>>(counts, nv, voxel_off, scan_work);
+  k_scan_blocks<<<1, 64, 0, st>>>(scan_work, nb);
+  k_scan_add<<<grid1d(nv, BLK), BLK, 0, st>>>(voxel_off, nv, scan_work, nb, counts);
+  k_det_map_fill<<<grid1d(npx, BLK), BLK, 0, st>>>(*d, voxel_off, counts, pairs, pbits);
+  return check_launch("gx_detector_map_build");
+}
+
+int gx_detector_adjoint_mapped(const gx_detector *d, const double *g_out, int64_t flags, double *work,
+                               const int64_t *voxel_off, const uint32_t *pairs, unsigned long long *g_fx,
+                               double *fx_scale, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(d != nullptr, "NULL detector");
+  GX_REQUIRE(d->n_pix >= 1 && d->n_pix <= 65536 && d->n_orient >= 1, "detector sizes");
+  GX_REQUIRE(d->cy >= 1 && d->cx >= 1 && d->cz >= 1, "empty iq grid");
+  GX_REQUIRE(d->det && d->orient_w && g_out && work && voxel_off && pairs && g_fx && fx_scale,
+             "NULL adjoint buffer");
+  hipStream_t st = as_stream(stream);
+  const int64_t n = d->n_pix, npx = n * n;
+  const unsigned nblk = grid1d(npx, BLK);
+  double *t = work, *g_raw = work + npx, *part = work + 2 * npx;
+  k_det_fin_adj<<<nblk, BLK, 0, st>>>(d->det, g_out, n, n, flags, t, part);
+  if (flags & GX_FIN_MIRROR)
+    k_detector_finalize<<<nblk, BLK, 0, st>>>(t, n, n, GX_FIN_MIRROR, g_raw);
+  else
+    g_raw = t;
+  k_det_scale<<<1, BLK, 0, st>>>(part, nblk, (flags & GX_FIN_MIRROR) ? 4.0 : 1.0, d->orient_w, d->n_orient,
+                                 fx_scale);
+  const int64_t nv = d->cy * d->cx * d->cz;
+  unsigned gv = grid1d(nv * GX_WAVE, BLK);
+  k_det_adj_map<<<gv > 65536 ? 65536 : gv, BLK, 0, st>>>(*d, g_raw, fx_scale, voxel_off, pairs, g_fx,
+                                                         map_pixel_bits(npx));
+  return check_launch("gx_detector_adjoint_mapped");
 }
 
 int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags, double *work,
