@@ -366,6 +366,8 @@ def extra_configs(args, engine, torch, which=None):
     if which is None:
         out["adjoint"] = adjoint_config(engine, torch, precision='fp32')
         out["adjoint_fp64"] = adjoint_config(engine, torch, precision='fp64')
+        out["adjoint_polymer"] = adjoint_polymer_config(engine, torch)
+        out["adjoint_large"] = adjoint_large_config(engine, torch, n_slabs=16)
     reps = 3
     if which in (None, "ensemble"):
         out["ensemble"] = ensemble_config(engine, torch, reps)

@@ -132,6 +132,54 @@ def test_row_grouping():
     assert np.array_equal(np.sort(rows), np.sort(np.minimum(zi, N)))
 
 
+def test_stable_regrouping():
+    """With the permutation and scratch requested, atoms keep their original order
+    within each row (the differentiable path relies on it for reproducible sums)."""
+    import torch
+    from giwaxsim_amd import _lib
+    from giwaxsim_amd._lib import GxSlab, call
+    coords, elements, p = _case('polymer256_fill_smooth')
+    ids, ftab, counts, _ = engine.element_table(elements, 12700.0)
+    N, r = 256, p['r']
+    zmin = float(coords[:, 2].min())
+    dev = torch.device('cuda')
+    c = torch.as_tensor(coords, device=dev)
+    x, y, z = (c[:, k].contiguous() for k in range(3))
+    el = torch.as_tensor(ids, device=dev).to(torch.int32)
+    ft = torch.as_tensor(np.stack([ftab.real, ftab.imag], 1), device=dev).contiguous()
+    n = len(coords)
+    buf = {k: torch.empty(n, dtype=dt, device=dev) for k, dt in
+           (('sx', torch.float64), ('sy', torch.float64), ('szrow', torch.int32), ('szf', torch.float64),
+            ('sperm', torch.int32), ('selem', torch.int32))}
+    sq = torch.empty(2 * n, dtype=torch.int32, device=dev)
+    row_off = torch.empty(N + 2, dtype=torch.int32, device=dev)
+    row_exp = torch.empty(N, dtype=torch.int32, device=dev)
+    work = torch.empty(N * len(ftab) + 2 * (N + 1), dtype=torch.int32, device=dev)
+    stable = torch.empty(-(-n // _lib.GX_STABLE_CHUNK) * (N + 1), dtype=torch.int32, device=dev)
+    ptr = engine.ptr
+    d = GxSlab(n_atoms=n, n_elem=len(ftab), N=N, r=r, zmin=zmin, x=ptr(x).value, y=ptr(y).value,
+               z=ptr(z).value, elem=ptr(el).value, f_table=ptr(ft).value, sx=ptr(buf['sx']).value,
+               sy=ptr(buf['sy']).value, sq=ptr(sq).value, szrow=ptr(buf['szrow']).value,
+               row_off=ptr(row_off).value, row_exp=ptr(row_exp).value, work=ptr(work).value,
+               szf=ptr(buf['szf']).value, sperm=ptr(buf['sperm']).value, selem=ptr(buf['selem']).value,
+               stable_work=ptr(stable).value)
+    call("gx_slab_sort_rows", ctypes_byref(d), engine.stream_handle())
+    perm = buf['sperm'].cpu().numpy()
+    zi = O.floor_div(coords[:, 2] - zmin, r).astype(int)
+    rows = np.minimum(zi, N)
+    want = np.lexsort((np.arange(n), rows))  # by row, then original index
+    assert np.array_equal(perm, want)
+    assert np.array_equal(buf['sx'].cpu().numpy(), coords[want, 0])
+    tz = (coords[want, 2] - zmin) * (1.0 / r) - zi[want]
+    assert np.allclose(buf['szf'].cpu().numpy(), np.clip(tz, 0, 1), rtol=0, atol=1e-12)
+    assert np.array_equal(buf['selem'].cpu().numpy(), ids[want])
+
+
+def ctypes_byref(d):
+    import ctypes
+    return ctypes.byref(d)
+
+
 def test_deterministic():
     coords, elements, p = _case('polymer256_fill_smooth')
     ids, ftab, counts, _ = engine.element_table(elements, 12700.0)
