@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_void_p
 from .build import LIB
 
 GX_OK = 0
-GX_ABI_VERSION = 2   # structure layouts below match this version of the header
+GX_ABI_VERSION = 3   # structure layouts below match this version of the header
 GX_FP_STRIDE = 24
 GX_MOMENTS_WORK = 774
 GX_FIN_MIRROR = 1
@@ -61,6 +61,7 @@ class GxFrames(ctypes.Structure):
         ("ymin_key", c_void_p), ("idx_bounds", c_void_p), ("col_nm", c_void_p),
         ("col_sy", c_void_p), ("col_sz", c_void_p), ("inter", c_void_p), ("col_spec", c_void_p),
         ("bin_sum", c_void_p), ("bin_cnt", c_void_p), ("status", c_void_p),
+        ("row_variant", c_int64),
     ]
 
 

@@ -676,6 +676,109 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
 }
 
 // ------------------------------------------------------------------------------------
+// row pass, block-wide variant: a block owns RB consecutive z rows of one frame; all
+// 256 threads deposit the atoms of those rows at once (more loads in flight than
+// one wave streaming one row), then each wave transforms its rows.  Same arithmetic
+// and outputs as k_rowpass; the autotuner picks between the two shapes.
+// ------------------------------------------------------------------------------------
+template <int N, int RB, typename C>
+__global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0) {
+  using T = typename Real<C>::T;
+  constexpr int P = N / GX_WAVE;
+  constexpr int R0 = FftFirst<N>::R;
+  constexpr int ROWB = (N * (int)sizeof(C) > N * 8) ? N * (int)sizeof(C) : N * 8;  // bytes per row
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *stage = reinterpret_cast<C *>(smem_raw + (size_t)RB * ROWB);  // [W][RB]
+  C *tspec = stage + d.W * RB;                                      // [W]
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int wave = threadIdx.x / GX_WAVE;
+
+  const int fb = blockIdx.y;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = jhi - jlo + 1;
+  if (wf <= 0) return;  // block-uniform
+  if (wf > d.W) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
+    return;
+  }
+  const bool dense = d.fill_bkg || d.smooth;
+  if (dense) {
+    const C *spec = reinterpret_cast<const C *>(d.col_spec) + (int64_t)fb * d.W;
+    for (int w = threadIdx.x; w < wf; w += BLK) tspec[w] = spec[w];
+  }
+  const int z0 = blockIdx.x * RB;
+  for (int i = threadIdx.x; i < RB * 2 * N; i += BLK) {
+    const int row = i / (2 * N), k = i - row * 2 * N;
+    reinterpret_cast<int32_t *>(smem_raw + (size_t)row * ROWB)[k] = 0;
+  }
+  __syncthreads();
+
+  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;  // keep ya <= y < yb
+  const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
+  const double r = d.r, rinv = 1.0 / d.r;
+  // 1) deposit the atoms of rows z0 .. z0+RB-1 that lie inside the filled z range
+  const int a_lo = d.row_off[z0], a_hi = d.row_off[z0 + RB];
+  for (int a = a_lo + threadIdx.x; a < a_hi; a += BLK) {
+    const int v = d.azrow[a];
+    if (d.fill_bkg && !(v >= zlo && v < zhi)) continue;
+    const double yy = (d.ax[a] * s + d.ay[a] * c) - ymin;
+    const long long yi = to_int(floor_div(yy, r, rinv));
+    if (yi >= ya && yi < yb) {
+      int32_t *acc = reinterpret_cast<int32_t *>(smem_raw + (size_t)(v - z0) * ROWB);
+      const int2 q = reinterpret_cast<const int2 *>(d.aq)[a];
+      atomicAdd(&acc[yi], q.x);
+      atomicAdd(&acc[N + yi], q.y);
+    }
+  }
+  __syncthreads();
+  // 2) per row: smoothed-edge mask, FFT along y, kept band
+  T my[P];
+#pragma unroll
+  for (int t = 0; t < P / R0; ++t)
+#pragma unroll
+    for (int q = 0; q < R0; ++q)
+      my[t * R0 + q] = d.smooth ? (T)d.col_sy[(int64_t)fb * N + lane + GX_WAVE * t + q * (N / R0)] : (T)1;
+  const C *tw = reinterpret_cast<const C *>(d.twiddle);
+  for (int rb = wave; rb < RB; rb += BLK / GX_WAVE) {
+    const int z = z0 + rb;
+    const bool inside = !d.fill_bkg || (z >= zlo && z < zhi);
+    const bool atoms = inside && d.row_off[z + 1] > d.row_off[z];
+    unsigned char *rowp = smem_raw + (size_t)rb * ROWB;
+    const int32_t *acc = reinterpret_cast<const int32_t *>(rowp);
+    C v[P];
+    if (atoms) {
+      const T inv = (T)ldexp(1.0, -d.row_exp[z]);
+#pragma unroll
+      for (int t = 0; t < P / R0; ++t)
+#pragma unroll
+        for (int q = 0; q < R0; ++q) {
+          const int y = lane + GX_WAVE * t + q * (N / R0);
+          const T m = my[t * R0 + q];
+          v[t * R0 + q] = C{((T)acc[y] * inv) * m, ((T)acc[N + y] * inv) * m};
+        }
+      wave_sync();
+      fft_wave<N, C>(v, reinterpret_cast<C *>(rowp), tw, lane);
+    } else {
+#pragma unroll
+      for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
+    }
+    const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + z] : 1.0;
+    stage_row_band<N, C>(v, lane, jlo, wf, inside, d, mz, tspec, stage, RB, rb);
+  }
+  __syncthreads();
+  C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * d.Z;
+  for (int idx = threadIdx.x; idx < wf * RB; idx += BLK) {
+    const int w = idx / RB, q = idx - w * RB;
+    out[(int64_t)w * d.Z + z0 + q] = stage[idx];
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // column pass
 // ------------------------------------------------------------------------------------
 template <int N, typename C>
@@ -1081,10 +1184,38 @@ int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st)
   return check_launch("k_rowpass");
 }
 
+template <int N, int RB, typename C>
+int launch_rowpass_blk_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  constexpr int ROWB = (N * (int)sizeof(C) > N * 8) ? N * (int)sizeof(C) : N * 8;
+  const size_t lds = (size_t)RB * ROWB + (size_t)d->W * (RB + 1) * sizeof(C);
+  if (lds > 160 * 1024)
+    return set_error(GX_ERR_UNSUPPORTED, "block-wide row pass needs %zu bytes of LDS (N=%d, RB=%d)", lds, N, RB);
+  static bool attr_set = false;
+  if (!attr_set) {
+    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass_blk<N, RB, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               160 * 1024));
+    attr_set = true;
+  }
+  dim3 grid((unsigned)(d->Z / RB), (unsigned)nf);
+  k_rowpass_blk<N, RB, C><<<grid, BLK, lds, st>>>(*d, f0);
+  return check_launch("k_rowpass_blk");
+}
+
+// row_variant: 0 per-wave rows (2 rows per wave for N <= 1024), 1 block-wide deposit
+// over 4 rows, 2 block-wide over 8 rows
+template <int N, typename C>
+int launch_rowpass_v(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  switch (d->row_variant) {
+    case 1: return launch_rowpass_blk_t<N, 4, C>(d, f0, nf, st);
+    case 2: return launch_rowpass_blk_t<N, 8, C>(d, f0, nf, st);
+    default: return launch_rowpass_t<N, C>(d, f0, nf, st);
+  }
+}
+
 template <int N>
 int launch_rowpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
-  if (d->fft_f64) return launch_rowpass_t<N, double2>(d, f0, nf, st);
-  return launch_rowpass_t<N, float2>(d, f0, nf, st);
+  if (d->fft_f64) return launch_rowpass_v<N, double2>(d, f0, nf, st);
+  return launch_rowpass_v<N, float2>(d, f0, nf, st);
 }
 
 template <int N>
@@ -1125,6 +1256,8 @@ int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
   GX_REQUIRE(d->Z >= 8 && d->Z <= d->N && d->Z % 8 == 0, "row range Z=%lld must be a multiple of 8 in [8, N]",
              (long long)d->Z);
   GX_REQUIRE(d->c_n >= 1, "empty cropped grid");
+  GX_REQUIRE(d->row_variant >= 0 && d->row_variant <= 2, "row_variant=%lld outside [0, 2]",
+             (long long)d->row_variant);
   GX_REQUIRE(d->fparams && d->ymin_key && d->idx_bounds && d->inter && d->bin_sum && d->bin_cnt && d->twiddle &&
                  d->status && d->row_off && d->row_exp,
              "a required device buffer of gx_frames is NULL");

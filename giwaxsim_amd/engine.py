@@ -489,7 +489,7 @@ class FrameRunner:
     """Runs every phi-frame of one slab into shared bins, in batches."""
 
     def __init__(self, plan: FramePlan, slab: Slab, bins: Bins, fill_bkg, smooth, avg,
-                 batch=None, precision='auto'):
+                 batch=None, precision='auto', variant=None):
         dev = device()
         N, W, Z = plan.N, plan.W, plan.Z
         self.plan, self.slab, self.bins = plan, slab, bins
@@ -545,7 +545,10 @@ class FrameRunner:
             col_nm=ptr(self.col_nm).value, col_sy=ptr(self.col_sy).value,
             col_sz=ptr(self.col_sz).value, inter=ptr(self.inter).value,
             col_spec=ptr(self.col_spec).value,
-            bin_sum=ptr(bins.sum).value, bin_cnt=ptr(bins.cnt).value, status=ptr(self.status).value)
+            bin_sum=ptr(bins.sum).value, bin_cnt=ptr(bins.cnt).value, status=ptr(self.status).value,
+            row_variant=0 if variant is None else int(variant))
+        if variant is None and is_pow2_grid(N):
+            self.desc.row_variant = tuned_variant(self)
 
     def run(self, frames=None):
         """Enqueue frames [lo, hi) (default: all) on the current stream."""
@@ -566,7 +569,7 @@ class FrameRunner:
 
 def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, fill_bkg=False,
                   smooth=0, f_table=None, elem_ids=None, batch=None, want64=True, frames=None,
-                  precision='auto'):
+                  precision='auto', variant=None):
     """Full voxel-grid path of one slab.  Returns dict with device iq32/iq64,
     host axes and the objects that produced them.  ``frames=(lo, hi)``
     restricts the run to phi-frames [lo, hi) (bounded samples)."""
@@ -597,7 +600,8 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
     scale = bin_scale_for([(plan.n_frames, pixels_per_bin(plan), B)])
     bins = Bins(plan.grid, scale)
     slab = Slab(coords, elem_ids, f_table, plan.N, r_voxel_size, zmin)
-    runner = FrameRunner(plan, slab, bins, fill_bkg, smooth, avg, batch=batch, precision=precision)
+    runner = FrameRunner(plan, slab, bins, fill_bkg, smooth, avg, batch=batch, precision=precision,
+                         variant=variant)
     runner.run(frames)
     runner.check()
     iq64, iq32 = bins.finalize(want64=want64, want32=True)
@@ -826,6 +830,55 @@ def tuned_batch(N, W, n_frames, f64=False, Z=None):
     return max(1, min(n_frames, _TUNED[key]))
 
 
+_TUNED_VARIANT = {}
+_SCRATCH_BINS = {}
+
+
+def tuned_variant(runner: FrameRunner):
+    """Row-pass shape (gx_frames.row_variant: one wave per row, or a block depositing
+    4 / 8 rows at once) for this runner's grid edge, precision, band width and atoms
+    per row: each shape is timed once on the runner's own first frames (into scratch
+    bins, so the real bins stay untouched) and the fastest is cached.  All shapes give
+    identical bins.  GIWAXSIM_AUTOTUNE=0 keeps shape 0."""
+    import os
+    if os.environ.get("GIWAXSIM_AUTOTUNE", "1") == "0":
+        return 0
+    p, d = runner.plan, runner.desc
+    n_rowed = max(1, runner.slab.n)
+    per_row = int(np.log2(max(1.0, n_rowed / max(1, p.Z))))
+    key = (p.N, (p.W + 63) // 64, runner.f64, per_row, torch.cuda.get_device_name())
+    if key in _TUNED_VARIANT:
+        return _TUNED_VARIANT[key]
+    g = p.grid
+    sb = _SCRATCH_BINS.get(g.c_n)
+    if sb is None:
+        sb = _SCRATCH_BINS[g.c_n] = Bins(g, 1.0)
+    keep = (d.bin_sum, d.bin_cnt, d.row_variant)
+    d.bin_sum, d.bin_cnt = ptr(sb.sum).value, ptr(sb.cnt).value
+    nfr = min(p.n_frames, runner.batch)
+    best, best_t = 0, float("inf")
+    try:
+        for v in (0, 1, 2):
+            d.row_variant = v
+            try:
+                runner.run((0, nfr))
+            except _lib.GxError:
+                continue  # shape does not fit this grid (LDS)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            runner.run((0, nfr))
+            e1.record()
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1)
+            if t < best_t:
+                best, best_t = v, t
+    finally:
+        d.bin_sum, d.bin_cnt, d.row_variant = keep
+    _TUNED_VARIANT[key] = best
+    return best
+
+
 def _tune(N, cap, f64):
     from .synth import polymer_slab
     r = 0.2 * 1024 / N
@@ -841,7 +894,7 @@ def _tune(N, cap, f64):
     best, best_t = cands[0], float("inf")
     for b in cands:
         rn = FrameRunner(plan, slab, bins, True, 10, avg, batch=b,
-                         precision='fp64' if f64 else 'fp32')
+                         precision='fp64' if f64 else 'fp32', variant=0)
         nfr = min(plan.n_frames, 4 * b)
         rn.run((0, nfr))
         torch.cuda.synchronize()

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define GX_ABI_VERSION 2
+#define GX_ABI_VERSION 3
 
 #define GX_OK 0
 #define GX_ERR_ARG 1         /* invalid argument (message says which) */
@@ -160,6 +160,8 @@ typedef struct gx_frames {
   unsigned long long *bin_sum; /* fixed-point sum of I (scale bin_scale) */
   int32_t *bin_cnt;
   int32_t *status;         /* [4] counters; [0] += frames without any valid atom */
+  int64_t row_variant;     /* row pass shape (same results): 0 one wave per z row, 1 / 2 a block
+                              deposits 4 / 8 rows at once, then one wave per row transforms */
 } gx_frames;
 
 /* ------------------------------------------------------------------------------------

@@ -132,6 +132,23 @@ def test_row_grouping():
     assert np.array_equal(np.sort(rows), np.sort(np.minimum(zi, N)))
 
 
+@pytest.mark.parametrize("name,frames", [('polymer256_fill_smooth', None), ('graphite512_fill', (0, 24)),
+                                         ('polymer1024_smooth', (100, 108))])
+def test_row_pass_shapes_identical(name, frames):
+    """The row-pass shapes the autotuner chooses between give identical bins."""
+    coords, elements, p = _case(name)
+    ids, ftab, counts, _ = engine.element_table(elements, 12700.0)
+    sums = []
+    for v in (0, 1, 2):
+        print(f"[gpu-frames] case={name} row_variant={v}", flush=True)
+        res = engine.run_voxelgrid(coords, elements, p['r'], p['qv'], p['mq'], 12700.0,
+                                   fill_bkg=p['fill'], smooth=p['smooth'], f_table=ftab, elem_ids=ids,
+                                   frames=frames, variant=v)
+        sums.append((res['bins'].sum.cpu().numpy().copy(), res['bins'].cnt.cpu().numpy().copy()))
+    for s_, c_ in sums[1:]:
+        assert np.array_equal(s_, sums[0][0]) and np.array_equal(c_, sums[0][1])
+
+
 def test_stable_regrouping():
     """With the permutation and scratch requested, atoms keep their original order
     within each row (the differentiable path relies on it for reproducible sums)."""
