@@ -79,20 +79,55 @@ __global__ void k_slab_bounds_fin(unsigned long long *keys) {
 }
 
 // work layout: [N * n_elem] per-row element counts | [N + 1] row counts | [N + 1] cursors
-__global__ __launch_bounds__(BLK) void k_slab_rowcount(gx_slab s) {
+//
+// Projected z row of atom a (z_indices, tools/voxelgrids.py:329): N for rows outside
+// [0, N) and for unknown element ids, which never reach a row.
+__device__ __forceinline__ int slab_row_of(const gx_slab &s, int64_t a, double rinv) {
   const int N = (int)s.N;
+  const double zz = s.z[a] - s.zmin;
+  const long long zi = to_int(floor_div(zz, s.r, rinv));
+  int row = (zi >= 0 && zi < N) ? (int)zi : N;
+  const int e = s.elem[a];
+  if (e < 0 || e >= s.n_elem) row = N;
+  return row;
+}
+
+// Row counting and placement work on chunks of SORT_CHUNK atoms per block: counts
+// go to LDS histograms first and reach the global counters once per (block, row),
+// so a row shared by many atoms costs one global atomic per block, not per atom.
+constexpr int SORT_CHUNK = 2048;
+constexpr int SORT_PER_THREAD = SORT_CHUNK / BLK;
+constexpr int SORT_LDS_INTS = 16384;  // 64 KB of LDS histograms at most
+
+// LDS: [N + 1] row counts | [N * n_elem] per-row element counts when elem_lds
+// (otherwise the element counts go straight to the global counters).
+__global__ __launch_bounds__(BLK) void k_slab_rowcount(gx_slab s, int elem_lds) {
+  extern __shared__ int hist[];
+  const int N = (int)s.N, M = N + 1;
+  const int ne = (int)s.n_elem;
+  int *hrow = hist, *hel = hist + M;
+  const int nh = M + (elem_lds ? N * ne : 0);
+  for (int i = threadIdx.x; i < nh; i += BLK) hist[i] = 0;
+  __syncthreads();
   const double rinv = 1.0 / s.r;
   int32_t *ecnt = s.work;
-  int32_t *rcnt = s.work + (int64_t)N * s.n_elem;
-  for (int64_t a = blockIdx.x * (int64_t)BLK + threadIdx.x; a < s.n_atoms; a += (int64_t)gridDim.x * BLK) {
-    double zz = s.z[a] - s.zmin;
-    long long zi = to_int(floor_div(zz, s.r, rinv));
-    int row = (zi >= 0 && zi < N) ? (int)zi : N;
-    const int e = s.elem[a];
-    if (e < 0 || e >= s.n_elem) row = N;  // unknown element ids never reach a row
-    atomicAdd(&rcnt[row], 1);
-    if (row < N) atomicAdd(&ecnt[(int64_t)row * s.n_elem + e], 1);
+  int32_t *rcnt = s.work + (int64_t)N * ne;
+  const int64_t a0 = blockIdx.x * (int64_t)SORT_CHUNK, a1 = min(s.n_atoms, a0 + SORT_CHUNK);
+  for (int64_t a = a0 + threadIdx.x; a < a1; a += BLK) {
+    const int row = slab_row_of(s, a, rinv);
+    atomicAdd(&hrow[row], 1);
+    if (row < N) {
+      const int e = s.elem[a];
+      if (elem_lds) atomicAdd(&hel[row * ne + e], 1);
+      else atomicAdd(&ecnt[(int64_t)row * ne + e], 1);
+    }
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < M; i += BLK)
+    if (hrow[i]) atomicAdd(&rcnt[i], hrow[i]);
+  if (elem_lds)
+    for (int i = threadIdx.x; i < N * ne; i += BLK)
+      if (hel[i]) atomicAdd(&ecnt[i], hel[i]);
 }
 
 // Single block: exclusive scan of the N+1 row counts, per-row fixed-point exponent.
@@ -143,24 +178,44 @@ __global__ __launch_bounds__(1024) void k_slab_scan(gx_slab s) {
   }
 }
 
+// Placement: every atom of the block takes a rank within its row from the LDS
+// counts; the block then reserves one contiguous range per row with a single
+// global atomic on the row's cursor and writes the atoms there.
+// LDS: [N + 1] block counts | [N + 1] block bases.
 __global__ __launch_bounds__(BLK) void k_slab_scatter(gx_slab s) {
-  const int N = (int)s.N;
+  extern __shared__ int hist[];
+  const int N = (int)s.N, M = N + 1;
+  int *cnt = hist, *base = hist + M;
+  for (int i = threadIdx.x; i < M; i += BLK) cnt[i] = 0;
+  __syncthreads();
   const double rinv = 1.0 / s.r;
   int32_t *rcnt = s.work + (int64_t)N * s.n_elem;
-  int32_t *cursor = rcnt + (N + 1);
-  for (int64_t a = blockIdx.x * (int64_t)BLK + threadIdx.x; a < s.n_atoms; a += (int64_t)gridDim.x * BLK) {
-    double zz = s.z[a] - s.zmin;
-    long long zi = to_int(floor_div(zz, s.r, rinv));
-    int row = (zi >= 0 && zi < N) ? (int)zi : N;
+  int32_t *cursor = rcnt + M;
+  const int64_t a0 = blockIdx.x * (int64_t)SORT_CHUNK;
+  int row[SORT_PER_THREAD], rank[SORT_PER_THREAD];
+#pragma unroll
+  for (int k = 0; k < SORT_PER_THREAD; ++k) {
+    const int64_t a = a0 + threadIdx.x + k * BLK;
+    row[k] = a < s.n_atoms ? slab_row_of(s, a, rinv) : -1;
+    rank[k] = row[k] >= 0 ? atomicAdd(&cnt[row[k]], 1) : 0;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < M; i += BLK)
+    if (cnt[i]) base[i] = atomicAdd(&cursor[i], cnt[i]);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SORT_PER_THREAD; ++k) {
+    if (row[k] < 0) continue;
+    const int64_t a = a0 + threadIdx.x + k * BLK;
+    const int rw = row[k];
+    const int pos = base[rw] + rank[k];
     const int e = s.elem[a];
-    if (e < 0 || e >= s.n_elem) row = N;
-    int pos = atomicAdd(&cursor[row], 1);
     s.sx[pos] = s.x[a];
     s.sy[pos] = s.y[a];
-    s.szrow[pos] = row;
+    s.szrow[pos] = rw;
     int qr = 0, qi = 0;
-    if (row < N) {
-      double sc = ldexp(1.0, s.row_exp[row]);
+    if (rw < N) {
+      const double sc = ldexp(1.0, s.row_exp[rw]);
       qr = (int)rint(s.f_table[2 * e] * sc);
       qi = (int)rint(s.f_table[2 * e + 1] * sc);
     }
@@ -168,8 +223,9 @@ __global__ __launch_bounds__(BLK) void k_slab_scatter(gx_slab s) {
     s.sq[2 * (int64_t)pos + 1] = qi;
     if (s.szf) {
       double t = 0.0;
-      if (row < N) {
-        t = zz * rinv - (double)zi;
+      if (rw < N) {
+        const double zz = s.z[a] - s.zmin;
+        t = zz * rinv - (double)to_int(floor_div(zz, s.r, rinv));
         t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
       }
       s.szf[pos] = t;
@@ -186,16 +242,6 @@ __global__ __launch_bounds__(BLK) void k_slab_scatter(gx_slab s) {
 // atoms in order, 256 at a time, ranking equal rows within a wave by ballots and
 // across waves in wave order.
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ int slab_row_of(const gx_slab &s, int64_t a, double rinv) {
-  const int N = (int)s.N;
-  const double zz = s.z[a] - s.zmin;
-  const long long zi = to_int(floor_div(zz, s.r, rinv));
-  int row = (zi >= 0 && zi < N) ? (int)zi : N;
-  const int e = s.elem[a];
-  if (e < 0 || e >= s.n_elem) row = N;
-  return row;
-}
-
 __global__ __launch_bounds__(BLK) void k_slab_stable_count(gx_slab s) {
   extern __shared__ int hist[];  // [N + 1]
   const int M = (int)s.N + 1;
@@ -302,16 +348,6 @@ __global__ void k_frames_init(unsigned long long *ymin_key, int32_t *bounds, int
 
 constexpr int STAT_CHUNK = 16384;  // atoms per block in the statistics kernels
 
-__device__ __forceinline__ double block_min_d(double v, double *sh) {
-  v = wave_min(v);
-  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double r = sh[0];
-  for (int i = 1; i < BLK / GX_WAVE; ++i) r = fmin(r, sh[i]);
-  return r;
-}
-
 __device__ __forceinline__ void block_minmax4(int (&v)[4], int *sh) {
   v[0] = wave_min_i(v[0]);
   v[1] = wave_max_i(v[1]);
@@ -329,37 +365,70 @@ __device__ __forceinline__ void block_minmax4(int (&v)[4], int *sh) {
   }
 }
 
-// One pass per frame over the atoms: min y' = x sin(phi) + y cos(phi) over all
-// atoms (canonical rotate_coords_z, voxelgrids.py:319-323) and min / max y' over
-// the atoms whose z row is inside the grid.  Grid (atom chunks, frames), one
-// atomic per block and key; min/max of doubles are exact, so order-free.
+// min y' = x sin(phi) + y cos(phi) over all atoms (canonical rotate_coords_z,
+// voxelgrids.py:319-323) and min / max y' over the atoms whose z row is inside the
+// grid.  Grid (atom chunks, groups of YMIN_FRAMES frames): each atom is read once per
+// group and rotated for every frame of it; one atomic per block, frame and key.
+// min/max of doubles are exact, so the result does not depend on the order.
+constexpr int YMIN_FRAMES = 8;
+constexpr int YMIN_CHUNK = 8192;
+
 __global__ __launch_bounds__(BLK) void k_frames_ymin(gx_frames d, int64_t f0, int64_t nf) {
-  __shared__ double sh[BLK / GX_WAVE];
-  const int fb = blockIdx.y;
-  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
-  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  constexpr int NW = BLK / GX_WAVE, K = 3 * YMIN_FRAMES;
+  __shared__ double sh[NW][K];
+  const int64_t fg = (int64_t)blockIdx.y * YMIN_FRAMES;
+  const int nfg = (int)min((int64_t)YMIN_FRAMES, nf - fg);
+  double sn[YMIN_FRAMES], cs[YMIN_FRAMES];
+#pragma unroll
+  for (int j = 0; j < YMIN_FRAMES; ++j) {
+    const double *fp = d.fparams + (f0 + fg + min(j, nfg - 1)) * GX_FP_STRIDE;
+    sn[j] = fp[GX_FP_SIN];
+    cs[j] = fp[GX_FP_COS];
+  }
   const int64_t n_rowed = d.row_off[d.N];
-  const int64_t a0 = (int64_t)blockIdx.x * STAT_CHUNK;
-  const int64_t a1 = min((int64_t)d.n_atoms, a0 + STAT_CHUNK);
-  double m = INFINITY, mz = INFINITY, Mz = -INFINITY;
+  const int64_t a0 = (int64_t)blockIdx.x * YMIN_CHUNK;
+  const int64_t a1 = min((int64_t)d.n_atoms, a0 + YMIN_CHUNK);
+  double m[YMIN_FRAMES], mz[YMIN_FRAMES], Mz[YMIN_FRAMES];
+#pragma unroll
+  for (int j = 0; j < YMIN_FRAMES; ++j) {
+    m[j] = INFINITY;
+    mz[j] = INFINITY;
+    Mz[j] = -INFINITY;
+  }
   for (int64_t a = a0 + threadIdx.x; a < a1; a += BLK) {
-    const double y = d.ax[a] * s + d.ay[a] * c;
-    m = fmin(m, y);
-    if (a < n_rowed) {
-      mz = fmin(mz, y);
-      Mz = fmax(Mz, y);
+    const double x = d.ax[a], y = d.ay[a];
+    const bool rowed = a < n_rowed;
+#pragma unroll
+    for (int j = 0; j < YMIN_FRAMES; ++j) {
+      const double v = x * sn[j] + y * cs[j];
+      m[j] = fmin(m[j], v);
+      if (rowed) {
+        mz[j] = fmin(mz[j], v);
+        Mz[j] = fmax(Mz[j], v);
+      }
     }
   }
-  m = block_min_d(m, sh);
+  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
+#pragma unroll
+  for (int j = 0; j < YMIN_FRAMES; ++j) {
+    const double a = wave_min(m[j]), b = wave_min(mz[j]), c = -wave_min(-Mz[j]);
+    if (lane == 0) {
+      sh[w][3 * j] = a;
+      sh[w][3 * j + 1] = b;
+      sh[w][3 * j + 2] = c;
+    }
+  }
   __syncthreads();
-  mz = block_min_d(mz, sh);
-  __syncthreads();
-  Mz = -block_min_d(-Mz, sh);
-  if (threadIdx.x == 0) {
-    unsigned long long *k = d.ymin_key + 3 * fb;
-    if (m < INFINITY) atomicMin(&k[0], dkey(m));
-    if (mz < INFINITY) atomicMin(&k[1], dkey(mz));
-    if (Mz > -INFINITY) atomicMax(&k[2], dkey(Mz));
+  if (threadIdx.x < 3 * nfg) {
+    const int j = threadIdx.x / 3, kind = threadIdx.x - 3 * j;
+    double v = sh[0][threadIdx.x];
+    for (int i = 1; i < NW; ++i) v = kind == 2 ? fmax(v, sh[i][threadIdx.x]) : fmin(v, sh[i][threadIdx.x]);
+    unsigned long long *key = d.ymin_key + 3 * (fg + j) + kind;
+    if (kind == 2) {
+      if (v > -INFINITY) atomicMax(key, dkey(v));
+    } else if (v < INFINITY) {
+      atomicMin(key, dkey(v));
+    }
   }
 }
 
@@ -1303,7 +1372,7 @@ int gx_slab_sort_rows(const gx_slab *s, gx_stream_t stream) {
   GX_REQUIRE(s != nullptr, "NULL slab");
   GX_REQUIRE(s->n_atoms > 0 && s->n_atoms < (1ll << 31), "atom count %lld outside [1, 2^31)",
              (long long)s->n_atoms);
-  GX_REQUIRE(s->N >= 1 && s->N <= (1 << 20), "grid edge N=%lld", (long long)s->N);
+  GX_REQUIRE(s->N >= 1 && s->N <= 4096, "grid edge N=%lld outside [1, 4096]", (long long)s->N);
   GX_REQUIRE(s->r > 0.0, "r_voxel_size must be positive");
   GX_REQUIRE(s->n_elem >= 1 && s->n_elem <= 128, "n_elem=%lld outside [1, 128]", (long long)s->n_elem);
   GX_REQUIRE(s->x && s->y && s->z && s->elem && s->f_table && s->sx && s->sy && s->sq && s->szrow && s->row_off &&
@@ -1312,19 +1381,19 @@ int gx_slab_sort_rows(const gx_slab *s, gx_stream_t stream) {
   hipStream_t st = as_stream(stream);
   const int64_t nwork = s->N * s->n_elem + 2 * (s->N + 1);
   k_fill_i32<<<grid1d(nwork, BLK) > 4096 ? 4096 : grid1d(nwork, BLK), BLK, 0, st>>>(s->work, nwork, 0);
-  unsigned g = grid1d(s->n_atoms, BLK);
-  if (g > 8192) g = 8192;
-  k_slab_rowcount<<<g, BLK, 0, st>>>(*s);
+  const unsigned g = grid1d(s->n_atoms, SORT_CHUNK);
+  const int elem_lds = (s->N + 1) + s->N * s->n_elem <= SORT_LDS_INTS;
+  const size_t lds_count = (size_t)((s->N + 1) + (elem_lds ? s->N * s->n_elem : 0)) * sizeof(int);
+  k_slab_rowcount<<<g, BLK, lds_count, st>>>(*s, elem_lds);
   k_slab_scan<<<1, 1024, 0, st>>>(*s);
   if (s->sperm && s->stable_work) {
-    GX_REQUIRE(s->N <= 4096, "stable regrouping needs N <= 4096 (got %lld)", (long long)s->N);
     const int64_t n_chunks = (s->n_atoms + GX_STABLE_CHUNK - 1) / GX_STABLE_CHUNK;
     const size_t lds = (size_t)(s->N + 1) * sizeof(int);
     k_slab_stable_count<<<(unsigned)n_chunks, BLK, lds, st>>>(*s);
     k_slab_stable_scan<<<grid1d(s->N + 1, BLK), BLK, 0, st>>>(*s, n_chunks);
     k_slab_stable_scatter<<<(unsigned)n_chunks, BLK, lds, st>>>(*s);
   } else {
-    k_slab_scatter<<<g, BLK, 0, st>>>(*s);
+    k_slab_scatter<<<g, BLK, 2 * (size_t)(s->N + 1) * sizeof(int), st>>>(*s);
   }
   return check_launch("gx_slab_sort_rows");
 }
@@ -1337,8 +1406,9 @@ int gx_frames_stats(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stre
   hipStream_t st = as_stream(stream);
   k_frames_init<<<grid1d(nf, BLK), BLK, 0, st>>>(d->ymin_key, d->idx_bounds, nf);
   if (d->n_atoms > 0) {
+    dim3 gy(grid1d(d->n_atoms, YMIN_CHUNK), grid1d(nf, YMIN_FRAMES));
+    k_frames_ymin<<<gy, BLK, 0, st>>>(*d, f0, nf);
     dim3 g(grid1d(d->n_atoms, STAT_CHUNK), (unsigned)nf);
-    k_frames_ymin<<<g, BLK, 0, st>>>(*d, f0, nf);
     k_frames_bounds_fast<<<grid1d(nf, 64), 64, 0, st>>>(*d, nf);
     k_frames_bounds<<<g, BLK, 0, st>>>(*d, f0, nf);
   }

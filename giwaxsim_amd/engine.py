@@ -65,6 +65,46 @@ def _empty(n, dtype):
     return torch.empty(int(max(n, 1)), dtype=dtype, device=device())
 
 
+# Device copies of constant tables (twiddles, chirp-z tables, Gaussian prefix sums,
+# per-plan frame records, element f tables), keyed by what defines them, so that a
+# repeated call with the same geometry uploads nothing.  Bounded; least recently
+# used entries are dropped.
+_DEV_CONST: dict = {}
+_DEV_CONST_MAX = 64
+
+
+def dev_const(key, make, dtype=None):
+    """Cached device tensor of the host array ``make()`` for ``key``.  A key may
+    hold an object whose identity defines the table (a cached plan): its entry is
+    only reused while that same object is alive in the entry."""
+    k = (key, torch.cuda.current_device())
+    hit = _DEV_CONST.pop(k, None)
+    if hit is not None:
+        _DEV_CONST[k] = hit
+        return hit[1]
+    arr = np.ascontiguousarray(make())
+    t = torch.as_tensor(arr).to(device=device(), dtype=dtype).contiguous()
+    _DEV_CONST[k] = (key, t)
+    while len(_DEV_CONST) > _DEV_CONST_MAX:
+        _DEV_CONST.pop(next(iter(_DEV_CONST)))
+    return t
+
+
+class _Ident:
+    """Hashable by identity: a dictionary key for an object that may not be
+    hashable by value (a plan), kept alive by the key itself."""
+    __slots__ = ("obj",)
+
+    def __init__(self, obj):
+        self.obj = obj
+
+    def __hash__(self):
+        return id(self.obj)
+
+    def __eq__(self, other):
+        return isinstance(other, _Ident) and other.obj is self.obj
+
+
 # --------------------------------------------------------------------------------------
 # element tables
 # --------------------------------------------------------------------------------------
@@ -407,24 +447,30 @@ def bin_scale_for(terms):
 # --------------------------------------------------------------------------------------
 # device runner of the phi-frame loop
 # --------------------------------------------------------------------------------------
-class Slab:
-    """Atoms of one slab on the device, regrouped by projected z row."""
+def f_table_dev(f_table):
+    """Device [n_elem][2] (re, im) copy of a complex f table (cached by value)."""
+    f_table = np.asarray(f_table, dtype=np.complex128)
 
-    def __init__(self, coords, elem_ids, f_table, N, r, zmin):
-        dev = device()
-        c = to_dev(coords, torch.float64)
-        self.n = int(c.shape[0])
-        self.x = c[:, 0].contiguous()
-        self.y = c[:, 1].contiguous()
-        self.z = c[:, 2].contiguous()
-        self.elem = to_dev(elem_ids, torch.int32)
+    def make():
         ft = np.empty((len(f_table), 2), dtype=np.float64)
         ft[:, 0] = np.real(f_table)
         ft[:, 1] = np.imag(f_table)
-        self.f_table = torch.as_tensor(ft).to(dev).contiguous()
-        self.n_elem = len(f_table)
-        self.N, self.r, self.zmin = int(N), float(r), float(zmin)
-        n = self.n
+        return ft
+    return dev_const(("f_table", f_table.tobytes()), make)
+
+
+class Slab:
+    """Atoms of one slab on the device, regrouped by projected z row.
+    ``columns``: the (x, y, z, elem) device columns when the caller already has
+    them (slab_bounds_device returns them).  A slab can be pointed at new atoms of
+    the same count (``bind``) and regrouped again (``sort``) without reallocating."""
+
+    def __init__(self, coords, elem_ids, f_table, N, r, zmin, columns=None, sort=True):
+        if columns is None:
+            columns = device_columns(coords, elem_ids)
+        n = int(columns[0].shape[0])
+        self.n, self.n_elem = n, len(f_table)
+        self.N, self.r = int(N), float(r)
         self.sx = _empty(n, torch.float64)
         self.sy = _empty(n, torch.float64)
         self.sq = _empty(2 * n, torch.int32)
@@ -433,27 +479,50 @@ class Slab:
         self.row_exp = _empty(N, torch.int32)
         self.work = _empty(N * self.n_elem + 2 * (N + 1), torch.int32)
         self.desc = GxSlab(
-            n_atoms=n, n_elem=self.n_elem, N=self.N, r=self.r, zmin=self.zmin,
-            x=ptr(self.x).value, y=ptr(self.y).value, z=ptr(self.z).value,
-            elem=ptr(self.elem).value, f_table=ptr(self.f_table).value,
+            n_atoms=n, n_elem=self.n_elem, N=self.N, r=self.r,
             sx=ptr(self.sx).value, sy=ptr(self.sy).value, sq=ptr(self.sq).value,
             szrow=ptr(self.szrow).value, row_off=ptr(self.row_off).value,
             row_exp=ptr(self.row_exp).value, work=ptr(self.work).value)
+        self.bind(columns, f_table, zmin)
+        if sort:
+            self.sort()
+
+    def bind(self, columns, f_table, zmin):
+        """Point the slab at the device columns (x, y, z, elem) of atoms of the same
+        count, with their f table and min z."""
+        x, y, z, el = columns
+        if int(x.shape[0]) != self.n or len(f_table) != self.n_elem:
+            raise ValueError("bind: atom count or element count differs from the slab's")
+        self.x, self.y, self.z, self.elem = x, y, z, el
+        self.f_table = f_table_dev(f_table)
+        self.zmin = float(zmin)
+        d = self.desc
+        d.x, d.y, d.z, d.elem = ptr(x).value, ptr(y).value, ptr(z).value, ptr(el).value
+        d.f_table, d.zmin = ptr(self.f_table).value, self.zmin
+
+    def sort(self):
         call("gx_slab_sort_rows", ctypes.byref(self.desc), stream_handle())
 
 
-def slab_bounds_device(coords, elem_ids, n_elem):
-    """(x_bound, y_bound, z_bound, zmin, elem_counts) of device-resident atoms."""
-    c = to_dev(coords, torch.float64)
-    x, y, z = c[:, 0].contiguous(), c[:, 1].contiguous(), c[:, 2].contiguous()
-    el = to_dev(elem_ids, torch.int32)
-    out = _empty(6, torch.float64)
-    cnt = _empty(n_elem, torch.int64)
-    d = GxSlab(n_atoms=int(c.shape[0]), n_elem=int(n_elem), x=ptr(x).value, y=ptr(y).value,
+def device_columns(coords, elem_ids):
+    """(x, y, z, elem) contiguous device columns of host or device atoms."""
+    x, y, z = to_dev(coords, torch.float64).t().contiguous()   # one transpose, three rows
+    return x, y, z, to_dev(elem_ids, torch.int32)
+
+
+def slab_bounds_device(coords, elem_ids, n_elem, with_columns=False):
+    """(x_bound, y_bound, z_bound, zmin, elem_counts) of device-resident atoms, one
+    read-back; with ``with_columns`` also the (x, y, z, elem) device columns."""
+    x, y, z, el = device_columns(coords, elem_ids)
+    out = _empty(6 + n_elem, torch.float64)   # min/max (FP64) | counts (int64 words)
+    cnt = out[6:].view(torch.int64)
+    d = GxSlab(n_atoms=int(x.shape[0]), n_elem=int(n_elem), x=ptr(x).value, y=ptr(y).value,
                z=ptr(z).value, elem=ptr(el).value)
     call("gx_slab_bounds", ctypes.byref(d), ptr(out), ptr(cnt), stream_handle())
-    mm = out.cpu().numpy()
-    return (mm[1] - mm[0], mm[3] - mm[2], mm[5] - mm[4], mm[4], cnt.cpu().numpy())
+    host = out.cpu()
+    mm = host[:6].numpy()
+    res = (mm[1] - mm[0], mm[3] - mm[2], mm[5] - mm[4], mm[4], host[6:].view(torch.int64).numpy())
+    return res + ((x, y, z, el),) if with_columns else res
 
 
 class Bins:
@@ -500,15 +569,16 @@ class FrameRunner:
             # keep the batch's intermediate (batch * W * Z complex) within ~160 MB
             batch = max(1, min(F, int(160e6 // (W * Z * esz))))
         self.batch = int(batch)
-        self.fparams = torch.as_tensor(plan.fparams).to(dev).contiguous()
-        self.tw = torch.as_tensor(twiddles(N, self.f64)).to(dev).contiguous()
-        self.tw64 = torch.as_tensor(twiddles(N, True)).to(dev).contiguous()
+        self.fparams = dev_const(("fparams", _Ident(plan)), lambda: plan.fparams)
+        self.tw = dev_const(("twiddles", N, self.f64), lambda: twiddles(N, self.f64))
+        self.tw64 = dev_const(("twiddles", N, True), lambda: twiddles(N, True))
         if is_pow2_grid(N):
             self.bs_M, self.bs = 0, (None, None, None)
         else:
             M, ch, K, twm = chirpz_tables(N, self.f64)
             self.bs_M = M
-            self.bs = tuple(torch.as_tensor(a).to(dev).contiguous() for a in (ch, K, twm))
+            self.bs = tuple(dev_const(("chirpz", N, self.f64, i), lambda a=a: a)
+                            for i, a in enumerate((ch, K, twm)))
         B = self.batch
         self.ymin_key = _empty(3 * B, torch.int64)
         self.idx_bounds = _empty(4 * B, torch.int32)
@@ -521,7 +591,7 @@ class FrameRunner:
         self.status = torch.zeros(4, dtype=torch.int32, device=dev)
         if smooth:
             cw, rad = gauss_prefix(smooth)
-            self.gauss = torch.as_tensor(cw).to(dev).contiguous()
+            self.gauss = dev_const(("gauss_prefix", int(smooth)), lambda: cw)
         else:
             self.gauss, rad = None, 0
         zr = int(np.floor_divide(np.float64(plan.z_bound), np.float64(plan.r))) \
@@ -550,6 +620,13 @@ class FrameRunner:
         if variant is None and is_pow2_grid(N):
             self.desc.row_variant = tuned_variant(self)
 
+    def rebind(self, avg):
+        """New avg_voxel_f and the bins' current scale, status cleared: the runner
+        is ready for a slab that was re-bound to new atoms of the same geometry."""
+        self.desc.avg_re, self.desc.avg_im = float(np.real(avg)), float(np.imag(avg))
+        self.desc.bin_scale = self.bins.scale
+        self.status.zero_()
+
     def run(self, frames=None):
         """Enqueue frames [lo, hi) (default: all) on the current stream."""
         lo, hi = (0, self.plan.n_frames) if frames is None else frames
@@ -567,18 +644,30 @@ class FrameRunner:
             raise RuntimeError("internal: a frame's column band exceeded the planned width")
 
 
+_WORKSPACES: dict = {}
+_WORKSPACES_MAX = 4
+
+
 def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, fill_bkg=False,
                   smooth=0, f_table=None, elem_ids=None, batch=None, want64=True, frames=None,
                   precision='auto', variant=None):
     """Full voxel-grid path of one slab.  Returns dict with device iq32/iq64,
     host axes and the objects that produced them.  ``frames=(lo, hi)``
-    restricts the run to phi-frames [lo, hi) (bounded samples)."""
+    restricts the run to phi-frames [lo, hi) (bounded samples).
+
+    The device buffers and descriptors (slab, bins, frame runner) are kept per
+    (geometry plan, atom and element counts, options, device, stream) and reused
+    by the next call with the same key; iq64 / iq32 are new tensors every call,
+    while the returned 'slab', 'bins' and 'runner' objects are overwritten by
+    that next call."""
     if f_table is None:
         elem_ids, f_table, counts, uniq = element_table(elements, energy)
     elif not torch.is_tensor(elem_ids):
         counts = np.bincount(np.asarray(elem_ids), minlength=len(f_table))
+    columns = None
     if torch.is_tensor(coords):
-        xb, yb, zb, zmin, counts = slab_bounds_device(coords, elem_ids, len(f_table))
+        xb, yb, zb, zmin, counts, columns = slab_bounds_device(coords, elem_ids, len(f_table),
+                                                              with_columns=True)
     else:
         coords = np.asarray(coords, dtype=np.float64)
         xb = np.max(coords[:, 0]) - np.min(coords[:, 0])
@@ -598,13 +687,32 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
     B = intensity_bound(sum_abs, plan.N, fill_bkg, smooth, avg, plan.max_voxels, yb, xb,
                         r_voxel_size)
     scale = bin_scale_for([(plan.n_frames, pixels_per_bin(plan), B)])
-    bins = Bins(plan.grid, scale)
-    slab = Slab(coords, elem_ids, f_table, plan.N, r_voxel_size, zmin)
-    runner = FrameRunner(plan, slab, bins, fill_bkg, smooth, avg, batch=batch, precision=precision,
-                         variant=variant)
+    if columns is None:
+        columns = device_columns(coords, elem_ids)
+    key = (_Ident(plan), int(columns[0].shape[0]), len(f_table), bool(fill_bkg), int(smooth),
+           precision, batch, variant, torch.cuda.current_device(),
+           torch.cuda.current_stream().cuda_stream)
+    ws = _WORKSPACES.pop(key, None)
+    if ws is None:
+        bins = Bins(plan.grid, scale)
+        slab = Slab(None, None, f_table, plan.N, r_voxel_size, zmin, columns=columns)
+        runner = FrameRunner(plan, slab, bins, fill_bkg, smooth, avg, batch=batch,
+                             precision=precision, variant=variant)
+    else:
+        slab, bins, runner = ws
+        bins.scale = float(scale)
+        bins.clear()
+        slab.bind(columns, f_table, zmin)
+        slab.sort()
+        runner.rebind(avg)
+    _WORKSPACES[key] = (slab, bins, runner)
+    while len(_WORKSPACES) > _WORKSPACES_MAX:
+        _WORKSPACES.pop(next(iter(_WORKSPACES)))
     runner.run(frames)
-    runner.check()
+    # the finalize is enqueued before the status read-back, so the device does not
+    # wait on the host between the frames and the finalize
     iq64, iq32 = bins.finalize(want64=want64, want32=True)
+    runner.check()
     ax = plan.grid.crop_axis
     return dict(iq64=iq64, iq32=iq32, qx=ax.copy(), qy=ax.copy(), qz=ax.copy(), plan=plan,
                 bins=bins, runner=runner, slab=slab, avg=avg)
