@@ -371,7 +371,8 @@ __device__ __forceinline__ void block_minmax4(int (&v)[4], int *sh) {
 // group and rotated for every frame of it; one atomic per block, frame and key.
 // min/max of doubles are exact, so the result does not depend on the order.
 constexpr int YMIN_FRAMES = 8;
-constexpr int YMIN_CHUNK = 8192;
+constexpr int YMIN_PER_THREAD = 8;
+constexpr int YMIN_CHUNK = YMIN_PER_THREAD * BLK;
 
 __global__ __launch_bounds__(BLK) void k_frames_ymin(gx_frames d, int64_t f0, int64_t nf) {
   constexpr int NW = BLK / GX_WAVE, K = 3 * YMIN_FRAMES;
@@ -386,8 +387,15 @@ __global__ __launch_bounds__(BLK) void k_frames_ymin(gx_frames d, int64_t f0, in
     cs[j] = fp[GX_FP_COS];
   }
   const int64_t n_rowed = d.row_off[d.N];
-  const int64_t a0 = (int64_t)blockIdx.x * YMIN_CHUNK;
-  const int64_t a1 = min((int64_t)d.n_atoms, a0 + YMIN_CHUNK);
+  const int64_t a0 = (int64_t)blockIdx.x * YMIN_CHUNK + threadIdx.x;
+  // every load of the thread's atoms is issued before the first use
+  double x[YMIN_PER_THREAD], y[YMIN_PER_THREAD];
+#pragma unroll
+  for (int k = 0; k < YMIN_PER_THREAD; ++k) {
+    const int64_t a = a0 + k * BLK;
+    x[k] = a < d.n_atoms ? d.ax[a] : 0.0;
+    y[k] = a < d.n_atoms ? d.ay[a] : 0.0;
+  }
   double m[YMIN_FRAMES], mz[YMIN_FRAMES], Mz[YMIN_FRAMES];
 #pragma unroll
   for (int j = 0; j < YMIN_FRAMES; ++j) {
@@ -395,12 +403,14 @@ __global__ __launch_bounds__(BLK) void k_frames_ymin(gx_frames d, int64_t f0, in
     mz[j] = INFINITY;
     Mz[j] = -INFINITY;
   }
-  for (int64_t a = a0 + threadIdx.x; a < a1; a += BLK) {
-    const double x = d.ax[a], y = d.ay[a];
+#pragma unroll
+  for (int k = 0; k < YMIN_PER_THREAD; ++k) {
+    const int64_t a = a0 + k * BLK;
+    if (a >= d.n_atoms) break;
     const bool rowed = a < n_rowed;
 #pragma unroll
     for (int j = 0; j < YMIN_FRAMES; ++j) {
-      const double v = x * sn[j] + y * cs[j];
+      const double v = x[k] * sn[j] + y[k] * cs[j];
       m[j] = fmin(m[j], v);
       if (rowed) {
         mz[j] = fmin(mz[j], v);
@@ -433,33 +443,38 @@ __global__ __launch_bounds__(BLK) void k_frames_ymin(gx_frames d, int64_t f0, in
 }
 
 // Index bounds from the extremes (floor division is monotone): valid when every
-// atom with z row < N also has y row < N and every atom's z row is < N; frames
-// where that fails are left at their initial values for k_frames_bounds.
-__global__ void k_frames_bounds_fast(gx_frames d, int64_t nf) {
-  const int fb = blockIdx.x * blockDim.x + threadIdx.x;
-  if (fb >= nf) return;
+// atom with z row < N also has y row < N and every atom's z row is < N; false for
+// frames where that fails, which take the per-atom pass below.
+__device__ __forceinline__ bool frame_bounds_fast(const gx_frames &d, int fb, int32_t (&b)[4]) {
   const unsigned long long *k = d.ymin_key + 3 * fb;
-  int32_t *b = d.idx_bounds + 4 * fb;
-  if (k[1] == ~0ull || d.zrow_max < 0) return;
+  if (d.zrow_max < 0) return false;
   const double ymin = dkey_inv(k[0]);
   const double rinv = 1.0 / d.r;
   const long long lo = to_int(floor_div(dkey_inv(k[1]) - ymin, d.r, rinv));
   const long long hi = to_int(floor_div(dkey_inv(k[2]) - ymin, d.r, rinv));
-  if (hi >= d.N) return;
+  if (hi >= d.N) return false;
   b[0] = (int32_t)lo;
   b[1] = (int32_t)hi;
   b[2] = 0;
   b[3] = (int32_t)d.zrow_max;
+  return true;
 }
 
-// y/z index bounds over valid atoms (voxelgrids.py:328-335, 346-349), per atom;
-// only for frames the fast path above could not settle.
+// y/z index bounds over valid atoms (voxelgrids.py:328-335, 346-349): from the
+// extremes when the fast path above settles the frame (block 0 writes them), else
+// per atom.  Every block of a frame takes the same branch.
 __global__ __launch_bounds__(BLK) void k_frames_bounds(gx_frames d, int64_t f0, int64_t nf) {
   __shared__ int sh[4 * (BLK / GX_WAVE)];
   const int N = (int)d.N;
   const double rinv = 1.0 / d.r;
   const int fb = blockIdx.y;
-  if (d.idx_bounds[4 * fb] != INT_MAX || d.ymin_key[3 * fb + 1] == ~0ull) return;  // block-uniform
+  if (d.ymin_key[3 * fb + 1] == ~0ull) return;  // no atom with a z row inside the grid
+  int32_t fast[4];
+  if (frame_bounds_fast(d, fb, fast)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      for (int k = 0; k < 4; ++k) d.idx_bounds[4 * fb + k] = fast[k];
+    return;
+  }
   const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
   const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
   const double ymin = dkey_inv(d.ymin_key[3 * fb]);
@@ -1409,7 +1424,6 @@ int gx_frames_stats(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stre
     dim3 gy(grid1d(d->n_atoms, YMIN_CHUNK), grid1d(nf, YMIN_FRAMES));
     k_frames_ymin<<<gy, BLK, 0, st>>>(*d, f0, nf);
     dim3 g(grid1d(d->n_atoms, STAT_CHUNK), (unsigned)nf);
-    k_frames_bounds_fast<<<grid1d(nf, 64), 64, 0, st>>>(*d, nf);
     k_frames_bounds<<<g, BLK, 0, st>>>(*d, f0, nf);
   }
   return check_launch("gx_frames_stats");
