@@ -188,12 +188,12 @@ class DifferentiableSlab:
         self.det_raw = _empty(npx * npx, torch.float64)
         self.det_out = _empty(npx * npx, torch.float64)
         self.det_work = _empty(2 * npx * npx + (npx * npx + 255) // 256, torch.float64)
-        # the detector adjoint gathers through a transpose map of the orientation
-        # gather when it fits (<= 8 GB of pair ids), else scatters into copies of its
-        # accumulator (<= 256 MB)
+        # the detector adjoint goes through a transpose map of the orientation gather
+        # when it fits (<= 8 GB of pair ids and their voxels), else scatters into copies
+        # of its accumulator (<= 256 MB)
         n_pairs = npx * npx * len(dp.orient_w)
         pbits = int(np.ceil(np.log2(max(npx * npx, 1))))
-        self.mapped = n_pairs * 4 <= 8e9 and pbits < 32 and len(dp.orient_w) <= 2 ** (32 - pbits)
+        self.mapped = n_pairs * 8 <= 8e9 and pbits < 32 and len(dp.orient_w) <= 2 ** (32 - pbits)
         self.n_rep = 1 if self.mapped else int(max(1, min(8, 256e6 // (8 * nb))))
         self.g_fx = _empty(self.n_rep * nb, torch.int64)
         self.det_map = None
@@ -211,10 +211,11 @@ class DifferentiableSlab:
             scan = _empty(-(-nb // 2048) + 1, torch.int64)
             off = _empty(nb + 1, torch.int64)
             pairs = _empty(n_pairs, torch.int32)
+            pair_voxel = _empty(n_pairs, torch.int32)
             call("gx_detector_map_build", ctypes.byref(self.detector_desc), ptr(cnt), ptr(scan), ptr(off),
-                 ptr(pairs), stream_handle())
-            self.det_map = (off, pairs)
-            del cnt, scan
+                 ptr(pairs), ptr(pair_voxel), stream_handle())
+            self.det_map = (pair_voxel, pairs)
+            del cnt, scan, off
         self._generation = 0
         self._prepared = None
 

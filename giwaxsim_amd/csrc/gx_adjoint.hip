@@ -16,7 +16,8 @@
 // min y' in d.ymin_key is reused):
 //   k_adj_rowpass<N>  as above
 //   k_adj_colpass<N>  per kept column: FFT along z, cotangent 2 g_bins[bin] F on the
-//                     binned pixels, adjoint DFT (conj FFT conj) back to z, in place
+//                     binned pixels, adjoint DFT (conj FFT conj) back to z, in place;
+//                     also resets each frame's argmin slot for k_adj_rowgrad
 //   k_adj_rowgrad<N>  per block of rows (+1 halo row): adjoint DFT along y of the kept
 //                     band, then every atom of the rows gathers its four nodes:
 //                     dL/dy', dL/dz, dL/df per (frame, atom) into g_part
@@ -86,40 +87,38 @@ __device__ __forceinline__ int block_min_int(int v, int *sh) {
 // Deposit exponent of row v: the row receives f w from atoms of rows v-1 and v with
 // w <= 1, so 2^S sum |f| + (rounding) stays below 2^62 (int64 accumulators, FP64
 // transforms) or 2^31 - 1 (int32, FP32 transforms).
-__global__ __launch_bounds__(BLK) void k_adj_rowexp(gx_frames d, gx_adjoint g) {
-  const int N = (int)d.N;
-  for (int v = blockIdx.x * BLK + threadIdx.x; v < N; v += gridDim.x * BLK) {
-    double bound = 0.0;
-    double cnt = 0.0;
-    for (int e = 0; e < g.n_elem; ++e) {
-      const double fm = fmax(fabs(g.f_table[2 * e]), fabs(g.f_table[2 * e + 1]));
-      double c = (double)g.ecnt[(int64_t)v * g.n_elem + e];
-      if (v > 0) c += (double)g.ecnt[(int64_t)(v - 1) * g.n_elem + e];
-      bound += c * fm;
-      cnt += c;
-    }
-    int S = 0;
-    if (bound > 0.0) {
-      const double room = d.fft_f64 ? 4611686018427387904.0 : 2147483647.0;  // int64 / int32 accumulators
-      S = ilogb((room - 2.0 * cnt - 2.0) / bound);
-      if (S > 200) S = 200;
-      if (S < -200) S = -200;
-    }
-    g.row_exp[v] = S;
+__device__ __forceinline__ int adj_row_exp(const gx_frames &d, const gx_adjoint &g, int v) {
+  double bound = 0.0;
+  double cnt = 0.0;
+  for (int e = 0; e < g.n_elem; ++e) {
+    const double fm = fmax(fabs(g.f_table[2 * e]), fabs(g.f_table[2 * e + 1]));
+    double c = (double)g.ecnt[(int64_t)v * g.n_elem + e];
+    if (v > 0) c += (double)g.ecnt[(int64_t)(v - 1) * g.n_elem + e];
+    bound += c * fm;
+    cnt += c;
   }
+  int S = 0;
+  if (bound > 0.0) {
+    const double room = d.fft_f64 ? 4611686018427387904.0 : 2147483647.0;  // int64 / int32 accumulators
+    S = ilogb((room - 2.0 * cnt - 2.0) / bound);
+    if (S > 200) S = 200;
+    if (S < -200) S = -200;
+  }
+  return S;
 }
 
-__global__ __launch_bounds__(BLK) void k_adj_init(gx_frames d, gx_adjoint g) {
+// Per-call preparation in one launch: the rows' deposit exponents, the inverse of the
+// regrouping permutation, zeroed per-atom gradient accumulators and avg_voxel_f slots.
+__global__ __launch_bounds__(BLK) void k_adj_prepare(gx_frames d, gx_adjoint g) {
+  const int64_t t0 = blockIdx.x * (int64_t)BLK + threadIdx.x, stride = (int64_t)gridDim.x * BLK;
   const int64_t n = d.n_atoms;
-  for (int64_t a = blockIdx.x * (int64_t)BLK + threadIdx.x; a < n; a += (int64_t)gridDim.x * BLK) {
+  if (t0 < 2 && g.g_avg) g.g_avg[t0] = 0.0;
+  for (int64_t v = t0; v < d.N; v += stride) g.row_exp[v] = adj_row_exp(d, g, (int)v);
+  for (int64_t a = t0; a < n; a += stride) {
     g.ainv[g.aperm[a]] = (int32_t)a;
 #pragma unroll
     for (int k = 0; k < 5; ++k) g.g_atom[k * n + a] = 0.0;
   }
-}
-
-__global__ void k_zero_f64(double *p, int64_t n) {
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
 }
 
 __global__ __launch_bounds__(BLK) void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -282,6 +281,7 @@ __global__ __launch_bounds__(BLK) void k_adj_colpass(gx_frames d, gx_adjoint g, 
   const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
   const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
   const int wcol = blockIdx.x * NWAVE + wave;
+  if (blockIdx.x == 0 && threadIdx.x == 0) g.amin[fb] = INT_MAX;  // argmin slot of k_adj_rowgrad
   if (wcol > jhi - jlo || wcol >= d.W) return;  // wave-uniform; no block barrier below
   const bool dense = d.fill_bkg || d.smooth;
   double2 *apart = reinterpret_cast<double2 *>(g.avg_part);
@@ -507,6 +507,7 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   C *rows = reinterpret_cast<C *>(smem_raw);  // [H][N] cotangent of the grid rows
   C *stage = rows;                            // [W][H], consumed before rows is written
   double *ftab = reinterpret_cast<double *>(rows + H * N);
+  double *msk = ftab + 2 * g.n_elem;          // [N] y part of the node mask (my iny)
   __shared__ double shd[NWAVE];
   __shared__ int shi[NWAVE];
   const int lane = threadIdx.x & (GX_WAVE - 1);
@@ -519,6 +520,14 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   const int64_t n = d.n_atoms;
 
   for (int i = threadIdx.x; i < 2 * g.n_elem; i += BLK) ftab[i] = g.f_table[i];
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
+  const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
+  const bool masked = d.fill_bkg || d.smooth;
+  if (masked)
+    for (int y = threadIdx.x; y < N; y += BLK)
+      msk[y] = (y >= ya && y < yb) ? (d.smooth ? d.col_sy[(int64_t)fb * N + y] : 1.0) : 0.0;
   const C *src = reinterpret_cast<const C *>(d.inter) + (int64_t)fb * d.W * d.Z;
   for (int idx = threadIdx.x; idx < wf * H; idx += BLK) {
     const int w = idx / H, q = idx - w * H;
@@ -543,12 +552,8 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   }
   __syncthreads();
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
-  // node mask M = (mz inz) x (my iny) of the deposit (1 without fill and smoothing)
-  const int32_t *bnd = d.idx_bounds + 4 * fb;
-  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
-  const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
-  const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
-  const bool masked = d.fill_bkg || d.smooth;
+  // node mask M = (mz inz) x (my iny) of the deposit (1 without fill and smoothing),
+  // applied to the transformed row as it leaves the registers
 #pragma unroll
   for (int h = 0; h < HW; ++h) {
     const int q = wave + NWAVE * h;
@@ -560,18 +565,21 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
     C *buf = rows + q * N;
     fft_wave<N, C>(x[h], buf, tw, lane);
     wave_sync();
+    if (masked) {
 #pragma unroll
-    for (int t = 0; t < P / RL; ++t)
+      for (int t = 0; t < P / RL; ++t)
 #pragma unroll
-      for (int qq = 0; qq < RL; ++qq) buf[lane + GX_WAVE * t + qq * (N / RL)] = cconj(x[h][t * RL + qq]);
-    if (masked) {  // each lane rescales the entries it wrote
-#pragma unroll 2
-      for (int y = lane; y < N; y += GX_WAVE) {
-        double m = (y >= ya && y < yb) ? mzv : 0.0;
-        if (d.smooth) m = m * d.col_sy[(int64_t)fb * N + y];
-        const C o = buf[y];
-        buf[y] = C{(T)((double)o.x * m), (T)((double)o.y * m)};
-      }
+        for (int qq = 0; qq < RL; ++qq) {
+          const int y = lane + GX_WAVE * t + qq * (N / RL);
+          const double m = mzv * msk[y];
+          const C o = cconj(x[h][t * RL + qq]);
+          buf[y] = C{(T)((double)o.x * m), (T)((double)o.y * m)};
+        }
+    } else {
+#pragma unroll
+      for (int t = 0; t < P / RL; ++t)
+#pragma unroll
+        for (int qq = 0; qq < RL; ++qq) buf[lane + GX_WAVE * t + qq * (N / RL)] = cconj(x[h][t * RL + qq]);
     }
   }
   __syncthreads();
@@ -629,25 +637,25 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
 // ------------------------------------------------------------------------------------
 // reductions (fixed partitions of the regrouped atom order: bitwise reproducible)
 // ------------------------------------------------------------------------------------
-// One thread per frame: S_f = sum of the frame's nb row-block partials (block order).
-__global__ __launch_bounds__(BLK) void k_adj_frame_sums(gx_adjoint g, int64_t nf, int nb, double *S) {
-  const int64_t fb = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (fb >= nf) return;
-  double acc = 0.0;
-  for (int b = 0; b < nb; ++b) acc += g.red[fb * nb + b];
-  S[fb] = acc;
-}
-
-// Origin term, one thread per frame: d(min y')/dy'_a = [a = argmin_f], so the argmin
-// atom's dL/dy' of frame f becomes dL/dy' - S_f.  Frames own distinct rows of g_part.
+// Origin term, one wave per frame: S_f = the sum of the frame's nb row-block partials
+// of dL/dy' (lane l adds blocks l, l + 64, ... in order, then a fixed butterfly), and
+// since d(min y')/dy'_a = [a = argmin_f], the argmin atom's dL/dy' of frame f becomes
+// dL/dy' - S_f.  Frames own distinct rows of g_part.
 template <typename T>
-__global__ __launch_bounds__(BLK) void k_adj_frame_origin(gx_frames d, gx_adjoint g, int64_t nf, const double *S) {
-  const int64_t fb = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (fb >= nf) return;
+__global__ __launch_bounds__(BLK) void k_adj_frame_origin(gx_frames d, gx_adjoint g, int64_t nf, int nb, double *S) {
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int64_t fb = (blockIdx.x * (int64_t)BLK + threadIdx.x) / GX_WAVE;
+  if (fb >= nf) return;  // wave-uniform
+  double acc = 0.0;
+  for (int b = lane; b < nb; b += GX_WAVE) acc += g.red[fb * nb + b];
+#pragma unroll
+  for (int o = GX_WAVE / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, GX_WAVE);
+  if (lane != 0) return;
+  S[fb] = acc;
   const int am = g.amin[fb];
   if (am == INT_MAX) return;
   T *u = reinterpret_cast<T *>(g.g_part) + fb * 4 * d.n_atoms + am;
-  *u = (T)((double)*u - S[fb]);
+  *u = (T)((double)*u - acc);
 }
 
 // g_atom += sum over the batch's frames of (sin, cos) dL/dy', dL/dz, dL/df.
@@ -842,8 +850,8 @@ int launch_adj_colpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, in
 template <int N, typename C>
 int launch_adj_rowgrad_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int H = RowGradShape<N, C>::H, RB = H - 1;
-  // rows (the staging tile aliases them, W <= N) + f table
-  const size_t lds = (size_t)H * N * sizeof(C) + (size_t)g->n_elem * 16;
+  // rows (the staging tile aliases them, W <= N) + f table + y mask
+  const size_t lds = (size_t)H * N * sizeof(C) + (size_t)g->n_elem * 16 + (size_t)N * 8;
   if (lds > 150 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "adjoint row pass needs %zu bytes of LDS (N=%d, W=%lld)", lds, N,
                      (long long)d->W);
@@ -942,12 +950,8 @@ int gx_adj_prepare(const gx_frames *d, const gx_adjoint *g, gx_stream_t stream) 
   if (rc) return rc;
   GX_REQUIRE(g->g_atom, "g_atom is NULL");
   hipStream_t st = as_stream(stream);
-  k_adj_rowexp<<<grid1d(d->N, BLK), BLK, 0, st>>>(*d, *g);
-  if (g->g_avg) k_zero_f64<<<1, 2, 0, st>>>(g->g_avg, 2);
-  if (d->n_atoms > 0) {
-    unsigned gr = grid1d(d->n_atoms, BLK);
-    k_adj_init<<<gr > 4096 ? 4096 : gr, BLK, 0, st>>>(*d, *g);
-  }
+  unsigned gr = grid1d(d->n_atoms > d->N ? d->n_atoms : d->N, BLK);
+  k_adj_prepare<<<gr > 4096 ? 4096 : gr, BLK, 0, st>>>(*d, *g);
   return check_launch("gx_adj_prepare");
 }
 
@@ -989,11 +993,11 @@ int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t n
   if (rc || nf == 0) return rc;
   hipStream_t st = as_stream(stream);
   double *S = g->red + d->batch * d->N;  // per-frame sums of dL/dy'
-  k_adj_frame_sums<<<grid1d(nf, BLK), BLK, 0, st>>>(*g, nf, rowgrad_blocks(d), S);
+  const unsigned gw = grid1d(nf * GX_WAVE, BLK);  // a wave per frame
   if (d->fft_f64)
-    k_adj_frame_origin<double><<<grid1d(nf, BLK), BLK, 0, st>>>(*d, *g, nf, S);
+    k_adj_frame_origin<double><<<gw, BLK, 0, st>>>(*d, *g, nf, rowgrad_blocks(d), S);
   else
-    k_adj_frame_origin<float><<<grid1d(nf, BLK), BLK, 0, st>>>(*d, *g, nf, S);
+    k_adj_frame_origin<float><<<gw, BLK, 0, st>>>(*d, *g, nf, rowgrad_blocks(d), S);
   if (d->n_atoms > 0) {
     if (d->fft_f64)
       k_adj_reduce<double><<<grid1d(d->n_atoms, RA), BLK, 0, st>>>(*d, *g, f0, nf);
@@ -1020,7 +1024,7 @@ int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t
     k_adj_avg_reduce<<<1, BLK, 0, st>>>(*d, *g, nf);
     if ((rc = check_launch("k_adj_avg_reduce"))) return rc;
   }
-  k_fill_i32<<<grid1d(nf, BLK), BLK, 0, st>>>(g->amin, nf, INT_MAX);
+  // k_adj_colpass has reset the frames' argmin slots
   if ((rc = dispatch(OP_ROWGRAD, d, g, f0, nf, st))) return rc;
   return gx_adj_reduce(d, g, f0, nf, stream);
 }

@@ -273,7 +273,7 @@ __global__ __launch_bounds__(BLK) void k_scan_add(int64_t *off, int64_t n, const
 }
 
 __global__ __launch_bounds__(BLK) void k_det_map_fill(gx_detector d, const int64_t *off, int32_t *cur, uint32_t *pairs,
-                                                      int pbits) {
+                                                      int32_t *pair_voxel, int pbits) {
   const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
   if (p >= d.n_pix * d.n_pix) return;
   double x0, y0, z0;
@@ -283,30 +283,58 @@ __global__ __launch_bounds__(BLK) void k_det_map_fill(gx_detector d, const int64
     const int64_t v = det_voxel(d, x0, y0, z0, o, dqinv);
     const int64_t pos = off[v] + atomicAdd(&cur[v], 1);
     pairs[pos] = ((uint32_t)o << pbits) | (uint32_t)p;
+    pair_voxel[pos] = (int32_t)v;
   }
 }
 
-// dL/d iq per voxel: one wave per voxel sums its pairs' g[p] w[o] in fixed point
-// (exact, so independent of the pairs' order)
+// dL/d iq per voxel from the transpose map, balanced over pairs: a wave takes
+// MAP_UNROLL chunks of 64 consecutive pairs of the voxel-sorted map (pair ids and
+// their voxels; every load of the chunks is issued before the first use), computes
+// each pair's fixed-point term g[p] w[o], sums runs of equal voxel with a segmented
+// scan across the lanes, and the last lane of each run adds the run's total to
+// g_fx[voxel] with an integer atomic (a hot voxel costs one atomic per 64 of its
+// pairs).  Integer sums are exact, so the result does not depend on the order.
+// g_fx starts at zero.
+constexpr int MAP_UNROLL = 4;
+
 __global__ __launch_bounds__(BLK) void k_det_adj_map(gx_detector d, const double *__restrict__ g,
-                                                     const double *fx_scale, const int64_t *__restrict__ off,
-                                                     const uint32_t *__restrict__ pairs, unsigned long long *g_fx,
-                                                     int pbits) {
+                                                     const double *fx_scale, const int32_t *__restrict__ pair_voxel,
+                                                     const uint32_t *__restrict__ pairs, int64_t n_pairs,
+                                                     unsigned long long *g_fx, int pbits) {
+  constexpr int SPAN = MAP_UNROLL * GX_WAVE;  // pairs per wave and step
   const int lane = threadIdx.x & (GX_WAVE - 1);
-  const int64_t nv = d.cy * d.cx * d.cz;
   const double sc = fx_scale[0];
   const uint32_t pmask = (pbits >= 32) ? 0xffffffffu : ((1u << pbits) - 1u);
-  for (int64_t v = (blockIdx.x * (int64_t)BLK + threadIdx.x) / GX_WAVE; v < nv;
-       v += (int64_t)gridDim.x * (BLK / GX_WAVE)) {
-    const int64_t a = off[v], b = off[v + 1];
-    long long acc = 0;
-    for (int64_t k = a + lane; k < b; k += GX_WAVE) {
-      const uint32_t id = pairs[k];
-      acc += llrint((g[id & pmask] * d.orient_w[id >> pbits]) * sc);
+  const int64_t wave0 = (blockIdx.x * (int64_t)BLK + threadIdx.x) / GX_WAVE;
+  const int64_t n_waves = (int64_t)gridDim.x * (BLK / GX_WAVE);
+  for (int64_t base = wave0 * SPAN; base < n_pairs; base += n_waves * SPAN) {
+    int v[MAP_UNROLL];
+    uint32_t id[MAP_UNROLL];
+#pragma unroll
+    for (int u = 0; u < MAP_UNROLL; ++u) {
+      const int64_t k = base + u * GX_WAVE + lane;
+      v[u] = k < n_pairs ? pair_voxel[k] : -1;
+      id[u] = k < n_pairs ? pairs[k] : 0u;
+    }
+    double gv[MAP_UNROLL], wv[MAP_UNROLL];
+#pragma unroll
+    for (int u = 0; u < MAP_UNROLL; ++u) {
+      gv[u] = v[u] >= 0 ? g[id[u] & pmask] : 0.0;
+      wv[u] = v[u] >= 0 ? d.orient_w[id[u] >> pbits] : 0.0;
     }
 #pragma unroll
-    for (int s = 32; s > 0; s >>= 1) acc += __shfl_xor(acc, s, GX_WAVE);
-    if (lane == 0) g_fx[v] = (unsigned long long)acc;
+    for (int u = 0; u < MAP_UNROLL; ++u) {
+      long long t = v[u] >= 0 ? llrint((gv[u] * wv[u]) * sc) : 0;
+      // inclusive scan within runs of equal voxel (contiguous: the map is voxel-sorted)
+#pragma unroll
+      for (int o = 1; o < GX_WAVE; o <<= 1) {
+        const long long tu = __shfl_up(t, o, GX_WAVE);
+        const int vu = __shfl_up(v[u], o, GX_WAVE);
+        if (lane >= o && vu == v[u]) t += tu;
+      }
+      const int vn = __shfl_down(v[u], 1, GX_WAVE);
+      if (v[u] >= 0 && (lane == GX_WAVE - 1 || vn != v[u])) atomicAdd(&g_fx[v[u]], (unsigned long long)t);
+    }
   }
 }
 
@@ -540,7 +568,7 @@ static int map_pixel_bits(int64_t npx) {
 }
 
 int gx_detector_map_build(const gx_detector *d, int32_t *counts, int64_t *scan_work, int64_t *voxel_off,
-                          uint32_t *pairs, gx_stream_t stream) {
+                          uint32_t *pairs, int32_t *pair_voxel, gx_stream_t stream) {
   clear_error();
   GX_REQUIRE(d != nullptr, "NULL detector");
   GX_REQUIRE(d->n_pix >= 1 && d->n_pix <= 65536 && d->n_init >= 0 && d->n_init <= 3 && d->n_orient >= 1,
@@ -548,7 +576,8 @@ int gx_detector_map_build(const gx_detector *d, int32_t *counts, int64_t *scan_w
   GX_REQUIRE(d->cy >= 1 && d->cx >= 1 && d->cz >= 1 && d->dq > 0.0, "iq grid");
   GX_REQUIRE(d->h_axis && d->v_axis && (d->n_init == 0 || d->init_R) && d->orient_R && d->orient_w,
              "a detector buffer is NULL");
-  GX_REQUIRE(counts && scan_work && voxel_off && pairs, "NULL map buffer");
+  GX_REQUIRE(counts && scan_work && voxel_off && pairs && pair_voxel, "NULL map buffer");
+  GX_REQUIRE(d->cy * d->cx * d->cz < (1ll << 31), "iq grid too large for 32-bit voxel ids");
   const int64_t npx = d->n_pix * d->n_pix, nv = d->cy * d->cx * d->cz;
   const int pbits = map_pixel_bits(npx);
   GX_REQUIRE(pbits < 32 && d->n_orient <= (1ll << (32 - pbits)),
@@ -561,18 +590,18 @@ int gx_detector_map_build(const gx_detector *d, int32_t *counts, int64_t *scan_w
   k_scan_local<<<(unsigned)nb, BLK, 0, st>>>(counts, nv, voxel_off, scan_work);
   k_scan_blocks<<<1, 64, 0, st>>>(scan_work, nb);
   k_scan_add<<<grid1d(nv, BLK), BLK, 0, st>>>(voxel_off, nv, scan_work, nb, counts);
-  k_det_map_fill<<<grid1d(npx, BLK), BLK, 0, st>>>(*d, voxel_off, counts, pairs, pbits);
+  k_det_map_fill<<<grid1d(npx, BLK), BLK, 0, st>>>(*d, voxel_off, counts, pairs, pair_voxel, pbits);
   return check_launch("gx_detector_map_build");
 }
 
 int gx_detector_adjoint_mapped(const gx_detector *d, const double *g_out, int64_t flags, double *work,
-                               const int64_t *voxel_off, const uint32_t *pairs, unsigned long long *g_fx,
+                               const int32_t *pair_voxel, const uint32_t *pairs, unsigned long long *g_fx,
                                double *fx_scale, gx_stream_t stream) {
   clear_error();
   GX_REQUIRE(d != nullptr, "NULL detector");
   GX_REQUIRE(d->n_pix >= 1 && d->n_pix <= 65536 && d->n_orient >= 1, "detector sizes");
   GX_REQUIRE(d->cy >= 1 && d->cx >= 1 && d->cz >= 1, "empty iq grid");
-  GX_REQUIRE(d->det && d->orient_w && g_out && work && voxel_off && pairs && g_fx && fx_scale,
+  GX_REQUIRE(d->det && d->orient_w && g_out && work && pair_voxel && pairs && g_fx && fx_scale,
              "NULL adjoint buffer");
   hipStream_t st = as_stream(stream);
   const int64_t n = d->n_pix, npx = n * n;
@@ -585,9 +614,11 @@ int gx_detector_adjoint_mapped(const gx_detector *d, const double *g_out, int64_
     g_raw = t;
   k_det_scale<<<1, BLK, 0, st>>>(part, nblk, (flags & GX_FIN_MIRROR) ? 4.0 : 1.0, d->orient_w, d->n_orient,
                                  fx_scale);
-  const int64_t nv = d->cy * d->cx * d->cz;
-  unsigned gv = grid1d(nv * GX_WAVE, BLK);
-  k_det_adj_map<<<gv > 65536 ? 65536 : gv, BLK, 0, st>>>(*d, g_raw, fx_scale, voxel_off, pairs, g_fx,
+  const int64_t nv = d->cy * d->cx * d->cz, n_pairs = npx * d->n_orient;
+  const unsigned gz = grid1d(nv, BLK);
+  k_zero_u64<<<gz > 8192 ? 8192 : gz, BLK, 0, st>>>(g_fx, nv);
+  const unsigned gp = grid1d(n_pairs, BLK * MAP_UNROLL);  // a wave per 64 MAP_UNROLL pairs, grid-strided
+  k_det_adj_map<<<gp > 65536 ? 65536 : gp, BLK, 0, st>>>(*d, g_raw, fx_scale, pair_voxel, pairs, n_pairs, g_fx,
                                                          map_pixel_bits(npx));
   return check_launch("gx_detector_adjoint_mapped");
 }

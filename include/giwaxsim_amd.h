@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define GX_ABI_VERSION 3
+#define GX_ABI_VERSION 4
 
 #define GX_OK 0
 #define GX_ERR_ARG 1         /* invalid argument (message says which) */
@@ -391,15 +391,18 @@ int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags
  * reference counterpart): for every voxel v of d's [cy][cx][cz] grid, the
  * (pixel p, orientation o) pairs whose gather reads v, stored as (o << b) | p with
  * b = ceil(log2(n_pix^2)) in pairs[voxel_off[v] .. voxel_off[v + 1]) (order within a
- * voxel unspecified; n_orient must fit the 32 - b remaining bits).
+ * voxel unspecified; n_orient must fit the 32 - b remaining bits), and beside each
+ * pair its voxel v in pair_voxel (the same positions).
  * Sizes: counts [nv] int32 scratch, scan_work [ceil(nv / 2048) + 1], voxel_off
- * [nv + 1], pairs [n_pix^2 n_orient], nv = cy cx cz.  d->iq and d->det are not used. */
+ * [nv + 1], pairs and pair_voxel [n_pix^2 n_orient], nv = cy cx cz < 2^31.  d->iq and
+ * d->det are not used. */
 int gx_detector_map_build(const gx_detector *d, int32_t *counts, int64_t *scan_work, int64_t *voxel_off,
-                          uint32_t *pairs, gx_stream_t stream);
-/* gx_detector_adjoint through the map: per-voxel sums instead of atomics; writes
- * one copy of g_fx (n_rep = 1 for gx_bins_adjoint). */
+                          uint32_t *pairs, int32_t *pair_voxel, gx_stream_t stream);
+/* gx_detector_adjoint through the map: runs of equal voxel are summed across a wave
+ * before one integer atomic per run; writes one copy of g_fx (n_rep = 1 for
+ * gx_bins_adjoint). */
 int gx_detector_adjoint_mapped(const gx_detector *d, const double *g_out, int64_t flags, double *work,
-                               const int64_t *voxel_off, const uint32_t *pairs, unsigned long long *g_fx,
+                               const int32_t *pair_voxel, const uint32_t *pairs, unsigned long long *g_fx,
                                double *fx_scale, gx_stream_t stream);
 
 #ifdef __cplusplus
