@@ -324,6 +324,21 @@ __device__ __forceinline__ void fft_block(C (&v)[M / GX_BLK_FFT], C* lds, const 
 }
 
 // ------------------------------------------------------------------------------------
+// XCD-aware tile order.  The workgroups of a launch are dispatched round-robin over
+// the 8 XCDs (block b runs on XCD b % 8), each with its own L2.  xcd_tile maps the
+// linear block id b of an n-block launch to a work item so that each XCD receives one
+// contiguous range of work items; with work items numbered row-block major and frame
+// minor, the blocks resident on one XCD at a time are the frames of one or two row
+// blocks, which then share those rows' atoms in that XCD's L2.
+// ------------------------------------------------------------------------------------
+constexpr int GX_XCDS = 8;
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
+  const int64_t q = n / GX_XCDS, r = n % GX_XCDS;
+  const int64_t x = b % GX_XCDS, slot = b / GX_XCDS;
+  return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
+// ------------------------------------------------------------------------------------
 // rectangular_collapse_lengths (tools/voxelgrids.py:238-309) at one x value,
 // from the per-frame record prepared by the host planner (same branch order
 // and the same FP64 operations as the reference's per-x loop).

@@ -637,7 +637,7 @@ __global__ __launch_bounds__(GX_WAVE) void k_frames_spectra(gx_frames d, int64_t
 constexpr int SCATTER_UNROLL = 4;
 
 template <int N, int RPW, typename C>
-__global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
+__global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_t nf) {
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
   constexpr int RB = (BLK / GX_WAVE) * RPW;  // rows per block
@@ -652,14 +652,25 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
   int32_t *acc_re = reinterpret_cast<int32_t *>(wbuf);
   int32_t *acc_im = acc_re + N;
 
-  const int fb = blockIdx.y;
+  // tile order: frame major (the rows of one frame together), or XCD-aware with the
+  // frames of a row block together (row_variant bit 2)
+  const int64_t nzb = gridDim.x / nf;
+  int fb, zb;
+  if (d.row_variant & 4) {
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    fb = (int)(tile % nf);
+    zb = (int)(tile / nf);
+  } else {
+    fb = (int)(blockIdx.x / nzb);
+    zb = (int)(blockIdx.x % nzb);
+  }
   const int64_t f = f0 + fb;
   const double *fp = d.fparams + f * GX_FP_STRIDE;
   const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
   const int wf = jhi - jlo + 1;
   if (wf <= 0) return;  // block-uniform
   if (wf > d.W) {       // planner guarantees wf <= W; never overrun the staging tile
-    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
+    if (threadIdx.x == 0 && zb == 0) atomicAdd(&d.status[1], 1);
     return;
   }
   const bool dense = d.fill_bkg || d.smooth;
@@ -678,7 +689,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
   const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
   const double r = d.r, rinv = 1.0 / d.r;
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
-  const int z0 = blockIdx.x * RB;
+  const int z0 = zb * RB;
 
   // y-mask of this lane's first-stage inputs (same for every row of the frame)
   T my[P];
@@ -766,7 +777,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0) {
 // and outputs as k_rowpass; the autotuner picks between the two shapes.
 // ------------------------------------------------------------------------------------
 template <int N, int RB, typename C>
-__global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0) {
+__global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0, int64_t nf) {
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
   constexpr int R0 = FftFirst<N>::R;
@@ -777,13 +788,24 @@ __global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0) {
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
 
-  const int fb = blockIdx.y;
+  // tile order: frame major (the rows of one frame together), or XCD-aware with the
+  // frames of a row block together (row_variant bit 2)
+  const int64_t nzb = gridDim.x / nf;
+  int fb, zb;
+  if (d.row_variant & 4) {
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    fb = (int)(tile % nf);
+    zb = (int)(tile / nf);
+  } else {
+    fb = (int)(blockIdx.x / nzb);
+    zb = (int)(blockIdx.x % nzb);
+  }
   const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
   const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
   const int wf = jhi - jlo + 1;
   if (wf <= 0) return;  // block-uniform
   if (wf > d.W) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
+    if (threadIdx.x == 0 && zb == 0) atomicAdd(&d.status[1], 1);
     return;
   }
   const bool dense = d.fill_bkg || d.smooth;
@@ -791,7 +813,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0) {
     const C *spec = reinterpret_cast<const C *>(d.col_spec) + (int64_t)fb * d.W;
     for (int w = threadIdx.x; w < wf; w += BLK) tspec[w] = spec[w];
   }
-  const int z0 = blockIdx.x * RB;
+  const int z0 = zb * RB;
   for (int i = threadIdx.x; i < RB * 2 * N; i += BLK) {
     const int row = i / (2 * N), k = i - row * 2 * N;
     reinterpret_cast<int32_t *>(smem_raw + (size_t)row * ROWB)[k] = 0;
@@ -1263,8 +1285,8 @@ int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st)
                                160 * 1024));
     attr_set = true;
   }
-  dim3 grid((unsigned)(d->Z / RB), (unsigned)nf);
-  k_rowpass<N, RPW, C><<<grid, BLK, lds, st>>>(*d, f0);
+  const unsigned nblk = (unsigned)((d->Z / RB) * nf);
+  k_rowpass<N, RPW, C><<<nblk, BLK, lds, st>>>(*d, f0, nf);
   return check_launch("k_rowpass");
 }
 
@@ -1280,16 +1302,16 @@ int launch_rowpass_blk_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t
                                160 * 1024));
     attr_set = true;
   }
-  dim3 grid((unsigned)(d->Z / RB), (unsigned)nf);
-  k_rowpass_blk<N, RB, C><<<grid, BLK, lds, st>>>(*d, f0);
+  const unsigned nblk = (unsigned)((d->Z / RB) * nf);
+  k_rowpass_blk<N, RB, C><<<nblk, BLK, lds, st>>>(*d, f0, nf);
   return check_launch("k_rowpass_blk");
 }
 
-// row_variant: 0 per-wave rows (2 rows per wave for N <= 1024), 1 block-wide deposit
-// over 4 rows, 2 block-wide over 8 rows
+// row_variant bits 0-1: 0 per-wave rows (2 rows per wave for N <= 1024), 1 block-wide
+// deposit over 4 rows, 2 block-wide over 8 rows; bit 2: XCD-aware tile order
 template <int N, typename C>
 int launch_rowpass_v(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
-  switch (d->row_variant) {
+  switch (d->row_variant & 3) {
     case 1: return launch_rowpass_blk_t<N, 4, C>(d, f0, nf, st);
     case 2: return launch_rowpass_blk_t<N, 8, C>(d, f0, nf, st);
     default: return launch_rowpass_t<N, C>(d, f0, nf, st);
@@ -1340,8 +1362,8 @@ int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
   GX_REQUIRE(d->Z >= 8 && d->Z <= d->N && d->Z % 8 == 0, "row range Z=%lld must be a multiple of 8 in [8, N]",
              (long long)d->Z);
   GX_REQUIRE(d->c_n >= 1, "empty cropped grid");
-  GX_REQUIRE(d->row_variant >= 0 && d->row_variant <= 2, "row_variant=%lld outside [0, 2]",
-             (long long)d->row_variant);
+  GX_REQUIRE(d->row_variant >= 0 && d->row_variant <= 6 && (d->row_variant & 3) != 3,
+             "row_variant=%lld is not one of 0, 1, 2, 4, 5, 6", (long long)d->row_variant);
   GX_REQUIRE(d->fparams && d->ymin_key && d->idx_bounds && d->inter && d->bin_sum && d->bin_cnt && d->twiddle &&
                  d->status && d->row_off && d->row_exp,
              "a required device buffer of gx_frames is NULL");

@@ -942,19 +942,24 @@ _TUNED_VARIANT = {}
 _SCRATCH_BINS = {}
 
 
+ROW_VARIANTS = (0, 1, 2, 4, 5, 6)
+
+
 def tuned_variant(runner: FrameRunner):
     """Row-pass shape (gx_frames.row_variant: one wave per row, or a block depositing
-    4 / 8 rows at once) for this runner's grid edge, precision, band width and atoms
-    per row: each shape is timed once on the runner's own first frames (into scratch
-    bins, so the real bins stay untouched) and the fastest is cached.  All shapes give
-    identical bins.  GIWAXSIM_AUTOTUNE=0 keeps shape 0."""
+    4 / 8 rows at once; each in frame-major or XCD-aware tile order) for this runner's
+    grid edge, precision, band width, atoms per row and atom count (whether the atoms
+    fit one XCD's L2): each shape is timed once on the runner's own first frames (into
+    scratch bins, so the real bins stay untouched) and the fastest is cached.  All
+    shapes give identical bins.  GIWAXSIM_AUTOTUNE=0 keeps shape 0."""
     import os
     if os.environ.get("GIWAXSIM_AUTOTUNE", "1") == "0":
         return 0
     p, d = runner.plan, runner.desc
     n_rowed = max(1, runner.slab.n)
     per_row = int(np.log2(max(1.0, n_rowed / max(1, p.Z))))
-    key = (p.N, (p.W + 63) // 64, runner.f64, per_row, torch.cuda.get_device_name())
+    key = (p.N, (p.W + 63) // 64, runner.f64, per_row, int(np.log2(n_rowed)),
+           torch.cuda.get_device_name())
     if key in _TUNED_VARIANT:
         return _TUNED_VARIANT[key]
     g = p.grid
@@ -966,7 +971,7 @@ def tuned_variant(runner: FrameRunner):
     nfr = min(p.n_frames, runner.batch)
     best, best_t = 0, float("inf")
     try:
-        for v in (0, 1, 2):
+        for v in ROW_VARIANTS:
             d.row_variant = v
             try:
                 runner.run((0, nfr))

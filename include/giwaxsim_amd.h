@@ -160,8 +160,11 @@ typedef struct gx_frames {
   unsigned long long *bin_sum; /* fixed-point sum of I (scale bin_scale) */
   int32_t *bin_cnt;
   int32_t *status;         /* [4] counters; [0] += frames without any valid atom */
-  int64_t row_variant;     /* row pass shape (same results): 0 one wave per z row, 1 / 2 a block
-                              deposits 4 / 8 rows at once, then one wave per row transforms */
+  int64_t row_variant;     /* row pass shape (same results): bits 0-1: 0 one wave per z row,
+                              1 / 2 a block deposits 4 / 8 rows at once, then one wave per row
+                              transforms; + 4: XCD-aware tile order (the frames of a row block
+                              run together on one XCD and share its atoms in that L2) instead
+                              of frame-major order */
 } gx_frames;
 
 /* ------------------------------------------------------------------------------------

@@ -139,7 +139,7 @@ def test_row_pass_shapes_identical(name, frames):
     coords, elements, p = _case(name)
     ids, ftab, counts, _ = engine.element_table(elements, 12700.0)
     sums = []
-    for v in (0, 1, 2):
+    for v in engine.ROW_VARIANTS:
         print(f"[gpu-frames] case={name} row_variant={v}", flush=True)
         res = engine.run_voxelgrid(coords, elements, p['r'], p['qv'], p['mq'], 12700.0,
                                    fill_bkg=p['fill'], smooth=p['smooth'], f_table=ftab, elem_ids=ids,
