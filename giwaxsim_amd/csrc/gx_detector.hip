@@ -59,15 +59,44 @@ __device__ __forceinline__ void det_pixel(const gx_detector &d, int64_t p, doubl
   for (int k = 0; k < d.n_init; ++k) rot3(d.init_R + 9 * k, x0, y0, z0);
 }
 
+// A rotation by an angle of 0 has the matrix diag(1, 1, 1) with +-0 elsewhere: applied
+// to finite coordinates it returns them unchanged up to the sign of a zero, which no
+// floor index below depends on, so it can be skipped.  ORIENT_FLAGS orientations get
+// a per-block table of which of their three rotations are such identities.
+__device__ __forceinline__ bool rot_is_identity(const double *R) {
+  return R[0] == 1.0 && R[4] == 1.0 && R[8] == 1.0 && R[1] == 0.0 && R[2] == 0.0 && R[3] == 0.0 &&
+         R[5] == 0.0 && R[6] == 0.0 && R[7] == 0.0;
+}
+
+constexpr int ORIENT_FLAGS = 16384;  // table entries (bytes of LDS) at most
+
+// Fills idf[o] (bit k: rotation k of orientation o is the identity) for the block;
+// every thread of the block must call it.  Returns the table length.
+__device__ __forceinline__ int orient_flags(const gx_detector &d, unsigned char *idf) {
+  const int n = (int)min(d.n_orient, (int64_t)ORIENT_FLAGS);
+  for (int o = threadIdx.x; o < n; o += blockDim.x) {
+    const double *R = d.orient_R + 27 * (int64_t)o;
+    idf[o] = (unsigned char)((rot_is_identity(R) ? 1 : 0) | (rot_is_identity(R + 9) ? 2 : 0) |
+                             (rot_is_identity(R + 18) ? 4 : 0));
+  }
+  __syncthreads();
+  return n;
+}
+
+__host__ __forceinline__ size_t orient_flags_lds(const gx_detector *d) {
+  return (size_t)(d->n_orient < ORIENT_FLAGS ? d->n_orient : ORIENT_FLAGS);
+}
+
 // flat [cy][cx][cz] voxel of the pixel's q point under orientation o: the psi,
-// phi, theta rotations, then the edge-clipped floor indices of intersect_detector
+// phi, theta rotations (those flagged in idf as identities skipped), then the
+// edge-clipped floor indices of intersect_detector
 __device__ __forceinline__ int64_t det_voxel(const gx_detector &d, double x0, double y0, double z0, int64_t o,
-                                             double dqinv) {
+                                             double dqinv, unsigned idf = 0) {
   const double *R = d.orient_R + 27 * o;
   double x = x0, y = y0, z = z0;
-  rot3(R, x, y, z);        // rotate_about_normal(psi)
-  rot3(R + 9, x, y, z);    // rotate_about_vertical(phi)
-  rot3(R + 18, x, y, z);   // rotate_about_horizontal(theta)
+  if (!(idf & 1)) rot3(R, x, y, z);        // rotate_about_normal(psi)
+  if (!(idf & 2)) rot3(R + 9, x, y, z);    // rotate_about_vertical(phi)
+  if (!(idf & 4)) rot3(R + 18, x, y, z);   // rotate_about_horizontal(theta)
   const long long xi = clip_idx(to_int(floor_div(x - d.qx_min, d.dq, dqinv)), d.cx - 1);
   const long long yi = clip_idx(to_int(floor_div(y - d.qy_min, d.dq, dqinv)), d.cy - 1);
   const long long zi = clip_idx(to_int(floor_div(z - d.qz_min, d.dq, dqinv)), d.cz - 1);
@@ -76,6 +105,8 @@ __device__ __forceinline__ int64_t det_voxel(const gx_detector &d, double x0, do
 
 template <typename T>
 __global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
+  extern __shared__ unsigned char idf[];
+  const int nflag = orient_flags(d, idf);
   const T *__restrict__ iq = reinterpret_cast<const T *>(d.iq);
   const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
   if (p >= d.n_pix * d.n_pix) return;
@@ -84,7 +115,7 @@ __global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
   const double dqinv = 1.0 / d.dq;
   double acc = 0.0;
   for (int64_t o = 0; o < d.n_orient; ++o) {
-    const double v = (double)iq[det_voxel(d, x0, y0, z0, o, dqinv)];
+    const double v = (double)iq[det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u)];
     acc = acc + v * d.orient_w[o];
   }
   d.det[p] = acc;
@@ -171,6 +202,8 @@ __global__ __launch_bounds__(BLK) void k_det_scale(const double *__restrict__ pa
 // the result does not depend on the grouping.
 __global__ __launch_bounds__(BLK) void k_det_adj(gx_detector d, const double *__restrict__ g, const double *fx_scale,
                                                  unsigned long long *g_fx, int64_t n_rep) {
+  extern __shared__ unsigned char idf[];
+  const int nflag = orient_flags(d, idf);
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
   const bool live = p < d.n_pix * d.n_pix;
@@ -186,7 +219,7 @@ __global__ __launch_bounds__(BLK) void k_det_adj(gx_detector d, const double *__
     int key = -1 - lane;  // unique: never merges with a neighbour
     long long val = 0;
     if (live) {
-      key = (int)det_voxel(d, x0, y0, z0, o, dqinv);
+      key = (int)det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u);
       val = llrint((gp * d.orient_w[o]) * sc);
     }
     const int up = __shfl_up(key, 1, GX_WAVE);
@@ -211,12 +244,15 @@ __global__ __launch_bounds__(BLK) void k_det_adj(gx_detector d, const double *__
 // (pixel, orientation) pairs that read it, as pair ids p * n_orient + o
 // ------------------------------------------------------------------------------------
 __global__ __launch_bounds__(BLK) void k_det_map_count(gx_detector d, int32_t *cnt) {
+  extern __shared__ unsigned char idf[];
+  const int nflag = orient_flags(d, idf);
   const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
   if (p >= d.n_pix * d.n_pix) return;
   double x0, y0, z0;
   det_pixel(d, p, x0, y0, z0);
   const double dqinv = 1.0 / d.dq;
-  for (int64_t o = 0; o < d.n_orient; ++o) atomicAdd(&cnt[det_voxel(d, x0, y0, z0, o, dqinv)], 1);
+  for (int64_t o = 0; o < d.n_orient; ++o)
+    atomicAdd(&cnt[det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u)], 1);
 }
 
 // exclusive scan of int32 counts into int64 offsets, SCAN_B elements per block
@@ -274,13 +310,15 @@ __global__ __launch_bounds__(BLK) void k_scan_add(int64_t *off, int64_t n, const
 
 __global__ __launch_bounds__(BLK) void k_det_map_fill(gx_detector d, const int64_t *off, int32_t *cur, uint32_t *pairs,
                                                       int32_t *pair_voxel, int pbits) {
+  extern __shared__ unsigned char idf[];
+  const int nflag = orient_flags(d, idf);
   const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
   if (p >= d.n_pix * d.n_pix) return;
   double x0, y0, z0;
   det_pixel(d, p, x0, y0, z0);
   const double dqinv = 1.0 / d.dq;
   for (int64_t o = 0; o < d.n_orient; ++o) {
-    const int64_t v = det_voxel(d, x0, y0, z0, o, dqinv);
+    const int64_t v = det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u);
     const int64_t pos = off[v] + atomicAdd(&cur[v], 1);
     pairs[pos] = ((uint32_t)o << pbits) | (uint32_t)p;
     pair_voxel[pos] = (int32_t)v;
@@ -554,9 +592,9 @@ int gx_detector_accumulate(const gx_detector *d, gx_stream_t stream) {
              "a detector buffer is NULL");
   const int64_t npx = d->n_pix * d->n_pix;
   if (d->iq_f64)
-    k_detector<double><<<grid1d(npx, BLK), BLK, 0, as_stream(stream)>>>(*d);
+    k_detector<double><<<grid1d(npx, BLK), BLK, orient_flags_lds(d), as_stream(stream)>>>(*d);
   else
-    k_detector<float><<<grid1d(npx, BLK), BLK, 0, as_stream(stream)>>>(*d);
+    k_detector<float><<<grid1d(npx, BLK), BLK, orient_flags_lds(d), as_stream(stream)>>>(*d);
   return check_launch("gx_detector_accumulate");
 }
 
@@ -586,11 +624,11 @@ int gx_detector_map_build(const gx_detector *d, int32_t *counts, int64_t *scan_w
   hipStream_t st = as_stream(stream);
   const int64_t nb = (nv + SCAN_B - 1) / SCAN_B;
   k_fill_zero_i32<<<grid1d(nv, BLK) > 8192 ? 8192 : grid1d(nv, BLK), BLK, 0, st>>>(counts, nv);
-  k_det_map_count<<<grid1d(npx, BLK), BLK, 0, st>>>(*d, counts);
+  k_det_map_count<<<grid1d(npx, BLK), BLK, orient_flags_lds(d), st>>>(*d, counts);
   k_scan_local<<<(unsigned)nb, BLK, 0, st>>>(counts, nv, voxel_off, scan_work);
   k_scan_blocks<<<1, 64, 0, st>>>(scan_work, nb);
   k_scan_add<<<grid1d(nv, BLK), BLK, 0, st>>>(voxel_off, nv, scan_work, nb, counts);
-  k_det_map_fill<<<grid1d(npx, BLK), BLK, 0, st>>>(*d, voxel_off, counts, pairs, pair_voxel, pbits);
+  k_det_map_fill<<<grid1d(npx, BLK), BLK, orient_flags_lds(d), st>>>(*d, voxel_off, counts, pairs, pair_voxel, pbits);
   return check_launch("gx_detector_map_build");
 }
 
@@ -652,7 +690,7 @@ int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags
   k_zero_u64<<<gz > 4096 ? 4096 : gz, BLK, 0, st>>>(g_fx, nv * n_rep);
   k_det_scale<<<1, BLK, 0, st>>>(part, nblk, (flags & GX_FIN_MIRROR) ? 4.0 : 1.0, d->orient_w, d->n_orient,
                                  fx_scale);
-  k_det_adj<<<nblk, BLK, 0, st>>>(*d, g_raw, fx_scale, g_fx, n_rep);
+  k_det_adj<<<nblk, BLK, orient_flags_lds(d), st>>>(*d, g_raw, fx_scale, g_fx, n_rep);
   return check_launch("gx_detector_adjoint");
 }
 
