@@ -371,8 +371,9 @@ __device__ __forceinline__ void block_minmax4(int (&v)[4], int *sh) {
 // group and rotated for every frame of it; one atomic per block, frame and key.
 // min/max of doubles are exact, so the result does not depend on the order.
 constexpr int YMIN_FRAMES = 8;
-constexpr int YMIN_PER_THREAD = 8;
-constexpr int YMIN_CHUNK = YMIN_PER_THREAD * BLK;
+constexpr int YMIN_PER_THREAD = 8;                   // atom loads in flight per thread
+constexpr int YMIN_ROUNDS = 4;                       // rounds per block (amortise the reduction)
+constexpr int YMIN_CHUNK = YMIN_ROUNDS * YMIN_PER_THREAD * BLK;
 
 __global__ __launch_bounds__(BLK) void k_frames_ymin(gx_frames d, int64_t f0, int64_t nf) {
   constexpr int NW = BLK / GX_WAVE, K = 3 * YMIN_FRAMES;
@@ -386,38 +387,68 @@ __global__ __launch_bounds__(BLK) void k_frames_ymin(gx_frames d, int64_t f0, in
     sn[j] = fp[GX_FP_SIN];
     cs[j] = fp[GX_FP_COS];
   }
+  // atoms with a z row inside the grid come first (regrouped order): a round of a
+  // thread is all rowed, all not, or (once) straddles; rowed atoms feed mz / Mz,
+  // the others mo, and min y' over all atoms is min(mz, mo)
   const int64_t n_rowed = d.row_off[d.N];
-  const int64_t a0 = (int64_t)blockIdx.x * YMIN_CHUNK + threadIdx.x;
-  // every load of the thread's atoms is issued before the first use
-  double x[YMIN_PER_THREAD], y[YMIN_PER_THREAD];
-#pragma unroll
-  for (int k = 0; k < YMIN_PER_THREAD; ++k) {
-    const int64_t a = a0 + k * BLK;
-    x[k] = a < d.n_atoms ? d.ax[a] : 0.0;
-    y[k] = a < d.n_atoms ? d.ay[a] : 0.0;
-  }
-  double m[YMIN_FRAMES], mz[YMIN_FRAMES], Mz[YMIN_FRAMES];
+  double mo[YMIN_FRAMES], mz[YMIN_FRAMES], Mz[YMIN_FRAMES];
 #pragma unroll
   for (int j = 0; j < YMIN_FRAMES; ++j) {
-    m[j] = INFINITY;
+    mo[j] = INFINITY;
     mz[j] = INFINITY;
     Mz[j] = -INFINITY;
   }
+  for (int rd = 0; rd < YMIN_ROUNDS; ++rd) {
+    const int64_t a0 = (int64_t)blockIdx.x * YMIN_CHUNK + (int64_t)rd * YMIN_PER_THREAD * BLK + threadIdx.x;
+    if (a0 >= d.n_atoms) break;
+    // every load of the round is issued before the first use
+    double x[YMIN_PER_THREAD], y[YMIN_PER_THREAD];
 #pragma unroll
-  for (int k = 0; k < YMIN_PER_THREAD; ++k) {
-    const int64_t a = a0 + k * BLK;
-    if (a >= d.n_atoms) break;
-    const bool rowed = a < n_rowed;
+    for (int k = 0; k < YMIN_PER_THREAD; ++k) {
+      const int64_t a = a0 + k * BLK;
+      x[k] = a < d.n_atoms ? d.ax[a] : 0.0;
+      y[k] = a < d.n_atoms ? d.ay[a] : 0.0;
+    }
+    const int64_t a_last = min(a0 + (YMIN_PER_THREAD - 1) * BLK, (int64_t)d.n_atoms - 1);
+    if (a_last < n_rowed) {  // all rowed
 #pragma unroll
-    for (int j = 0; j < YMIN_FRAMES; ++j) {
-      const double v = x[k] * sn[j] + y[k] * cs[j];
-      m[j] = fmin(m[j], v);
-      if (rowed) {
-        mz[j] = fmin(mz[j], v);
-        Mz[j] = fmax(Mz[j], v);
+      for (int k = 0; k < YMIN_PER_THREAD; ++k) {
+        if (a0 + k * BLK >= d.n_atoms) break;
+#pragma unroll
+        for (int j = 0; j < YMIN_FRAMES; ++j) {
+          const double v = x[k] * sn[j] + y[k] * cs[j];
+          mz[j] = fmin(mz[j], v);
+          Mz[j] = fmax(Mz[j], v);
+        }
+      }
+    } else if (a0 >= n_rowed) {  // none rowed
+#pragma unroll
+      for (int k = 0; k < YMIN_PER_THREAD; ++k) {
+        if (a0 + k * BLK >= d.n_atoms) break;
+#pragma unroll
+        for (int j = 0; j < YMIN_FRAMES; ++j) mo[j] = fmin(mo[j], x[k] * sn[j] + y[k] * cs[j]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < YMIN_PER_THREAD; ++k) {
+        const int64_t a = a0 + k * BLK;
+        if (a >= d.n_atoms) break;
+#pragma unroll
+        for (int j = 0; j < YMIN_FRAMES; ++j) {
+          const double v = x[k] * sn[j] + y[k] * cs[j];
+          if (a < n_rowed) {
+            mz[j] = fmin(mz[j], v);
+            Mz[j] = fmax(Mz[j], v);
+          } else {
+            mo[j] = fmin(mo[j], v);
+          }
+        }
       }
     }
   }
+  double m[YMIN_FRAMES];
+#pragma unroll
+  for (int j = 0; j < YMIN_FRAMES; ++j) m[j] = fmin(mz[j], mo[j]);
   const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
 #pragma unroll
   for (int j = 0; j < YMIN_FRAMES; ++j) {
