@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble (64 slabs) and large-slab (N=2048) figures")
     ap.add_argument("--only", choices=["adjoint", "adjoint_fp64", "adjoint_polymer", "adjoint_large",
-                                       "ensemble", "large_slab"],
+                                       "ensemble", "large_slab", "compare"],
                     help="run one of the extra workloads alone and print its JSON (for profiling)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="one step, then one 256 MiB device copy (counter calibration) and one "
@@ -369,11 +369,65 @@ def extra_configs(args, engine, torch, which=None):
         out["adjoint_polymer"] = adjoint_polymer_config(engine, torch)
         out["adjoint_large"] = adjoint_large_config(engine, torch, n_slabs=16)
     reps = 3
+    if which in (None, "compare"):
+        out["compare"] = compare_config(engine, torch)
     if which in (None, "ensemble"):
         out["ensemble"] = ensemble_config(engine, torch, reps)
     if which in (None, "large_slab"):
         out["large_slab"] = large_slab_config(engine, torch, reps)
     return out
+
+
+def compare_config(engine, torch, reps=20, cpu_reps=3):
+    """Experimental side of a fit evaluation (fit_slabsize.py fullfit_model after the
+    detector image): trim to the experimental range, shift_peak, scale / offset,
+    masked residual, on the device (FitComparison.residual) beside the oracle's numpy
+    / scipy version on the host, for the production workload's 512^2 mirrored
+    detector image and a synthetic 400 x 360 experimental map rebinned onto it."""
+    import oracle.comparison as OC
+    from giwaxsim_amd.tools.comparison import FitComparison, rebin_qmap_and_mask
+    W = workload()
+    ids, ftab, counts, _ = engine.element_table(W['elements'], W['energy'])
+    res = engine.run_voxelgrid(torch.as_tensor(W['coords']).cuda(), None, W['r'], W['qv'], W['max_q'],
+                               W['energy'], fill_bkg=W['fill'], smooth=W['smooth'], f_table=ftab,
+                               elem_ids=torch.as_tensor(ids).cuda(), want64=True)
+    from giwaxsim_amd.tools.comparison import _weights
+    psis = np.linspace(0, 180, 361)
+    one = np.array([0.0])
+    dplan = engine.detector_plan(512, W['max_q'], (0.15, 0, 0), ('psi', 'None', 'None'), psis,
+                                 _weights(None, psis), one, _weights(None, one), one, _weights(None, one))
+    det, _ = engine.run_detector(res['iq64'], res['qx'], res['qy'], res['qz'], dplan, True)
+    exp_qxy = np.linspace(0.0, 1.95, 400)
+    exp_qz = np.linspace(0.0, 1.9, 360)
+    X, Z = np.meshgrid(exp_qxy, exp_qz)
+    exp_img = 1e-3 * (2.0 + np.cos(9 * np.hypot(X, Z)))
+    mask = np.zeros(X.shape)
+    mask[:, :12] = 1
+    _, h_t, v_t = OC.trim_sim_data(np.zeros((len(dplan.v), len(dplan.h))), dplan.h, dplan.v, exp_qxy, exp_qz)
+    rebin_map, rebin_mask = rebin_qmap_and_mask(h_t, v_t, exp_img, exp_qxy, exp_qz, mask)
+    rebin_map = np.where(np.isnan(rebin_map), 0.0, rebin_map)
+    rebin_map[rebin_mask == 1] = 0
+    pad = (0.05, (1.2, 1.8))
+    fc = FitComparison(dplan.h, dplan.v, rebin_map, rebin_mask, exp_qxy, exp_qz, *pad)
+    fc.residual(det)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        r, s, b = fc.residual(det)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    det_np = det.cpu().numpy()
+    tc = time.perf_counter()
+    for _ in range(cpu_reps):
+        ref, _ = OC.fit_residual(det_np, dplan.h, dplan.v, rebin_map, rebin_mask, exp_qxy, exp_qz, *pad)
+    tcpu = (time.perf_counter() - tc) / cpu_reps
+    err = float(np.max(np.abs(r.cpu().numpy() - ref)) / np.max(np.abs(ref)))
+    return {"workload": "fit evaluation, experimental side: 512^2 mirrored detector image of the polymer "
+                        "slab trimmed to qxy 0..1.95, qz 0..1.9 ({} x {}), shift_peak pad 0.05 in "
+                        "1.2..1.8, scale/offset, masked residual".format(len(v_t), len(h_t)),
+            "ms_per_eval": dt * 1e3, "cpu_ms_per_eval": tcpu * 1e3,
+            "cpu_kind": "oracle (numpy + scipy.ndimage.map_coordinates, 1 thread)",
+            "max_rel_diff_vs_cpu": err}
 
 
 def ensemble_config(engine, torch, reps):

@@ -113,6 +113,10 @@ SIGNATURES = {
     "gx_bin_frame_f64": [_P, _I, _I, _P, _P, _P, POINTER(c_double), _D, _I, _P, _P, _P, _P],
     "gx_collapse_lengths": [POINTER(c_double), _P, _I, _P, _P],
     "gx_scale_offset_moments": [_P, _P, _P, _I, _P, POINTER(c_double), _P],
+    "gx_map_linear": [_P, _I, _I, _P, _P, _I, _D, _P, _P],
+    "gx_polar_pad": [_P, _I, _I, _P, _I, _P, _P],
+    "gx_csr_apply": [_P, _P, _P, _I, _P, _D, _P, _P],
+    "gx_fit_residual": [_P, _P, _P, _I, _D, _D, _P, _P],
     "gx_apply_f0": [_P, _I, _I, _I, _P, _P, _P, POINTER(c_double), _D, _P, _P],
     "gx_bins_accumulate_f64": [_P, _P, _I, _D, _P, _P, _P],
     "gx_adj_prepare": [POINTER(GxFrames), POINTER(GxAdjoint), _P],

@@ -1,6 +1,9 @@
 """Mirror of the reference's pipeline drivers (``tools/comparison.py``):
 ``voxelgridmaker_fitting``, ``detectormaker_fitting``,
-``optimize_scale_offset`` and the slab builder ``slabmaker_fitting``.
+``optimize_scale_offset``, the slab builder ``slabmaker_fitting``, and the
+experimental-comparison step of a fit evaluation (``trim_sim_data``,
+``rebin_qmap_and_mask``, ``linear_polar`` / ``polar_linear`` / ``add_pad`` /
+``shift_peak``, and ``FitComparison`` for fit_slabsize.py's residual).
 
 The two drivers run every frame / orientation on the GPU with all
 intermediate data device resident, and return numpy arrays in the
@@ -18,7 +21,9 @@ from .._lib import GX_MOMENTS_WORK, call
 from .utilities import calc_real_space_abc, load_pdb, load_xyz
 
 __all__ = ['slabmaker_fitting', 'voxelgridmaker_fitting', 'detectormaker_fitting',
-           'optimize_scale_offset', 'DeviceIQ']
+           'optimize_scale_offset', 'DeviceIQ', 'trim_sim_data', 'rebin_qmap_and_mask',
+           'linear_polar', 'polar_linear', 'add_pad', 'shift_peak', 'pad_columns', 'ShiftPeakPlan',
+           'RebinWeights', 'FitComparison']
 
 
 class DeviceIQ(np.ndarray):
@@ -141,3 +146,334 @@ def optimize_scale_offset(sim_map, rebin_map, rebin_mask):
     scale = ssy / sss if sss != 0 else 0.0
     offset = my - scale * ms
     return scale, offset
+
+
+# --------------------------------------------------------------------------------------
+# experimental comparison of a fit evaluation (tools/comparison.py:161-191, 406-445,
+# 469-592).  Sample positions, pad column maps and rebin weights depend on geometry
+# only: they are planned on the host with the reference's own formulas and cached;
+# the device applies them to every new image (gx_map_linear, gx_polar_pad,
+# gx_csr_apply, gx_fit_residual).
+# --------------------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def _is_dev(a):
+    torch = _torch()
+    return torch.is_tensor(a)
+
+
+def _as_dev(a):
+    torch = _torch()
+    if torch.is_tensor(a):
+        return a.to(device=engine.device(), dtype=torch.float64).contiguous()
+    return engine.to_dev(np.asarray(a, dtype=np.float64), torch.float64)
+
+
+def _map_linear_dev(img_dev, ys_dev, xs_dev, cval=0.0):
+    """scipy.ndimage.map_coordinates(img, (ys, xs), order=1, mode='constant', cval)."""
+    torch = _torch()
+    h, w = img_dev.shape
+    out = torch.empty(ys_dev.shape, dtype=torch.float64, device=img_dev.device)
+    call("gx_map_linear", engine.ptr(img_dev), h, w, engine.ptr(ys_dev), engine.ptr(xs_dev),
+         int(ys_dev.numel()), float(cval), engine.ptr(out), engine.stream_handle())
+    return out
+
+
+def _check_order(order):
+    if order != 1:
+        raise NotImplementedError("only order=1 (linear) interpolation, the reference's setting")
+
+
+def _linear_polar_coords(shape, o, r, out_shape):
+    """Sample positions of linear_polar (tools/comparison.py:475-494)."""
+    if o is None:
+        o = np.array(shape[:2]) / 2 - 0.5
+    else:
+        o = np.array(o)
+    if r is None:
+        r = np.sqrt((np.array(shape[:2]) ** 2).sum()) / 2
+    if out_shape is None:
+        out_shape = (int(round(r * 2 * np.pi)), int(round(r)))
+    out_h, out_w = out_shape
+    rs = np.linspace(0, r, out_w)
+    ts = np.linspace(0, 2 * np.pi, out_h)
+    xs = rs * np.cos(ts[:, None]) + o[1]
+    ys = rs * np.sin(ts[:, None]) + o[0]
+    return ys, xs
+
+
+def _polar_linear_coords(polar_shape, o, r, out_shape):
+    """Sample positions (theta row, radius column) of polar_linear
+    (tools/comparison.py:505-530)."""
+    if r is None:
+        r = polar_shape[1]
+    if out_shape is None:
+        out_shape = (int(r * 2), int(r * 2))
+    if o is None:
+        o = np.array(out_shape) / 2 - 0.5
+    else:
+        o = np.array(o)
+    out_h, out_w = out_shape
+    ys, xs = np.mgrid[:out_h, :out_w] - o[:, None, None]
+    rs = np.sqrt(ys ** 2 + xs ** 2)
+    ts = np.arctan2(ys, xs)
+    ts[ts < 0] += 2 * np.pi
+    rs = (rs / r) * (polar_shape[1] - 1)
+    ts = (ts / (2 * np.pi)) * (polar_shape[0] - 1)
+    return ts, rs
+
+
+def _out_shape(output):
+    if output is None:
+        return None
+    return tuple(output) if isinstance(output, tuple) else tuple(np.shape(output))
+
+
+def _deliver(out_dev, like, output):
+    """Device result -> the caller's kind: a device tensor for device input, else
+    numpy (written into ``output`` when that is an array, as map_coordinates does)."""
+    if _is_dev(like):
+        return out_dev
+    res = out_dev.cpu().numpy()
+    if isinstance(output, np.ndarray):
+        output[...] = res
+        return output
+    return res
+
+
+def linear_polar(img, o=None, r=None, output=None, order=1, cont=0):
+    """Cartesian image -> (theta, r) polar image (tools/comparison.py:469-497),
+    bilinear samples on the GPU."""
+    _check_order(order)
+    ys, xs = _linear_polar_coords(np.shape(img), o, r, _out_shape(output))
+    out = _map_linear_dev(_as_dev(img), _as_dev(ys), _as_dev(xs), cont)
+    return _deliver(out, img, output)
+
+
+def polar_linear(img, o=None, r=None, output=None, order=1, cont=0):
+    """(theta, r) polar image -> Cartesian image (tools/comparison.py:499-532),
+    bilinear samples on the GPU."""
+    _check_order(order)
+    ts, rs = _polar_linear_coords(np.shape(img), o, r, _out_shape(output))
+    out = _map_linear_dev(_as_dev(img), _as_dev(ts), _as_dev(rs), cont)
+    return _deliver(out, img, output)
+
+
+def pad_columns(width, r_axis, pad_width, pad_range):
+    """Column map of add_pad (tools/comparison.py:534-557): padded[:, c] =
+    polar[:, src[c]], or None when nothing is padded.  Same arithmetic and the
+    same errors as the reference's repeated np.insert."""
+    pad_range_min, pad_range_max = pad_range
+    widths = np.diff(r_axis)
+    widths_trim = widths[widths > 0]
+    pixel_width = np.mean(widths_trim)
+    pad_width_pixels = int(np.round(pad_width / pixel_width))
+    if pad_width_pixels == 0:
+        return None
+    pad_start_idx = int(np.argmin(np.abs(r_axis - pad_range_min)))
+    pad_end_idx = int(np.argmin(np.abs(r_axis - pad_range_max)))
+    pad_range_pixels = pad_end_idx - pad_start_idx
+    spacing = int((pad_range_pixels - 1) / pad_width_pixels - 1)
+    assert spacing >= 0, 'pad_range is too small for desired pad_width'
+    cols = list(range(int(width)))
+    for i in range(pad_width_pixels):
+        pad_idx = pad_start_idx + 2 * i + spacing * i
+        if pad_idx >= len(cols):
+            raise IndexError(f"index {pad_idx} is out of bounds for axis 1 with size {len(cols)}")
+        cols.insert(pad_idx + 1, cols[pad_idx])
+    return np.asarray(cols[:len(cols) - pad_width_pixels], dtype=np.int32)
+
+
+def _polar_pad_dev(polar_dev, src, mask_zero):
+    torch = _torch()
+    h, w = polar_dev.shape
+    if src is None:
+        src = np.arange(w, dtype=np.int32)
+    src_dev = torch.as_tensor(src, dtype=torch.int32).to(polar_dev.device)
+    out = torch.empty_like(polar_dev)
+    call("gx_polar_pad", engine.ptr(polar_dev), h, w, engine.ptr(src_dev), int(bool(mask_zero)),
+         engine.ptr(out), engine.stream_handle())
+    return out
+
+
+def add_pad(polar_image, r_axis, pad_width, pad_range):
+    """Duplicate columns of a polar image to stretch a radial band
+    (tools/comparison.py:534-557); the column copy runs on the GPU."""
+    src = pad_columns(np.shape(polar_image)[1], np.asarray(r_axis), pad_width, pad_range)
+    if src is None:
+        return polar_image
+    return _deliver(_polar_pad_dev(_as_dev(polar_image), src, False), polar_image, None)
+
+
+class ShiftPeakPlan:
+    """shift_peak (tools/comparison.py:559-592) for one trimmed detector geometry and
+    pad setting: the polar and inverse sample positions and the pad column map,
+    planned once; ``__call__`` applies them to a device image [v, h]."""
+
+    def __init__(self, shape, det_h, det_v, pad_width_qspace, pad_range_qspace):
+        det_h = np.asarray(det_h, dtype=np.float64)
+        det_v = np.asarray(det_v, dtype=np.float64)
+        row, col = shape
+        self.shape = (int(row), int(col))
+        radius = float(np.sqrt(row ** 2 + col ** 2))
+        o = (int(np.argmin(np.abs(det_v))), int(np.argmin(np.abs(det_h))))
+        ys, xs = _linear_polar_coords(self.shape, o, radius, None)
+        self.polar_shape = ys.shape
+        self.ys, self.xs = _as_dev(ys), _as_dev(xs)
+        # r_axis = row 0 of linear_polar(hypot(det_h, det_v)), sampled like the image
+        xx, yy = np.meshgrid(det_h, det_v)
+        r_dev = _as_dev(np.hypot(xx, yy))
+        r_axis = _map_linear_dev(r_dev, self.ys[0].contiguous(), self.xs[0].contiguous(), 0.0).cpu().numpy()
+        self.src = pad_columns(self.polar_shape[1], r_axis, pad_width_qspace, pad_range_qspace)
+        ts, rs = _polar_linear_coords(self.polar_shape, o, None, self.shape)
+        self.ts, self.rs = _as_dev(ts), _as_dev(rs)
+
+    def __call__(self, image_dev):
+        polar = _map_linear_dev(image_dev, self.ys, self.xs, 0.0)
+        padded = _polar_pad_dev(polar, self.src, True)
+        return _map_linear_dev(padded, self.ts, self.rs, 0.0)
+
+
+_SHIFT_PLANS = {}
+
+
+def _shift_plan(shape, det_h, det_v, pad_width, pad_range):
+    torch = _torch()
+    key = (tuple(shape), np.asarray(det_h, np.float64).tobytes(), np.asarray(det_v, np.float64).tobytes(),
+           float(pad_width), tuple(float(p) for p in pad_range), torch.cuda.current_device())
+    plan = _SHIFT_PLANS.get(key)
+    if plan is None:
+        if len(_SHIFT_PLANS) >= 8:
+            _SHIFT_PLANS.pop(next(iter(_SHIFT_PLANS)))
+        plan = _SHIFT_PLANS[key] = ShiftPeakPlan(shape, det_h, det_v, pad_width, pad_range)
+    return plan
+
+
+def shift_peak(image, det_h, det_v, pad_width_qspace, pad_range_qspace):
+    """Stretch the pi-pi band of a detector image radially by pad_width_qspace inside
+    pad_range_qspace (tools/comparison.py:559-592), on the GPU."""
+    plan = _shift_plan(np.shape(image), det_h, det_v, pad_width_qspace, pad_range_qspace)
+    return _deliver(plan(_as_dev(image)), image, None)
+
+
+def trim_sim_data(sim_det_ints, det_h, det_v, exp_qxy, exp_qz):
+    """Trim a detector image and its axes to the experimental q range
+    (tools/comparison.py:161-191).  A device image stays on the device."""
+    qxy_min, qxy_max = np.min(exp_qxy), np.max(exp_qxy)
+    qz_min, qz_max = np.min(exp_qz), np.max(exp_qz)
+    det_h = np.asarray(det_h)
+    det_v = np.asarray(det_v)
+    h_mask = (det_h >= qxy_min) & (det_h <= qxy_max)
+    v_mask = (det_v >= qz_min) & (det_v <= qz_max)
+    if _is_dev(sim_det_ints):
+        torch = _torch()
+        iv = torch.as_tensor(np.nonzero(v_mask)[0]).to(sim_det_ints.device)
+        ih = torch.as_tensor(np.nonzero(h_mask)[0]).to(sim_det_ints.device)
+        trimmed = sim_det_ints.index_select(0, iv).index_select(1, ih).contiguous()
+    else:
+        trimmed = sim_det_ints[v_mask, :][:, h_mask]
+    return trimmed, det_h[h_mask], det_v[v_mask]
+
+
+class RebinWeights:
+    """griddata(points2, values, (grid1_x, grid1_z), method='linear') as sparse
+    weights: the barycentric coordinates of every target point in its triangle of
+    the Delaunay triangulation of the source grid (the triangulation and point
+    location griddata itself uses, scipy.spatial.Delaunay / find_simplex), three
+    entries per target row, none outside the hull (NaN there, griddata's fill)."""
+
+    def __init__(self, qxy1, qz1, qxy2, qz2):
+        from scipy.spatial import Delaunay
+        torch = _torch()
+        grid1_x, grid1_z = np.meshgrid(qxy1, qz1)
+        grid2_x, grid2_z = np.meshgrid(qxy2, qz2)
+        points2 = np.vstack((grid2_x.ravel(), grid2_z.ravel())).T
+        xi = np.vstack((grid1_x.ravel(), grid1_z.ravel())).T
+        tri = Delaunay(points2)
+        simplex = tri.find_simplex(xi)
+        inside = simplex >= 0
+        T = tri.transform[simplex[inside]]                      # [m, 3, 2]
+        d = xi[inside] - T[:, 2, :]
+        c = np.einsum('mij,mj->mi', T[:, :2, :], d)
+        bary = np.concatenate([c, 1.0 - c.sum(axis=1, keepdims=True)], axis=1)
+        verts = tri.simplices[simplex[inside]]
+        n_rows = xi.shape[0]
+        counts = np.where(inside, 3, 0)
+        row_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        self.shape = grid1_x.shape
+        self.n_rows = n_rows
+        dev = engine.device()
+        self.row_ptr = torch.as_tensor(row_ptr).to(dev)
+        self.col = torch.as_tensor(verts.astype(np.int32).ravel()).to(dev)
+        self.val = torch.as_tensor(bary.ravel()).to(dev)
+
+    def apply(self, values_dev, empty=np.nan):
+        torch = _torch()
+        y = torch.empty(self.n_rows, dtype=torch.float64, device=values_dev.device)
+        call("gx_csr_apply", engine.ptr(self.row_ptr), engine.ptr(self.col), engine.ptr(self.val), self.n_rows,
+             engine.ptr(values_dev), float(empty), engine.ptr(y), engine.stream_handle())
+        return y.view(self.shape)
+
+
+def rebin_qmap_and_mask(qxy1, qz1, qmap2, qxy2, qz2, qmap2_mask):
+    """Rebin an experimental map and its mask onto the simulated (qxy1, qz1) grid
+    (tools/comparison.py:406-445): griddata 'linear' as precomputed barycentric
+    weights applied on the GPU; the mask is 1 wherever its interpolant is > 0."""
+    W = RebinWeights(qxy1, qz1, qxy2, qz2)
+    rebinned = W.apply(_as_dev(np.asarray(qmap2, dtype=np.float64).ravel()))
+    mask_r = W.apply(_as_dev(np.asarray(qmap2_mask, dtype=np.float64).ravel()))
+    mask = (mask_r > 0).to(dtype=_torch().int64)
+    return rebinned.cpu().numpy(), mask.cpu().numpy()
+
+
+def _scale_offset_dev(sim_dev, ref_dev, mask_u8_dev):
+    """optimize_scale_offset (tools/comparison.py:873-882) on device arrays."""
+    torch = _torch()
+    work = torch.empty(GX_MOMENTS_WORK, dtype=torch.float64, device=sim_dev.device)
+    res = (ctypes.c_double * 6)()
+    call("gx_scale_offset_moments", engine.ptr(sim_dev), engine.ptr(ref_dev), engine.ptr(mask_u8_dev),
+         int(sim_dev.numel()), engine.ptr(work), res, engine.stream_handle())
+    n, ms, my, sss, ssy = res[0], res[1], res[2], res[3], res[4]
+    if n == 0:
+        raise np.linalg.LinAlgError("no unmasked pixels")
+    scale = ssy / sss if sss != 0 else 0.0
+    return scale, my - scale * ms
+
+
+class FitComparison:
+    """The experimental side of fit_slabsize.py's fullfit_model (19-47) for a fixed
+    experimental map: trim the simulated detector image to the experimental range,
+    shift_peak, least-squares scale / offset over unmasked pixels, masked residual
+    -- all on the device, from the device detector image."""
+
+    def __init__(self, det_h, det_v, rebin_map, rebin_mask, exp_qxy, exp_qz, pad_width, pad_range):
+        torch = _torch()
+        qxy_min, qxy_max = np.min(exp_qxy), np.max(exp_qxy)
+        qz_min, qz_max = np.min(exp_qz), np.max(exp_qz)
+        det_h, det_v = np.asarray(det_h), np.asarray(det_v)
+        h_mask = (det_h >= qxy_min) & (det_h <= qxy_max)
+        v_mask = (det_v >= qz_min) & (det_v <= qz_max)
+        dev = engine.device()
+        self.iv = torch.as_tensor(np.nonzero(v_mask)[0]).to(dev)
+        self.ih = torch.as_tensor(np.nonzero(h_mask)[0]).to(dev)
+        self.h_trim, self.v_trim = det_h[h_mask], det_v[v_mask]
+        shape = (len(self.v_trim), len(self.h_trim))
+        self.shift = _shift_plan(shape, self.h_trim, self.v_trim, pad_width, pad_range)
+        self.ref = _as_dev(np.asarray(rebin_map, dtype=np.float64))
+        self.mask = torch.as_tensor((np.asarray(rebin_mask) != 0).astype(np.uint8)).to(dev)
+        assert tuple(self.ref.shape) == shape, "rebin_map must match the trimmed detector grid"
+        self.n = int(np.prod(shape))
+
+    def residual(self, det_dev):
+        """(residual device tensor [n], scale, offset) for a device image [v, h]."""
+        torch = _torch()
+        sim = det_dev.index_select(0, self.iv).index_select(1, self.ih).contiguous()
+        sim = self.shift(sim)
+        scale, offset = _scale_offset_dev(sim, self.ref, self.mask)
+        out = torch.empty(self.n, dtype=torch.float64, device=sim.device)
+        call("gx_fit_residual", engine.ptr(sim), engine.ptr(self.ref), engine.ptr(self.mask), self.n,
+             float(scale), float(offset), engine.ptr(out), engine.stream_handle())
+        return out, scale, offset

@@ -294,6 +294,34 @@ int gx_scale_offset_moments(const double *sim, const double *exp_, const uint8_t
 #define GX_MOMENTS_WORK 774
 
 /* ------------------------------------------------------------------------------------
+ * Experimental comparison of a fit evaluation (tools/comparison.py, fit_slabsize.py).
+ * Sample positions, pad maps and rebin weights depend on geometry only and are
+ * planned on the host; these entry points apply them to device images.
+ * ---------------------------------------------------------------------------------- */
+/* scipy.ndimage.map_coordinates(img, (ys, xs), order=1, mode='constant', cval) on a
+ * row-major [h][w] FP64 image: bilinear inside [0, h-1] x [0, w-1], cval outside.
+ * Replaces the map_coordinates calls of linear_polar / polar_linear
+ * (tools/comparison.py:496, 531). */
+int gx_map_linear(const double *img, int64_t h, int64_t w, const double *ys, const double *xs, int64_t n, double cval,
+                  double *out, gx_stream_t stream);
+/* add_pad (tools/comparison.py:534-557) as a column map, out[r][c] =
+ * polar[r][src_col[c]] (0 <= src_col[c] < w) on [h][w] FP64 images, and with
+ * mask_zero = 1 shift_peak's `polar_image_pad[polar_image == 0] = 0` (577): out is 0
+ * wherever polar is.  out must not alias polar. */
+int gx_polar_pad(const double *polar, int64_t h, int64_t w, const int32_t *src_col, int64_t mask_zero, double *out,
+                 gx_stream_t stream);
+/* y[r] = sum_k val[k] x[col[k]] over row r of a CSR matrix (row_ptr [n_rows + 1]);
+ * rows without entries get `empty`.  With the barycentric weights of
+ * scipy.interpolate.griddata(method='linear') it replaces the two griddata calls of
+ * rebin_qmap_and_mask (tools/comparison.py:431, 436). */
+int gx_csr_apply(const int64_t *row_ptr, const int32_t *col, const double *val, int64_t n_rows, const double *x,
+                 double empty, double *y, gx_stream_t stream);
+/* Residual of fullfit_model (fit_slabsize.py:42-45):
+ * out = (ref - (scale sim + offset)) * (mask == 0). */
+int gx_fit_residual(const double *sim, const double *ref, const uint8_t *mask, int64_t n, double scale,
+                    double offset, double *out, gx_stream_t stream);
+
+/* ------------------------------------------------------------------------------------
  * Differentiable variant of the phi-frame path and its hand-written adjoint.
  *
  * The reference has no gradients; these entry points have no counterpart in it.

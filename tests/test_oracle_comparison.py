@@ -1,0 +1,101 @@
+"""CPU pins of the experimental-comparison oracle (oracle/comparison.py) and of the
+host planning of the device version (pad column map), by closed forms:
+
+* bilinear interpolation reproduces linear functions exactly, so linear_polar of
+  img[i, j] = a i + b j + c equals a ys + b xs + c at every sample inside the image
+  and the constant value outside, with ys = o0 + rs sin(ts), xs = o1 + rs cos(ts);
+  polar_linear of P[t, r] = a t + b r + c likewise at its sample positions;
+* add_pad on an image whose columns are numbered duplicates exactly the columns the
+  reference's repeated np.insert duplicates (hand-derived case), and the product's
+  column map gives the same image;
+* griddata 'linear' reproduces a linear function inside the hull and is NaN
+  outside; a masked source pixel masks every target point whose triangle touches it;
+* optimize_scale_offset recovers an exact affine relation.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import comparison as C
+from giwaxsim_amd.tools.comparison import pad_columns
+
+
+def test_linear_polar_of_linear_image_is_exact():
+    h, w = 9, 12
+    i, j = np.mgrid[:h, :w].astype(float)
+    a, b, c = 0.7, -1.3, 5.0
+    img = a * i + b * j + c
+    o, r = (4, 5), 6.0
+    out = C.linear_polar(img, o=o, r=r, cont=-99.0)
+    oh, ow = out.shape
+    assert (oh, ow) == (int(round(r * 2 * np.pi)), int(round(r)))
+    rs = np.linspace(0, r, ow)
+    ts = np.linspace(0, 2 * np.pi, oh)
+    xs = rs * np.cos(ts[:, None]) + o[1]
+    ys = rs * np.sin(ts[:, None]) + o[0]
+    inside = (ys >= 0) & (ys <= h - 1) & (xs >= 0) & (xs <= w - 1)
+    want = np.where(inside, a * ys + b * xs + c, -99.0)
+    assert inside.any() and (~inside).any()
+    assert np.allclose(out, want, rtol=0, atol=1e-12)
+
+
+def test_polar_linear_of_linear_polar_image_is_exact():
+    ph, pw = 40, 15
+    t, rr = np.mgrid[:ph, :pw].astype(float)
+    P = 0.25 * t - 0.5 * rr + 2.0
+    out = C.polar_linear(P, o=(6, 7), r=None, output=(13, 16))
+    ys, xs = np.mgrid[:13, :16] - np.array([6, 7])[:, None, None]
+    rs = np.sqrt(ys ** 2 + xs ** 2)
+    ts = np.arctan2(ys, xs)
+    ts[ts < 0] += 2 * np.pi
+    rs = rs / pw * (pw - 1)
+    ts = ts / (2 * np.pi) * (ph - 1)
+    inside = (ts <= ph - 1) & (rs <= pw - 1)
+    assert np.allclose(out, np.where(inside, 0.25 * ts - 0.5 * rs + 2.0, 0.0), rtol=0, atol=1e-12)
+
+
+def test_add_pad_column_duplication_by_hand():
+    # pixel width 0.1, pad 0.2 -> 2 columns; range 0.5..1.5 -> start 5, end 15, spacing 3:
+    # copy of column 5 inserted after it, then of (shifted) column 10 = original 9
+    W = 20
+    r_axis = np.arange(W) * 0.1
+    img = np.tile(np.arange(W, dtype=float), (3, 1))
+    out = C.add_pad(img, r_axis, 0.2, (0.5, 1.5))
+    want_cols = [0, 1, 2, 3, 4, 5, 5, 6, 7, 8, 9, 9, 10, 11, 12, 13, 14, 15, 16, 17]
+    assert np.array_equal(out[0], np.array(want_cols, dtype=float))
+    src = pad_columns(W, r_axis, 0.2, (0.5, 1.5))
+    assert np.array_equal(src, want_cols)
+    assert np.array_equal(img[:, src], out)
+    assert pad_columns(W, r_axis, 0.01, (0.5, 1.5)) is None  # rounds to 0 pixels
+    with pytest.raises(AssertionError):
+        pad_columns(W, r_axis, 0.3, (0.5, 0.6))  # range too small for the pad
+
+
+def test_rebin_linear_function_and_mask():
+    qxy2 = np.linspace(-0.1, 2.0, 41)
+    qz2 = np.linspace(-0.05, 1.5, 33)
+    X2, Z2 = np.meshgrid(qxy2, qz2)
+    m2 = 3.0 * X2 - 2.0 * Z2 + 0.5
+    mask2 = np.zeros_like(m2)
+    mask2[10, 20] = 1
+    qxy1 = np.linspace(0.0, 2.1, 37)   # last columns outside the source hull
+    qz1 = np.linspace(0.0, 1.4, 29)
+    r, mk = C.rebin_qmap_and_mask(qxy1, qz1, m2, qxy2, qz2, mask2)
+    X1, Z1 = np.meshgrid(qxy1, qz1)
+    inside = X1 <= 2.0
+    assert np.allclose(r[inside], (3.0 * X1 - 2.0 * Z1 + 0.5)[inside], rtol=0, atol=1e-12)
+    assert np.all(np.isnan(r[~inside]))
+    # targets within one source cell of the masked pixel are masked, far ones are not
+    near = (np.abs(X1 - qxy2[20]) < (qxy2[1] - qxy2[0])) & (np.abs(Z1 - qz2[10]) < (qz2[1] - qz2[0]))
+    assert mk[near].any() and not mk[~near].any()
+
+
+def test_optimize_scale_offset_exact():
+    rng = np.random.default_rng(0)
+    sim = rng.normal(size=(20, 30))
+    ref = 2.5 * sim - 0.75
+    mask = np.zeros_like(sim, dtype=int)
+    mask[3:5] = 1
+    ref[3:5] = 1e6  # masked pixels do not count
+    s, b = C.optimize_scale_offset(sim, ref, mask)
+    assert abs(s - 2.5) < 1e-12 and abs(b + 0.75) < 1e-12
