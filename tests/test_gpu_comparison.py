@@ -10,7 +10,10 @@ calls), on the same seeded inputs:
   identical masks;
 * FitComparison.residual (trim, shift_peak, scale / offset, masked residual, all on
   the device) vs the oracle's fit_residual: <= 1e-10 of max|residual|, scale and
-  offset to 1e-10.
+  offset to 1e-10;
+* fit_slabsize.fullfit_model end to end (slab builder, GPU voxel grid and detector,
+  device comparison) vs the oracle pipeline on the graphite test cell: <= 1e-4 of the
+  experimental map's scale (the image itself carries the FP32-transform error).
 """
 
 import sys
@@ -136,3 +139,49 @@ def test_rebin_and_fit_residual_match_oracle():
     assert e_rb <= 1e-12
     assert e_res <= 1e-10
     assert abs(s_gpu - s_ref) <= 1e-10 * abs(s_ref) and abs(b_gpu - b_ref) <= 1e-10 * abs(b_ref)
+
+
+def test_fullfit_model_matches_oracle_pipeline():
+    """fit_slabsize.fullfit_model (slab -> GPU voxel grid -> GPU detector -> device
+    comparison) vs the oracle pipeline (voxelgridmaker, detectormaker, fit_residual) on
+    the graphite test cell; the simulated image is the FP32-transform path (1e-5 above
+    1e-6 max), so the residual is compared to 1e-4 of the experimental map's scale."""
+    import os
+    import oracle as O
+    from giwaxsim_amd.fit_slabsize import fullfit_model
+    from giwaxsim_amd.tools import comparison as GC
+    golden = os.path.join(os.path.dirname(__file__), 'golden')
+    _say("[gpu-compare] fullfit_model on graphite_medium.xyz 20 x 20 x 14 A, N=256, 64 px x 19 psi")
+    qv = 2 * np.pi / (0.25 * 256) * (1 + 1e-9)
+    psis = np.linspace(0, 180, 19)
+    one = np.array([0.0])
+    fixed_slab = (f'{golden}/graphite_medium.xyz', 2.456, 2.456, 6.696, 90.0, 90.0, 120.0)
+    fixed_vox = (0.25, qv, 2.0, 12700.0, True, 2)
+    fixed_det = (64, (0.15, 0, 0), ('psi', 'None', 'None'), psis, None, one, None, one, None)
+    # experimental map on its own grid, rebinned onto the trimmed simulated grid
+    exp_qxy = np.linspace(0.05, 1.9, 90)
+    exp_qz = np.linspace(0.0, 1.85, 80)
+    X, Z = np.meshgrid(exp_qxy, exp_qz)
+    exp_img = 1e-4 * (1.5 + np.cos(6 * np.hypot(X, Z)))
+    mask = np.zeros(X.shape)
+    mask[:, :5] = 1
+    h_full = np.linspace(-2.0, 2.0, 64)
+    _, h_t, v_t = C.trim_sim_data(np.zeros((64, 64)), h_full, h_full, exp_qxy, exp_qz)
+    rebin_map, rebin_mask = C.rebin_qmap_and_mask(h_t, v_t, exp_img, exp_qxy, exp_qz, mask)
+    rebin_map = np.where(np.isnan(rebin_map), 0.0, rebin_map)
+    rebin_map[rebin_mask == 1] = 0
+    fixed_exp = (rebin_map, rebin_mask, exp_qxy, exp_qz, 0.05, (1.0, 1.6))
+    params = {'x_size': 20.0, 'y_size': 20.0, 'z_size': 14.0}
+    r_gpu = fullfit_model(params, fixed_slab, fixed_vox, fixed_det, fixed_exp)
+    # oracle pipeline
+    coords, el = GC.slabmaker_fitting(fixed_slab[0], 20.0, 20.0, 14.0, *fixed_slab[1:])
+    ids, ftab, _, _ = engine.element_table(el, 12700.0)
+    ref = O.voxelgridmaker(coords, ftab[ids], 0.25, qv, 2.0, fill_bkg=True, smooth=2)
+    d, h, v = O.detectormaker(ref['iq'], ref['qx'], ref['qy'], ref['qz'], 64, 2.0, (0.15, 0, 0),
+                              ('psi', 'None', 'None'), psis, None, one, None, one, None, mirror=True)
+    assert np.array_equal(h, h_full)
+    r_ref, _ = C.fit_residual(d, h, v, rebin_map, rebin_mask, exp_qxy, exp_qz, 0.05, (1.0, 1.6))
+    err = np.max(np.abs(r_gpu - r_ref)) / np.max(np.abs(rebin_map))
+    _say(f"  residual: max |gpu - oracle| / max|map| = {err:.2e}; |r| {np.linalg.norm(r_ref):.3e}")
+    assert r_gpu.shape == r_ref.shape
+    assert err <= 1e-4
