@@ -28,6 +28,7 @@ inside the grid edge.
 from __future__ import annotations
 
 import ctypes
+import hashlib
 import math
 
 import numpy as np
@@ -45,6 +46,11 @@ def _uniform(a):
     a = np.atleast_1d(np.asarray(a, dtype=np.float64))
     return a, np.ones_like(a) / len(a)
 
+
+
+# transpose maps of the detector gather, by geometry (see DifferentiableSlab)
+MAP_BYTES_MAX = 32e9
+_DET_MAPS: dict = {}
 
 class _DetectorImage(torch.autograd.Function):
     @staticmethod
@@ -189,11 +195,12 @@ class DifferentiableSlab:
         self.det_out = _empty(npx * npx, torch.float64)
         self.det_work = _empty(2 * npx * npx + (npx * npx + 255) // 256, torch.float64)
         # the detector adjoint goes through a transpose map of the orientation gather
-        # when it fits (<= 8 GB of pair ids and their voxels), else scatters into copies
-        # of its accumulator (<= 256 MB)
+        # when it fits (<= 32 GB of pair ids and their voxels, of the 288 GB of HBM),
+        # else scatters into copies of its accumulator (<= 256 MB)
         n_pairs = npx * npx * len(dp.orient_w)
         pbits = int(np.ceil(np.log2(max(npx * npx, 1))))
-        self.mapped = n_pairs * 8 <= 8e9 and pbits < 32 and len(dp.orient_w) <= 2 ** (32 - pbits)
+        self.mapped = (n_pairs * 8 <= MAP_BYTES_MAX and pbits < 32 and len(dp.orient_w) <= 2 ** (32 - pbits)
+                       and nb < 2 ** 31)
         self.n_rep = 1 if self.mapped else int(max(1, min(8, 256e6 // (8 * nb))))
         self.g_fx = _empty(self.n_rep * nb, torch.int64)
         self.det_map = None
@@ -207,15 +214,24 @@ class DifferentiableSlab:
             cy=g.c_n, cx=g.c_n, cz=g.c_n, qx_min=float(np.min(ax)), qy_min=float(np.min(ax)),
             qz_min=float(np.min(ax)), dq=float(np.diff(ax)[0]), det=ptr(self.det_raw).value)
         if self.mapped:
-            cnt = _empty(nb, torch.int32)
-            scan = _empty(-(-nb // 2048) + 1, torch.int64)
-            off = _empty(nb + 1, torch.int64)
-            pairs = _empty(n_pairs, torch.int32)
-            pair_voxel = _empty(n_pairs, torch.int32)
-            call("gx_detector_map_build", ctypes.byref(self.detector_desc), ptr(cnt), ptr(scan), ptr(off),
-                 ptr(pairs), ptr(pair_voxel), stream_handle())
-            self.det_map = (pair_voxel, pairs)
-            del cnt, scan, off
+            # the map depends on the detector geometry and the q grid only: models of
+            # the same geometry (a batch of slabs) share the last one built
+            key = hashlib.sha1(b''.join(np.ascontiguousarray(a, dtype=np.float64).tobytes() for a in
+                                        (dp.h, dp.v, dp.init_R, dp.orient_R, np.array(
+                                            [g.c_n, np.min(ax), np.diff(ax)[0], dev.index or 0])))).hexdigest()
+            hit = _DET_MAPS.get(key)
+            if hit is None:
+                _DET_MAPS.clear()
+                cnt = _empty(nb, torch.int32)
+                scan = _empty(-(-nb // 2048) + 1, torch.int64)
+                off = _empty(nb + 1, torch.int64)
+                pairs = _empty(n_pairs, torch.int32)
+                pair_voxel = _empty(n_pairs, torch.int32)
+                call("gx_detector_map_build", ctypes.byref(self.detector_desc), ptr(cnt), ptr(scan), ptr(off),
+                     ptr(pairs), ptr(pair_voxel), stream_handle())
+                hit = _DET_MAPS[key] = (pair_voxel, pairs)
+                del cnt, scan, off
+            self.det_map = hit
         self._generation = 0
         self._prepared = None
 
