@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble (64 slabs) and large-slab (N=2048) figures")
     ap.add_argument("--only", choices=["adjoint", "adjoint_fp64", "adjoint_polymer", "adjoint_large",
-                                       "ensemble", "large_slab", "compare"],
+                                       "ensemble", "large_slab", "compare", "slab_build"],
                     help="run one of the extra workloads alone and print its JSON (for profiling)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="one step, then one 256 MiB device copy (counter calibration) and one "
@@ -371,6 +371,8 @@ def extra_configs(args, engine, torch, which=None):
     reps = 3
     if which in (None, "compare"):
         out["compare"] = compare_config(engine, torch)
+    if which in (None, "slab_build"):
+        out["slab_build"] = slab_build_config(engine, torch)
     if which in (None, "ensemble"):
         out["ensemble"] = ensemble_config(engine, torch, reps)
     if which in (None, "large_slab"):
@@ -428,6 +430,41 @@ def compare_config(engine, torch, reps=20, cpu_reps=3):
             "ms_per_eval": dt * 1e3, "cpu_ms_per_eval": tcpu * 1e3,
             "cpu_kind": "oracle (numpy + scipy.ndimage.map_coordinates, 1 thread)",
             "max_rel_diff_vs_cpu": err}
+
+
+def slab_build_config(engine, torch, reps=20, cpu_reps=3):
+    """Slab builder of a fit evaluation (tools/comparison.py slabmaker_fitting): the
+    PM6 polymer cell (3840 atoms, 44.456 x 45.726 x 40.097 A) tiled 9 x 3 x 11 and
+    cropped to 176 x 43 x 186 A, on the device (SlabBuilder.build: tile, bounds, crop,
+    bounds, shift, two small read-backs) beside the oracle's numpy version."""
+    import oracle.comparison as OC
+    from giwaxsim_amd.tools.comparison import SlabBuilder
+    from giwaxsim_amd.tools.utilities import calc_real_space_abc, load_pdb
+    path = os.path.join(ROOT, "tests", "golden", "PM6_sample.pdb")
+    cell = (44.456, 45.726, 40.097, 90.0, 90.0, 90.0)
+    size = (176.0, 43.0, 186.0)
+    sb = SlabBuilder(path, *cell)
+    x, y, z, el = sb.build(*size)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        x, y, z, el = sb.build(*size)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    coords, elements = load_pdb(path)
+    a, b, c = calc_real_space_abc(*cell)
+    tc = time.perf_counter()
+    for _ in range(cpu_reps):
+        ref, ref_el = OC.slabmaker(coords, elements, *size, a, b, c)
+    tcpu = (time.perf_counter() - tc) / cpu_reps
+    got = np.stack([t_.cpu().numpy() for t_ in (x, y, z)], axis=1)
+    n_rep = len(coords) * 9 * 3 * 11
+    return {"workload": "PM6 cell (3840 atoms) tiled 9 x 3 x 11 ({} atoms), cropped to 176 x 43 x 186 A "
+                        "({} atoms)".format(n_rep, len(ref)),
+            "ms_per_build": dt * 1e3, "cpu_ms_per_build": tcpu * 1e3,
+            "cpu_kind": "oracle (numpy, 1 thread)",
+            "bit_exact_vs_cpu": bool(np.array_equal(got, ref) and
+                                     np.array_equal(sb.uniq[el.cpu().numpy()], ref_el))}
 
 
 def ensemble_config(engine, torch, reps):

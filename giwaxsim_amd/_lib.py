@@ -113,6 +113,9 @@ SIGNATURES = {
     "gx_bin_frame_f64": [_P, _I, _I, _P, _P, _P, POINTER(c_double), _D, _I, _P, _P, _P, _P],
     "gx_collapse_lengths": [POINTER(c_double), _P, _I, _P, _P],
     "gx_scale_offset_moments": [_P, _P, _P, _I, _P, POINTER(c_double), _P],
+    "gx_slab_tile": [_P, _P, _I, _I, _I, _I, POINTER(c_double), _P, _P, _P, _P, _P],
+    "gx_slab_crop": [_P, _P, _P, _P, _I, POINTER(c_double), _P, _P, _P, _P, _P, POINTER(c_int64), _P],
+    "gx_shift_min": [_P, _P, _P, _I, _P, _P],
     "gx_map_linear": [_P, _I, _I, _P, _P, _I, _D, _P, _P],
     "gx_polar_pad": [_P, _I, _I, _P, _I, _P, _P],
     "gx_csr_apply": [_P, _P, _P, _I, _P, _D, _P, _P],

@@ -505,7 +505,11 @@ class Slab:
 
 
 def device_columns(coords, elem_ids):
-    """(x, y, z, elem) contiguous device columns of host or device atoms."""
+    """(x, y, z, elem) contiguous device columns of host or device atoms; ``coords``
+    may already be a tuple of three FP64 device columns (the slab builder's output)."""
+    if isinstance(coords, tuple):
+        x, y, z = (to_dev(c, torch.float64) for c in coords)
+        return x, y, z, to_dev(elem_ids, torch.int32)
     x, y, z = to_dev(coords, torch.float64).t().contiguous()   # one transpose, three rows
     return x, y, z, to_dev(elem_ids, torch.int32)
 
@@ -651,7 +655,8 @@ _WORKSPACES_MAX = 4
 def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, fill_bkg=False,
                   smooth=0, f_table=None, elem_ids=None, batch=None, want64=True, frames=None,
                   precision='auto', variant=None):
-    """Full voxel-grid path of one slab.  Returns dict with device iq32/iq64,
+    """Full voxel-grid path of one slab (``coords``: host (n, 3), device (n, 3) or a tuple
+    of device x, y, z columns).  Returns dict with device iq32/iq64,
     host axes and the objects that produced them.  ``frames=(lo, hi)``
     restricts the run to phi-frames [lo, hi) (bounded samples).
 
@@ -665,7 +670,7 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
     elif not torch.is_tensor(elem_ids):
         counts = np.bincount(np.asarray(elem_ids), minlength=len(f_table))
     columns = None
-    if torch.is_tensor(coords):
+    if torch.is_tensor(coords) or isinstance(coords, tuple):
         xb, yb, zb, zmin, counts, columns = slab_bounds_device(coords, elem_ids, len(f_table),
                                                               with_columns=True)
     else:

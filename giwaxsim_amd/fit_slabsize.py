@@ -3,10 +3,10 @@ reference's ``fit_slabsize.py`` (fullfit_model 19-47).
 
 ``fullfit_model`` keeps the reference's signature and returns the residual vector
 (numpy).  ``SlabSizeModel`` is the same model with its experimental side planned once
-(``tools.comparison.FitComparison``): each evaluation builds the slab (host), runs
-the voxel grid and the detector image on the GPU, and trims, shift_peaks, fits scale
-and offset and forms the masked residual on the device; only the residual vector
-comes back.  The reference drives it with lmfit's Nelder-Mead (lmfit 1.x,
+(``tools.comparison.FitComparison``) and its unit cell uploaded once
+(``tools.comparison.SlabBuilder``): each evaluation tiles and crops the slab, runs the
+voxel grid and the detector image, and trims, shift_peaks, fits scale and offset and
+forms the masked residual, all on the device; only the residual vector comes back.  The reference drives it with lmfit's Nelder-Mead (lmfit 1.x,
 ``lmfit.minimize(..., method='nelder')``); any minimiser of the residual norm works
 with ``SlabSizeModel.residual``.
 """
@@ -16,7 +16,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import engine
-from .tools.comparison import FitComparison, _weights, slabmaker_fitting
+from .tools.comparison import FitComparison, SlabBuilder, _weights
 
 __all__ = ['fullfit_model', 'SlabSizeModel']
 
@@ -47,14 +47,20 @@ class SlabSizeModel:
                                              _weights(theta_wp, thetas))
         self.exp = fixed_exp_params
         self.cmp = None
+        self.builder = None
+        self.f_table = None
 
     def image(self, x_size, y_size, z_size):
         """Device detector image [v, h] (mirrored) of the slab of the given size."""
-        input_filepath, a, b, c, alpha, beta, gamma = self.slab
         r_voxel_size, q_voxel_size, max_q, energy, fill_bkg, smooth = self.vox
-        coords, elements = slabmaker_fitting(input_filepath, x_size, y_size, z_size, a, b, c, alpha, beta, gamma)
-        res = engine.run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy,
-                                   fill_bkg=fill_bkg, smooth=int(smooth), want64=True)
+        if self.builder is None:
+            self.builder = SlabBuilder(*self.slab)
+            # f of every element of the cell (an element the crop drops has count 0)
+            self.f_table = engine.element_table(self.builder.uniq, energy)[1]
+        x, y, z, el = self.builder.build(x_size, y_size, z_size)
+        res = engine.run_voxelgrid((x, y, z), None, r_voxel_size, q_voxel_size, max_q, energy,
+                                   fill_bkg=fill_bkg, smooth=int(smooth), f_table=self.f_table, elem_ids=el,
+                                   want64=True)
         det, _ = engine.run_detector(res['iq64'], res['qx'], res['qy'], res['qz'], self.det_plan, True)
         return det
 

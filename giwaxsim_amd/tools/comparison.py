@@ -17,13 +17,13 @@ import ctypes
 import numpy as np
 
 from .. import engine
-from .._lib import GX_MOMENTS_WORK, call
+from .._lib import GX_MOMENTS_WORK, GxSlab, call
 from .utilities import calc_real_space_abc, load_pdb, load_xyz
 
 __all__ = ['slabmaker_fitting', 'voxelgridmaker_fitting', 'detectormaker_fitting',
            'optimize_scale_offset', 'DeviceIQ', 'trim_sim_data', 'rebin_qmap_and_mask',
            'linear_polar', 'polar_linear', 'add_pad', 'shift_peak', 'pad_columns', 'ShiftPeakPlan',
-           'RebinWeights', 'FitComparison']
+           'RebinWeights', 'FitComparison', 'SlabBuilder']
 
 
 class DeviceIQ(np.ndarray):
@@ -34,52 +34,93 @@ class DeviceIQ(np.ndarray):
         self._gx_device = getattr(obj, '_gx_device', None) if obj is not None else None
 
 
-def slabmaker_fitting(input_filepath, x_size, y_size, z_size, a, b, c, alpha, beta, gamma):
-    """Tile the unit cell and crop a centred slab (tools/comparison.py:595-671).
-    Structure preparation on the host (outside the forward hot path)."""
+def _load_cell(input_filepath):
     if input_filepath.lower().endswith('.xyz'):
-        coords, elements = load_xyz(input_filepath)
-    elif input_filepath.lower().endswith('.pdb'):
-        coords, elements = load_pdb(input_filepath)
-    else:
-        raise Exception('Files must be a .pdb or .xyz file')
-    a_vect, b_vect, c_vect = calc_real_space_abc(a, b, c, alpha, beta, gamma)
-    num_x = int(np.ceil(2 * x_size / a_vect[0]))
-    num_y = int(np.ceil(2 * y_size / b_vect[1]))
-    num_z = int(np.ceil(2 * z_size / c_vect[2]))
-    # replicate along a, then b, then c: copies 0..num (the reference appends
-    # num translated copies of the running set each time)
-    for vec, num in ((np.array([a_vect[0], 0, 0]), num_x),
-                     (np.array([b_vect[0], b_vect[1], 0]), num_y),
-                     (np.array([c_vect[0], c_vect[1], c_vect[2]]), num_z)):
-        base_c, base_e = coords.copy(), elements.copy()
-        parts_c, parts_e = [coords], [elements]
-        for i in range(num):
-            parts_c.append(base_c + vec * (i + 1))
-            parts_e.append(base_e)
-        coords = np.concatenate(parts_c, axis=0)
-        elements = np.concatenate(parts_e, axis=0)
-    x_max = np.max(coords[:, 0]) - np.min(coords[:, 0])
-    y_max = np.max(coords[:, 1]) - np.min(coords[:, 1])
-    z_max = np.max(coords[:, 2]) - np.min(coords[:, 2])
-    assert x_max > x_size, "x_max must be greater than x_size"
-    assert y_max > y_size, "y_max must be greater than y_size"
-    assert z_max > z_size, "z_max must be greater than z_size"
-    x_buffer = (x_max - x_size) / 2
-    y_buffer = (y_max - y_size) / 2
-    z_buffer = (z_max - z_size) / 2
-    coords[:, 0] -= np.min(coords[:, 0])
-    coords[:, 1] -= np.min(coords[:, 1])
-    coords[:, 2] -= np.min(coords[:, 2])
-    mask = ((coords[:, 0] >= x_buffer) & (coords[:, 0] <= x_max - x_buffer) &
-            (coords[:, 1] >= y_buffer) & (coords[:, 1] <= y_max - y_buffer) &
-            (coords[:, 2] >= z_buffer) & (coords[:, 2] <= z_max - z_buffer))
-    cf = coords[mask]
-    ef = elements[mask]
-    cf[:, 0] -= np.min(cf[:, 0])
-    cf[:, 1] -= np.min(cf[:, 1])
-    cf[:, 2] -= np.min(cf[:, 2])
-    return cf, ef
+        return load_xyz(input_filepath)
+    if input_filepath.lower().endswith('.pdb'):
+        return load_pdb(input_filepath)
+    raise Exception('Files must be a .pdb or .xyz file')
+
+
+class SlabBuilder:
+    """slabmaker_fitting (tools/comparison.py:595-671) for one unit-cell file and cell,
+    on the device: the cell is loaded and uploaded once, ``build(x, y, z)`` replicates
+    it (gx_slab_tile: the reference's concatenation order and addition order, so the
+    coordinates are bit-identical), takes the bounding box (gx_slab_bounds, exact),
+    keeps the centred box in the original order (gx_slab_crop) and shifts to the
+    origin (gx_shift_min).  The host only sees the six extremes (for the reference's
+    asserts and buffers) and the kept count."""
+
+    def __init__(self, input_filepath, a, b, c, alpha, beta, gamma):
+        import torch
+        coords, elements = _load_cell(input_filepath)
+        self.uniq, inv = np.unique(np.asarray(elements), return_inverse=True)
+        self.a_vect, self.b_vect, self.c_vect = calc_real_space_abc(a, b, c, alpha, beta, gamma)
+        dev = engine.device()
+        self.n_cell = len(coords)
+        self.cell = torch.as_tensor(np.ascontiguousarray(coords, dtype=np.float64)).to(dev)
+        self.cell_elem = torch.as_tensor(inv.astype(np.int32)).to(dev)
+        self.vec = (ctypes.c_double * 6)(self.a_vect[0], self.b_vect[0], self.b_vect[1], self.c_vect[0],
+                                         self.c_vect[1], self.c_vect[2])
+
+    def _bounds(self, x, y, z, el):
+        import torch
+        out = torch.empty(6 + len(self.uniq), dtype=torch.float64, device=x.device)
+        d = GxSlab(n_atoms=int(x.numel()), n_elem=len(self.uniq), x=engine.ptr(x).value, y=engine.ptr(y).value,
+                   z=engine.ptr(z).value, elem=engine.ptr(el).value)
+        call("gx_slab_bounds", ctypes.byref(d), engine.ptr(out), engine.ptr(out[6:].view(torch.int64)),
+             engine.stream_handle())
+        return out   # xmin, xmax, ymin, ymax, zmin, zmax | per-element counts (int64 words)
+
+    def build(self, x_size, y_size, z_size):
+        """Device (x, y, z, elem_id) columns of the slab; elem ids index ``self.uniq``."""
+        import torch
+        num_x = int(np.ceil(2 * x_size / self.a_vect[0]))
+        num_y = int(np.ceil(2 * y_size / self.b_vect[1]))
+        num_z = int(np.ceil(2 * z_size / self.c_vect[2]))
+        n = self.n_cell * (num_x + 1) * (num_y + 1) * (num_z + 1)
+        dev = self.cell.device
+        x, y, z = (torch.empty(n, dtype=torch.float64, device=dev) for _ in range(3))
+        el = torch.empty(n, dtype=torch.int32, device=dev)
+        call("gx_slab_tile", engine.ptr(self.cell), engine.ptr(self.cell_elem), self.n_cell, num_x, num_y, num_z,
+             self.vec, engine.ptr(x), engine.ptr(y), engine.ptr(z), engine.ptr(el), engine.stream_handle())
+        mm = self._bounds(x, y, z, el)[:6].cpu().numpy()
+        x_max, y_max, z_max = mm[1] - mm[0], mm[3] - mm[2], mm[5] - mm[4]
+        assert x_max > x_size, "x_max must be greater than x_size"
+        assert y_max > y_size, "y_max must be greater than y_size"
+        assert z_max > z_size, "z_max must be greater than z_size"
+        x_buffer = (x_max - x_size) / 2
+        y_buffer = (y_max - y_size) / 2
+        z_buffer = (z_max - z_size) / 2
+        lim = (ctypes.c_double * 9)(mm[0], mm[2], mm[4], x_buffer, x_max - x_buffer, y_buffer, y_max - y_buffer,
+                                    z_buffer, z_max - z_buffer)
+        ox, oy, oz = (torch.empty(n, dtype=torch.float64, device=dev) for _ in range(3))
+        oel = torch.empty(n, dtype=torch.int32, device=dev)
+        work = torch.empty((n + 2047) // 2048 + 1, dtype=torch.int64, device=dev)
+        kept = ctypes.c_int64(0)
+        call("gx_slab_crop", engine.ptr(x), engine.ptr(y), engine.ptr(z), engine.ptr(el), n, lim, engine.ptr(ox),
+             engine.ptr(oy), engine.ptr(oz), engine.ptr(oel), engine.ptr(work), ctypes.byref(kept),
+             engine.stream_handle())
+        m = int(kept.value)
+        del x, y, z, el
+        if m == 0:   # the reference's np.min of the empty selection raises
+            raise ValueError("zero-size array to reduction operation minimum which has no identity")
+        ox, oy, oz, oel = ox[:m], oy[:m], oz[:m], oel[:m]
+        mm2 = self._bounds(ox, oy, oz, oel)
+        call("gx_shift_min", engine.ptr(ox), engine.ptr(oy), engine.ptr(oz), m, engine.ptr(mm2),
+             engine.stream_handle())
+        return ox, oy, oz, oel
+
+
+def slabmaker_fitting(input_filepath, x_size, y_size, z_size, a, b, c, alpha, beta, gamma):
+    """Tile the unit cell and crop a centred slab (tools/comparison.py:595-671): the
+    replication, bounding boxes, crop and shifts on the GPU (``SlabBuilder``), the
+    coordinates and the order of the atoms identical to the reference's."""
+    import torch
+    sb = SlabBuilder(input_filepath, a, b, c, alpha, beta, gamma)
+    x, y, z, el = sb.build(x_size, y_size, z_size)
+    coords = torch.stack((x, y, z), dim=1).cpu().numpy()
+    return coords, sb.uniq[el.cpu().numpy()]
 
 
 def voxelgridmaker_fitting(coords, elements, r_voxel_size, q_voxel_size, max_q, energy,

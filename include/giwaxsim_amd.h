@@ -294,6 +294,28 @@ int gx_scale_offset_moments(const double *sim, const double *exp_, const uint8_t
 #define GX_MOMENTS_WORK 774
 
 /* ------------------------------------------------------------------------------------
+ * Slab builder of a fit evaluation (tools/comparison.py slabmaker_fitting 595-671).
+ * ---------------------------------------------------------------------------------- */
+/* Replicate a unit cell: atom o = ((kz (num_y + 1) + ky) (num_x + 1) + kx) n_cell + i of
+ * the reference's concatenations (x copies, then y copies of that set, then z copies)
+ * at cell[i] + kx a + ky b + kz c, added in that order with a = (v0, 0, 0),
+ * b = (v1, v2, 0), c = (v3, v4, v5) (host_vec[6]); elem[o] = cell_elem[i].  Outputs
+ * [n_cell (num_x + 1)(num_y + 1)(num_z + 1)] (< 2^31).  Replaces comparison.py:604-633. */
+int gx_slab_tile(const double *cell_xyz, const int32_t *cell_elem, int64_t n_cell, int64_t num_x, int64_t num_y,
+                 int64_t num_z, const double *host_vec, double *x, double *y, double *z, int32_t *elem,
+                 gx_stream_t stream);
+/* The centred crop (comparison.py:648-663): keep atom a when lo_k <= c_k - min_k <= hi_k
+ * on every axis (host_lim[9] = x, y, z minima, then lo, hi per axis), written shifted
+ * (c_k - min_k) in the original order.  work [ceil(n / 2048) + 1].  Blocks until the
+ * kept count is in *host_count. */
+int gx_slab_crop(const double *x, const double *y, const double *z, const int32_t *elem, int64_t n,
+                 const double *host_lim, double *ox, double *oy, double *oz, int32_t *oelem, int64_t *work,
+                 int64_t *host_count, gx_stream_t stream);
+/* x -= minmax[0], y -= minmax[2], z -= minmax[4] (gx_slab_bounds' device output):
+ * the final shift to the origin (comparison.py:667-669). */
+int gx_shift_min(double *x, double *y, double *z, int64_t n, const double *minmax, gx_stream_t stream);
+
+/* ------------------------------------------------------------------------------------
  * Experimental comparison of a fit evaluation (tools/comparison.py, fit_slabsize.py).
  * Sample positions, pad maps and rebin weights depend on geometry only and are
  * planned on the host; these entry points apply them to device images.

@@ -15,6 +15,8 @@ calls the reference makes (scipy.ndimage.map_coordinates, scipy.interpolate.grid
 * ``optimize_scale_offset`` tools/comparison.py:873-882
 * ``fit_residual``     the residual of fullfit_model, fit_slabsize.py:19-47
   (after the detector image: trim, shift_peak, scale / offset, masked residual)
+* ``slabmaker``        tools/comparison.py:595-671 after the file load and the lattice
+  vectors: replicate, bounding box, shift, centred crop, shift
 """
 
 from __future__ import annotations
@@ -140,3 +142,34 @@ def fit_residual(det_sum, det_h, det_v, rebin_map, rebin_mask, exp_qxy, exp_qz, 
     sim_pad = shift_peak(sim_trim.copy(), h_trim, v_trim, pad_width, pad_range)
     s, b = optimize_scale_offset(sim_pad, rebin_map, rebin_mask)
     return ((rebin_map - (s * sim_pad + b)) * (rebin_mask == 0)).ravel(), (s, b)
+
+
+def slabmaker(coords, elements, x_size, y_size, z_size, a_vect, b_vect, c_vect):
+    """tools/comparison.py:608-671: copies 1..num of the running set appended along a,
+    then b, then c (each copy = set + (i + 1) * vector), the bounding box, the shift to
+    the origin, the centred box of the requested size and the final shift."""
+    num_x = int(np.ceil(2 * x_size / a_vect[0]))
+    num_y = int(np.ceil(2 * y_size / b_vect[1]))
+    num_z = int(np.ceil(2 * z_size / c_vect[2]))
+    for num, vec in ((num_x, (a_vect[0], 0, 0)), (num_y, (b_vect[0], b_vect[1], 0)),
+                     (num_z, (c_vect[0], c_vect[1], c_vect[2]))):
+        base_c, base_e = coords.copy(), elements.copy()
+        for i in range(num):
+            step = np.array([vec[0] * (i + 1), vec[1] * (i + 1), vec[2] * (i + 1)])
+            coords = np.concatenate((coords, base_c + step), axis=0)
+            elements = np.concatenate((elements, base_e), axis=0)
+    x_max = np.max(coords[:, 0]) - np.min(coords[:, 0])
+    y_max = np.max(coords[:, 1]) - np.min(coords[:, 1])
+    z_max = np.max(coords[:, 2]) - np.min(coords[:, 2])
+    assert x_max > x_size, "x_max must be greater than x_size"
+    assert y_max > y_size, "y_max must be greater than y_size"
+    assert z_max > z_size, "z_max must be greater than z_size"
+    xb, yb, zb = (x_max - x_size) / 2, (y_max - y_size) / 2, (z_max - z_size) / 2
+    for k in range(3):
+        coords[:, k] -= np.min(coords[:, k])
+    keep = ((coords[:, 0] >= xb) & (coords[:, 0] <= x_max - xb) & (coords[:, 1] >= yb) &
+            (coords[:, 1] <= y_max - yb) & (coords[:, 2] >= zb) & (coords[:, 2] <= z_max - zb))
+    out, out_e = coords[keep], elements[keep]
+    for k in range(3):
+        out[:, k] -= np.min(out[:, k])
+    return out, out_e

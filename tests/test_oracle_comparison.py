@@ -10,7 +10,11 @@ host planning of the device version (pad column map), by closed forms:
   column map gives the same image;
 * griddata 'linear' reproduces a linear function inside the hull and is NaN
   outside; a masked source pixel masks every target point whose triangle touches it;
-* optimize_scale_offset recovers an exact affine relation.
+* optimize_scale_offset recovers an exact affine relation;
+* slabmaker on a two-atom cubic cell keeps exactly the translates of the centred box,
+  cell atom fastest, then x, y, z copies (half-integer coordinates, exact); on a
+  two-element hexagonal cell its output equals, as a set, a brute-force enumeration of
+  the translates inside the box, with zero minima on every axis.
 """
 
 import numpy as np
@@ -99,3 +103,59 @@ def test_optimize_scale_offset_exact():
     ref[3:5] = 1e6  # masked pixels do not count
     s, b = C.optimize_scale_offset(sim, ref, mask)
     assert abs(s - 2.5) < 1e-12 and abs(b + 0.75) < 1e-12
+
+
+def test_slabmaker_cubic_lattice_by_hand():
+    """a = b = c = 1, C at the origin and O at (0.5, 0.5, 0), sizes (2.3, 3.7, 1.2):
+    5, 8 and 3 copies give extents 5.5, 8.5 and 3; the buffers 1.6, 2.4 and 0.9 keep
+    C at copies kx in {2, 3}, ky in {3..6}, kz in {1, 2} and O at kx in {2, 3},
+    ky in {2..5}, kz in {1, 2}; minima (2, 2.5, 1).  The reference's order is cell atom
+    fastest, then kx, then ky, then kz (all values exact multiples of 0.5)."""
+    eye = np.eye(3)
+    cell = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0]])
+    out, el = C.slabmaker(cell, np.array(['C', 'O']), 2.3, 3.7, 1.2, eye[0], eye[1], eye[2])
+    want, want_el = [], []
+    for kz in range(4):
+        for ky in range(9):
+            for kx in range(6):
+                if kx in (2, 3) and 3 <= ky <= 6 and kz in (1, 2):
+                    want.append((kx - 2, ky - 2.5, kz - 1))
+                    want_el.append('C')
+                if kx in (2, 3) and 2 <= ky <= 5 and kz in (1, 2):
+                    want.append((kx + 0.5 - 2, ky + 0.5 - 2.5, kz - 1))
+                    want_el.append('O')
+    assert np.array_equal(out, np.array(want)) and list(el) == want_el
+
+
+def test_slabmaker_hexagonal_cell_brute_force():
+    a, c = 2.5, 6.5
+    av = np.array([a, 0.0, 0.0])
+    bv = np.array([a * np.cos(np.radians(120)), a * np.sin(np.radians(120)), 0.0])
+    cv = np.array([0.0, 0.0, c])
+    cell = np.array([[0.0, 0.0, 0.0], [0.3, 1.1, 3.2]])
+    cel = np.array(['C', 'O'])
+    size = (7.0, 6.0, 9.0)
+    out, el = C.slabmaker(cell.copy(), cel.copy(), *size, av, bv, cv)
+    assert np.all(out.min(axis=0) == 0.0) and np.all(out.max(axis=0) <= np.array(size) + 1e-9)
+    nx, ny, nz = (int(np.ceil(2 * s / v)) for s, v in zip(size, (av[0], bv[1], cv[2])))
+    pts, els = [], []
+    for i in range(2):
+        for kx in range(nx + 1):
+            for ky in range(ny + 1):
+                for kz in range(nz + 1):
+                    pts.append(cell[i] + kx * av + ky * bv + kz * cv)
+                    els.append(cel[i])
+    pts, els = np.array(pts), np.array(els)
+    ext = pts.max(axis=0) - pts.min(axis=0)
+    sh = pts - pts.min(axis=0)
+    buf = (ext - np.array(size)) / 2
+    keep = np.all((sh >= buf - 1e-9) & (sh <= ext - buf + 1e-9), axis=1)
+    want = sh[keep] - sh[keep].min(axis=0)
+    assert len(out) == int(keep.sum()) > 0
+
+    def canon(p, e):
+        order = np.lexsort((np.round(p[:, 0], 6), np.round(p[:, 1], 6), np.round(p[:, 2], 6), e))
+        return p[order], e[order]
+    p1, e1 = canon(out, el)
+    p2, e2 = canon(want, els[keep])
+    assert np.array_equal(e1, e2) and np.max(np.abs(p1 - p2)) < 1e-9
