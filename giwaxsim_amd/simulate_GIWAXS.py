@@ -6,9 +6,11 @@ Reads the same ``key=value`` config (keys and defaults of
 simulate_GIWAXS.py:18-82), builds each slab, runs the GPU voxel-grid and
 detector paths and saves ``det_h.npy``, ``det_v.npy``, ``det_sum.npy``
 (+ PNGs when matplotlib is available) and ``config.txt`` per input file,
-like simulate_GIWAXS.py:96-218.  The experimental-comparison branch
-(``img_path``) needs ``fabio`` and the rebinning of
-tools/comparison.py:406-445, which this package does not provide.
+like simulate_GIWAXS.py:96-218.  With ``img_path`` set, the experimental-comparison
+branch (simulate_GIWAXS.py:123-169) mirrors, trims and rebins the experimental map,
+pads the simulated peaks, fits scale / offset when ``fit_scale_offset`` and saves the
+trimmed axes, ``rebin_map.npy``, ``sim_comp_map.npy`` and the comparison figures; the
+image is read from ``.npy`` directly, other formats through fabio when installed.
 """
 
 from __future__ import annotations
@@ -61,7 +63,13 @@ def main(config):
     theta_weights_path = config.get('theta_weights_path', None)
     mirror = str_to_bool(config.get('mirror', 'False'))
 
+    mask_path = config.get('mask_path', None)
     img_path = config.get('img_path', None)
+    qxy_path = config.get('qxy_path', None)
+    qz_path = config.get('qz_path', None)
+    fit_scale_offset = str_to_bool(config.get('fit_scale_offset', 'False'))
+    pad_width = float(config.get('pad_width', 0))
+    pad_range = (float(config.get('pad_range_min', 0)), float(config.get('pad_range_max', 0)))
     save_folder = config.get('save_folder', os.getcwd())
     os.makedirs(save_folder, exist_ok=True)
 
@@ -102,8 +110,10 @@ def main(config):
         sub = f'{save_folder}/{gen_names[i]}'
         os.makedirs(sub, exist_ok=True)
         if img_path:
-            raise NotImplementedError("experimental comparison (img_path) needs fabio and the "
-                                      "q-map rebinning, which giwaxsim_amd does not provide")
+            outputs[gen_names[i]] = _compare(sub, det_sum, det_h, det_v, img_path, qxy_path, qz_path, mask_path,
+                                             pad_width, pad_range, fit_scale_offset, max_q)
+            save_config_to_txt(config, f'{sub}/config.txt')
+            continue
         if mirror:
             kh = np.where(det_h >= 0)[0]
             kv = np.where(det_v >= 0)[0]
@@ -117,6 +127,44 @@ def main(config):
         _plots(sub, det_sum, det_h, det_v)
         save_config_to_txt(config, f'{sub}/config.txt')
     return outputs
+
+
+def _compare(sub, det_sum, det_h, det_v, img_path, qxy_path, qz_path, mask_path, pad_width, pad_range,
+             fit_scale_offset, max_q):
+    """simulate_GIWAXS.py:123-169 for one simulated image."""
+    from .tools.comparison import (load_exp_image, mirror_qmap_positive_qxy_only, optimize_scale_offset,
+                                   plot_fit, plot_linecuts, rebin_qmap_and_mask, shift_peak, trim_sim_data)
+    exp_img = load_exp_image(img_path)
+    exp_qxy = np.loadtxt(qxy_path)
+    exp_qz = np.loadtxt(qz_path)
+    mask = np.load(mask_path, allow_pickle=False)
+    exp_img, exp_qxy, exp_qz = mirror_qmap_positive_qxy_only(exp_img, exp_qxy, exp_qz)
+    sim_trim, det_h_trim, det_v_trim = trim_sim_data(det_sum, det_h, det_v, exp_qxy, exp_qz)
+    rebin_map, rebin_mask = rebin_qmap_and_mask(det_h_trim, det_v_trim, exp_img, exp_qxy, exp_qz, mask)
+    rebin_map[rebin_mask == 1] = 0
+    sim_trim[rebin_mask == 1] = 0
+    if pad_width > 0:
+        sim_pad = shift_peak(sim_trim.copy(), det_h_trim, det_v_trim, pad_width, pad_range)
+    else:
+        sim_pad = sim_trim.copy()
+    if fit_scale_offset:
+        scale, offset = optimize_scale_offset(sim_pad, rebin_map, rebin_mask)
+        sim_comp_map = scale * sim_pad + offset
+        sim_comp_map[rebin_mask == 1] = 0
+    else:
+        sim_comp_map = sim_pad.copy()
+    diff_map = np.where(rebin_map > 1e-10, sim_comp_map - rebin_map, 0)
+    np.save(f'{sub}/det_h.npy', det_h_trim)
+    np.save(f'{sub}/det_v.npy', det_v_trim)
+    np.save(f'{sub}/rebin_map.npy', rebin_map)
+    np.save(f'{sub}/sim_comp_map.npy', sim_comp_map)
+    try:
+        import matplotlib  # noqa: F401
+    except ImportError:
+        return rebin_map, sim_comp_map, diff_map
+    plot_fit(f'{sub}/fit_result.png', rebin_map, sim_comp_map, diff_map, det_h_trim, det_v_trim, max_q, '')
+    plot_linecuts(f'{sub}/fit_result_linecut.png', rebin_map, sim_comp_map, det_h_trim, det_v_trim, '')
+    return rebin_map, sim_comp_map, diff_map
 
 
 def _plots(sub, det_sum, det_h, det_v):

@@ -23,7 +23,8 @@ from .utilities import calc_real_space_abc, load_pdb, load_xyz
 __all__ = ['slabmaker_fitting', 'voxelgridmaker_fitting', 'detectormaker_fitting',
            'optimize_scale_offset', 'DeviceIQ', 'trim_sim_data', 'rebin_qmap_and_mask',
            'linear_polar', 'polar_linear', 'add_pad', 'shift_peak', 'pad_columns', 'ShiftPeakPlan',
-           'RebinWeights', 'FitComparison', 'SlabBuilder']
+           'RebinWeights', 'rebin_weights', 'FitComparison', 'SlabBuilder', 'mirror_qmap_positive_qxy_only', 'evaluate_fit',
+           'make_linecut', 'plot_fit', 'plot_linecuts', 'load_exp_image']
 
 
 class DeviceIQ(np.ndarray):
@@ -51,9 +52,11 @@ class SlabBuilder:
     origin (gx_shift_min).  The host only sees the six extremes (for the reference's
     asserts and buffers) and the kept count."""
 
-    def __init__(self, input_filepath, a, b, c, alpha, beta, gamma):
+    def __init__(self, input_filepath, a, b, c, alpha, beta, gamma, cell=None):
+        """``cell=(coords, elements)`` gives the unit cell directly (``input_filepath``
+        is then not read)."""
         import torch
-        coords, elements = _load_cell(input_filepath)
+        coords, elements = _load_cell(input_filepath) if cell is None else cell
         self.uniq, inv = np.unique(np.asarray(elements), return_inverse=True)
         self.a_vect, self.b_vect, self.c_vect = calc_real_space_abc(a, b, c, alpha, beta, gamma)
         dev = engine.device()
@@ -72,8 +75,10 @@ class SlabBuilder:
              engine.stream_handle())
         return out   # xmin, xmax, ymin, ymax, zmin, zmax | per-element counts (int64 words)
 
-    def build(self, x_size, y_size, z_size):
-        """Device (x, y, z, elem_id) columns of the slab; elem ids index ``self.uniq``."""
+    def build(self, x_size, y_size, z_size, shift_again=True):
+        """Device (x, y, z, elem_id) columns of the slab; elem ids index ``self.uniq``.
+        ``shift_again=False`` skips the final shift of the kept atoms to the origin
+        (the standalone slabmaker.py:83-116 writes them as cut)."""
         import torch
         num_x = int(np.ceil(2 * x_size / self.a_vect[0]))
         num_y = int(np.ceil(2 * y_size / self.b_vect[1]))
@@ -103,12 +108,13 @@ class SlabBuilder:
              engine.stream_handle())
         m = int(kept.value)
         del x, y, z, el
-        if m == 0:   # the reference's np.min of the empty selection raises
+        if m == 0 and shift_again:   # the reference's np.min of the empty selection raises
             raise ValueError("zero-size array to reduction operation minimum which has no identity")
         ox, oy, oz, oel = ox[:m], oy[:m], oz[:m], oel[:m]
-        mm2 = self._bounds(ox, oy, oz, oel)
-        call("gx_shift_min", engine.ptr(ox), engine.ptr(oy), engine.ptr(oz), m, engine.ptr(mm2),
-             engine.stream_handle())
+        if shift_again:
+            mm2 = self._bounds(ox, oy, oz, oel)
+            call("gx_shift_min", engine.ptr(ox), engine.ptr(oy), engine.ptr(oz), m, engine.ptr(mm2),
+                 engine.stream_handle())
         return ox, oy, oz, oel
 
 
@@ -419,37 +425,46 @@ def trim_sim_data(sim_det_ints, det_h, det_v, exp_qxy, exp_qz):
     return trimmed, det_h[h_mask], det_v[v_mask]
 
 
-class RebinWeights:
+def rebin_weights(qxy1, qz1, qxy2, qz2):
     """griddata(points2, values, (grid1_x, grid1_z), method='linear') as sparse
-    weights: the barycentric coordinates of every target point in its triangle of
-    the Delaunay triangulation of the source grid (the triangulation and point
-    location griddata itself uses, scipy.spatial.Delaunay / find_simplex), three
-    entries per target row, none outside the hull (NaN there, griddata's fill)."""
+    weights (host): (row_ptr int64 [n+1], col int32, val FP64, target shape).  The
+    triangle of each target point comes from the Delaunay triangulation of the source
+    grid and its point location (scipy.spatial.Delaunay / find_simplex, which griddata
+    uses), its barycentric coordinates in griddata's arithmetic:
+    c_i = T_i0 (x0 - r0) + T_i1 (x1 - r1), c_2 = (1 - c_0) - c_1, so that applying
+    the weights in the stored order (0 + c_0 v_0 + c_1 v_1 + c_2 v_2) reproduces
+    griddata bit for bit; rows outside the hull are empty (NaN there, griddata's fill)."""
+    from scipy.spatial import Delaunay
+    grid1_x, grid1_z = np.meshgrid(qxy1, qz1)
+    grid2_x, grid2_z = np.meshgrid(qxy2, qz2)
+    points2 = np.vstack((grid2_x.ravel(), grid2_z.ravel())).T
+    xi = np.vstack((grid1_x.ravel(), grid1_z.ravel())).T
+    tri = Delaunay(points2)
+    simplex = tri.find_simplex(xi)
+    inside = simplex >= 0
+    T = tri.transform[simplex[inside]]                      # [m, 3, 2]
+    d0 = xi[inside, 0] - T[:, 2, 0]
+    d1 = xi[inside, 1] - T[:, 2, 1]
+    c0 = T[:, 0, 0] * d0 + T[:, 0, 1] * d1
+    c1 = T[:, 1, 0] * d0 + T[:, 1, 1] * d1
+    bary = np.stack([c0, c1, (1.0 - c0) - c1], axis=1)
+    verts = tri.simplices[simplex[inside]]
+    counts = np.where(inside, 3, 0)
+    row_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    return row_ptr, verts.astype(np.int32).ravel(), bary.ravel(), grid1_x.shape
+
+
+class RebinWeights:
+    """``rebin_weights`` resident on the device, applied by gx_csr_apply."""
 
     def __init__(self, qxy1, qz1, qxy2, qz2):
-        from scipy.spatial import Delaunay
         torch = _torch()
-        grid1_x, grid1_z = np.meshgrid(qxy1, qz1)
-        grid2_x, grid2_z = np.meshgrid(qxy2, qz2)
-        points2 = np.vstack((grid2_x.ravel(), grid2_z.ravel())).T
-        xi = np.vstack((grid1_x.ravel(), grid1_z.ravel())).T
-        tri = Delaunay(points2)
-        simplex = tri.find_simplex(xi)
-        inside = simplex >= 0
-        T = tri.transform[simplex[inside]]                      # [m, 3, 2]
-        d = xi[inside] - T[:, 2, :]
-        c = np.einsum('mij,mj->mi', T[:, :2, :], d)
-        bary = np.concatenate([c, 1.0 - c.sum(axis=1, keepdims=True)], axis=1)
-        verts = tri.simplices[simplex[inside]]
-        n_rows = xi.shape[0]
-        counts = np.where(inside, 3, 0)
-        row_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
-        self.shape = grid1_x.shape
-        self.n_rows = n_rows
+        row_ptr, col, val, self.shape = rebin_weights(qxy1, qz1, qxy2, qz2)
+        self.n_rows = len(row_ptr) - 1
         dev = engine.device()
         self.row_ptr = torch.as_tensor(row_ptr).to(dev)
-        self.col = torch.as_tensor(verts.astype(np.int32).ravel()).to(dev)
-        self.val = torch.as_tensor(bary.ravel()).to(dev)
+        self.col = torch.as_tensor(col).to(dev)
+        self.val = torch.as_tensor(val).to(dev)
 
     def apply(self, values_dev, empty=np.nan):
         torch = _torch()
@@ -508,13 +523,162 @@ class FitComparison:
         assert tuple(self.ref.shape) == shape, "rebin_map must match the trimmed detector grid"
         self.n = int(np.prod(shape))
 
+    def padded(self, det_dev):
+        """Trimmed, shift_peak-ed device image [v_trim, h_trim] of a device image [v, h]."""
+        sim = det_dev.index_select(0, self.iv).index_select(1, self.ih).contiguous()
+        return self.shift(sim)
+
     def residual(self, det_dev):
         """(residual device tensor [n], scale, offset) for a device image [v, h]."""
         torch = _torch()
-        sim = det_dev.index_select(0, self.iv).index_select(1, self.ih).contiguous()
-        sim = self.shift(sim)
+        sim = self.padded(det_dev)
         scale, offset = _scale_offset_dev(sim, self.ref, self.mask)
         out = torch.empty(self.n, dtype=torch.float64, device=sim.device)
         call("gx_fit_residual", engine.ptr(sim), engine.ptr(self.ref), engine.ptr(self.mask), self.n,
              float(scale), float(offset), engine.ptr(out), engine.stream_handle())
         return out, scale, offset
+
+
+# --------------------------------------------------------------------------------------
+# experimental inputs and result figures of the comparison / fit scripts (host side)
+# --------------------------------------------------------------------------------------
+def load_exp_image(img_path):
+    """The experimental q-map image.  ``.npy`` is read with numpy (no pickles); other
+    formats go through fabio, as the reference's ``fabio.open(img_path).data``
+    (simulate_GIWAXS.py:124, fit_slabsize.py:139)."""
+    if str(img_path).lower().endswith('.npy'):
+        return np.load(img_path, allow_pickle=False)
+    try:
+        import fabio
+    except ImportError as e:
+        raise ImportError(f"reading {img_path} needs fabio (not installed); save the image as .npy") from e
+    return fabio.open(img_path).data
+
+
+def mirror_qmap_positive_qxy_only(qmap, qxy, qz):
+    """tools/comparison.py:268-319: keep the qxy > 0 columns; where the mirrored
+    qxy < 0 column (zero-padded to the positive side's width) is brighter than 1.1x
+    the positive pixel, take it (detector gaps); put the qxy == 0 column first."""
+    qmap = np.asarray(qmap)
+    qxy = np.asarray(qxy)
+    keep = qxy > 0
+    q_pos = qxy[keep]
+    m_pos = qmap[:, keep]
+    m_neg = qmap[:, qxy < 0][:, ::-1]
+    if m_neg.shape[1] < m_pos.shape[1]:
+        m_neg = np.pad(m_neg, ((0, 0), (0, m_pos.shape[1] - m_neg.shape[1])), mode='constant', constant_values=0)
+    brighter = m_pos * 1.1 < m_neg
+    m_pos[brighter] = m_neg[brighter]
+    if np.any(qxy == 0):
+        zero_col = qmap[:, np.flatnonzero(qxy == 0)].reshape(-1, 1)
+        m_pos = np.concatenate((zero_col, m_pos), axis=1)
+        q_pos = np.concatenate(([0], q_pos))
+    return m_pos, q_pos, qz
+
+
+def evaluate_fit(best_params, fixed_slab_params, fixed_voxelgrid_params, fixed_detectormaker_params,
+                 fixed_exp_params):
+    """(rebin_map, sim_comp_map, diff_map) of the slab of size ``best_params``
+    (tools/comparison.py:884-912): the fit model's image, trimmed and shift_peak-ed on
+    the device, scaled by the least-squares scale / offset, masked pixels zeroed, and
+    the difference where the experimental map exceeds 1e-10."""
+    from ..fit_slabsize import model_for
+    model = model_for(fixed_slab_params, fixed_voxelgrid_params, fixed_detectormaker_params, fixed_exp_params)
+    rebin_map, rebin_mask = fixed_exp_params[0], fixed_exp_params[1]
+    sim_pad, scale, offset = model.padded(*best_params)
+    sim_comp_map = scale * sim_pad.cpu().numpy() + offset
+    sim_comp_map[rebin_mask == 1] = 0
+    diff_map = np.where(rebin_map > 1e-10, sim_comp_map - rebin_map, 0)
+    return rebin_map, sim_comp_map, diff_map
+
+
+def _pyplot():
+    import matplotlib
+    matplotlib.use('Agg')
+    import matplotlib.pyplot as plt
+    return matplotlib, plt
+
+
+def _q_axes(ax, max_q=None, fontsize=12):
+    from matplotlib import ticker
+    ax.set_xlabel('$\\mathregular{q_{xy}}$ ($\\AA^{-1}$)', fontsize=fontsize)
+    ax.set_ylabel('$\\mathregular{q_z}$ ($\\AA^{-1}$)', fontsize=fontsize)
+    ax.yaxis.set_major_locator(ticker.MultipleLocator(0.5))
+    ax.xaxis.set_major_locator(ticker.MultipleLocator(0.5))
+    if max_q is not None:
+        ax.set_xlim(0, max_q)
+        ax.set_ylim(0, max_q)
+
+
+def plot_fit(savepath, rebin_map, sim_comp_map, diff_map, det_h_trim, det_v_trim, max_q, suptitle):
+    """Simulated, experimental and difference maps side by side
+    (tools/comparison.py:914-975): turbo colour scale on the experimental 1st-99.9th
+    percentiles for both maps, symmetric bwr scale for the difference."""
+    matplotlib, plt = _pyplot()
+    extent = (np.min(det_h_trim), np.max(det_h_trim), np.min(det_v_trim), np.max(det_v_trim))
+    norm = matplotlib.colors.Normalize(vmin=np.percentile(rebin_map, 1), vmax=np.percentile(rebin_map, 99.9))
+    bound = max(abs(np.percentile(diff_map, 0.1)), abs(np.percentile(diff_map, 99.9)))
+    fig, (ax_sim, ax_exp, ax_diff) = plt.subplots(1, 3, figsize=(12, 6))
+    panels = ((ax_exp, rebin_map, norm, 'turbo', 'Experimental GIWAXS'),
+              (ax_sim, sim_comp_map, norm, 'turbo', 'Simulated GIWAXS'),
+              (ax_diff, diff_map, matplotlib.colors.Normalize(vmin=-bound, vmax=bound), 'bwr', 'Sim. - Exp.'))
+    for ax, img, nrm, cmap, title in panels:
+        im = ax.imshow(img, norm=nrm, extent=extent, cmap=cmap, origin='lower')
+        cb = fig.colorbar(im, ax=ax, shrink=0.45, aspect=25, pad=0.05)
+        if ax is ax_diff:
+            cb.set_label('(Percent)', fontsize=12)
+        _q_axes(ax, max_q)
+        ax.set_title(title, fontsize=12)
+    plt.suptitle(suptitle)
+    plt.tight_layout()
+    plt.savefig(savepath, dpi=300)
+    plt.close(fig)
+
+
+def make_linecut(polar_image, chi_axis, r_axis, lower_chi, upper_chi):
+    """tools/comparison.py:977-989: mean over the polar rows with lower < chi < upper
+    (pixels below 0.1 ignored), dropping q = 0 and empty columns."""
+    cut = polar_image[np.logical_and(chi_axis > lower_chi, chi_axis < upper_chi), :]
+    cut[cut < 0.1] = np.nan
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter('ignore', RuntimeWarning)
+        ints = np.nanmean(cut, axis=0)
+    qs = r_axis.copy()
+    assert len(ints) == len(qs), f'intensity array shape {np.shape(ints)} does not match q array shape {np.shape(qs)}'
+    keep = (qs != 0) & (ints == ints)
+    return qs[keep], ints[keep]
+
+
+def plot_linecuts(savepath, rebin_map, sim_comp_map, det_h_trim, det_v_trim, suptitle):
+    """In-plane (chi 0-30), out-of-plane (60-90) and full (0-90) radial line cuts of
+    the simulated and experimental maps (tools/comparison.py:991-1060), from their
+    polar warps (``linear_polar`` on the device) about the q = 0 pixel."""
+    from matplotlib import ticker
+    _, plt = _pyplot()
+    xx, yy = np.meshgrid(det_h_trim, det_v_trim)
+    r, chi = np.hypot(xx, yy), np.arctan2(yy, xx)
+    rows, cols = sim_comp_map.shape
+    radius = float(np.sqrt(rows ** 2 + cols ** 2))
+    o = (int(np.argmin(np.abs(det_v_trim))), int(np.argmin(np.abs(det_h_trim))))
+    warp = lambda img: linear_polar(np.ascontiguousarray(img, dtype=np.float64), o=o, r=radius)
+    polar_sim, polar_exp, polar_r, polar_chi = (warp(m) for m in (sim_comp_map, rebin_map, r, chi))
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter('ignore', RuntimeWarning)
+        chi_axis = np.nanmean(np.where(polar_chi == 0, np.nan, polar_chi), axis=1) * (180 / np.pi)
+        r_axis = np.nanmean(np.where(polar_r == 0, np.nan, polar_r), axis=0)
+    fig, axes = plt.subplots(1, 3, figsize=(12, 4))
+    for ax, (lo, hi), title in zip(axes, ((0, 30), (60, 90), (0, 90)),
+                                   ('IP comparions', 'OOP comparions', 'Full comparions')):
+        ax.plot(*make_linecut(polar_exp, chi_axis, r_axis, lo, hi), label='Experiment')
+        ax.plot(*make_linecut(polar_sim, chi_axis, r_axis, lo, hi), label='Simulation')
+        ax.set_xlabel('$\\mathregular{q_{r}}$ ($\\AA^{-1}$)', fontsize=12)
+        ax.set_ylabel('Intensity (a.u.)', fontsize=12)
+        ax.xaxis.set_major_locator(ticker.MultipleLocator(0.5))
+        ax.set_title(title, fontsize=12)
+        ax.legend(prop={'size': 12})
+    plt.suptitle(suptitle)
+    plt.tight_layout()
+    plt.savefig(savepath, dpi=300)
+    plt.close(fig)

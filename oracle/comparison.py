@@ -16,7 +16,8 @@ calls the reference makes (scipy.ndimage.map_coordinates, scipy.interpolate.grid
 * ``fit_residual``     the residual of fullfit_model, fit_slabsize.py:19-47
   (after the detector image: trim, shift_peak, scale / offset, masked residual)
 * ``slabmaker``        tools/comparison.py:595-671 after the file load and the lattice
-  vectors: replicate, bounding box, shift, centred crop, shift
+  vectors: replicate, bounding box, shift, centred crop, shift (without the last shift:
+  the standalone slabmaker.py:33-116)
 """
 
 from __future__ import annotations
@@ -144,10 +145,12 @@ def fit_residual(det_sum, det_h, det_v, rebin_map, rebin_mask, exp_qxy, exp_qz, 
     return ((rebin_map - (s * sim_pad + b)) * (rebin_mask == 0)).ravel(), (s, b)
 
 
-def slabmaker(coords, elements, x_size, y_size, z_size, a_vect, b_vect, c_vect):
+def slabmaker(coords, elements, x_size, y_size, z_size, a_vect, b_vect, c_vect, shift_again=True):
     """tools/comparison.py:608-671: copies 1..num of the running set appended along a,
     then b, then c (each copy = set + (i + 1) * vector), the bounding box, the shift to
-    the origin, the centred box of the requested size and the final shift."""
+    the origin, the centred box of the requested size and the final shift.  With
+    ``shift_again=False``, the standalone slabmaker.py:33-116 (the same replication,
+    whose appended element blocks equal the running set's, and no final shift)."""
     num_x = int(np.ceil(2 * x_size / a_vect[0]))
     num_y = int(np.ceil(2 * y_size / b_vect[1]))
     num_z = int(np.ceil(2 * z_size / c_vect[2]))
@@ -170,6 +173,7 @@ def slabmaker(coords, elements, x_size, y_size, z_size, a_vect, b_vect, c_vect):
     keep = ((coords[:, 0] >= xb) & (coords[:, 0] <= x_max - xb) & (coords[:, 1] >= yb) &
             (coords[:, 1] <= y_max - yb) & (coords[:, 2] >= zb) & (coords[:, 2] <= z_max - zb))
     out, out_e = coords[keep], elements[keep]
-    for k in range(3):
-        out[:, k] -= np.min(out[:, k])
+    if shift_again:
+        for k in range(3):
+            out[:, k] -= np.min(out[:, k])
     return out, out_e

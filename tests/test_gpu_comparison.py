@@ -6,8 +6,8 @@ calls), on the same seeded inputs:
   positions including exact edges and points just outside: max |diff| <= 1e-14 max|img|;
 * linear_polar, polar_linear, add_pad, shift_peak (drop-in, numpy in and out) on a
   detector-like image with exact zeros: <= 1e-12 of max|image|; add_pad exact;
-* rebin_qmap_and_mask vs griddata: same NaN positions, values <= 1e-12 relative,
-  identical masks;
+* rebin_qmap_and_mask vs griddata: same NaN positions, values and masks bit-exact
+  (griddata's barycentric arithmetic, FMA-free kernel);
 * FitComparison.residual (trim, shift_peak, scale / offset, masked residual, all on
   the device) vs the oracle's fit_residual: <= 1e-10 of max|residual|, scale and
   offset to 1e-10;
@@ -136,7 +136,7 @@ def test_rebin_and_fit_residual_match_oracle():
     e_res = np.max(np.abs(res_gpu - res_ref)) / np.max(np.abs(res_ref))
     _say(f"  rebin {e_rb:.2e}; residual {e_res:.2e}; scale {s_gpu:.9e} vs {s_ref:.9e}; "
          f"offset {b_gpu:.9e} vs {b_ref:.9e}")
-    assert e_rb <= 1e-12
+    assert e_rb == 0.0
     assert e_res <= 1e-10
     assert abs(s_gpu - s_ref) <= 1e-10 * abs(s_ref) and abs(b_gpu - b_ref) <= 1e-10 * abs(b_ref)
 

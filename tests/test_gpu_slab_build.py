@@ -9,7 +9,9 @@ ragged sizes; the PM6 polymer cell (CRYST1 44.456 45.726 40.097, 90 90 90) at th
 reference's own PM6 configuration 176 x 43 x 186 A (1.14 M replicated atoms) and a
 smaller box; one builder reused across sizes (the fit loop's pattern, smaller after
 larger); and a box that keeps no atom, which raises ValueError as the reference's
-np.min of the empty selection does.
+np.min of the empty selection does.  The standalone slab maker (giwaxsim_amd.slabmaker,
+reference slabmaker.py: no final shift, an .xyz file written) gives the same file
+text as the oracle's atoms through write_xyz, and an empty file for an empty box.
 """
 
 import os
@@ -34,11 +36,11 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _oracle(cell, size):
+def _oracle(cell, size, shift_again=True):
     path = cell[0]
     coords, el = load_xyz(path) if path.endswith('.xyz') else load_pdb(path)
     a, b, c = calc_real_space_abc(*cell[1:])
-    return C.slabmaker(coords, el, *size, a, b, c)
+    return C.slabmaker(coords, el, *size, a, b, c, shift_again=shift_again)
 
 
 def _device(builder, size):
@@ -76,3 +78,22 @@ def test_empty_box_raises_like_reference():
         _oracle(GRAPHITE, (1.0, 1.0, 1.0))
     with pytest.raises(ValueError):
         G.SlabBuilder(*GRAPHITE).build(1.0, 1.0, 1.0)
+
+
+@pytest.mark.parametrize('cell,size', [(PM6, (176.0, 43.0, 186.0)), (GRAPHITE, (1.0, 1.0, 1.0))],
+                         ids=['pm6', 'graphite_empty'])
+def test_standalone_slabmaker_file(tmp_path, cell, size):
+    from giwaxsim_amd import slabmaker
+    from giwaxsim_amd.tools.utilities import write_xyz
+    _say(f"[gpu-slab] standalone slabmaker {os.path.basename(cell[0])} size {size}")
+    keys = ('a', 'b', 'c', 'alpha', 'beta', 'gamma')
+    config = {'input_filepath': cell[0], 'output_folder': str(tmp_path / 'out'), 'gen_name': 'cut',
+              'x_size': str(size[0]), 'y_size': str(size[1]), 'z_size': str(size[2])}
+    config.update({k: str(v) for k, v in zip(keys, cell[1:])})
+    path = slabmaker.main(config)
+    assert os.path.basename(path) == f'cut_rect_cut{int(size[0])}x{int(size[1])}x{int(size[2])}.xyz'
+    want, want_el = _oracle(cell, size, shift_again=False)
+    ref = tmp_path / 'ref.xyz'
+    write_xyz(str(ref), want, want_el)
+    assert open(path).read() == ref.read_text()
+    _say(f"  {len(want)} atoms written")

@@ -125,6 +125,10 @@ def test_slabmaker_cubic_lattice_by_hand():
                     want.append((kx + 0.5 - 2, ky + 0.5 - 2.5, kz - 1))
                     want_el.append('O')
     assert np.array_equal(out, np.array(want)) and list(el) == want_el
+    # without the final shift (standalone slabmaker.py): the first shift's minima are
+    # (0, 0, 0), so the kept atoms sit at their lattice positions
+    cut, el2 = C.slabmaker(cell, np.array(['C', 'O']), 2.3, 3.7, 1.2, eye[0], eye[1], eye[2], shift_again=False)
+    assert np.array_equal(cut, np.array(want) + np.array([2.0, 2.5, 1.0])) and list(el2) == want_el
 
 
 def test_slabmaker_hexagonal_cell_brute_force():
