@@ -543,16 +543,21 @@ class Bins:
     def clear(self):
         call("gx_bins_clear", ptr(self.sum), ptr(self.cnt), self.grid.c_n ** 3, stream_handle())
 
-    def finalize(self, want64=True, want32=True):
-        """(iq64, iq32) device tensors [c, c, c] indexed [qy, qx, qz]."""
+    def finalize(self, want64=True, want32=True, f0_element='C'):
+        """(iq64, iq32) device tensors [c, c, c] indexed [qy, qx, qz], times
+        (f0(|q|) / Z)^2 of ``f0_element`` (None: no f0 factor; the coefficients
+        (0, ..., 0, c = 1) with Z = 1 make the factor exactly 1)."""
         g = self.grid
         n = g.c_n ** 3
         iq64 = _empty(n, torch.float64) if want64 else None
         iq32 = _empty(n, torch.float32) if want32 else None
         start, step, stop = g.linspace_params
-        aff = _lib.dbl_array(aff_dict['C'])
+        if f0_element is None:
+            aff, z = _lib.dbl_array((0.0,) * 8 + (1.0,)), 1.0
+        else:
+            aff, z = _lib.dbl_array(aff_dict[f0_element]), float(ptable[f0_element])
         call("gx_bins_finalize", ptr(self.sum), ptr(self.cnt), g.c_n, self.scale, g.c_lo, g.q_num,
-             start, step, stop, aff, float(ptable['C']), ptr(iq64), ptr(iq32), stream_handle())
+             start, step, stop, aff, z, ptr(iq64), ptr(iq32), stream_handle())
         shp = (g.c_n,) * 3
         return (iq64.view(shp) if iq64 is not None else None,
                 iq32.view(shp) if iq32 is not None else None)
@@ -654,7 +659,7 @@ _WORKSPACES_MAX = 4
 
 def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, fill_bkg=False,
                   smooth=0, f_table=None, elem_ids=None, batch=None, want64=True, frames=None,
-                  precision='auto', variant=None):
+                  precision='auto', variant=None, f0_element='C'):
     """Full voxel-grid path of one slab (``coords``: host (n, 3), device (n, 3) or a tuple
     of device x, y, z columns).  Returns dict with device iq32/iq64,
     host axes and the objects that produced them.  ``frames=(lo, hi)``
@@ -664,7 +669,8 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
     (geometry plan, atom and element counts, options, device, stream) and reused
     by the next call with the same key; iq64 / iq32 are new tensors every call,
     while the returned 'slab', 'bins' and 'runner' objects are overwritten by
-    that next call."""
+    that next call.  ``f0_element``: the element of the (f0/Z)^2 factor applied at the
+    end (the reference's 'C'), None for none."""
     if f_table is None:
         elem_ids, f_table, counts, uniq = element_table(elements, energy)
     elif not torch.is_tensor(elem_ids):
@@ -716,7 +722,7 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
     runner.run(frames)
     # the finalize is enqueued before the status read-back, so the device does not
     # wait on the host between the frames and the finalize
-    iq64, iq32 = bins.finalize(want64=want64, want32=True)
+    iq64, iq32 = bins.finalize(want64=want64, want32=True, f0_element=f0_element)
     runner.check()
     ax = plan.grid.crop_axis
     return dict(iq64=iq64, iq32=iq32, qx=ax.copy(), qy=ax.copy(), qz=ax.copy(), plan=plan,
@@ -812,9 +818,11 @@ def detector_plan(num_pixels, max_q, angle_init_vals, angle_init_axs, psis, psi_
                         orient_w=np.array(ws, dtype=np.float64))
 
 
-def run_detector(iq_dev, qx, qy, qz, plan: DetectorPlan, mirror):
+def run_detector(iq_dev, qx, qy, qz, plan: DetectorPlan, mirror, floor_scale=True):
     """Orientation sum + finalize on the device.  ``iq_dev``: float32 or float64
-    device tensor [cy, cx, cz].  Returns the FP64 device image [n, n]."""
+    device tensor [cy, cx, cz].  Returns the FP64 device image [n, n] (mirrored when
+    ``mirror``; with ``floor_scale`` NaN and <= 0 floored to 1e-6 and times 1e-6) and
+    the accumulated image."""
     dev = device()
     n = plan.n_pix
     h = torch.as_tensor(plan.h).to(dev)
@@ -835,7 +843,7 @@ def run_detector(iq_dev, qx, qy, qz, plan: DetectorPlan, mirror):
     st = stream_handle()
     call("gx_detector_accumulate", ctypes.byref(d), st)
     out = _empty(n * n, torch.float64)
-    flags = _lib.GX_FIN_FLOOR_SCALE | (_lib.GX_FIN_MIRROR if mirror else 0)
+    flags = (_lib.GX_FIN_FLOOR_SCALE if floor_scale else 0) | (_lib.GX_FIN_MIRROR if mirror else 0)
     call("gx_detector_finalize", ptr(det), n, n, flags, ptr(out), st)
     return out.view(n, n), det.view(n, n)
 

@@ -156,6 +156,16 @@ def detectormaker_fitting(iq, qx, qy, qz, num_pixels, max_q, angle_init_vals, an
     Orientations run in (psi outer, phi, theta inner) order and are summed in
     that order in FP64 on the GPU (deterministic; the reference's threaded
     ``+=`` is racy).  Returns (det_sum [v, h] FP64, det_h, det_v)."""
+    out, h, v = detector_image(iq, qx, qy, qz, num_pixels, max_q, angle_init_vals, angle_init_axs, psis,
+                               psi_weights_path, phis, phi_weights_path, thetas, theta_weights_path, mirror)
+    return out.cpu().numpy(), h, v
+
+
+def detector_image(iq, qx, qy, qz, num_pixels, max_q, angle_init_vals, angle_init_axs, psis, psi_weights_path,
+                   phis, phi_weights_path, thetas, theta_weights_path, mirror, floor_scale=True):
+    """The device image of detectormaker_fitting (``floor_scale``: NaN / <= 0 -> 1e-6,
+    then * 1e-6) or, without ``floor_scale``, the mirrored sum the legacy
+    detectormaker floors on its own.  Returns (device image [v, h], det_h, det_v)."""
     import torch
     psi_w = _weights(psi_weights_path, psis)
     phi_w = _weights(phi_weights_path, phis)
@@ -171,8 +181,8 @@ def detectormaker_fitting(iq, qx, qy, qz, num_pixels, max_q, angle_init_vals, an
     dev_iq = getattr(iq, '_gx_device', None)
     if dev_iq is None or tuple(dev_iq.shape) != tuple(np.shape(iq)):
         dev_iq = engine.to_dev(np.asarray(iq, dtype=np.float64), torch.float64)
-    out, _ = engine.run_detector(dev_iq, qx, qy, qz, plan, mirror)
-    return out.cpu().numpy(), plan.h, plan.v
+    out, _ = engine.run_detector(dev_iq, qx, qy, qz, plan, mirror, floor_scale=floor_scale)
+    return out, plan.h, plan.v
 
 
 def optimize_scale_offset(sim_map, rebin_map, rebin_mask):
