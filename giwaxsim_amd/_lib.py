@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_void_p
 from .build import LIB
 
 GX_OK = 0
-GX_ABI_VERSION = 4   # structure layouts below match this version of the header
+GX_ABI_VERSION = 5   # structure layouts below match this version of the header
 GX_FP_STRIDE = 24
 GX_MOMENTS_WORK = 774
 GX_FIN_MIRROR = 1
@@ -129,6 +129,7 @@ SIGNATURES = {
     "gx_adj_rowgrad": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
     "gx_adj_reduce": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
     "gx_adj_backward": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
+    "gx_adj_backward_stored": [POINTER(GxFrames), POINTER(GxAdjoint), _I, _I, _P],
     "gx_adj_finish": [POINTER(GxFrames), POINTER(GxAdjoint), _P, _P, _P],
     "gx_bins_adjoint": [_P, _I, _P, _P, _I, _I, _I, _D, _D, _D, POINTER(c_double), _D, _P, _P],
     "gx_detector_adjoint": [POINTER(GxDetector), _P, _I, _P, _P, _I, _P, _P],

@@ -10,8 +10,9 @@ the forward path's (tools/voxelgrids.py:381-503, tools/comparison.py:765-870).
 ``DifferentiableSlab`` maps (atom coordinates, per-element complex f) to the
 finalized detector image through ``torch.autograd.Function``; its backward is
 the HIP adjoint (include/giwaxsim_amd.h, gx_adj_* / gx_bins_adjoint /
-gx_detector_adjoint), which recomputes every phi-frame's transform instead of
-storing it.  Gradients: dL/d coords [n, 3], dL/d f_re and dL/d f_im per
+gx_detector_adjoint), which recomputes the phi-frames' row transforms instead of
+storing them all; the batch the forward left in its frame buffer (all frames, when
+they fit in one batch) is reused rather than recomputed.  Gradients: dL/d coords [n, 3], dL/d f_re and dL/d f_im per
 element.  A scale / flat background on the image is ordinary torch algebra
 (``image_loss``).
 
@@ -249,9 +250,14 @@ class DifferentiableSlab:
         esz = 16 if self.f64 else 8
         B = self._batch_arg
         if B is None:
-            cap_inter = 512e6 // (W * Zc * esz)
-            cap_part = 2e9 // (4 * esz // 2 * max(n, 1))
-            B = max(1, min(nf, int(cap_inter), int(cap_part), 8192))
+            # frame buffer + per-atom partial gradients of a batch; up to a quarter of
+            # the free device memory (at most 64 GB, at least 2.5 GB), so that a slab's
+            # frames fit in as few batches as possible (the backward reuses the
+            # forward's last batch instead of recomputing it)
+            per_frame = W * Zc * esz + 4 * (esz // 2) * max(n, 1)
+            free, _ = torch.cuda.mem_get_info()
+            budget = max(2.5e9, min(0.25 * free, 64e9))
+            B = max(1, min(nf, int(budget // per_frame), 8192))
         if getattr(self, 'batch', None) not in (None, B):
             raise RuntimeError("internal: batch size changed")
         self.batch = int(B)
@@ -376,6 +382,7 @@ class DifferentiableSlab:
         self.finalize_image(st)
         self._check_status()
         self._prepared = (fr, adj)
+        self._inter_fr = fr
         self._generation += 1
         return self.det_out.view(self.n_pix, self.n_pix).clone()
 
@@ -415,8 +422,13 @@ class DifferentiableSlab:
         call("gx_bins_adjoint", ptr(self.g_fx), self.n_rep, ptr(self.fx_scale), ptr(self.bin_cnt), g.c_n, g.c_lo,
              g.q_num, start, step, stop, aff, float(ptable['C']), ptr(self.g_bins), st)
         call("gx_adj_prepare", ctypes.byref(fr), ctypes.byref(adj), st)
-        for f0, nf, d in self.batches(fr):
-            call("gx_adj_backward", ctypes.byref(d), ctypes.byref(adj), f0, nf, st)
+        # last batch first: the frame buffer still holds its row pass when the forward
+        # of these descriptors was the last thing enqueued on this model
+        stored = getattr(self, '_inter_fr', None) is fr
+        self._inter_fr = None
+        for i, (f0, nf, d) in enumerate(reversed(self.batches(fr))):
+            name = "gx_adj_backward_stored" if (stored and i == 0) else "gx_adj_backward"
+            call(name, ctypes.byref(d), ctypes.byref(adj), f0, nf, st)
         call("gx_adj_finish", ctypes.byref(fr), ctypes.byref(adj), ptr(self.g_coords), ptr(self.g_f), st)
 
     def forward_enqueue(self, fr, adj, st):
@@ -427,6 +439,7 @@ class DifferentiableSlab:
         for f0, nf, d in self.batches(fr):
             call("gx_adj_forward", ctypes.byref(d), ctypes.byref(adj), f0, nf, st)
         self.finalize_image(st)
+        self._inter_fr = fr
 
 
 def image_loss(image, target, scale, background):

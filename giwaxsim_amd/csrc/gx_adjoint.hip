@@ -156,8 +156,9 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
   constexpr int R0 = FftFirst<N>::R;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   AU *accs = reinterpret_cast<AU *>(smem_raw);                 // [RB][2][N]
-  C *stage = reinterpret_cast<C *>(accs + (size_t)RB * 2 * N);  // [W][RB]
-  C *tspec = stage + d.W * RB;                                  // [W] background spectrum
+  constexpr int SP = RB + 1;                                    // staging row stride (odd)
+  C *stage = reinterpret_cast<C *>(accs + (size_t)RB * 2 * N);  // [W][SP]
+  C *tspec = stage + d.W * SP;                                  // [W] background spectrum
   double *ftab = reinterpret_cast<double *>(tspec + d.W);       // [n_elem][2]
   __shared__ double rsc[RB];
   const int lane = threadIdx.x & (GX_WAVE - 1);
@@ -256,13 +257,13 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
       for (int q = 0; q < P; ++q) x[q] = C{(T)0, (T)0};
     }
     const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + v] : 1.0;
-    stage_row_band<N, C>(x, lane, jlo, wf, inside, d, mz, tspec, stage, RB, rb);
+    stage_row_band<N, C>(x, lane, jlo, wf, inside, d, mz, tspec, stage, SP, rb);
   }
   __syncthreads();
   C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * d.Z;
   for (int idx = threadIdx.x; idx < wf * RB; idx += BLK) {
     const int w = idx / RB, q = idx - w * RB;
-    if (z0 + q < d.Z) out[(int64_t)w * d.Z + z0 + q] = stage[idx];
+    if (z0 + q < d.Z) out[(int64_t)w * d.Z + z0 + q] = stage[w * SP + q];
   }
 }
 
@@ -825,7 +826,7 @@ __global__ __launch_bounds__(BLK) void k_bins_adjoint(const unsigned long long *
 template <int N, typename C>
 int launch_adj_rowpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int RB = RowPassShape<N, C>::RB;
-  const size_t lds = (size_t)RB * N * sizeof(C) + (size_t)d->W * (RB + 1) * sizeof(C) + (size_t)g->n_elem * 16;
+  const size_t lds = (size_t)RB * N * sizeof(C) + (size_t)d->W * (RB + 2) * sizeof(C) + (size_t)g->n_elem * 16;
   if (lds > 150 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "differentiable row pass needs %zu bytes of LDS (N=%d, W=%lld)", lds, N,
                      (long long)d->W);
@@ -1007,7 +1008,8 @@ int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t n
   return check_launch("gx_adj_reduce");
 }
 
-int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream) {
+static int adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream,
+                        bool recompute) {
   clear_error();
   int rc = check_adj(d, g, f0, nf, true);
   if (rc || nf == 0) return rc;
@@ -1017,7 +1019,7 @@ int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t
     if ((rc = gx_frames_columns(d, f0, nf, stream))) return rc;
     if ((rc = dispatch(OP_AVGSPEC, d, g, f0, nf, st))) return rc;
   }
-  if ((rc = dispatch(OP_ROWPASS, d, g, f0, nf, st))) return rc;
+  if (recompute && (rc = dispatch(OP_ROWPASS, d, g, f0, nf, st))) return rc;
   if ((rc = dispatch(OP_COLPASS, d, g, f0, nf, st))) return rc;
   if (dense) {
     k_adj_avg_frames<<<(unsigned)nf, BLK, 0, st>>>(*d, *g, f0);
@@ -1027,6 +1029,14 @@ int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t
   // k_adj_colpass has reset the frames' argmin slots
   if ((rc = dispatch(OP_ROWGRAD, d, g, f0, nf, st))) return rc;
   return gx_adj_reduce(d, g, f0, nf, stream);
+}
+
+int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream) {
+  return adj_backward(d, g, f0, nf, stream, true);
+}
+
+int gx_adj_backward_stored(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream) {
+  return adj_backward(d, g, f0, nf, stream, false);
 }
 
 int gx_adj_finish(const gx_frames *d, const gx_adjoint *g, double *g_coords, double *g_f, gx_stream_t stream) {

@@ -366,19 +366,22 @@ __device__ __forceinline__ double collapse_length(const double *fp, double x) {
 // differentiable variant (same arithmetic in both, by construction).
 //
 // stage_row_band: the kept fftshifted column band [jlo, jlo + wf) of one row
-// transform v (last-stage register layout) into stage[w * RB + rb].  A row
-// outside the filled z range is the constant A = avg_voxel_f * max_voxels
-// (only k = 0 is non-zero); inside, with fill or smoothing, the frame's
-// background spectrum U is added: o = mz (v + U) (+ A N at k = 0 when smoothing).
+// transform v (last-stage register layout) into stage[w * SP + rb] (SP, the tile's
+// row stride, is odd so that the lanes' stores spread over the LDS banks).  A row
+// outside the filled z range is the constant A = avg_voxel_f * max_voxels (only
+// k = 0 is non-zero); inside, with fill or smoothing, the frame's background
+// spectrum U is added: o = mz (v + U) (+ A N at k = 0 when smoothing), in the
+// transform's precision.
 // ------------------------------------------------------------------------------------
 template <int N, typename C>
 __device__ __forceinline__ void stage_row_band(const C (&v)[N / GX_WAVE], int lane, int jlo, int wf, bool inside,
-                                               const gx_frames &d, double mz, const C *tspec, C *stage, int RB,
+                                               const gx_frames &d, double mz, const C *tspec, C *stage, int SP,
                                                int rb) {
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE, RL = FftLast<N>::R;
   const bool dense = d.fill_bkg || d.smooth;
-  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
+  const T mzt = (T)mz;
+  const C a0 = C{(T)(d.avg_re * d.max_voxels * (double)N), (T)(d.avg_im * d.max_voxels * (double)N)};
 #pragma unroll
   for (int t = 0; t < P / RL; ++t) {
 #pragma unroll
@@ -388,18 +391,18 @@ __device__ __forceinline__ void stage_row_band(const C (&v)[N / GX_WAVE], int la
       if (w < 0 || w >= wf) continue;
       C o = v[t * RL + q];
       if (!inside) {
-        o = (k == 0) ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
+        o = (k == 0) ? a0 : C{(T)0, (T)0};
       } else if (dense) {
         const C u = tspec[w];
-        double ore = mz * ((double)o.x + (double)u.x);
-        double oim = mz * ((double)o.y + (double)u.y);
+        T ore = mzt * (o.x + u.x);
+        T oim = mzt * (o.y + u.y);
         if (d.smooth && k == 0) {
-          ore = ore + avv_re * (double)N;
-          oim = oim + avv_im * (double)N;
+          ore = ore + a0.x;
+          oim = oim + a0.y;
         }
-        o = C{(T)ore, (T)oim};
+        o = C{ore, oim};
       }
-      stage[w * RB + rb] = o;
+      stage[w * SP + rb] = o;
     }
   }
 }

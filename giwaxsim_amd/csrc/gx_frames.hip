@@ -678,8 +678,9 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
   C *wbuf = smem + wave * N;
-  C *stage = smem + (BLK / GX_WAVE) * N;  // [W][RB]
-  C *tspec = stage + d.W * RB;            // [W] background spectrum of this frame
+  constexpr int SP = RB + 1;               // row stride of the staging tile (odd)
+  C *stage = smem + (BLK / GX_WAVE) * N;  // [W][SP]
+  C *tspec = stage + d.W * SP;            // [W] background spectrum of this frame
   int32_t *acc_re = reinterpret_cast<int32_t *>(wbuf);
   int32_t *acc_im = acc_re + N;
 
@@ -789,7 +790,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_
     }
     // 5) kept columns [jlo, jhi]: add the background spectrum, stage
     const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + z] : 1.0;
-    stage_row_band<N, C>(v, lane, jlo, wf, inside, d, mz, tspec, stage, RB, rb);
+    stage_row_band<N, C>(v, lane, jlo, wf, inside, d, mz, tspec, stage, SP, rb);
     wave_sync();
   }
   __syncthreads();
@@ -797,7 +798,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_
   C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * d.Z;
   for (int idx = threadIdx.x; idx < wf * RB; idx += BLK) {
     const int w = idx / RB, q = idx - w * RB;
-    out[(int64_t)w * d.Z + z0 + q] = stage[idx];
+    out[(int64_t)w * d.Z + z0 + q] = stage[w * SP + q];
   }
 }
 
@@ -814,8 +815,9 @@ __global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0, in
   constexpr int R0 = FftFirst<N>::R;
   constexpr int ROWB = (N * (int)sizeof(C) > N * 8) ? N * (int)sizeof(C) : N * 8;  // bytes per row
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C *stage = reinterpret_cast<C *>(smem_raw + (size_t)RB * ROWB);  // [W][RB]
-  C *tspec = stage + d.W * RB;                                      // [W]
+  constexpr int SP = RB + 1;                                        // staging row stride (odd)
+  C *stage = reinterpret_cast<C *>(smem_raw + (size_t)RB * ROWB);  // [W][SP]
+  C *tspec = stage + d.W * SP;                                      // [W]
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
 
@@ -905,13 +907,13 @@ __global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0, in
       for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
     }
     const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + z] : 1.0;
-    stage_row_band<N, C>(v, lane, jlo, wf, inside, d, mz, tspec, stage, RB, rb);
+    stage_row_band<N, C>(v, lane, jlo, wf, inside, d, mz, tspec, stage, SP, rb);
   }
   __syncthreads();
   C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * d.Z;
   for (int idx = threadIdx.x; idx < wf * RB; idx += BLK) {
     const int w = idx / RB, q = idx - w * RB;
-    out[(int64_t)w * d.Z + z0 + q] = stage[idx];
+    out[(int64_t)w * d.Z + z0 + q] = stage[w * SP + q];
   }
 }
 
@@ -1306,7 +1308,7 @@ template <int N, typename C>
 int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int RPW = (N <= 1024) ? 2 : 1;
   constexpr int RB = (BLK / GX_WAVE) * RPW;
-  size_t lds = ((size_t)(BLK / GX_WAVE) * N + (size_t)d->W * (RB + 1)) * sizeof(C);
+  size_t lds = ((size_t)(BLK / GX_WAVE) * N + (size_t)d->W * (RB + 2)) * sizeof(C);
   if (lds > 160 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "row pass needs %zu bytes of LDS (N=%d, W=%lld, %s)", lds, N,
                      (long long)d->W, sizeof(C) == 16 ? "FP64" : "FP32");
@@ -1324,7 +1326,7 @@ int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st)
 template <int N, int RB, typename C>
 int launch_rowpass_blk_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int ROWB = (N * (int)sizeof(C) > N * 8) ? N * (int)sizeof(C) : N * 8;
-  const size_t lds = (size_t)RB * ROWB + (size_t)d->W * (RB + 1) * sizeof(C);
+  const size_t lds = (size_t)RB * ROWB + (size_t)d->W * (RB + 2) * sizeof(C);
   if (lds > 160 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "block-wide row pass needs %zu bytes of LDS (N=%d, RB=%d)", lds, N, RB);
   static bool attr_set = false;

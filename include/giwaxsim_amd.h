@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define GX_ABI_VERSION 4
+#define GX_ABI_VERSION 5
 
 #define GX_OK 0
 #define GX_ERR_ARG 1         /* invalid argument (message says which) */
@@ -418,6 +418,10 @@ int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t n
  * d->idx_bounds must hold the statistics of the same frames and atoms (gx_frames_stats,
  * run by gx_adj_forward). */
 int gx_adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
+/* gx_adj_backward without the row-pass recompute: d->inter must still hold the row
+ * pass that gx_adj_forward wrote for the same frames and atoms (nothing has run on
+ * d->inter since).  Saves one row pass when the caller keeps the forward's frames. */
+int gx_adj_backward_stored(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, gx_stream_t stream);
 /* Origin min z term, the avg_voxel_f terms (fill_bkg / smooth), per-element f
  * gradients and the coordinate gradients in the original atom order:
  * g_coords [n_atoms][3], g_f [n_elem][2]. */

@@ -12,6 +12,9 @@ Checks, through the C ABI (giwaxsim_amd.adjoint):
   (sum of the coordinate gradient over atoms ~ 0), the image is homogeneous of
   degree 2 in f (Euler: sum f dL/df = 2 L on unclipped pixels), and two
   backward passes are bitwise identical;
+* the backward that reuses the forward's last batch of row transforms
+  (gx_adj_backward_stored) gives bitwise the gradients of the one that recomputes
+  every batch, in one batch and in four;
 * the differentiation workload (graphite 40 x 40 x 30 A, N = 512, 453 frames,
   256^2 detector, psi 0..90 step 2, target with c x 1.02, fitted scale and
   background, taken 5% away from the least-squares optimum so that dL/ds and
@@ -116,6 +119,38 @@ def test_invariances_and_determinism(fill, smooth):
     lhs = float((model.f_re0 * gfr).sum() + (model.f_im0 * gfi).sum())
     rhs = 2.0 * float((img * G)[live].sum())
     assert abs(lhs - rhs) <= 1e-10 * float((img * G).abs().sum())
+
+
+@pytest.mark.parametrize("precision,batch", [('fp64', None), ('fp32', 3)])
+def test_stored_frames_backward_is_bitwise_recompute(precision, batch):
+    _say(f"[gpu-adjoint] stored vs recomputed row pass in the backward, {precision}, batch={batch}")
+    coords, el = graphite_slab(size=(16.0, 16.0, 12.0), seed=21)
+    el = el.copy()
+    el[::7] = 'N'
+    N = 256
+    qv = 2 * np.pi / (R * N) * (1 + 1e-9)
+    det = dict(num_pixels=48, max_q=MAXQ, psis=list(np.arange(0.0, 91.0, 10.0)), phis=[0.0],
+               thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
+    model = DifferentiableSlab(el, coords, R, qv, MAXQ, det, frames=(0, 10), precision=precision,
+                               batch=batch, fill_bkg=True, smooth=3)
+    dev = torch.device('cuda')
+    c = torch.tensor(coords, dtype=torch.float64, device=dev)
+    fr, adj = model.descriptors(c, model.f_re0, model.f_im0)
+    G = torch.as_tensor(np.random.default_rng(5).normal(size=48 * 48), device=dev)
+    from giwaxsim_amd.engine import stream_handle
+    st = stream_handle()
+    out = []
+    for stored in (True, False):
+        model.forward_enqueue(fr, adj, st)
+        if not stored:
+            model._inter_fr = None   # force the recompute of every batch
+        model.backward_enqueue(fr, adj, G, st)
+        torch.cuda.synchronize()
+        model._check_status()
+        out.append((model.g_coords.clone(), model.g_f.clone()))
+    assert len(model.batches(fr)) == (1 if batch is None else 4)
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+    assert float(out[0][0].abs().max()) > 0
 
 
 def _workload():
