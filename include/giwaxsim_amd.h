@@ -15,7 +15,9 @@
  *     buffer (sizes given by the comments), so calls are safe to enqueue
  *     into a caller's stream and graph.
  *   - `stream` is a hipStream_t (NULL = the null stream).  Work is only
- *     enqueued; nothing synchronises the host.
+ *     enqueued; nothing synchronises the host, except the entry points that
+ *     return a value in host memory (gx_scale_offset_moments, gx_slab_crop),
+ *     which wait for their own stream and say so.
  *   - Structures contain only 8-byte members (int64_t, double, pointers).
  */
 #ifndef GIWAXSIM_AMD_H
