@@ -260,6 +260,29 @@ def _time_adjoint(model, coords, engine, torch, reps):
     return ev[0].elapsed_time(ev[1]) / reps, ev[1].elapsed_time(ev[2]) / reps
 
 
+def _plain_forward_ms(engine, torch, coords, elements, r, qv, mq, det, reps, fill=False, smooth=0):
+    """Wall time (ms) of the reference-parity forward (nearest-grid-point frames, bins,
+    iq, detector image) of the same slab and detector, from device-resident atoms."""
+    ct = torch.as_tensor(coords).cuda()
+    ids, ftab, _, _ = engine.element_table(elements, 12700.0)
+    idd = torch.as_tensor(ids).cuda()
+    psis = np.asarray(det['psis'], dtype=np.float64)
+    dplan = engine.detector_plan(det['num_pixels'], mq, det['angle_init_vals'], det['angle_init_axs'], psis,
+                                 np.full(len(psis), 1.0 / len(psis)), [0.0], [1.0], [0.0], [1.0])
+
+    def ngp():
+        res = engine.run_voxelgrid(ct, None, r, qv, mq, 12700.0, fill_bkg=fill, smooth=smooth, f_table=ftab,
+                                   elem_ids=idd)
+        engine.run_detector(res['iq64'], res['qx'], res['qy'], res['qz'], dplan, det['mirror'])
+    ngp()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        ngp()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / reps * 1e3
+
+
 def adjoint_config(engine, torch, reps=10, precision='fp32'):
     """Differentiation workload: graphite 40 x 40 x 30 A (5.6k atoms), r 0.25 A,
     N = 512, 453 phi-frames, max_q 2.5, 256^2 detector with psi 0..90 step 2
@@ -278,24 +301,7 @@ def adjoint_config(engine, torch, reps=10, precision='fp32'):
                thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
     model = DifferentiableSlab(eg, cg, r, qv, mq, det, precision=precision)
     fwd, fb = _time_adjoint(model, cg, engine, torch, reps)
-    ct = torch.as_tensor(cg).cuda()
-    # the reference-parity (nearest-grid-point) forward of the same slab and detector
-    ids, ftab, _, _ = engine.element_table(eg, 12700.0)
-    idd = torch.as_tensor(ids).cuda()
-    dplan = engine.detector_plan(256, mq, (0.15,), ('psi',), np.arange(0.0, 91.0, 2.0),
-                                 np.full(46, 1.0 / 46), [0.0], [1.0], [0.0], [1.0])
-
-    def ngp():
-        res = engine.run_voxelgrid(ct, None, r, qv, mq, 12700.0, f_table=ftab, elem_ids=idd)
-        engine.run_detector(res['iq64'], res['qx'], res['qy'], res['qz'], dplan, True)
-        return res
-    res = ngp()
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for _ in range(reps):
-        ngp()
-    torch.cuda.synchronize()
-    ngp_ms = (time.perf_counter() - t) / reps * 1e3
+    ngp_ms = _plain_forward_ms(engine, torch, cg, eg, r, qv, mq, det, reps)
     F = model.plan.n_frames
     return {"workload": "graphite 40x40x30 A, 5.6k atoms, N=512, 453 phi-frames, 256^2 detector, "
                         f"psi 0..90 step 2; cloud-in-cell deposit, {precision.upper()} transforms; "
@@ -319,12 +325,15 @@ def adjoint_polymer_config(engine, torch, reps=3):
     model = DifferentiableSlab(W['elements'], W['coords'], W['r'], W['qv'], W['max_q'], det,
                                fill_bkg=W['fill'], smooth=W['smooth'])
     fwd, fb = _time_adjoint(model, W['coords'], engine, torch, reps)
+    ngp_ms = _plain_forward_ms(engine, torch, W['coords'], W['elements'], W['r'], W['qv'], W['max_q'], det,
+                               max(reps, 5), fill=W['fill'], smooth=W['smooth'])
     F = model.plan.n_frames
     return {"workload": "polymer slab 100^3 A, 110k atoms, N=1024, 581 phi-frames, fill_bkg, smooth 10, "
                         "512^2 detector, psi 0..180 step 0.5; cloud-in-cell deposit, FP32 transforms; "
                         "gradients w.r.t. all coordinates and per-element f",
             "n_atoms": len(W['coords']), "frames": F, "batch": model.batch, "fwd_ms": fwd,
             "fwd_bwd_ms": fb, "bwd_ms": fb - fwd, "fwd_bwd_over_fwd": fb / fwd,
+            "reference_parity_fwd_ms": ngp_ms, "fwd_bwd_over_reference_parity_fwd": fb / ngp_ms,
             "fwd_phi_frames_per_s": F / (fwd * 1e-3), "fwd_bwd_phi_frames_per_s": F / (fb * 1e-3)}
 
 
@@ -357,12 +366,16 @@ def adjoint_large_config(engine, torch, n_slabs=16):
         F = model.plan.n_frames
         peak = max(peak, torch.cuda.max_memory_allocated())
         del model
+        if i == 0:   # the reference-parity forward of the first slab and the same detector
+            ngp_ms = _plain_forward_ms(engine, torch, c, e, r, qv, mq, det, 2, fill=True, smooth=20)
+    fb_slab = fb_tot / n_slabs
     return {"workload": f"{n_slabs} polymer slabs 200x200x60 A (~1.3M atoms each), N=2048, fill_bkg, "
                         "smooth 20, max_q 3.0, 2048^2 detector psi step 0.25; forward + backward "
                         "w.r.t. coordinates, forward's last frame batch reused, others recomputed",
             "n_slabs": n_slabs, "n_atoms": n_atoms, "frames_per_slab": F,
             "fwd_ms_total": fwd_tot, "fwd_bwd_ms_total": fb_tot, "fwd_bwd_over_fwd": fb_tot / fwd_tot,
-            "fwd_bwd_ms_per_slab": fb_tot / n_slabs, "peak_hbm_gb": peak / 1e9}
+            "fwd_bwd_ms_per_slab": fb_slab, "reference_parity_fwd_ms_per_slab": ngp_ms,
+            "fwd_bwd_over_reference_parity_fwd": fb_slab / ngp_ms, "peak_hbm_gb": peak / 1e9}
 
 
 def extra_configs(args, engine, torch, which=None):
