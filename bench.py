@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble (64 slabs) and large-slab (N=2048) figures")
     ap.add_argument("--only", choices=["adjoint", "adjoint_fp64", "adjoint_polymer", "adjoint_large",
-                                       "ensemble", "large_slab", "compare", "slab_build"],
+                                       "ensemble", "large_slab", "compare", "slab_build", "fit_eval"],
                     help="run one of the extra workloads alone and print its JSON (for profiling)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="one step, then one 256 MiB device copy (counter calibration) and one "
@@ -394,11 +394,57 @@ def extra_configs(args, engine, torch, which=None):
         out["compare"] = compare_config(engine, torch)
     if which in (None, "slab_build"):
         out["slab_build"] = slab_build_config(engine, torch)
+    if which in (None, "fit_eval"):
+        out["fit_eval"] = fit_eval_config(engine, torch)
     if which in (None, "ensemble"):
         out["ensemble"] = ensemble_config(engine, torch, reps)
     if which in (None, "large_slab"):
         out["large_slab"] = large_slab_config(engine, torch, reps)
     return out
+
+
+def fit_eval_config(engine, torch, reps=5):
+    """One evaluation of the slab-size fit model (fit_slabsize.py fullfit_model) at the
+    reference's fit configuration (config_templates/fit_slabsize_config.txt geometry):
+    the PM6 cell tiled and cropped to 176 x 43 x 186 A, r 0.3 A, q_voxel 0.02 (N = 1048,
+    chirp-z transforms), max_q 2, fill_bkg, smooth 25, 250 px detector, psi 0..90 x 91,
+    phi 0..179 x 180 (16,380 orientations, psi outer), 90-degree psi / phi / psi initial
+    rotations, pad 0.12 in 1.05..1.55; a synthetic experimental map on 981 x 1043
+    (qxy, qz) grids of the reference's TC47 shape.  Slab build, voxel grid, detector and
+    comparison all on the device; only the residual vector returns."""
+    from giwaxsim_amd.fit_slabsize import SlabSizeModel
+    from giwaxsim_amd.tools.comparison import mirror_qmap_positive_qxy_only, rebin_qmap_and_mask, trim_sim_data
+    path = os.path.join(ROOT, "tests", "golden", "PM6_sample.pdb")
+    slab = (path, 44.456, 45.726, 40.097, 90.0, 90.0, 90.0)
+    vox = (0.3, 0.02, 2.0, 12700.0, True, 25)
+    psis = np.linspace(0, 90, 91)
+    phis = np.linspace(0, 179, 180)
+    thetas = np.linspace(0, 1, 1)
+    det = (250, (90.0, 90.0, 90.0), ('psi', 'phi', 'psi'), psis, None, phis, None, thetas, None)
+    exp_qxy = np.linspace(-1.304, 2.111, 981)
+    exp_qz = np.linspace(-0.0287, 2.508, 1043)
+    X, Z = np.meshgrid(exp_qxy, exp_qz)
+    exp_img = 1e3 * (1.5 + np.cos(7 * np.hypot(X, Z)))
+    e, q2, z2 = mirror_qmap_positive_qxy_only(exp_img, exp_qxy, exp_qz)
+    h = np.linspace(-2.0, 2.0, 250)
+    _, ht, vt = trim_sim_data(np.zeros((250, 250)), h, h, q2, z2)
+    mask = np.zeros(e.shape)
+    mask[:, :10] = 1
+    rebin_map, rebin_mask = rebin_qmap_and_mask(ht, vt, e, q2, z2, mask)
+    rebin_map[rebin_mask == 1] = 0
+    model = SlabSizeModel(slab, vox, det, (rebin_map, rebin_mask, q2, z2, 0.12, (1.05, 1.55)))
+    r, _ = model.residual(176.0, 43.0, 186.0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(reps):
+        r, (sc, off) = model.residual(176.0 + 0.5 * k, 43.0, 186.0)   # a new slab each evaluation
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    return {"workload": "fit_slabsize evaluation at the reference fit geometry: PM6 slab 176 x 43 x 186 A, "
+                        "N=1048 (chirp-z), fill_bkg, smooth 25, 250^2 detector x 16,380 orientations, "
+                        "synthetic 981 x 1043 experimental map; slab, voxel grid, detector, comparison on the "
+                        "device", "ms_per_eval": dt * 1e3, "n_pixels_residual": int(r.size),
+            "orientations": int(len(psis) * len(phis) * len(thetas))}
 
 
 def compare_config(engine, torch, reps=20, cpu_reps=3):

@@ -283,7 +283,19 @@ def row_range(N, r, z_bound):
     return int(min(N, max(8, -(-zr // 8) * 8)))
 
 
-def build_frame_plan(N, r, g: QGrid, x_bound, y_bound, z_bound=None) -> FramePlan:
+_FRAME_GEOM: dict = {}
+_FRAME_GEOM_MAX = 16
+
+
+def _frame_geometry(N, r, g: QGrid):
+    """The slab-independent part of the frame records for grid edge N, voxel r and q
+    grid g: per frame sin / cos, the horizontal q-axis linspaces and the kept column
+    band, plus the kept detector rows.  Kept per (N, r, g) so that a fit loop, whose
+    slab bounds change every evaluation, only recomputes the collapse records."""
+    key = (int(N), float(r), id(g))
+    hit = _FRAME_GEOM.get(key)
+    if hit is not None and hit[0] is g:
+        return hit[1]
     q_h = np.fft.fftfreq(N, d=r) * 2 * np.pi
     q_h_shifted = np.fft.fftshift(q_h)
     q_v_shifted = np.fft.fftshift(np.fft.fftfreq(N, d=r) * 2 * np.pi)
@@ -297,7 +309,6 @@ def build_frame_plan(N, r, g: QGrid, x_bound, y_bound, z_bound=None) -> FramePla
         raise RuntimeError("kept detector rows are not one contiguous band")
     F = len(g.phis)
     fp = np.zeros((F, _lib.GX_FP_STRIDE), dtype=np.float64)
-    max_voxels = np.sqrt(np.float64(x_bound) ** 2 + np.float64(y_bound) ** 2) // r
     widths = np.zeros(F, dtype=np.int64)
     for f, phi in enumerate(g.phis):
         phi_rad = np.radians(phi)
@@ -309,8 +320,6 @@ def build_frame_plan(N, r, g: QGrid, x_bound, y_bound, z_bound=None) -> FramePla
         left_qx = -right_qx
         fp[f, _lib.FP_HX_START:_lib.FP_HX_STOP + 1] = _linspace_params(left_qx, right_qx, N)
         fp[f, _lib.FP_HY_START:_lib.FP_HY_STOP + 1] = _linspace_params(left_qy, right_qy, N)
-        rec = collapse_record(phi, y_bound, x_bound)
-        fp[f, _lib.FP_LEN_MODE:_lib.FP_CT + 1] = rec
         # kept columns (tools/voxelgrids.py:475, 497-498) inside the crop
         hqx = np.linspace(left_qx, right_qx, num=N)
         hqy = np.linspace(left_qy, right_qy, num=N)
@@ -327,8 +336,22 @@ def build_frame_plan(N, r, g: QGrid, x_bound, y_bound, z_bound=None) -> FramePla
             widths[f] = len(js)
         else:
             fp[f, _lib.FP_JLO], fp[f, _lib.FP_JHI] = 0, -1
+    fp.setflags(write=False)
+    value = (fp, int(max(1, widths.max())), int(rows[0]), int(rows[-1]))
+    if len(_FRAME_GEOM) >= _FRAME_GEOM_MAX:
+        _FRAME_GEOM.pop(next(iter(_FRAME_GEOM)))
+    _FRAME_GEOM[key] = (g, value)
+    return value
+
+
+def build_frame_plan(N, r, g: QGrid, x_bound, y_bound, z_bound=None) -> FramePlan:
+    fp0, W, i_lo, i_hi = _frame_geometry(N, r, g)
+    fp = fp0.copy()
+    max_voxels = np.sqrt(np.float64(x_bound) ** 2 + np.float64(y_bound) ** 2) // r
+    for f, phi in enumerate(g.phis):
+        fp[f, _lib.FP_LEN_MODE:_lib.FP_CT + 1] = collapse_record(phi, y_bound, x_bound)
     return FramePlan(N=N, r=r, grid=g, x_bound=float(x_bound), y_bound=float(y_bound), fparams=fp,
-                     W=int(max(1, widths.max())), i_lo=int(rows[0]), i_hi=int(rows[-1]),
+                     W=W, i_lo=i_lo, i_hi=i_hi,
                      q_val=1.0 / (N * r), max_voxels=float(max_voxels), Z=row_range(N, r, z_bound),
                      z_bound=None if z_bound is None else float(z_bound))
 
