@@ -121,6 +121,52 @@ __global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
   d.det[p] = acc;
 }
 
+// The same sum for detectors with few pixels and many orientations (a fit's 250^2 x 16k):
+// one pixel per lane of 64, COOP_NW waves per block.  The waves gather chunks of
+// COOP_K orientations each into LDS (v * w, the same product), then wave 0 adds the
+// chunk in orientation order, so the result is bitwise the one-thread-per-pixel sum
+// while COOP_NW times more gathers are in flight per pixel.
+constexpr int COOP_NW = 8;                  // waves per block
+constexpr int COOP_K = 8;                   // orientations per wave and chunk
+constexpr int64_t COOP_MAX_PIX = 1 << 17;   // above this the per-pixel kernel has the occupancy
+template <typename T>
+__global__ __launch_bounds__(COOP_NW *GX_WAVE) void k_detector_coop(gx_detector d) {
+  extern __shared__ __align__(16) unsigned char coop_smem[];
+  double *prod = reinterpret_cast<double *>(coop_smem);  // [COOP_NW * COOP_K][64]
+  unsigned char *idf = coop_smem + (size_t)COOP_NW * COOP_K * GX_WAVE * sizeof(double);
+  const int nflag = orient_flags(d, idf);
+  const T *__restrict__ iq = reinterpret_cast<const T *>(d.iq);
+  const int lane = threadIdx.x & (GX_WAVE - 1), wave = threadIdx.x / GX_WAVE;
+  const int64_t npx = d.n_pix * d.n_pix;
+  const int64_t p = blockIdx.x * (int64_t)GX_WAVE + lane;
+  const bool valid = p < npx;
+  double x0 = 0.0, y0 = 0.0, z0 = 0.0;
+  det_pixel(d, valid ? p : 0, x0, y0, z0);
+  const double dqinv = 1.0 / d.dq;
+  constexpr int CH = COOP_NW * COOP_K;
+  double acc = 0.0;
+  for (int64_t o0 = 0; o0 < d.n_orient; o0 += CH) {
+#pragma unroll
+    for (int k = 0; k < COOP_K; ++k) {
+      const int j = wave * COOP_K + k;
+      const int64_t o = o0 + j;
+      double pr = 0.0;
+      if (valid && o < d.n_orient) {
+        const double v = (double)iq[det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u)];
+        pr = v * d.orient_w[o];
+      }
+      prod[j * GX_WAVE + lane] = pr;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const int nj = (int)min((int64_t)CH, d.n_orient - o0);
+      for (int j = 0; j < nj; ++j) acc = acc + prod[j * GX_WAVE + lane];
+    }
+    __syncthreads();
+  }
+  if (wave == 0 && valid) d.det[p] = acc;
+}
+
 // ------------------------------------------------------------------------------------
 // adjoint of the detector stage
 // ------------------------------------------------------------------------------------
@@ -591,6 +637,16 @@ int gx_detector_accumulate(const gx_detector *d, gx_stream_t stream) {
                  (d->n_orient == 0 || (d->orient_R && d->orient_w)),
              "a detector buffer is NULL");
   const int64_t npx = d->n_pix * d->n_pix;
+  if (npx <= COOP_MAX_PIX && d->n_orient >= COOP_NW * COOP_K) {
+    // few pixels, many orientations: COOP_NW waves gather for the same 64 pixels
+    const unsigned nb = (unsigned)((npx + GX_WAVE - 1) / GX_WAVE);
+    const size_t lds = (size_t)COOP_NW * COOP_K * GX_WAVE * sizeof(double) + orient_flags_lds(d);
+    if (d->iq_f64)
+      k_detector_coop<double><<<nb, COOP_NW * GX_WAVE, lds, as_stream(stream)>>>(*d);
+    else
+      k_detector_coop<float><<<nb, COOP_NW * GX_WAVE, lds, as_stream(stream)>>>(*d);
+    return check_launch("gx_detector_accumulate");
+  }
   if (d->iq_f64)
     k_detector<double><<<grid1d(npx, BLK), BLK, orient_flags_lds(d), as_stream(stream)>>>(*d);
   else

@@ -2,7 +2,10 @@
 against the FP64 oracle.  Fed the same FP64 I(q) grid, the GPU image is
 bit-identical: same rotation matrices, the same FP64 operations per pixel
 (no FMA), exact floor-division indices and the same (psi, phi, theta)
-summation order."""
+summation order.  The bit-exact cases (<= 65^2 pixels, 222 orientations: three full
+64-orientation chunks and a partial one) run the cooperative gather used for small
+detectors with many orientations; the weights case (16 orientations) and the golden
+500^2 image run the one-thread-per-pixel kernel."""
 
 import numpy as np
 import pytest
