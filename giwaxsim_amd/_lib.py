@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_void_p
 from .build import LIB
 
 GX_OK = 0
-GX_ABI_VERSION = 5   # structure layouts below match this version of the header
+GX_ABI_VERSION = 6   # structure layouts below match this version of the header
 GX_FP_STRIDE = 24
 GX_MOMENTS_WORK = 774
 GX_FIN_MIRROR = 1
@@ -62,6 +62,8 @@ class GxFrames(ctypes.Structure):
         ("col_sy", c_void_p), ("col_sz", c_void_p), ("inter", c_void_p), ("col_spec", c_void_p),
         ("bin_sum", c_void_p), ("bin_cnt", c_void_p), ("status", c_void_p),
         ("row_variant", c_int64),
+        ("bs_rlo", c_int64), ("bs_rk", c_int64), ("bs_Mc", c_int64), ("bs_clo", c_int64), ("bs_ck", c_int64),
+        ("bs_kernel_c", c_void_p), ("bs_twiddle_c", c_void_p),
     ]
 
 

@@ -979,19 +979,21 @@ __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
 }
 
 // ------------------------------------------------------------------------------------
-// grid edges that are not powers of two: chirp-z (Bluestein) transforms
+// grid edges that are not powers of two: zoomed chirp-z (Bluestein) transforms
 //
-// X[k] = c[k] sum_n (x[n] c[n]) conj(c[k - n]),  c[n] = exp(-i pi n^2 / N),
-// a linear convolution evaluated as IFFT_M(FFT_M(x c) K) with K = FFT_M of the
-// zero-padded conj(c) (host, FP64) and M >= 2N - 1 a power of two; the inverse
-// transform is conj(FFT(conj(.))) and K already carries the 1/M.
+// With nk = (n^2 + k^2 - (k - n)^2) / 2 for any integers, X[k] = c[k] sum_n (x[n] c[n])
+// conj(c[k - n]), c[m] = exp(-i pi m^2 / N) = c[-m].  Only a band of signed
+// frequencies k = k_lo + j, j < K, is kept, so the sum is a linear convolution of the
+// N inputs with the lags t = j - n in [-(N-1), K): a circular convolution of length
+// M >= N + K - 1 (a power of two) evaluated as IFFT_M(FFT_M(x c) H), H = FFT_M of
+// h[t] = conj(c[k_lo + t]) wrapped (host, FP64, already divided by M); the inverse is
+// conj(FFT(conj(.))).  Output j of the inverse transform is X[k_lo + j].
 // ------------------------------------------------------------------------------------
 template <int M, typename C>
-__device__ __forceinline__ void chirpz_block(C (&v)[M / GX_BLK_FFT], C *lds, const gx_frames &d, int tid) {
+__device__ __forceinline__ void chirpz_block(C (&v)[M / GX_BLK_FFT], C *lds, const C *__restrict__ K,
+                                             const C *__restrict__ tw, int tid) {
   constexpr int P = M / GX_BLK_FFT;
   constexpr int R = BlkPlan<M>::R;
-  const C *tw = reinterpret_cast<const C *>(d.bs_twiddle);
-  const C *K = reinterpret_cast<const C *>(d.bs_kernel);
   fft_block<M, C>(v, lds, tw, tid);
 #pragma unroll
   for (int t = 0; t < P / R; ++t)
@@ -1078,7 +1080,8 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_rowpass_bs(gx_frames d, int64_t 
   const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
   const int wf = jhi - jlo + 1;
   if (wf <= 0) return;
-  if (wf > d.W) {
+  // the planner sizes W and the chirp-z band to every frame's kept columns
+  if (wf > d.W || jlo < d.bs_rlo + N / 2 || jhi > d.bs_rlo + d.bs_rk - 1 + N / 2) {
     if (tid == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
     return;
   }
@@ -1133,7 +1136,8 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_rowpass_bs(gx_frames d, int64_t 
           v[t * R + q] = x;
         }
       __syncthreads();
-      chirpz_block<M, C>(v, fbuf, d, tid);
+      chirpz_block<M, C>(v, fbuf, reinterpret_cast<const C *>(d.bs_kernel),
+                         reinterpret_cast<const C *>(d.bs_twiddle), tid);
     } else {
 #pragma unroll
       for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
@@ -1143,11 +1147,12 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_rowpass_bs(gx_frames d, int64_t 
     for (int t = 0; t < P / R; ++t)
 #pragma unroll
       for (int q = 0; q < R; ++q) {
-        const int k = tid + GX_BLK_FFT * t + q * (M / R);
-        if (k >= N) continue;
-        const int w = (k + N / 2) % N - jlo;
+        const int j = tid + GX_BLK_FFT * t + q * (M / R);
+        if (j >= (int)d.bs_rk) continue;
+        const int k = (int)d.bs_rlo + j;   // signed frequency, -N/2 <= k < N - N/2
+        const int w = k + N / 2 - jlo;     // fftshifted column within the frame's band
         if (w < 0 || w >= wf) continue;
-        C o = atoms ? cmul(v[t * R + q], chirp[k]) : C{(T)0, (T)0};
+        C o = atoms ? cmul(v[t * R + q], chirp[k < 0 ? -k : k]) : C{(T)0, (T)0};
         if (!inside) {
           o = (k == 0) ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
         } else if (dense) {
@@ -1205,7 +1210,8 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_colpass_bs(gx_frames d, int64_t 
       const int z = tid + GX_BLK_FFT * t + q * (M / R);
       v[t * R + q] = (z < N) ? cmul(col_input(d, col, z, k0, u, sz_row), chirp[z]) : C{(T)0, (T)0};
     }
-  chirpz_block<M, C>(v, fbuf, d, tid);
+  chirpz_block<M, C>(v, fbuf, reinterpret_cast<const C *>(d.bs_kernel_c),
+                     reinterpret_cast<const C *>(d.bs_twiddle_c), tid);
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;
   const double val = d.q_val;
   const int nn = (N - 1) / 2 + 1;  // fftfreq: k < nn are the non-negative frequencies
@@ -1213,15 +1219,17 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_colpass_bs(gx_frames d, int64_t 
   for (int t = 0; t < P / R; ++t)
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const int k = tid + GX_BLK_FFT * t + q * (M / R);
-      if (k >= N) continue;
-      const int i = (k + N / 2) % N;
+      const int j = tid + GX_BLK_FFT * t + q * (M / R);
+      if (j >= (int)d.bs_ck) continue;
+      const int k = (int)d.bs_clo + j;     // signed frequency, -N/2 <= k < N - N/2
+      const int i = k + N / 2;             // fftshifted row
       if (i < d.i_lo || i > d.i_hi) continue;
-      const int kint = (k < nn) ? k : k - N;
+      const int kk = k < 0 ? k + N : k;
+      const int kint = (kk < nn) ? kk : kk - N;
       const double qz = (((double)kint * val) * 2.0) * M_PI;
       const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
       if (bz < 0 || bz >= d.c_n) continue;
-      const double I = (double)cnorm(cmul(v[t * R + q], chirp[k]));
+      const double I = (double)cnorm(cmul(v[t * R + q], chirp[k < 0 ? -k : k]));
       const unsigned long long fx = (unsigned long long)llrint(I * d.bin_scale);
       atomicAdd(&d.bin_sum[cbase + bz], fx);
       atomicAdd(&d.bin_cnt[cbase + bz], 1);
@@ -1255,13 +1263,13 @@ int launch_bs(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st, bool r
     }
   } else {
     dim3 grid((unsigned)d->W, (unsigned)nf);
-    switch (d->bs_M) {
+    switch (d->bs_Mc) {
       case 1024: k_colpass_bs<1024, C><<<grid, GX_BLK_FFT, 0, st>>>(*d, f0); return check_launch("k_colpass_bs");
       case 2048: k_colpass_bs<2048, C><<<grid, GX_BLK_FFT, 0, st>>>(*d, f0); return check_launch("k_colpass_bs");
       case 4096: k_colpass_bs<4096, C><<<grid, GX_BLK_FFT, 0, st>>>(*d, f0); return check_launch("k_colpass_bs");
     }
   }
-  return set_error(GX_ERR_UNSUPPORTED, "chirp-z length M=%lld", (long long)d->bs_M);
+  return set_error(GX_ERR_UNSUPPORTED, "chirp-z length M=%lld", (long long)(row ? d->bs_M : d->bs_Mc));
 }
 
 // ------------------------------------------------------------------------------------
@@ -1384,8 +1392,20 @@ int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
   GX_REQUIRE(pow2_grid(d->N) ? d->bs_M == 0 : (d->bs_M == 1024 || d->bs_M == 2048 || d->bs_M == 4096),
              "grid edge N=%lld needs bs_M = 0 for 256..2048 powers of two, else a chirp-z length "
              "1024/2048/4096 (got %lld)", (long long)d->N, (long long)d->bs_M);
-  GX_REQUIRE(!d->bs_M || (d->bs_M >= 2 * d->N - 1 && d->bs_chirp && d->bs_kernel && d->bs_twiddle),
-             "chirp-z tables missing or M < 2N-1");
+  GX_REQUIRE(!d->bs_M || (d->bs_chirp && d->bs_kernel && d->bs_twiddle && d->bs_kernel_c && d->bs_twiddle_c),
+             "chirp-z tables missing");
+  GX_REQUIRE(!d->bs_M || (d->bs_Mc == 1024 || d->bs_Mc == 2048 || d->bs_Mc == 4096),
+             "chirp-z column length %lld", (long long)d->bs_Mc);
+  GX_REQUIRE(!d->bs_M || (d->bs_rk >= 1 && d->bs_rk <= d->N && d->bs_M >= d->N + d->bs_rk - 1 &&
+                          d->bs_rlo >= -(d->N / 2) && d->bs_rlo + d->bs_rk <= d->N - d->N / 2),
+             "chirp-z row band [%lld, +%lld) does not fit M=%lld or the frequencies of N=%lld",
+             (long long)d->bs_rlo, (long long)d->bs_rk, (long long)d->bs_M, (long long)d->N);
+  GX_REQUIRE(!d->bs_M || (d->bs_ck >= 1 && d->bs_ck <= d->N && d->bs_Mc >= d->N + d->bs_ck - 1 &&
+                          d->bs_clo >= -(d->N / 2) && d->bs_clo + d->bs_ck <= d->N - d->N / 2 &&
+                          d->bs_clo + d->N / 2 <= d->i_lo && d->bs_clo + d->bs_ck - 1 + d->N / 2 >= d->i_hi),
+             "chirp-z column band [%lld, +%lld) does not fit M=%lld or cover rows [%lld, %lld]",
+             (long long)d->bs_clo, (long long)d->bs_ck, (long long)d->bs_Mc, (long long)d->i_lo,
+             (long long)d->i_hi);
   GX_REQUIRE(d->r > 0.0, "r_voxel_size must be positive");
   GX_REQUIRE(nf >= 0 && nf <= d->batch, "batch of %lld frames exceeds workspace batch %lld", (long long)nf,
              (long long)d->batch);

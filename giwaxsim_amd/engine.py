@@ -371,6 +371,42 @@ def is_pow2_grid(N):
     return N in (256, 512, 1024, 2048)
 
 
+@functools.lru_cache(maxsize=32)
+def chirpz_band_tables(N: int, k_lo: int, K: int, f64: bool = False):
+    """Zoomed chirp-z of a length-N DFT kept at the signed frequencies k = k_lo + j,
+    j < K: the length M >= N + K - 1 (a power of two, at least 1024), the kernel
+    H = FFT_M(h) / M with h[t] = conj(c[k_lo + t]) = exp(+i pi ((k_lo + t)^2 mod 2N) / N)
+    for the lags t in [-(N-1), K) placed at t mod M (exact integer squares, FP64), and
+    the length-M twiddles."""
+    M = next((m for m in (1024, 2048, 4096) if m >= N + K - 1), None)
+    if M is None:
+        raise NotImplementedError(f"chirp-z band of {K} frequencies at N={N} needs a transform longer than 4096")
+    t = np.arange(-(N - 1), K, dtype=np.int64)
+    m = np.abs(k_lo + t)
+    ph = np.pi * ((m * m) % (2 * N)).astype(np.float64) / N
+    b = np.zeros(M, dtype=np.complex128)
+    b[t % M] = np.cos(ph) + 1j * np.sin(ph)
+    H = np.fft.fft(b) / M
+    out = np.empty(2 * M, dtype=np.float64 if f64 else np.float32)
+    out[0::2] = H.real
+    out[1::2] = H.imag
+    return M, out, twiddles(M, f64)
+
+
+def chirpz_bands(plan):
+    """(k_lo, K) of the row pass (every frame's kept columns) and of the column pass
+    (kept rows [i_lo, i_hi]) as signed frequencies k = column (row) - N // 2."""
+    fp = plan.fparams
+    lo, hi = fp[:, _lib.FP_JLO], fp[:, _lib.FP_JHI]
+    ok = hi >= lo
+    if ok.any():
+        jlo, jhi = int(lo[ok].min()), int(hi[ok].max())
+    else:
+        jlo = jhi = plan.N // 2
+    h = plan.N // 2
+    return (jlo - h, jhi - jlo + 1), (plan.i_lo - h, plan.i_hi - plan.i_lo + 1)
+
+
 @functools.lru_cache(maxsize=16)
 def chirpz_tables(N: int, f64: bool = False):
     """Chirp-z (Bluestein) tables of a length-N DFT through power-of-two
@@ -606,11 +642,16 @@ class FrameRunner:
         self.tw64 = dev_const(("twiddles", N, True), lambda: twiddles(N, True))
         if is_pow2_grid(N):
             self.bs_M, self.bs = 0, (None, None, None)
+            self.bs_band = (0, 0, 0, 0, 0, None, None)
         else:
-            M, ch, K, twm = chirpz_tables(N, self.f64)
+            _, ch, _, _ = chirpz_tables(N, self.f64)
+            (rlo, rk), (clo, ck) = chirpz_bands(plan)
+            M, Kr, twr = chirpz_band_tables(N, rlo, rk, self.f64)
+            Mc, Kc, twc = chirpz_band_tables(N, clo, ck, self.f64)
             self.bs_M = M
-            self.bs = tuple(dev_const(("chirpz", N, self.f64, i), lambda a=a: a)
-                            for i, a in enumerate((ch, K, twm)))
+            dc = lambda key, a: dev_const(("chirpz", N, self.f64) + key, lambda: a)
+            self.bs = (dc(("c",), ch), dc(("K", rlo, rk), Kr), dc(("tw", M), twr))
+            self.bs_band = (rlo, rk, Mc, clo, ck, dc(("K", clo, ck), Kc), dc(("tw", Mc), twc))
         B = self.batch
         self.ymin_key = _empty(3 * B, torch.int64)
         self.idx_bounds = _empty(4 * B, torch.int32)
@@ -641,7 +682,9 @@ class FrameRunner:
             i_lo=plan.i_lo, i_hi=plan.i_hi, W=W, Z=Z, bin_scale=bins.scale, fft_f64=int(self.f64),
             twiddle=ptr(self.tw).value, twiddle64=ptr(self.tw64).value, bs_M=self.bs_M,
             bs_chirp=ptr(self.bs[0]).value, bs_kernel=ptr(self.bs[1]).value,
-            bs_twiddle=ptr(self.bs[2]).value,
+            bs_twiddle=ptr(self.bs[2]).value, bs_rlo=self.bs_band[0], bs_rk=self.bs_band[1],
+            bs_Mc=self.bs_band[2], bs_clo=self.bs_band[3], bs_ck=self.bs_band[4],
+            bs_kernel_c=ptr(self.bs_band[5]).value, bs_twiddle_c=ptr(self.bs_band[6]).value,
             n_frames=F, fparams=ptr(self.fparams).value, batch=self.batch,
             ymin_key=ptr(self.ymin_key).value, idx_bounds=ptr(self.idx_bounds).value,
             col_nm=ptr(self.col_nm).value, col_sy=ptr(self.col_sy).value,

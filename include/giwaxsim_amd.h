@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GX_ABI_VERSION 5
+#define GX_ABI_VERSION 6
 
 #define GX_OK 0
 #define GX_ERR_ARG 1         /* invalid argument (message says which) */
@@ -140,11 +140,16 @@ typedef struct gx_frames {
   int64_t fft_f64;         /* 0: FP32 transforms (float2), 1: FP64 transforms (double2) */
   const void *twiddle;     /* [N] exp(-2 pi i m / N) as float2 or double2 (fft_f64) */
   const double *twiddle64; /* [N] the same as double2 (per-frame background spectra) */
-  /* grid edges that are not powers of two: chirp-z (Bluestein) transforms of length M */
-  int64_t bs_M;            /* 0: N is a power of two (in-wave transforms); else 1024, 2048 or 4096 >= 2N-1 */
-  const void *bs_chirp;    /* [N] exp(-i pi (n^2 mod 2N) / N) as float2 / double2 */
-  const void *bs_kernel;   /* [M] FFT_M of the zero-padded conjugate chirp, divided by M */
-  const void *bs_twiddle;  /* [M] exp(-2 pi i m / M) */
+  /* grid edges that are not powers of two: zoomed chirp-z (Bluestein) transforms.  The
+     row pass evaluates the DFT only at k = bs_rlo + j, j < bs_rk (signed frequencies
+     covering every frame's kept columns), as a circular convolution of length
+     bs_M >= N + bs_rk - 1; the column pass likewise at k = bs_clo + j, j < bs_ck (the
+     kept rows [i_lo, i_hi]), length bs_Mc >= N + bs_ck - 1 */
+  int64_t bs_M;            /* 0: N is a power of two (in-wave transforms); else 1024, 2048 or 4096 */
+  const void *bs_chirp;    /* [N] c[n] = exp(-i pi (n^2 mod 2N) / N) as float2 / double2 */
+  const void *bs_kernel;   /* [bs_M] FFT of h[t] = conj(c[|bs_rlo + t|]), lags t in
+                              [-(N-1), bs_rk) at t mod bs_M, divided by bs_M */
+  const void *bs_twiddle;  /* [bs_M] exp(-2 pi i m / bs_M) */
   int64_t n_frames;        /* frames described by fparams */
   const double *fparams;   /* [n_frames][GX_FP_STRIDE] */
   /* per-frame scratch for a batch of up to `batch` frames */
@@ -167,6 +172,12 @@ typedef struct gx_frames {
                               transforms; + 4: XCD-aware tile order (the frames of a row block
                               run together on one XCD and share its atoms in that L2) instead
                               of frame-major order */
+  /* zoomed chirp-z bands (bs_M != 0 only) */
+  int64_t bs_rlo, bs_rk;   /* row pass: first signed frequency and band length */
+  int64_t bs_Mc;           /* column pass: transform length, 1024, 2048 or 4096 */
+  int64_t bs_clo, bs_ck;   /* column pass: first signed frequency and band length */
+  const void *bs_kernel_c; /* [bs_Mc] the column band's kernel (as bs_kernel) */
+  const void *bs_twiddle_c; /* [bs_Mc] exp(-2 pi i m / bs_Mc) */
 } gx_frames;
 
 /* ------------------------------------------------------------------------------------

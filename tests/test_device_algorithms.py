@@ -120,3 +120,24 @@ def test_chirpz_tables(N):
     conv = np.conj(np.fft.fft(np.conj(np.fft.fft(a) * Kc)))   # IFFT via conj(FFT(conj)); 1/M in K
     X = c * conv[:N]
     assert np.allclose(X, np.fft.fft(x), rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize("N,k_lo,K", [(131, -30, 61), (300, -150, 300), (1048, -95, 190),
+                                      (1048, -141, 283), (700, 5, 40), (2000, -1000, 100)])
+def test_chirpz_band_tables(N, k_lo, K):
+    """The zoomed chirp-z with the product's band tables reproduces the length-N DFT
+    at the signed frequencies k_lo .. k_lo + K - 1 (wrapped mod N), in the kernels'
+    arithmetic: input chirp c[n], circular convolution of length M >= N + K - 1,
+    output j times c[|k_lo + j|]."""
+    M, Kt, _ = engine.chirpz_band_tables(N, k_lo, K, True)
+    assert M >= N + K - 1 and M in (1024, 2048, 4096)
+    _, ch, _, _ = engine.chirpz_tables(N, True)
+    c = ch[0::2] + 1j * ch[1::2]
+    H = Kt[0::2] + 1j * Kt[1::2]
+    x = np.random.default_rng(N + K).normal(size=N) + 1j * np.random.default_rng(N - K).normal(size=N)
+    a = np.zeros(M, complex)
+    a[:N] = x * c
+    conv = np.conj(np.fft.fft(np.conj(np.fft.fft(a) * H)))
+    k = k_lo + np.arange(K)
+    X = c[np.abs(k)] * conv[:K]
+    assert np.allclose(X, np.fft.fft(x)[k % N], rtol=1e-12, atol=1e-10)
