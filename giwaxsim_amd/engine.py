@@ -653,14 +653,17 @@ class FrameRunner:
             self.bs = (dc(("c",), ch), dc(("K", rlo, rk), Kr), dc(("tw", M), twr))
             self.bs_band = (rlo, rk, Mc, clo, ck, dc(("K", clo, ck), Kc), dc(("tw", Mc), twc))
         B = self.batch
-        self.ymin_key = _empty(3 * B, torch.int64)
-        self.idx_bounds = _empty(4 * B, torch.int32)
-        self.col_nm = _empty(B * N, torch.int32) if fill_bkg else None
-        self.col_sy = _empty(B * N, torch.float64) if smooth else None
-        self.col_sz = _empty(B * N, torch.float64) if smooth else None
+        # per-frame statistics and column data for every frame (small: the statistics,
+        # column masks and background spectra of all frames are computed in one launch
+        # each); the row-transformed frames only for one batch
+        self.ymin_key = _empty(3 * F, torch.int64)
+        self.idx_bounds = _empty(4 * F, torch.int32)
+        self.col_nm = _empty(F * N, torch.int32) if fill_bkg else None
+        self.col_sy = _empty(F * N, torch.float64) if smooth else None
+        self.col_sz = _empty(F * N, torch.float64) if smooth else None
         cdt = torch.float64 if self.f64 else torch.float32
         self.inter = _empty(B * W * Z * 2, cdt)
-        self.col_spec = _empty(B * W * 2, cdt) if (fill_bkg or smooth) else None
+        self.col_spec = _empty(F * W * 2, cdt) if (fill_bkg or smooth) else None
         self.status = torch.zeros(4, dtype=torch.int32, device=dev)
         if smooth:
             cw, rad = gauss_prefix(smooth)
@@ -703,12 +706,38 @@ class FrameRunner:
         self.status.zero_()
 
     def run(self, frames=None):
-        """Enqueue frames [lo, hi) (default: all) on the current stream."""
+        """Enqueue frames [lo, hi) (default: all) on the current stream: the per-frame
+        statistics and columns of all of them in one launch each, then the row and
+        column passes batch by batch (each batch's descriptor addresses its frames'
+        slots of the all-frame arrays)."""
         lo, hi = (0, self.plan.n_frames) if frames is None else frames
+        n = hi - lo
+        if n <= 0:
+            return
         st = stream_handle()
+        d = self.desc
+        da = GxFrames.from_buffer_copy(d)
+        da.batch = self.plan.n_frames
+        call("gx_frames_stats", ctypes.byref(da), lo, n, st)
+        if d.fill_bkg or d.smooth:
+            call("gx_frames_columns", ctypes.byref(da), lo, n, st)
+        N, W = self.plan.N, self.plan.W
+        esz = 16 if self.f64 else 8
         for f0 in range(lo, hi, self.batch):
             nf = min(self.batch, hi - f0)
-            call("gx_frames_forward", ctypes.byref(self.desc), f0, nf, st)
+            k = f0 - lo
+            db = GxFrames.from_buffer_copy(d)
+            db.ymin_key = d.ymin_key + 8 * 3 * k
+            db.idx_bounds = d.idx_bounds + 4 * 4 * k
+            if d.col_nm:
+                db.col_nm = d.col_nm + 4 * N * k
+            if d.col_sy:
+                db.col_sy = d.col_sy + 8 * N * k
+                db.col_sz = d.col_sz + 8 * N * k
+            if d.col_spec:
+                db.col_spec = d.col_spec + esz * W * k
+            call("gx_frames_rowpass", ctypes.byref(db), f0, nf, st)
+            call("gx_frames_colpass", ctypes.byref(db), f0, nf, st)
 
     def check(self):
         s = self.status.cpu().numpy()
