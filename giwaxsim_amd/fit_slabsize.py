@@ -141,8 +141,8 @@ class _Param:
     """A bounded fit parameter; like an lmfit Parameter, a value outside [min, max]
     is clipped to the bound (vary or not)."""
 
-    def __init__(self, value, min, max, vary):
-        self.min, self.max, self.vary = float(min), float(max), bool(vary)
+    def __init__(self, value, lo, hi, vary):
+        self.min, self.max, self.vary = float(lo), float(hi), bool(vary)
         self.init_value = float(value)
         self.value = float(np.clip(self.init_value, self.min, self.max))
 
