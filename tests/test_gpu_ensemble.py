@@ -1,5 +1,6 @@
 """Ensembles (many slabs, uneven atom counts and grid edges, one shared I(q)
-grid) and HIP-graph replay of the whole forward pass."""
+grid) and HIP-graph replay of the whole forward pass; also with slabs whose grid edge
+is not a power of two (N = 300: zoomed chirp-z transforms) beside a N = 512 slab."""
 
 import numpy as np
 import pytest
@@ -15,13 +16,17 @@ QV = 2 * np.pi / (0.4 * 512) * (1 + 1e-9)   # ~0.0307, shared by every slab
 MQ = 2.0
 
 
-def _slabs():
+def _slabs(kind='pow2'):
+    if kind == 'chirpz':
+        return (ensemble_slabs(n_slabs=2, seed=8, size_range=(16.0, 24.0), grids=(300,), q_voxel=QV) +
+                ensemble_slabs(n_slabs=1, seed=9, size_range=(16.0, 24.0), grids=(512,), q_voxel=QV))
     return ensemble_slabs(n_slabs=3, seed=7, size_range=(16.0, 28.0), grids=(256, 512), q_voxel=QV)
 
 
-def test_ensemble_pooled_bins_exact():
-    print("[gpu-ensemble] pooled vs per-slab", flush=True)
-    slabs = _slabs()
+@pytest.mark.parametrize("kind", ['pow2', 'chirpz'])
+def test_ensemble_pooled_bins_exact(kind):
+    print(f"[gpu-ensemble] pooled vs per-slab ({kind})", flush=True)
+    slabs = _slabs(kind)
     ens = engine.Ensemble(slabs, QV, MQ, 12700.0, fill_bkg=True, smooth=2, batch=64)
     ens.run()
     ens.check()
@@ -38,9 +43,10 @@ def test_ensemble_pooled_bins_exact():
     assert torch.equal(tot_s, pooled_s) and torch.equal(tot_c, pooled_c)
 
 
-def test_ensemble_vs_oracle():
-    print("[gpu-ensemble] vs oracle", flush=True)
-    slabs = _slabs()
+@pytest.mark.parametrize("kind", ['pow2', 'chirpz'])
+def test_ensemble_vs_oracle(kind):
+    print(f"[gpu-ensemble] vs oracle ({kind})", flush=True)
+    slabs = _slabs(kind)
     ens = engine.Ensemble(slabs, QV, MQ, 12700.0, fill_bkg=True, smooth=2, batch=64)
     iq64, _ = ens.run()
     vg = vc = None
@@ -63,9 +69,10 @@ def test_ensemble_vs_oracle():
     assert rel.max() <= 1e-5
 
 
-def test_graph_replay_matches_eager():
-    print("[gpu-ensemble] graph replay", flush=True)
-    slabs = _slabs()
+@pytest.mark.parametrize("kind", ['pow2', 'chirpz'])
+def test_graph_replay_matches_eager(kind):
+    print(f"[gpu-ensemble] graph replay ({kind})", flush=True)
+    slabs = _slabs(kind)
     ens = engine.Ensemble(slabs, QV, MQ, 12700.0, fill_bkg=True, smooth=2, batch=64)
     iq_eager = ens.run()[0].clone()
     torch.cuda.synchronize()
