@@ -37,7 +37,7 @@ import torch
 
 from . import _lib
 from ._lib import GxAdjoint, GxDetector, GxFrames, GxSlab, call
-from .engine import (_empty, bin_scale_for, collapse_record, detector_plan, device, element_table,
+from .engine import (_empty, bin_scale_for, collapse_records, detector_plan, device, element_table,
                      fft_precision, frame_plan, gauss_prefix, intensity_bound, is_pow2_grid,
                      pixels_per_bin, ptr, stream_handle, twiddles)
 from .tools.ptable_dict import aff_dict, ptable
@@ -330,8 +330,7 @@ class DifferentiableSlab:
         max_voxels = np.sqrt(np.float64(xb) ** 2 + np.float64(yb) ** 2) // r
         if self.fill and self._records_for != (xb, yb):
             fpr = self.plan.fparams.copy()
-            for f, phi in enumerate(self.plan.grid.phis):
-                fpr[f, _lib.FP_LEN_MODE:_lib.FP_CT + 1] = collapse_record(phi, yb, xb)
+            fpr[:, _lib.FP_LEN_MODE:_lib.FP_CT + 1] = collapse_records(self.plan.grid.phis, yb, xb)
             self.fparams.copy_(torch.as_tensor(fpr))
             self._records_for = (xb, yb)
         B = intensity_bound(sum_abs, N, self.fill, self.smooth, self.avg, max_voxels, yb, xb, r)

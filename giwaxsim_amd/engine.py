@@ -227,6 +227,61 @@ def collapse_record(phi, hor_length, ver_length):
             ver_length * np.cos(theta_rad), np.tan(theta_rad), np.cos(phi_rad), np.cos(theta_rad))
 
 
+_COLLAPSE_TRIG: dict = {}
+_COLLAPSE_TRIG_MAX = 16
+
+
+def _collapse_trig(phis):
+    """The bound-independent part of collapse_record for every phi, [F, 8]:
+    (mode, swap, cos phi, cos theta, sin phi, sin theta, tan phi, tan theta) after the
+    phi > 90 fold, from the same scalar calls as collapse_record.  Kept per phi list."""
+    phis = np.asarray(phis, dtype=np.float64)
+    key = phis.tobytes()
+    hit = _COLLAPSE_TRIG.get(key)
+    if hit is not None:
+        return hit
+    t = np.zeros((len(phis), 8), dtype=np.float64)
+    for f, phi in enumerate(phis):
+        swap = phi > 90
+        if swap:
+            phi = phi - 90
+        theta_rad = np.deg2rad(90 - phi)
+        phi_rad = np.deg2rad(phi)
+        mode = 1 if phi == 0 else 2 if phi == 90 else 0
+        t[f] = (mode, swap, np.cos(phi_rad), np.cos(theta_rad), np.sin(phi_rad), np.sin(theta_rad),
+                np.tan(phi_rad), np.tan(theta_rad))
+    t.setflags(write=False)
+    if len(_COLLAPSE_TRIG) >= _COLLAPSE_TRIG_MAX:
+        _COLLAPSE_TRIG.pop(next(iter(_COLLAPSE_TRIG)))
+    _COLLAPSE_TRIG[key] = t
+    return t
+
+
+def collapse_records(phis, hor_length, ver_length):
+    """collapse_record of every phi at once, [F, 13], bit-identical to the per-frame
+    records: the trigonometry is cached per phi list, and what depends on the bounds
+    is products, quotients, sums and square roots (correctly rounded elementwise) and
+    squares taken as the same float64 scalar powers."""
+    mode, swap, cp, ct, sp, st, tp, tt = _collapse_trig(phis).T
+    hor0, ver0 = np.float64(hor_length), np.float64(ver_length)
+    sw = swap != 0
+    hor = np.where(sw, ver0, hor0)
+    ver = np.where(sw, hor0, ver0)
+    vc = ver * ct
+    stop2 = hor * cp
+    first = vc < stop2
+    with np.errstate(divide='ignore', invalid='ignore'):
+        mid = np.where(first, ver / st, hor / sp)
+        ver2 = np.where(sw, hor0 ** 2, ver0 ** 2)
+        vc2 = np.array([v ** 2 for v in vc], dtype=np.float64)
+        A = np.sqrt(ver2 - vc2)
+    s1 = np.where(first, vc, stop2)
+    s2 = np.where(first, stop2, vc)
+    rec = np.stack([mode, hor, ver, s1, s2, s1 + s2, mid, A, tp, vc, tt, cp, ct], axis=1)
+    rec[mode != 0, 3:] = 0.0
+    return rec
+
+
 @dataclass
 class FramePlan:
     """Atom-independent plan of the phi-frame loop of one slab."""
@@ -348,8 +403,7 @@ def build_frame_plan(N, r, g: QGrid, x_bound, y_bound, z_bound=None) -> FramePla
     fp0, W, i_lo, i_hi = _frame_geometry(N, r, g)
     fp = fp0.copy()
     max_voxels = np.sqrt(np.float64(x_bound) ** 2 + np.float64(y_bound) ** 2) // r
-    for f, phi in enumerate(g.phis):
-        fp[f, _lib.FP_LEN_MODE:_lib.FP_CT + 1] = collapse_record(phi, y_bound, x_bound)
+    fp[:, _lib.FP_LEN_MODE:_lib.FP_CT + 1] = collapse_records(g.phis, y_bound, x_bound)
     return FramePlan(N=N, r=r, grid=g, x_bound=float(x_bound), y_bound=float(y_bound), fparams=fp,
                      W=W, i_lo=i_lo, i_hi=i_hi,
                      q_val=1.0 / (N * r), max_voxels=float(max_voxels), Z=row_range(N, r, z_bound),

@@ -106,6 +106,23 @@ def test_collapse_record_matches_oracle():
             assert np.array_equal(dev, O.rectangular_collapse_lengths(x, hor, ver, phi))
 
 
+def test_collapse_records_equal_per_frame_records():
+    """The vectorised records of a whole phi list (what the frame plans use) equal the
+    per-frame collapse_record bit for bit, on the phi lists of the voxel-grid q grids,
+    a 1-degree list with 90 and 180, and random phis, for equal, unequal and random
+    bounds (either order)."""
+    rng = np.random.default_rng(11)
+    lists = [engine.qgrid(0.02, 2.0).phis, engine.qgrid(2 * np.pi / (0.25 * 256) * (1 + 1e-9), 2.0).phis,
+             np.arange(0, 181, 1.0), np.sort(rng.uniform(0, 180, 300)), np.linspace(0, 179.86, 97)]
+    bounds = [(100.0, 100.0), (37.5, 141.2), (186.0, 43.0)] + [tuple(rng.uniform(5, 400, 2)) for _ in range(6)]
+    for phis in lists:
+        for hor, ver in bounds:
+            got = engine.collapse_records(phis, hor, ver)
+            want = np.array([engine.collapse_record(phi, hor, ver) for phi in phis], dtype=np.float64)
+            assert got.shape == want.shape
+            assert np.array_equal(got.view(np.int64), want.view(np.int64)), (hor, ver)
+
+
 def test_rotate_coords_z_y():
     rng = np.random.default_rng(3)
     c = rng.uniform(-50, 50, size=(1000, 3))
