@@ -21,7 +21,7 @@ from __future__ import annotations
 import ctypes
 import functools
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -322,11 +322,20 @@ def qgrid_from_axis(q_axis, phis, max_q=None):
                  c_lo=c_lo, c_n=c_n)
 
 
+@functools.lru_cache(maxsize=8)
+def qgrid_full(q_voxel_size: float, max_q: float) -> QGrid:
+    """qgrid with every q_num^3 voxel binned (no downselect_voxelgrid crop), the grid
+    generate_voxel_grid_low_mem returns (tools/voxelgrids.py:587-592, 636)."""
+    g = qgrid(q_voxel_size, max_q)
+    return replace(g, c_lo=0, c_n=g.q_num)
+
+
 @functools.lru_cache(maxsize=32)
 def frame_plan(r_voxel_size: float, q_voxel_size: float, max_q: float, x_bound: float,
-               y_bound: float, z_bound: float) -> FramePlan:
+               y_bound: float, z_bound: float, crop: bool = True) -> FramePlan:
     N = check_physical(r_voxel_size, q_voxel_size, max_q, x_bound, y_bound, z_bound)
-    return build_frame_plan(N, r_voxel_size, qgrid(q_voxel_size, max_q), x_bound, y_bound, z_bound)
+    g = qgrid(q_voxel_size, max_q) if crop else qgrid_full(q_voxel_size, max_q)
+    return build_frame_plan(N, r_voxel_size, g, x_bound, y_bound, z_bound)
 
 
 def row_range(N, r, z_bound):
@@ -808,7 +817,7 @@ _WORKSPACES_MAX = 4
 
 def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, fill_bkg=False,
                   smooth=0, f_table=None, elem_ids=None, batch=None, want64=True, frames=None,
-                  precision='auto', variant=None, f0_element='C'):
+                  precision='auto', variant=None, f0_element='C', crop=True):
     """Full voxel-grid path of one slab (``coords``: host (n, 3), device (n, 3) or a tuple
     of device x, y, z columns).  Returns dict with device iq32/iq64,
     host axes and the objects that produced them.  ``frames=(lo, hi)``
@@ -819,7 +828,8 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
     by the next call with the same key; iq64 / iq32 are new tensors every call,
     while the returned 'slab', 'bins' and 'runner' objects are overwritten by
     that next call.  ``f0_element``: the element of the (f0/Z)^2 factor applied at the
-    end (the reference's 'C'), None for none."""
+    end (the reference's 'C'), None for none.  ``crop=False`` bins every voxel of the
+    q_num^3 grid (the grid before downselect_voxelgrid) and returns the full axes."""
     if f_table is None:
         elem_ids, f_table, counts, uniq = element_table(elements, energy)
     elif not torch.is_tensor(elem_ids):
@@ -835,7 +845,7 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
         zb = np.max(coords[:, 2]) - np.min(coords[:, 2])
         zmin = np.min(coords[:, 2])
     plan = frame_plan(float(r_voxel_size), float(q_voxel_size), float(max_q), float(xb), float(yb),
-                      float(zb))
+                      float(zb), bool(crop))
     if not _lib.load().gx_supported_grid(plan.N):
         raise NotImplementedError(
             f"grid_size N={plan.N} (from 2*pi/(q_voxel_size*r_voxel_size)) is outside the "

@@ -20,7 +20,7 @@ from .._lib import call, dbl_array
 from .ptable_dict import aff_dict, ptable
 
 __all__ = ['downselect_voxelgrid', 'rectangular_collapse_lengths', 'rotate_project_fft_coords',
-           'process_file2', 'add_f0_q_3d']
+           'process_file2', 'add_f0_q_3d', 'generate_voxel_grid_low_mem', 'structure_voxel_grid', 'shell_f0']
 
 
 def downselect_voxelgrid(grid, x_axis, y_axis, z_axis, max_val):
@@ -148,3 +148,80 @@ def rotate_project_fft_coords(args):
     finally:
         s_shm.close()
         c_shm.close()
+
+
+# --------------------------------------------------------------------------------------
+# whole-structure voxel grid (tools/voxelgrids.py:535-722)
+# --------------------------------------------------------------------------------------
+def shell_f0(q_val, element):
+    """f0 of get_element_f0_dict (tools/utilities.py:318-339): the Cromer-Mann sum with
+    q_val where the formula has q^2."""
+    a = aff_dict[element]
+    return (a[0] * np.exp(-a[1] * (q_val) / (16 * np.pi ** 2)) +
+            a[2] * np.exp(-a[3] * (q_val) / (16 * np.pi ** 2)) +
+            a[4] * np.exp(-a[5] * (q_val) / (16 * np.pi ** 2)) +
+            a[6] * np.exp(-a[7] * (q_val) / (16 * np.pi ** 2)) +
+            a[8])
+
+
+def structure_voxel_grid(input_path, r_voxel_size, q_voxel_size, max_q, aff_num_qs, energy, fill_bkg=False,
+                         smooth=0, crop=True):
+    """(iq [qy, qx, qz] FP64 numpy, qx, qy, qz) of one .xyz / .pdb file, without the
+    f0 factor: the full q_num^3 grid of generate_voxel_grid_low_mem (crop=False) or
+    that grid cropped by downselect_voxelgrid (crop=True), from the device frame loop.
+
+    ``aff_num_qs > 1`` follows the reference's q-shell branch (tools/voxelgrids.py:650-707)
+    as it computes: each shell's pass overwrites the grid, so the result is the last
+    shell's grid (f = f0(q_shell) + f1 + i f2, f0 evaluated at q, not q^2) with the voxels
+    of that shell doubled.  Only that last pass is run."""
+    from .utilities import load_pdb, load_xyz
+    if input_path[-3:] == 'xyz':
+        coords, elements = load_xyz(input_path)
+    elif input_path[-3:] == 'pdb':
+        coords, elements = load_pdb(input_path)
+    else:
+        raise Exception('files must be a .pdb or .xyz file')
+    if aff_num_qs == 1:
+        res = engine.run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy,
+                                   fill_bkg=fill_bkg, smooth=int(smooth), f0_element=None, crop=crop)
+        return res['iq64'].cpu().numpy(), res['qx'], res['qy'], res['qz']
+    if aff_num_qs > 1:
+        from .f1f2 import f1_f2
+        ids, _, _, uniq = engine.element_table(elements, energy)
+        grid = engine.qgrid(float(q_voxel_size), float(max_q))
+        step = grid.max_q_diag / int(aff_num_qs)
+        q_val = 0.5 * step + (int(aff_num_qs) - 1) * step    # the last shell's centre
+        upper_q, lower_q = q_val + step / 2, q_val - step / 2
+        f_table = np.array([shell_f0(q_val, str(e)) + complex(*f1_f2(str(e), energy)) for e in uniq],
+                           dtype=np.complex128)
+        res = engine.run_voxelgrid(coords, None, r_voxel_size, q_voxel_size, max_q, energy, fill_bkg=fill_bkg,
+                                   smooth=int(smooth), f_table=f_table, elem_ids=ids, f0_element=None, crop=crop)
+        iq = res['iq64'].cpu().numpy()
+        qx, qy, qz = res['qx'], res['qy'], res['qz']
+        qxm, qym, qzm = np.meshgrid(qx, qy, qz)
+        qr = np.sqrt(qxm ** 2 + qym ** 2 + qzm ** 2)
+        iq += np.where((qr <= upper_q) & (qr > lower_q), iq, 0)
+        return iq, qx, qy, qz
+    raise Exception('Invalid aff_num_qs value. Must be non-negative integer')
+
+
+def generate_voxel_grid_low_mem(input_path, r_voxel_size, q_voxel_size, max_q, aff_num_qs, energy, gen_name,
+                                output_dir=None, scratch_folder=None, num_cpus=None, fill_bkg=False, smooth=0):
+    """I(q) voxel grid of one structure file on the full q_num^3 grid, all phi-frames on
+    the device (tools/voxelgrids.py:535-722): returns (iq [qy, qx, qz], qx, qy, qz), or
+    with ``output_dir`` writes ``{output_dir}/{gen_name}_output_files/{gen_name}_iq|qx|qy|qz.npy``
+    and returns None.  Voxels no frame reaches are 0.  ``scratch_folder`` and ``num_cpus``
+    are accepted and unused."""
+    import os
+    iq, qx, qy, qz = structure_voxel_grid(input_path, r_voxel_size, q_voxel_size, max_q, aff_num_qs, energy,
+                                          fill_bkg=fill_bkg, smooth=smooth, crop=False)
+    if not output_dir:
+        return iq, qx, qy, qz
+    save_path = f'{output_dir}/{gen_name}_output_files/'
+    if not os.path.exists(save_path):
+        os.mkdir(save_path)
+    np.save(f'{save_path}{gen_name}_iq.npy', iq)
+    np.save(f'{save_path}{gen_name}_qx.npy', qx)
+    np.save(f'{save_path}{gen_name}_qy.npy', qy)
+    np.save(f'{save_path}{gen_name}_qz.npy', qz)
+    return None

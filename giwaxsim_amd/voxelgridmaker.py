@@ -29,24 +29,10 @@ import time
 
 import numpy as np
 
-from . import engine
-from .tools.ptable_dict import aff_dict
-from .tools.utilities import (load_pdb, load_xyz, most_common_element, parse_config_file, save_config_to_txt,
-                              str_to_bool)
-from .tools.voxelgrids import add_f0_q_3d
+from .tools.utilities import most_common_element, parse_config_file, save_config_to_txt, str_to_bool
+from .tools.voxelgrids import add_f0_q_3d, shell_f0, structure_voxel_grid
 
 __all__ = ['main', 'voxel_grid_cropped', 'shell_f0']
-
-
-def shell_f0(q_val, element):
-    """f0 of get_element_f0_dict (tools/utilities.py:318-339): the Cromer-Mann sum with
-    q_val where the formula has q^2."""
-    a = aff_dict[element]
-    return (a[0] * np.exp(-a[1] * (q_val) / (16 * np.pi ** 2)) +
-            a[2] * np.exp(-a[3] * (q_val) / (16 * np.pi ** 2)) +
-            a[4] * np.exp(-a[5] * (q_val) / (16 * np.pi ** 2)) +
-            a[6] * np.exp(-a[7] * (q_val) / (16 * np.pi ** 2)) +
-            a[8])
 
 
 def voxel_grid_cropped(input_path, r_voxel_size, q_voxel_size, max_q, aff_num_qs, energy, fill_bkg=False,
@@ -54,34 +40,8 @@ def voxel_grid_cropped(input_path, r_voxel_size, q_voxel_size, max_q, aff_num_qs
     """(iq [qy, qx, qz] FP64 numpy, qx, qy, qz) of one structure file on the cropped
     grid, without the f0 factor (generate_voxel_grid_low_mem followed by
     downselect_voxelgrid)."""
-    if input_path[-3:] == 'xyz':
-        coords, elements = load_xyz(input_path)
-    elif input_path[-3:] == 'pdb':
-        coords, elements = load_pdb(input_path)
-    else:
-        raise Exception('files must be a .pdb or .xyz file')
-    if aff_num_qs == 1:
-        res = engine.run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy,
-                                   fill_bkg=fill_bkg, smooth=int(smooth), f0_element=None)
-        return res['iq64'].cpu().numpy(), res['qx'], res['qy'], res['qz']
-    if aff_num_qs > 1:
-        from .tools.f1f2 import f1_f2
-        ids, _, _, uniq = engine.element_table(elements, energy)
-        grid = engine.qgrid(float(q_voxel_size), float(max_q))
-        step = grid.max_q_diag / int(aff_num_qs)
-        q_val = 0.5 * step + (int(aff_num_qs) - 1) * step    # the last shell's centre
-        upper_q, lower_q = q_val + step / 2, q_val - step / 2
-        f_table = np.array([shell_f0(q_val, str(e)) + complex(*f1_f2(str(e), energy)) for e in uniq],
-                           dtype=np.complex128)
-        res = engine.run_voxelgrid(coords, None, r_voxel_size, q_voxel_size, max_q, energy, fill_bkg=fill_bkg,
-                                   smooth=int(smooth), f_table=f_table, elem_ids=ids, f0_element=None)
-        iq = res['iq64'].cpu().numpy()
-        qx, qy, qz = res['qx'], res['qy'], res['qz']
-        qxm, qym, qzm = np.meshgrid(qx, qy, qz)
-        qr = np.sqrt(qxm ** 2 + qym ** 2 + qzm ** 2)
-        iq += np.where((qr <= upper_q) & (qr > lower_q), iq, 0)
-        return iq, qx, qy, qz
-    raise Exception('Invalid aff_num_qs value. Must be non-negative integer')
+    return structure_voxel_grid(input_path, r_voxel_size, q_voxel_size, max_q, aff_num_qs, energy,
+                                fill_bkg=fill_bkg, smooth=smooth, crop=True)
 
 
 def main(config):

@@ -7,6 +7,10 @@ old_modules/detectormaker.py) on the graphite test cell (N = 256), against the o
   of its maximum; the saved axes are the oracle's crop axes exactly;
 * stage 1, aff_num_qs = 3: the last q shell's grid (f = f0(q_shell) + f1 + i f2,
   no final f0 factor) with the shell's voxels doubled, to the same bar;
+* generate_voxel_grid_low_mem (tools/voxelgrids.py:535-722) on the full q_num^3 grid:
+  the oracle's sum / count (0 where no frame reaches) to the same bar, zeros in the same
+  voxels, the full q axis, cropped by downselect_voxelgrid equal to the stage-1 grid bit
+  for bit, and the output_dir form writing the four arrays;
 * stage 2: the image read back from the handoff, summed over 19 psi orientations and
   mirrored, equals the oracle detector fed the same I(q) bit for bit (the legacy floor
   keeps 1e-6 and skips the 1e-6 rescale, so legacy * 1e-6 == oracle exactly); a second
@@ -106,3 +110,32 @@ def test_detectormaker_stage(tmp_path):
     assert np.array_equal(det * 1e-6, want)
     _say(f"  image {det.shape} bit-exact vs oracle (x 1e-6)")
     assert os.path.basename(D.main(dict(cfg))) == 'gr1_det_sum1'
+
+
+def test_generate_voxel_grid_low_mem_full_grid(tmp_path):
+    import oracle as O
+    from giwaxsim_amd import voxelgridmaker as V
+    from giwaxsim_amd.tools.utilities import load_xyz
+    from giwaxsim_amd.tools.voxelgrids import downselect_voxelgrid, generate_voxel_grid_low_mem
+    _say("[gpu-legacy] generate_voxel_grid_low_mem, graphite, full grid")
+    path = f'{GOLDEN}/graphite_medium.xyz'
+    iq, qx, qy, qz = generate_voxel_grid_low_mem(path, 0.25, QV, 2.0, 1, 12700, 'gr', fill_bkg=True, smooth=2)
+    coords, el = load_xyz(path)
+    ids, ftab, _, _ = engine.element_table(el, 12700.0)
+    ref = O.voxelgridmaker(coords, ftab[ids], 0.25, QV, 2.0, fill_bkg=True, smooth=2)
+    want = np.divide(ref['sum'], ref['count'], out=np.zeros_like(ref['sum']), where=ref['count'] != 0)
+    assert iq.shape == want.shape == (len(qx),) * 3
+    assert np.array_equal(qx, ref['plan']['q_axis']) and np.array_equal(qy, qx) and np.array_equal(qz, qx)
+    assert np.array_equal(iq == 0, want == 0)
+    err = _close(iq, want)
+    _say(f"  iq {iq.shape}: max rel err {err:.2e}, {int(np.sum(want == 0))} unreached voxels")
+    assert err <= 1e-5 and np.sum(want == 0) > 0
+    small, ax, _, _ = downselect_voxelgrid(iq, qx, qy, qz, 2.0)
+    crop, cax, _, _ = V.voxel_grid_cropped(path, 0.25, QV, 2.0, 1, 12700, fill_bkg=True, smooth=2)
+    assert np.array_equal(ax, cax) and np.array_equal(small, crop)
+    assert generate_voxel_grid_low_mem(path, 0.25, QV, 2.0, 1, 12700, 'gr', output_dir=str(tmp_path),
+                                       fill_bkg=True, smooth=2) is None
+    saved = np.load(tmp_path / 'gr_output_files' / 'gr_iq.npy', allow_pickle=False)
+    assert np.array_equal(saved, iq)
+    for k in ('qx', 'qy', 'qz'):
+        assert np.array_equal(np.load(tmp_path / 'gr_output_files' / f'gr_{k}.npy', allow_pickle=False), qx)
