@@ -18,13 +18,17 @@ import numpy as np
 
 from .. import engine
 from .._lib import GX_MOMENTS_WORK, GxSlab, call
+from .ptable_dict import aff_dict, ptable
 from .utilities import calc_real_space_abc, load_pdb, load_xyz
 
 __all__ = ['slabmaker_fitting', 'voxelgridmaker_fitting', 'detectormaker_fitting',
            'optimize_scale_offset', 'DeviceIQ', 'trim_sim_data', 'rebin_qmap_and_mask',
            'linear_polar', 'polar_linear', 'add_pad', 'shift_peak', 'pad_columns', 'ShiftPeakPlan',
            'RebinWeights', 'rebin_weights', 'FitComparison', 'SlabBuilder', 'mirror_qmap_positive_qxy_only', 'evaluate_fit',
-           'make_linecut', 'plot_fit', 'plot_linecuts', 'load_exp_image']
+           'make_linecut', 'plot_fit', 'plot_linecuts', 'load_exp_image', 'mirror_qmap_fix_detgap',
+           'mirror_qmap_positive_qxy_only_old', 'find_nearest', 'match_qmap_index', 'normalize_qmap',
+           'normalize_qmap_position', 'mask_forbidden_pixels', 'create_wedge_mask', 'rebin_and_combine_qmaps',
+           'scale_offset_mask_qmap', 'add_f0_q_dependence']
 
 
 class DeviceIQ(np.ndarray):
@@ -565,12 +569,11 @@ def load_exp_image(img_path):
     return fabio.open(img_path).data
 
 
-def mirror_qmap_positive_qxy_only(qmap, qxy, qz):
-    """tools/comparison.py:268-319: keep the qxy > 0 columns; where the mirrored
-    qxy < 0 column (zero-padded to the positive side's width) is brighter than 1.1x
-    the positive pixel, take it (detector gaps); put the qxy == 0 column first."""
-    qmap = np.asarray(qmap)
-    qxy = np.asarray(qxy)
+def _positive_side_filled(qmap, qxy):
+    """(qxy > 0 columns of qmap, their qxy) with each pixel replaced by its mirrored
+    qxy < 0 partner where that one is brighter than 1.1x (the mirrored side
+    zero-padded to the positive side's width): the detector-gap fill shared by
+    tools/comparison.py:268-319 and 322-378."""
     keep = qxy > 0
     q_pos = qxy[keep]
     m_pos = qmap[:, keep]
@@ -579,11 +582,163 @@ def mirror_qmap_positive_qxy_only(qmap, qxy, qz):
         m_neg = np.pad(m_neg, ((0, 0), (0, m_pos.shape[1] - m_neg.shape[1])), mode='constant', constant_values=0)
     brighter = m_pos * 1.1 < m_neg
     m_pos[brighter] = m_neg[brighter]
+    return m_pos, q_pos
+
+
+def mirror_qmap_positive_qxy_only(qmap, qxy, qz):
+    """tools/comparison.py:268-319: keep the qxy > 0 columns; where the mirrored
+    qxy < 0 column (zero-padded to the positive side's width) is brighter than 1.1x
+    the positive pixel, take it (detector gaps); put the qxy == 0 column first."""
+    qmap = np.asarray(qmap)
+    qxy = np.asarray(qxy)
+    m_pos, q_pos = _positive_side_filled(qmap, qxy)
     if np.any(qxy == 0):
         zero_col = qmap[:, np.flatnonzero(qxy == 0)].reshape(-1, 1)
         m_pos = np.concatenate((zero_col, m_pos), axis=1)
         q_pos = np.concatenate(([0], q_pos))
     return m_pos, q_pos, qz
+
+
+def mirror_qmap_fix_detgap(qmap, qxy, qz):
+    """tools/comparison.py:322-378: the gap-filled qxy > 0 side (as
+    mirror_qmap_positive_qxy_only) reflected across qxy = 0, [-q reversed, q], with the
+    qxy == 0 column, when there is one, appended at the END (qxy 0 last)."""
+    qmap = np.asarray(qmap)
+    qxy = np.asarray(qxy)
+    m_pos, q_pos = _positive_side_filled(qmap, qxy)
+    q_out = np.concatenate((-q_pos[::-1], q_pos))
+    m_out = np.concatenate((m_pos[:, ::-1], m_pos), axis=1)
+    if np.any(qxy == 0):
+        m_out = np.concatenate((m_out, qmap[:, np.flatnonzero(qxy == 0)].reshape(-1, 1)), axis=1)
+        q_out = np.append(q_out, 0)
+    return m_out, q_out, qz
+
+
+def mirror_qmap_positive_qxy_only_old(qmap, qxy, qz):
+    """tools/comparison.py:239-266: the qxy >= 0 columns reflected across qxy = 0
+    ([-q reversed, q]; a qxy == 0 column appears twice), no gap fill."""
+    qmap = np.asarray(qmap)
+    qxy = np.asarray(qxy)
+    keep = qxy >= 0
+    q_pos, m_pos = qxy[keep], qmap[:, keep]
+    return np.concatenate((m_pos[:, ::-1], m_pos), axis=1), np.concatenate((-q_pos[::-1], q_pos)), qz
+
+
+def find_nearest(array, value):
+    """Index of the element of ``array`` closest to ``value`` (first on ties),
+    tools/comparison.py:18-21."""
+    return np.abs(np.asarray(array) - value).argmin()
+
+
+def match_qmap_index(qxy_val, qz_val, qxy, qz):
+    """(qz index, qxy index) of the map pixel closest to (qxy_val, qz_val),
+    tools/comparison.py:23-45."""
+    return np.abs(qz - qz_val).argmin(), np.abs(qxy - qxy_val).argmin()
+
+
+def normalize_qmap(qmap1, qmap2):
+    """qmap1 scaled so its NaN-ignoring sum equals qmap2's, tools/comparison.py:47-55."""
+    return qmap1 * (np.nansum(qmap2) / np.nansum(qmap1))
+
+
+def normalize_qmap_position(qmap1, qmap2, qxy_axis, qz_axis, qxy_pos, qz_pos):
+    """qmap1 scaled so its pixel nearest (qxy_pos, qz_pos) equals qmap2's there (both
+    maps on the same axes), tools/comparison.py:57-71; prints the pixel and values."""
+    ix = np.argmin(np.abs(qxy_axis - qxy_pos))
+    iz = np.argmin(np.abs(qz_axis - qz_pos))
+    v1, v2 = qmap1[iz, ix], qmap2[iz, ix]
+    print(f'qxy_idx={ix}, qz_idx={iz}, qmap1_int={v1}, qmap2_int={v2}')
+    return qmap1 * (v2 / v1)
+
+
+def _wedge(qxy, qz, alpha_i_deg, energy):
+    """Boolean [qz, qxy] map of the missing wedge: |qxy| below the smallest qxy
+    reachable at each qz for incidence angle alpha_i and wavelength 12400 / energy A
+    (exit angle theta = asin(qz wl / 2 pi - sin alpha_i) + alpha_i, qxy_min =
+    |2 pi / wl (cos(theta - alpha_i) - cos alpha_i)|), tools/comparison.py:106-114.
+    A qz the geometry cannot reach (NaN theta) masks nothing."""
+    wl = 12400 / energy
+    a = np.deg2rad(alpha_i_deg)
+    qz = np.asarray(qz, dtype=np.float64)
+    with np.errstate(invalid='ignore'):
+        theta = np.arcsin(qz * wl / (2 * np.pi) - np.sin(a)) + a
+        qxy_min = np.abs((2 * np.pi / wl) * (np.cos(theta - a) - np.cos(a)))
+        return np.abs(np.asarray(qxy))[None, :] < qxy_min[:, None]
+
+
+def mask_forbidden_pixels(image, qxy, qz, alpha_i_deg, energy):
+    """Copy of ``image`` [qz, qxy] with the missing-wedge pixels set to NaN,
+    tools/comparison.py:73-116."""
+    out = np.array(image, copy=True)
+    out[_wedge(qxy, qz, alpha_i_deg, energy)] = np.nan
+    return out
+
+
+def create_wedge_mask(image, qxy, qz, alpha_i_deg, energy):
+    """Integer [qz, qxy] mask, 1 on the missing-wedge pixels and 0 elsewhere,
+    tools/comparison.py:118-159 (``image`` is not used)."""
+    return _wedge(qxy, qz, alpha_i_deg, energy).astype(np.int64)
+
+
+def rebin_and_combine_qmaps(qmap1, qxy1, qz1, qmap2, qxy2, qz2, pos=0):
+    """qmap1 with its columns from qxy = 0 on replaced by qmap2 linearly interpolated
+    onto qmap1's grid (NaN outside qmap2's hull) and scaled: to qmap1 at pos = (qxy, qz)
+    when given, else by qmap1's 15th over the interpolated map's 75th NaN-percentile,
+    tools/comparison.py:193-237.  The interpolation weights are rebin_weights',
+    bit-identical to scipy's griddata 'linear'."""
+    qmap2 = np.asarray(qmap2, dtype=np.float64)
+    row_ptr, col, val, shape = rebin_weights(np.asarray(qxy1), np.asarray(qz1), np.asarray(qxy2), np.asarray(qz2))
+    rebinned = _csr_apply_host(row_ptr, col, val, qmap2.ravel()).reshape(shape)
+    if pos:
+        rebinned = normalize_qmap_position(rebinned, qmap1, qxy1, qz1, pos[0], pos[1])
+    else:
+        rebinned = rebinned * (np.nanpercentile(qmap1, 15) / np.nanpercentile(rebinned, 75))
+    zero = np.abs(qxy1).argmin()
+    out = np.array(qmap1, copy=True)
+    out[:, zero:] = rebinned[:, zero:]
+    return out
+
+
+def _csr_apply_host(row_ptr, col, val, x):
+    """out[r] = 0 + sum over the row's entries in stored order of val * x[col]; NaN for
+    rows without entries (gx_csr_apply's arithmetic on the host)."""
+    n = len(row_ptr) - 1
+    counts = np.diff(row_ptr)
+    out = np.full(n, np.nan)
+    rows = np.flatnonzero(counts)
+    if len(rows) == 0:
+        return out
+    width = int(counts.max())
+    acc = np.zeros(len(rows))
+    for k in range(width):
+        has = counts[rows] > k
+        idx = row_ptr[rows[has]] + k
+        acc[has] = acc[has] + val[idx] * x[col[idx]]
+    out[rows] = acc
+    return out
+
+
+def scale_offset_mask_qmap(qmap, qmap_mask, scale=1, offset=0):
+    """qmap * scale + offset with the pixels where qmap_mask == 1 set to 0,
+    tools/comparison.py:447-467."""
+    return np.where(qmap_mask == 1, 0, qmap * scale + offset)
+
+
+def add_f0_q_dependence(det_img, det_h, det_v, element):
+    """det_img [v, h] multiplied IN PLACE by f0(|q|) / Z of ``element`` (Cromer-Mann
+    a0 e^(-a1 q^2 / 16 pi^2) + ... + a6 e^(-a7 q^2 / 16 pi^2) + a8, terms added in that
+    order, q^2 = qxy^2 + qz^2 at each pixel) and returned, tools/comparison.py:380-404."""
+    a = aff_dict[element]
+    qxy, qz = np.meshgrid(np.asarray(det_h, dtype=np.float64), np.asarray(det_v, dtype=np.float64))
+    q2 = np.sqrt(qxy ** 2 + qz ** 2) ** 2
+    c = 16 * np.pi ** 2
+    fq = a[0] * np.exp(-a[1] * q2 / c)
+    fq = fq + a[2] * np.exp(-a[3] * q2 / c)
+    fq = fq + a[4] * np.exp(-a[5] * q2 / c)
+    fq = fq + a[6] * np.exp(-a[7] * q2 / c)
+    fq = fq + a[8]
+    det_img *= fq / ptable[element]
+    return det_img
 
 
 def evaluate_fit(best_params, fixed_slab_params, fixed_voxelgrid_params, fixed_detectormaker_params,
