@@ -20,7 +20,8 @@ from .._lib import call, dbl_array
 from .ptable_dict import aff_dict, ptable
 
 __all__ = ['downselect_voxelgrid', 'rectangular_collapse_lengths', 'rotate_project_fft_coords',
-           'process_file2', 'add_f0_q_3d', 'generate_voxel_grid_low_mem', 'structure_voxel_grid', 'shell_f0']
+           'process_file2', 'process_file', 'frames_to_iq_parallel', 'add_f0_q_3d', 'generate_voxel_grid_low_mem',
+           'structure_voxel_grid', 'shell_f0']
 
 
 def downselect_voxelgrid(grid, x_axis, y_axis, z_axis, max_val):
@@ -76,6 +77,27 @@ def _attach(name, shape):
     return shm, np.ndarray(shape, dtype=np.float64, buffer=shm.buf)
 
 
+def _bin_frame_dev(iq_2d, det_h_qx, det_h_qy, det_v_qz, qx, qy, qz, gs, gc, status):
+    """Enqueue the binning of one frame into the device sum / count grids gs, gc
+    (flattened [q_num]^3); pixels outside the grid are counted in status."""
+    import torch
+    iq = engine.to_dev(np.asarray(iq_2d, dtype=np.float64), torch.float64)
+    nv, nh = iq.shape
+    hx = engine.to_dev(det_h_qx, torch.float64)
+    hy = engine.to_dev(det_h_qy, torch.float64)
+    vz = engine.to_dev(det_v_qz, torch.float64)
+    qlim = dbl_array([np.min(qx), np.max(qx), np.min(qy), np.max(qy), np.min(qz), np.max(qz)])
+    call("gx_bin_frame_f64", engine.ptr(iq), nv, nh, engine.ptr(hx), engine.ptr(hy),
+         engine.ptr(vz), qlim, float(np.diff(qz)[0]), len(qx), engine.ptr(gs), engine.ptr(gc),
+         engine.ptr(status), engine.stream_handle())
+
+
+def _check_status(status):
+    n = int(status.item())
+    if n:
+        raise IndexError(f"{n} pixel(s) index outside the voxel grid")
+
+
 def process_file2(iq_2d, det_h_qx, det_h_qy, det_v_qz, qx, qy, qz, voxel_grid_shm_name,
                   voxel_grid_count_shm_name):
     """Bin one frame into the named (q_num,)^3 FP64 sum/count grids
@@ -86,27 +108,52 @@ def process_file2(iq_2d, det_h_qx, det_h_qy, det_v_qz, qx, qy, qz, voxel_grid_sh
     s_shm, vg = _attach(voxel_grid_shm_name, shape)
     c_shm, vc = _attach(voxel_grid_count_shm_name, shape)
     try:
-        dev = engine.device()
         gs = engine.to_dev(vg.ravel(), torch.float64)
         gc = engine.to_dev(vc.ravel(), torch.float64)
-        iq = engine.to_dev(np.asarray(iq_2d, dtype=np.float64), torch.float64)
-        nv, nh = iq.shape
-        hx = engine.to_dev(det_h_qx, torch.float64)
-        hy = engine.to_dev(det_h_qy, torch.float64)
-        vz = engine.to_dev(det_v_qz, torch.float64)
-        qlim = dbl_array([np.min(qx), np.max(qx), np.min(qy), np.max(qy), np.min(qz), np.max(qz)])
-        status = torch.zeros(1, dtype=torch.int32, device=dev)
-        call("gx_bin_frame_f64", engine.ptr(iq), nv, nh, engine.ptr(hx), engine.ptr(hy),
-             engine.ptr(vz), qlim, float(np.diff(qz)[0]), q_num, engine.ptr(gs), engine.ptr(gc),
-             engine.ptr(status), engine.stream_handle())
-        if int(status.item()):
-            raise IndexError(f"{int(status.item())} pixel(s) index outside the voxel grid")
+        status = torch.zeros(1, dtype=torch.int32, device=engine.device())
+        _bin_frame_dev(iq_2d, det_h_qx, det_h_qy, det_v_qz, qx, qy, qz, gs, gc, status)
+        _check_status(status)
         vg[...] = gs.cpu().numpy().reshape(shape)
         vc[...] = gc.cpu().numpy().reshape(shape)
         del vg, vc
     finally:
         s_shm.close()
         c_shm.close()
+
+
+def _load_frame(filepath):
+    """(iq, h_qx, h_qy, v_qz) of a frame saved as {filepath}_iq|h_qx|h_qy|v_qz.npy."""
+    return tuple(np.load(f'{filepath}_{k}.npy', allow_pickle=False) for k in ('iq', 'h_qx', 'h_qy', 'v_qz'))
+
+
+def process_file(filepath, q_num, qx, qy, qz, voxel_grid_shm_name, voxel_grid_count_shm_name):
+    """Bin one saved frame ({filepath}_iq.npy, _h_qx.npy, _h_qy.npy, _v_qz.npy) into the
+    named (q_num,)^3 shared grids, as process_file2 bins an in-memory frame
+    (tools/voxelgrids.py:418-462)."""
+    if int(q_num) != len(qx):
+        raise ValueError(f"q_num {q_num} != len(qx) {len(qx)}")
+    process_file2(*_load_frame(filepath), qx, qy, qz, voxel_grid_shm_name, voxel_grid_count_shm_name)
+
+
+def frames_to_iq_parallel(filepaths, q_num, qx, qy, qz):
+    """Mean I(q) [qy, qx, qz] over saved frames (tools/voxelgrids.py:508-533): every frame
+    binned into one pair of device sum / count grids, then sum / count with 0 where no
+    pixel landed.  Frames are binned with FP64 atomics, so the sums' last bits depend on
+    arrival order, as the reference's thread-pool order does."""
+    import torch
+    _same_axes(qx, qy, qz)
+    if int(q_num) != len(qx):
+        raise ValueError(f"q_num {q_num} != len(qx) {len(qx)}")
+    dev = engine.device()
+    n = int(q_num) ** 3
+    gs = torch.zeros(n, dtype=torch.float64, device=dev)
+    gc = torch.zeros(n, dtype=torch.float64, device=dev)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    for fp in filepaths:
+        _bin_frame_dev(*_load_frame(fp), qx, qy, qz, gs, gc, status)
+    _check_status(status)
+    iq = torch.where(gc != 0, gs / torch.where(gc != 0, gc, 1.0), 0.0)
+    return iq.cpu().numpy().reshape((int(q_num),) * 3)
 
 
 def rotate_project_fft_coords(args):

@@ -62,6 +62,44 @@ def test_process_file2():
             s.unlink()
 
 
+def test_frames_from_files(tmp_path):
+    """process_file (tools/voxelgrids.py:418-462) equals process_file2 on the same saved
+    frame, and frames_to_iq_parallel (508-533) over three saved frames equals the
+    oracle's binning of each (sum / count, 0 where no pixel lands)."""
+    rng = np.random.default_rng(4)
+    N = 64
+    q = np.linspace(-1.5, 1.5, 31)
+    paths = []
+    wg = np.zeros((31,) * 3)
+    wc = np.zeros((31,) * 3)
+    for k, phi in enumerate((0.0, 23.0, 61.5)):
+        iq = rng.uniform(0, 100, size=(N, N))
+        hx, hy, vz, _ = O.frame_axes(N, 0.5, phi)
+        base = str(tmp_path / f'frame{k}')
+        for name, arr in (('iq', iq), ('h_qx', hx), ('h_qy', hy), ('v_qz', vz)):
+            np.save(f'{base}_{name}.npy', arr)
+        paths.append(base)
+        O.bin_frame(iq, hx, hy, vz, q, q, q, wg, wc)
+    got = V.frames_to_iq_parallel(paths, 31, q, q, q)
+    want = np.divide(wg, wc, out=np.zeros_like(wg), where=wc != 0)
+    assert np.array_equal(got == 0, want == 0) and (want == 0).any()
+    assert np.allclose(got, want, rtol=1e-13, atol=0)
+    s1, g1 = _shm((31,) * 3)
+    s2, c1 = _shm((31,) * 3)
+    s3, g2 = _shm((31,) * 3)
+    s4, c2 = _shm((31,) * 3)
+    try:
+        V.process_file(paths[1], 31, q, q, q, s1.name, s2.name)
+        V.process_file2(*(np.load(f'{paths[1]}_{n}.npy') for n in ('iq', 'h_qx', 'h_qy', 'v_qz')), q, q, q,
+                        s3.name, s4.name)
+        assert np.array_equal(c1, c2) and np.allclose(g1, g2, rtol=1e-14, atol=0) and c1.sum() > 0
+    finally:
+        del g1, c1, g2, c2
+        for s in (s1, s2, s3, s4):
+            s.close()
+            s.unlink()
+
+
 @pytest.mark.parametrize("fill,smooth", [(False, 0), (True, 2)])
 def test_rotate_project_fft_coords(fill, smooth):
     rng = np.random.default_rng(2)
