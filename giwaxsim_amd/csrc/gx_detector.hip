@@ -125,46 +125,103 @@ __global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
 // one pixel per lane of 64, COOP_NW waves per block.  The waves gather chunks of
 // COOP_K orientations each into LDS (v * w, the same product), then wave 0 adds the
 // chunk in orientation order, so the result is bitwise the one-thread-per-pixel sum
-// while COOP_NW times more gathers are in flight per pixel.
+// while COOP_NW times more gathers are in flight per pixel.  An orientation is the same
+// for the whole wave, so its matrices and weight are scalar loads and their tests scalar
+// compares of bit patterns: rotations that are identities are skipped, and the
+// psi-rotated point is kept from the wave's previous orientation when that one has the
+// bitwise same psi matrix (psi is the outer loop of the orientation product).  Indices
+// are 32-bit (the launch checks cy cx cz < 2^31) and the floor division takes
+// floor_div_q's one-multiply path.
 constexpr int COOP_NW = 8;                  // waves per block
 constexpr int COOP_K = 8;                   // orientations per wave and chunk
 constexpr int64_t COOP_MAX_PIX = 1 << 17;   // above this the per-pixel kernel has the occupancy
+
+// exact floor(a / b) with a one-multiply path: t = a * binv differs from the real
+// quotient by at most |a / b| 2^-52 (< 2^-26 while |t| < 2^26), so when t is more than
+// 2^-24 from an integer (t - floor(t) is exact for t >= 0 and within 2^-53 otherwise),
+// floor(t) is the exact floor; floor_div decides the rest.
+__device__ __forceinline__ double floor_div_q(double a, double b, double binv) {
+  const double t = a * binv;
+  const double q = floor(t);
+  const double f = t - q;
+  if (__builtin_expect(f > 0x1p-24 && f < 1.0 - 0x1p-24 && fabs(t) < 0x1p26, 1)) return q;
+  return floor_div(a, b, binv);
+}
+
+// clip of an integral floor result to [0, hi], as clip_idx of its integer
+__device__ __forceinline__ int clip_q(double q, double hi) { return (int)fmin(fmax(q, 0.0), hi); }
+
+__device__ __forceinline__ unsigned long long dbits(double v) { return (unsigned long long)__double_as_longlong(v); }
+
+// rot_is_identity on bit patterns (+-0 off the diagonal), for uniform matrices
+__device__ __forceinline__ bool rot_is_identity_u(const double *R) {
+  constexpr unsigned long long one = 0x3FF0000000000000ull;
+  const unsigned long long off = dbits(R[1]) | dbits(R[2]) | dbits(R[3]) | dbits(R[5]) | dbits(R[6]) | dbits(R[7]);
+  return dbits(R[0]) == one && dbits(R[4]) == one && dbits(R[8]) == one && (off << 1) == 0;
+}
+
+__device__ __forceinline__ bool same_matrix_u(const double *A, const double *B) {
+  unsigned long long x = 0;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) x |= dbits(A[e]) ^ dbits(B[e]);
+  return x == 0;
+}
+
 template <typename T>
-__global__ __launch_bounds__(COOP_NW *GX_WAVE) void k_detector_coop(gx_detector d) {
+__global__ __launch_bounds__(COOP_NW *GX_WAVE) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_detector_coop(gx_detector d) {
   extern __shared__ __align__(16) unsigned char coop_smem[];
   double *prod = reinterpret_cast<double *>(coop_smem);  // [COOP_NW * COOP_K][64]
-  unsigned char *idf = coop_smem + (size_t)COOP_NW * COOP_K * GX_WAVE * sizeof(double);
-  const int nflag = orient_flags(d, idf);
+  const double *__restrict__ oR = d.orient_R;
+  const double *__restrict__ ow = d.orient_w;
   const T *__restrict__ iq = reinterpret_cast<const T *>(d.iq);
-  const int lane = threadIdx.x & (GX_WAVE - 1), wave = threadIdx.x / GX_WAVE;
+  double *__restrict__ det = d.det;
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / GX_WAVE);
   const int64_t npx = d.n_pix * d.n_pix;
   const int64_t p = blockIdx.x * (int64_t)GX_WAVE + lane;
   const bool valid = p < npx;
   double x0 = 0.0, y0 = 0.0, z0 = 0.0;
   det_pixel(d, valid ? p : 0, x0, y0, z0);
   const double dqinv = 1.0 / d.dq;
+  const double hx = (double)(d.cx - 1), hy = (double)(d.cy - 1), hz = (double)(d.cz - 1);
+  const int cx = (int)d.cx, cz = (int)d.cz;
+  const int64_t n_orient = d.n_orient;
   constexpr int CH = COOP_NW * COOP_K;
   double acc = 0.0;
-  for (int64_t o0 = 0; o0 < d.n_orient; o0 += CH) {
-#pragma unroll
+  double xs = 0.0, ys = 0.0, zs = 0.0;  // pixel after the psi rotation of the previous orientation
+  for (int64_t o0 = 0; o0 < n_orient; o0 += CH) {
+#pragma unroll 2  // two orientations' gathers in flight per wave; 49 VGPRs
     for (int k = 0; k < COOP_K; ++k) {
       const int j = wave * COOP_K + k;
       const int64_t o = o0 + j;
       double pr = 0.0;
-      if (valid && o < d.n_orient) {
-        const double v = (double)iq[det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u)];
-        pr = v * d.orient_w[o];
+      if (o < n_orient) {
+        const double *R = oR + 27 * o;
+        if (k == 0 || !same_matrix_u(R, R - 27)) {  // rotate_about_normal(psi)
+          xs = x0;
+          ys = y0;
+          zs = z0;
+          if (!rot_is_identity_u(R)) rot3(R, xs, ys, zs);
+        }
+        double x = xs, y = ys, z = zs;
+        if (!rot_is_identity_u(R + 9)) rot3(R + 9, x, y, z);    // rotate_about_vertical(phi)
+        if (!rot_is_identity_u(R + 18)) rot3(R + 18, x, y, z);  // rotate_about_horizontal(theta)
+        const int xi = clip_q(floor_div_q(x - d.qx_min, d.dq, dqinv), hx);
+        const int yi = clip_q(floor_div_q(y - d.qy_min, d.dq, dqinv), hy);
+        const int zi = clip_q(floor_div_q(z - d.qz_min, d.dq, dqinv), hz);
+        if (valid) pr = (double)iq[(yi * cx + xi) * cz + zi] * ow[o];
       }
       prod[j * GX_WAVE + lane] = pr;
     }
     __syncthreads();
     if (wave == 0) {
-      const int nj = (int)min((int64_t)CH, d.n_orient - o0);
+      const int nj = (int)min((int64_t)CH, n_orient - o0);
       for (int j = 0; j < nj; ++j) acc = acc + prod[j * GX_WAVE + lane];
     }
     __syncthreads();
   }
-  if (wave == 0 && valid) d.det[p] = acc;
+  if (wave == 0 && valid) det[p] = acc;
 }
 
 // ------------------------------------------------------------------------------------
@@ -637,10 +694,10 @@ int gx_detector_accumulate(const gx_detector *d, gx_stream_t stream) {
                  (d->n_orient == 0 || (d->orient_R && d->orient_w)),
              "a detector buffer is NULL");
   const int64_t npx = d->n_pix * d->n_pix;
-  if (npx <= COOP_MAX_PIX && d->n_orient >= COOP_NW * COOP_K) {
+  if (npx <= COOP_MAX_PIX && d->n_orient >= COOP_NW * COOP_K && d->cy * d->cx * d->cz < (1ll << 31)) {
     // few pixels, many orientations: COOP_NW waves gather for the same 64 pixels
     const unsigned nb = (unsigned)((npx + GX_WAVE - 1) / GX_WAVE);
-    const size_t lds = (size_t)COOP_NW * COOP_K * GX_WAVE * sizeof(double) + orient_flags_lds(d);
+    const size_t lds = (size_t)COOP_NW * COOP_K * GX_WAVE * sizeof(double);
     if (d->iq_f64)
       k_detector_coop<double><<<nb, COOP_NW * GX_WAVE, lds, as_stream(stream)>>>(*d);
     else
