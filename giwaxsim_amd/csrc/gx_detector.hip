@@ -103,39 +103,6 @@ __device__ __forceinline__ int64_t det_voxel(const gx_detector &d, double x0, do
   return (yi * d.cx + xi) * d.cz + zi;
 }
 
-template <typename T>
-__global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
-  extern __shared__ unsigned char idf[];
-  const int nflag = orient_flags(d, idf);
-  const T *__restrict__ iq = reinterpret_cast<const T *>(d.iq);
-  const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (p >= d.n_pix * d.n_pix) return;
-  double x0, y0, z0;
-  det_pixel(d, p, x0, y0, z0);
-  const double dqinv = 1.0 / d.dq;
-  double acc = 0.0;
-  for (int64_t o = 0; o < d.n_orient; ++o) {
-    const double v = (double)iq[det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u)];
-    acc = acc + v * d.orient_w[o];
-  }
-  d.det[p] = acc;
-}
-
-// The same sum for detectors with few pixels and many orientations (a fit's 250^2 x 16k):
-// one pixel per lane of 64, COOP_NW waves per block.  The waves gather chunks of
-// COOP_K orientations each into LDS (v * w, the same product), then wave 0 adds the
-// chunk in orientation order, so the result is bitwise the one-thread-per-pixel sum
-// while COOP_NW times more gathers are in flight per pixel.  An orientation is the same
-// for the whole wave, so its matrices and weight are scalar loads and their tests scalar
-// compares of bit patterns: rotations that are identities are skipped, and the
-// psi-rotated point is kept from the wave's previous orientation when that one has the
-// bitwise same psi matrix (psi is the outer loop of the orientation product).  Indices
-// are 32-bit (the launch checks cy cx cz < 2^31) and the floor division takes
-// floor_div_q's one-multiply path.
-constexpr int COOP_NW = 8;                  // waves per block
-constexpr int COOP_K = 8;                   // orientations per wave and chunk
-constexpr int64_t COOP_MAX_PIX = 1 << 17;   // above this the per-pixel kernel has the occupancy
-
 // exact floor(a / b) with a one-multiply path: t = a * binv differs from the real
 // quotient by at most |a / b| 2^-52 (< 2^-26 while |t| < 2^26), so when t is more than
 // 2^-24 from an integer (t - floor(t) is exact for t >= 0 and within 2^-53 otherwise),
@@ -166,6 +133,57 @@ __device__ __forceinline__ bool same_matrix_u(const double *A, const double *B) 
   for (int e = 0; e < 9; ++e) x |= dbits(A[e]) ^ dbits(B[e]);
   return x == 0;
 }
+
+// One thread per pixel, orientations in (psi outer, phi, theta inner) order with an FP64
+// accumulator; identity rotations from the block's LDS flag table.  I32 selects 32-bit
+// voxel indices (cy cx cz < 2^31) and floor_div_q's one-multiply path.
+template <typename T, bool I32>
+__global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
+  extern __shared__ unsigned char idf[];
+  const int nflag = orient_flags(d, idf);
+  const T *__restrict__ iq = reinterpret_cast<const T *>(d.iq);
+  const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (p >= d.n_pix * d.n_pix) return;
+  double x0, y0, z0;
+  det_pixel(d, p, x0, y0, z0);
+  const double dqinv = 1.0 / d.dq;
+  const double hx = (double)(d.cx - 1), hy = (double)(d.cy - 1), hz = (double)(d.cz - 1);
+  double acc = 0.0;
+  for (int64_t o = 0; o < d.n_orient; ++o) {
+    T v;
+    if (I32) {
+      const unsigned f = o < nflag ? idf[o] : 0u;
+      const double *R = d.orient_R + 27 * o;
+      double x = x0, y = y0, z = z0;
+      if (!(f & 1)) rot3(R, x, y, z);        // rotate_about_normal(psi)
+      if (!(f & 2)) rot3(R + 9, x, y, z);    // rotate_about_vertical(phi)
+      if (!(f & 4)) rot3(R + 18, x, y, z);   // rotate_about_horizontal(theta)
+      const int xi = clip_q(floor_div_q(x - d.qx_min, d.dq, dqinv), hx);
+      const int yi = clip_q(floor_div_q(y - d.qy_min, d.dq, dqinv), hy);
+      const int zi = clip_q(floor_div_q(z - d.qz_min, d.dq, dqinv), hz);
+      v = iq[(yi * (int)d.cx + xi) * (int)d.cz + zi];
+    } else {
+      v = iq[det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u)];
+    }
+    acc = acc + (double)v * d.orient_w[o];
+  }
+  d.det[p] = acc;
+}
+
+// The same sum for detectors with few pixels and many orientations (a fit's 250^2 x 16k):
+// one pixel per lane of 64, COOP_NW waves per block.  The waves gather chunks of
+// COOP_K orientations each into LDS (v * w, the same product), then wave 0 adds the
+// chunk in orientation order, so the result is bitwise the one-thread-per-pixel sum
+// while COOP_NW times more gathers are in flight per pixel.  An orientation is the same
+// for the whole wave, so its matrices and weight are scalar loads and their tests scalar
+// compares of bit patterns: rotations that are identities are skipped, and the
+// psi-rotated point is kept from the wave's previous orientation when that one has the
+// bitwise same psi matrix (psi is the outer loop of the orientation product).  Indices
+// are 32-bit (the launch checks cy cx cz < 2^31) and the floor division takes
+// floor_div_q's one-multiply path.
+constexpr int COOP_NW = 8;                  // waves per block
+constexpr int COOP_K = 8;                   // orientations per wave and chunk
+constexpr int64_t COOP_MAX_PIX = 1 << 17;   // above this the per-pixel kernel has the occupancy
 
 template <typename T>
 __global__ __launch_bounds__(COOP_NW *GX_WAVE) __attribute__((amdgpu_waves_per_eu(8, 8)))
@@ -704,10 +722,19 @@ int gx_detector_accumulate(const gx_detector *d, gx_stream_t stream) {
       k_detector_coop<float><<<nb, COOP_NW * GX_WAVE, lds, as_stream(stream)>>>(*d);
     return check_launch("gx_detector_accumulate");
   }
-  if (d->iq_f64)
-    k_detector<double><<<grid1d(npx, BLK), BLK, orient_flags_lds(d), as_stream(stream)>>>(*d);
-  else
-    k_detector<float><<<grid1d(npx, BLK), BLK, orient_flags_lds(d), as_stream(stream)>>>(*d);
+  const bool i32 = d->cy * d->cx * d->cz < (1ll << 31);
+  const unsigned g = grid1d(npx, BLK);
+  if (d->iq_f64) {
+    if (i32)
+      k_detector<double, true><<<g, BLK, orient_flags_lds(d), as_stream(stream)>>>(*d);
+    else
+      k_detector<double, false><<<g, BLK, orient_flags_lds(d), as_stream(stream)>>>(*d);
+  } else {
+    if (i32)
+      k_detector<float, true><<<g, BLK, orient_flags_lds(d), as_stream(stream)>>>(*d);
+    else
+      k_detector<float, false><<<g, BLK, orient_flags_lds(d), as_stream(stream)>>>(*d);
+  }
   return check_launch("gx_detector_accumulate");
 }
 
