@@ -1061,8 +1061,13 @@ __global__ __launch_bounds__(BLK) void k_frames_spectra_dft(gx_frames d, int64_t
   }
 }
 
+// Waves per SIMD the FP32 chirp-z row pass is compiled for, up to M = 2048: at 4 (128 VGPRs,
+// a 92-byte spill) a fit evaluation at N = 1048 measured ~10% faster than at 3 (155 VGPRs,
+// no spill); FP64 and M = 4096 keep the compiler's choice.
+template <int M, typename C> struct RowBsWaves { static constexpr int value = (sizeof(C) == 8 && M <= 2048) ? 4 : 1; };
 template <int M, typename C>
-__global__ __launch_bounds__(GX_BLK_FFT) void k_rowpass_bs(gx_frames d, int64_t f0) {
+__global__ __launch_bounds__(GX_BLK_FFT) __attribute__((amdgpu_waves_per_eu(RowBsWaves<M, C>::value)))
+void k_rowpass_bs(gx_frames d, int64_t f0) {
   using T = typename Real<C>::T;
   constexpr int P = M / GX_BLK_FFT;
   constexpr int R = BlkPlan<M>::R;
@@ -1106,6 +1111,13 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_rowpass_bs(gx_frames d, int64_t 
     const bool inside = !d.fill_bkg || (z >= zlo && z < zhi);
     const int a0 = d.row_off[z], a1 = d.row_off[z + 1];
     const bool atoms = inside && a1 > a0;  // block-uniform
+    // the tables are the same for every row: an opaque copy of their addresses per row
+    // keeps the compiler from holding all of a thread's table values in registers
+    // across the row loop (204 VGPRs, two waves per SIMD, for M = 2048 in FP32)
+    const C *chirp_r = chirp, *kern_r = reinterpret_cast<const C *>(d.bs_kernel),
+            *tw_r = reinterpret_cast<const C *>(d.bs_twiddle);
+    const double *sy_r = d.col_sy;
+    asm volatile("" : "+s"(chirp_r), "+s"(kern_r), "+s"(tw_r), "+s"(sy_r));
     C v[P];
     if (atoms) {
       for (int y = tid; y < N; y += GX_BLK_FFT) {
@@ -1130,14 +1142,13 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_rowpass_bs(gx_frames d, int64_t 
           const int n = tid + GX_BLK_FFT * t + q * (M / R);
           C x = C{(T)0, (T)0};
           if (n < N) {
-            const T m = d.smooth ? (T)d.col_sy[(int64_t)fb * N + n] : (T)1;
-            x = cmul(C{((T)acc_re[n] * inv) * m, ((T)acc_im[n] * inv) * m}, chirp[n]);
+            const T m = d.smooth ? (T)sy_r[(int64_t)fb * N + n] : (T)1;
+            x = cmul(C{((T)acc_re[n] * inv) * m, ((T)acc_im[n] * inv) * m}, chirp_r[n]);
           }
           v[t * R + q] = x;
         }
       __syncthreads();
-      chirpz_block<M, C>(v, fbuf, reinterpret_cast<const C *>(d.bs_kernel),
-                         reinterpret_cast<const C *>(d.bs_twiddle), tid);
+      chirpz_block<M, C>(v, fbuf, kern_r, tw_r, tid);
     } else {
 #pragma unroll
       for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
@@ -1152,7 +1163,7 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_rowpass_bs(gx_frames d, int64_t 
         const int k = (int)d.bs_rlo + j;   // signed frequency, -N/2 <= k < N - N/2
         const int w = k + N / 2 - jlo;     // fftshifted column within the frame's band
         if (w < 0 || w >= wf) continue;
-        C o = atoms ? cmul(v[t * R + q], chirp[k < 0 ? -k : k]) : C{(T)0, (T)0};
+        C o = atoms ? cmul(v[t * R + q], chirp_r[k < 0 ? -k : k]) : C{(T)0, (T)0};
         if (!inside) {
           o = (k == 0) ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
         } else if (dense) {
