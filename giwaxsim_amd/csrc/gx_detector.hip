@@ -142,8 +142,14 @@ __global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
   extern __shared__ unsigned char idf[];
   const int nflag = orient_flags(d, idf);
   const T *__restrict__ iq = reinterpret_cast<const T *>(d.iq);
-  const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (p >= d.n_pix * d.n_pix) return;
+  // the block's pixels: a 16 x 16 tile, each wave 16 columns x 4 rows of it (fewer distinct
+  // lines per gather than 64 pixels of one row: 2048^2 x 721 6.3 -> 6.1 ms, 512^2 unchanged)
+  const int64_t ntc = (d.n_pix + 15) / 16;
+  const int64_t trow = blockIdx.x / ntc, tcol = blockIdx.x - trow * ntc;
+  const int lane = threadIdx.x & (GX_WAVE - 1), wv = threadIdx.x / GX_WAVE;
+  const int64_t prow = trow * 16 + wv * 4 + lane / 16, pcol = tcol * 16 + lane % 16;
+  if (prow >= d.n_pix || pcol >= d.n_pix) return;
+  const int64_t p = prow * d.n_pix + pcol;
   double x0, y0, z0;
   det_pixel(d, p, x0, y0, z0);
   const double dqinv = 1.0 / d.dq;
@@ -730,7 +736,7 @@ int gx_detector_accumulate(const gx_detector *d, gx_stream_t stream) {
     return check_launch("gx_detector_accumulate");
   }
   const bool i32 = d->cy * d->cx * d->cz < (1ll << 31);
-  const unsigned g = grid1d(npx, BLK);
+  const unsigned g = (unsigned)(((d->n_pix + 15) / 16) * ((d->n_pix + 15) / 16));
   if (d->iq_f64) {
     if (i32)
       k_detector<double, true><<<g, BLK, orient_flags_lds(d), as_stream(stream)>>>(*d);
