@@ -208,7 +208,8 @@ void k_detector_coop(gx_detector d) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / GX_WAVE);
   // the block's 64 pixels: a tile of COOP_TW columns x COOP_TH rows of the detector
   const int64_t ntc = (d.n_pix + COOP_TW - 1) / COOP_TW;
-  const int64_t trow = blockIdx.x / ntc, tcol = blockIdx.x - trow * ntc;
+  const int64_t tb = xcd_tile(blockIdx.x, gridDim.x);  // each XCD a contiguous band of tiles
+  const int64_t trow = tb / ntc, tcol = tb - trow * ntc;
   const int64_t prow = trow * COOP_TH + lane / COOP_TW, pcol = tcol * COOP_TW + lane % COOP_TW;
   const bool valid = prow < d.n_pix && pcol < d.n_pix;
   const int64_t p = valid ? prow * d.n_pix + pcol : 0;
