@@ -43,6 +43,33 @@ def test_detectormaker_bit_exact(n_pix, mirror, init):
     assert np.array_equal(got, want), np.max(np.abs(got - want) / np.abs(want))
 
 
+@pytest.mark.parametrize("n_psi", [16, 65])
+def test_detector_lattice_floor_cases(n_psi):
+    """Pixels and voxel edges on a 0.07 lattice (17 voxels, 49 pixels over the same range):
+    h - q_min and v - q_min are near multiples of dq, and for four of the pixel values
+    a * (1 / dq) rounds across an integer (floor(a * (1 / dq)) != a // dq), so those indices
+    need the exact floor-division path, not the one-multiply one.  psi 0 / 90 / 180 keep the
+    lattice; 16 orientations run the per-pixel kernel, 65 the cooperative one.  The image
+    equals the oracle's bit for bit."""
+    print(f"[gpu-detector] lattice floor cases, {n_psi} orientations", flush=True)
+    q = np.arange(-8, 9) * 0.07
+    mq = float(np.max(q))
+    rng = np.random.default_rng(5)
+    iq = np.exp(rng.normal(size=(len(q),) * 3))
+    psis = np.linspace(0, 180, n_psi)
+    assert 90.0 in psis or n_psi == 16
+    one = np.array([0.0])
+    got, h, v = detectormaker_fitting(iq, q, q, q, 49, mq, (0, 0, 0), ('None',) * 3, psis, None, one,
+                                      None, one, None, mirror=False)
+    want, _, _ = O.detectormaker(iq, q, q, q, 49, mq, (0, 0, 0), ('None',) * 3, psis, None, one, None,
+                                 one, None, mirror=False)
+    # the lattice really produces quotients whose one-multiply floor is off by one
+    a = h - np.min(q)
+    dq = np.diff(q)[0]
+    assert np.sum(np.floor(a * (1.0 / dq)) != np.floor_divide(a, dq)) == 4
+    assert np.array_equal(got, want)
+
+
 def test_weights_from_files(tmp_path):
     iq, q = _iq(1)
     psis = np.linspace(75, 90, 16)
