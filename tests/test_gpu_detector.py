@@ -5,7 +5,8 @@ bit-identical: same rotation matrices, the same FP64 operations per pixel
 summation order.  The bit-exact cases (<= 65^2 pixels, 222 orientations: three full
 64-orientation chunks and a partial one) run the cooperative gather used for small
 detectors with many orientations; the weights case (16 orientations) and the golden
-500^2 image run the one-thread-per-pixel kernel."""
+500^2 image run the one-thread-per-pixel kernel.  A lattice case puts pixel values where the
+one-multiply floor is off by one, for both kernels."""
 
 import numpy as np
 import pytest
