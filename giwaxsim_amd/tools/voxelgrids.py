@@ -81,8 +81,16 @@ def _bin_frame_dev(iq_2d, det_h_qx, det_h_qy, det_v_qz, qx, qy, qz, gs, gc, stat
     """Enqueue the binning of one frame into the device sum / count grids gs, gc
     (flattened [q_num]^3); pixels outside the grid are counted in status."""
     import torch
-    iq = engine.to_dev(np.asarray(iq_2d, dtype=np.float64), torch.float64)
-    nv, nh = iq.shape
+    iq_2d = np.asarray(iq_2d, dtype=np.float64)
+    if iq_2d.ndim != 2:
+        raise ValueError(f"frame image must be 2-D, got shape {iq_2d.shape}")
+    nv, nh = iq_2d.shape
+    if not (np.size(det_h_qx) == nh and np.size(det_h_qy) == nh and np.size(det_v_qz) == nv):
+        raise ValueError(f"frame axes ({np.size(det_h_qx)}, {np.size(det_h_qy)}, {np.size(det_v_qz)}) do not "
+                         f"match the image's {nh} columns and {nv} rows")
+    if len(qx) < 2 or gs.numel() != len(qx) ** 3 or gc.numel() != len(qx) ** 3:
+        raise ValueError("sum / count grids must hold len(qx)^3 voxels")
+    iq = engine.to_dev(iq_2d, torch.float64)
     hx = engine.to_dev(det_h_qx, torch.float64)
     hy = engine.to_dev(det_h_qy, torch.float64)
     vz = engine.to_dev(det_v_qz, torch.float64)

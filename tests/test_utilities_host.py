@@ -8,10 +8,12 @@ closed forms:
 * fft_gaussian (tools/utilities.py:260-274): 1 at q = 0, exp(-q^2 sigma^2 / 2) along
   each axis (the transform of a unit-integral Gaussian), separable, [qy, qx, qz] order;
 * create_shared_array (tools/utilities.py:378-395): a fresh block of prod(shape)
-  float64 values that a second handle of the same name sees.
+  float64 values that a second handle of the same name sees;
+* the saved-frame binning's host checks of frame and grid shapes.
 """
 
 import numpy as np
+import pytest
 
 from giwaxsim_amd.tools.utilities import create_shared_array, fft_gaussian, gaussian_kernel
 
@@ -59,3 +61,20 @@ def test_create_shared_array():
     finally:
         shm.close()
         shm.unlink()
+
+
+def test_frame_binning_rejects_mismatched_shapes():
+    """The saved-frame binning (giwaxsim_amd.tools.voxelgrids, behind process_file,
+    process_file2 and frames_to_iq_parallel) checks the frame's axes against its image
+    and the grids against the q axis on the host, before any device call."""
+    import torch
+    from giwaxsim_amd.tools import voxelgrids as V
+    q = np.linspace(-1, 1, 5)
+    g = torch.zeros(125, dtype=torch.float64)
+    c = torch.zeros(125, dtype=torch.float64)
+    with pytest.raises(ValueError, match='do not match'):
+        V._bin_frame_dev(np.zeros((4, 6)), np.zeros(5), np.zeros(6), np.zeros(4), q, q, q, g, c, None)
+    with pytest.raises(ValueError, match='2-D'):
+        V._bin_frame_dev(np.zeros(6), np.zeros(6), np.zeros(6), np.zeros(4), q, q, q, g, c, None)
+    with pytest.raises(ValueError, match='len\\(qx\\)\\^3'):
+        V._bin_frame_dev(np.zeros((4, 6)), np.zeros(6), np.zeros(6), np.zeros(4), q, q, q, g[:64], c, None)
