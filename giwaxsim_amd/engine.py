@@ -821,7 +821,8 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
     """Full voxel-grid path of one slab (``coords``: host (n, 3), device (n, 3) or a tuple
     of device x, y, z columns).  Returns dict with device iq32/iq64,
     host axes and the objects that produced them.  ``frames=(lo, hi)``
-    restricts the run to phi-frames [lo, hi) (bounded samples).
+    restricts the run to phi-frames [lo, hi) (bounded samples); a list of such
+    windows bins the frames of every window into the same bins.
 
     The device buffers and descriptors (slab, bins, frame runner) are kept per
     (geometry plan, atom and element counts, options, device, stream) and reused
@@ -878,7 +879,11 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
     _WORKSPACES[key] = (slab, bins, runner)
     while len(_WORKSPACES) > _WORKSPACES_MAX:
         _WORKSPACES.pop(next(iter(_WORKSPACES)))
-    runner.run(frames)
+    if frames is not None and len(frames) and not np.isscalar(frames[0]):
+        for window in frames:
+            runner.run(tuple(window))
+    else:
+        runner.run(frames)
     # the finalize is enqueued before the status read-back, so the device does not
     # wait on the host between the frames and the finalize
     iq64, iq32 = bins.finalize(want64=want64, want32=True, f0_element=f0_element)

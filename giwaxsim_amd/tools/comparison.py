@@ -22,21 +22,13 @@ from .ptable_dict import aff_dict, ptable
 from .utilities import calc_real_space_abc, load_pdb, load_xyz
 
 __all__ = ['slabmaker_fitting', 'voxelgridmaker_fitting', 'detectormaker_fitting',
-           'optimize_scale_offset', 'DeviceIQ', 'trim_sim_data', 'rebin_qmap_and_mask',
+           'optimize_scale_offset', 'trim_sim_data', 'rebin_qmap_and_mask',
            'linear_polar', 'polar_linear', 'add_pad', 'shift_peak', 'pad_columns', 'ShiftPeakPlan',
            'RebinWeights', 'rebin_weights', 'FitComparison', 'SlabBuilder', 'mirror_qmap_positive_qxy_only', 'evaluate_fit',
            'make_linecut', 'plot_fit', 'plot_linecuts', 'load_exp_image', 'mirror_qmap_fix_detgap',
            'mirror_qmap_positive_qxy_only_old', 'find_nearest', 'match_qmap_index', 'normalize_qmap',
            'normalize_qmap_position', 'mask_forbidden_pixels', 'create_wedge_mask', 'rebin_and_combine_qmaps',
            'scale_offset_mask_qmap', 'add_f0_q_dependence']
-
-
-class DeviceIQ(np.ndarray):
-    """numpy FP64 iq that also keeps the device copy it was read back from,
-    so ``detectormaker_fitting`` can gather from it without a re-upload."""
-
-    def __array_finalize__(self, obj):
-        self._gx_device = getattr(obj, '_gx_device', None) if obj is not None else None
 
 
 def _load_cell(input_filepath):
@@ -143,9 +135,7 @@ def voxelgridmaker_fitting(coords, elements, r_voxel_size, q_voxel_size, max_q, 
     """
     res = engine.run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy,
                                fill_bkg=fill_bkg, smooth=int(smooth), want64=True)
-    iq = res['iq64'].cpu().numpy().view(DeviceIQ)
-    iq._gx_device = res['iq64']
-    return iq, res['qx'], res['qy'], res['qz']
+    return res['iq64'].cpu().numpy(), res['qx'], res['qy'], res['qz']
 
 
 def _weights(path, angles):
@@ -169,7 +159,8 @@ def detector_image(iq, qx, qy, qz, num_pixels, max_q, angle_init_vals, angle_ini
                    phis, phi_weights_path, thetas, theta_weights_path, mirror, floor_scale=True):
     """The device image of detectormaker_fitting (``floor_scale``: NaN / <= 0 -> 1e-6,
     then * 1e-6) or, without ``floor_scale``, the mirrored sum the legacy
-    detectormaker floors on its own.  Returns (device image [v, h], det_h, det_v)."""
+    detectormaker floors on its own.  ``iq`` is a host array (uploaded on every call) or
+    a device tensor (used as it is).  Returns (device image [v, h], det_h, det_v)."""
     import torch
     psi_w = _weights(psi_weights_path, psis)
     phi_w = _weights(phi_weights_path, phis)
@@ -182,9 +173,9 @@ def detector_image(iq, qx, qy, qz, num_pixels, max_q, angle_init_vals, angle_ini
     assert np.abs(1 - np.sum(th_w)) < 0.01, 'theta weights must sum to 1'
     plan = engine.detector_plan(num_pixels, max_q, angle_init_vals, angle_init_axs, psis, psi_w,
                                 phis, phi_w, thetas, th_w)
-    dev_iq = getattr(iq, '_gx_device', None)
-    if dev_iq is None or tuple(dev_iq.shape) != tuple(np.shape(iq)):
-        dev_iq = engine.to_dev(np.asarray(iq, dtype=np.float64), torch.float64)
+    # a host iq is uploaded on every call: the caller may have edited it or derived it
+    # from an earlier result, so no device copy of an earlier array is reused
+    dev_iq = engine.to_dev(iq if torch.is_tensor(iq) else np.asarray(iq, dtype=np.float64), torch.float64)
     out, _ = engine.run_detector(dev_iq, qx, qy, qz, plan, mirror, floor_scale=floor_scale)
     return out, plan.h, plan.v
 

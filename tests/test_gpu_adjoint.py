@@ -231,3 +231,60 @@ def test_workload_gradients_vs_central_differences():
     assert max(ef) <= 1e-7
     assert abs(fds - s.grad.item()) <= 1e-7 * abs(fds)
     assert abs(fdb - b.grad.item()) <= 1e-7 * abs(fdb)
+
+
+_LARGE_REF = {}
+
+
+def _large_case(N):
+    """A 40 x 40 x 30 A polymer slab (C/H/N/O/S/F) on the production grids: N = 1024
+    (r 0.2 A, max_q 2.0, smooth 10) and N = 2048 (r 0.15 A, max_q 3.0, smooth 20), both
+    with fill_bkg; three phi-frames and a 64^2 detector over psi 0..90 in 15 degree steps.
+    The FP64 autograd reference's image and gradients are computed once per N."""
+    from giwaxsim_amd.synth import polymer_slab
+    r, mq, smooth = {1024: (0.2, 2.0, 10), 2048: (0.15, 3.0, 20)}[N]
+    coords, el = polymer_slab(size=(40.0, 40.0, 30.0), seed=31)
+    qv = 2 * np.pi / (r * N) * (1 + 1e-9)
+    det = dict(num_pixels=64, max_q=mq, psis=list(np.arange(0.0, 91.0, 15.0)), phis=[0.0],
+               thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
+    frames = (100, 103)
+    G = np.random.default_rng(7).normal(size=(64, 64))
+    if N not in _LARGE_REF:
+        _say(f"[gpu-adjoint] FP64 autograd reference N={N}, {len(coords)} atoms, frames {frames}")
+        uniq = sorted(set(el))
+        ids = np.array([uniq.index(e) for e in el])
+        from giwaxsim_amd.engine import element_table
+        _, ftab, _, _ = element_table(el, 12700.0)
+        ref = CicModel(coords, ids, r, qv, mq, det, frames=frames, fill_bkg=True, smooth=smooth)
+        ct = torch.tensor(coords, dtype=torch.float64, requires_grad=True)
+        frt = torch.tensor(ftab.real.copy(), requires_grad=True)
+        fit = torch.tensor(ftab.imag.copy(), requires_grad=True)
+        want = ref(ct, frt, fit)
+        (want * torch.as_tensor(G)).sum().backward()
+        _LARGE_REF[N] = (want.detach().numpy(), ct.grad.numpy(), frt.grad.numpy(), fit.grad.numpy())
+    return coords, el, r, qv, mq, det, frames, smooth, G, _LARGE_REF[N]
+
+
+@pytest.mark.parametrize("N,precision,tol_img,tol_grad", [(1024, 'fp64', 1e-10, 1e-10),
+                                                          (1024, 'fp32', 2e-6, 1e-4),
+                                                          (2048, 'fp64', 1e-10, 1e-10),
+                                                          (2048, 'fp32', 2e-6, 1e-4)])
+def test_production_grids_match_autograd(N, precision, tol_img, tol_grad):
+    coords, el, r, qv, mq, det, frames, smooth, G, (w, gw, gfr, gfi) = _large_case(N)
+    _say(f"[gpu-adjoint] N={N} {precision} fill smooth {smooth}, frames {frames}")
+    model = DifferentiableSlab(el, coords, r, qv, mq, det, frames=frames, precision=precision,
+                               fill_bkg=True, smooth=smooth)
+    assert model.plan.N == N
+    dev = torch.device('cuda')
+    c = torch.tensor(coords, dtype=torch.float64, device=dev, requires_grad=True)
+    fr = model.f_re0.clone().requires_grad_(True)
+    fi = model.f_im0.clone().requires_grad_(True)
+    img = model(c, fr, fi)
+    (img * torch.as_tensor(G, device=dev)).sum().backward()
+    e_img = np.max(np.abs(img.detach().cpu().numpy() - w)) / np.max(np.abs(w))
+    e_c = np.max(np.abs(c.grad.cpu().numpy() - gw)) / np.max(np.abs(gw))
+    e_fr = np.max(np.abs(fr.grad.cpu().numpy() - gfr)) / np.max(np.abs(gfr))
+    e_fi = np.max(np.abs(fi.grad.cpu().numpy() - gfi)) / np.max(np.abs(gfi))
+    _say(f"  image {e_img:.2e}  d/dcoords {e_c:.2e}  d/df_re {e_fr:.2e}  d/df_im {e_fi:.2e}")
+    assert e_img <= tol_img
+    assert e_c <= tol_grad and e_fr <= tol_grad and e_fi <= tol_grad

@@ -25,12 +25,28 @@ def test_collapse_lengths_bit_exact():
         assert np.array_equal(got, O.rectangular_collapse_lengths(x, 100.0, 37.5, phi))
 
 
-def test_add_f0():
+@pytest.mark.parametrize("element", ['C', 'H', 'O', 'S', 'Cl', 'Br', 'Na', 'Pb'])
+def test_add_f0(element):
+    from giwaxsim_amd.tools.ptable_dict import aff_dict, ptable
     rng = np.random.default_rng(0)
     q = np.linspace(-2.1, 2.1, 23)
     iq = rng.uniform(size=(23, 23, 23))
-    assert np.allclose(V.add_f0_q_3d(iq, q, q, q, 'C'), O.add_f0_q_3d(iq, q, q, q), rtol=1e-14,
-                       atol=0)
+    want = O.add_f0_q_3d(iq, q, q, q, aff=aff_dict[element], Z=ptable[element])
+    assert np.allclose(V.add_f0_q_3d(iq, q, q, q, element), want, rtol=1e-14, atol=0)
+
+
+def test_avg_voxel_f_closed_form():
+    """The mean f per voxel of a rectangular lattice of n_x n_y n_z atoms, spacing a:
+    f n_x n_y n_z r^3 / ((n_x - 1)(n_y - 1)(n_z - 1) a^3) (unrotated bounding box)."""
+    nx, ny, nz, a, r = 9, 7, 5, 2.0, 0.25
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing='ij')
+    coords = np.stack([i.ravel() * a, j.ravel() * a, k.ravel() * a], axis=1).astype(np.float64)
+    elements = np.full(len(coords), 'C')
+    res = engine.run_voxelgrid(coords, elements, r, 2 * np.pi / (r * 256) * (1 + 1e-9), 2.0, 12700.0,
+                               fill_bkg=True, frames=(0, 2))
+    f = engine.element_table(['C'], 12700.0)[1][0]
+    want = f * nx * ny * nz * r ** 3 / ((nx - 1) * (ny - 1) * (nz - 1) * a ** 3)
+    assert res['avg'] == pytest.approx(want, rel=1e-13)
 
 
 def _shm(shape):
@@ -191,3 +207,29 @@ def test_simulate_giwaxs_config(tmp_path):
     d = d[np.ix_(np.where(v >= 0)[0], np.where(h >= 0)[0])]
     m = d > 1e-6 * d.max()
     assert np.max(np.abs(det[m] - d[m]) / d[m]) <= 1e-5
+
+
+def test_detector_uses_the_array_it_is_given():
+    """detectormaker_fitting gathers from the iq it is passed, not from a device copy of
+    an earlier result: arrays derived from voxelgridmaker_fitting's iq (scaled, flipped,
+    copied, edited in place) give the oracle's image of that same array, bit for bit."""
+    coords, elements = U.load_xyz(f'{GOLDEN}/graphite_medium.xyz')
+    r, qv, mq, E = 0.25, 2 * np.pi / (0.25 * 256) * (1 + 1e-9), 2.0, 12700.0
+    iq, qx, qy, qz = C.voxelgridmaker_fitting(coords, elements, r, qv, mq, E)
+    psis = np.linspace(0, 180, 13)
+    one = np.array([0.0])
+    args = (qx, qy, qz, 64, 2.0, (0.15, 0, 0), ('psi', 'None', 'None'), psis, None, one, None, one,
+            None)
+
+    def check(a, what):
+        got, _, _ = C.detectormaker_fitting(a, *args, mirror=True)
+        want, _, _ = O.detectormaker(np.asarray(a), *args, mirror=True)
+        assert np.array_equal(got, want), what
+        return got
+
+    base = check(iq, 'as returned')
+    assert not np.array_equal(check(iq * 2, 'iq * 2'), base)
+    assert not np.array_equal(check(np.flip(iq, 0), 'flipped along qy'), base)
+    check(iq.copy(), 'copy')
+    iq[:, :, : iq.shape[2] // 2] = 0.0
+    assert not np.array_equal(check(iq, 'edited in place'), base)
