@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_void_p
 from .build import LIB
 
 GX_OK = 0
-GX_ABI_VERSION = 6   # structure layouts below match this version of the header
+GX_ABI_VERSION = 7   # structure layouts below match this version of the header
 GX_FP_STRIDE = 24
 GX_MOMENTS_WORK = 774
 GX_FIN_MIRROR = 1
@@ -64,6 +64,7 @@ class GxFrames(ctypes.Structure):
         ("row_variant", c_int64),
         ("bs_rlo", c_int64), ("bs_rk", c_int64), ("bs_Mc", c_int64), ("bs_clo", c_int64), ("bs_ck", c_int64),
         ("bs_kernel_c", c_void_p), ("bs_twiddle_c", c_void_p),
+        ("avg_q", c_void_p), ("inter_rm", c_int64),
     ]
 
 
@@ -101,6 +102,7 @@ _I = c_int64
 SIGNATURES = {
     "gx_slab_bounds": [POINTER(GxSlab), _P, _P, _P],
     "gx_slab_sort_rows": [POINTER(GxSlab), _P],
+    "gx_slab_avg": [POINTER(GxSlab), _P, _D, _D, _P, _P],
     "gx_frames_stats": [POINTER(GxFrames), _I, _I, _P],
     "gx_frames_columns": [POINTER(GxFrames), _I, _I, _P],
     "gx_frames_rowpass": [POINTER(GxFrames), _I, _I, _P],
@@ -112,7 +114,7 @@ SIGNATURES = {
     "gx_detector_finalize": [_P, _I, _I, _I, _P, _P],
     "gx_rotate_points": [POINTER(c_double), _P, _P, _P, _I, _P, _P, _P, _P],
     "gx_intersect": [_P, _I, _I, _I, _D, _D, _D, _D, _P, _P, _P, _I, _D, _I, _P, _P],
-    "gx_bin_frame_f64": [_P, _I, _I, _P, _P, _P, POINTER(c_double), _D, _I, _P, _P, _P, _P],
+    "gx_bin_frame_f64": [_P, _I, _I, _P, _P, _P, POINTER(c_double), _D, _I, _P, _P, _P, _I, _P],
     "gx_collapse_lengths": [POINTER(c_double), _P, _I, _P, _P],
     "gx_scale_offset_moments": [_P, _P, _P, _I, _P, POINTER(c_double), _P],
     "gx_slab_tile": [_P, _P, _I, _I, _I, _I, POINTER(c_double), _P, _P, _P, _P, _P],

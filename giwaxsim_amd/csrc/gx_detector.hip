@@ -574,25 +574,93 @@ struct QLim {
   double v[6];
 };
 
+// process_file2's binning (tools/voxelgrids.py:474-503) of one frame into FP64 grids,
+// bit-identical to np.add.at and deterministic.  The horizontal axes are linspaces and
+// the vertical one is sorted, so every bin index is monotone along its axis: the pixels
+// of one bin form a rectangle (a run of rows times a run of columns).  The thread at a
+// rectangle's first row and column adds its pixels to the bin in np.add.at's order
+// (row-major over the masked frame), starting from the grid's current value; no other
+// thread touches that bin in this launch.
+struct ColBin {
+  bool ok;
+  long long x, y;
+};
+__device__ __forceinline__ ColBin col_bin(const double *hqx, const double *hqy, int64_t j, const QLim &L, double dq,
+                                          double dqinv) {
+  const double qx = hqx[j], qy = hqy[j];
+  ColBin c;
+  c.ok = qx <= L.v[1] && qx >= L.v[0] && qy <= L.v[3] && qy >= L.v[2];
+  c.x = c.ok ? to_int(floor_div(qx - L.v[0], dq, dqinv)) : 0;
+  c.y = c.ok ? to_int(floor_div(qy - L.v[2], dq, dqinv)) : 0;
+  return c;
+}
+__device__ __forceinline__ bool row_bin(const double *vqz, int64_t i, const QLim &L, double dq, double dqinv,
+                                        long long *z) {
+  const double qz = vqz[i];
+  if (!(qz <= L.v[5] && qz >= L.v[4])) return false;
+  *z = to_int(floor_div(qz - L.v[4], dq, dqinv));
+  return true;
+}
+
+// Axes that are not monotone (a caller's own arrays): one atomic add per pixel, in no
+// particular order.
+__global__ __launch_bounds__(BLK) void k_bin_frame_f64_any(const double *iq2d, int64_t nv, int64_t nh,
+                                                           const double *hqx, const double *hqy, const double *vqz,
+                                                           QLim L, double dq, int64_t q_num, double *gsum,
+                                                           double *gcnt, int32_t *status) {
+  const double dqinv = 1.0 / dq;
+  for (int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x; p < nv * nh; p += (int64_t)gridDim.x * BLK) {
+    const int64_t i = p / nh, j = p - i * nh;
+    long long zi;
+    if (!row_bin(vqz, i, L, dq, dqinv, &zi)) continue;
+    const ColBin c = col_bin(hqx, hqy, j, L, dq, dqinv);
+    if (!c.ok) continue;
+    if (c.x < 0 || c.x >= q_num || c.y < 0 || c.y >= q_num || zi < 0 || zi >= q_num) {
+      atomicAdd(&status[0], 1);
+      continue;
+    }
+    const int64_t b = (c.y * q_num + c.x) * q_num + zi;
+    atomicAdd(&gsum[b], iq2d[p]);
+    atomicAdd(&gcnt[b], 1.0);
+  }
+}
+
 __global__ __launch_bounds__(BLK) void k_bin_frame_f64(const double *iq2d, int64_t nv, int64_t nh, const double *hqx,
                                                        const double *hqy, const double *vqz, QLim L, double dq,
                                                        int64_t q_num, double *gsum, double *gcnt, int32_t *status) {
   const double dqinv = 1.0 / dq;
   for (int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x; p < nv * nh; p += (int64_t)gridDim.x * BLK) {
     const int64_t i = p / nh, j = p - i * nh;
-    const double qx = hqx[j], qy = hqy[j], qz = vqz[i];
-    if (!(qx <= L.v[1] && qx >= L.v[0] && qy <= L.v[3] && qy >= L.v[2])) continue;
-    if (!(qz <= L.v[5] && qz >= L.v[4])) continue;
-    const long long xi = to_int(floor_div(qx - L.v[0], dq, dqinv));
-    const long long yi = to_int(floor_div(qy - L.v[2], dq, dqinv));
-    const long long zi = to_int(floor_div(qz - L.v[4], dq, dqinv));
-    if (xi < 0 || xi >= q_num || yi < 0 || yi >= q_num || zi < 0 || zi >= q_num) {
+    long long zi, zp;
+    if (!row_bin(vqz, i, L, dq, dqinv, &zi)) continue;
+    if (i > 0 && row_bin(vqz, i - 1, L, dq, dqinv, &zp) && zp == zi) continue;  // not the run's first row
+    const ColBin c = col_bin(hqx, hqy, j, L, dq, dqinv);
+    if (!c.ok) continue;
+    if (j > 0) {
+      const ColBin cp = col_bin(hqx, hqy, j - 1, L, dq, dqinv);
+      if (cp.ok && cp.x == c.x && cp.y == c.y) continue;  // not the run's first column
+    }
+    if (c.x < 0 || c.x >= q_num || c.y < 0 || c.y >= q_num || zi < 0 || zi >= q_num) {
       atomicAdd(&status[0], 1);
       continue;
     }
-    const int64_t b = (yi * q_num + xi) * q_num + zi;
-    atomicAdd(&gsum[b], iq2d[p]);
-    atomicAdd(&gcnt[b], 1.0);
+    int64_t j1 = j + 1;
+    while (j1 < nh) {
+      const ColBin cn = col_bin(hqx, hqy, j1, L, dq, dqinv);
+      if (!cn.ok || cn.x != c.x || cn.y != c.y) break;
+      ++j1;
+    }
+    const int64_t b = (c.y * q_num + c.x) * q_num + zi;
+    double acc = gsum[b];
+    int64_t rows = 0;
+    for (int64_t ii = i; ii < nv; ++ii) {
+      long long zn;
+      if (ii > i && !(row_bin(vqz, ii, L, dq, dqinv, &zn) && zn == zi)) break;
+      for (int64_t jj = j; jj < j1; ++jj) acc = acc + iq2d[ii * nh + jj];
+      ++rows;
+    }
+    gsum[b] = acc;
+    gcnt[b] = gcnt[b] + (double)(rows * (j1 - j));
   }
 }
 
@@ -888,7 +956,7 @@ int gx_intersect(const double *iq, int64_t cy, int64_t cx, int64_t cz, double qx
 
 int gx_bin_frame_f64(const double *iq2d, int64_t nv, int64_t nh, const double *hqx, const double *hqy,
                      const double *vqz, const double *host_qlim, double dq, int64_t q_num, double *grid_sum,
-                     double *grid_cnt, int32_t *status, gx_stream_t stream) {
+                     double *grid_cnt, int32_t *status, int64_t monotone_axes, gx_stream_t stream) {
   clear_error();
   GX_REQUIRE(iq2d && hqx && hqy && vqz && host_qlim && grid_sum && grid_cnt && status, "NULL argument");
   GX_REQUIRE(nv >= 0 && nh >= 0 && q_num >= 1 && dq > 0.0, "bad sizes");
@@ -897,8 +965,12 @@ int gx_bin_frame_f64(const double *iq2d, int64_t nv, int64_t nh, const double *h
   for (int i = 0; i < 6; ++i) L.v[i] = host_qlim[i];
   unsigned g = grid1d(nv * nh, BLK);
   if (g > 16384) g = 16384;
-  k_bin_frame_f64<<<g, BLK, 0, as_stream(stream)>>>(iq2d, nv, nh, hqx, hqy, vqz, L, dq, q_num, grid_sum, grid_cnt,
-                                                     status);
+  if (monotone_axes)
+    k_bin_frame_f64<<<g, BLK, 0, as_stream(stream)>>>(iq2d, nv, nh, hqx, hqy, vqz, L, dq, q_num, grid_sum, grid_cnt,
+                                                       status);
+  else
+    k_bin_frame_f64_any<<<g, BLK, 0, as_stream(stream)>>>(iq2d, nv, nh, hqx, hqy, vqz, L, dq, q_num, grid_sum,
+                                                           grid_cnt, status);
   return check_launch("gx_bin_frame_f64");
 }
 

@@ -269,6 +269,157 @@ __device__ __forceinline__ void fft_wave(C (&v)[N / GX_WAVE], C* lds, const C* _
 }
 
 // ------------------------------------------------------------------------------------
+// Banded variant of fft_wave for the three-stage plans (N = 512, 1024, 2048), used by
+// the forward row and column passes, which keep only the outputs whose signed
+// frequency lies in a band [klo, khi] around 0.
+//
+// * LDS layout.  fft_wave's first exchange stores butterfly j's R0 outputs at
+//   j R0 + r, so the lanes of one store are R0 complex apart and meet in the same
+//   LDS banks.  Here element i of the first exchange sits at i + i / R0 and element i
+//   of the second at i + R0 (i / (R0 R1)): for every store and load of both exchanges
+//   the address is a lane-dependent base plus a compile-time offset (immediate
+//   offsets, as before) and consecutive lanes fall in different banks.  The buffer
+//   holds N + N / min(R0, R1) values (fft_band_buf).  Values and operation order are
+//   fft_wave's, so results are bitwise the same.
+// * Last stage.  Butterfly j produces X[j + r N/RL], r < RL; with the band narrower
+//   than N / RL only r = 0 (k = j) and r = RL - 1 (k = j - N/RL, signed) can be kept.
+//   For each group of 64 butterflies (wave-uniform) the needed outputs are computed:
+//   all of them through Dft<RL> when a middle r is needed, else only r = 0 and / or
+//   RL - 1, with Dft<RL>'s own additions in its own order (bitwise the same values).
+//   Outputs that are not computed are left undefined; callers use only k in the band.
+// ------------------------------------------------------------------------------------
+template <int N> struct FftBand {
+  using PL = FftPlan<N>;
+  static_assert(PL::NST == 3, "banded transforms use the three-stage plans");
+  static constexpr int R0 = PL::R0, R1 = PL::R1, RL = PL::R2;
+  static constexpr int PAD = (N / R0 > N / R1) ? N / R0 : N / R1;
+  static constexpr int BUF = N + PAD;  // complex values per wave
+};
+template <int N> constexpr int fft_band_buf() { return FftBand<N>::BUF; }
+
+// r = 0 and r = R - 1 outputs of Dft<R> (R = 4, 8) in Dft<R>'s operation order
+template <int R, typename C> struct DftEnds;
+template <typename C> struct DftEnds<4, C> {
+  static __device__ __forceinline__ void run(C *v, bool want0, bool wantL) {
+    if (want0) {
+      const C t0 = cadd(v[0], v[2]), t2 = cadd(v[1], v[3]);
+      const C x0 = cadd(t0, t2);
+      if (wantL) {
+        const C t1 = csub(v[0], v[2]), t3 = cmul_mi(csub(v[1], v[3]));
+        v[3] = csub(t1, t3);
+      }
+      v[0] = x0;
+    } else if (wantL) {
+      const C t1 = csub(v[0], v[2]), t3 = cmul_mi(csub(v[1], v[3]));
+      v[3] = csub(t1, t3);
+    }
+  }
+};
+template <typename C> struct DftEnds<8, C> {
+  static __device__ __forceinline__ void run(C *v, bool want0, bool wantL) {
+    using T = typename Real<C>::T;
+    const T c = (T)0.70710678118654752440;
+    // Dft<8>: e = Dft<4>(v0, v2, v4, v6), o = Dft<4>(v1, v3, v5, v7);
+    // out0 = e0 + o0, out7 = e3 - w3 with w3 = (c (o3.y - o3.x), -c (o3.x + o3.y))
+    C e0, o0, e3, o3;
+    if (want0) {
+      e0 = cadd(cadd(v[0], v[4]), cadd(v[2], v[6]));
+      o0 = cadd(cadd(v[1], v[5]), cadd(v[3], v[7]));
+    }
+    if (wantL) {
+      e3 = csub(csub(v[0], v[4]), cmul_mi(csub(v[2], v[6])));
+      o3 = csub(csub(v[1], v[5]), cmul_mi(csub(v[3], v[7])));
+    }
+    if (want0) v[0] = cadd(e0, o0);
+    if (wantL) {
+      const C w3 = C{c * (o3.y - o3.x), -c * (o3.x + o3.y)};
+      v[7] = csub(e3, w3);
+    }
+  }
+};
+
+template <int N, typename C>
+__device__ __forceinline__ void fft_wave_band(C (&v)[N / GX_WAVE], C *lds, const C *__restrict__ tw, int lane,
+                                              int klo, int khi) {
+  constexpr int P = N / GX_WAVE;
+  using FB = FftBand<N>;
+  constexpr int R0 = FB::R0, R1 = FB::R1, RL = FB::RL;
+  constexpr int NB0 = P / R0, NB1 = P / R1, NBL = P / RL;
+  // stage 1 (radix R0, no twiddles); element i -> i + i / R0
+  fft_butterflies<N, R0, 1, P, C>(v, tw, lane);
+#pragma unroll
+  for (int t = 0; t < NB0; ++t) {
+    const int j = lane + GX_WAVE * t;
+    C *p = lds + j * (R0 + 1);
+#pragma unroll
+    for (int r = 0; r < R0; ++r) p[r] = v[t * R0 + r];
+  }
+  wave_sync();
+  {
+    const C *p = lds + lane + lane / R0;
+#pragma unroll
+    for (int t = 0; t < NB1; ++t)
+#pragma unroll
+      for (int r = 0; r < R1; ++r)
+        v[t * R1 + r] = p[(GX_WAVE * t) + (GX_WAVE * t) / R0 + r * (N / R1 + N / (R0 * R1))];
+  }
+  wave_sync();
+  // stage 2 (radix R1, Ns = R0); element i -> i + R0 (i / (R0 R1))
+  fft_butterflies<N, R1, R0, P, C>(v, tw, lane);
+#pragma unroll
+  for (int t = 0; t < NB1; ++t) {
+    const int j = lane + GX_WAVE * t;
+    C *p = lds + (j / R0) * (R0 * R1 + R0) + (j & (R0 - 1));
+#pragma unroll
+    for (int r = 0; r < R1; ++r) p[r * R0] = v[t * R1 + r];
+  }
+  wave_sync();
+  {
+    const C *p = lds + lane;
+#pragma unroll
+    for (int t = 0; t < NBL; ++t)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) v[t * RL + r] = p[GX_WAVE * t + r * (N / RL + R0)];
+  }
+  wave_sync();
+  // stage 3 (radix RL, Ns = R0 R1 = N / RL), outputs pruned to the band
+  constexpr int NS = R0 * R1;
+#pragma unroll
+  for (int t = 0; t < NBL; ++t) {
+    const int j0 = GX_WAVE * t;  // first butterfly of this group (j = j0 .. j0 + 63)
+    // signed k of output r: r = 0: j; r = RL - 1: j - NS; 1 <= r <= RL/2: j + r NS >= j0 + NS;
+    // RL/2 < r <= RL - 2: j + r NS - N <= j0 + 63 - 2 NS
+    const bool want0 = j0 <= khi && j0 + GX_WAVE - 1 >= klo;
+    const bool wantL = j0 - NS <= khi && j0 + GX_WAVE - 1 - NS >= klo;
+    const bool wantMid = khi >= j0 + NS || klo <= j0 + GX_WAVE - 1 - 2 * NS;
+    if (!(want0 || wantL || wantMid)) continue;
+    const int j = lane + j0;
+    const int k = j & (NS - 1);
+#pragma unroll
+    for (int r = 1; r < RL; ++r) v[t * RL + r] = cmul(v[t * RL + r], tw[(r * k) * (N / (NS * RL))]);
+    if (wantMid)
+      Dft<RL, C>::run(&v[t * RL]);
+    else
+      DftEnds<RL, C>::run(&v[t * RL], want0, wantL);
+  }
+}
+
+// Per-wave LDS (complex values) of fft_wave_kept and the transform itself: the banded,
+// padded transform for the three-stage plans, fft_wave (all outputs) for N = 256.
+template <int N> constexpr int fft_kept_buf() {
+  if constexpr (FftPlan<N>::NST == 3) return FftBand<N>::BUF;
+  return N;
+}
+template <int N, typename C>
+__device__ __forceinline__ void fft_wave_kept(C (&v)[N / GX_WAVE], C *lds, const C *__restrict__ tw, int lane,
+                                              int klo, int khi) {
+  if constexpr (FftPlan<N>::NST == 3)
+    fft_wave_band<N, C>(v, lds, tw, lane, klo, khi);
+  else
+    fft_wave<N, C>(v, lds, tw, lane);
+}
+
+// ------------------------------------------------------------------------------------
 // Block-level Stockham transforms of length M over GX_BLK_FFT = 256 threads
 // (P = M / 256 points per thread), used by the chirp-z (Bluestein) path for
 // grid edges that are not powers of two.  The plans have equal first and last
@@ -428,6 +579,17 @@ __device__ __forceinline__ C col_input(const gx_frames &d, const C *col, int z, 
   }
   return C{(T)0, (T)0};
 }
+
+// avg_voxel_f of the deposited f values, when the caller supplies it on the device
+// (gx_frames.avg_q), replaces the descriptor's host value; a kernel that reads the
+// average (directly or through stage_row_band / col_input) resolves it once on entry.
+#define GX_RESOLVE_AVG(d)          \
+  do {                             \
+    if ((d).avg_q) {               \
+      (d).avg_re = (d).avg_q[0];   \
+      (d).avg_im = (d).avg_q[1];   \
+    }                              \
+  } while (0)
 
 // numpy linspace element i of n: i * step + start, the last one exactly stop.
 __device__ __forceinline__ double lin_axis(int64_t i, int64_t n, double start, double step, double stop) {

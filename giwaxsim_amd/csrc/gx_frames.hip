@@ -158,7 +158,16 @@ __global__ __launch_bounds__(1024) void k_slab_scan(gx_slab s) {
     run += rcnt[i];
   }
   if (t == T - 1) s.row_off[M] = (int32_t)part[T - 1];
-  // per-row exponent: largest S with 2^S * sum|f| + cnt/2 <= 2^31 - 1 for re and im
+  // Fixed-point exponent, one for the whole slab: the largest S with
+  // 2^S * sum|f| + cnt/2 <= 2^31 - 1 for re and im in every row.  With one scale an
+  // element's f is rounded to the same value in every row, so the deposit is exactly
+  // that of f' = f + eps_e: for one element a pure scale of the image, and no
+  // row-to-row pattern that interference between rows could amplify (per-row scales
+  // left ~1e-5 relative errors at the weakest voxels of layered slabs).
+  __shared__ int smin;
+  if (t == 0) smin = 60;
+  __syncthreads();
+  int my_min = 60;
   for (int row = t; row < N; row += T) {
     double bound = 0.0;
     int cnt = 0;
@@ -168,14 +177,33 @@ __global__ __launch_bounds__(1024) void k_slab_scan(gx_slab s) {
       bound += (double)c * fmax(fr, fi);
       cnt += c;
     }
-    int S = 0;
     if (bound > 0.0) {
       double room = 2147483647.0 - 0.5 * (double)cnt - 1.0;
-      S = ilogb(room / bound);  // floor(log2(room / bound))
-      if (S > 60) S = 60;
+      const int S = ilogb(room / bound);  // floor(log2(room / bound))
+      my_min = min(my_min, S);
     }
-    s.row_exp[row] = S;
   }
+  atomicMin(&smin, my_min);
+  __syncthreads();
+  for (int row = t; row < N; row += T) s.row_exp[row] = smin;
+}
+
+// avg_voxel_f of the deposited values f'_e = rint(f_e 2^S) 2^-S (one thread; n_elem is
+// small): (sum_e n_e f'_e / volume) * r^3, the reference's operation order
+// (tools/comparison.py:741-744) on the quantised table.
+__global__ void k_slab_avg(const double *__restrict__ f_table, const int32_t *__restrict__ row_exp,
+                           const int64_t *__restrict__ cnt, int64_t n_elem, double volume, double r3,
+                           double *out) {
+  if (threadIdx.x != 0) return;
+  const double sc = ldexp(1.0, row_exp[0]), isc = ldexp(1.0, -row_exp[0]);
+  double re = 0.0, im = 0.0;
+  for (int64_t e = 0; e < n_elem; ++e) {
+    const double n = (double)cnt[e];
+    re = re + n * (rint(f_table[2 * e] * sc) * isc);
+    im = im + n * (rint(f_table[2 * e + 1] * sc) * isc);
+  }
+  out[0] = (re / volume) * r3;
+  out[1] = (im / volume) * r3;
 }
 
 // Placement: every atom of the block takes a rank within its row from the LDS
@@ -609,6 +637,7 @@ __global__ __launch_bounds__(BLK) void k_frames_columns(gx_frames d, int64_t f0,
 // ------------------------------------------------------------------------------------
 template <int N, typename C>
 __global__ __launch_bounds__(GX_WAVE) void k_frames_spectra(gx_frames d, int64_t f0) {
+  GX_RESOLVE_AVG(d);
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
   constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
@@ -669,6 +698,7 @@ constexpr int SCATTER_UNROLL = 4;
 
 template <int N, int RPW, typename C>
 __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_t nf) {
+  GX_RESOLVE_AVG(d);
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
   constexpr int RB = (BLK / GX_WAVE) * RPW;  // rows per block
@@ -677,9 +707,10 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_
   C *smem = reinterpret_cast<C *>(smem_raw);
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
-  C *wbuf = smem + wave * N;
+  constexpr int WB = fft_kept_buf<N>();   // per-wave transform buffer (complex values)
+  C *wbuf = smem + wave * WB;
   constexpr int SP = RB + 1;               // row stride of the staging tile (odd)
-  C *stage = smem + (BLK / GX_WAVE) * N;  // [W][SP]
+  C *stage = smem + (BLK / GX_WAVE) * WB;  // [W][SP]
   C *tspec = stage + d.W * SP;            // [W] background spectrum of this frame
   int32_t *acc_re = reinterpret_cast<int32_t *>(wbuf);
   int32_t *acc_im = acc_re + N;
@@ -782,8 +813,8 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_
         }
       }
       wave_sync();
-      // 4) FFT along y
-      fft_wave<N, C>(v, wbuf, tw, lane);
+      // 4) FFT along y (outputs of the kept band)
+      fft_wave_kept<N, C>(v, wbuf, tw, lane, jlo - N / 2, jhi - N / 2);
     } else {
 #pragma unroll
       for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
@@ -810,10 +841,12 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_
 // ------------------------------------------------------------------------------------
 template <int N, int RB, typename C>
 __global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0, int64_t nf) {
+  GX_RESOLVE_AVG(d);
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
   constexpr int R0 = FftFirst<N>::R;
-  constexpr int ROWB = (N * (int)sizeof(C) > N * 8) ? N * (int)sizeof(C) : N * 8;  // bytes per row
+  constexpr int ROWB = (fft_kept_buf<N>() * (int)sizeof(C) > N * 8) ? fft_kept_buf<N>() * (int)sizeof(C)
+                                                                     : N * 8;  // bytes per row
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int SP = RB + 1;                                        // staging row stride (odd)
   C *stage = reinterpret_cast<C *>(smem_raw + (size_t)RB * ROWB);  // [W][SP]
@@ -901,7 +934,7 @@ __global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0, in
           v[t * R0 + q] = C{((T)acc[y] * inv) * m, ((T)acc[N + y] * inv) * m};
         }
       wave_sync();
-      fft_wave<N, C>(v, reinterpret_cast<C *>(rowp), tw, lane);
+      fft_wave_kept<N, C>(v, reinterpret_cast<C *>(rowp), tw, lane, jlo - N / 2, jhi - N / 2);
     } else {
 #pragma unroll
       for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
@@ -918,13 +951,294 @@ __global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0, in
 }
 
 // ------------------------------------------------------------------------------------
+// row pass, row-major intermediate (N = 512, 1024, 2048; gx_frames.inter_rm = 1)
+//
+// Each wave takes RPWV consecutive z rows of one frame: LDS fixed-point deposit of the
+// row's atoms, first-stage inputs S[y] my[y] (the same arithmetic as k_rowpass), the
+// banded in-wave FFT along y (fft_wave_band: only the kept columns' outputs), and the
+// kept band written straight from registers to inter[frame][z][w] (consecutive lanes,
+// consecutive columns).  No staging tile and no block barrier: the waves of a block
+// are independent.  The background spectrum, the z mask and the rows outside the filled
+// z range are applied by the column pass (k_colpass_rm), with the arithmetic that
+// stage_row_band applies in the column-major layout, so the bins are the same.  Rows
+// inside the filled range without atoms are written as zeros.
+// ------------------------------------------------------------------------------------
+// Rows per wave: more rows amortise the per-frame prologue (the smoothed y mask in
+// registers), but at N = 2048 a loop over rows keeps more than 256 registers live
+// (spills), while one row per wave is straight-line code (measured on the 1.3M-atom
+// slab: 0.61 ms per 119-frame launch with one row per wave, 1.08 ms with two).
+template <int N> struct RowRm {
+  static constexpr int RPWV = (N <= 1024) ? 8 : 1;
+  static constexpr int RB = (BLK / GX_WAVE) * RPWV;        // rows per block
+};
+
+template <int N, typename C, int RPWV, int WPE>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_rowpass_rm(gx_frames d, int64_t f0, int64_t nf) {
+  using T = typename Real<C>::T;
+  constexpr int P = N / GX_WAVE;
+  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  constexpr int BUF = fft_band_buf<N>();
+  constexpr int RB = (BLK / GX_WAVE) * RPWV;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int wave = threadIdx.x / GX_WAVE;
+  C *wbuf = reinterpret_cast<C *>(smem_raw) + wave * BUF;
+  int32_t *acc_re = reinterpret_cast<int32_t *>(wbuf);
+  int32_t *acc_im = acc_re + N;
+
+  // tile order: frame major, or XCD-aware with the frames of a row block together (bit 2)
+  const int64_t nzb = gridDim.x / nf;
+  int fb, zb;
+  if (d.row_variant & 4) {
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    fb = (int)(tile % nf);
+    zb = (int)(tile / nf);
+  } else {
+    fb = (int)(blockIdx.x / nzb);
+    zb = (int)(blockIdx.x % nzb);
+  }
+  const int64_t f = f0 + fb;
+  const double *fp = d.fparams + f * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = jhi - jlo + 1;
+  if (wf <= 0) return;  // block-uniform
+  if (wf > d.W) {       // planner guarantees wf <= W; never write past a row
+    if (threadIdx.x == 0 && zb == 0) atomicAdd(&d.status[1], 1);
+    return;
+  }
+  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;  // keep ya <= y < yb
+  const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
+  const double r = d.r, rinv = 1.0 / d.r;
+  const C *tw = reinterpret_cast<const C *>(d.twiddle);
+  const int klo = jlo - N / 2, khi = jhi - N / 2;  // kept signed frequencies
+  C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.Z * d.W;
+
+  T my[P];
+#pragma unroll
+  for (int t = 0; t < P / R0; ++t)
+#pragma unroll
+    for (int q = 0; q < R0; ++q)
+      my[t * R0 + q] = d.smooth ? (T)d.col_sy[(int64_t)fb * N + lane + GX_WAVE * t + q * (N / R0)] : (T)1;
+
+  const int zw = zb * RB + wave * RPWV;
+  for (int rr = 0; rr < RPWV; ++rr) {
+    const int z = zw + rr;
+    if (z >= d.Z) break;
+    if (d.fill_bkg && !(z >= zlo && z < zhi)) continue;  // synthesised by the column pass
+    C *orow = out + (int64_t)z * d.W;
+    const int a0 = d.row_off[z], a1 = d.row_off[z + 1];
+    if (a1 <= a0) {
+      for (int w = lane; w < wf; w += GX_WAVE) orow[w] = C{(T)0, (T)0};
+      continue;
+    }
+    // 1) zero the fixed-point accumulators (2N int32, 16 bytes per store)
+#pragma unroll
+    for (int i = 0; i < (2 * N) / (4 * GX_WAVE); ++i)
+      reinterpret_cast<int4 *>(acc_re)[lane + GX_WAVE * i] = make_int4(0, 0, 0, 0);
+    wave_sync();
+    // 2) scatter this row's atoms (np.add.at(grid, (z, y), f), voxelgrids.py:341)
+    for (int a = a0 + lane; a < a1; a += GX_WAVE * SCATTER_UNROLL) {
+      double xs[SCATTER_UNROLL], ys[SCATTER_UNROLL];
+      int2 qs[SCATTER_UNROLL];
+#pragma unroll
+      for (int u = 0; u < SCATTER_UNROLL; ++u) {
+        const int au = a + u * GX_WAVE;
+        const bool ok = au < a1;
+        xs[u] = ok ? d.ax[au] : 0.0;
+        ys[u] = ok ? d.ay[au] : 0.0;
+        qs[u] = ok ? reinterpret_cast<const int2 *>(d.aq)[au] : make_int2(0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < SCATTER_UNROLL; ++u) {
+        if (a + u * GX_WAVE >= a1) break;
+        const double yy = (xs[u] * s + ys[u] * c) - ymin;
+        const long long yi = to_int(floor_div(yy, r, rinv));
+        if (yi >= ya && yi < yb) {
+          atomicAdd(&acc_re[yi], qs[u].x);
+          atomicAdd(&acc_im[yi], qs[u].y);
+        }
+      }
+    }
+    wave_sync();
+    // 3) first-stage inputs S[y] my[y], exact fixed point -> T
+    const T inv = (T)ldexp(1.0, -d.row_exp[z]);
+    C v[P];
+#pragma unroll
+    for (int t = 0; t < P / R0; ++t)
+#pragma unroll
+      for (int q = 0; q < R0; ++q) {
+        const int y = lane + GX_WAVE * t + q * (N / R0);
+        const T m = my[t * R0 + q];
+        v[t * R0 + q] = C{((T)acc_re[y] * inv) * m, ((T)acc_im[y] * inv) * m};
+      }
+    wave_sync();
+    // 4) FFT along y, outputs of the kept band only
+    fft_wave_band<N, C>(v, wbuf, tw, lane, klo, khi);
+    // 5) kept band, row-major
+#pragma unroll
+    for (int t = 0; t < P / RL; ++t)
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        const int k = lane + GX_WAVE * t + q * (N / RL);
+        const int w = ((k + N / 2) & (N - 1)) - jlo;
+        if (w >= 0 && w < wf) orow[w] = v[t * RL + q];
+      }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// column pass over the row-major intermediate (gx_frames.inter_rm = 1)
+//
+// A block of WAVES waves takes WAVES consecutive kept columns of one frame.  Phase 1:
+// the block reads the [Z][WAVES] tile of the row pass's output (WAVES consecutive
+// complex values per row) and turns each element into the column transform's input:
+// rows outside the filled z range are the constant A N at the k = 0 column (else 0);
+// inside, with fill or smoothing, mz[z] (raw + U[w]) (+ A N at k = 0 when smoothing) --
+// stage_row_band's arithmetic.  Phase 2: each wave loads its column (rows >= Z from
+// col_input), the tile's LDS becomes the waves' FFT buffers, and the banded FFT along z
+// is followed by |F|^2 and the binning of k_colpass.
+// ------------------------------------------------------------------------------------
+template <int N, typename C> struct ColRm {
+  static constexpr int WAVES = (8 * fft_band_buf<N>() * (int)sizeof(C) <= 80 * 1024) ? 8 : 4;
+};
+
+template <int N, typename C, int WAVES>
+__global__ __launch_bounds__(WAVES * GX_WAVE) void k_colpass_rm(gx_frames d, int64_t f0) {
+  GX_RESOLVE_AVG(d);
+  using T = typename Real<C>::T;
+  constexpr int P = N / GX_WAVE;
+  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  constexpr int BUF = fft_band_buf<N>();
+  constexpr int TS = WAVES + 1;  // tile row stride (odd: a wave's column reads spread over banks)
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *tile = reinterpret_cast<C *>(smem_raw);
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int wave = threadIdx.x / GX_WAVE;
+  // work item = (frame, column group), numbered frame-major and handed to the XCDs in
+  // contiguous ranges: the groups of a frame share the 128-byte lines of each row of the
+  // intermediate, so they run on one XCD and read each line into one L2
+  const int ng = (int)((d.W + WAVES - 1) / WAVES);
+  const int64_t item = xcd_tile(blockIdx.x, gridDim.x);
+  const int fb = (int)(item / ng);
+  const int64_t f = f0 + fb;
+  const double *fp = d.fparams + f * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = jhi - jlo + 1;
+  const int w0 = (int)(item - (int64_t)fb * ng) * WAVES;
+  if (wf <= 0 || wf > d.W || w0 >= wf) return;  // block-uniform, before any barrier
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int zlo = bnd[2], zhi = bnd[3];
+  const bool dense = d.fill_bkg || d.smooth;
+  const C a0 = C{(T)(d.avg_re * d.max_voxels * (double)N), (T)(d.avg_im * d.max_voxels * (double)N)};
+  const C *spec = reinterpret_cast<const C *>(d.col_spec) + (int64_t)fb * d.W;
+  const double *sz_row = d.col_sz + (int64_t)fb * N;
+  const C *src = reinterpret_cast<const C *>(d.inter) + (int64_t)fb * d.Z * d.W + w0;
+
+  // phase 1: the tile of column inputs for rows z < Z.  Thread t takes column t % WAVES
+  // and rows t / WAVES + 64 i; every load is issued before the first is used.
+  {
+    constexpr int NIT = N / GX_WAVE;  // Z <= N
+    const int cl = threadIdx.x % WAVES, zt = threadIdx.x / WAVES;
+    const int w = w0 + cl;
+    const bool kcol = w < wf;
+    const bool k0 = kcol && jlo + w == N / 2;
+    const C outside = k0 ? a0 : C{(T)0, (T)0};
+    const C u = (kcol && dense) ? spec[w] : C{(T)0, (T)0};
+    constexpr int CH = NIT < 8 ? NIT : 8;  // rows in flight per thread
+    for (int i0 = 0; i0 < NIT && zt + GX_WAVE * i0 < d.Z; i0 += CH) {
+      C x[CH];
+      double mz[CH];
+#pragma unroll
+      for (int ii = 0; ii < CH; ++ii) {
+        const int z = zt + GX_WAVE * (i0 + ii);
+        const bool ld = kcol && z < d.Z && (!d.fill_bkg || (z >= zlo && z < zhi));
+        x[ii] = ld ? src[(int64_t)z * d.W + cl] : outside;
+        mz[ii] = (ld && d.smooth) ? sz_row[z] : 1.0;
+      }
+#pragma unroll
+      for (int ii = 0; ii < CH; ++ii) {
+        const int z = zt + GX_WAVE * (i0 + ii);
+        if (z >= d.Z) break;
+        const bool inside = kcol && (!d.fill_bkg || (z >= zlo && z < zhi));
+        if (inside && dense) {
+          const T mzt = (T)mz[ii];
+          T ore = mzt * (x[ii].x + u.x);
+          T oim = mzt * (x[ii].y + u.y);
+          if (d.smooth && k0) {
+            ore = ore + a0.x;
+            oim = oim + a0.y;
+          }
+          x[ii] = C{ore, oim};
+        }
+        tile[z * TS + cl] = x[ii];
+      }
+    }
+  }
+  __syncthreads();
+  // phase 2: this wave's column
+  const int wcol = w0 + wave;
+  const bool active = wcol < wf;
+  const int j = jlo + wcol;
+  const bool k0 = (j == N / 2);
+  C u = C{(T)0, (T)0};
+  if (active && !d.fill_bkg && d.smooth) u = spec[wcol];
+  C v[P];
+#pragma unroll
+  for (int t = 0; t < P / R0; ++t)
+#pragma unroll
+    for (int q = 0; q < R0; ++q) {
+      const int z = lane + GX_WAVE * t + q * (N / R0);
+      v[t * R0 + q] = (z < d.Z) ? tile[z * TS + wave] : col_input(d, (const C *)nullptr, z, k0, u, sz_row);
+    }
+  __syncthreads();  // the tile is read; its LDS becomes the FFT buffers
+  if (!active) return;
+  // det_h_qx / det_h_qy of this column (numpy linspace: j*step + start, last = stop)
+  const double hx = (j == N - 1) ? fp[GX_FP_HX_STOP] : (double)j * fp[GX_FP_HX_STEP] + fp[GX_FP_HX_START];
+  const double hy = (j == N - 1) ? fp[GX_FP_HY_STOP] : (double)j * fp[GX_FP_HY_STEP] + fp[GX_FP_HY_START];
+  const double dqinv = 1.0 / d.dq;
+  const long long bx = to_int(floor_div(hx - d.qmin, d.dq, dqinv)) - d.c_lo;
+  const long long by = to_int(floor_div(hy - d.qmin, d.dq, dqinv)) - d.c_lo;
+  if (bx < 0 || bx >= d.c_n || by < 0 || by >= d.c_n) return;
+  const C *tw = reinterpret_cast<const C *>(d.twiddle);
+  fft_wave_band<N, C>(v, reinterpret_cast<C *>(smem_raw) + wave * BUF, tw, lane, (int)d.i_lo - N / 2,
+                      (int)d.i_hi - N / 2);
+
+  const int64_t cbase = (by * d.c_n + bx) * d.c_n;
+  const double val = d.q_val;
+#pragma unroll
+  for (int t = 0; t < P / RL; ++t) {
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int k = lane + GX_WAVE * t + q * (N / RL);
+      const int i = (k + N / 2) & (N - 1);
+      if (i < d.i_lo || i > d.i_hi) continue;
+      // det_v_qz[i] = fftshift(fftfreq(N, r) * 2 * pi)[i]
+      const int kint = (k < N / 2) ? k : k - N;
+      const double qz = (((double)kint * val) * 2.0) * M_PI;
+      const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
+      if (bz < 0 || bz >= d.c_n) continue;
+      const double I = (double)cnorm(v[t * RL + q]);
+      const unsigned long long fx = (unsigned long long)llrint(I * d.bin_scale);
+      atomicAdd(&d.bin_sum[cbase + bz], fx);
+      atomicAdd(&d.bin_cnt[cbase + bz], 1);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // column pass
 // ------------------------------------------------------------------------------------
 template <int N, typename C>
 __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
+  GX_RESOLVE_AVG(d);
   constexpr int P = N / GX_WAVE;
   constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
-  __shared__ C lds[BLK / GX_WAVE][N];
+  __shared__ C lds[BLK / GX_WAVE][fft_kept_buf<N>()];
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
   const int fb = blockIdx.y;
@@ -954,7 +1268,7 @@ __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
 #pragma unroll
     for (int q = 0; q < R0; ++q) v[t * R0 + q] = col_input(d, col, lane + GX_WAVE * t + q * (N / R0), k0, u, sz_row);
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
-  fft_wave<N, C>(v, lds[wave], tw, lane);
+  fft_wave_kept<N, C>(v, lds[wave], tw, lane, (int)d.i_lo - N / 2, (int)d.i_hi - N / 2);
 
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;
   const double val = d.q_val;
@@ -1011,6 +1325,7 @@ __device__ __forceinline__ void chirpz_block(C (&v)[M / GX_BLK_FFT], C *lds, con
 
 // per-frame background spectrum on the kept columns by a direct FP64 DFT
 __global__ __launch_bounds__(BLK) void k_frames_spectra_dft(gx_frames d, int64_t f0) {
+  GX_RESOLVE_AVG(d);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double2 *u = reinterpret_cast<double2 *>(smem_raw);  // [N]
   const int N = (int)d.N;
@@ -1068,6 +1383,7 @@ template <int M, typename C> struct RowBsWaves { static constexpr int value = (s
 template <int M, typename C>
 __global__ __launch_bounds__(GX_BLK_FFT) __attribute__((amdgpu_waves_per_eu(RowBsWaves<M, C>::value)))
 void k_rowpass_bs(gx_frames d, int64_t f0) {
+  GX_RESOLVE_AVG(d);
   using T = typename Real<C>::T;
   constexpr int P = M / GX_BLK_FFT;
   constexpr int R = BlkPlan<M>::R;
@@ -1189,6 +1505,7 @@ void k_rowpass_bs(gx_frames d, int64_t f0) {
 
 template <int M, typename C>
 __global__ __launch_bounds__(GX_BLK_FFT) void k_colpass_bs(gx_frames d, int64_t f0) {
+  GX_RESOLVE_AVG(d);
   using T = typename Real<C>::T;
   constexpr int P = M / GX_BLK_FFT;
   constexpr int R = BlkPlan<M>::R;
@@ -1327,7 +1644,7 @@ template <int N, typename C>
 int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int RPW = (N <= 1024) ? 2 : 1;
   constexpr int RB = (BLK / GX_WAVE) * RPW;
-  size_t lds = ((size_t)(BLK / GX_WAVE) * N + (size_t)d->W * (RB + 2)) * sizeof(C);
+  size_t lds = ((size_t)(BLK / GX_WAVE) * fft_kept_buf<N>() + (size_t)d->W * (RB + 2)) * sizeof(C);
   if (lds > 160 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "row pass needs %zu bytes of LDS (N=%d, W=%lld, %s)", lds, N,
                      (long long)d->W, sizeof(C) == 16 ? "FP64" : "FP32");
@@ -1344,7 +1661,7 @@ int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st)
 
 template <int N, int RB, typename C>
 int launch_rowpass_blk_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
-  constexpr int ROWB = (N * (int)sizeof(C) > N * 8) ? N * (int)sizeof(C) : N * 8;
+  constexpr int ROWB = (fft_kept_buf<N>() * (int)sizeof(C) > N * 8) ? fft_kept_buf<N>() * (int)sizeof(C) : N * 8;
   const size_t lds = (size_t)RB * ROWB + (size_t)d->W * (RB + 2) * sizeof(C);
   if (lds > 160 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "block-wide row pass needs %zu bytes of LDS (N=%d, RB=%d)", lds, N, RB);
@@ -1370,14 +1687,70 @@ int launch_rowpass_v(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st)
   }
 }
 
+template <int N, typename C, int RPWV, int WPE>
+int launch_rowpass_rm_x(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  constexpr int RB = (BLK / GX_WAVE) * RPWV;
+  const size_t lds = (size_t)(BLK / GX_WAVE) * fft_band_buf<N>() * sizeof(C);
+  static bool attr_set = false;
+  if (!attr_set) {
+    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass_rm<N, C, RPWV, WPE>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_set = true;
+  }
+  const unsigned nblk = (unsigned)(((d->Z + RB - 1) / RB) * nf);
+  k_rowpass_rm<N, C, RPWV, WPE><<<nblk, BLK, lds, st>>>(*d, f0, nf);
+  return check_launch("k_rowpass_rm");
+}
+
+template <int N, typename C>
+int launch_rowpass_rm_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  return launch_rowpass_rm_x<N, C, RowRm<N>::RPWV, 1>(d, f0, nf, st);
+}
+
+template <int N, typename C, int WAVES>
+int launch_colpass_rm_w(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  const size_t tile = (size_t)d->Z * (WAVES + 1) * sizeof(C);
+  const size_t bufs = (size_t)WAVES * fft_band_buf<N>() * sizeof(C);
+  const size_t lds = tile > bufs ? tile : bufs;
+  if (lds > 160 * 1024)
+    return set_error(GX_ERR_UNSUPPORTED, "column pass needs %zu bytes of LDS (N=%d, Z=%lld)", lds, N,
+                     (long long)d->Z);
+  static bool attr_set = false;
+  if (!attr_set) {
+    GX_HIP(hipFuncSetAttribute((const void *)k_colpass_rm<N, C, WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               160 * 1024));
+    attr_set = true;
+  }
+  const unsigned nblk = (unsigned)(grid1d(d->W, WAVES) * nf);
+  k_colpass_rm<N, C, WAVES><<<nblk, WAVES * GX_WAVE, lds, st>>>(*d, f0);
+  return check_launch("k_colpass_rm");
+}
+
+template <int N, typename C>
+int launch_colpass_rm_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  return launch_colpass_rm_w<N, C, ColRm<N, C>::WAVES>(d, f0, nf, st);
+}
+
 template <int N>
 int launch_rowpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  if constexpr (N != 256) {
+    if (d->inter_rm) {
+      if (d->fft_f64) return launch_rowpass_rm_t<N, double2>(d, f0, nf, st);
+      return launch_rowpass_rm_t<N, float2>(d, f0, nf, st);
+    }
+  }
   if (d->fft_f64) return launch_rowpass_v<N, double2>(d, f0, nf, st);
   return launch_rowpass_v<N, float2>(d, f0, nf, st);
 }
 
 template <int N>
 int launch_colpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  if constexpr (N != 256) {
+    if (d->inter_rm) {
+      if (d->fft_f64) return launch_colpass_rm_t<N, double2>(d, f0, nf, st);
+      return launch_colpass_rm_t<N, float2>(d, f0, nf, st);
+    }
+  }
   dim3 grid(grid1d(d->W, BLK / GX_WAVE), (unsigned)nf);
   if (d->fft_f64)
     k_colpass<N, double2><<<grid, BLK, 0, st>>>(*d, f0);
@@ -1428,6 +1801,9 @@ int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
   GX_REQUIRE(d->c_n >= 1, "empty cropped grid");
   GX_REQUIRE(d->row_variant >= 0 && d->row_variant <= 6 && (d->row_variant & 3) != 3,
              "row_variant=%lld is not one of 0, 1, 2, 4, 5, 6", (long long)d->row_variant);
+  GX_REQUIRE(!d->inter_rm || ((d->N == 512 || d->N == 1024 || d->N == 2048) && (d->row_variant & 3) == 0),
+             "the row-major intermediate needs N = 512, 1024 or 2048 and the per-wave row shape "
+             "(N=%lld, row_variant=%lld)", (long long)d->N, (long long)d->row_variant);
   GX_REQUIRE(d->fparams && d->ymin_key && d->idx_bounds && d->inter && d->bin_sum && d->bin_cnt && d->twiddle &&
                  d->status && d->row_off && d->row_exp,
              "a required device buffer of gx_frames is NULL");
@@ -1497,6 +1873,16 @@ int gx_slab_sort_rows(const gx_slab *s, gx_stream_t stream) {
     k_slab_scatter<<<g, BLK, 2 * (size_t)(s->N + 1) * sizeof(int), st>>>(*s);
   }
   return check_launch("gx_slab_sort_rows");
+}
+
+int gx_slab_avg(const gx_slab *s, const int64_t *elem_count, double volume, double r3, double *out,
+                gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(s != nullptr && s->f_table && s->row_exp && elem_count && out, "NULL argument");
+  GX_REQUIRE(s->n_elem >= 1 && s->n_elem <= 128, "n_elem=%lld outside [1, 128]", (long long)s->n_elem);
+  GX_REQUIRE(volume > 0.0 && r3 > 0.0, "volume and r^3 must be positive");
+  k_slab_avg<<<1, GX_WAVE, 0, as_stream(stream)>>>(s->f_table, s->row_exp, elem_count, s->n_elem, volume, r3, out);
+  return check_launch("gx_slab_avg");
 }
 
 int gx_frames_stats(const gx_frames *d, int64_t f0, int64_t nf, gx_stream_t stream) {

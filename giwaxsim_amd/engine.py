@@ -689,7 +689,10 @@ class FrameRunner:
     """Runs every phi-frame of one slab into shared bins, in batches."""
 
     def __init__(self, plan: FramePlan, slab: Slab, bins: Bins, fill_bkg, smooth, avg,
-                 batch=None, precision='auto', variant=None):
+                 batch=None, precision='auto', variant=None, counts=None, volume=None):
+        """``counts`` (atoms per element, all atoms) and ``volume`` (x_b y_b z_b): the
+        fill and smoothing terms then use avg_voxel_f of the f values the fixed-point
+        deposit adds (gx_slab_avg, computed on the device) instead of ``avg``."""
         dev = device()
         N, W, Z = plan.N, plan.W, plan.Z
         self.plan, self.slab, self.bins = plan, slab, bins
@@ -757,14 +760,30 @@ class FrameRunner:
             col_sz=ptr(self.col_sz).value, inter=ptr(self.inter).value,
             col_spec=ptr(self.col_spec).value,
             bin_sum=ptr(bins.sum).value, bin_cnt=ptr(bins.cnt).value, status=ptr(self.status).value,
-            row_variant=0 if variant is None else int(variant))
+            row_variant=0)
+        set_row_variant(self.desc, 0 if variant is None else int(variant))
+        self.avg_q = _empty(2, torch.float64)
+        if counts is not None:
+            self._deposited_avg(counts, volume)
         if variant is None and is_pow2_grid(N):
-            self.desc.row_variant = tuned_variant(self)
+            set_row_variant(self.desc, tuned_variant(self))
 
-    def rebind(self, avg):
+    def _deposited_avg(self, counts, volume):
+        """Enqueue gx_slab_avg (after the slab's regrouping) into ``avg_q`` and point the
+        descriptor at it."""
+        counts = np.ascontiguousarray(np.asarray(counts, dtype=np.int64))
+        cdev = dev_const(("elem_counts", counts.tobytes()), lambda: counts, torch.int64)
+        call("gx_slab_avg", ctypes.byref(self.slab.desc), ptr(cdev), float(volume), float(self.plan.r ** 3),
+             ptr(self.avg_q), stream_handle())
+        self.desc.avg_q = ptr(self.avg_q).value
+
+    def rebind(self, avg, counts=None, volume=None):
         """New avg_voxel_f and the bins' current scale, status cleared: the runner
         is ready for a slab that was re-bound to new atoms of the same geometry."""
         self.desc.avg_re, self.desc.avg_im = float(np.real(avg)), float(np.imag(avg))
+        self.desc.avg_q = None
+        if counts is not None:
+            self._deposited_avg(counts, volume)
         self.desc.bin_scale = self.bins.scale
         self.status.zero_()
 
@@ -868,14 +887,14 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
         bins = Bins(plan.grid, scale)
         slab = Slab(None, None, f_table, plan.N, r_voxel_size, zmin, columns=columns)
         runner = FrameRunner(plan, slab, bins, fill_bkg, smooth, avg, batch=batch,
-                             precision=precision, variant=variant)
+                             precision=precision, variant=variant, counts=counts, volume=volume)
     else:
         slab, bins, runner = ws
         bins.scale = float(scale)
         bins.clear()
         slab.bind(columns, f_table, zmin)
         slab.sort()
-        runner.rebind(avg)
+        runner.rebind(avg, counts=counts, volume=volume)
     _WORKSPACES[key] = (slab, bins, runner)
     while len(_WORKSPACES) > _WORKSPACES_MAX:
         _WORKSPACES.pop(next(iter(_WORKSPACES)))
@@ -1050,16 +1069,16 @@ class Ensemble:
             B = intensity_bound(float(np.sum(counts * np.abs(ftab))), plan.N, fill_bkg, smooth, avg,
                                 plan.max_voxels, yb, xb, r)
             terms.append((plan.n_frames, pixels_per_bin(plan), B))
-            prepared.append((coords, ids, plan, avg, np.min(coords[:, 2])))
+            prepared.append((coords, ids, plan, avg, np.min(coords[:, 2]), counts, xb * yb * zb))
         self.bins = Bins(self.grid, bin_scale_for(terms))
         self.runners = []
-        for coords, ids, plan, avg, zmin in prepared:
+        for coords, ids, plan, avg, zmin, counts, vol in prepared:
             slab = Slab(coords, ids, ftab, plan.N, plan.r, zmin)
             f64 = fft_precision(fill_bkg, smooth, precision)
             self.runners.append(FrameRunner(plan, slab, self.bins, fill_bkg, smooth, avg,
                                             batch=batch if batch is not None else
                                             tuned_batch(plan.N, plan.W, plan.n_frames, f64, plan.Z),
-                                            precision=precision))
+                                            precision=precision, counts=counts, volume=vol))
         self.n_frames = sum(rn.plan.n_frames for rn in self.runners)
         self.graph = None
         self.iq64 = self.iq32 = None
@@ -1119,16 +1138,30 @@ _TUNED_VARIANT = {}
 _SCRATCH_BINS = {}
 
 
-ROW_VARIANTS = (0, 1, 2, 4, 5, 6)
+# 0 / 4: row-major intermediate (per-wave rows, no staging); 8 / 12: the column-major
+# intermediate staged per block; 1 / 5: block-wide deposit over 4 rows (column-major).
+# Block-wide deposit over 8 rows (2 / 6) is never faster and does not fit N = 2048.
+ROW_VARIANTS = (0, 4, 8, 12, 1, 5)
+
+
+def set_row_variant(desc, v):
+    """Row-pass shape ``v`` on a frame descriptor: bits 0-1 and 2 are gx_frames.row_variant
+    (0 one wave per row, 1 / 2 block-wide deposit over 4 / 8 rows; + 4 XCD-aware order);
+    with shape 0 on N = 512, 1024, 2048 the row-major intermediate (inter_rm) is used
+    unless bit 3 asks for the column-major one."""
+    v = int(v)
+    desc.row_variant = v & 7
+    desc.inter_rm = int(desc.N in (512, 1024, 2048) and desc.bs_M == 0 and (v & 3) == 0 and not (v & 8))
 
 
 def tuned_variant(runner: FrameRunner):
-    """Row-pass shape (gx_frames.row_variant: one wave per row, or a block depositing
-    4 / 8 rows at once; each in frame-major or XCD-aware tile order) for this runner's
-    grid edge, precision, band width, atoms per row and atom count (whether the atoms
-    fit one XCD's L2): each shape is timed once on the runner's own first frames (into
-    scratch bins, so the real bins stay untouched) and the fastest is cached.  All
-    shapes give identical bins.  GIWAXSIM_AUTOTUNE=0 keeps shape 0."""
+    """Frame-pass shape (ROW_VARIANTS, see set_row_variant: the row-major intermediate
+    with per-wave rows, or the column-major one with per-wave rows or a block-wide
+    deposit; each in frame-major or XCD-aware tile order) for this runner's grid edge,
+    precision, band width, atoms per row and atom count (whether the atoms fit one
+    XCD's L2): each shape's row and column passes are timed once on the runner's own
+    first frames (into scratch bins, so the real bins stay untouched) and the fastest
+    is cached.  All shapes give identical bins.  GIWAXSIM_AUTOTUNE=0 keeps shape 0."""
     import os
     if os.environ.get("GIWAXSIM_AUTOTUNE", "1") == "0":
         return 0
@@ -1143,13 +1176,13 @@ def tuned_variant(runner: FrameRunner):
     sb = _SCRATCH_BINS.get(g.c_n)
     if sb is None:
         sb = _SCRATCH_BINS[g.c_n] = Bins(g, 1.0)
-    keep = (d.bin_sum, d.bin_cnt, d.row_variant)
+    keep = (d.bin_sum, d.bin_cnt, d.row_variant, d.inter_rm)
     d.bin_sum, d.bin_cnt = ptr(sb.sum).value, ptr(sb.cnt).value
     nfr = min(p.n_frames, runner.batch)
     best, best_t = 0, float("inf")
     try:
         for v in ROW_VARIANTS:
-            d.row_variant = v
+            set_row_variant(d, v)
             try:
                 runner.run((0, nfr))
             except _lib.GxError:
@@ -1164,7 +1197,7 @@ def tuned_variant(runner: FrameRunner):
             if t < best_t:
                 best, best_t = v, t
     finally:
-        d.bin_sum, d.bin_cnt, d.row_variant = keep
+        d.bin_sum, d.bin_cnt, d.row_variant, d.inter_rm = keep
     _TUNED_VARIANT[key] = best
     return best
 

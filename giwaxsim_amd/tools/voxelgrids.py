@@ -95,9 +95,17 @@ def _bin_frame_dev(iq_2d, det_h_qx, det_h_qy, det_v_qz, qx, qy, qz, gs, gc, stat
     hy = engine.to_dev(det_h_qy, torch.float64)
     vz = engine.to_dev(det_v_qz, torch.float64)
     qlim = dbl_array([np.min(qx), np.max(qx), np.min(qy), np.max(qy), np.min(qz), np.max(qz)])
+    monotone = all(_sorted(a) for a in (det_h_qx, det_h_qy, det_v_qz))
     call("gx_bin_frame_f64", engine.ptr(iq), nv, nh, engine.ptr(hx), engine.ptr(hy),
          engine.ptr(vz), qlim, float(np.diff(qz)[0]), len(qx), engine.ptr(gs), engine.ptr(gc),
-         engine.ptr(status), engine.stream_handle())
+         engine.ptr(status), int(monotone), engine.stream_handle())
+
+
+def _sorted(a):
+    """True if ``a`` is non-decreasing or non-increasing (then every floor-divided bin
+    index along it is too, and each bin's pixels form one rectangle)."""
+    d = np.diff(np.asarray(a, dtype=np.float64).ravel())
+    return bool(np.all(d >= 0) or np.all(d <= 0))
 
 
 def _check_status(status):
@@ -146,8 +154,9 @@ def process_file(filepath, q_num, qx, qy, qz, voxel_grid_shm_name, voxel_grid_co
 def frames_to_iq_parallel(filepaths, q_num, qx, qy, qz):
     """Mean I(q) [qy, qx, qz] over saved frames (tools/voxelgrids.py:508-533): every frame
     binned into one pair of device sum / count grids, then sum / count with 0 where no
-    pixel landed.  Frames are binned with FP64 atomics, so the sums' last bits depend on
-    arrival order, as the reference's thread-pool order does."""
+    pixel landed.  Frames are binned one after another in the given order, each with
+    np.add.at's order of additions (gx_bin_frame_f64 with sorted axes), so the result is
+    deterministic; the reference's thread pool adds frames in completion order."""
     import torch
     _same_axes(qx, qy, qz)
     if int(q_num) != len(qx):

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GX_ABI_VERSION 6
+#define GX_ABI_VERSION 7
 
 #define GX_OK 0
 #define GX_ERR_ARG 1         /* invalid argument (message says which) */
@@ -90,7 +90,8 @@ typedef struct gx_slab {
   int32_t *sq;            /* [2 n_atoms] fixed-point (re, im) of f, scale 2^row_exp[row] */
   int32_t *szrow;         /* [n_atoms] z row of each regrouped atom (N for rows >= N) */
   int32_t *row_off;       /* [N + 2] first atom of each row; row_off[N] = atoms with row < N */
-  int32_t *row_exp;       /* [N] per-row fixed-point exponent */
+  int32_t *row_exp;       /* [N] fixed-point exponent of each row (gx_slab_sort_rows writes one
+                             slab-wide value, the largest that no row's sum can overflow) */
   int32_t *work;          /* [N * n_elem + 2 * (N + 1)] scratch counters; the first N * n_elem
                              entries keep the per-row element counts after the call */
   /* optional outputs (NULL = not written), used by the differentiable variant */
@@ -160,7 +161,8 @@ typedef struct gx_frames {
   int32_t *col_nm;         /* [batch][N] num_missing_voxels per column (fill_bkg) */
   double *col_sy;          /* [batch][N] smoothed y mask (smooth) */
   double *col_sz;          /* [batch][N] smoothed z mask (smooth) */
-  void *inter;             /* [batch][W][Z] float2 or double2: row-transformed, column-pruned frame */
+  void *inter;             /* [batch][W][Z] (inter_rm = 0) or [batch][Z][W] (inter_rm = 1) float2 or
+                              double2: row-transformed, column-pruned frame */
   void *col_spec;          /* [batch][W] float2 or double2: spectrum of the frame's background /
                               smoothing terms on the kept columns (fill_bkg or smooth) */
   /* output: cropped bins [c][c][c] indexed [qy][qx][qz] */
@@ -178,6 +180,15 @@ typedef struct gx_frames {
   int64_t bs_clo, bs_ck;   /* column pass: first signed frequency and band length */
   const void *bs_kernel_c; /* [bs_Mc] the column band's kernel (as bs_kernel) */
   const void *bs_twiddle_c; /* [bs_Mc] exp(-2 pi i m / bs_Mc) */
+  const double *avg_q;     /* NULL, or device [2] avg_voxel_f of the f values the deposit adds
+                              (gx_slab_avg), used in place of avg_re, avg_im so that the fill
+                              and smoothing terms match the deposited atoms */
+  int64_t inter_rm;        /* 0: inter is [batch][W][Z] (the row pass applies the background and
+                              z-mask terms and stores transposed); 1 (N = 512, 1024, 2048 with
+                              the per-wave row shape): inter is [batch][Z][W], the row pass writes
+                              the raw kept band of each row straight from registers
+                              and the column pass applies those terms.  gx_frames_rowpass and
+                              gx_frames_colpass of one frame batch must use the same value */
 } gx_frames;
 
 /* ------------------------------------------------------------------------------------
@@ -213,6 +224,14 @@ int gx_slab_bounds(const gx_slab *s, double *out_minmax, int64_t *out_elem_count
  * (z_indices of tools/voxelgrids.py:329; the rotation leaves z unchanged) and
  * quantises each atom's f to the row's fixed-point scale. */
 int gx_slab_sort_rows(const gx_slab *s, gx_stream_t stream);
+
+/* avg_voxel_f = (sum f / (x_b y_b z_b)) r^3 (tools/comparison.py:741-744) of the f values
+ * that gx_slab_sort_rows' fixed point deposits, f'_e = rint(f_e 2^S) 2^-S with the slab's
+ * exponent S: device out[2] = (re, im) from the device per-element atom counts
+ * elem_count[n_elem] (all atoms), volume = x_b y_b z_b and r3 = r_voxel_size^3.
+ * gx_frames.avg_q takes it, so the background fill matches the deposit exactly. */
+int gx_slab_avg(const gx_slab *s, const int64_t *elem_count, double volume, double r3, double *out,
+                gx_stream_t stream);
 
 /* Per-frame statistics for frames [f0, f0+nf) of d->fparams: min rotated y
  * (tools/voxelgrids.py:319-323) and the index bounds y_min, y_max, z_min,
@@ -276,11 +295,14 @@ int gx_intersect(const double *iq, int64_t cy, int64_t cx, int64_t cz, double qx
  * bins (q - q_min) // dq and adds I and 1 into FP64 grids
  * [q_num][q_num][q_num] indexed [qy][qx][qz].  Pixels whose bin falls
  * outside the grid are skipped and counted in status[0] (the reference would
- * raise IndexError). */
+ * raise IndexError).  monotone_axes = 1 (each of the three axes sorted, as the
+ * frame axes of tools/voxelgrids.py:394-402 are): every bin's pixels form one
+ * rectangle and are added in np.add.at's order, so the grids are bit-identical to
+ * the reference's and deterministic; 0: any axes, one FP64 atomic add per pixel. */
 int gx_bin_frame_f64(const double *iq2d, int64_t nv, int64_t nh, const double *hqx,
                      const double *hqy, const double *vqz, const double *host_qlim, double dq,
                      int64_t q_num, double *grid_sum, double *grid_cnt, int32_t *status,
-                     gx_stream_t stream);
+                     int64_t monotone_axes, gx_stream_t stream);
 
 /* rectangular_collapse_lengths (tools/voxelgrids.py:238-309) for n x values
  * using a frame record (host_fp, GX_FP_STRIDE doubles) from the planner. */

@@ -65,10 +65,22 @@ def test_process_file2():
     s1, g = _shm((31,) * 3)
     s2, c = _shm((31,) * 3)
     try:
+        g[...] = rng.uniform(0, 1, size=g.shape)   # binning adds to what the grids hold
+        c[...] = 3.0
+        wg, wc = g.copy(), c.copy()
         V.process_file2(iq, hx, hy, vz, q, q, q, s1.name, s2.name)
-        wg = np.zeros((31,) * 3)
-        wc = np.zeros((31,) * 3)
         O.bin_frame(iq, hx, hy, vz, q, q, q, wg, wc)
+        # sorted axes: each bin's pixels are added in np.add.at's order, bit for bit
+        assert np.array_equal(c, wc)
+        assert np.array_equal(g, wg)
+        # axes in no order (a caller's own arrays): atomic adds, same counts, sums to rounding
+        perm = rng.permutation(N)
+        g[...] = 0.0
+        c[...] = 0.0
+        wg[...] = 0.0
+        wc[...] = 0.0
+        V.process_file2(iq[:, perm], hx[perm], hy[perm], vz, q, q, q, s1.name, s2.name)
+        O.bin_frame(iq[:, perm], hx[perm], hy[perm], vz, q, q, q, wg, wc)
         assert np.array_equal(c, wc)
         assert np.allclose(g, wg, rtol=1e-14, atol=0)
     finally:
@@ -99,7 +111,7 @@ def test_frames_from_files(tmp_path):
     got = V.frames_to_iq_parallel(paths, 31, q, q, q)
     want = np.divide(wg, wc, out=np.zeros_like(wg), where=wc != 0)
     assert np.array_equal(got == 0, want == 0) and (want == 0).any()
-    assert np.allclose(got, want, rtol=1e-13, atol=0)
+    assert np.array_equal(got, want)
     s1, g1 = _shm((31,) * 3)
     s2, c1 = _shm((31,) * 3)
     s3, g2 = _shm((31,) * 3)
@@ -108,7 +120,7 @@ def test_frames_from_files(tmp_path):
         V.process_file(paths[1], 31, q, q, q, s1.name, s2.name)
         V.process_file2(*(np.load(f'{paths[1]}_{n}.npy') for n in ('iq', 'h_qx', 'h_qy', 'v_qz')), q, q, q,
                         s3.name, s4.name)
-        assert np.array_equal(c1, c2) and np.allclose(g1, g2, rtol=1e-14, atol=0) and c1.sum() > 0
+        assert np.array_equal(c1, c2) and np.array_equal(g1, g2) and c1.sum() > 0
     finally:
         del g1, c1, g2, c2
         for s in (s1, s2, s3, s4):
