@@ -17,6 +17,7 @@ GX_OK = 0
 GX_ABI_VERSION = 7   # structure layouts below match this version of the header
 GX_FP_STRIDE = 24
 GX_MOMENTS_WORK = 774
+GX_CROP_NOALLOC_MAX_ATOMS = 134217728
 GX_FIN_MIRROR = 1
 GX_FIN_FLOOR_SCALE = 2
 
@@ -119,6 +120,7 @@ SIGNATURES = {
     "gx_scale_offset_moments": [_P, _P, _P, _I, _P, POINTER(c_double), _P],
     "gx_slab_tile": [_P, _P, _I, _I, _I, _I, POINTER(c_double), _P, _P, _P, _P, _P],
     "gx_slab_crop": [_P, _P, _P, _P, _I, POINTER(c_double), _P, _P, _P, _P, _P, POINTER(c_int64), _P],
+    "gx_slab_crop_noalloc": [_P, _P, _P, _P, _I, POINTER(c_double), _P, _P, _P, _P, POINTER(c_int64), _P],
     "gx_shift_min": [_P, _P, _P, _I, _P, _P],
     "gx_map_linear": [_P, _I, _I, _P, _P, _I, _D, _P, _P],
     "gx_polar_pad": [_P, _I, _I, _P, _I, _P, _P],

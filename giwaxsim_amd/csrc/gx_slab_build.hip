@@ -16,7 +16,15 @@
 
 #include <math.h>
 
+#include <mutex>
+
 using namespace gx;
+
+// Block counts of gx_slab_crop_noalloc, held by the library on every device it runs on
+// (until the process ends): one int64 per 2048-atom block plus the total, for up to
+// GX_CROP_NOALLOC_MAX_ATOMS atoms (512 KiB).  In host code this name is not a device
+// address: the entry point uses the per-device address from hipGetSymbolAddress.
+static __device__ __align__(256) int64_t scratch[GX_CROP_NOALLOC_MAX_ATOMS / 2048 + 1];
 
 namespace {
 
@@ -189,6 +197,50 @@ int gx_slab_crop(const double *x, const double *y, const double *z, const int32_
   k_crop_scan<<<1, 1024, 0, st>>>(work, nb);
   k_crop_scatter<<<(unsigned)nb, BLK, 0, st>>>(x, y, z, elem, n, lim, work, ox, oy, oz, oelem);
   int rc = check_launch("gx_slab_crop");
+  if (rc) return rc;
+  GX_HIP(hipMemcpyAsync(host_count, work + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  GX_HIP(hipStreamSynchronize(st));
+  return GX_OK;
+}
+
+int gx_slab_crop_noalloc(const double *x, const double *y, const double *z, const int32_t *elem, int64_t n,
+                         const double *host_lim, double *ox, double *oy, double *oz, int32_t *oelem,
+                         int64_t *host_count, gx_stream_t stream) {
+  clear_error();
+  GX_REQUIRE(n >= 1 && n <= GX_CROP_NOALLOC_MAX_ATOMS,
+             "gx_slab_crop_noalloc: n=%lld atoms outside [1, %lld] (the library's block-count array holds %lld "
+             "blocks of %d atoms); use gx_slab_crop with a caller-owned work buffer",
+             (long long)n, (long long)GX_CROP_NOALLOC_MAX_ATOMS, (long long)(GX_CROP_NOALLOC_MAX_ATOMS / CROP_B),
+             CROP_B);
+  GX_REQUIRE(x && y && z && elem && host_lim && ox && oy && oz && oelem && host_count, "NULL buffer");
+  // the array's device address, looked up once per device with that device current
+  static std::mutex mu;
+  static int64_t *addr[GX_MAX_DEVICES] = {};
+  int dev = 0;
+  GX_HIP(hipGetDevice(&dev));
+  GX_REQUIRE(dev >= 0 && dev < GX_MAX_DEVICES, "device %d beyond the %d the library tracks", dev, GX_MAX_DEVICES);
+  int64_t *work = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+      void *p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || p == nullptr)
+        return set_error(GX_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      addr[dev] = static_cast<int64_t *>(p);
+    }
+    work = addr[dev];
+  }
+  // every entry a call reads (the nb block counts, then the total) is written by that
+  // call before it is read, so nothing needs clearing between calls
+  hipStream_t st = as_stream(stream);
+  Lim9 lim;
+  for (int k = 0; k < 9; ++k) lim.v[k] = host_lim[k];
+  const int64_t nb = (n + CROP_B - 1) / CROP_B;
+  k_crop_count<<<(unsigned)nb, BLK, 0, st>>>(x, y, z, n, lim, work);
+  k_crop_scan<<<1, 1024, 0, st>>>(work, nb);
+  k_crop_scatter<<<(unsigned)nb, BLK, 0, st>>>(x, y, z, elem, n, lim, work, ox, oy, oz, oelem);
+  int rc = check_launch("gx_slab_crop_noalloc");
   if (rc) return rc;
   GX_HIP(hipMemcpyAsync(host_count, work + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   GX_HIP(hipStreamSynchronize(st));

@@ -346,6 +346,19 @@ int gx_slab_tile(const double *cell_xyz, const int32_t *cell_elem, int64_t n_cel
 int gx_slab_crop(const double *x, const double *y, const double *z, const int32_t *elem, int64_t n,
                  const double *host_lim, double *ox, double *oy, double *oz, int32_t *oelem, int64_t *work,
                  int64_t *host_count, gx_stream_t stream);
+
+/* No counterpart in the reference.  gx_slab_crop for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight fit loops, harnesses that forbid hipMalloc):
+ * the same outputs, with the block counts in a library-owned device array instead of the
+ * caller's `work` (GX_CROP_NOALLOC_MAX_ATOMS / 2048 + 1 int64, 512 KiB per device, held until
+ * the process ends).  n <= GX_CROP_NOALLOC_MAX_ATOMS, else GX_ERR_ARG.  Calls share that
+ * array: two calls that overlap on one device (other threads or streams) race.  Allocates
+ * nothing; blocks until the kept count is in *host_count. */
+#define GX_CROP_NOALLOC_MAX_ATOMS 134217728
+#define GX_MAX_DEVICES 64
+int gx_slab_crop_noalloc(const double *x, const double *y, const double *z, const int32_t *elem, int64_t n,
+                         const double *host_lim, double *ox, double *oy, double *oz, int32_t *oelem,
+                         int64_t *host_count, gx_stream_t stream);
 /* x -= minmax[0], y -= minmax[2], z -= minmax[4] (gx_slab_bounds' device output):
  * the final shift to the origin (comparison.py:667-669). */
 int gx_shift_min(double *x, double *y, double *z, int64_t n, const double *minmax, gx_stream_t stream);
