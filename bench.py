@@ -38,31 +38,20 @@ def parse():
                     help="phi-frames of the same slab timed on the host (CPU baseline sample)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="repeat the CPU baseline sample until this much CPU time has been measured")
+    ap.add_argument("--cpu-frames-1t", type=int, default=48,
+                    help="phi-frames of the single-threaded CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=20)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble (64 slabs) and large-slab (N=2048) figures")
     ap.add_argument("--only", choices=["adjoint", "adjoint_fp64", "adjoint_polymer", "adjoint_large",
-                                       "ensemble", "large_slab", "compare", "slab_build", "fit_eval"],
+                                       "ensemble", "large_slab", "compare", "slab_build", "fit_eval",
+                                       "graphite50"],
                     help="run one of the extra workloads alone and print its JSON (for profiling)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="one step, then one 256 MiB device copy (counter calibration) and one "
                          "launch of each frame kernel; for rocprofv3 --pmc passes")
     return ap.parse_args()
-
-
-def measured_traffic_per_frame():
-    """Calibrated HBM bytes per frame of the frame kernels, measured with two
-    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) on this workload and kept in
-    profiles/frames_polymer100_N1024_counters.md."""
-    path = os.path.join(ROOT, "profiles", "frames_polymer100_N1024_counters.md")
-    try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_frame:"):
-                return {k: float(v) for k, v in (t.split("=") for t in line.split(":", 1)[1].split())}
-    except OSError:
-        pass
-    return {}
 
 
 def workload():
@@ -114,8 +103,12 @@ def main():
         dst.copy_(src)                      # 256 MiB read + 256 MiB write
         runner = res['runner']
         nb = runner.batch
-        for kname in ("gx_frames_stats", "gx_frames_columns", "gx_frames_rowpass", "gx_frames_colpass"):
-            call(kname, ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
+        keep = (runner.desc.row_variant, runner.desc.inter_rm)
+        for shape in (0, 8):   # row-major and column-major intermediates, one launch of each kernel
+            engine.set_row_variant(runner.desc, shape)
+            for kname in ("gx_frames_stats", "gx_frames_columns", "gx_frames_rowpass", "gx_frames_colpass"):
+                call(kname, ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
+        runner.desc.row_variant, runner.desc.inter_rm = keep
         torch.cuda.synchronize()
         print(json.dumps({"profile_kernels": True, "frames_per_launch": nb, "copy_bytes": 2 * 4 * src.numel()}))
         return
@@ -133,50 +126,14 @@ def main():
     N = plan.N
     n_atoms = len(W['coords'])
 
-    # ---- dominant kernel, timed live with HIP events on the stream it runs on
+    # ---- frame kernels, timed live with HIP events on the stream they run on
     runner = res['runner']
-    st = torch.cuda.current_stream()
-    nb = runner.batch
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    call("gx_frames_stats", ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
-    call("gx_frames_columns", ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
-    times = {}
-    for kname in ("gx_frames_rowpass", "gx_frames_colpass"):
-        call(kname, ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
-        ev[0].record(st)
-        for _ in range(args.kernel_iters):
-            call(kname, ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
-        ev[1].record(st)
-        torch.cuda.synchronize()
-        times[kname] = ev[0].elapsed_time(ev[1]) / args.kernel_iters / 1e3  # s per launch
-    runner.bins.clear()
-    dom = max(times, key=times.get)
-    # algorithmic bytes per frame (fixed contract): 24 n_atoms + 16 N^2 + 16 P.
-    P_mean = float(res['bins'].cnt.sum().item()) / F
-    B_frame = 24 * n_atoms + 16 * N * N + 16 * P_mean
-    # the row pass is charged the atom reads and the frame write (24 n_a + 8 N^2),
-    # the column pass the frame read and the bin read-modify-writes (8 N^2 + 16 P)
-    alg = {"gx_frames_rowpass": nb * (24 * n_atoms + 8 * N * N),
-           "gx_frames_colpass": nb * (8 * N * N + 16 * P_mean)}
-    achieved = alg[dom] / times[dom] / 1e9
-    traffic = measured_traffic_per_frame()
+    times = frame_kernel_times(runner, engine, torch, call, args.kernel_iters)
+    roof = frame_roofline(res, times, n_atoms, engine)
 
     # ---- detector images/s
-    from giwaxsim_amd.tools.comparison import _weights
-    psis = np.linspace(0, 180, 361)
-    one = np.array([0.0])
-    dplan = engine.detector_plan(512, W['max_q'], (0.15, 0, 0), ('psi', 'None', 'None'), psis,
-                                 _weights(None, psis), one, _weights(None, one), one,
-                                 _weights(None, one))
-    iq = res['iq64']
-    engine.run_detector(iq, res['qx'], res['qy'], res['qz'], dplan, True)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    nd = max(3, args.steps // 4)
-    for _ in range(nd):
-        engine.run_detector(iq, res['qx'], res['qy'], res['qz'], dplan, True)
-    torch.cuda.synchronize()
-    det_s = (time.perf_counter() - t1) / nd
+    det_s = detector_seconds(engine, torch, res, 512, W['max_q'], (0.15, 0, 0), ('psi', 'None', 'None'),
+                             np.linspace(0, 180, 361), max(3, args.steps // 4))
 
     # ---- other synthetic configurations, reported beside the headline figure
     extra = {}
@@ -186,21 +143,7 @@ def main():
     # ---- CPU baseline: the oracle in reference-faithful threaded mode on a bounded sample
     cpu = None
     if not args.no_cpu:
-        import oracle as O
-        phis = plan.grid.phis[:args.cpu_frames]
-        workers = O.default_threads()
-        # whole passes over the sample until at least --cpu-seconds of CPU work (at most 8)
-        passes, tcpu = 0, 0.0
-        while passes < 8 and (passes == 0 or tcpu < args.cpu_seconds):
-            tc = time.perf_counter()
-            O.voxelgridmaker(W['coords'], ftab[ids], W['r'], W['qv'], W['max_q'], fill_bkg=W['fill'],
-                             smooth=W['smooth'], phis=phis, threads=None)
-            tcpu += time.perf_counter() - tc
-            passes += 1
-        cpu = {"value": passes * len(phis) / tcpu, "unit": "phi-frames/s", "cores": workers, "kind": "port",
-               "sample": f"{passes} pass(es) over {len(phis)} of {F} phi-frames of the same slab, FP64 numpy "
-                         f"oracle with ThreadPoolExecutor() default workers ({workers}) on {os.cpu_count()} "
-                         f"host CPUs; {tcpu:.1f} s"}
+        cpu = cpu_baseline(W, ftab, ids, plan, args)
 
     out = {
         "metric": "phi-frames/s (project+FFT+3D bin)",
@@ -217,24 +160,187 @@ def main():
         "data": "synthetic (seeded polymer slab, 100^3 A)",
         "config": {"workload": "polymer slab 100^3 A, 110k atoms, N=1024, 581 phi-frames, fill_bkg, "
                                "smooth 10 vox, max_q 2.0, bins 185^3 cropped to 131^3",
-                   "n_atoms": n_atoms, "grid": N, "frames": F, "W": plan.W, "batch": runner.batch},
-        "hbm_gbs_algorithmic_step": B_frame * F / dt / 1e9,
+                   "n_atoms": n_atoms, "grid": N, "frames": F, "W": plan.W, "batch": runner.batch,
+                   "frame_pass_shape": shape_name(runner)},
         "detector_images_per_s": 1.0 / det_s,
         "detector_ms_per_image": det_s * 1e3,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": (traffic[dom.split("_")[-1]] * nb) if dom.split("_")[-1] in traffic
-                     else None,
-                     "traffic_note": "calibrated rocprofv3 FETCH_SIZE+WRITE_SIZE bytes per launch, "
-                                     "profiles/frames_polymer100_N1024_counters.md",
-                     "algorithmic_bytes_per_launch": alg[dom],
-                     "launch_ms": times[dom] * 1e3, "frames_per_launch": nb,
-                     "kernel_ms": {k: v * 1e3 for k, v in times.items()}},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     out.update(extra)
     print(json.dumps(out))
 
+
+
+PEAK_FP32_VALU_TFLOPS = 78.6   # MI355X unpacked FP32 vector rate (packed FP32 is not used)
+CLOCK_GHZ = 2.4                # MI355X peak engine clock
+N_SIMD = 256 * 4
+
+
+def shape_name(runner):
+    d = runner.desc
+    rm = "row-major intermediate, per-wave rows" if d.inter_rm else (
+        "column-major intermediate, " + ("per-wave rows" if (d.row_variant & 3) == 0 else
+                                         f"block deposit over {4 * (d.row_variant & 3)} rows"))
+    return rm + (", XCD-aware order" if d.row_variant & 4 else ", frame-major order")
+
+
+def frame_kernel_times(runner, engine, torch, call, iters):
+    """Seconds per launch of the row and column passes over the runner's first frame batch,
+    from HIP events recorded on the stream the kernels are launched on."""
+    st = torch.cuda.current_stream()
+    nb = runner.batch
+    h = engine.stream_handle()
+    call("gx_frames_stats", ctypes.byref(runner.desc), 0, nb, h)
+    if runner.desc.fill_bkg or runner.desc.smooth:
+        call("gx_frames_columns", ctypes.byref(runner.desc), 0, nb, h)
+    times = {}
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for kname in ("gx_frames_rowpass", "gx_frames_colpass"):
+        call(kname, ctypes.byref(runner.desc), 0, nb, h)
+        ev[0].record(st)
+        for _ in range(iters):
+            call(kname, ctypes.byref(runner.desc), 0, nb, h)
+        ev[1].record(st)
+        torch.cuda.synchronize()
+        times[kname] = ev[0].elapsed_time(ev[1]) / iters / 1e3
+    runner.bins.clear()
+    return times
+
+
+def counters_profile():
+    """Per-kernel figures of profiles/frames_polymer100_N1024_counters.md (rocprofv3 --pmc
+    passes of `bench.py --profile-kernels` with each frame-pass shape): calibrated HBM bytes
+    per frame and the VALU issue share of each row / column pass kernel."""
+    path = os.path.join(ROOT, "profiles", "frames_polymer100_N1024_counters.md")
+    out = {}
+    try:
+        for line in open(path):
+            for key in ("traffic_bytes_per_frame:", "valu_issue_share:"):
+                if line.startswith(key):
+                    out[key[:-1]] = {k: float(v) for k, v in (t.split("=") for t in line.split(":", 1)[1].split())}
+    except OSError:
+        pass
+    return out
+
+
+def frame_roofline(res, times, n_atoms, engine):
+    """The roofline record of the dominant frame kernel.  `achieved` / `frac`: the fixed
+    algorithmic-byte contract (24 n_atoms + 16 N^2 + 16 P per phi-frame: the row pass is
+    charged 24 n_atoms + 8 N^2, the column pass 8 N^2 + 16 P) over the live launch time;
+    `hbm_traffic_frac`: calibrated FETCH+WRITE bytes of the same kernel (counter profile)
+    over the live launch time; `issue`: what bounds it (VALU issue share from the counter
+    profile, FFT FLOP/s against the unpacked FP32 vector rate)."""
+    runner, plan = res['runner'], res['plan']
+    F, N, nb, W = plan.n_frames, plan.N, runner.batch, plan.W
+    P_mean = float(res['bins'].cnt.sum().item()) / F
+    alg = {"gx_frames_rowpass": nb * (24 * n_atoms + 8 * N * N),
+           "gx_frames_colpass": nb * (8 * N * N + 16 * P_mean)}
+    dom = max(times, key=times.get)
+    achieved = alg[dom] / times[dom] / 1e9
+    fam = {"gx_frames_rowpass": "k_rowpass_rm" if runner.desc.inter_rm else "k_rowpass",
+           "gx_frames_colpass": "k_colpass_rm" if runner.desc.inter_rm else "k_colpass"}
+    prof = counters_profile()
+    tr = prof.get("traffic_bytes_per_frame", {}).get(fam[dom])
+    traffic = tr * nb if tr is not None else None
+    # FFT flops: 5 N log2 N per transform; rows that hold atoms (row pass), kept columns (column pass)
+    off = runner.slab.row_off[:plan.Z + 1].cpu().numpy()
+    rows = int(np.count_nonzero(np.diff(off) > 0))
+    fft_flops = {"gx_frames_rowpass": nb * rows * 5 * N * np.log2(N),
+                 "gx_frames_colpass": nb * W * 5 * N * np.log2(N)}
+    issue = {}
+    for k in times:
+        tf = fft_flops[k] / times[k] / 1e12
+        issue[k] = {"fft_tflops": tf, "fft_frac_of_fp32_valu": tf / PEAK_FP32_VALU_TFLOPS,
+                    "valu_issue_share": prof.get("valu_issue_share", {}).get(fam[k])}
+    return {"bound": "hbm", "kernel": dom, "kernel_function": fam[dom], "achieved": achieved,
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+            "hbm_traffic_frac": (traffic / times[dom] / 1e9 / PEAK_HBM_GBS) if traffic else None,
+            "traffic_note": "calibrated rocprofv3 FETCH_SIZE+WRITE_SIZE bytes per launch, "
+                            "profiles/frames_polymer100_N1024_counters.md",
+            "algorithmic_bytes_per_launch": alg[dom], "launch_ms": times[dom] * 1e3,
+            "frames_per_launch": nb, "kernel_ms": {k: v * 1e3 for k, v in times.items()},
+            "issue": issue}
+
+
+def detector_seconds(engine, torch, res, n_pix, max_q, init_vals, init_axs, psis, reps):
+    from giwaxsim_amd.tools.comparison import _weights
+    one = np.array([0.0])
+    dplan = engine.detector_plan(n_pix, max_q, init_vals, init_axs, psis, _weights(None, psis), one,
+                                 _weights(None, one), one, _weights(None, one))
+    iq = res['iq64']
+    engine.run_detector(iq, res['qx'], res['qy'], res['qz'], dplan, True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        engine.run_detector(iq, res['qx'], res['qy'], res['qz'], dplan, True)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t1) / reps
+
+
+def host_description():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    blas = None
+    try:
+        blas = np.show_config(mode="dicts")["Build Dependencies"]["blas"]["name"]
+    except Exception:
+        pass
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    return model, blas, env
+
+
+def cpu_baseline(W, ftab, ids, plan, args):
+    """The oracle timed on this host: the reference's ThreadPoolExecutor() fan-out over frames
+    and over detector orientations (tools/comparison.py:759-762, 835-853) on bounded samples,
+    and single-threaded samples of both."""
+    import oracle as O
+    F = plan.n_frames
+    phis = plan.grid.phis[:args.cpu_frames]
+    workers = O.default_threads()
+    fv = ftab[ids]
+    passes, tcpu = 0, 0.0
+    res = None
+    while passes < 8 and (passes == 0 or tcpu < args.cpu_seconds):
+        tc = time.perf_counter()
+        res = O.voxelgridmaker(W['coords'], fv, W['r'], W['qv'], W['max_q'], fill_bkg=W['fill'],
+                               smooth=W['smooth'], phis=phis, threads=None)
+        tcpu += time.perf_counter() - tc
+        passes += 1
+    # one thread, a sample of frames spread over phi
+    ph1 = plan.grid.phis[::max(1, F // args.cpu_frames_1t)][:args.cpu_frames_1t]
+    tc = time.perf_counter()
+    O.voxelgridmaker(W['coords'], fv, W['r'], W['qv'], W['max_q'], fill_bkg=W['fill'], smooth=W['smooth'],
+                     phis=ph1, threads=0)
+    t1 = time.perf_counter() - tc
+    # detector stage: 512^2 px, psi 0..180 step 0.5 (361 orientations), 0.15 deg psi tilt
+    psis = np.linspace(0, 180, 361)
+    one = np.array([0.0])
+    dargs = (res['iq'], res['qx'], res['qy'], res['qz'], 512, W['max_q'], (0.15, 0, 0), ('psi', 'None', 'None'))
+    tc = time.perf_counter()
+    O.detectormaker(*dargs, psis, None, one, None, one, None, mirror=True, threads=None)
+    tdet = time.perf_counter() - tc
+    ps1 = psis[::12]
+    tc = time.perf_counter()
+    O.detectormaker(*dargs, ps1, None, one, None, one, None, mirror=True, threads=0)
+    tdet1 = (time.perf_counter() - tc) * len(psis) / len(ps1)
+    model, blas, env = host_description()
+    return {"value": passes * len(phis) / tcpu, "unit": "phi-frames/s", "cores": os.cpu_count(),
+            "kind": "port",
+            "sample": f"{passes} pass(es) over {len(phis)} of {F} phi-frames of the same slab, FP64 numpy "
+                      f"oracle with ThreadPoolExecutor() default workers ({workers}); {tcpu:.1f} s",
+            "workers": workers, "value_1thread": len(ph1) / t1,
+            "sample_1thread": f"{len(ph1)} phi-frames spread over 0..180 deg, threads=0; {t1:.1f} s",
+            "detector_s_per_image": tdet, "detector_workers": workers,
+            "detector_s_per_image_1thread": tdet1,
+            "detector_sample_1thread": f"{len(ps1)} of 361 orientations, scaled to 361",
+            "cpu_model": model, "numpy_blas": blas, "blas_thread_env": env}
 
 def _time_adjoint(model, coords, engine, torch, reps):
     """(forward ms, forward + backward ms) of a DifferentiableSlab from device-resident
@@ -390,6 +496,8 @@ def extra_configs(args, engine, torch, which=None):
         out["adjoint_polymer"] = adjoint_polymer_config(engine, torch)
         out["adjoint_large"] = adjoint_large_config(engine, torch, n_slabs=16)
     reps = 3
+    if which in (None, "graphite50"):
+        out["graphite50"] = graphite50_config(engine, torch, reps=10)
     if which in (None, "compare"):
         out["compare"] = compare_config(engine, torch)
     if which in (None, "slab_build"):
@@ -401,6 +509,39 @@ def extra_configs(args, engine, torch, which=None):
     if which in (None, "large_slab"):
         out["large_slab"] = large_slab_config(engine, torch, reps)
     return out
+
+
+def graphite50_config(engine, torch, reps=10):
+    """The small plain configuration: AB graphite 50^3 A (14.5k atoms, jitter 0.02 A), r 0.25 A,
+    N = 512, no fill, no smoothing, max_q 2.5 (453 phi-frames, FP64 transforms: a sparse grid),
+    and its 256^2 detector over psi 0..180 in 1 degree steps (181 orientations)."""
+    from giwaxsim_amd._lib import call
+    from giwaxsim_amd.synth import graphite_slab
+    c, e = graphite_slab(size=(50.0, 50.0, 50.0), seed=0)
+    r = 0.25
+    qv = 2 * np.pi / (r * 512) * (1 + 1e-9)
+    ids, ftab, _, _ = engine.element_table(e, 12700.0)
+    cd, idd = torch.as_tensor(c).cuda(), torch.as_tensor(ids).cuda()
+    run = lambda: engine.run_voxelgrid(cd, None, r, qv, 2.5, 12700.0, f_table=ftab, elem_ids=idd)
+    res = run()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        res = run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    times = frame_kernel_times(res['runner'], engine, torch, call, 20)
+    roof = frame_roofline(res, times, len(c), engine)
+    for k in ("traffic", "hbm_traffic_frac", "traffic_note"):   # counters are for the polymer workload
+        roof.pop(k, None)
+    det_s = detector_seconds(engine, torch, res, 256, 2.5, (0, 0, 0), ('None',) * 3, np.linspace(0, 180, 181), 10)
+    F = res['plan'].n_frames
+    return {"workload": "graphite 50^3 A, 14.5k atoms, r 0.25 A, N=512, no fill/smooth (FP64 transforms), "
+                        "max_q 2.5, 453 phi-frames; detector 256^2, psi 0..180 step 1",
+            "n_atoms": len(c), "frames": F, "ms_per_step": dt * 1e3, "phi_frames_per_s": F / dt,
+            "fft_precision": "fp64" if res['runner'].f64 else "fp32",
+            "frame_pass_shape": shape_name(res['runner']), "roofline": roof,
+            "detector_ms_per_image": det_s * 1e3, "detector_images_per_s": 1.0 / det_s}
 
 
 def fit_eval_config(engine, torch, reps=5):
@@ -534,7 +675,13 @@ def slab_build_config(engine, torch, reps=20, cpu_reps=3):
                                      np.array_equal(sb.uniq[el.cpu().numpy()], ref_el))}
 
 
-def ensemble_config(engine, torch, reps):
+def ensemble_config(engine, torch, reps, cpu_frames=3):
+    """64 polymer slabs of uneven sizes and grid edges into one shared grid: the whole pass
+    captured in a HIP graph and replayed, against 64 sequential plain runs (each slab through
+    run_voxelgrid on its own, as the reference's input_folder loop processes file after
+    file) on the GPU, and against the oracle on the host (single thread), timed on a sample
+    of frames of one slab per grid edge and scaled by the frames of each edge."""
+    import oracle as O
     from giwaxsim_amd.synth import ensemble_slabs
     qv = 2 * np.pi / (0.2 * 1024) * (1 + 1e-9)
     slabs = ensemble_slabs(n_slabs=64, seed=0, q_voxel=qv)
@@ -547,10 +694,55 @@ def ensemble_config(engine, torch, reps):
         ens.replay()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t) / reps
+    plans = {}
+    for rn in ens.runners:
+        p = plans.setdefault(int(rn.plan.N), {"slabs": 0, "frames_per_launch": set(), "shapes": set()})
+        p["slabs"] += 1
+        p["frames_per_launch"].add(int(rn.batch))
+        p["shapes"].add(shape_name(rn))
+    plans = {str(k): {"slabs": v["slabs"], "frames_per_launch": sorted(v["frames_per_launch"]),
+                      "frame_pass_shapes": sorted(v["shapes"])} for k, v in sorted(plans.items())}
+    # 64 sequential plain runs on the GPU (atoms already resident)
+    dev = [(torch.as_tensor(c).cuda(), e, r) for c, e, r in slabs]
+    prep = []
+    for cd, e, r in dev:
+        ids, ftab, _, _ = engine.element_table(e, 12700.0)
+        prep.append((cd, torch.as_tensor(ids).cuda(), ftab, r))
+
+    def sequential():
+        for cd, idd, ftab, r in prep:
+            engine.run_voxelgrid(cd, None, r, qv, 2.0, 12700.0, fill_bkg=True, smooth=10, f_table=ftab,
+                                 elem_ids=idd)
+    sequential()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    sequential()
+    torch.cuda.synchronize()
+    dt_seq = time.perf_counter() - t
+    # the oracle, one thread: cpu_frames frames of one slab per grid edge
+    per_edge = {}
+    frames_per_edge = {}
+    for (c, e, r), rn in zip(slabs, ens.runners):
+        N = int(rn.plan.N)
+        frames_per_edge[N] = frames_per_edge.get(N, 0) + rn.plan.n_frames
+        if N in per_edge:
+            continue
+        ids, ftab, _, _ = engine.element_table(e, 12700.0)
+        phis = rn.plan.grid.phis[::max(1, rn.plan.n_frames // cpu_frames)][:cpu_frames]
+        tc = time.perf_counter()
+        O.voxelgridmaker(c, ftab[ids], r, qv, 2.0, fill_bkg=True, smooth=10, phis=phis, threads=0)
+        per_edge[N] = ((time.perf_counter() - tc) / len(phis), len(c))
+    cpu_est = sum(per_edge[N][0] * frames_per_edge[N] for N in frames_per_edge)
     return {"workload": "64 polymer slabs, edges U[40,160] A, N in {256,512,1024,2048}, "
                         "one shared 185^3 grid (131^3 crop), fill_bkg, smooth 10",
             "n_atoms": int(sum(len(c) for c, _, _ in slabs)), "frames": ens.n_frames,
-            "ms_per_pass": dt * 1e3, "phi_frames_per_s": ens.n_frames / dt}
+            "ms_per_pass": dt * 1e3, "phi_frames_per_s": ens.n_frames / dt, "plans_by_grid_edge": plans,
+            "sequential_plain_runs_ms": dt_seq * 1e3, "speedup_vs_sequential_plain_runs": dt_seq / dt,
+            "cpu_oracle_s_estimate": cpu_est, "speedup_vs_cpu_oracle": cpu_est / dt,
+            "cpu_note": f"oracle single thread, {cpu_frames} frames of the first slab of each grid edge "
+                        "(seconds per frame of that slab) times the frames of that edge; per-slab atom "
+                        "counts differ, so this is an estimate",
+            "cpu_s_per_frame_by_edge": {str(k): v[0] for k, v in sorted(per_edge.items())}}
 
 
 def large_slab_config(engine, torch, reps):
