@@ -182,7 +182,12 @@ def shape_name(runner):
     rm = "row-major intermediate, per-wave rows" if d.inter_rm else (
         "column-major intermediate, " + ("per-wave rows" if (d.row_variant & 3) == 0 else
                                          f"block deposit over {4 * (d.row_variant & 3)} rows"))
-    return rm + (", XCD-aware order" if d.row_variant & 4 else ", frame-major order")
+    plans = {1024: ("16.16.4", "16.8.8", "8.16.8", "8.8.16"), 2048: ("16.16.8", "16.8.16", "8.16.16")}
+    out = rm + (", XCD-aware order" if d.row_variant & 4 else ", frame-major order")
+    if int(d.N) in plans and not d.bs_M:
+        p = plans[int(d.N)]
+        out += f", radix {p[d.fft_plan_row]} rows / {p[d.fft_plan_col]} columns"
+    return out
 
 
 def frame_kernel_times(runner, engine, torch, call, iters):

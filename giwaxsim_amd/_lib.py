@@ -65,7 +65,7 @@ class GxFrames(ctypes.Structure):
         ("row_variant", c_int64),
         ("bs_rlo", c_int64), ("bs_rk", c_int64), ("bs_Mc", c_int64), ("bs_clo", c_int64), ("bs_ck", c_int64),
         ("bs_kernel_c", c_void_p), ("bs_twiddle_c", c_void_p),
-        ("avg_q", c_void_p), ("inter_rm", c_int64),
+        ("avg_q", c_void_p), ("inter_rm", c_int64), ("fft_plan_row", c_int64), ("fft_plan_col", c_int64),
     ]
 
 

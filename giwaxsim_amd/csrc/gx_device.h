@@ -227,26 +227,36 @@ __device__ __forceinline__ void fft_load(C (&v)[P], const C* lds, int lane) {
   }
 }
 
-// Radix plans (first stage radix is R0) of the in-wave transforms (64 lanes).
-template <int N> struct FftPlan;
-template <> struct FftPlan<256>  { static constexpr int R0 = 4,  R1 = 4,  R2 = 4, R3 = 4; static constexpr int NST = 4; };
-template <> struct FftPlan<512>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 8, R3 = 1; static constexpr int NST = 3; };
-template <> struct FftPlan<1024> { static constexpr int R0 = 16, R1 = 16, R2 = 4, R3 = 1; static constexpr int NST = 3; };
-template <> struct FftPlan<2048> { static constexpr int R0 = 16, R1 = 16, R2 = 8, R3 = 1; static constexpr int NST = 3; };
+// Radix plans (first stage radix is R0) of the in-wave transforms (64 lanes).  Plan PL
+// = 0 is the default; N = 1024 and 2048 have two more orderings of their radices (the
+// frame passes' autotuner picks one per pass and workload); N = 256 and 512 have one.
+template <int N, int PL = 0> struct FftPlan;
+template <int PL> struct FftPlan<256, PL> { static constexpr int R0 = 4, R1 = 4, R2 = 4, R3 = 4; static constexpr int NST = 4; };
+template <int PL> struct FftPlan<512, PL> { static constexpr int R0 = 8, R1 = 8, R2 = 8, R3 = 1; static constexpr int NST = 3; };
+template <> struct FftPlan<1024, 0> { static constexpr int R0 = 16, R1 = 16, R2 = 4, R3 = 1; static constexpr int NST = 3; };
+template <> struct FftPlan<1024, 1> { static constexpr int R0 = 16, R1 = 8, R2 = 8, R3 = 1; static constexpr int NST = 3; };
+template <> struct FftPlan<1024, 2> { static constexpr int R0 = 8, R1 = 16, R2 = 8, R3 = 1; static constexpr int NST = 3; };
+template <> struct FftPlan<1024, 3> { static constexpr int R0 = 8, R1 = 8, R2 = 16, R3 = 1; static constexpr int NST = 3; };
+template <> struct FftPlan<2048, 0> { static constexpr int R0 = 16, R1 = 16, R2 = 8, R3 = 1; static constexpr int NST = 3; };
+template <> struct FftPlan<2048, 1> { static constexpr int R0 = 16, R1 = 8, R2 = 16, R3 = 1; static constexpr int NST = 3; };
+template <> struct FftPlan<2048, 2> { static constexpr int R0 = 8, R1 = 16, R2 = 16, R3 = 1; static constexpr int NST = 3; };
+// radix plans per grid edge (plan indices 0 .. gx_fft_plans(N) - 1)
+constexpr int GX_FFT_PLANS = 4;
+__host__ __device__ constexpr int gx_fft_plans(long long N) { return N == 1024 ? 4 : N == 2048 ? 3 : 1; }
 
-template <int N> struct FftFirst { static constexpr int R = FftPlan<N>::R0; };
-template <int N> struct FftLast {
-  static constexpr int R = FftPlan<N>::NST == 4 ? FftPlan<N>::R3 : FftPlan<N>::R2;
+template <int N, int PL = 0> struct FftFirst { static constexpr int R = FftPlan<N, PL>::R0; };
+template <int N, int PL = 0> struct FftLast {
+  static constexpr int R = FftPlan<N, PL>::NST == 4 ? FftPlan<N, PL>::R3 : FftPlan<N, PL>::R2;
 };
 
 // Runs the whole transform.  On entry v holds the first-stage inputs
 // (v[t R0 + r] = x[lane + 64 t + r N/R0]); on exit v holds the outputs in
 // last-stage order (v[t RL + r] = X[lane + 64 t + r N/RL]).  `lds` is this
 // wave's private buffer of N complex values; it is free again on return.
-template <int N, typename C>
+template <int N, typename C, int PLAN = 0>
 __device__ __forceinline__ void fft_wave(C (&v)[N / GX_WAVE], C* lds, const C* __restrict__ tw, int lane) {
   constexpr int P = N / GX_WAVE;
-  using PL = FftPlan<N>;
+  using PL = FftPlan<N, PLAN>;
   constexpr int R0 = PL::R0, R1 = PL::R1, R2 = PL::R2, R3 = PL::R3;
   fft_butterflies<N, R0, 1, P, C>(v, tw, lane);
   fft_store<N, R0, 1, P, C>(v, lds, lane);
@@ -288,14 +298,17 @@ __device__ __forceinline__ void fft_wave(C (&v)[N / GX_WAVE], C* lds, const C* _
 //   RL - 1, with Dft<RL>'s own additions in its own order (bitwise the same values).
 //   Outputs that are not computed are left undefined; callers use only k in the band.
 // ------------------------------------------------------------------------------------
-template <int N> struct FftBand {
-  using PL = FftPlan<N>;
+template <int N, int PLAN = 0> struct FftBand {
+  using PL = FftPlan<N, PLAN>;
   static_assert(PL::NST == 3, "banded transforms use the three-stage plans");
   static constexpr int R0 = PL::R0, R1 = PL::R1, RL = PL::R2;
   static constexpr int PAD = (N / R0 > N / R1) ? N / R0 : N / R1;
   static constexpr int BUF = N + PAD;  // complex values per wave
 };
-template <int N> constexpr int fft_band_buf() { return FftBand<N>::BUF; }
+// Buffer of every plan of N (kernels size their LDS once for all plans)
+template <int N> constexpr int fft_band_buf() {
+  return N + N / 8;  // >= N + max(N / R0, N / R1) for radices of 8 and 16
+}
 
 // r = 0 and r = R - 1 outputs of Dft<R> (R = 4, 8) in Dft<R>'s operation order
 template <int R, typename C> struct DftEnds;
@@ -338,11 +351,12 @@ template <typename C> struct DftEnds<8, C> {
   }
 };
 
-template <int N, typename C>
+template <int N, typename C, int PLAN = 0>
 __device__ __forceinline__ void fft_wave_band(C (&v)[N / GX_WAVE], C *lds, const C *__restrict__ tw, int lane,
                                               int klo, int khi) {
   constexpr int P = N / GX_WAVE;
-  using FB = FftBand<N>;
+  using FB = FftBand<N, PLAN>;
+  static_assert(FB::BUF <= fft_band_buf<N>(), "plan buffer exceeds the per-wave LDS");
   constexpr int R0 = FB::R0, R1 = FB::R1, RL = FB::RL;
   constexpr int NB0 = P / R0, NB1 = P / R1, NBL = P / RL;
   // stage 1 (radix R0, no twiddles); element i -> i + i / R0
@@ -397,26 +411,30 @@ __device__ __forceinline__ void fft_wave_band(C (&v)[N / GX_WAVE], C *lds, const
     const int k = j & (NS - 1);
 #pragma unroll
     for (int r = 1; r < RL; ++r) v[t * RL + r] = cmul(v[t * RL + r], tw[(r * k) * (N / (NS * RL))]);
-    if (wantMid)
+    if constexpr (RL == 4 || RL == 8) {
+      if (wantMid)
+        Dft<RL, C>::run(&v[t * RL]);
+      else
+        DftEnds<RL, C>::run(&v[t * RL], want0, wantL);
+    } else {
       Dft<RL, C>::run(&v[t * RL]);
-    else
-      DftEnds<RL, C>::run(&v[t * RL], want0, wantL);
+    }
   }
 }
 
 // Per-wave LDS (complex values) of fft_wave_kept and the transform itself: the banded,
 // padded transform for the three-stage plans, fft_wave (all outputs) for N = 256.
 template <int N> constexpr int fft_kept_buf() {
-  if constexpr (FftPlan<N>::NST == 3) return FftBand<N>::BUF;
+  if constexpr (FftPlan<N>::NST == 3) return fft_band_buf<N>();
   return N;
 }
-template <int N, typename C>
+template <int N, typename C, int PLAN = 0>
 __device__ __forceinline__ void fft_wave_kept(C (&v)[N / GX_WAVE], C *lds, const C *__restrict__ tw, int lane,
                                               int klo, int khi) {
-  if constexpr (FftPlan<N>::NST == 3)
-    fft_wave_band<N, C>(v, lds, tw, lane, klo, khi);
+  if constexpr (FftPlan<N, PLAN>::NST == 3)
+    fft_wave_band<N, C, PLAN>(v, lds, tw, lane, klo, khi);
   else
-    fft_wave<N, C>(v, lds, tw, lane);
+    fft_wave<N, C, PLAN>(v, lds, tw, lane);
 }
 
 // ------------------------------------------------------------------------------------
@@ -524,12 +542,12 @@ __device__ __forceinline__ double collapse_length(const double *fp, double x) {
 // spectrum U is added: o = mz (v + U) (+ A N at k = 0 when smoothing), in the
 // transform's precision.
 // ------------------------------------------------------------------------------------
-template <int N, typename C>
+template <int N, typename C, int PLAN = 0>
 __device__ __forceinline__ void stage_row_band(const C (&v)[N / GX_WAVE], int lane, int jlo, int wf, bool inside,
                                                const gx_frames &d, double mz, const C *tspec, C *stage, int SP,
                                                int rb) {
   using T = typename Real<C>::T;
-  constexpr int P = N / GX_WAVE, RL = FftLast<N>::R;
+  constexpr int P = N / GX_WAVE, RL = FftLast<N, PLAN>::R;
   const bool dense = d.fill_bkg || d.smooth;
   const T mzt = (T)mz;
   const C a0 = C{(T)(d.avg_re * d.max_voxels * (double)N), (T)(d.avg_im * d.max_voxels * (double)N)};

@@ -696,13 +696,13 @@ __global__ __launch_bounds__(GX_WAVE) void k_frames_spectra(gx_frames d, int64_t
 // ------------------------------------------------------------------------------------
 constexpr int SCATTER_UNROLL = 4;
 
-template <int N, int RPW, typename C>
+template <int N, int RPW, typename C, int PL = 0>
 __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_t nf) {
   GX_RESOLVE_AVG(d);
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
   constexpr int RB = (BLK / GX_WAVE) * RPW;  // rows per block
-  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  constexpr int R0 = FftFirst<N, PL>::R, RL = FftLast<N, PL>::R;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *smem = reinterpret_cast<C *>(smem_raw);
   const int lane = threadIdx.x & (GX_WAVE - 1);
@@ -814,14 +814,14 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_
       }
       wave_sync();
       // 4) FFT along y (outputs of the kept band)
-      fft_wave_kept<N, C>(v, wbuf, tw, lane, jlo - N / 2, jhi - N / 2);
+      fft_wave_kept<N, C, PL>(v, wbuf, tw, lane, jlo - N / 2, jhi - N / 2);
     } else {
 #pragma unroll
       for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
     }
     // 5) kept columns [jlo, jhi]: add the background spectrum, stage
     const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + z] : 1.0;
-    stage_row_band<N, C>(v, lane, jlo, wf, inside, d, mz, tspec, stage, SP, rb);
+    stage_row_band<N, C, PL>(v, lane, jlo, wf, inside, d, mz, tspec, stage, SP, rb);
     wave_sync();
   }
   __syncthreads();
@@ -839,12 +839,12 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_
 // one wave streaming one row), then each wave transforms its rows.  Same arithmetic
 // and outputs as k_rowpass; the autotuner picks between the two shapes.
 // ------------------------------------------------------------------------------------
-template <int N, int RB, typename C>
+template <int N, int RB, typename C, int PL = 0>
 __global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0, int64_t nf) {
   GX_RESOLVE_AVG(d);
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
-  constexpr int R0 = FftFirst<N>::R;
+  constexpr int R0 = FftFirst<N, PL>::R;
   constexpr int ROWB = (fft_kept_buf<N>() * (int)sizeof(C) > N * 8) ? fft_kept_buf<N>() * (int)sizeof(C)
                                                                      : N * 8;  // bytes per row
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -934,13 +934,13 @@ __global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0, in
           v[t * R0 + q] = C{((T)acc[y] * inv) * m, ((T)acc[N + y] * inv) * m};
         }
       wave_sync();
-      fft_wave_kept<N, C>(v, reinterpret_cast<C *>(rowp), tw, lane, jlo - N / 2, jhi - N / 2);
+      fft_wave_kept<N, C, PL>(v, reinterpret_cast<C *>(rowp), tw, lane, jlo - N / 2, jhi - N / 2);
     } else {
 #pragma unroll
       for (int q = 0; q < P; ++q) v[q] = C{(T)0, (T)0};
     }
     const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + z] : 1.0;
-    stage_row_band<N, C>(v, lane, jlo, wf, inside, d, mz, tspec, stage, SP, rb);
+    stage_row_band<N, C, PL>(v, lane, jlo, wf, inside, d, mz, tspec, stage, SP, rb);
   }
   __syncthreads();
   C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * d.Z;
@@ -972,12 +972,12 @@ template <int N> struct RowRm {
   static constexpr int RB = (BLK / GX_WAVE) * RPWV;        // rows per block
 };
 
-template <int N, typename C, int RPWV, int WPE>
+template <int N, typename C, int RPWV, int WPE, int PL = 0>
 __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_rowpass_rm(gx_frames d, int64_t f0, int64_t nf) {
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
-  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  constexpr int R0 = FftFirst<N, PL>::R, RL = FftLast<N, PL>::R;
   constexpr int BUF = fft_band_buf<N>();
   constexpr int RB = (BLK / GX_WAVE) * RPWV;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1078,7 +1078,7 @@ void k_rowpass_rm(gx_frames d, int64_t f0, int64_t nf) {
       }
     wave_sync();
     // 4) FFT along y, outputs of the kept band only
-    fft_wave_band<N, C>(v, wbuf, tw, lane, klo, khi);
+    fft_wave_band<N, C, PL>(v, wbuf, tw, lane, klo, khi);
     // 5) kept band, row-major
 #pragma unroll
     for (int t = 0; t < P / RL; ++t)
@@ -1107,12 +1107,12 @@ template <int N, typename C> struct ColRm {
   static constexpr int WAVES = (8 * fft_band_buf<N>() * (int)sizeof(C) <= 80 * 1024) ? 8 : 4;
 };
 
-template <int N, typename C, int WAVES>
+template <int N, typename C, int WAVES, int PL = 0>
 __global__ __launch_bounds__(WAVES * GX_WAVE) void k_colpass_rm(gx_frames d, int64_t f0) {
   GX_RESOLVE_AVG(d);
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
-  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  constexpr int R0 = FftFirst<N, PL>::R, RL = FftLast<N, PL>::R;
   constexpr int BUF = fft_band_buf<N>();
   constexpr int TS = WAVES + 1;  // tile row stride (odd: a wave's column reads spread over banks)
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1205,7 +1205,7 @@ __global__ __launch_bounds__(WAVES * GX_WAVE) void k_colpass_rm(gx_frames d, int
   const long long by = to_int(floor_div(hy - d.qmin, d.dq, dqinv)) - d.c_lo;
   if (bx < 0 || bx >= d.c_n || by < 0 || by >= d.c_n) return;
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
-  fft_wave_band<N, C>(v, reinterpret_cast<C *>(smem_raw) + wave * BUF, tw, lane, (int)d.i_lo - N / 2,
+  fft_wave_band<N, C, PL>(v, reinterpret_cast<C *>(smem_raw) + wave * BUF, tw, lane, (int)d.i_lo - N / 2,
                       (int)d.i_hi - N / 2);
 
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;
@@ -1233,11 +1233,11 @@ __global__ __launch_bounds__(WAVES * GX_WAVE) void k_colpass_rm(gx_frames d, int
 // ------------------------------------------------------------------------------------
 // column pass
 // ------------------------------------------------------------------------------------
-template <int N, typename C>
+template <int N, typename C, int PL = 0>
 __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
   GX_RESOLVE_AVG(d);
   constexpr int P = N / GX_WAVE;
-  constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
+  constexpr int R0 = FftFirst<N, PL>::R, RL = FftLast<N, PL>::R;
   __shared__ C lds[BLK / GX_WAVE][fft_kept_buf<N>()];
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
@@ -1268,7 +1268,7 @@ __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
 #pragma unroll
     for (int q = 0; q < R0; ++q) v[t * R0 + q] = col_input(d, col, lane + GX_WAVE * t + q * (N / R0), k0, u, sz_row);
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
-  fft_wave_kept<N, C>(v, lds[wave], tw, lane, (int)d.i_lo - N / 2, (int)d.i_hi - N / 2);
+  fft_wave_kept<N, C, PL>(v, lds[wave], tw, lane, (int)d.i_lo - N / 2, (int)d.i_hi - N / 2);
 
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;
   const double val = d.q_val;
@@ -1640,7 +1640,7 @@ __global__ __launch_bounds__(BLK) void k_bins_finalize(const unsigned long long 
   }
 }
 
-template <int N, typename C>
+template <int N, typename C, int PL>
 int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int RPW = (N <= 1024) ? 2 : 1;
   constexpr int RB = (BLK / GX_WAVE) * RPW;
@@ -1650,16 +1650,16 @@ int launch_rowpass_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st)
                      (long long)d->W, sizeof(C) == 16 ? "FP64" : "FP32");
   static bool attr_set = false;
   if (!attr_set) {
-    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass<N, RPW, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass<N, RPW, C, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                160 * 1024));
     attr_set = true;
   }
   const unsigned nblk = (unsigned)((d->Z / RB) * nf);
-  k_rowpass<N, RPW, C><<<nblk, BLK, lds, st>>>(*d, f0, nf);
+  k_rowpass<N, RPW, C, PL><<<nblk, BLK, lds, st>>>(*d, f0, nf);
   return check_launch("k_rowpass");
 }
 
-template <int N, int RB, typename C>
+template <int N, int RB, typename C, int PL>
 int launch_rowpass_blk_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int ROWB = (fft_kept_buf<N>() * (int)sizeof(C) > N * 8) ? fft_kept_buf<N>() * (int)sizeof(C) : N * 8;
   const size_t lds = (size_t)RB * ROWB + (size_t)d->W * (RB + 2) * sizeof(C);
@@ -1667,47 +1667,47 @@ int launch_rowpass_blk_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t
     return set_error(GX_ERR_UNSUPPORTED, "block-wide row pass needs %zu bytes of LDS (N=%d, RB=%d)", lds, N, RB);
   static bool attr_set = false;
   if (!attr_set) {
-    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass_blk<N, RB, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass_blk<N, RB, C, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                160 * 1024));
     attr_set = true;
   }
   const unsigned nblk = (unsigned)((d->Z / RB) * nf);
-  k_rowpass_blk<N, RB, C><<<nblk, BLK, lds, st>>>(*d, f0, nf);
+  k_rowpass_blk<N, RB, C, PL><<<nblk, BLK, lds, st>>>(*d, f0, nf);
   return check_launch("k_rowpass_blk");
 }
 
 // row_variant bits 0-1: 0 per-wave rows (2 rows per wave for N <= 1024), 1 block-wide
 // deposit over 4 rows, 2 block-wide over 8 rows; bit 2: XCD-aware tile order
-template <int N, typename C>
+template <int N, typename C, int PL>
 int launch_rowpass_v(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   switch (d->row_variant & 3) {
-    case 1: return launch_rowpass_blk_t<N, 4, C>(d, f0, nf, st);
-    case 2: return launch_rowpass_blk_t<N, 8, C>(d, f0, nf, st);
-    default: return launch_rowpass_t<N, C>(d, f0, nf, st);
+    case 1: return launch_rowpass_blk_t<N, 4, C, PL>(d, f0, nf, st);
+    case 2: return launch_rowpass_blk_t<N, 8, C, PL>(d, f0, nf, st);
+    default: return launch_rowpass_t<N, C, PL>(d, f0, nf, st);
   }
 }
 
-template <int N, typename C, int RPWV, int WPE>
+template <int N, typename C, int RPWV, int WPE, int PL>
 int launch_rowpass_rm_x(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int RB = (BLK / GX_WAVE) * RPWV;
   const size_t lds = (size_t)(BLK / GX_WAVE) * fft_band_buf<N>() * sizeof(C);
   static bool attr_set = false;
   if (!attr_set) {
-    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass_rm<N, C, RPWV, WPE>,
+    GX_HIP(hipFuncSetAttribute((const void *)k_rowpass_rm<N, C, RPWV, WPE, PL>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_set = true;
   }
   const unsigned nblk = (unsigned)(((d->Z + RB - 1) / RB) * nf);
-  k_rowpass_rm<N, C, RPWV, WPE><<<nblk, BLK, lds, st>>>(*d, f0, nf);
+  k_rowpass_rm<N, C, RPWV, WPE, PL><<<nblk, BLK, lds, st>>>(*d, f0, nf);
   return check_launch("k_rowpass_rm");
 }
 
-template <int N, typename C>
+template <int N, typename C, int PL>
 int launch_rowpass_rm_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
-  return launch_rowpass_rm_x<N, C, RowRm<N>::RPWV, 1>(d, f0, nf, st);
+  return launch_rowpass_rm_x<N, C, RowRm<N>::RPWV, 1, PL>(d, f0, nf, st);
 }
 
-template <int N, typename C, int WAVES>
+template <int N, typename C, int WAVES, int PL>
 int launch_colpass_rm_w(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   const size_t tile = (size_t)d->Z * (WAVES + 1) * sizeof(C);
   const size_t bufs = (size_t)WAVES * fft_band_buf<N>() * sizeof(C);
@@ -1717,46 +1717,73 @@ int launch_colpass_rm_w(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t 
                      (long long)d->Z);
   static bool attr_set = false;
   if (!attr_set) {
-    GX_HIP(hipFuncSetAttribute((const void *)k_colpass_rm<N, C, WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    GX_HIP(hipFuncSetAttribute((const void *)k_colpass_rm<N, C, WAVES, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                160 * 1024));
     attr_set = true;
   }
   const unsigned nblk = (unsigned)(grid1d(d->W, WAVES) * nf);
-  k_colpass_rm<N, C, WAVES><<<nblk, WAVES * GX_WAVE, lds, st>>>(*d, f0);
+  k_colpass_rm<N, C, WAVES, PL><<<nblk, WAVES * GX_WAVE, lds, st>>>(*d, f0);
   return check_launch("k_colpass_rm");
 }
 
-template <int N, typename C>
+template <int N, typename C, int PL>
 int launch_colpass_rm_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
-  return launch_colpass_rm_w<N, C, ColRm<N, C>::WAVES>(d, f0, nf, st);
+  return launch_colpass_rm_w<N, C, ColRm<N, C>::WAVES, PL>(d, f0, nf, st);
 }
+
+template <int N, int PL>
+int launch_rowpass_p(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  if constexpr (N != 256) {
+    if (d->inter_rm) {
+      if (d->fft_f64) return launch_rowpass_rm_t<N, double2, PL>(d, f0, nf, st);
+      return launch_rowpass_rm_t<N, float2, PL>(d, f0, nf, st);
+    }
+  }
+  if (d->fft_f64) return launch_rowpass_v<N, double2, PL>(d, f0, nf, st);
+  return launch_rowpass_v<N, float2, PL>(d, f0, nf, st);
+}
+
+template <int N, int PL>
+constexpr bool plan_exists() { return PL < gx_fft_plans(N); }
 
 template <int N>
 int launch_rowpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
-  if constexpr (N != 256) {
-    if (d->inter_rm) {
-      if (d->fft_f64) return launch_rowpass_rm_t<N, double2>(d, f0, nf, st);
-      return launch_rowpass_rm_t<N, float2>(d, f0, nf, st);
-    }
+  if constexpr (plan_exists<N, 1>()) {
+    if (d->fft_plan_row == 1) return launch_rowpass_p<N, 1>(d, f0, nf, st);
+    if (d->fft_plan_row == 2) return launch_rowpass_p<N, 2>(d, f0, nf, st);
   }
-  if (d->fft_f64) return launch_rowpass_v<N, double2>(d, f0, nf, st);
-  return launch_rowpass_v<N, float2>(d, f0, nf, st);
+  if constexpr (plan_exists<N, 3>()) {
+    if (d->fft_plan_row == 3) return launch_rowpass_p<N, 3>(d, f0, nf, st);
+  }
+  return launch_rowpass_p<N, 0>(d, f0, nf, st);
 }
 
-template <int N>
-int launch_colpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+template <int N, int PL>
+int launch_colpass_p(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   if constexpr (N != 256) {
     if (d->inter_rm) {
-      if (d->fft_f64) return launch_colpass_rm_t<N, double2>(d, f0, nf, st);
-      return launch_colpass_rm_t<N, float2>(d, f0, nf, st);
+      if (d->fft_f64) return launch_colpass_rm_t<N, double2, PL>(d, f0, nf, st);
+      return launch_colpass_rm_t<N, float2, PL>(d, f0, nf, st);
     }
   }
   dim3 grid(grid1d(d->W, BLK / GX_WAVE), (unsigned)nf);
   if (d->fft_f64)
-    k_colpass<N, double2><<<grid, BLK, 0, st>>>(*d, f0);
+    k_colpass<N, double2, PL><<<grid, BLK, 0, st>>>(*d, f0);
   else
-    k_colpass<N, float2><<<grid, BLK, 0, st>>>(*d, f0);
+    k_colpass<N, float2, PL><<<grid, BLK, 0, st>>>(*d, f0);
   return check_launch("k_colpass");
+}
+
+template <int N>
+int launch_colpass(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
+  if constexpr (plan_exists<N, 1>()) {
+    if (d->fft_plan_col == 1) return launch_colpass_p<N, 1>(d, f0, nf, st);
+    if (d->fft_plan_col == 2) return launch_colpass_p<N, 2>(d, f0, nf, st);
+  }
+  if constexpr (plan_exists<N, 3>()) {
+    if (d->fft_plan_col == 3) return launch_colpass_p<N, 3>(d, f0, nf, st);
+  }
+  return launch_colpass_p<N, 0>(d, f0, nf, st);
 }
 
 template <int N>
@@ -1801,6 +1828,12 @@ int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
   GX_REQUIRE(d->c_n >= 1, "empty cropped grid");
   GX_REQUIRE(d->row_variant >= 0 && d->row_variant <= 6 && (d->row_variant & 3) != 3,
              "row_variant=%lld is not one of 0, 1, 2, 4, 5, 6", (long long)d->row_variant);
+  {
+    const int np = d->bs_M ? 1 : gx_fft_plans(d->N);
+    GX_REQUIRE(d->fft_plan_row >= 0 && d->fft_plan_row < np && d->fft_plan_col >= 0 && d->fft_plan_col < np,
+               "fft_plan_row / fft_plan_col (%lld, %lld) outside [0, %d) for N = %lld", (long long)d->fft_plan_row,
+               (long long)d->fft_plan_col, np, (long long)d->N);
+  }
   GX_REQUIRE(!d->inter_rm || ((d->N == 512 || d->N == 1024 || d->N == 2048) && (d->row_variant & 3) == 0),
              "the row-major intermediate needs N = 512, 1024 or 2048 and the per-wave row shape "
              "(N=%lld, row_variant=%lld)", (long long)d->N, (long long)d->row_variant);

@@ -1144,24 +1144,36 @@ _SCRATCH_BINS = {}
 ROW_VARIANTS = (0, 4, 8, 12, 1, 5)
 
 
+# radix plans of the wave transforms (gx_frames.fft_plan_row / fft_plan_col): N = 1024 has
+# 16.16.4, 16.8.8, 8.16.8 and 8.8.16; N = 2048 has 16.16.8, 16.8.16 and 8.16.16; other edges one
+FFT_PLANS = {1024: 4, 2048: 3}
+
+
 def set_row_variant(desc, v):
-    """Row-pass shape ``v`` on a frame descriptor: bits 0-1 and 2 are gx_frames.row_variant
+    """Frame-pass shape ``v`` on a frame descriptor: bits 0-1 and 2 are gx_frames.row_variant
     (0 one wave per row, 1 / 2 block-wide deposit over 4 / 8 rows; + 4 XCD-aware order);
     with shape 0 on N = 512, 1024, 2048 the row-major intermediate (inter_rm) is used
-    unless bit 3 asks for the column-major one."""
+    unless bit 3 asks for the column-major one; bits 4-5 and 6-7 are the row and column
+    passes' radix plans (0 where the grid edge has one plan)."""
     v = int(v)
     desc.row_variant = v & 7
     desc.inter_rm = int(desc.N in (512, 1024, 2048) and desc.bs_M == 0 and (v & 3) == 0 and not (v & 8))
+    n_plans = FFT_PLANS.get(int(desc.N), 1) if desc.bs_M == 0 else 1
+    pr, pc = (v >> 4) & 3, (v >> 6) & 3
+    desc.fft_plan_row = pr if pr < n_plans else 0
+    desc.fft_plan_col = pc if pc < n_plans else 0
 
 
 def tuned_variant(runner: FrameRunner):
     """Frame-pass shape (ROW_VARIANTS, see set_row_variant: the row-major intermediate
     with per-wave rows, or the column-major one with per-wave rows or a block-wide
-    deposit; each in frame-major or XCD-aware tile order) for this runner's grid edge,
-    precision, band width, atoms per row and atom count (whether the atoms fit one
-    XCD's L2): each shape's row and column passes are timed once on the runner's own
-    first frames (into scratch bins, so the real bins stay untouched) and the fastest
-    is cached.  All shapes give identical bins.  GIWAXSIM_AUTOTUNE=0 keeps shape 0."""
+    deposit; each in frame-major or XCD-aware tile order) and radix plans for this
+    runner's grid edge, precision, band width, atoms per row and atom count (whether the
+    atoms fit one XCD's L2).  The shapes are timed with plan 0, then the row pass's
+    plans with the best shape, then the column pass's, each candidate twice (best of
+    two) on the runner's own first frames (into scratch bins, so the real bins stay
+    untouched); the fastest is cached.  All shapes give identical bins, the plans agree
+    to rounding.  GIWAXSIM_AUTOTUNE=0 keeps shape 0 and plan 0."""
     import os
     if os.environ.get("GIWAXSIM_AUTOTUNE", "1") == "0":
         return 0
@@ -1176,28 +1188,38 @@ def tuned_variant(runner: FrameRunner):
     sb = _SCRATCH_BINS.get(g.c_n)
     if sb is None:
         sb = _SCRATCH_BINS[g.c_n] = Bins(g, 1.0)
-    keep = (d.bin_sum, d.bin_cnt, d.row_variant, d.inter_rm)
+    keep = (d.bin_sum, d.bin_cnt, d.row_variant, d.inter_rm, d.fft_plan_row, d.fft_plan_col)
     d.bin_sum, d.bin_cnt = ptr(sb.sum).value, ptr(sb.cnt).value
     nfr = min(p.n_frames, runner.batch)
-    best, best_t = 0, float("inf")
-    try:
-        for v in ROW_VARIANTS:
-            set_row_variant(d, v)
-            try:
-                runner.run((0, nfr))
-            except _lib.GxError:
-                continue  # shape does not fit this grid (LDS)
+
+    def timed(v):
+        set_row_variant(d, v)
+        try:
+            runner.run((0, nfr))
+        except _lib.GxError:
+            return float("inf")  # shape does not fit this grid (LDS)
+        best_t = float("inf")
+        for _ in range(2):
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             runner.run((0, nfr))
             e1.record()
             torch.cuda.synchronize()
-            t = e0.elapsed_time(e1)
-            if t < best_t:
-                best, best_t = v, t
+            best_t = min(best_t, e0.elapsed_time(e1))
+        return best_t
+
+    try:
+        times = {v: timed(v) for v in ROW_VARIANTS}
+        best = min(times, key=times.get)
+        n_plans = FFT_PLANS.get(p.N, 1) if runner.bs_M == 0 else 1
+        for shift in (4, 6):
+            for pl in range(1, n_plans):
+                v = (best & ~(3 << shift)) | (pl << shift)
+                times[v] = timed(v)
+            best = min(times, key=times.get)
     finally:
-        d.bin_sum, d.bin_cnt, d.row_variant, d.inter_rm = keep
+        d.bin_sum, d.bin_cnt, d.row_variant, d.inter_rm, d.fft_plan_row, d.fft_plan_col = keep
     _TUNED_VARIANT[key] = best
     return best
 

@@ -189,6 +189,10 @@ typedef struct gx_frames {
                               the raw kept band of each row straight from registers
                               and the column pass applies those terms.  gx_frames_rowpass and
                               gx_frames_colpass of one frame batch must use the same value */
+  int64_t fft_plan_row;    /* radix plan of the row / column pass transforms (N = 1024: 0 = 16.16.4, */
+  int64_t fft_plan_col;    /* 1 = 16.8.8, 2 = 8.16.8, 3 = 8.8.16; N = 2048: 0 = 16.16.8, 1 = 16.8.16,
+                              2 = 8.16.16; other N and chirp-z grids: 0 only); the bins agree to
+                              rounding across plans */
 } gx_frames;
 
 /* ------------------------------------------------------------------------------------

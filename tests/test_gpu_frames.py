@@ -34,6 +34,9 @@ def _case(name):
     if name == 'polymer1024_smooth':
         c, e = polymer_slab(size=(40.0, 40.0, 30.0), seed=4)
         return c, e, dict(r=0.2, qv=2 * np.pi / (0.2 * 1024) * (1 + 1e-9), mq=2.0, fill=True, smooth=10)
+    if name == 'polymer2048_fill_smooth':
+        c, e = polymer_slab(size=(30.0, 30.0, 20.0), seed=8)
+        return c, e, dict(r=0.1, qv=2 * np.pi / (0.1 * 2048) * (1 + 1e-9), mq=3.0, fill=True, smooth=20)
     if name == 'graphite300':          # not a power of two: chirp-z transforms, FP64 (sparse)
         c, e = graphite_slab(size=(16.0, 16.0, 16.0), seed=5)
         return c, e, dict(r=0.25, qv=2 * np.pi / (0.25 * 300) * (1 + 1e-9), mq=2.5, fill=False, smooth=0)
@@ -147,6 +150,40 @@ def test_row_pass_shapes_identical(name, frames):
         sums.append((res['bins'].sum.cpu().numpy().copy(), res['bins'].cnt.cpu().numpy().copy()))
     for s_, c_ in sums[1:]:
         assert np.array_equal(s_, sums[0][0]) and np.array_equal(c_, sums[0][1])
+
+
+_REFS = {}
+
+
+_PLAN_CASES = [(name, frames, shape, pr, pc)
+               for name, frames, n_plans in (('polymer1024_smooth', (300, 304), 4),
+                                             ('polymer2048_fill_smooth', (500, 503), 3))
+               for shape, pr, pc in ((0, 1, 2), (0, 2, 1), (8, 1, 1), (8, 2, 2), (1, 2, 1), (0, 3, 3),
+                                     (8, 3, 0))
+               if pr < n_plans and pc < n_plans]
+
+
+@pytest.mark.parametrize("name,frames,shape,plan_row,plan_col", _PLAN_CASES)
+def test_radix_plans_match_oracle(name, frames, shape, plan_row, plan_col):
+    """Every radix plan of the N = 1024 / 2048 wave transforms, in the row-major and
+    column-major frame-pass shapes, meets the oracle bar (the plans differ from each
+    other only by rounding)."""
+    coords, elements, p = _case(name)
+    ids, ftab, counts, _ = engine.element_table(elements, 12700.0)
+    if name not in _REFS:
+        g = engine.qgrid(p['qv'], p['mq'])
+        _REFS[name] = O.voxelgridmaker(coords, ftab[ids], p['r'], p['qv'], p['mq'], fill_bkg=p['fill'],
+                                       smooth=p['smooth'], phis=g.phis[frames[0]:frames[1]])
+    v = shape | (plan_row << 4) | (plan_col << 6)
+    print(f"[gpu-frames] case={name} frames={frames} shape={shape} plans=({plan_row}, {plan_col})",
+          flush=True)
+    res = engine.run_voxelgrid(coords, elements, p['r'], p['qv'], p['mq'], 12700.0,
+                               fill_bkg=p['fill'], smooth=p['smooth'], f_table=ftab, elem_ids=ids,
+                               frames=frames, variant=v)
+    d = res['runner'].desc
+    assert (d.fft_plan_row, d.fft_plan_col) == (plan_row, plan_col)
+    assert res['plan'].N == int(name[7:11]) and not res['runner'].f64
+    _check(_REFS[name], res)
 
 
 def test_stable_regrouping():
