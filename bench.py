@@ -103,14 +103,26 @@ def main():
         dst.copy_(src)                      # 256 MiB read + 256 MiB write
         runner = res['runner']
         nb = runner.batch
-        keep = (runner.desc.row_variant, runner.desc.inter_rm)
+        d = runner.desc
+        keep = (d.row_variant, d.inter_rm, d.fft_plan_row, d.fft_plan_col)
+        plans = (1 << 4) | (1 << 6)   # radix 16.8.8 in both passes (fixed, so every pass profiles the same code)
         for shape in (0, 8):   # row-major and column-major intermediates, one launch of each kernel
-            engine.set_row_variant(runner.desc, shape)
+            engine.set_row_variant(d, shape | plans)
             for kname in ("gx_frames_stats", "gx_frames_columns", "gx_frames_rowpass", "gx_frames_colpass"):
-                call(kname, ctypes.byref(runner.desc), 0, nb, engine.stream_handle())
-        runner.desc.row_variant, runner.desc.inter_rm = keep
+                call(kname, ctypes.byref(d), 0, nb, engine.stream_handle())
+        d.row_variant, d.inter_rm, d.fft_plan_row, d.fft_plan_col = keep
+        # one detector image at this workload's geometry (512^2, 361 orientations) and one
+        # at the 1.3M-atom slab's (2048^2, 721 orientations, 295^3 I(q); values synthetic)
+        detector_seconds(engine, torch, res, 512, W['max_q'], (0.15, 0, 0), ('psi', 'None', 'None'),
+                         np.linspace(0, 180, 361), 0)
+        g5 = engine.qgrid(2 * np.pi / (0.15 * 2048) * (1 + 1e-9), 3.0)
+        ax5 = g5.crop_axis
+        iq5 = torch.rand((g5.c_n,) * 3, dtype=torch.float64, device=dev)
+        detector_seconds(engine, torch, {'iq64': iq5, 'qx': ax5, 'qy': ax5, 'qz': ax5}, 2048, 3.0,
+                         (0.15, 0, 0), ('psi', 'None', 'None'), np.linspace(0, 180, 721), 0)
         torch.cuda.synchronize()
-        print(json.dumps({"profile_kernels": True, "frames_per_launch": nb, "copy_bytes": 2 * 4 * src.numel()}))
+        print(json.dumps({"profile_kernels": True, "frames_per_launch": nb, "copy_bytes": 2 * 4 * src.numel(),
+                          "plans": [int(d.fft_plan_row), int(d.fft_plan_col)]}))
         return
 
     for _ in range(args.warmup):
@@ -215,8 +227,9 @@ def frame_kernel_times(runner, engine, torch, call, iters):
 
 def counters_profile():
     """Per-kernel figures of profiles/frames_polymer100_N1024_counters.md (rocprofv3 --pmc
-    passes of `bench.py --profile-kernels` with each frame-pass shape): calibrated HBM bytes
-    per frame and the VALU issue share of each row / column pass kernel."""
+    passes of `bench.py --profile-kernels`, each frame-pass shape with radix 16.8.8 in both
+    passes): calibrated HBM bytes per frame and the VALU issue share of each row / column
+    pass kernel."""
     path = os.path.join(ROOT, "profiles", "frames_polymer100_N1024_counters.md")
     out = {}
     try:
@@ -280,7 +293,7 @@ def detector_seconds(engine, torch, res, n_pix, max_q, init_vals, init_axs, psis
     for _ in range(reps):
         engine.run_detector(iq, res['qx'], res['qy'], res['qz'], dplan, True)
     torch.cuda.synchronize()
-    return (time.perf_counter() - t1) / reps
+    return (time.perf_counter() - t1) / max(reps, 1)
 
 
 def host_description():
