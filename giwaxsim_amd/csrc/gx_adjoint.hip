@@ -135,6 +135,10 @@ template <typename C> struct DepositAcc;
 template <> struct DepositAcc<float2> { using T = int32_t; using U = unsigned int; static constexpr int BITS = 31; };
 template <> struct DepositAcc<double2> { using T = long long; using U = unsigned long long; static constexpr int BITS = 62; };
 
+// Atoms per thread in flight in the atom loops of the row-wise kernels: the loops are
+// bound by the latency of the atoms' loads otherwise.
+constexpr int AU_UNROLL = 4;
+
 // Rows per block of the differentiable row pass: RB rows of accumulators in 32 KB of
 // LDS (at most 8), so that three blocks share a CU.
 template <int N, typename C>
@@ -192,32 +196,48 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
   const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
   const double r = d.r, rinv = 1.0 / d.r;
 
-  // 1) deposit: atoms of rows z0-1 .. z0+RB-1 into the block's rows
+  // 1) deposit: atoms of rows z0-1 .. z0+RB-1 into the block's rows, AU_UNROLL atoms
+  //    per thread in flight (every load of a group issued before the first is used)
   const int a_lo = d.row_off[z0 > 0 ? z0 - 1 : 0], a_hi = d.row_off[min(z0 + RB, (int)d.Z)];
-  for (int a = a_lo + threadIdx.x; a < a_hi; a += BLK) {
-    const int lv = d.azrow[a] - z0;  // -1 .. RB-1
-    long long u0;
-    double tu;
-    cic_u(d.ax[a], d.ay[a], s, c, ymin, r, rinv, u0, tu);
-    if (u0 < 0 || u0 + 1 >= N) {
-      atomicAdd(&d.status[2], 1);
-      continue;
-    }
-    const double tv = g.afz[a];
-    const int e = g.aelem[a];
-    const double fr = ftab[2 * e], fi = ftab[2 * e + 1];
+  for (int a = a_lo + threadIdx.x; a < a_hi; a += BLK * AU_UNROLL) {
+    double xs[AU_UNROLL], ys[AU_UNROLL], tvs[AU_UNROLL];
+    int zr[AU_UNROLL], es[AU_UNROLL];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int row = lv + h;
-      if (row < 0 || row >= RB) continue;
-      const double wv = h ? tv : 1.0 - tv;
-      const double sc = rsc[row];
-      const double w0 = wv * (1.0 - tu), w1 = wv * tu;
-      AU *ar = accs + (size_t)row * 2 * N;
-      atomicAdd(&ar[u0], (AU)(AT)llrint((fr * w0) * sc));
-      atomicAdd(&ar[N + u0], (AU)(AT)llrint((fi * w0) * sc));
-      atomicAdd(&ar[u0 + 1], (AU)(AT)llrint((fr * w1) * sc));
-      atomicAdd(&ar[N + u0 + 1], (AU)(AT)llrint((fi * w1) * sc));
+    for (int k = 0; k < AU_UNROLL; ++k) {
+      const int ak = a + k * BLK;
+      const bool ok = ak < a_hi;
+      xs[k] = ok ? d.ax[ak] : 0.0;
+      ys[k] = ok ? d.ay[ak] : 0.0;
+      tvs[k] = ok ? g.afz[ak] : 0.0;
+      zr[k] = ok ? d.azrow[ak] : 0;
+      es[k] = ok ? g.aelem[ak] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < AU_UNROLL; ++k) {
+      if (a + k * BLK >= a_hi) break;
+      const int lv = zr[k] - z0;  // -1 .. RB-1
+      long long u0;
+      double tu;
+      cic_u(xs[k], ys[k], s, c, ymin, r, rinv, u0, tu);
+      if (u0 < 0 || u0 + 1 >= N) {
+        atomicAdd(&d.status[2], 1);
+        continue;
+      }
+      const double tv = tvs[k];
+      const double fr = ftab[2 * es[k]], fi = ftab[2 * es[k] + 1];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int row = lv + h;
+        if (row < 0 || row >= RB) continue;
+        const double wv = h ? tv : 1.0 - tv;
+        const double sc = rsc[row];
+        const double w0 = wv * (1.0 - tu), w1 = wv * tu;
+        AU *ar = accs + (size_t)row * 2 * N;
+        atomicAdd(&ar[u0], (AU)(AT)llrint((fr * w0) * sc));
+        atomicAdd(&ar[N + u0], (AU)(AT)llrint((fi * w0) * sc));
+        atomicAdd(&ar[u0 + 1], (AU)(AT)llrint((fr * w1) * sc));
+        atomicAdd(&ar[N + u0 + 1], (AU)(AT)llrint((fi * w1) * sc));
+      }
     }
   }
   __syncthreads();
@@ -592,40 +612,58 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   T *gp = reinterpret_cast<T *>(g.g_part) + (int64_t)fb * 4 * n;
   double su = 0.0;   // this block's share of sum_a dL/dy'_a
   int am = INT_MAX;  // smallest regrouped index attaining min y'
-  for (int a = a_lo + threadIdx.x; a < a_hi; a += BLK) {
-    if ((d.ax[a] * s + d.ay[a] * c) == ymin) am = min(am, a);
-    const int vr = d.azrow[a] - z0;  // 0 .. RB-1
-    long long u0;
-    double tu;
-    cic_u(d.ax[a], d.ay[a], s, c, ymin, r, rinv, u0, tu);
-    const double tv = g.afz[a];
-    const int e = g.aelem[a];
-    const double fr = ftab[2 * e], fi = ftab[2 * e + 1];
-    double gu = 0.0, gv = 0.0, gfr = 0.0, gfi = 0.0;
-    if (u0 >= 0 && u0 + 1 < N && vr >= 0 && vr < RB) {
-      const C *r0 = rows + vr * N + u0;
-      const C *r1 = r0 + N;
-      const double g00r = (double)r0[0].x, g00i = (double)r0[0].y;
-      const double g01r = (double)r0[1].x, g01i = (double)r0[1].y;
-      const double g10r = (double)r1[0].x, g10i = (double)r1[0].y;
-      const double g11r = (double)r1[1].x, g11i = (double)r1[1].y;
-      // h = Re(conj(dL/dG) f) at the four nodes
-      const double h00 = g00r * fr + g00i * fi, h01 = g01r * fr + g01i * fi;
-      const double h10 = g10r * fr + g10i * fi, h11 = g11r * fr + g11i * fi;
-      const double w00 = (1.0 - tv) * (1.0 - tu), w01 = (1.0 - tv) * tu;
-      const double w10 = tv * (1.0 - tu), w11 = tv * tu;
-      gu = ((1.0 - tv) * (h01 - h00) + tv * (h11 - h10)) * rinv;
-      gv = ((1.0 - tu) * (h10 - h00) + tu * (h11 - h01)) * rinv;
-      gfr = ((w00 * g00r + w01 * g01r) + w10 * g10r) + w11 * g11r;
-      gfi = ((w00 * g00i + w01 * g01i) + w10 * g10i) + w11 * g11i;
-    } else {
-      atomicAdd(&d.status[2], 1);
+  // AU_UNROLL atoms per thread in flight; each thread still takes its atoms in
+  // increasing order (the same partial sums)
+  for (int a0 = a_lo + threadIdx.x; a0 < a_hi; a0 += BLK * AU_UNROLL) {
+    double xs[AU_UNROLL], ys[AU_UNROLL], tvs[AU_UNROLL];
+    int zr[AU_UNROLL], es[AU_UNROLL];
+#pragma unroll
+    for (int k = 0; k < AU_UNROLL; ++k) {
+      const int ak = a0 + k * BLK;
+      const bool ok = ak < a_hi;
+      xs[k] = ok ? d.ax[ak] : 0.0;
+      ys[k] = ok ? d.ay[ak] : 0.0;
+      tvs[k] = ok ? g.afz[ak] : 0.0;
+      zr[k] = ok ? d.azrow[ak] : 0;
+      es[k] = ok ? g.aelem[ak] : 0;
     }
-    gp[a] = (T)gu;
-    gp[n + a] = (T)gv;
-    gp[2 * n + a] = (T)gfr;
-    gp[3 * n + a] = (T)gfi;
-    su += gu;
+#pragma unroll
+    for (int k = 0; k < AU_UNROLL; ++k) {
+      const int a = a0 + k * BLK;
+      if (a >= a_hi) break;
+      if ((xs[k] * s + ys[k] * c) == ymin) am = min(am, a);
+      const int vr = zr[k] - z0;  // 0 .. RB-1
+      long long u0;
+      double tu;
+      cic_u(xs[k], ys[k], s, c, ymin, r, rinv, u0, tu);
+      const double tv = tvs[k];
+      const double fr = ftab[2 * es[k]], fi = ftab[2 * es[k] + 1];
+      double gu = 0.0, gv = 0.0, gfr = 0.0, gfi = 0.0;
+      if (u0 >= 0 && u0 + 1 < N && vr >= 0 && vr < RB) {
+        const C *r0 = rows + vr * N + u0;
+        const C *r1 = r0 + N;
+        const double g00r = (double)r0[0].x, g00i = (double)r0[0].y;
+        const double g01r = (double)r0[1].x, g01i = (double)r0[1].y;
+        const double g10r = (double)r1[0].x, g10i = (double)r1[0].y;
+        const double g11r = (double)r1[1].x, g11i = (double)r1[1].y;
+        // h = Re(conj(dL/dG) f) at the four nodes
+        const double h00 = g00r * fr + g00i * fi, h01 = g01r * fr + g01i * fi;
+        const double h10 = g10r * fr + g10i * fi, h11 = g11r * fr + g11i * fi;
+        const double w00 = (1.0 - tv) * (1.0 - tu), w01 = (1.0 - tv) * tu;
+        const double w10 = tv * (1.0 - tu), w11 = tv * tu;
+        gu = ((1.0 - tv) * (h01 - h00) + tv * (h11 - h10)) * rinv;
+        gv = ((1.0 - tu) * (h10 - h00) + tu * (h11 - h01)) * rinv;
+        gfr = ((w00 * g00r + w01 * g01r) + w10 * g10r) + w11 * g11r;
+        gfi = ((w00 * g00i + w01 * g01i) + w10 * g10i) + w11 * g11i;
+      } else {
+        atomicAdd(&d.status[2], 1);
+      }
+      gp[a] = (T)gu;
+      gp[n + a] = (T)gv;
+      gp[2 * n + a] = (T)gfr;
+      gp[3 * n + a] = (T)gfi;
+      su += gu;
+    }
   }
   su = block_sum(su, shd);
   am = block_min_int(am, shi);

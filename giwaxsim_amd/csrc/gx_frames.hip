@@ -1148,7 +1148,11 @@ __global__ __launch_bounds__(WAVES * GX_WAVE) void k_colpass_rm(gx_frames d, int
     const bool kcol = w < wf;
     const bool k0 = kcol && jlo + w == N / 2;
     const C outside = k0 ? a0 : C{(T)0, (T)0};
-    const C u = (kcol && dense) ? spec[w] : C{(T)0, (T)0};
+    // (branches, not conditional expressions, around the loads: a conditional whose
+    // operands are a memory element and a local value may be compiled to a select of
+    // addresses and a flat load through the private aperture)
+    C u = C{(T)0, (T)0};
+    if (kcol && dense) u = spec[w];
     constexpr int CH = NIT < 8 ? NIT : 8;  // rows in flight per thread
     for (int i0 = 0; i0 < NIT && zt + GX_WAVE * i0 < d.Z; i0 += CH) {
       C x[CH];
@@ -1157,7 +1161,8 @@ __global__ __launch_bounds__(WAVES * GX_WAVE) void k_colpass_rm(gx_frames d, int
       for (int ii = 0; ii < CH; ++ii) {
         const int z = zt + GX_WAVE * (i0 + ii);
         const bool ld = kcol && z < d.Z && (!d.fill_bkg || (z >= zlo && z < zhi));
-        x[ii] = ld ? src[(int64_t)z * d.W + cl] : outside;
+        x[ii] = outside;
+        if (ld) x[ii] = src[(int64_t)z * d.W + cl];
         mz[ii] = (ld && d.smooth) ? sz_row[z] : 1.0;
       }
 #pragma unroll
@@ -1193,7 +1198,10 @@ __global__ __launch_bounds__(WAVES * GX_WAVE) void k_colpass_rm(gx_frames d, int
 #pragma unroll
     for (int q = 0; q < R0; ++q) {
       const int z = lane + GX_WAVE * t + q * (N / R0);
-      v[t * R0 + q] = (z < d.Z) ? tile[z * TS + wave] : col_input(d, (const C *)nullptr, z, k0, u, sz_row);
+      if (z < d.Z)
+        v[t * R0 + q] = tile[z * TS + wave];
+      else
+        v[t * R0 + q] = col_input(d, (const C *)nullptr, z, k0, u, sz_row);
     }
   __syncthreads();  // the tile is read; its LDS becomes the FFT buffers
   if (!active) return;
