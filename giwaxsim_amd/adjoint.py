@@ -172,6 +172,7 @@ class DifferentiableSlab:
         self.g_avg = _empty(2, torch.float64)
         self._records_for = (xb, yb)
         self.g_atom = _empty(5 * n, torch.float64)
+        self.e_sum = _empty(2 * self.n_elem, torch.float64)
         self.g_coords = _empty(3 * n, torch.float64)
         self.g_f = _empty(2 * self.n_elem, torch.float64)
 
@@ -265,6 +266,9 @@ class DifferentiableSlab:
         cdt = torch.float64 if self.f64 else torch.float32
         self.inter = _empty(self.batch * W * Zc * 2, cdt)
         self.g_part = _empty(self.batch * 4 * n, torch.float64 if self.f64 else torch.float32)
+        # per-element dL/df of each frame's row blocks (at most Zc blocks per frame)
+        self.e_part = (_empty(self.batch * Zc * self.n_elem * 2, torch.float64)
+                       if self.n_elem <= _lib.GX_ADJ_ELEM_REG else None)
 
     def batches(self, fr):
         """(f0, nf, descriptor) per batch: the descriptor's statistics pointers
@@ -361,7 +365,7 @@ class DifferentiableSlab:
             g_atom=ptr(self.g_atom).value, x=ptr(self.x).value, y=ptr(self.y).value,
             ext=ptr(self.minmax).value, ecount=ptr(self.ecount).value, vol_r3=r ** 3 / vol,
             avg_spec=ptr(self.avg_spec).value, avg_part=ptr(self.avg_part).value,
-            g_avg=ptr(self.g_avg).value)
+            g_avg=ptr(self.g_avg).value, e_part=ptr(self.e_part).value, e_sum=ptr(self.e_sum).value)
         return fr, adj
 
     def _check_status(self):

@@ -81,6 +81,27 @@ __device__ __forceinline__ int block_min_int(int v, int *sh) {
   return r;
 }
 
+// NV block-wide sums at once, in block_sum's order; sh holds NWAVE * NV doubles
+template <int NV>
+__device__ __forceinline__ void block_sum_multi(double (&v)[NV], double *sh) {
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off, GX_WAVE);
+  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) sh[w * NV + i] = v[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double r = 0.0;
+    for (int k = 0; k < NWAVE; ++k) r += sh[k * NV + i];
+    v[i] = r;
+  }
+}
+
 // ------------------------------------------------------------------------------------
 // preparation
 // ------------------------------------------------------------------------------------
@@ -113,6 +134,7 @@ __global__ __launch_bounds__(BLK) void k_adj_prepare(gx_frames d, gx_adjoint g) 
   const int64_t t0 = blockIdx.x * (int64_t)BLK + threadIdx.x, stride = (int64_t)gridDim.x * BLK;
   const int64_t n = d.n_atoms;
   if (t0 < 2 && g.g_avg) g.g_avg[t0] = 0.0;
+  if (g.e_sum && t0 < 2 * g.n_elem) g.e_sum[t0] = 0.0;
   for (int64_t v = t0; v < d.N; v += stride) g.row_exp[v] = adj_row_exp(d, g, (int)v);
   for (int64_t a = t0; a < n; a += stride) {
     g.ainv[g.aperm[a]] = (int32_t)a;
@@ -612,6 +634,12 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   T *gp = reinterpret_cast<T *>(g.g_part) + (int64_t)fb * 4 * n;
   double su = 0.0;   // this block's share of sum_a dL/dy'_a
   int am = INT_MAX;  // smallest regrouped index attaining min y'
+  // dL/df per element in registers (each thread's atoms in increasing order), reduced
+  // over the block at the end, when there are at most GX_ADJ_ELEM_REG elements
+  const bool esum = g.n_elem <= GX_ADJ_ELEM_REG;
+  double ef[2 * GX_ADJ_ELEM_REG];
+#pragma unroll
+  for (int q = 0; q < 2 * GX_ADJ_ELEM_REG; ++q) ef[q] = 0.0;
   // AU_UNROLL atoms per thread in flight; each thread still takes its atoms in
   // increasing order (the same partial sums)
   for (int a0 = a_lo + threadIdx.x; a0 < a_hi; a0 += BLK * AU_UNROLL) {
@@ -660,8 +688,17 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
       }
       gp[a] = (T)gu;
       gp[n + a] = (T)gv;
-      gp[2 * n + a] = (T)gfr;
-      gp[3 * n + a] = (T)gfi;
+      if (esum) {
+#pragma unroll
+        for (int q = 0; q < GX_ADJ_ELEM_REG; ++q)
+          if (es[k] == q) {
+            ef[2 * q] += gfr;
+            ef[2 * q + 1] += gfi;
+          }
+      } else {
+        gp[2 * n + a] = (T)gfr;
+        gp[3 * n + a] = (T)gfi;
+      }
       su += gu;
     }
   }
@@ -671,6 +708,48 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
     g.red[(int64_t)fb * gridDim.x + blockIdx.x] = su;
     if (am != INT_MAX) atomicMin(&g.amin[fb], am);
   }
+  if (esum) {  // block-uniform
+    __shared__ double she[NWAVE * 2 * GX_ADJ_ELEM_REG];
+    block_sum_multi<2 * GX_ADJ_ELEM_REG>(ef, she);
+    if (threadIdx.x == 0) {
+      double *ep = g.e_part + ((int64_t)fb * gridDim.x + blockIdx.x) * 2 * g.n_elem;
+      for (int q = 0; q < 2 * g.n_elem; ++q) ep[q] = ef[q];
+    }
+  }
+}
+
+// e_sum += the batch's per-frame, per-row-block dL/df sums, in two steps with fixed
+// partitions: grid (RED, n_elem) sums chunk blockIdx.x of element blockIdx.y's nfb
+// entries (in order per thread, then a fixed tree) into red; one block adds the RED
+// chunk sums of each element in order.
+__global__ __launch_bounds__(BLK) void k_adj_esum(gx_adjoint g, int64_t nfb) {
+  __shared__ double shd[NWAVE];
+  const int b = blockIdx.x, e = blockIdx.y;
+  const int64_t chunk = (nfb + RED - 1) / RED;
+  const int64_t i0 = b * chunk, i1 = min(nfb, i0 + chunk);
+  double re = 0.0, im = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += BLK) {
+    re += g.e_part[(i * g.n_elem + e) * 2];
+    im += g.e_part[(i * g.n_elem + e) * 2 + 1];
+  }
+  re = block_sum(re, shd);
+  im = block_sum(im, shd);
+  if (threadIdx.x == 0) {
+    g.red[((int64_t)e * RED + b) * 2] = re;
+    g.red[((int64_t)e * RED + b) * 2 + 1] = im;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void k_adj_esum_fin(gx_adjoint g) {
+  const int e = threadIdx.x;
+  if (e >= g.n_elem) return;
+  double re = 0.0, im = 0.0;
+  for (int b = 0; b < RED; ++b) {
+    re += g.red[((int64_t)e * RED + b) * 2];
+    im += g.red[((int64_t)e * RED + b) * 2 + 1];
+  }
+  g.e_sum[2 * e] += re;
+  g.e_sum[2 * e + 1] += im;
 }
 
 // ------------------------------------------------------------------------------------
@@ -708,6 +787,7 @@ __global__ __launch_bounds__(BLK) void k_adj_reduce(gx_frames d, gx_adjoint g, i
   const int al = threadIdx.x % RA, j = threadIdx.x / RA;
   const int64_t a = blockIdx.x * (int64_t)RA + al;
   const T *gpart = reinterpret_cast<const T *>(g.g_part);
+  const bool esum = g.n_elem <= GX_ADJ_ELEM_REG;  // dL/df is in e_part instead
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (a < n_rowed) {
     for (int64_t fb = j; fb < nf; fb += RF) {
@@ -717,8 +797,10 @@ __global__ __launch_bounds__(BLK) void k_adj_reduce(gx_frames d, gx_adjoint g, i
       acc[0] += fp[GX_FP_SIN] * u;
       acc[1] += fp[GX_FP_COS] * u;
       acc[2] += (double)gp[n + a];
-      acc[3] += (double)gp[2 * n + a];
-      acc[4] += (double)gp[3 * n + a];
+      if (!esum) {
+        acc[3] += (double)gp[2 * n + a];
+        acc[4] += (double)gp[3 * n + a];
+      }
     }
   }
 #pragma unroll
@@ -747,6 +829,7 @@ __global__ __launch_bounds__(BLK) void k_adj_fin_sums(gx_frames d, gx_adjoint g,
   const int64_t chunk = (n + RED - 1) / RED;
   const int64_t i0 = b * chunk, i1 = min(n, i0 + chunk);
   const bool box = d.fill_bkg || d.smooth;
+  const bool esum = g.n_elem <= GX_ADJ_ELEM_REG;
   double s0 = 0.0, s1 = 0.0;
   int am[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX};
   for (int64_t i = i0 + threadIdx.x; i < i1; i += BLK) {
@@ -761,10 +844,14 @@ __global__ __launch_bounds__(BLK) void k_adj_fin_sums(gx_frames d, gx_adjoint g,
         if (g.y[i] == g.ext[3]) am[3] = min(am[3], (int)i);
         if (g.z[i] == g.ext[5]) am[5] = min(am[5], (int)i);
       }
-    } else if (g.aelem[i] == y - 1) {
+    } else if (!esum && g.aelem[i] == y - 1) {
       s0 += g.g_atom[3 * n + i];
       s1 += g.g_atom[4 * n + i];
     }
+  }
+  if (esum && y > 0 && b == 0 && threadIdx.x == 0) {  // dL/df reduced per element already
+    s0 = g.e_sum[2 * (y - 1)];
+    s1 = g.e_sum[2 * (y - 1) + 1];
   }
   s0 = block_sum(s0, shd);
   s1 = block_sum(s1, shd);
@@ -973,6 +1060,8 @@ int check_adj(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, b
              "a required buffer of gx_adjoint is NULL");
   if (backward) {
     GX_REQUIRE(g->g_bins && g->g_part && g->red && g->amin && g->g_atom, "a backward buffer of gx_adjoint is NULL");
+    GX_REQUIRE(g->n_elem > GX_ADJ_ELEM_REG || (g->e_part && g->e_sum),
+               "n_elem <= %d needs gx_adjoint.e_part and e_sum", GX_ADJ_ELEM_REG);
     GX_REQUIRE(!dense || (g->x && g->y && g->ext && g->ecount && g->avg_spec && g->avg_part && g->g_avg),
                "fill_bkg / smooth need gx_adjoint.x, y, ext, ecount, avg_spec, avg_part, g_avg");
   }
@@ -1042,6 +1131,10 @@ int gx_adj_reduce(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t n
       k_adj_reduce<double><<<grid1d(d->n_atoms, RA), BLK, 0, st>>>(*d, *g, f0, nf);
     else
       k_adj_reduce<float><<<grid1d(d->n_atoms, RA), BLK, 0, st>>>(*d, *g, f0, nf);
+  }
+  if (g->n_elem <= GX_ADJ_ELEM_REG) {  // red is free again (k_adj_frame_origin has read it)
+    k_adj_esum<<<dim3(RED, (unsigned)g->n_elem), BLK, 0, st>>>(*g, nf * rowgrad_blocks(d));
+    k_adj_esum_fin<<<1, BLK, 0, st>>>(*g);
   }
   return check_launch("gx_adj_reduce");
 }

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GX_ABI_VERSION 7
+#define GX_ABI_VERSION 8
 
 #define GX_OK 0
 #define GX_ERR_ARG 1         /* invalid argument (message says which) */
@@ -419,6 +419,7 @@ int gx_fit_residual(const double *sim, const double *ref, const uint8_t *mask, i
  * and d->Z > the largest v0 + 1.
  * ---------------------------------------------------------------------------------- */
 #define GX_ADJ_RED_BLOCKS 16
+#define GX_ADJ_ELEM_REG 6   /* up to this many elements, dL/df is reduced per element in-block */
 typedef struct gx_adjoint {
   int64_t n_elem;
   const double *f_table;   /* [n_elem][2] per-element f (re, im) */
@@ -433,7 +434,9 @@ typedef struct gx_adjoint {
   /* backward */
   const double *g_bins;    /* [c^3] dL / d(bin sum of I) (gx_bins_adjoint) */
   void *g_part;            /* [batch][4][n_atoms] per-frame dL/dy', dL/dz, dL/df re, im; float
-                              with FP32 transforms (fft_f64 = 0), double with FP64 */
+                              with FP32 transforms (fft_f64 = 0), double with FP64; with
+                              n_elem <= GX_ADJ_ELEM_REG only dL/dy', dL/dz are written (dL/df
+                              goes to e_part) */
   double *red;             /* [max(batch (N + 1), 2 (1 + n_elem) GX_ADJ_RED_BLOCKS)] reduction scratch */
   int32_t *amin;           /* [batch + 7] argmin / argmax atoms (original index) */
   double *g_atom;          /* [5][n_atoms] accumulated dL/dx, dL/dy, dL/dz, dL/df re, im per
@@ -447,6 +450,10 @@ typedef struct gx_adjoint {
   void *avg_spec;          /* [batch][N + W] double2: FFT of Pz (all k), FFT of Py (kept columns) */
   double *avg_part;        /* [batch][W][2] per-column share of dL/d avg_voxel_f */
   double *g_avg;           /* [2] dL/d avg_voxel_f (re, im), accumulated (zeroed by gx_adj_prepare) */
+  /* n_elem <= GX_ADJ_ELEM_REG only (may be NULL otherwise): dL/df summed per element
+     inside each gradient block instead of per atom */
+  double *e_part;          /* [batch][Z][n_elem][2] per frame and row block (at most Z blocks) */
+  double *e_sum;           /* [n_elem][2] accumulated over batches (zeroed by gx_adj_prepare) */
 } gx_adjoint;
 
 /* Deposit exponents for the current f_table, inverse permutation, zero g_atom. */

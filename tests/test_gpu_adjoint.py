@@ -28,6 +28,7 @@ import numpy as np
 import pytest
 import torch
 
+from giwaxsim_amd import _lib
 from giwaxsim_amd.adjoint import DifferentiableSlab, fit_scale_background, image_loss
 from giwaxsim_amd.synth import graphite_slab
 from oracle.cic_model import CicModel
@@ -38,11 +39,18 @@ R = 0.25
 MAXQ = 2.5
 
 
-def _small(precision, fill=False, smooth=0):
+# more elements than the kernels reduce per element in registers (GX_ADJ_ELEM_REG):
+# the per-atom dL/df path
+MANY = ('C', 'N', 'O', 'S', 'F', 'H', 'Si')
+
+
+def _small(precision, fill=False, smooth=0, many=False):
     coords, el = graphite_slab(size=(16.0, 16.0, 12.0), seed=21)
     # a second element so per-element gradients are exercised
     el = el.copy()
     el[::7] = 'N'
+    if many:
+        el = np.array([MANY[i % len(MANY)] for i in range(len(el))])
     N = 256
     qv = 2 * np.pi / (R * N) * (1 + 1e-9)
     det = dict(num_pixels=48, max_q=MAXQ, psis=list(np.arange(0.0, 91.0, 10.0)), phis=[0.0],
@@ -60,14 +68,16 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("precision,fill,smooth,tol_img,tol_grad",
-                         [('fp64', False, 0, 1e-12, 1e-11), ('fp32', False, 0, 2e-6, 1e-4),
-                          ('fp64', True, 0, 1e-10, 1e-11), ('fp64', False, 3, 1e-10, 1e-11),
-                          ('fp64', True, 3, 1e-10, 1e-11), ('fp32', True, 3, 2e-6, 1e-4)])
-def test_forward_and_gradients_match_autograd(precision, fill, smooth, tol_img, tol_grad):
+@pytest.mark.parametrize("precision,fill,smooth,tol_img,tol_grad,many",
+                         [('fp64', False, 0, 1e-12, 1e-11, False), ('fp32', False, 0, 2e-6, 1e-4, False),
+                          ('fp64', True, 0, 1e-10, 1e-11, False), ('fp64', False, 3, 1e-10, 1e-11, False),
+                          ('fp64', True, 3, 1e-10, 1e-11, False), ('fp32', True, 3, 2e-6, 1e-4, False),
+                          ('fp64', True, 3, 1e-10, 1e-11, True), ('fp32', False, 0, 2e-6, 1e-4, True)])
+def test_forward_and_gradients_match_autograd(precision, fill, smooth, tol_img, tol_grad, many):
     _say(f"[gpu-adjoint] small graphite N=256 frames 0..10 precision={precision} fill={fill} "
-         f"smooth={smooth}")
-    coords, model, ref = _small(precision, fill, smooth)
+         f"smooth={smooth} elements={len(MANY) if many else 2}")
+    coords, model, ref = _small(precision, fill, smooth, many)
+    assert (model.n_elem > _lib.GX_ADJ_ELEM_REG) == many
     dev = torch.device('cuda')
     G = np.random.default_rng(1).normal(size=(48, 48))
     # HIP
