@@ -1103,8 +1103,11 @@ void k_rowpass_rm(gx_frames d, int64_t f0, int64_t nf) {
 // col_input), the tile's LDS becomes the waves' FFT buffers, and the banded FFT along z
 // is followed by |F|^2 and the binning of k_colpass.
 // ------------------------------------------------------------------------------------
+// Four waves per block: with eight, a block's load phase and transform phase alternate in
+// step with too few blocks per CU to overlap them (N = 1024 FP32: 0.128 -> 0.116 ms per
+// 213-frame launch with four; two tie four).
 template <int N, typename C> struct ColRm {
-  static constexpr int WAVES = (8 * fft_band_buf<N>() * (int)sizeof(C) <= 80 * 1024) ? 8 : 4;
+  static constexpr int WAVES = 4;
 };
 
 template <int N, typename C, int WAVES, int PL = 0>
