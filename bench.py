@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--kernel-iters", type=int, default=20)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble (64 slabs) and large-slab (N=2048) figures")
-    ap.add_argument("--only", choices=["adjoint", "adjoint_fp64", "adjoint_polymer", "adjoint_large",
+    ap.add_argument("--only", choices=["adjoint", "adjoint_fp64", "adjoint_polymer", "adjoint_large", "adjoint_fit",
                                        "ensemble", "large_slab", "compare", "slab_build", "fit_eval",
                                        "graphite50"],
                     help="run one of the extra workloads alone and print its JSON (for profiling)")
@@ -77,6 +77,8 @@ def main():
             print(json.dumps(adjoint_polymer_config(engine, torch, reps=args.steps)))
         elif args.only == "adjoint_large":
             print(json.dumps(adjoint_large_config(engine, torch, n_slabs=args.steps)))
+        elif args.only == "adjoint_fit":
+            print(json.dumps(adjoint_fit_config(engine, torch, reps=args.steps)))
         elif args.only.startswith("adjoint"):
             prec = "fp64" if args.only == "adjoint_fp64" else "fp32"
             print(json.dumps(adjoint_config(engine, torch, reps=args.steps, precision=prec)))
@@ -390,9 +392,10 @@ def _plain_forward_ms(engine, torch, coords, elements, r, qv, mq, det, reps, fil
     ct = torch.as_tensor(coords).cuda()
     ids, ftab, _, _ = engine.element_table(elements, 12700.0)
     idd = torch.as_tensor(ids).cuda()
-    psis = np.asarray(det['psis'], dtype=np.float64)
+    psis, phis, thetas = (np.asarray(det.get(k, [0.0]), dtype=np.float64) for k in ('psis', 'phis', 'thetas'))
+    uni = lambda a: np.full(len(a), 1.0 / len(a))
     dplan = engine.detector_plan(det['num_pixels'], mq, det['angle_init_vals'], det['angle_init_axs'], psis,
-                                 np.full(len(psis), 1.0 / len(psis)), [0.0], [1.0], [0.0], [1.0])
+                                 uni(psis), phis, uni(phis), thetas, uni(thetas))
 
     def ngp():
         res = engine.run_voxelgrid(ct, None, r, qv, mq, 12700.0, fill_bkg=fill, smooth=smooth, f_table=ftab,
@@ -461,6 +464,31 @@ def adjoint_polymer_config(engine, torch, reps=3):
             "fwd_phi_frames_per_s": F / (fwd * 1e-3), "fwd_bwd_phi_frames_per_s": F / (fb * 1e-3)}
 
 
+def adjoint_fit_config(engine, torch, reps=3):
+    """The differentiable variant at the reference's fit geometry
+    (config_templates/fit_slabsize_config.txt): the PM6 cell tiled and cropped to
+    176 x 43 x 186 A (66k atoms), r 0.3 A, q_voxel 0.02 -> N = 1048 (zoomed chirp-z
+    band transforms and their adjoints), max_q 2, fill_bkg, smooth 25, 892 phi-frames,
+    250^2 detector, psi 0..90 x 91 and phi 0..179 x 180 (16,380 orientations) after the
+    90-degree psi / phi / psi initial rotations; FP32 transforms."""
+    from giwaxsim_amd.adjoint import DifferentiableSlab
+    from giwaxsim_amd.tools.comparison import slabmaker_fitting
+    coords, el = slabmaker_fitting(os.path.join(ROOT, "tests", "golden", "PM6_sample.pdb"), 176.0, 43.0, 186.0,
+                                   44.456, 45.726, 40.097, 90.0, 90.0, 90.0)
+    det = dict(num_pixels=250, max_q=2.0, psis=list(np.linspace(0, 90, 91)), phis=list(np.linspace(0, 179, 180)),
+               thetas=[0.0], angle_init_vals=(90.0, 90.0, 90.0), angle_init_axs=('psi', 'phi', 'psi'), mirror=True)
+    model = DifferentiableSlab(el, coords, 0.3, 0.02, 2.0, det, fill_bkg=True, smooth=25)
+    fwd, fb = _time_adjoint(model, coords, engine, torch, reps)
+    ngp_ms = _plain_forward_ms(engine, torch, coords, el, 0.3, 0.02, 2.0, det, max(reps, 3), fill=True, smooth=25)
+    F = model.plan.n_frames
+    return {"workload": "PM6 slab 176 x 43 x 186 A (fit geometry), N=1048 (chirp-z), fill_bkg, smooth 25, "
+                        "892 phi-frames, 250^2 detector x 16,380 orientations; cloud-in-cell deposit, FP32 "
+                        "transforms; gradients w.r.t. all coordinates and per-element f",
+            "n_atoms": len(coords), "grid": model.plan.N, "frames": F, "batch": model.batch, "fwd_ms": fwd,
+            "fwd_bwd_ms": fb, "bwd_ms": fb - fwd, "fwd_bwd_over_fwd": fb / fwd,
+            "reference_parity_fwd_ms": ngp_ms, "fwd_bwd_over_reference_parity_fwd": fb / ngp_ms}
+
+
 def adjoint_large_config(engine, torch, n_slabs=16):
     """Large slabs through the differentiable variant: n_slabs polymer slabs of
     200 x 200 x 60 A (~1.3M atoms each, seeds 0..n-1: seeded crystallite
@@ -512,6 +540,7 @@ def extra_configs(args, engine, torch, which=None):
         out["adjoint"] = adjoint_config(engine, torch, precision='fp32')
         out["adjoint_fp64"] = adjoint_config(engine, torch, precision='fp64')
         out["adjoint_polymer"] = adjoint_polymer_config(engine, torch)
+        out["adjoint_fit"] = adjoint_fit_config(engine, torch)
         out["adjoint_large"] = adjoint_large_config(engine, torch, n_slabs=16)
     reps = 3
     if which in (None, "graphite50"):

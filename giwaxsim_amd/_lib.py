@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_void_p
 from .build import LIB
 
 GX_OK = 0
-GX_ABI_VERSION = 8   # structure layouts below match this version of the header
+GX_ABI_VERSION = 9   # structure layouts below match this version of the header
 GX_FP_STRIDE = 24
 GX_MOMENTS_WORK = 774
 GX_CROP_NOALLOC_MAX_ATOMS = 134217728
@@ -94,7 +94,7 @@ class GxAdjoint(ctypes.Structure):
         ("g_atom", c_void_p),
         ("x", c_void_p), ("y", c_void_p), ("ext", c_void_p), ("ecount", c_void_p),
         ("vol_r3", c_double), ("avg_spec", c_void_p), ("avg_part", c_void_p), ("g_avg", c_void_p),
-        ("e_part", c_void_p), ("e_sum", c_void_p),
+        ("e_part", c_void_p), ("e_sum", c_void_p), ("bs_kernel_adj", c_void_p), ("bs_kernel_adj_c", c_void_p),
     ]
 
 

@@ -22,8 +22,10 @@ masks M and a fixed pattern p (the filled region and smoothed edges follow the
 integer index bounds, which are piecewise constant), and avg_voxel_f = sum f r^3
 / (x_b y_b z_b) carries gradient to f and to the atoms that fix the bounding box.
 
-Limits: power-of-two grid edges (256..2048), every atom at least one voxel
-inside the grid edge.
+Grid edges that are powers of two (256..2048) use the in-wave transforms; other edges
+in [16, 2048] (the reference's fit geometry has N = 1048) the forward path's zoomed
+chirp-z band transforms and their adjoints (band values -> grid points through the same
+block transforms).  Every atom at least one voxel inside the grid edge.
 """
 
 from __future__ import annotations
@@ -37,7 +39,8 @@ import torch
 
 from . import _lib
 from ._lib import GxAdjoint, GxDetector, GxFrames, GxSlab, call
-from .engine import (_empty, bin_scale_for, collapse_records, detector_plan, device, element_table,
+from .engine import (_empty, bin_scale_for, chirpz_adjoint_tables, chirpz_band_tables, chirpz_bands,
+                     chirpz_tables, collapse_records, detector_plan, device, element_table,
                      fft_precision, frame_plan, gauss_prefix, intensity_bound, is_pow2_grid,
                      pixels_per_bin, ptr, stream_handle, twiddles)
 from .tools.ptable_dict import aff_dict, ptable
@@ -104,9 +107,8 @@ class DifferentiableSlab:
         xb, yb, zb = (float(np.ptp(c0[:, i])) for i in range(3))
         self.r = r = float(r_voxel_size)
         plan = frame_plan(r, float(q_voxel_size), float(max_q), xb, yb, zb)
-        if not is_pow2_grid(plan.N):
-            raise NotImplementedError(f"the differentiable path needs a power-of-two grid edge "
-                                      f"256..2048, got N={plan.N}")
+        if not is_pow2_grid(plan.N) and not 16 <= plan.N <= 2048:
+            raise NotImplementedError(f"the differentiable path needs a grid edge in [16, 2048], got N={plan.N}")
         self.plan = plan
         N, W = plan.N, plan.W
         F = plan.n_frames
@@ -139,6 +141,18 @@ class DifferentiableSlab:
         self.fparams = torch.as_tensor(plan.fparams).to(dev).contiguous()
         self.tw = torch.as_tensor(twiddles(N, self.f64)).to(dev).contiguous()
         self.tw64 = torch.as_tensor(twiddles(N, True)).to(dev).contiguous()
+        # edges that are not powers of two: the forward path's zoomed chirp-z bands and
+        # the kernels of their adjoints (band values -> grid points)
+        self.bs = None
+        if not is_pow2_grid(N):
+            t = lambda a: torch.as_tensor(a).to(dev).contiguous()
+            _, ch, _, _ = chirpz_tables(N, self.f64)
+            (rlo, rk), (clo, ck) = chirpz_bands(plan)
+            M, Kr, twr = chirpz_band_tables(N, rlo, rk, self.f64)
+            Mc, Kc, twc = chirpz_band_tables(N, clo, ck, self.f64)
+            self.bs = dict(M=M, Mc=Mc, rlo=rlo, rk=rk, clo=clo, ck=ck, chirp=t(ch), K=t(Kr), tw=t(twr),
+                           Kc=t(Kc), twc=t(twc), Kadj=t(chirpz_adjoint_tables(N, rlo, rk, M, self.f64)),
+                           Kadj_c=t(chirpz_adjoint_tables(N, clo, ck, Mc, self.f64)))
         cdt = torch.float64 if self.f64 else torch.float32
         # per-frame statistics of every frame (the backward reuses the forward's)
         self.ymin_key = _empty(3 * F, torch.int64)
@@ -241,7 +255,7 @@ class DifferentiableSlab:
     def _rows_for(self, zb):
         """Row range [0, Z) of a slab of z extent zb: rows v0 and v0 + 1 of every atom."""
         N, r = self.plan.N, self.r
-        return int(min(N, max(8, -(-(int(np.floor_divide(zb, r)) + 2) // 8) * 8)))
+        return int(min(N // 8 * 8, max(8, -(-(int(np.floor_divide(zb, r)) + 2) // 8) * 8)))
 
     def _size_batch(self, Z):
         """(Re)size the per-batch buffers for row range Z (with 16 rows of slack)."""
@@ -356,6 +370,12 @@ class DifferentiableSlab:
             ymin_key=ptr(self.ymin_key).value, idx_bounds=ptr(self.idx_bounds).value,
             inter=ptr(self.inter).value, bin_sum=ptr(self.bin_sum).value,
             bin_cnt=ptr(self.bin_cnt).value, status=ptr(self.status).value)
+        if self.bs is not None:
+            b = self.bs
+            fr.bs_M, fr.bs_Mc = b['M'], b['Mc']
+            fr.bs_rlo, fr.bs_rk, fr.bs_clo, fr.bs_ck = b['rlo'], b['rk'], b['clo'], b['ck']
+            fr.bs_chirp, fr.bs_kernel, fr.bs_twiddle = ptr(b['chirp']).value, ptr(b['K']).value, ptr(b['tw']).value
+            fr.bs_kernel_c, fr.bs_twiddle_c = ptr(b['Kc']).value, ptr(b['twc']).value
         adj = GxAdjoint(
             n_elem=self.n_elem, f_table=ptr(self.ftab).value, afz=ptr(self.szf).value,
             aperm=ptr(self.sperm).value, aelem=ptr(self.selem).value, ecnt=ptr(self.work).value,
@@ -365,7 +385,9 @@ class DifferentiableSlab:
             g_atom=ptr(self.g_atom).value, x=ptr(self.x).value, y=ptr(self.y).value,
             ext=ptr(self.minmax).value, ecount=ptr(self.ecount).value, vol_r3=r ** 3 / vol,
             avg_spec=ptr(self.avg_spec).value, avg_part=ptr(self.avg_part).value,
-            g_avg=ptr(self.g_avg).value, e_part=ptr(self.e_part).value, e_sum=ptr(self.e_sum).value)
+            g_avg=ptr(self.g_avg).value, e_part=ptr(self.e_part).value, e_sum=ptr(self.e_sum).value,
+            bs_kernel_adj=ptr(self.bs['Kadj']).value if self.bs else None,
+            bs_kernel_adj_c=ptr(self.bs['Kadj_c']).value if self.bs else None)
         return fr, adj
 
     def _check_status(self):

@@ -525,6 +525,109 @@ __global__ __launch_bounds__(BLK) void k_adj_avg_reduce(gx_frames d, gx_adjoint 
 // ------------------------------------------------------------------------------------
 // adjoint of the row pass: rows of the grid's cotangent, gathered at the atoms
 // ------------------------------------------------------------------------------------
+// The gather of the gradient kernels, once `rows` holds the masked cotangent of grid rows
+// z0 .. z0 + RB (row stride N): every atom of rows z0 .. z0 + RB - 1 gathers its four
+// nodes and writes its dL/dy', dL/dz (and, with more than GX_ADJ_ELEM_REG elements,
+// dL/df re, im) for this frame to g_part; the block's share of sum_a dL/dy'_a (fixed
+// partition of the regrouped order) goes to red, the argmin of y' to amin and, with
+// at most GX_ADJ_ELEM_REG elements, the block's per-element dL/df to e_part.
+template <typename C>
+__device__ __forceinline__ void rowgrad_gather(const gx_frames &d, const gx_adjoint &g, const double *fp, int fb,
+                                               int z0, int RB, int N, const C *rows, const double *ftab,
+                                               double *shd, int *shi) {
+  using T = typename Real<C>::T;
+  const int64_t n = d.n_atoms;
+  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
+  const double r = d.r, rinv = 1.0 / d.r;
+  const int a_lo = d.row_off[z0], a_hi = d.row_off[min(z0 + RB, (int)d.Z)];
+  T *gp = reinterpret_cast<T *>(g.g_part) + (int64_t)fb * 4 * n;
+  double su = 0.0;   // this block's share of sum_a dL/dy'_a
+  int am = INT_MAX;  // smallest regrouped index attaining min y'
+  // dL/df per element in registers (each thread's atoms in increasing order), reduced
+  // over the block at the end, when there are at most GX_ADJ_ELEM_REG elements
+  const bool esum = g.n_elem <= GX_ADJ_ELEM_REG;
+  double ef[2 * GX_ADJ_ELEM_REG];
+#pragma unroll
+  for (int q = 0; q < 2 * GX_ADJ_ELEM_REG; ++q) ef[q] = 0.0;
+  // AU_UNROLL atoms per thread in flight; each thread still takes its atoms in
+  // increasing order (the same partial sums)
+  for (int a0 = a_lo + threadIdx.x; a0 < a_hi; a0 += BLK * AU_UNROLL) {
+    double xs[AU_UNROLL], ys[AU_UNROLL], tvs[AU_UNROLL];
+    int zr[AU_UNROLL], es[AU_UNROLL];
+#pragma unroll
+    for (int k = 0; k < AU_UNROLL; ++k) {
+      const int ak = a0 + k * BLK;
+      const bool ok = ak < a_hi;
+      xs[k] = ok ? d.ax[ak] : 0.0;
+      ys[k] = ok ? d.ay[ak] : 0.0;
+      tvs[k] = ok ? g.afz[ak] : 0.0;
+      zr[k] = ok ? d.azrow[ak] : 0;
+      es[k] = ok ? g.aelem[ak] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < AU_UNROLL; ++k) {
+      const int a = a0 + k * BLK;
+      if (a >= a_hi) break;
+      if ((xs[k] * s + ys[k] * c) == ymin) am = min(am, a);
+      const int vr = zr[k] - z0;  // 0 .. RB-1
+      long long u0;
+      double tu;
+      cic_u(xs[k], ys[k], s, c, ymin, r, rinv, u0, tu);
+      const double tv = tvs[k];
+      const double fr = ftab[2 * es[k]], fi = ftab[2 * es[k] + 1];
+      double gu = 0.0, gv = 0.0, gfr = 0.0, gfi = 0.0;
+      if (u0 >= 0 && u0 + 1 < N && vr >= 0 && vr < RB) {
+        const C *r0 = rows + vr * N + u0;
+        const C *r1 = r0 + N;
+        const double g00r = (double)r0[0].x, g00i = (double)r0[0].y;
+        const double g01r = (double)r0[1].x, g01i = (double)r0[1].y;
+        const double g10r = (double)r1[0].x, g10i = (double)r1[0].y;
+        const double g11r = (double)r1[1].x, g11i = (double)r1[1].y;
+        // h = Re(conj(dL/dG) f) at the four nodes
+        const double h00 = g00r * fr + g00i * fi, h01 = g01r * fr + g01i * fi;
+        const double h10 = g10r * fr + g10i * fi, h11 = g11r * fr + g11i * fi;
+        const double w00 = (1.0 - tv) * (1.0 - tu), w01 = (1.0 - tv) * tu;
+        const double w10 = tv * (1.0 - tu), w11 = tv * tu;
+        gu = ((1.0 - tv) * (h01 - h00) + tv * (h11 - h10)) * rinv;
+        gv = ((1.0 - tu) * (h10 - h00) + tu * (h11 - h01)) * rinv;
+        gfr = ((w00 * g00r + w01 * g01r) + w10 * g10r) + w11 * g11r;
+        gfi = ((w00 * g00i + w01 * g01i) + w10 * g10i) + w11 * g11i;
+      } else {
+        atomicAdd(&d.status[2], 1);
+      }
+      gp[a] = (T)gu;
+      gp[n + a] = (T)gv;
+      if (esum) {
+#pragma unroll
+        for (int q = 0; q < GX_ADJ_ELEM_REG; ++q)
+          if (es[k] == q) {
+            ef[2 * q] += gfr;
+            ef[2 * q + 1] += gfi;
+          }
+      } else {
+        gp[2 * n + a] = (T)gfr;
+        gp[3 * n + a] = (T)gfi;
+      }
+      su += gu;
+    }
+  }
+  su = block_sum(su, shd);
+  am = block_min_int(am, shi);
+  if (threadIdx.x == 0) {
+    g.red[(int64_t)fb * gridDim.x + blockIdx.x] = su;
+    if (am != INT_MAX) atomicMin(&g.amin[fb], am);
+  }
+  if (esum) {  // block-uniform
+    __shared__ double she[NWAVE * 2 * GX_ADJ_ELEM_REG];
+    block_sum_multi<2 * GX_ADJ_ELEM_REG>(ef, she);
+    if (threadIdx.x == 0) {
+      double *ep = g.e_part + ((int64_t)fb * gridDim.x + blockIdx.x) * 2 * g.n_elem;
+      for (int q = 0; q < 2 * g.n_elem; ++q) ep[q] = ef[q];
+    }
+  }
+}
+
 template <int N, typename C>
 struct RowGradShape {
   // H rows per block in 32 KB of LDS (64 KB measured slower: fewer blocks per CU):
@@ -627,95 +730,7 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   }
   __syncthreads();
 
-  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
-  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
-  const double r = d.r, rinv = 1.0 / d.r;
-  const int a_lo = d.row_off[z0], a_hi = d.row_off[min(z0 + RB, (int)d.Z)];
-  T *gp = reinterpret_cast<T *>(g.g_part) + (int64_t)fb * 4 * n;
-  double su = 0.0;   // this block's share of sum_a dL/dy'_a
-  int am = INT_MAX;  // smallest regrouped index attaining min y'
-  // dL/df per element in registers (each thread's atoms in increasing order), reduced
-  // over the block at the end, when there are at most GX_ADJ_ELEM_REG elements
-  const bool esum = g.n_elem <= GX_ADJ_ELEM_REG;
-  double ef[2 * GX_ADJ_ELEM_REG];
-#pragma unroll
-  for (int q = 0; q < 2 * GX_ADJ_ELEM_REG; ++q) ef[q] = 0.0;
-  // AU_UNROLL atoms per thread in flight; each thread still takes its atoms in
-  // increasing order (the same partial sums)
-  for (int a0 = a_lo + threadIdx.x; a0 < a_hi; a0 += BLK * AU_UNROLL) {
-    double xs[AU_UNROLL], ys[AU_UNROLL], tvs[AU_UNROLL];
-    int zr[AU_UNROLL], es[AU_UNROLL];
-#pragma unroll
-    for (int k = 0; k < AU_UNROLL; ++k) {
-      const int ak = a0 + k * BLK;
-      const bool ok = ak < a_hi;
-      xs[k] = ok ? d.ax[ak] : 0.0;
-      ys[k] = ok ? d.ay[ak] : 0.0;
-      tvs[k] = ok ? g.afz[ak] : 0.0;
-      zr[k] = ok ? d.azrow[ak] : 0;
-      es[k] = ok ? g.aelem[ak] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < AU_UNROLL; ++k) {
-      const int a = a0 + k * BLK;
-      if (a >= a_hi) break;
-      if ((xs[k] * s + ys[k] * c) == ymin) am = min(am, a);
-      const int vr = zr[k] - z0;  // 0 .. RB-1
-      long long u0;
-      double tu;
-      cic_u(xs[k], ys[k], s, c, ymin, r, rinv, u0, tu);
-      const double tv = tvs[k];
-      const double fr = ftab[2 * es[k]], fi = ftab[2 * es[k] + 1];
-      double gu = 0.0, gv = 0.0, gfr = 0.0, gfi = 0.0;
-      if (u0 >= 0 && u0 + 1 < N && vr >= 0 && vr < RB) {
-        const C *r0 = rows + vr * N + u0;
-        const C *r1 = r0 + N;
-        const double g00r = (double)r0[0].x, g00i = (double)r0[0].y;
-        const double g01r = (double)r0[1].x, g01i = (double)r0[1].y;
-        const double g10r = (double)r1[0].x, g10i = (double)r1[0].y;
-        const double g11r = (double)r1[1].x, g11i = (double)r1[1].y;
-        // h = Re(conj(dL/dG) f) at the four nodes
-        const double h00 = g00r * fr + g00i * fi, h01 = g01r * fr + g01i * fi;
-        const double h10 = g10r * fr + g10i * fi, h11 = g11r * fr + g11i * fi;
-        const double w00 = (1.0 - tv) * (1.0 - tu), w01 = (1.0 - tv) * tu;
-        const double w10 = tv * (1.0 - tu), w11 = tv * tu;
-        gu = ((1.0 - tv) * (h01 - h00) + tv * (h11 - h10)) * rinv;
-        gv = ((1.0 - tu) * (h10 - h00) + tu * (h11 - h01)) * rinv;
-        gfr = ((w00 * g00r + w01 * g01r) + w10 * g10r) + w11 * g11r;
-        gfi = ((w00 * g00i + w01 * g01i) + w10 * g10i) + w11 * g11i;
-      } else {
-        atomicAdd(&d.status[2], 1);
-      }
-      gp[a] = (T)gu;
-      gp[n + a] = (T)gv;
-      if (esum) {
-#pragma unroll
-        for (int q = 0; q < GX_ADJ_ELEM_REG; ++q)
-          if (es[k] == q) {
-            ef[2 * q] += gfr;
-            ef[2 * q + 1] += gfi;
-          }
-      } else {
-        gp[2 * n + a] = (T)gfr;
-        gp[3 * n + a] = (T)gfi;
-      }
-      su += gu;
-    }
-  }
-  su = block_sum(su, shd);
-  am = block_min_int(am, shi);
-  if (threadIdx.x == 0) {
-    g.red[(int64_t)fb * gridDim.x + blockIdx.x] = su;
-    if (am != INT_MAX) atomicMin(&g.amin[fb], am);
-  }
-  if (esum) {  // block-uniform
-    __shared__ double she[NWAVE * 2 * GX_ADJ_ELEM_REG];
-    block_sum_multi<2 * GX_ADJ_ELEM_REG>(ef, she);
-    if (threadIdx.x == 0) {
-      double *ep = g.e_part + ((int64_t)fb * gridDim.x + blockIdx.x) * 2 * g.n_elem;
-      for (int q = 0; q < 2 * g.n_elem; ++q) ep[q] = ef[q];
-    }
-  }
+  rowgrad_gather<C>(d, g, fp, fb, z0, RB, N, rows, ftab, shd, shi);
 }
 
 // e_sum += the batch's per-frame, per-row-block dL/df sums, in two steps with fixed
@@ -750,6 +765,403 @@ __global__ __launch_bounds__(BLK) void k_adj_esum_fin(gx_adjoint g) {
   }
   g.e_sum[2 * e] += re;
   g.e_sum[2 * e + 1] += im;
+}
+
+// ------------------------------------------------------------------------------------
+// grid edges that are not powers of two: the differentiable passes on the forward
+// path's zoomed chirp-z band transforms (block-wide, length M)
+//
+// The band transform X = S F_N x has the adjoint x_bar = conj(F_N^T conj(S^T X_bar)): the
+// band DFT run from the K band values (k = k_lo + j) to the N grid points o.  With
+// o k = (o^2 + k^2 - (k - o)^2) / 2 it is c[o] sum_j (u_j c[|k_j|]) conj(c[k_lo - (o - j)]),
+// a convolution in the lag o - j, evaluated like the forward band with the same length M
+// and twiddles and the kernel spectrum gx_adjoint.bs_kernel_adj (_c for columns).
+// ------------------------------------------------------------------------------------
+constexpr int BS_RB = 8;       // rows per block of the chirp-z row pass
+constexpr int BS_GRAD_H = 4;   // rows per block of the chirp-z gradient gather (+ halo)
+
+// forward: cloud-in-cell deposit of rows v - 1 and v, node mask, band transform along y,
+// staged band stored transposed (as k_adj_rowpass)
+template <int M, typename C>
+__global__ __launch_bounds__(GX_BLK_FFT) void k_adj_rowpass_bs(gx_frames d, gx_adjoint g, int64_t f0) {
+  using T = typename Real<C>::T;
+  using AT = typename DepositAcc<C>::T;
+  using AU = typename DepositAcc<C>::U;
+  constexpr int P = M / GX_BLK_FFT;
+  constexpr int R = BlkPlan<M>::R;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *fbuf = reinterpret_cast<C *>(smem_raw);  // [M] transform exchange; aliased by the accumulators [2][N]
+  AU *acc = reinterpret_cast<AU *>(fbuf);
+  const int N = (int)d.N;
+  C *stage = fbuf + M;                                    // [W][BS_RB]
+  C *tspec = stage + d.W * BS_RB;                         // [W]
+  double *ftab = reinterpret_cast<double *>(tspec + d.W);  // [n_elem][2]
+  const int tid = threadIdx.x;
+  const int fb = blockIdx.y;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = jhi - jlo + 1;
+  if (wf <= 0) return;  // block-uniform
+  if (wf > d.W || jlo < d.bs_rlo + N / 2 || jhi > d.bs_rlo + d.bs_rk - 1 + N / 2) {
+    if (tid == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
+    return;
+  }
+  const bool dense = d.fill_bkg || d.smooth;
+  if (dense) {
+    const C *spec = reinterpret_cast<const C *>(d.col_spec) + (int64_t)fb * d.W;
+    for (int w = tid; w < wf; w += GX_BLK_FFT) tspec[w] = spec[w];
+  }
+  for (int i = tid; i < 2 * g.n_elem; i += GX_BLK_FFT) ftab[i] = g.f_table[i];
+  const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
+  const double ymin = dkey_inv(d.ymin_key[3 * fb]);
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
+  const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
+  const double r = d.r, rinv = 1.0 / d.r;
+  const double avv_re = d.avg_re * d.max_voxels, avv_im = d.avg_im * d.max_voxels;
+  const C *chirp = reinterpret_cast<const C *>(d.bs_chirp);
+  const C *kern = reinterpret_cast<const C *>(d.bs_kernel);
+  const C *tw = reinterpret_cast<const C *>(d.bs_twiddle);
+  const int z0 = blockIdx.x * BS_RB;
+  __syncthreads();
+  for (int rb = 0; rb < BS_RB; ++rb) {
+    const int v = z0 + rb;
+    const bool inside = !d.fill_bkg || (v >= zlo && v < zhi);
+    const int a_lo = d.row_off[v > 0 ? v - 1 : 0], a_hi = d.row_off[v + 1];
+    const bool atoms = inside && a_hi > a_lo;  // block-uniform
+    C x[P];
+    if (atoms) {
+      for (int y = tid; y < 2 * N; y += GX_BLK_FFT) acc[y] = 0;
+      __syncthreads();
+      const double sc = ldexp(1.0, g.row_exp[v]);
+      for (int a = a_lo + tid; a < a_hi; a += GX_BLK_FFT) {
+        const bool own = d.azrow[a] == v;  // row v: weight 1 - tv; row v - 1: tv
+        long long u0;
+        double tu;
+        cic_u(d.ax[a], d.ay[a], s, c, ymin, r, rinv, u0, tu);
+        if (u0 < 0 || u0 + 1 >= N) {
+          atomicAdd(&d.status[2], 1);
+          continue;
+        }
+        const double tv = g.afz[a];
+        const int e = g.aelem[a];
+        const double fr = ftab[2 * e], fi = ftab[2 * e + 1];
+        const double wv = own ? 1.0 - tv : tv;
+        const double w0 = wv * (1.0 - tu), w1 = wv * tu;
+        atomicAdd(&acc[u0], (AU)(AT)llrint((fr * w0) * sc));
+        atomicAdd(&acc[N + u0], (AU)(AT)llrint((fi * w0) * sc));
+        atomicAdd(&acc[u0 + 1], (AU)(AT)llrint((fr * w1) * sc));
+        atomicAdd(&acc[N + u0 + 1], (AU)(AT)llrint((fi * w1) * sc));
+      }
+      __syncthreads();
+      const double inv = 1.0 / sc;
+#pragma unroll
+      for (int t = 0; t < P / R; ++t)
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int n = tid + GX_BLK_FFT * t + q * (M / R);
+          C xv = C{(T)0, (T)0};
+          if (n < N) {
+            T m = (T)0;
+            if (n >= ya && n < yb) m = d.smooth ? (T)d.col_sy[(int64_t)fb * N + n] : (T)1;
+            xv = cmul(C{(T)((double)(AT)acc[n] * inv) * m, (T)((double)(AT)acc[N + n] * inv) * m}, chirp[n]);
+          }
+          x[t * R + q] = xv;
+        }
+      __syncthreads();
+      chirpz_block<M, C>(x, fbuf, kern, tw, tid);
+    } else {
+#pragma unroll
+      for (int q = 0; q < P; ++q) x[q] = C{(T)0, (T)0};
+    }
+    const double mz = d.smooth ? d.col_sz[(int64_t)fb * N + v] : 1.0;
+#pragma unroll
+    for (int t = 0; t < P / R; ++t)
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int j = tid + GX_BLK_FFT * t + q * (M / R);
+        if (j >= (int)d.bs_rk) continue;
+        const int k = (int)d.bs_rlo + j;  // signed frequency
+        const int w = k + N / 2 - jlo;    // fftshifted column within the frame's band
+        if (w < 0 || w >= wf) continue;
+        C o = C{(T)0, (T)0};
+        if (atoms) o = cmul(x[t * R + q], chirp[k < 0 ? -k : k]);
+        if (!inside) {
+          o = (k == 0) ? C{(T)(avv_re * (double)N), (T)(avv_im * (double)N)} : C{(T)0, (T)0};
+        } else if (dense) {
+          const C u = tspec[w];
+          double ore = mz * ((double)o.x + (double)u.x);
+          double oim = mz * ((double)o.y + (double)u.y);
+          if (d.smooth && k == 0) {
+            ore = ore + avv_re * (double)N;
+            oim = oim + avv_im * (double)N;
+          }
+          o = C{(T)ore, (T)oim};
+        }
+        stage[w * BS_RB + rb] = o;
+      }
+    __syncthreads();
+  }
+  C *out = reinterpret_cast<C *>(d.inter) + (int64_t)fb * d.W * d.Z;
+  for (int idx = tid; idx < wf * BS_RB; idx += GX_BLK_FFT) {
+    const int w = idx / BS_RB, q = idx - w * BS_RB;
+    if (z0 + q < d.Z) out[(int64_t)w * d.Z + z0 + q] = stage[idx];
+  }
+}
+
+// adjoint of the chirp-z column pass (in place on d.inter), one block per kept column:
+// band transform along z, cotangent 2 (dL/d sum)[bin] F on binned pixels (and the
+// column's share of dL/d avg), adjoint band transform back to z < Z
+template <int M, typename C>
+__global__ __launch_bounds__(GX_BLK_FFT) void k_adj_colpass_bs(gx_frames d, gx_adjoint g, int64_t f0) {
+  using T = typename Real<C>::T;
+  constexpr int P = M / GX_BLK_FFT;
+  constexpr int R = BlkPlan<M>::R;
+  __shared__ C fbuf[M];
+  __shared__ double shd[GX_BLK_FFT / GX_WAVE];
+  const int N = (int)d.N;
+  const int tid = threadIdx.x;
+  const int fb = blockIdx.y;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  if (blockIdx.x == 0 && tid == 0) g.amin[fb] = INT_MAX;  // argmin slot of the gradient gather
+  const int wcol = blockIdx.x;
+  if (wcol > jhi - jlo || wcol >= d.W) return;  // block-uniform
+  const bool dense = d.fill_bkg || d.smooth;
+  double2 *apart = reinterpret_cast<double2 *>(g.avg_part);
+  C *col = reinterpret_cast<C *>(d.inter) + ((int64_t)fb * d.W + wcol) * d.Z;
+  const int j = jlo + wcol;
+  const double hx = (j == N - 1) ? fp[GX_FP_HX_STOP] : (double)j * fp[GX_FP_HX_STEP] + fp[GX_FP_HX_START];
+  const double hy = (j == N - 1) ? fp[GX_FP_HY_STOP] : (double)j * fp[GX_FP_HY_STEP] + fp[GX_FP_HY_START];
+  const double dqinv = 1.0 / d.dq;
+  const long long bx = to_int(floor_div(hx - d.qmin, d.dq, dqinv)) - d.c_lo;
+  const long long by = to_int(floor_div(hy - d.qmin, d.dq, dqinv)) - d.c_lo;
+  if (bx < 0 || bx >= d.c_n || by < 0 || by >= d.c_n) {  // never binned: zero cotangent
+    for (int z = tid; z < d.Z; z += GX_BLK_FFT) col[z] = C{(T)0, (T)0};
+    if (dense && tid == 0) apart[(int64_t)fb * d.W + wcol] = make_double2(0.0, 0.0);
+    return;
+  }
+  const bool k0 = (j == N / 2);
+  C u = C{(T)0, (T)0};
+  if (!d.fill_bkg && d.smooth) u = reinterpret_cast<const C *>(d.col_spec)[(int64_t)fb * d.W + wcol];
+  const double *sz_row = d.col_sz + (int64_t)fb * N;
+  const C *chirp = reinterpret_cast<const C *>(d.bs_chirp);
+  const C *tw = reinterpret_cast<const C *>(d.bs_twiddle_c);
+  C v[P];
+#pragma unroll
+  for (int t = 0; t < P / R; ++t)
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int z = tid + GX_BLK_FFT * t + q * (M / R);
+      C e = C{(T)0, (T)0};
+      if (z < N) e = cmul(col_input(d, (const C *)col, z, k0, u, sz_row), chirp[z]);
+      v[t * R + q] = e;
+    }
+  chirpz_block<M, C>(v, fbuf, reinterpret_cast<const C *>(d.bs_kernel_c), tw, tid);
+  const int64_t cbase = (by * d.c_n + bx) * d.c_n;
+  const double val = d.q_val;
+  const int nn = (N - 1) / 2 + 1;  // fftfreq: k < nn are the non-negative frequencies
+  const double2 *zhat = reinterpret_cast<const double2 *>(g.avg_spec) + (int64_t)fb * (N + d.W);
+  double are = 0.0, aim = 0.0, dc_re = 0.0, dc_im = 0.0;
+#pragma unroll
+  for (int t = 0; t < P / R; ++t)
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int jj = tid + GX_BLK_FFT * t + q * (M / R);
+      C a = C{(T)0, (T)0};
+      if (jj < (int)d.bs_ck) {
+        const int k = (int)d.bs_clo + jj;  // signed frequency
+        const int i = k + N / 2;           // fftshifted row
+        const int kk = k < 0 ? k + N : k;  // natural index
+        const C ck = chirp[k < 0 ? -k : k];
+        double gr = 0.0, gi = 0.0;
+        if (i >= d.i_lo && i <= d.i_hi) {
+          const int kint = (kk < nn) ? kk : kk - N;
+          const double qz = (((double)kint * val) * 2.0) * M_PI;
+          const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
+          if (bz >= 0 && bz < d.c_n) {
+            const double gI = 2.0 * g.g_bins[cbase + bz];
+            const C F = cmul(v[t * R + q], ck);
+            gr = gI * (double)F.x;
+            gi = gI * (double)F.y;
+          }
+        }
+        if (dense) {
+          const double2 zh = zhat[kk];
+          are += gr * zh.x + gi * zh.y;
+          aim += gi * zh.x - gr * zh.y;
+          if (kk == 0) {
+            dc_re = gr;
+            dc_im = gi;
+          }
+        }
+        a = cmul(C{(T)gr, (T)(-gi)}, ck);  // conj(dL/dF) c[|k|]
+      }
+      v[t * R + q] = a;
+    }
+  if (dense) {  // block-uniform
+    are = block_sum(are, shd);
+    aim = block_sum(aim, shd);
+    dc_re = block_sum(dc_re, shd);
+    dc_im = block_sum(dc_im, shd);
+    if (tid == 0) {
+      const double2 yh = zhat[N + wcol];
+      double pre = are * yh.x + aim * yh.y, pim = aim * yh.x - are * yh.y;
+      if (k0) {
+        const double c0 = d.max_voxels * (double)N * (double)N;
+        pre = pre + c0 * dc_re;
+        pim = pim + c0 * dc_im;
+      }
+      apart[(int64_t)fb * d.W + wcol] = make_double2(pre, pim);
+    }
+  }
+  __syncthreads();  // fbuf is read by the last stage of the forward band transform
+  chirpz_block<M, C>(v, fbuf, reinterpret_cast<const C *>(g.bs_kernel_adj_c), tw, tid);
+  // dL/dx[z] = conj(c[z] y[z])
+#pragma unroll
+  for (int t = 0; t < P / R; ++t)
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int z = tid + GX_BLK_FFT * t + q * (M / R);
+      if (z < d.Z) col[z] = cconj(cmul(v[t * R + q], chirp[z]));
+    }
+}
+
+// FFT(Pz) at the column band and FFT(Py) on the kept columns by direct FP64 DFTs (as
+// k_adj_avg_spectra for power-of-two edges), grid (nf, 2)
+__global__ __launch_bounds__(BLK) void k_adj_avg_spectra_dft(gx_frames d, gx_adjoint g, int64_t f0) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double *p = reinterpret_cast<double *>(smem_raw);  // [N]
+  const int N = (int)d.N;
+  const int fb = blockIdx.x;
+  const bool along_y = blockIdx.y == 1;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = max(0, min(jhi - jlo + 1, (int)d.W));
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  const int ya = ylo == INT_MAX ? N : ylo + 1, yb = yhi == INT_MIN ? 0 : yhi;
+  for (int i = threadIdx.x; i < N; i += BLK) {
+    double v;
+    if (!along_y) {
+      v = (!d.fill_bkg || (i >= zlo && i < zhi)) ? 1.0 : 0.0;
+      if (d.smooth) v = v * d.col_sz[(int64_t)fb * N + i];
+    } else {
+      if (d.fill_bkg)
+        v = (i >= ya && i < yb) ? (double)d.col_nm[(int64_t)fb * N + i] - d.max_voxels : 0.0;
+      else
+        v = -d.max_voxels;
+      if (d.smooth) v = v * d.col_sy[(int64_t)fb * N + i];
+    }
+    p[i] = v;
+  }
+  __syncthreads();
+  const double2 *tw = reinterpret_cast<const double2 *>(d.twiddle64);
+  double2 *out = reinterpret_cast<double2 *>(g.avg_spec) + (int64_t)fb * (N + d.W);
+  const int n_out = along_y ? wf : (int)d.bs_ck;
+  for (int o = threadIdx.x; o < n_out; o += BLK) {
+    int k;  // natural frequency index
+    if (along_y) {
+      k = (jlo + o + N - N / 2) % N;
+    } else {
+      const int ks = (int)d.bs_clo + o;
+      k = ks < 0 ? ks + N : ks;
+    }
+    double re = 0.0, im = 0.0;
+    int m = 0;
+    for (int i = 0; i < N; ++i) {
+      const double2 t = tw[m];
+      re = fma(p[i], t.x, re);
+      im = fma(p[i], t.y, im);
+      m += k;
+      if (m >= N) m -= N;
+    }
+    if (along_y)
+      out[N + o] = make_double2(re, im);
+    else
+      out[k] = make_double2(re, im);
+  }
+}
+
+// adjoint of the chirp-z row pass: per block BS_GRAD_H rows (the last the halo row), the
+// kept band of each row, conjugated, through the adjoint band transform to the N grid
+// points, masked; then the gather of k_adj_rowgrad
+template <int M, typename C>
+__global__ __launch_bounds__(GX_BLK_FFT) void k_adj_rowgrad_bs(gx_frames d, gx_adjoint g, int64_t f0) {
+  using T = typename Real<C>::T;
+  constexpr int P = M / GX_BLK_FFT;
+  constexpr int R = BlkPlan<M>::R;
+  constexpr int H = BS_GRAD_H, RB = H - 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int N = (int)d.N;
+  C *rows = reinterpret_cast<C *>(smem_raw);                // [H][N] cotangent of the grid rows
+  C *fbuf = rows + H * N;                                   // [M]
+  C *stage = fbuf + M;                                      // [W][H]
+  double *ftab = reinterpret_cast<double *>(stage + d.W * H);  // [n_elem][2]
+  double *msk = ftab + 2 * g.n_elem;                        // [N]
+  __shared__ double shd[GX_BLK_FFT / GX_WAVE];
+  __shared__ int shi[GX_BLK_FFT / GX_WAVE];
+  const int tid = threadIdx.x;
+  const int fb = blockIdx.y;
+  const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
+  const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
+  const int wf = max(0, min(jhi - jlo + 1, (int)d.W));
+  const int z0 = blockIdx.x * RB;
+  for (int i = tid; i < 2 * g.n_elem; i += GX_BLK_FFT) ftab[i] = g.f_table[i];
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
+  const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
+  const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
+  const bool masked = d.fill_bkg || d.smooth;
+  for (int y = tid; y < N; y += GX_BLK_FFT)
+    msk[y] = (y >= ya && y < yb) ? (d.smooth ? d.col_sy[(int64_t)fb * N + y] : 1.0) : 0.0;
+  const C *src = reinterpret_cast<const C *>(d.inter) + (int64_t)fb * d.W * d.Z;
+  for (int idx = tid; idx < wf * H; idx += GX_BLK_FFT) {
+    const int w = idx / H, q = idx - w * H;
+    const int v = z0 + q;
+    C e = C{(T)0, (T)0};
+    if (v < d.Z) e = src[(int64_t)w * d.Z + v];
+    stage[idx] = e;
+  }
+  __syncthreads();
+  const C *chirp = reinterpret_cast<const C *>(d.bs_chirp);
+  const C *kadj = reinterpret_cast<const C *>(g.bs_kernel_adj);
+  const C *tw = reinterpret_cast<const C *>(d.bs_twiddle);
+  for (int h = 0; h < H; ++h) {
+    const int v = z0 + h;
+    double mzv = 1.0;
+    if (d.fill_bkg && !(v >= zlo && v < zhi)) mzv = 0.0;
+    if (d.smooth && v < N) mzv = mzv * d.col_sz[(int64_t)fb * N + v];
+    C x[P];
+#pragma unroll
+    for (int t = 0; t < P / R; ++t)
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int jj = tid + GX_BLK_FFT * t + q * (M / R);
+        C a = C{(T)0, (T)0};
+        if (jj < (int)d.bs_rk) {
+          const int k = (int)d.bs_rlo + jj;
+          const int w = k + N / 2 - jlo;
+          if (w >= 0 && w < wf) a = cmul(cconj(stage[w * H + h]), chirp[k < 0 ? -k : k]);
+        }
+        x[t * R + q] = a;
+      }
+    __syncthreads();  // the previous row's transform has read fbuf
+    chirpz_block<M, C>(x, fbuf, kadj, tw, tid);
+#pragma unroll
+    for (int t = 0; t < P / R; ++t)
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int o = tid + GX_BLK_FFT * t + q * (M / R);
+        if (o >= N) continue;
+        const C y = cconj(cmul(x[t * R + q], chirp[o]));
+        const double m = masked ? mzv * msk[o] : 1.0;
+        rows[h * N + o] = masked ? C{(T)((double)y.x * m), (T)((double)y.y * m)} : y;
+      }
+  }
+  __syncthreads();
+  rowgrad_gather<C>(d, g, fp, fb, z0, RB, N, rows, ftab, shd, shi);
 }
 
 // ------------------------------------------------------------------------------------
@@ -994,6 +1406,7 @@ int launch_adj_rowgrad_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, in
 
 // row blocks per frame of k_adj_rowgrad (its partial sums per frame)
 int rowgrad_blocks(const gx_frames *d) {
+  if (d->bs_M) return (int)((d->Z + BS_GRAD_H - 2) / (BS_GRAD_H - 1));
   int rb = 1;
   switch (d->N) {
     case 256: rb = d->fft_f64 ? RowGradShape<256, double2>::H - 1 : RowGradShape<256, float2>::H - 1; break;
@@ -1022,7 +1435,66 @@ int dispatch_n(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64
   return launch_adj_rowgrad_t<N, float2>(d, g, f0, nf, st);
 }
 
+template <int M, typename C>
+int launch_adj_bs_t(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
+  if (op == OP_ROWPASS) {
+    const size_t lds = (size_t)M * sizeof(C) + (size_t)d->W * (BS_RB + 1) * sizeof(C) + (size_t)g->n_elem * 16;
+    if (lds > 150 * 1024)
+      return set_error(GX_ERR_UNSUPPORTED, "chirp-z differentiable row pass needs %zu bytes of LDS", lds);
+    static bool attr_set = false;
+    if (!attr_set) {
+      GX_HIP(hipFuncSetAttribute((const void *)k_adj_rowpass_bs<M, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 150 * 1024));
+      attr_set = true;
+    }
+    dim3 grid((unsigned)((d->Z + BS_RB - 1) / BS_RB), (unsigned)nf);
+    k_adj_rowpass_bs<M, C><<<grid, GX_BLK_FFT, lds, st>>>(*d, *g, f0);
+    return check_launch("k_adj_rowpass_bs");
+  }
+  if (op == OP_COLPASS) {
+    dim3 grid((unsigned)d->W, (unsigned)nf);
+    k_adj_colpass_bs<M, C><<<grid, GX_BLK_FFT, 0, st>>>(*d, *g, f0);
+    return check_launch("k_adj_colpass_bs");
+  }
+  // OP_ROWGRAD (150 KB of dynamic LDS at most: the gather's reductions use static LDS)
+  const size_t lds = ((size_t)BS_GRAD_H * d->N + M + (size_t)d->W * BS_GRAD_H) * sizeof(C) +
+                     (size_t)g->n_elem * 16 + (size_t)d->N * 8;
+  if (lds > 150 * 1024)
+    return set_error(GX_ERR_UNSUPPORTED, "chirp-z adjoint row pass needs %zu bytes of LDS (N=%lld, W=%lld)", lds,
+                     (long long)d->N, (long long)d->W);
+  static bool attr_set = false;
+  if (!attr_set) {
+    GX_HIP(hipFuncSetAttribute((const void *)k_adj_rowgrad_bs<M, C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               150 * 1024));
+    attr_set = true;
+  }
+  dim3 grid((unsigned)rowgrad_blocks(d), (unsigned)nf);
+  k_adj_rowgrad_bs<M, C><<<grid, GX_BLK_FFT, lds, st>>>(*d, *g, f0);
+  return check_launch("k_adj_rowgrad_bs");
+}
+
+template <typename C>
+int dispatch_bs_c(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
+  const int64_t M = (op == OP_COLPASS) ? d->bs_Mc : d->bs_M;
+  switch (M) {
+    case 1024: return launch_adj_bs_t<1024, C>(op, d, g, f0, nf, st);
+    case 2048: return launch_adj_bs_t<2048, C>(op, d, g, f0, nf, st);
+    case 4096: return launch_adj_bs_t<4096, C>(op, d, g, f0, nf, st);
+    default: return set_error(GX_ERR_UNSUPPORTED, "chirp-z transform length %lld", (long long)M);
+  }
+}
+
+int dispatch_bs(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
+  if (op == OP_AVGSPEC) {
+    k_adj_avg_spectra_dft<<<dim3((unsigned)nf, 2), BLK, (size_t)d->N * sizeof(double), st>>>(*d, *g, f0);
+    return check_launch("k_adj_avg_spectra_dft");
+  }
+  if (d->fft_f64) return dispatch_bs_c<double2>(op, d, g, f0, nf, st);
+  return dispatch_bs_c<float2>(op, d, g, f0, nf, st);
+}
+
 int dispatch(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
+  if (d->bs_M) return dispatch_bs(op, d, g, f0, nf, st);
   switch (d->N) {
     case 256: return dispatch_n<256>(op, d, g, f0, nf, st);
     case 512: return dispatch_n<512>(op, d, g, f0, nf, st);
@@ -1035,13 +1507,27 @@ int dispatch(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t
 
 int check_adj(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, bool backward) {
   GX_REQUIRE(d != nullptr && g != nullptr, "NULL descriptor");
-  GX_REQUIRE(d->N == 256 || d->N == 512 || d->N == 1024 || d->N == 2048,
-             "differentiable path: grid edge N=%lld is not 256/512/1024/2048", (long long)d->N);
+  if (d->bs_M) {
+    GX_REQUIRE(d->N >= 16 && d->N <= 2048, "differentiable path: grid edge N=%lld outside [16, 2048]",
+               (long long)d->N);
+    GX_REQUIRE(d->bs_chirp && d->bs_kernel && d->bs_twiddle && d->bs_kernel_c && d->bs_twiddle_c && d->twiddle64,
+               "chirp-z grid edge N=%lld needs the band tables of gx_frames", (long long)d->N);
+    GX_REQUIRE((d->bs_M == 1024 || d->bs_M == 2048 || d->bs_M == 4096) &&
+                   (d->bs_Mc == 1024 || d->bs_Mc == 2048 || d->bs_Mc == 4096) && d->bs_M >= d->N + d->bs_rk - 1 &&
+                   d->bs_Mc >= d->N + d->bs_ck - 1,
+               "chirp-z transform lengths (%lld, %lld) do not fit N=%lld", (long long)d->bs_M, (long long)d->bs_Mc,
+               (long long)d->N);
+  } else {
+    GX_REQUIRE(d->N == 256 || d->N == 512 || d->N == 1024 || d->N == 2048,
+               "differentiable path: grid edge N=%lld is not 256/512/1024/2048 (other edges need chirp-z tables)",
+               (long long)d->N);
+  }
   const bool dense = d->fill_bkg || d->smooth;
   GX_REQUIRE(!d->fill_bkg || d->col_nm, "fill_bkg needs col_nm");
   GX_REQUIRE(!d->smooth || (d->col_sy && d->col_sz && d->gauss_cw), "smooth needs col_sy, col_sz, gauss_cw");
   GX_REQUIRE(!dense || (d->col_spec && d->twiddle64), "fill_bkg / smooth need col_spec and twiddle64");
-  GX_REQUIRE(d->bs_M == 0, "differentiable path: chirp-z tables given for a power-of-two grid");
+  GX_REQUIRE(d->bs_M == 0 || (d->N != 256 && d->N != 512 && d->N != 1024 && d->N != 2048),
+             "differentiable path: chirp-z tables given for a power-of-two grid");
   GX_REQUIRE(d->r > 0.0, "r_voxel_size must be positive");
   GX_REQUIRE(nf >= 0 && nf <= d->batch, "batch of %lld frames exceeds workspace batch %lld", (long long)nf,
              (long long)d->batch);
@@ -1062,6 +1548,8 @@ int check_adj(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, b
     GX_REQUIRE(g->g_bins && g->g_part && g->red && g->amin && g->g_atom, "a backward buffer of gx_adjoint is NULL");
     GX_REQUIRE(g->n_elem > GX_ADJ_ELEM_REG || (g->e_part && g->e_sum),
                "n_elem <= %d needs gx_adjoint.e_part and e_sum", GX_ADJ_ELEM_REG);
+    GX_REQUIRE(!d->bs_M || (g->bs_kernel_adj && g->bs_kernel_adj_c),
+               "chirp-z grid edge needs gx_adjoint.bs_kernel_adj and bs_kernel_adj_c");
     GX_REQUIRE(!dense || (g->x && g->y && g->ext && g->ecount && g->avg_spec && g->avg_part && g->g_avg),
                "fill_bkg / smooth need gx_adjoint.x, y, ext, ecount, avg_spec, avg_part, g_avg");
   }

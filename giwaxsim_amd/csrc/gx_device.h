@@ -492,6 +492,29 @@ __device__ __forceinline__ void fft_block(C (&v)[M / GX_BLK_FFT], C* lds, const 
   }
 }
 
+// Zoomed chirp-z band transform on a whole block (see the chirp-z passes): the circular
+// convolution IFFT_M(FFT_M(v) K) of the block's length-M input v with the kernel spectrum
+// K (already divided by M), as conj(FFT(conj(FFT(v) K))); the output is in v's layout.
+template <int M, typename C>
+__device__ __forceinline__ void chirpz_block(C (&v)[M / GX_BLK_FFT], C *lds, const C *__restrict__ K,
+                                             const C *__restrict__ tw, int tid) {
+  constexpr int P = M / GX_BLK_FFT;
+  constexpr int R = BlkPlan<M>::R;
+  fft_block<M, C>(v, lds, tw, tid);
+#pragma unroll
+  for (int t = 0; t < P / R; ++t)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = tid + GX_BLK_FFT * t + r * (M / R);
+      const C x = cmul(v[t * R + r], K[k]);
+      v[t * R + r] = C{x.x, -x.y};
+    }
+  __syncthreads();
+  fft_block<M, C>(v, lds, tw, tid);
+#pragma unroll
+  for (int q = 0; q < P; ++q) v[q] = C{v[q].x, -v[q].y};
+}
+
 // ------------------------------------------------------------------------------------
 // XCD-aware tile order.  The workgroups of a launch are dispatched round-robin over
 // the 8 XCDs (block b runs on XCD b % 8), each with its own L2.  xcd_tile maps the

@@ -1314,26 +1314,6 @@ __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
 // h[t] = conj(c[k_lo + t]) wrapped (host, FP64, already divided by M); the inverse is
 // conj(FFT(conj(.))).  Output j of the inverse transform is X[k_lo + j].
 // ------------------------------------------------------------------------------------
-template <int M, typename C>
-__device__ __forceinline__ void chirpz_block(C (&v)[M / GX_BLK_FFT], C *lds, const C *__restrict__ K,
-                                             const C *__restrict__ tw, int tid) {
-  constexpr int P = M / GX_BLK_FFT;
-  constexpr int R = BlkPlan<M>::R;
-  fft_block<M, C>(v, lds, tw, tid);
-#pragma unroll
-  for (int t = 0; t < P / R; ++t)
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int k = tid + GX_BLK_FFT * t + r * (M / R);
-      const C x = cmul(v[t * R + r], K[k]);
-      v[t * R + r] = C{x.x, -x.y};
-    }
-  __syncthreads();
-  fft_block<M, C>(v, lds, tw, tid);
-#pragma unroll
-  for (int q = 0; q < P; ++q) v[q] = C{v[q].x, -v[q].y};
-}
-
 // per-frame background spectrum on the kept columns by a direct FP64 DFT
 __global__ __launch_bounds__(BLK) void k_frames_spectra_dft(gx_frames d, int64_t f0) {
   GX_RESOLVE_AVG(d);

@@ -456,6 +456,27 @@ def chirpz_band_tables(N: int, k_lo: int, K: int, f64: bool = False):
     return M, out, twiddles(M, f64)
 
 
+def chirpz_adjoint_tables(N: int, k_lo: int, K: int, M: int, f64: bool = False):
+    """Kernel spectrum of the adjoint of the zoomed chirp-z band (the band DFT run from
+    the K band values k = k_lo + j to the N grid points o): with o k = (o^2 + k^2 -
+    (k - o)^2) / 2 the sum over j is a convolution in the lag t = o - j of
+    a_j = u_{k_j} c[|k_j|] with h[t] = conj(c[k_lo - t]) = exp(+i pi ((k_lo - t)^2 mod 2N) / N),
+    t in [-(K-1), N), placed at t mod M; H = FFT_M(h) / M (exact integer squares, FP64).
+    M is the forward band's length (M >= N + K - 1)."""
+    if M < N + K - 1:
+        raise ValueError(f"chirp-z adjoint needs M >= N + K - 1 (M={M}, N={N}, K={K})")
+    t = np.arange(-(K - 1), N, dtype=np.int64)
+    m = np.abs(k_lo - t)
+    ph = np.pi * ((m * m) % (2 * N)).astype(np.float64) / N
+    b = np.zeros(M, dtype=np.complex128)
+    b[t % M] = np.cos(ph) + 1j * np.sin(ph)
+    H = np.fft.fft(b) / M
+    out = np.empty(2 * M, dtype=np.float64 if f64 else np.float32)
+    out[0::2] = H.real
+    out[1::2] = H.imag
+    return out
+
+
 def chirpz_bands(plan):
     """(k_lo, K) of the row pass (every frame's kept columns) and of the column pass
     (kept rows [i_lo, i_hi]) as signed frequencies k = column (row) - N // 2."""

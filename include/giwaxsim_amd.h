@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GX_ABI_VERSION 8
+#define GX_ABI_VERSION 9
 
 #define GX_OK 0
 #define GX_ERR_ARG 1         /* invalid argument (message says which) */
@@ -454,6 +454,12 @@ typedef struct gx_adjoint {
      inside each gradient block instead of per atom */
   double *e_part;          /* [batch][Z][n_elem][2] per frame and row block (at most Z blocks) */
   double *e_sum;           /* [n_elem][2] accumulated over batches (zeroed by gx_adj_prepare) */
+  /* grid edges that are not powers of two (gx_frames.bs_M != 0): kernel spectra of the
+     adjoint band transforms (band values -> grid points), same lengths and layouts as
+     gx_frames.bs_kernel / bs_kernel_c: FFT_M(h) / M, h[t mod M] = exp(+i pi ((k_lo - t)^2 mod 2N) / N)
+     for t in [-(K-1), N), k_lo / K the row band (bs_rlo, bs_rk) or column band (bs_clo, bs_ck) */
+  const void *bs_kernel_adj;
+  const void *bs_kernel_adj_c;
 } gx_adjoint;
 
 /* Deposit exponents for the current f_table, inverse permutation, zero g_atom. */

@@ -141,3 +141,34 @@ def test_chirpz_band_tables(N, k_lo, K):
     k = k_lo + np.arange(K)
     X = c[np.abs(k)] * conv[:K]
     assert np.allclose(X, np.fft.fft(x)[k % N], rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize("N,k_lo,K", [(131, -30, 61), (300, -150, 300), (1048, -95, 190),
+                                      (1048, -141, 283), (700, 5, 40)])
+def test_chirpz_adjoint_tables(N, k_lo, K):
+    """The adjoint band transform of the differentiable path, in the kernels' arithmetic
+    (band values times c[|k|] at positions j < K, circular convolution of the forward
+    band's length M with the adjoint kernel, output o times c[o]), equals the
+    conjugate transpose of the band DFT: y_o = sum_j u_j exp(-2 pi i o k_j / N), and
+    <S F x, u> = <x, conj(adjoint(conj(u)))> for random x, u."""
+    M, _, _ = engine.chirpz_band_tables(N, k_lo, K, True)
+    Ha = engine.chirpz_adjoint_tables(N, k_lo, K, M, True)
+    _, ch, _, _ = engine.chirpz_tables(N, True)
+    c = ch[0::2] + 1j * ch[1::2]
+    H = Ha[0::2] + 1j * Ha[1::2]
+    rng = np.random.default_rng(N * 7 + K)
+    u = rng.normal(size=K) + 1j * rng.normal(size=K)
+    k = k_lo + np.arange(K)
+    a = np.zeros(M, complex)
+    a[:K] = u * c[np.abs(k)]
+    conv = np.conj(np.fft.fft(np.conj(np.fft.fft(a) * H)))
+    y = c[np.arange(N)] * conv[:N]
+    want = np.exp(-2j * np.pi * np.outer(np.arange(N), k % N) / N) @ u
+    assert np.allclose(y, want, rtol=1e-11, atol=1e-9 * np.abs(want).max())
+    # adjoint identity with the forward band transform
+    x = rng.normal(size=N) + 1j * rng.normal(size=N)
+    X = np.fft.fft(x)[k % N]
+    a2 = np.zeros(M, complex)
+    a2[:K] = np.conj(u) * c[np.abs(k)]
+    xbar = np.conj(c[np.arange(N)] * np.conj(np.fft.fft(np.conj(np.fft.fft(a2) * H)))[:N])
+    assert np.isclose(np.vdot(u, X), np.vdot(xbar, x), rtol=1e-11)

@@ -44,14 +44,13 @@ MAXQ = 2.5
 MANY = ('C', 'N', 'O', 'S', 'F', 'H', 'Si')
 
 
-def _small(precision, fill=False, smooth=0, many=False):
+def _small(precision, fill=False, smooth=0, many=False, N=256):
     coords, el = graphite_slab(size=(16.0, 16.0, 12.0), seed=21)
     # a second element so per-element gradients are exercised
     el = el.copy()
     el[::7] = 'N'
     if many:
         el = np.array([MANY[i % len(MANY)] for i in range(len(el))])
-    N = 256
     qv = 2 * np.pi / (R * N) * (1 + 1e-9)
     det = dict(num_pixels=48, max_q=MAXQ, psis=list(np.arange(0.0, 91.0, 10.0)), phis=[0.0],
                thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
@@ -68,16 +67,20 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("precision,fill,smooth,tol_img,tol_grad,many",
-                         [('fp64', False, 0, 1e-12, 1e-11, False), ('fp32', False, 0, 2e-6, 1e-4, False),
-                          ('fp64', True, 0, 1e-10, 1e-11, False), ('fp64', False, 3, 1e-10, 1e-11, False),
-                          ('fp64', True, 3, 1e-10, 1e-11, False), ('fp32', True, 3, 2e-6, 1e-4, False),
-                          ('fp64', True, 3, 1e-10, 1e-11, True), ('fp32', False, 0, 2e-6, 1e-4, True)])
-def test_forward_and_gradients_match_autograd(precision, fill, smooth, tol_img, tol_grad, many):
-    _say(f"[gpu-adjoint] small graphite N=256 frames 0..10 precision={precision} fill={fill} "
+@pytest.mark.parametrize("precision,fill,smooth,tol_img,tol_grad,many,N",
+                         [('fp64', False, 0, 1e-12, 1e-11, False, 256), ('fp32', False, 0, 2e-6, 1e-4, False, 256),
+                          ('fp64', True, 0, 1e-10, 1e-11, False, 256), ('fp64', False, 3, 1e-10, 1e-11, False, 256),
+                          ('fp64', True, 3, 1e-10, 1e-11, False, 256), ('fp32', True, 3, 2e-6, 1e-4, False, 256),
+                          ('fp64', True, 3, 1e-10, 1e-11, True, 256), ('fp32', False, 0, 2e-6, 1e-4, True, 256),
+                          # edges that are not powers of two: chirp-z band transforms and their adjoints
+                          ('fp64', False, 0, 1e-10, 1e-10, False, 300), ('fp64', True, 3, 1e-10, 1e-10, False, 300),
+                          ('fp32', True, 3, 2e-6, 1e-4, False, 300), ('fp64', True, 3, 1e-10, 1e-10, True, 200)])
+def test_forward_and_gradients_match_autograd(precision, fill, smooth, tol_img, tol_grad, many, N):
+    _say(f"[gpu-adjoint] small graphite N={N} frames 0..10 precision={precision} fill={fill} "
          f"smooth={smooth} elements={len(MANY) if many else 2}")
-    coords, model, ref = _small(precision, fill, smooth, many)
+    coords, model, ref = _small(precision, fill, smooth, many, N)
     assert (model.n_elem > _lib.GX_ADJ_ELEM_REG) == many
+    assert model.plan.N == N and (model.bs is None) == (N == 256)
     dev = torch.device('cuda')
     G = np.random.default_rng(1).normal(size=(48, 48))
     # HIP
@@ -248,13 +251,15 @@ _LARGE_REF = {}
 
 def _large_case(N):
     """A 40 x 40 x 30 A polymer slab (C/H/N/O/S/F) on the production grids: N = 1024
-    (r 0.2 A, max_q 2.0, smooth 10) and N = 2048 (r 0.15 A, max_q 3.0, smooth 20), both
+    (r 0.2 A, max_q 2.0, smooth 10), N = 2048 (r 0.15 A, max_q 3.0, smooth 20) and the
+    reference's fit geometry N = 1048 (r 0.3 A, q voxel 0.02, smooth 5: chirp-z bands), all
     with fill_bkg; three phi-frames and a 64^2 detector over psi 0..90 in 15 degree steps.
     The FP64 autograd reference's image and gradients are computed once per N."""
     from giwaxsim_amd.synth import polymer_slab
-    r, mq, smooth = {1024: (0.2, 2.0, 10), 2048: (0.15, 3.0, 20)}[N]
+    r, mq, smooth = {1024: (0.2, 2.0, 10), 2048: (0.15, 3.0, 20), 1048: (0.3, 2.0, 5)}[N]
     coords, el = polymer_slab(size=(40.0, 40.0, 30.0), seed=31)
-    qv = 2 * np.pi / (r * N) * (1 + 1e-9)
+    # N = 1048: the reference's example / fit geometry (r 0.3 A, q voxel 0.02), chirp-z bands
+    qv = 0.02 if N == 1048 else 2 * np.pi / (r * N) * (1 + 1e-9)
     det = dict(num_pixels=64, max_q=mq, psis=list(np.arange(0.0, 91.0, 15.0)), phis=[0.0],
                thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
     frames = (100, 103)
@@ -278,7 +283,9 @@ def _large_case(N):
 @pytest.mark.parametrize("N,precision,tol_img,tol_grad", [(1024, 'fp64', 1e-10, 1e-10),
                                                           (1024, 'fp32', 2e-6, 1e-4),
                                                           (2048, 'fp64', 1e-10, 1e-10),
-                                                          (2048, 'fp32', 2e-6, 1e-4)])
+                                                          (2048, 'fp32', 2e-6, 1e-4),
+                                                          (1048, 'fp64', 1e-10, 1e-10),
+                                                          (1048, 'fp32', 2e-6, 1e-4)])
 def test_production_grids_match_autograd(N, precision, tol_img, tol_grad):
     coords, el, r, qv, mq, det, frames, smooth, G, (w, gw, gfr, gfi) = _large_case(N)
     _say(f"[gpu-adjoint] N={N} {precision} fill smooth {smooth}, frames {frames}")
