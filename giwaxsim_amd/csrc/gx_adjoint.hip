@@ -531,12 +531,20 @@ __global__ __launch_bounds__(BLK) void k_adj_avg_reduce(gx_frames d, gx_adjoint 
 // dL/df re, im) for this frame to g_part; the block's share of sum_a dL/dy'_a (fixed
 // partition of the regrouped order) goes to red, the argmin of y' to amin and, with
 // at most GX_ADJ_ELEM_REG elements, the block's per-element dL/df to e_part.
+// With `mzr` given, the rows hold the unmasked cotangent and the gather applies the node
+// mask M = (mz inz)[row] (my iny)[y] itself: mzr[h] for the block's rows, my from col_sy.
+__device__ __forceinline__ double node_my(const gx_frames &d, const double *sy_row, int ya, int yb, long long y) {
+  return (y >= ya && y < yb) ? (d.smooth ? sy_row[y] : 1.0) : 0.0;
+}
+
 template <typename C>
 __device__ __forceinline__ void rowgrad_gather(const gx_frames &d, const gx_adjoint &g, const double *fp, int fb,
                                                int z0, int RB, int N, const C *rows, const double *ftab,
-                                               double *shd, int *shi) {
+                                               double *shd, int *shi, const double *mzr = nullptr,
+                                               int ya = 0, int yb = 0) {
   using T = typename Real<C>::T;
   const int64_t n = d.n_atoms;
+  const double *sy_row = d.col_sy + (int64_t)fb * N;
   const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
   const double ymin = dkey_inv(d.ymin_key[3 * fb]);
   const double r = d.r, rinv = 1.0 / d.r;
@@ -580,10 +588,17 @@ __device__ __forceinline__ void rowgrad_gather(const gx_frames &d, const gx_adjo
       if (u0 >= 0 && u0 + 1 < N && vr >= 0 && vr < RB) {
         const C *r0 = rows + vr * N + u0;
         const C *r1 = r0 + N;
-        const double g00r = (double)r0[0].x, g00i = (double)r0[0].y;
-        const double g01r = (double)r0[1].x, g01i = (double)r0[1].y;
-        const double g10r = (double)r1[0].x, g10i = (double)r1[0].y;
-        const double g11r = (double)r1[1].x, g11i = (double)r1[1].y;
+        double g00r = (double)r0[0].x, g00i = (double)r0[0].y;
+        double g01r = (double)r0[1].x, g01i = (double)r0[1].y;
+        double g10r = (double)r1[0].x, g10i = (double)r1[0].y;
+        double g11r = (double)r1[1].x, g11i = (double)r1[1].y;
+        if (mzr) {
+          const double my0 = node_my(d, sy_row, ya, yb, u0), my1 = node_my(d, sy_row, ya, yb, u0 + 1);
+          const double m00 = mzr[vr] * my0, m01 = mzr[vr] * my1;
+          const double m10 = mzr[vr + 1] * my0, m11 = mzr[vr + 1] * my1;
+          g00r *= m00; g00i *= m00; g01r *= m01; g01i *= m01;
+          g10r *= m10; g10i *= m10; g11r *= m11; g11i *= m11;
+        }
         // h = Re(conj(dL/dG) f) at the four nodes
         const double h00 = g00r * fr + g00i * fi, h01 = g01r * fr + g01i * fi;
         const double h10 = g10r * fr + g10i * fi, h11 = g11r * fr + g11i * fi;
@@ -653,7 +668,7 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   C *rows = reinterpret_cast<C *>(smem_raw);  // [H][N] cotangent of the grid rows
   C *stage = rows;                            // [W][H], consumed before rows is written
   double *ftab = reinterpret_cast<double *>(rows + H * N);
-  double *msk = ftab + 2 * g.n_elem;          // [N] y part of the node mask (my iny)
+  __shared__ double mzr[H];                   // z part of the node mask of the block's rows
   __shared__ double shd[NWAVE];
   __shared__ int shi[NWAVE];
   const int lane = threadIdx.x & (GX_WAVE - 1);
@@ -671,9 +686,13 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
   const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
   const bool masked = d.fill_bkg || d.smooth;
-  if (masked)
-    for (int y = threadIdx.x; y < N; y += BLK)
-      msk[y] = (y >= ya && y < yb) ? (d.smooth ? d.col_sy[(int64_t)fb * N + y] : 1.0) : 0.0;
+  if (threadIdx.x < H) {
+    const int v = z0 + threadIdx.x;
+    double mzv = 1.0;
+    if (d.fill_bkg && !(v >= zlo && v < zhi)) mzv = 0.0;
+    if (d.smooth && v < N) mzv = mzv * d.col_sz[(int64_t)fb * N + v];
+    mzr[threadIdx.x] = mzv;
+  }
   const C *src = reinterpret_cast<const C *>(d.inter) + (int64_t)fb * d.W * d.Z;
   for (int idx = threadIdx.x; idx < wf * H; idx += BLK) {
     const int w = idx / H, q = idx - w * H;
@@ -698,39 +717,24 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   }
   __syncthreads();
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
-  // node mask M = (mz inz) x (my iny) of the deposit (1 without fill and smoothing),
-  // applied to the transformed row as it leaves the registers
+  // the transformed rows leave the registers conjugated and unmasked: the gather applies
+  // the node mask M = (mz inz) x (my iny) at the four nodes it reads (no N-double mask
+  // table in LDS, one more block per CU at N = 2048)
 #pragma unroll
   for (int h = 0; h < HW; ++h) {
     const int q = wave + NWAVE * h;
     if (q >= H) break;  // wave-uniform
-    const int v = z0 + q;
-    double mzv = 1.0;
-    if (d.fill_bkg && !(v >= zlo && v < zhi)) mzv = 0.0;
-    if (d.smooth && v < N) mzv = mzv * d.col_sz[(int64_t)fb * N + v];
     C *buf = rows + q * N;
     fft_wave<N, C>(x[h], buf, tw, lane);
     wave_sync();
-    if (masked) {
 #pragma unroll
-      for (int t = 0; t < P / RL; ++t)
+    for (int t = 0; t < P / RL; ++t)
 #pragma unroll
-        for (int qq = 0; qq < RL; ++qq) {
-          const int y = lane + GX_WAVE * t + qq * (N / RL);
-          const double m = mzv * msk[y];
-          const C o = cconj(x[h][t * RL + qq]);
-          buf[y] = C{(T)((double)o.x * m), (T)((double)o.y * m)};
-        }
-    } else {
-#pragma unroll
-      for (int t = 0; t < P / RL; ++t)
-#pragma unroll
-        for (int qq = 0; qq < RL; ++qq) buf[lane + GX_WAVE * t + qq * (N / RL)] = cconj(x[h][t * RL + qq]);
-    }
+      for (int qq = 0; qq < RL; ++qq) buf[lane + GX_WAVE * t + qq * (N / RL)] = cconj(x[h][t * RL + qq]);
   }
   __syncthreads();
 
-  rowgrad_gather<C>(d, g, fp, fb, z0, RB, N, rows, ftab, shd, shi);
+  rowgrad_gather<C>(d, g, fp, fb, z0, RB, N, rows, ftab, shd, shi, masked ? mzr : nullptr, ya, yb);
 }
 
 // e_sum += the batch's per-frame, per-row-block dL/df sums, in two steps with fixed
@@ -1388,8 +1392,8 @@ int launch_adj_colpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, in
 template <int N, typename C>
 int launch_adj_rowgrad_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int H = RowGradShape<N, C>::H, RB = H - 1;
-  // rows (the staging tile aliases them, W <= N) + f table + y mask
-  const size_t lds = (size_t)H * N * sizeof(C) + (size_t)g->n_elem * 16 + (size_t)N * 8;
+  // rows (the staging tile aliases them, W <= N) + f table
+  const size_t lds = (size_t)H * N * sizeof(C) + (size_t)g->n_elem * 16;
   if (lds > 150 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "adjoint row pass needs %zu bytes of LDS (N=%d, W=%lld)", lds, N,
                      (long long)d->W);
