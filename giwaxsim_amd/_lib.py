@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_void_p
 from .build import LIB
 
 GX_OK = 0
-GX_ABI_VERSION = 9   # structure layouts below match this version of the header
+GX_ABI_VERSION = 10  # structure layouts below match this version of the header
 GX_FP_STRIDE = 24
 GX_MOMENTS_WORK = 774
 GX_CROP_NOALLOC_MAX_ATOMS = 134217728
@@ -141,8 +141,9 @@ SIGNATURES = {
     "gx_adj_finish": [POINTER(GxFrames), POINTER(GxAdjoint), _P, _P, _P],
     "gx_bins_adjoint": [_P, _I, _P, _P, _I, _I, _I, _D, _D, _D, POINTER(c_double), _D, _P, _P],
     "gx_detector_adjoint": [POINTER(GxDetector), _P, _I, _P, _P, _I, _P, _P],
-    "gx_detector_map_build": [POINTER(GxDetector), _P, _P, _P, _P, _P, _P],
-    "gx_detector_adjoint_mapped": [POINTER(GxDetector), _P, _I, _P, _P, _P, _P, _P, _P],
+    "gx_detector_map_count": [POINTER(GxDetector), _P, _P, _P, _P],
+    "gx_detector_map_fill": [POINTER(GxDetector), _P, _P, _P, _P, _P, _I, _P],
+    "gx_detector_adjoint_mapped": [POINTER(GxDetector), _P, _I, _P, _P, _P, _P, _I, _P, _P, _P],
 }
 EXPORTS = ["gx_last_error", "gx_abi_version", "gx_supported_grid"] + list(SIGNATURES)
 

@@ -209,17 +209,7 @@ class DifferentiableSlab:
         self.det_ow = torch.as_tensor(dp.orient_w).to(dev).contiguous()
         self.det_raw = _empty(npx * npx, torch.float64)
         self.det_out = _empty(npx * npx, torch.float64)
-        self.det_work = _empty(2 * npx * npx + (npx * npx + 255) // 256, torch.float64)
-        # the detector adjoint goes through a transpose map of the orientation gather
-        # when it fits (<= 32 GB of pair ids and their voxels, of the 288 GB of HBM),
-        # else scatters into copies of its accumulator (<= 256 MB)
-        n_pairs = npx * npx * len(dp.orient_w)
-        pbits = int(np.ceil(np.log2(max(npx * npx, 1))))
-        self.mapped = (n_pairs * 8 <= MAP_BYTES_MAX and pbits < 32 and len(dp.orient_w) <= 2 ** (32 - pbits)
-                       and nb < 2 ** 31)
-        self.n_rep = 1 if self.mapped else int(max(1, min(8, 256e6 // (8 * nb))))
-        self.g_fx = _empty(self.n_rep * nb, torch.int64)
-        self.det_map = None
+        self.det_work = _empty(3 * npx * npx + (npx * npx + 255) // 256, torch.float64)
         self.fx_scale = _empty(1, torch.float64)
         ax = g.crop_axis
         self.detector_desc = GxDetector(
@@ -229,27 +219,46 @@ class DifferentiableSlab:
             orient_w=ptr(self.det_ow).value, iq=ptr(self.iq64).value, iq_f64=1,
             cy=g.c_n, cx=g.c_n, cz=g.c_n, qx_min=float(np.min(ax)), qy_min=float(np.min(ax)),
             qz_min=float(np.min(ax)), dq=float(np.diff(ax)[0]), det=ptr(self.det_raw).value)
-        if self.mapped:
+        # the detector adjoint goes through a transpose map of the orientation gather
+        # when it fits (<= 32 GB of runs of pixels, 9 bytes a run, of the 288 GB of HBM),
+        # else scatters into copies of its accumulator (<= 256 MB)
+        pbits = int(np.ceil(np.log2(max(npx * npx, 1))))
+        self.det_map = None
+        if pbits < 32 and len(dp.orient_w) <= 2 ** (32 - pbits) and nb < 2 ** 31:
             # the map depends on the detector geometry and the q grid only: models of
             # the same geometry (a batch of slabs) share the last one built
             key = hashlib.sha1(b''.join(np.ascontiguousarray(a, dtype=np.float64).tobytes() for a in
                                         (dp.h, dp.v, dp.init_R, dp.orient_R, np.array(
                                             [g.c_n, np.min(ax), np.diff(ax)[0], dev.index or 0])))).hexdigest()
-            hit = _DET_MAPS.get(key)
-            if hit is None:
+            if key not in _DET_MAPS:
                 _DET_MAPS.clear()
-                cnt = _empty(nb, torch.int32)
-                scan = _empty(-(-nb // 2048) + 1, torch.int64)
-                off = _empty(nb + 1, torch.int64)
-                pairs = _empty(n_pairs, torch.int32)
-                pair_voxel = _empty(n_pairs, torch.int32)
-                call("gx_detector_map_build", ctypes.byref(self.detector_desc), ptr(cnt), ptr(scan), ptr(off),
-                     ptr(pairs), ptr(pair_voxel), stream_handle())
-                hit = _DET_MAPS[key] = (pair_voxel, pairs)
-                del cnt, scan, off
-            self.det_map = hit
+                _DET_MAPS[key] = self._build_map(nb)
+            self.det_map = _DET_MAPS[key]
+        self.mapped = self.det_map is not None
+        self.n_rep = 1 if self.mapped else int(max(1, min(8, 256e6 // (8 * nb))))
+        self.g_fx = _empty(self.n_rep * nb, torch.int64)
         self._generation = 0
         self._prepared = None
+
+    # ----------------------------------------------------------------------------
+    def _build_map(self, nb):
+        """The detector's transpose map as runs of pixels (gx_detector_map_count, then
+        gx_detector_map_fill into arrays of exactly the counted size): (run_voxel, runs,
+        run_len, n_runs), or None when the runs would take more than MAP_BYTES_MAX."""
+        st = stream_handle()
+        cnt = _empty(nb, torch.int32)
+        scan = _empty(-(-nb // 2048) + 1, torch.int64)
+        off = _empty(nb + 1, torch.int64)
+        call("gx_detector_map_count", ctypes.byref(self.detector_desc), ptr(cnt), ptr(scan), ptr(off), st)
+        n_runs = int(off[nb].item())
+        if n_runs < 1 or n_runs * 9 > MAP_BYTES_MAX:
+            return None
+        runs = _empty(n_runs, torch.int32)
+        run_voxel = _empty(n_runs, torch.int32)
+        run_len = _empty(n_runs, torch.uint8)
+        call("gx_detector_map_fill", ctypes.byref(self.detector_desc), ptr(off), ptr(cnt), ptr(runs),
+             ptr(run_voxel), ptr(run_len), n_runs, st)
+        return run_voxel, runs, run_len, n_runs
 
     # ----------------------------------------------------------------------------
     def _rows_for(self, zb):
@@ -437,8 +446,8 @@ class DifferentiableSlab:
         g = self.plan.grid
         if self.mapped:
             call("gx_detector_adjoint_mapped", ctypes.byref(self.detector_desc), ptr(g_out), self._flags(),
-                 ptr(self.det_work), ptr(self.det_map[0]), ptr(self.det_map[1]), ptr(self.g_fx),
-                 ptr(self.fx_scale), st)
+                 ptr(self.det_work), ptr(self.det_map[0]), ptr(self.det_map[1]), ptr(self.det_map[2]),
+                 self.det_map[3], ptr(self.g_fx), ptr(self.fx_scale), st)
         else:
             call("gx_detector_adjoint", ctypes.byref(self.detector_desc), ptr(g_out), self._flags(),
                  ptr(self.det_work), ptr(self.g_fx), self.n_rep, ptr(self.fx_scale), st)

@@ -375,19 +375,45 @@ __global__ __launch_bounds__(BLK) void k_det_adj(gx_detector d, const double *__
 }
 
 // ------------------------------------------------------------------------------------
-// transpose map of the orientation gather (fixed detector geometry): per voxel, the
-// (pixel, orientation) pairs that read it, as pair ids p * n_orient + o
+// transpose map of the orientation gather (fixed detector geometry), stored as runs.
+// One lane per pixel.  Neighbouring pixels of a row mostly read the same voxel (a
+// detector pixel is several times finer than a voxel), so per orientation each wave
+// splits its lanes into runs of equal voxel; a run (consecutive pixels p .. p + len - 1,
+// len <= 64, one orientation o, one voxel v) is one entry of the map: (o << b) | p, len
+// and v.  Only a run's first lane issues the atomic.  The order of a voxel's runs
+// depends on the timing of the atomics; the adjoint sums exact integers, so its result
+// does not.
 // ------------------------------------------------------------------------------------
+
+// heads of the runs of equal key across the wave (bit l: lane l starts a run; lane 0
+// always does).  Lanes without a voxel carry a key of their own (-1 - lane).
+__device__ __forceinline__ unsigned long long run_heads(int key, int lane) {
+  const int up = __shfl_up(key, 1, GX_WAVE);
+  return __ballot(lane == 0 || up != key);
+}
+
+// length of the run that starts at `lane` (a head)
+__device__ __forceinline__ int run_length(unsigned long long heads, int lane) {
+  const unsigned long long above = lane == GX_WAVE - 1 ? 0ull : heads >> (lane + 1);
+  return above ? __builtin_ctzll(above) + 1 : GX_WAVE - lane;
+}
+
+// runs per voxel
 __global__ __launch_bounds__(BLK) void k_det_map_count(gx_detector d, int32_t *cnt) {
   extern __shared__ unsigned char idf[];
   const int nflag = orient_flags(d, idf);
+  const int lane = threadIdx.x & (GX_WAVE - 1);
   const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (p >= d.n_pix * d.n_pix) return;
-  double x0, y0, z0;
-  det_pixel(d, p, x0, y0, z0);
+  const bool live = p < d.n_pix * d.n_pix;
+  if (__all(!live)) return;  // wave-uniform: the lane exchanges below need the whole wave
+  double x0 = 0.0, y0 = 0.0, z0 = 0.0;
+  if (live) det_pixel(d, p, x0, y0, z0);
   const double dqinv = 1.0 / d.dq;
-  for (int64_t o = 0; o < d.n_orient; ++o)
-    atomicAdd(&cnt[det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u)], 1);
+  for (int64_t o = 0; o < d.n_orient; ++o) {
+    const int key = live ? (int)det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u) : -1 - lane;
+    const unsigned long long heads = run_heads(key, lane);
+    if (live && ((heads >> lane) & 1ull)) atomicAdd(&cnt[key], 1);
+  }
 }
 
 // exclusive scan of int32 counts into int64 offsets, SCAN_B elements per block
@@ -443,62 +469,106 @@ __global__ __launch_bounds__(BLK) void k_scan_add(int64_t *off, int64_t n, const
   if (i == 0) off[n] = bsum[nb];
 }
 
-__global__ __launch_bounds__(BLK) void k_det_map_fill(gx_detector d, const int64_t *off, int32_t *cur, uint32_t *pairs,
-                                                      int32_t *pair_voxel, int pbits) {
+// the same runs as k_det_map_count, each written at its voxel's next slot (positions
+// at or beyond `cap` are not written)
+__global__ __launch_bounds__(BLK) void k_det_map_fill(gx_detector d, const int64_t *off, int32_t *cur, uint32_t *runs,
+                                                      int32_t *run_voxel, uint8_t *run_len, int64_t cap, int pbits) {
   extern __shared__ unsigned char idf[];
   const int nflag = orient_flags(d, idf);
+  const int lane = threadIdx.x & (GX_WAVE - 1);
   const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (p >= d.n_pix * d.n_pix) return;
-  double x0, y0, z0;
-  det_pixel(d, p, x0, y0, z0);
+  const bool live = p < d.n_pix * d.n_pix;
+  if (__all(!live)) return;  // wave-uniform
+  double x0 = 0.0, y0 = 0.0, z0 = 0.0;
+  if (live) det_pixel(d, p, x0, y0, z0);
   const double dqinv = 1.0 / d.dq;
   for (int64_t o = 0; o < d.n_orient; ++o) {
-    const int64_t v = det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u);
-    const int64_t pos = off[v] + atomicAdd(&cur[v], 1);
-    pairs[pos] = ((uint32_t)o << pbits) | (uint32_t)p;
-    pair_voxel[pos] = (int32_t)v;
+    const int key = live ? (int)det_voxel(d, x0, y0, z0, o, dqinv, o < nflag ? idf[o] : 0u) : -1 - lane;
+    const unsigned long long heads = run_heads(key, lane);
+    if (live && ((heads >> lane) & 1ull)) {
+      const int64_t pos = off[key] + atomicAdd(&cur[key], 1);
+      if (pos < cap) {
+        runs[pos] = ((uint32_t)o << pbits) | (uint32_t)p;
+        run_voxel[pos] = key;
+        run_len[pos] = (uint8_t)run_length(heads, lane);
+      }
+    }
   }
 }
 
-// dL/d iq per voxel from the transpose map, balanced over pairs: a wave takes
-// MAP_UNROLL chunks of 64 consecutive pairs of the voxel-sorted map (pair ids and
-// their voxels; every load of the chunks is issued before the first use), computes
-// each pair's fixed-point term g[p] w[o], sums runs of equal voxel with a segmented
-// scan across the lanes, and the last lane of each run adds the run's total to
-// g_fx[voxel] with an integer atomic (a hot voxel costs one atomic per 64 of its
-// pairs).  Integer sums are exact, so the result does not depend on the order.
+// per-pixel fixed-point terms g[p] w0 (w0 = the first orientation's weight) as
+// inclusive prefix sums within each aligned segment of 64 pixels (the segments the map's
+// runs are cut in): a run p .. p + len - 1 at an orientation of weight w0 sums to
+// S[p + len - 1] - S[p - 1] (0 for the first pixel of a segment), the same integers the
+// per-pixel loop adds.  The scale bounds every partial sum below 2^61.
+__global__ __launch_bounds__(BLK) void k_det_adj_prefix(const double *__restrict__ g, const double *orient_w,
+                                                        const double *fx_scale, int64_t npx, long long *S) {
+  const int lane = threadIdx.x & (GX_WAVE - 1);
+  const int64_t p = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  const double sc = fx_scale[0], w0 = orient_w[0];
+  long long t = 0;
+  if (p < npx) t = llrint((g[p] * w0) * sc);
+#pragma unroll
+  for (int o = 1; o < GX_WAVE; o <<= 1) {
+    const long long u = __shfl_up(t, o, GX_WAVE);
+    if (lane >= o) t += u;
+  }
+  if (p < npx) S[p] = t;
+}
+
+// dL/d iq per voxel from the transpose map, balanced over runs: a wave takes
+// MAP_UNROLL chunks of 64 consecutive runs of the voxel-sorted map (every load of the
+// chunks is issued before the first use); each lane sums its run's fixed-point terms
+// g[p] w[o] (one per pixel, rounded as the scatter path rounds them: from the prefix
+// table when w[o] is the first orientation's weight, the usual case of uniform weights,
+// else pixel by pixel), the wave sums runs of equal voxel with a segmented scan across
+// the lanes, and the last lane of each segment adds its total to g_fx[voxel] with an
+// integer atomic.  Integer sums are exact, so the result does not depend on the order.
 // g_fx starts at zero.
 constexpr int MAP_UNROLL = 4;
 
 __global__ __launch_bounds__(BLK) void k_det_adj_map(gx_detector d, const double *__restrict__ g,
-                                                     const double *fx_scale, const int32_t *__restrict__ pair_voxel,
-                                                     const uint32_t *__restrict__ pairs, int64_t n_pairs,
+                                                     const long long *__restrict__ S, const double *fx_scale,
+                                                     const int32_t *__restrict__ run_voxel,
+                                                     const uint32_t *__restrict__ runs,
+                                                     const uint8_t *__restrict__ run_len, int64_t n_runs,
                                                      unsigned long long *g_fx, int pbits) {
-  constexpr int SPAN = MAP_UNROLL * GX_WAVE;  // pairs per wave and step
+  constexpr int SPAN = MAP_UNROLL * GX_WAVE;  // runs per wave and step
   const int lane = threadIdx.x & (GX_WAVE - 1);
-  const double sc = fx_scale[0];
+  const double sc = fx_scale[0], w0 = d.orient_w[0];
   const uint32_t pmask = (pbits >= 32) ? 0xffffffffu : ((1u << pbits) - 1u);
   const int64_t wave0 = (blockIdx.x * (int64_t)BLK + threadIdx.x) / GX_WAVE;
   const int64_t n_waves = (int64_t)gridDim.x * (BLK / GX_WAVE);
-  for (int64_t base = wave0 * SPAN; base < n_pairs; base += n_waves * SPAN) {
-    int v[MAP_UNROLL];
+  for (int64_t base = wave0 * SPAN; base < n_runs; base += n_waves * SPAN) {
+    int v[MAP_UNROLL], len[MAP_UNROLL];
     uint32_t id[MAP_UNROLL];
 #pragma unroll
     for (int u = 0; u < MAP_UNROLL; ++u) {
       const int64_t k = base + u * GX_WAVE + lane;
-      v[u] = k < n_pairs ? pair_voxel[k] : -1;
-      id[u] = k < n_pairs ? pairs[k] : 0u;
+      v[u] = -1;
+      id[u] = 0u;
+      len[u] = 0;
+      if (k < n_runs) {
+        v[u] = run_voxel[k];
+        id[u] = runs[k];
+        len[u] = run_len[k];
+      }
     }
-    double gv[MAP_UNROLL], wv[MAP_UNROLL];
 #pragma unroll
     for (int u = 0; u < MAP_UNROLL; ++u) {
-      gv[u] = v[u] >= 0 ? g[id[u] & pmask] : 0.0;
-      wv[u] = v[u] >= 0 ? d.orient_w[id[u] >> pbits] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < MAP_UNROLL; ++u) {
-      long long t = v[u] >= 0 ? llrint((gv[u] * wv[u]) * sc) : 0;
-      // inclusive scan within runs of equal voxel (contiguous: the map is voxel-sorted)
+      long long t = 0;
+      if (v[u] >= 0) {
+        const uint32_t p = id[u] & pmask;
+        const double w = d.orient_w[id[u] >> pbits];
+        if (w == w0) {
+          long long lo = 0;
+          if (p & (GX_WAVE - 1)) lo = S[p - 1];
+          t = S[p + len[u] - 1] - lo;
+        } else {
+          for (int j = 0; j < len[u]; ++j) t += llrint((g[p + j] * w) * sc);
+        }
+      }
+      // inclusive scan within segments of equal voxel (contiguous: the map is voxel-sorted)
 #pragma unroll
       for (int o = 1; o < GX_WAVE; o <<= 1) {
         const long long tu = __shfl_up(t, o, GX_WAVE);
@@ -827,22 +897,29 @@ static int map_pixel_bits(int64_t npx) {
   return b;
 }
 
-int gx_detector_map_build(const gx_detector *d, int32_t *counts, int64_t *scan_work, int64_t *voxel_off,
-                          uint32_t *pairs, int32_t *pair_voxel, gx_stream_t stream) {
-  clear_error();
+static int check_map_detector(const gx_detector *d) {
   GX_REQUIRE(d != nullptr, "NULL detector");
   GX_REQUIRE(d->n_pix >= 1 && d->n_pix <= 65536 && d->n_init >= 0 && d->n_init <= 3 && d->n_orient >= 1,
              "detector sizes");
   GX_REQUIRE(d->cy >= 1 && d->cx >= 1 && d->cz >= 1 && d->dq > 0.0, "iq grid");
   GX_REQUIRE(d->h_axis && d->v_axis && (d->n_init == 0 || d->init_R) && d->orient_R && d->orient_w,
              "a detector buffer is NULL");
-  GX_REQUIRE(counts && scan_work && voxel_off && pairs && pair_voxel, "NULL map buffer");
   GX_REQUIRE(d->cy * d->cx * d->cz < (1ll << 31), "iq grid too large for 32-bit voxel ids");
-  const int64_t npx = d->n_pix * d->n_pix, nv = d->cy * d->cx * d->cz;
+  const int64_t npx = d->n_pix * d->n_pix;
   const int pbits = map_pixel_bits(npx);
   GX_REQUIRE(pbits < 32 && d->n_orient <= (1ll << (32 - pbits)),
-             "%lld orientations x %lld pixels do not pack into 32-bit pair ids", (long long)d->n_orient,
+             "%lld orientations x %lld pixels do not pack into 32-bit run ids", (long long)d->n_orient,
              (long long)npx);
+  return GX_OK;
+}
+
+int gx_detector_map_count(const gx_detector *d, int32_t *counts, int64_t *scan_work, int64_t *voxel_off,
+                          gx_stream_t stream) {
+  clear_error();
+  const int rc = check_map_detector(d);
+  if (rc != GX_OK) return rc;
+  GX_REQUIRE(counts && scan_work && voxel_off, "NULL map buffer");
+  const int64_t npx = d->n_pix * d->n_pix, nv = d->cy * d->cx * d->cz;
   hipStream_t st = as_stream(stream);
   const int64_t nb = (nv + SCAN_B - 1) / SCAN_B;
   k_fill_zero_i32<<<grid1d(nv, BLK) > 8192 ? 8192 : grid1d(nv, BLK), BLK, 0, st>>>(counts, nv);
@@ -850,23 +927,39 @@ int gx_detector_map_build(const gx_detector *d, int32_t *counts, int64_t *scan_w
   k_scan_local<<<(unsigned)nb, BLK, 0, st>>>(counts, nv, voxel_off, scan_work);
   k_scan_blocks<<<1, 64, 0, st>>>(scan_work, nb);
   k_scan_add<<<grid1d(nv, BLK), BLK, 0, st>>>(voxel_off, nv, scan_work, nb, counts);
-  k_det_map_fill<<<grid1d(npx, BLK), BLK, orient_flags_lds(d), st>>>(*d, voxel_off, counts, pairs, pair_voxel, pbits);
-  return check_launch("gx_detector_map_build");
+  return check_launch("gx_detector_map_count");
+}
+
+int gx_detector_map_fill(const gx_detector *d, const int64_t *voxel_off, int32_t *counts, uint32_t *runs,
+                         int32_t *run_voxel, uint8_t *run_len, int64_t capacity, gx_stream_t stream) {
+  clear_error();
+  const int rc = check_map_detector(d);
+  if (rc != GX_OK) return rc;
+  GX_REQUIRE(counts && voxel_off && runs && run_voxel && run_len, "NULL map buffer");
+  GX_REQUIRE(capacity >= 0, "capacity=%lld", (long long)capacity);
+  const int64_t npx = d->n_pix * d->n_pix;
+  k_det_map_fill<<<grid1d(npx, BLK), BLK, orient_flags_lds(d), as_stream(stream)>>>(
+      *d, voxel_off, counts, runs, run_voxel, run_len, capacity, map_pixel_bits(npx));
+  return check_launch("gx_detector_map_fill");
 }
 
 int gx_detector_adjoint_mapped(const gx_detector *d, const double *g_out, int64_t flags, double *work,
-                               const int32_t *pair_voxel, const uint32_t *pairs, unsigned long long *g_fx,
-                               double *fx_scale, gx_stream_t stream) {
+                               const int32_t *run_voxel, const uint32_t *runs, const uint8_t *run_len, int64_t n_runs,
+                               unsigned long long *g_fx, double *fx_scale, gx_stream_t stream) {
   clear_error();
   GX_REQUIRE(d != nullptr, "NULL detector");
   GX_REQUIRE(d->n_pix >= 1 && d->n_pix <= 65536 && d->n_orient >= 1, "detector sizes");
   GX_REQUIRE(d->cy >= 1 && d->cx >= 1 && d->cz >= 1, "empty iq grid");
-  GX_REQUIRE(d->det && d->orient_w && g_out && work && pair_voxel && pairs && g_fx && fx_scale,
+  GX_REQUIRE(d->det && d->orient_w && g_out && work && run_voxel && runs && run_len && g_fx && fx_scale,
              "NULL adjoint buffer");
-  hipStream_t st = as_stream(stream);
   const int64_t n = d->n_pix, npx = n * n;
+  GX_REQUIRE(n_runs >= 1 && n_runs <= npx * d->n_orient, "n_runs=%lld outside [1, n_pix^2 n_orient]",
+             (long long)n_runs);
+  hipStream_t st = as_stream(stream);
   const unsigned nblk = grid1d(npx, BLK);
-  double *t = work, *g_raw = work + npx, *part = work + 2 * npx;
+  double *t = work, *g_raw = work + npx;
+  long long *S = reinterpret_cast<long long *>(work + 2 * npx);
+  double *part = work + 3 * npx;
   k_det_fin_adj<<<nblk, BLK, 0, st>>>(d->det, g_out, n, n, flags, t, part);
   if (flags & GX_FIN_MIRROR)
     k_detector_finalize<<<nblk, BLK, 0, st>>>(t, n, n, GX_FIN_MIRROR, g_raw);
@@ -874,12 +967,13 @@ int gx_detector_adjoint_mapped(const gx_detector *d, const double *g_out, int64_
     g_raw = t;
   k_det_scale<<<1, BLK, 0, st>>>(part, nblk, (flags & GX_FIN_MIRROR) ? 4.0 : 1.0, d->orient_w, d->n_orient,
                                  fx_scale);
-  const int64_t nv = d->cy * d->cx * d->cz, n_pairs = npx * d->n_orient;
+  k_det_adj_prefix<<<nblk, BLK, 0, st>>>(g_raw, d->orient_w, fx_scale, npx, S);
+  const int64_t nv = d->cy * d->cx * d->cz;
   const unsigned gz = grid1d(nv, BLK);
   k_zero_u64<<<gz > 8192 ? 8192 : gz, BLK, 0, st>>>(g_fx, nv);
-  const unsigned gp = grid1d(n_pairs, BLK * MAP_UNROLL);  // a wave per 64 MAP_UNROLL pairs, grid-strided
-  k_det_adj_map<<<gp > 65536 ? 65536 : gp, BLK, 0, st>>>(*d, g_raw, fx_scale, pair_voxel, pairs, n_pairs, g_fx,
-                                                         map_pixel_bits(npx));
+  const unsigned gp = grid1d(n_runs, BLK * MAP_UNROLL);  // a wave per 64 MAP_UNROLL runs, grid-strided
+  k_det_adj_map<<<gp > 65536 ? 65536 : gp, BLK, 0, st>>>(*d, g_raw, S, fx_scale, run_voxel, runs, run_len, n_runs,
+                                                         g_fx, map_pixel_bits(npx));
   return check_launch("gx_detector_adjoint_mapped");
 }
 

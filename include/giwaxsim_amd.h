@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GX_ABI_VERSION 9
+#define GX_ABI_VERSION 10
 
 #define GX_OK 0
 #define GX_ERR_ARG 1         /* invalid argument (message says which) */
@@ -510,22 +510,33 @@ int gx_detector_adjoint(const gx_detector *d, const double *g_out, int64_t flags
                         unsigned long long *g_fx, int64_t n_rep, double *fx_scale, gx_stream_t stream);
 
 /* Transpose map of the orientation gather for a fixed detector geometry (no
- * reference counterpart): for every voxel v of d's [cy][cx][cz] grid, the
- * (pixel p, orientation o) pairs whose gather reads v, stored as (o << b) | p with
- * b = ceil(log2(n_pix^2)) in pairs[voxel_off[v] .. voxel_off[v + 1]) (order within a
- * voxel unspecified; n_orient must fit the 32 - b remaining bits), and beside each
- * pair its voxel v in pair_voxel (the same positions).
- * Sizes: counts [nv] int32 scratch, scan_work [ceil(nv / 2048) + 1], voxel_off
- * [nv + 1], pairs and pair_voxel [n_pix^2 n_orient], nv = cy cx cz < 2^31.  d->iq and
- * d->det are not used. */
-int gx_detector_map_build(const gx_detector *d, int32_t *counts, int64_t *scan_work, int64_t *voxel_off,
-                          uint32_t *pairs, int32_t *pair_voxel, gx_stream_t stream);
-/* gx_detector_adjoint through the map: runs of equal voxel are summed across a wave
- * before one integer atomic per run; writes one copy of g_fx (n_rep = 1 for
- * gx_bins_adjoint). */
+ * reference counterpart), as runs: a run is up to 64 consecutive pixels p .. p + len - 1
+ * whose gather at one orientation o reads the same voxel v of d's [cy][cx][cz] grid
+ * (a detector pixel is usually several times finer than a voxel).  Built in two calls:
+ *
+ * gx_detector_map_count writes voxel_off [nv + 1], the exclusive prefix sum of the
+ * runs per voxel (voxel_off[nv] = the number of runs), and leaves counts zeroed for the
+ * fill.  Sizes: counts [nv] int32 scratch, scan_work [ceil(nv / 2048) + 1],
+ * nv = cy cx cz < 2^31; n_orient must fit the 32 - b bits above the pixel,
+ * b = ceil(log2(n_pix^2)).  d->iq and d->det are not used.
+ *
+ * gx_detector_map_fill then writes, for every voxel v, its runs in
+ * [voxel_off[v], voxel_off[v + 1]) (order within a voxel unspecified): runs = (o << b) | p
+ * of the run's first pixel, run_voxel = v, run_len = len.  The three arrays hold
+ * `capacity` entries (>= voxel_off[nv]); positions beyond it are not written. */
+int gx_detector_map_count(const gx_detector *d, int32_t *counts, int64_t *scan_work, int64_t *voxel_off,
+                          gx_stream_t stream);
+int gx_detector_map_fill(const gx_detector *d, const int64_t *voxel_off, int32_t *counts, uint32_t *runs,
+                         int32_t *run_voxel, uint8_t *run_len, int64_t capacity, gx_stream_t stream);
+/* gx_detector_adjoint through the map (n_runs = voxel_off[nv] of the count): each run's
+ * per-pixel fixed-point terms are summed in a lane (from a table of their prefix sums
+ * for orientations of the first orientation's weight), runs of equal voxel across a
+ * wave, then one integer atomic per voxel segment; writes one copy of g_fx (n_rep = 1
+ * for gx_bins_adjoint), bit for bit the sums of gx_detector_adjoint.
+ * work: [3 n^2 + ceil(n^2 / 256)] doubles. */
 int gx_detector_adjoint_mapped(const gx_detector *d, const double *g_out, int64_t flags, double *work,
-                               const int32_t *pair_voxel, const uint32_t *pairs, unsigned long long *g_fx,
-                               double *fx_scale, gx_stream_t stream);
+                               const int32_t *run_voxel, const uint32_t *runs, const uint8_t *run_len, int64_t n_runs,
+                               unsigned long long *g_fx, double *fx_scale, gx_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,9 @@ Checks, through the C ABI (giwaxsim_amd.adjoint):
   256^2 detector, psi 0..90 step 2, target with c x 1.02, fitted scale and
   background, taken 5% away from the least-squares optimum so that dL/ds and
   dL/db are not zero): gradients vs FP64 central differences (h = 1e-4 A) of
-  the GPU forward on 64 random coordinates, and of f, scale and background.
+  the GPU forward on 64 random coordinates, and of f, scale and background;
+  and the same loss, image and gradients vs the FP64 autograd reference over all
+  453 frames (FP64 and FP32 transforms, the bars above).
 """
 
 import sys
@@ -244,6 +246,64 @@ def test_workload_gradients_vs_central_differences():
     assert max(ef) <= 1e-7
     assert abs(fds - s.grad.item()) <= 1e-7 * abs(fds)
     assert abs(fdb - b.grad.item()) <= 1e-7 * abs(fdb)
+
+
+_WORKLOAD_REF = {}
+
+
+@pytest.mark.parametrize("precision,tol_img,tol_grad", [('fp64', 1e-10, 1e-10), ('fp32', 2e-6, 1e-4)])
+def test_workload_matches_autograd(precision, tol_img, tol_grad):
+    """The differentiation workload end to end (all 453 frames, the 256^2 detector, the
+    image loss at a scale and background 5% off the least-squares optimum) against the
+    FP64 autograd reference: image and dL/d coords, dL/d f_re, dL/d f_im, dL/ds, dL/db."""
+    coords, el, target, tel, qv, det = _workload()
+    dev = torch.device('cuda')
+    tmodel = DifferentiableSlab(tel, target, R, qv, MAXQ, det, precision='fp64')
+    with torch.no_grad():
+        T = tmodel(torch.tensor(target, device=dev))
+    model = DifferentiableSlab(el, coords, R, qv, MAXQ, det, precision=precision)
+    assert model.plan.N == 512 and model.plan.n_frames == 453
+    with torch.no_grad():
+        img0 = model(torch.tensor(coords, device=dev))
+    s_fit, b_fit = fit_scale_background(img0, T)
+    s0, b0 = 1.05 * s_fit, b_fit + 0.05 * float(T.mean())
+    _say(f"[gpu-adjoint] workload graphite 40x40x30 ({len(coords)} atoms) N=512 all frames, "
+         f"{precision} vs FP64 autograd")
+    Tc = T.cpu()
+    if 'ref' not in _WORKLOAD_REF:
+        ids = np.array([model.elements.index(e) for e in el])
+        ref = CicModel(coords, ids, R, qv, MAXQ, det)
+        ct = torch.tensor(coords, dtype=torch.float64, requires_grad=True)
+        frt = model.f_re0.cpu().clone().requires_grad_(True)
+        fit = model.f_im0.cpu().clone().requires_grad_(True)
+        st = torch.tensor(s0, dtype=torch.float64, requires_grad=True)
+        bt = torch.tensor(b0, dtype=torch.float64, requires_grad=True)
+        want = ref(ct, frt, fit)
+        image_loss(want, Tc, st, bt).backward()
+        _WORKLOAD_REF['ref'] = (s0, b0, want.detach().numpy(), ct.grad.numpy(), frt.grad.numpy(),
+                                fit.grad.numpy(), st.grad.item(), bt.grad.item())
+    # the scale and background of the first call for both (the fit of an FP32 image differs
+    # by ~1e-7)
+    s0, b0, w, gw, gfr, gfi, gs, gb = _WORKLOAD_REF['ref']
+    _say(f"  scale {s0:.6e} background {b0:.6e}")
+    c = torch.tensor(coords, dtype=torch.float64, device=dev, requires_grad=True)
+    fr = model.f_re0.clone().requires_grad_(True)
+    fi = model.f_im0.clone().requires_grad_(True)
+    s = torch.tensor(s0, dtype=torch.float64, device=dev, requires_grad=True)
+    b = torch.tensor(b0, dtype=torch.float64, device=dev, requires_grad=True)
+    img = model(c, fr, fi)
+    image_loss(img, T, s, b).backward()
+    e_img = np.max(np.abs(img.detach().cpu().numpy() - w)) / np.max(np.abs(w))
+    e_c = np.max(np.abs(c.grad.cpu().numpy() - gw)) / np.max(np.abs(gw))
+    e_fr = np.max(np.abs(fr.grad.cpu().numpy() - gfr)) / np.max(np.abs(gfr))
+    e_fi = np.max(np.abs(fi.grad.cpu().numpy() - gfi)) / np.max(np.abs(gfi))
+    e_s = abs(s.grad.item() - gs) / abs(gs)
+    e_b = abs(b.grad.item() - gb) / abs(gb)
+    _say(f"  image {e_img:.2e}  d/dcoords {e_c:.2e}  d/df_re {e_fr:.2e}  d/df_im {e_fi:.2e}  "
+         f"d/ds {e_s:.2e}  d/db {e_b:.2e}")
+    assert e_img <= tol_img
+    assert e_c <= tol_grad and e_fr <= tol_grad and e_fi <= tol_grad
+    assert e_s <= tol_grad and e_b <= tol_grad
 
 
 _LARGE_REF = {}
