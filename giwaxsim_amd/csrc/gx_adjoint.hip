@@ -43,12 +43,17 @@ constexpr int RED = GX_ADJ_RED_BLOCKS;
 
 // fractional grid coordinate along y of an atom in a frame: u = (y' - ymin) / r,
 // u0 = exact floor (the same index arithmetic as the forward path), tu in [0, 1]
-__device__ __forceinline__ void cic_u(double x, double y, double s, double c, double ymin, double r, double rinv,
-                                      long long &u0, double &tu) {
+// returns the cell u0 when both nodes u0, u0 + 1 lie in a row of N points, else -1
+// (then tu is not set); the floor division is the reference's `//`, and the int32
+// conversion of the integral quotient is exact in that range
+__device__ __forceinline__ int cic_u(double x, double y, double s, double c, double ymin, double r, double rinv,
+                                     int N, double &tu) {
   const double yy = (x * s + y * c) - ymin;
-  u0 = to_int(floor_div(yy, r, rinv));
-  const double t = yy * rinv - (double)u0;
+  const double q = floor_div(yy, r, rinv);
+  if (!(q >= 0.0 && q < (double)(N - 1))) return -1;
+  const double t = yy * rinv - q;
   tu = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+  return (int)q;
 }
 
 template <typename C>
@@ -157,6 +162,13 @@ template <typename C> struct DepositAcc;
 template <> struct DepositAcc<float2> { using T = int32_t; using U = unsigned int; static constexpr int BITS = 31; };
 template <> struct DepositAcc<double2> { using T = long long; using U = unsigned long long; static constexpr int BITS = 62; };
 
+// a deposit term rounded to the nearest integer (ties to even, as llrint) in the
+// accumulator's type: the row exponent keeps every term inside it, so the int32 case
+// converts directly (two instructions instead of llrint's int64 conversion)
+template <typename AT> __device__ __forceinline__ AT fx_round(double x);
+template <> __device__ __forceinline__ int32_t fx_round<int32_t>(double x) { return (int32_t)rint(x); }
+template <> __device__ __forceinline__ long long fx_round<long long>(double x) { return llrint(x); }
+
 // Atoms per thread in flight in the atom loops of the row-wise kernels: the loops are
 // bound by the latency of the atoms' loads otherwise.
 constexpr int AU_UNROLL = 4;
@@ -173,7 +185,7 @@ struct RowPassShape {
 // of rows z0-1 .. z0+RB-1 once (an atom of row v0 reaches rows v0 and v0+1), then
 // each wave transforms its rows.
 template <int N, typename C>
-__global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, int64_t f0) {
+__global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf) {
   using T = typename Real<C>::T;
   using AT = typename DepositAcc<C>::T;
   using AU = typename DepositAcc<C>::U;
@@ -190,16 +202,17 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
 
-  const int fb = blockIdx.y;
+  const RowFrame rf = row_frame_tile(blockIdx.x, (d.Z + RB - 1) / RB, nf);
+  const int fb = rf.fb;
   const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
   const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
   const int wf = jhi - jlo + 1;
   if (wf <= 0) return;  // block-uniform
   if (wf > d.W) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
+    if (threadIdx.x == 0 && rf.rb == 0) atomicAdd(&d.status[1], 1);
     return;
   }
-  const int z0 = blockIdx.x * RB;
+  const int z0 = rf.rb * RB;
   for (int i = threadIdx.x; i < 2 * g.n_elem; i += BLK) ftab[i] = g.f_table[i];
   if (d.fill_bkg || d.smooth) {
     const C *spec = reinterpret_cast<const C *>(d.col_spec) + (int64_t)fb * d.W;
@@ -238,10 +251,9 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
     for (int k = 0; k < AU_UNROLL; ++k) {
       if (a + k * BLK >= a_hi) break;
       const int lv = zr[k] - z0;  // -1 .. RB-1
-      long long u0;
-      double tu;
-      cic_u(xs[k], ys[k], s, c, ymin, r, rinv, u0, tu);
-      if (u0 < 0 || u0 + 1 >= N) {
+      double tu = 0.0;
+      const int u0 = cic_u(xs[k], ys[k], s, c, ymin, r, rinv, N, tu);
+      if (u0 < 0) {
         atomicAdd(&d.status[2], 1);
         continue;
       }
@@ -252,13 +264,15 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
         const int row = lv + h;
         if (row < 0 || row >= RB) continue;
         const double wv = h ? tv : 1.0 - tv;
+        // the power-of-two row scale sc goes into the weights: f (w sc) rounds to
+        // exactly (f w) sc
         const double sc = rsc[row];
-        const double w0 = wv * (1.0 - tu), w1 = wv * tu;
+        const double w0 = (wv * (1.0 - tu)) * sc, w1 = (wv * tu) * sc;
         AU *ar = accs + (size_t)row * 2 * N;
-        atomicAdd(&ar[u0], (AU)(AT)llrint((fr * w0) * sc));
-        atomicAdd(&ar[N + u0], (AU)(AT)llrint((fi * w0) * sc));
-        atomicAdd(&ar[u0 + 1], (AU)(AT)llrint((fr * w1) * sc));
-        atomicAdd(&ar[N + u0 + 1], (AU)(AT)llrint((fi * w1) * sc));
+        atomicAdd(&ar[u0], (AU)fx_round<AT>(fr * w0));
+        atomicAdd(&ar[N + u0], (AU)fx_round<AT>(fi * w0));
+        atomicAdd(&ar[u0 + 1], (AU)fx_round<AT>(fr * w1));
+        atomicAdd(&ar[N + u0 + 1], (AU)fx_round<AT>(fi * w1));
       }
     }
   }
@@ -539,7 +553,7 @@ __device__ __forceinline__ double node_my(const gx_frames &d, const double *sy_r
 
 template <typename C>
 __device__ __forceinline__ void rowgrad_gather(const gx_frames &d, const gx_adjoint &g, const double *fp, int fb,
-                                               int z0, int RB, int N, const C *rows, const double *ftab,
+                                               int rb, int nrb, int z0, int RB, int N, const C *rows, const double *ftab,
                                                double *shd, int *shi, const double *mzr = nullptr,
                                                int ya = 0, int yb = 0) {
   using T = typename Real<C>::T;
@@ -579,13 +593,12 @@ __device__ __forceinline__ void rowgrad_gather(const gx_frames &d, const gx_adjo
       if (a >= a_hi) break;
       if ((xs[k] * s + ys[k] * c) == ymin) am = min(am, a);
       const int vr = zr[k] - z0;  // 0 .. RB-1
-      long long u0;
-      double tu;
-      cic_u(xs[k], ys[k], s, c, ymin, r, rinv, u0, tu);
+      double tu = 0.0;
+      const int u0 = cic_u(xs[k], ys[k], s, c, ymin, r, rinv, N, tu);
       const double tv = tvs[k];
       const double fr = ftab[2 * es[k]], fi = ftab[2 * es[k] + 1];
       double gu = 0.0, gv = 0.0, gfr = 0.0, gfi = 0.0;
-      if (u0 >= 0 && u0 + 1 < N && vr >= 0 && vr < RB) {
+      if (u0 >= 0 && vr >= 0 && vr < RB) {
         const C *r0 = rows + vr * N + u0;
         const C *r1 = r0 + N;
         double g00r = (double)r0[0].x, g00i = (double)r0[0].y;
@@ -614,12 +627,14 @@ __device__ __forceinline__ void rowgrad_gather(const gx_frames &d, const gx_adjo
       gp[a] = (T)gu;
       gp[n + a] = (T)gv;
       if (esum) {
+        // ef[e] += term for the atom's element, as fma(1, term, ef) there and
+        // fma(0, term, ef) elsewhere: the same sums, without a branch per element
 #pragma unroll
-        for (int q = 0; q < GX_ADJ_ELEM_REG; ++q)
-          if (es[k] == q) {
-            ef[2 * q] += gfr;
-            ef[2 * q + 1] += gfi;
-          }
+        for (int q = 0; q < GX_ADJ_ELEM_REG; ++q) {
+          const double sel = es[k] == q ? 1.0 : 0.0;
+          ef[2 * q] = fma(sel, gfr, ef[2 * q]);
+          ef[2 * q + 1] = fma(sel, gfi, ef[2 * q + 1]);
+        }
       } else {
         gp[2 * n + a] = (T)gfr;
         gp[3 * n + a] = (T)gfi;
@@ -630,14 +645,14 @@ __device__ __forceinline__ void rowgrad_gather(const gx_frames &d, const gx_adjo
   su = block_sum(su, shd);
   am = block_min_int(am, shi);
   if (threadIdx.x == 0) {
-    g.red[(int64_t)fb * gridDim.x + blockIdx.x] = su;
+    g.red[(int64_t)fb * nrb + rb] = su;
     if (am != INT_MAX) atomicMin(&g.amin[fb], am);
   }
   if (esum) {  // block-uniform
     __shared__ double she[NWAVE * 2 * GX_ADJ_ELEM_REG];
     block_sum_multi<2 * GX_ADJ_ELEM_REG>(ef, she);
     if (threadIdx.x == 0) {
-      double *ep = g.e_part + ((int64_t)fb * gridDim.x + blockIdx.x) * 2 * g.n_elem;
+      double *ep = g.e_part + ((int64_t)fb * nrb + rb) * 2 * g.n_elem;
       for (int q = 0; q < 2 * g.n_elem; ++q) ep[q] = ef[q];
     }
   }
@@ -645,10 +660,12 @@ __device__ __forceinline__ void rowgrad_gather(const gx_frames &d, const gx_adjo
 
 template <int N, typename C>
 struct RowGradShape {
-  // H rows per block in 32 KB of LDS (64 KB measured slower: fewer blocks per CU):
-  // RB = H - 1 rows own atoms, the last one is the halo row v0 + 1
+  // H rows per block in 32 KB of LDS, RB = H - 1 rows own atoms, the last one is the
+  // halo row v0 + 1.  At N = 2048 FP32 three rows (48 KB): each row transformed 1.5
+  // times instead of twice (measured 1.2 ms faster per 1307-frame slab; four rows,
+  // 64 KB, lost more to occupancy than they saved)
   static constexpr int FIT = 32768 / (N * (int)sizeof(C));
-  static constexpr int H = FIT < 2 ? 2 : (FIT < 8 ? FIT : 8);
+  static constexpr int H = (N == 2048 && sizeof(C) == 8) ? 3 : (FIT < 2 ? 2 : (FIT < 8 ? FIT : 8));
 };
 
 // One block = RB rows (+ the halo row) of one frame: the kept band of each row,
@@ -658,7 +675,7 @@ struct RowGradShape {
 // sum_a dL/dy'_a (regrouped order, fixed partition) goes to red, the argmin of y' to
 // amin.
 template <int N, typename C>
-__global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, int64_t f0) {
+__global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, int64_t f0, int64_t nf) {
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
   constexpr int H = RowGradShape<N, C>::H, RB = H - 1;
@@ -673,11 +690,13 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   __shared__ int shi[NWAVE];
   const int lane = threadIdx.x & (GX_WAVE - 1);
   const int wave = threadIdx.x / GX_WAVE;
-  const int fb = blockIdx.y;
+  const int nrb = (d.Z + RB - 1) / RB;
+  const RowFrame rf = row_frame_tile(blockIdx.x, nrb, nf);
+  const int fb = rf.fb;
   const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
   const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
   const int wf = max(0, min(jhi - jlo + 1, (int)d.W));
-  const int z0 = blockIdx.x * RB;
+  const int z0 = rf.rb * RB;
   const int64_t n = d.n_atoms;
 
   for (int i = threadIdx.x; i < 2 * g.n_elem; i += BLK) ftab[i] = g.f_table[i];
@@ -734,7 +753,7 @@ __global__ __launch_bounds__(BLK) void k_adj_rowgrad(gx_frames d, gx_adjoint g, 
   }
   __syncthreads();
 
-  rowgrad_gather<C>(d, g, fp, fb, z0, RB, N, rows, ftab, shd, shi, masked ? mzr : nullptr, ya, yb);
+  rowgrad_gather<C>(d, g, fp, fb, rf.rb, nrb, z0, RB, N, rows, ftab, shd, shi, masked ? mzr : nullptr, ya, yb);
 }
 
 // e_sum += the batch's per-frame, per-row-block dL/df sums, in two steps with fixed
@@ -841,10 +860,9 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_adj_rowpass_bs(gx_frames d, gx_a
       const double sc = ldexp(1.0, g.row_exp[v]);
       for (int a = a_lo + tid; a < a_hi; a += GX_BLK_FFT) {
         const bool own = d.azrow[a] == v;  // row v: weight 1 - tv; row v - 1: tv
-        long long u0;
-        double tu;
-        cic_u(d.ax[a], d.ay[a], s, c, ymin, r, rinv, u0, tu);
-        if (u0 < 0 || u0 + 1 >= N) {
+        double tu = 0.0;
+        const int u0 = cic_u(d.ax[a], d.ay[a], s, c, ymin, r, rinv, N, tu);
+        if (u0 < 0) {
           atomicAdd(&d.status[2], 1);
           continue;
         }
@@ -852,11 +870,11 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_adj_rowpass_bs(gx_frames d, gx_a
         const int e = g.aelem[a];
         const double fr = ftab[2 * e], fi = ftab[2 * e + 1];
         const double wv = own ? 1.0 - tv : tv;
-        const double w0 = wv * (1.0 - tu), w1 = wv * tu;
-        atomicAdd(&acc[u0], (AU)(AT)llrint((fr * w0) * sc));
-        atomicAdd(&acc[N + u0], (AU)(AT)llrint((fi * w0) * sc));
-        atomicAdd(&acc[u0 + 1], (AU)(AT)llrint((fr * w1) * sc));
-        atomicAdd(&acc[N + u0 + 1], (AU)(AT)llrint((fi * w1) * sc));
+        const double w0 = (wv * (1.0 - tu)) * sc, w1 = (wv * tu) * sc;  // f (w sc) = (f w) sc exactly
+        atomicAdd(&acc[u0], (AU)fx_round<AT>(fr * w0));
+        atomicAdd(&acc[N + u0], (AU)fx_round<AT>(fi * w0));
+        atomicAdd(&acc[u0 + 1], (AU)fx_round<AT>(fr * w1));
+        atomicAdd(&acc[N + u0 + 1], (AU)fx_round<AT>(fi * w1));
       }
       __syncthreads();
       const double inv = 1.0 / sc;
@@ -1165,7 +1183,7 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_adj_rowgrad_bs(gx_frames d, gx_a
       }
   }
   __syncthreads();
-  rowgrad_gather<C>(d, g, fp, fb, z0, RB, N, rows, ftab, shd, shi);
+  rowgrad_gather<C>(d, g, fp, fb, blockIdx.x, gridDim.x, z0, RB, N, rows, ftab, shd, shi);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1377,8 +1395,9 @@ int launch_adj_rowpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, in
                                150 * 1024));
     attr_set = true;
   }
-  dim3 grid((unsigned)((d->Z + RB - 1) / RB), (unsigned)nf);
-  k_adj_rowpass<N, C><<<grid, BLK, lds, st>>>(*d, *g, f0);
+  const int64_t nb = (d->Z + RB - 1) / RB * nf;
+  GX_REQUIRE(nb < (1ll << 31), "%lld row blocks x frames in one launch", (long long)nb);
+  k_adj_rowpass<N, C><<<(unsigned)nb, BLK, lds, st>>>(*d, *g, f0, nf);
   return check_launch("k_adj_rowpass");
 }
 
@@ -1403,8 +1422,9 @@ int launch_adj_rowgrad_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, in
                                150 * 1024));
     attr_set = true;
   }
-  dim3 grid((unsigned)((d->Z + RB - 1) / RB), (unsigned)nf);
-  k_adj_rowgrad<N, C><<<grid, BLK, lds, st>>>(*d, *g, f0);
+  const int64_t nb = (d->Z + RB - 1) / RB * nf;
+  GX_REQUIRE(nb < (1ll << 31), "%lld row blocks x frames in one launch", (long long)nb);
+  k_adj_rowgrad<N, C><<<(unsigned)nb, BLK, lds, st>>>(*d, *g, f0, nf);
   return check_launch("k_adj_rowgrad");
 }
 

@@ -530,6 +530,27 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
   return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 
+// (row block, frame) of block b of a launch over nrb row blocks x nf frames (a 1-D grid
+// of nrb nf blocks).  Each XCD takes one contiguous range of work items (xcd_tile),
+// numbered in tiles of GX_ROW_TILE neighbouring row blocks x all frames with the row
+// block fastest: the blocks resident on one XCD at a time are a few frames of those row
+// blocks, which share the rows' atoms and the frame-buffer lines of neighbouring rows
+// (a 128-byte line holds 16 rows of a z-contiguous column) in that XCD's L2.
+#ifndef GX_ROW_TILE
+#define GX_ROW_TILE 8
+#endif
+struct RowFrame {
+  int rb, fb;
+};
+__device__ __forceinline__ RowFrame row_frame_tile(int64_t b, int64_t nrb, int64_t nf) {
+  const int64_t item = xcd_tile(b, nrb * nf);
+  const int64_t span = (int64_t)GX_ROW_TILE * nf;
+  const int64_t t = item / span, rem = item - t * span;
+  const int64_t tw = min((int64_t)GX_ROW_TILE, nrb - t * GX_ROW_TILE);  // row blocks in this tile
+  const int64_t f = rem / tw;
+  return {(int)(t * GX_ROW_TILE + (rem - f * tw)), (int)f};
+}
+
 // ------------------------------------------------------------------------------------
 // rectangular_collapse_lengths (tools/voxelgrids.py:238-309) at one x value,
 // from the per-frame record prepared by the host planner (same branch order
