@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_void_p
 from .build import LIB
 
 GX_OK = 0
-GX_ABI_VERSION = 10  # structure layouts below match this version of the header
+GX_ABI_VERSION = 11  # structure layouts below match this version of the header
 GX_FP_STRIDE = 24
 GX_MOMENTS_WORK = 774
 GX_CROP_NOALLOC_MAX_ATOMS = 134217728
@@ -66,6 +66,7 @@ class GxFrames(ctypes.Structure):
         ("bs_rlo", c_int64), ("bs_rk", c_int64), ("bs_Mc", c_int64), ("bs_clo", c_int64), ("bs_ck", c_int64),
         ("bs_kernel_c", c_void_p), ("bs_twiddle_c", c_void_p),
         ("avg_q", c_void_p), ("inter_rm", c_int64), ("fft_plan_row", c_int64), ("fft_plan_col", c_int64),
+        ("f_band", c_void_p),
     ]
 
 

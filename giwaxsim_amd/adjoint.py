@@ -270,6 +270,9 @@ class DifferentiableSlab:
         """(Re)size the per-batch buffers for row range Z (with 16 rows of slack)."""
         N, W, n = self.plan.N, self.plan.W, self.n
         Zc = int(min(N, Z + 16))
+        # kept rows of the column transforms: the forward stores them for the backward of
+        # its last batch (power-of-two grids)
+        K = int(self.plan.i_hi - self.plan.i_lo + 1) if self.bs is None else 0
         nf = self.frames[1] - self.frames[0]
         esz = 16 if self.f64 else 8
         B = self._batch_arg
@@ -278,7 +281,7 @@ class DifferentiableSlab:
             # the free device memory (at most 64 GB, at least 2.5 GB), so that a slab's
             # frames fit in as few batches as possible (the backward reuses the
             # forward's last batch instead of recomputing it)
-            per_frame = W * Zc * esz + 4 * (esz // 2) * max(n, 1)
+            per_frame = W * Zc * esz + 4 * (esz // 2) * max(n, 1) + W * K * esz
             free, _ = torch.cuda.mem_get_info()
             budget = max(2.5e9, min(0.25 * free, 64e9))
             B = max(1, min(nf, int(budget // per_frame), 8192))
@@ -288,6 +291,7 @@ class DifferentiableSlab:
         self.Z_cap = Zc
         cdt = torch.float64 if self.f64 else torch.float32
         self.inter = _empty(self.batch * W * Zc * 2, cdt)
+        self.f_band = _empty(self.batch * W * K * 2, cdt) if K else None
         self.g_part = _empty(self.batch * 4 * n, torch.float64 if self.f64 else torch.float32)
         # per-element dL/df of each frame's row blocks (at most Zc blocks per frame)
         self.e_part = (_empty(self.batch * Zc * self.n_elem * 2, torch.float64)
@@ -378,7 +382,8 @@ class DifferentiableSlab:
             bs_M=0, n_frames=p.n_frames, fparams=ptr(self.fparams).value, batch=self.batch,
             ymin_key=ptr(self.ymin_key).value, idx_bounds=ptr(self.idx_bounds).value,
             inter=ptr(self.inter).value, bin_sum=ptr(self.bin_sum).value,
-            bin_cnt=ptr(self.bin_cnt).value, status=ptr(self.status).value)
+            bin_cnt=ptr(self.bin_cnt).value, status=ptr(self.status).value,
+            f_band=ptr(self.f_band).value if self.f_band is not None else None)
         if self.bs is not None:
             b = self.bs
             fr.bs_M, fr.bs_Mc = b['M'], b['Mc']

@@ -327,7 +327,7 @@ __global__ __launch_bounds__(BLK) void k_adj_rowpass(gx_frames d, gx_adjoint g, 
 // adjoint of the column pass (in place on d.inter)
 // ------------------------------------------------------------------------------------
 template <int N, typename C>
-__global__ __launch_bounds__(BLK) void k_adj_colpass(gx_frames d, gx_adjoint g, int64_t f0) {
+__global__ __launch_bounds__(BLK) void k_adj_colpass(gx_frames d, gx_adjoint g, int64_t f0, bool stored) {
   using T = typename Real<C>::T;
   constexpr int P = N / GX_WAVE;
   constexpr int R0 = FftFirst<N>::R, RL = FftLast<N>::R;
@@ -360,25 +360,32 @@ __global__ __launch_bounds__(BLK) void k_adj_colpass(gx_frames d, gx_adjoint g, 
   if (!d.fill_bkg && d.smooth) u = reinterpret_cast<const C *>(d.col_spec)[(int64_t)fb * d.W + wcol];
   const double *sz_row = d.col_sz + (int64_t)fb * N;
   C v[P];
-#pragma unroll
-  for (int t = 0; t < P / R0; ++t)
-#pragma unroll
-    for (int q = 0; q < R0; ++q) v[t * R0 + q] = col_input(d, (const C *)col, lane + GX_WAVE * t + q * (N / R0), k0, u, sz_row);
   const C *tw = reinterpret_cast<const C *>(d.twiddle);
   C *buf = lds[wave];
-  fft_wave<N, C>(v, buf, tw, lane);
-  wave_sync();
-  // conj of the cotangent dL/dF = 2 (dL/dI) F at its natural index k; with fill or
-  // smoothing also this column's share sum_k dL/dF_k conj(Pz^[k]) conj(Py^[j]) of dL/d avg
-  // F at its natural index k into LDS, then, per k (rolled loop, each lane its own k):
-  // conj of the cotangent dL/dF = 2 (dL/dI) F on binned pixels, and with fill or
-  // smoothing this column's share sum_k dL/dF_k conj(Pz^[k]) conj(Py^[j]) of dL/d avg
-  // (+ max_voxels N^2 dL/dF_0 on the k = 0 column, the pattern's constant)
+  // F at its natural index k into LDS: the kept rows i_lo..i_hi (the only ones read
+  // below) from the forward's stored band, or the column's transform recomputed (plan 0,
+  // as the forward's column pass with fft_plan_col = 0: bitwise the same values)
+  if (stored) {
+    const C *fsrc = reinterpret_cast<const C *>(d.f_band) + ((int64_t)fb * d.W + wcol) * (d.i_hi - d.i_lo + 1);
+    for (int i = (int)d.i_lo + lane; i <= (int)d.i_hi; i += GX_WAVE) buf[(i + N / 2) & (N - 1)] = fsrc[i - d.i_lo];
+  } else {
 #pragma unroll
-  for (int t = 0; t < P / RL; ++t)
+    for (int t = 0; t < P / R0; ++t)
 #pragma unroll
-    for (int q = 0; q < RL; ++q) buf[lane + GX_WAVE * t + q * (N / RL)] = v[t * RL + q];
+      for (int q = 0; q < R0; ++q)
+        v[t * R0 + q] = col_input(d, (const C *)col, lane + GX_WAVE * t + q * (N / R0), k0, u, sz_row);
+    fft_wave<N, C>(v, buf, tw, lane);
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < P / RL; ++t)
+#pragma unroll
+      for (int q = 0; q < RL; ++q) buf[lane + GX_WAVE * t + q * (N / RL)] = v[t * RL + q];
+  }
   wave_sync();
+  // per k (rolled loop, each lane its own k): conj of the cotangent dL/dF = 2 (dL/dI) F on
+  // binned pixels, and with fill or smoothing this column's share
+  // sum_k dL/dF_k conj(Pz^[k]) conj(Py^[j]) of dL/d avg (+ max_voxels N^2 dL/dF_0 on the
+  // k = 0 column, the pattern's constant)
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;
   const double val = d.q_val;
   const double2 *zhat = reinterpret_cast<const double2 *>(g.avg_spec) + (int64_t)fb * (N + d.W);
@@ -1402,9 +1409,10 @@ int launch_adj_rowpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, in
 }
 
 template <int N, typename C>
-int launch_adj_colpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
+int launch_adj_colpass_t(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st,
+                         bool stored) {
   dim3 grid(grid1d(d->W, NWAVE), (unsigned)nf);
-  k_adj_colpass<N, C><<<grid, BLK, 0, st>>>(*d, *g, f0);
+  k_adj_colpass<N, C><<<grid, BLK, 0, st>>>(*d, *g, f0, stored);
   return check_launch("k_adj_colpass");
 }
 
@@ -1441,7 +1449,9 @@ int rowgrad_blocks(const gx_frames *d) {
   return (int)((d->Z + rb - 1) / rb);
 }
 
-enum Op { OP_ROWPASS, OP_COLPASS, OP_ROWGRAD, OP_AVGSPEC };
+// OP_COLPASS_STORED: the column adjoint on the kept band the forward's column pass
+// stored (gx_frames.f_band) instead of recomputing each column's transform
+enum Op { OP_ROWPASS, OP_COLPASS, OP_COLPASS_STORED, OP_ROWGRAD, OP_AVGSPEC };
 
 template <int N>
 int dispatch_n(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
@@ -1451,11 +1461,13 @@ int dispatch_n(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64
   }
   if (d->fft_f64) {
     if (op == OP_ROWPASS) return launch_adj_rowpass_t<N, double2>(d, g, f0, nf, st);
-    if (op == OP_COLPASS) return launch_adj_colpass_t<N, double2>(d, g, f0, nf, st);
+    if (op == OP_COLPASS || op == OP_COLPASS_STORED)
+      return launch_adj_colpass_t<N, double2>(d, g, f0, nf, st, op == OP_COLPASS_STORED);
     return launch_adj_rowgrad_t<N, double2>(d, g, f0, nf, st);
   }
   if (op == OP_ROWPASS) return launch_adj_rowpass_t<N, float2>(d, g, f0, nf, st);
-  if (op == OP_COLPASS) return launch_adj_colpass_t<N, float2>(d, g, f0, nf, st);
+  if (op == OP_COLPASS || op == OP_COLPASS_STORED)
+    return launch_adj_colpass_t<N, float2>(d, g, f0, nf, st, op == OP_COLPASS_STORED);
   return launch_adj_rowgrad_t<N, float2>(d, g, f0, nf, st);
 }
 
@@ -1552,6 +1564,10 @@ int check_adj(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, b
   GX_REQUIRE(!dense || (d->col_spec && d->twiddle64), "fill_bkg / smooth need col_spec and twiddle64");
   GX_REQUIRE(d->bs_M == 0 || (d->N != 256 && d->N != 512 && d->N != 1024 && d->N != 2048),
              "differentiable path: chirp-z tables given for a power-of-two grid");
+  GX_REQUIRE(!d->inter_rm, "differentiable path: the intermediate is [batch][W][Z] (inter_rm = 0)");
+  GX_REQUIRE(!d->f_band || (d->bs_M == 0 && d->fft_plan_col == 0),
+             "differentiable path: f_band needs a power-of-two grid and fft_plan_col = 0 (the column "
+             "adjoint recomputes the other batches' transforms with that plan)");
   GX_REQUIRE(d->r > 0.0, "r_voxel_size must be positive");
   GX_REQUIRE(nf >= 0 && nf <= d->batch, "batch of %lld frames exceeds workspace batch %lld", (long long)nf,
              (long long)d->batch);
@@ -1663,7 +1679,9 @@ static int adj_backward(const gx_frames *d, const gx_adjoint *g, int64_t f0, int
     if ((rc = dispatch(OP_AVGSPEC, d, g, f0, nf, st))) return rc;
   }
   if (recompute && (rc = dispatch(OP_ROWPASS, d, g, f0, nf, st))) return rc;
-  if ((rc = dispatch(OP_COLPASS, d, g, f0, nf, st))) return rc;
+  // the batch the forward left also has its columns' kept band stored (when f_band is given)
+  const Op col = (!recompute && d->f_band) ? OP_COLPASS_STORED : OP_COLPASS;
+  if ((rc = dispatch(col, d, g, f0, nf, st))) return rc;
   if (dense) {
     k_adj_avg_frames<<<(unsigned)nf, BLK, 0, st>>>(*d, *g, f0);
     k_adj_avg_reduce<<<1, BLK, 0, st>>>(*d, *g, nf);

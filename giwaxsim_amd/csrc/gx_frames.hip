@@ -1283,6 +1283,9 @@ __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
 
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;
   const double val = d.q_val;
+  // the kept band of this column's transform for the differentiable backward
+  C *fband = d.f_band ? reinterpret_cast<C *>(d.f_band) + ((int64_t)fb * d.W + wcol) * (d.i_hi - d.i_lo + 1)
+                      : nullptr;
 #pragma unroll
   for (int t = 0; t < P / RL; ++t) {
 #pragma unroll
@@ -1290,6 +1293,7 @@ __global__ __launch_bounds__(BLK) void k_colpass(gx_frames d, int64_t f0) {
       const int k = lane + GX_WAVE * t + q * (N / RL);
       const int i = (k + N / 2) & (N - 1);
       if (i < d.i_lo || i > d.i_hi) continue;
+      if (fband) fband[i - d.i_lo] = v[t * RL + q];
       // det_v_qz[i] = fftshift(fftfreq(N, r) * 2 * pi)[i]
       const int kint = (k < N / 2) ? k : k - N;
       const double qz = (((double)kint * val) * 2.0) * M_PI;
@@ -1828,6 +1832,8 @@ int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
   GX_REQUIRE(!d->inter_rm || ((d->N == 512 || d->N == 1024 || d->N == 2048) && (d->row_variant & 3) == 0),
              "the row-major intermediate needs N = 512, 1024 or 2048 and the per-wave row shape "
              "(N=%lld, row_variant=%lld)", (long long)d->N, (long long)d->row_variant);
+  GX_REQUIRE(!d->f_band || (!d->bs_M && !d->inter_rm),
+             "f_band (the stored kept band) needs a power-of-two N and inter_rm = 0");
   GX_REQUIRE(d->fparams && d->ymin_key && d->idx_bounds && d->inter && d->bin_sum && d->bin_cnt && d->twiddle &&
                  d->status && d->row_off && d->row_exp,
              "a required device buffer of gx_frames is NULL");

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GX_ABI_VERSION 10
+#define GX_ABI_VERSION 11
 
 #define GX_OK 0
 #define GX_ERR_ARG 1         /* invalid argument (message says which) */
@@ -193,6 +193,11 @@ typedef struct gx_frames {
   int64_t fft_plan_col;    /* 1 = 16.8.8, 2 = 8.16.8, 3 = 8.8.16; N = 2048: 0 = 16.16.8, 1 = 16.8.16,
                               2 = 8.16.16; other N and chirp-z grids: 0 only); the bins agree to
                               rounding across plans */
+  void *f_band;            /* NULL, or [batch][W][i_hi - i_lo + 1] float2 / double2: the column
+                              pass also stores each binned column's transform at the kept rows
+                              i_lo..i_hi (natural order), which the differentiable backward of
+                              the same batch reads back (gx_adj_backward_stored).  Power-of-two
+                              N with inter_rm = 0 only */
 } gx_frames;
 
 /* ------------------------------------------------------------------------------------
