@@ -42,6 +42,13 @@ __device__ __forceinline__ double floor_div(double a, double b, double binv) {
 // numpy `.astype(int)` of a floor-div result (already integral): truncation.
 __device__ __forceinline__ long long to_int(double q) { return (long long)q; }
 
+// to_int(floor_div(a, b)) as an index when it lies in [lo, hi) (lo >= 0), else -1: the
+// range test on the integral quotient, then one int32 conversion (exact in that range)
+__device__ __forceinline__ int floor_div_in(double a, double b, double binv, int lo, int hi) {
+  const double q = floor_div(a, b, binv);
+  return (q >= (double)lo && q < (double)hi) ? (int)q : -1;
+}
+
 // ------------------------------------------------------------------------------------
 // order-preserving uint64 keys for doubles (for atomicMin/atomicMax)
 // ------------------------------------------------------------------------------------

@@ -793,8 +793,8 @@ __global__ __launch_bounds__(BLK) void k_rowpass(gx_frames d, int64_t f0, int64_
         for (int u = 0; u < SCATTER_UNROLL; ++u) {
           if (a + u * GX_WAVE >= a1) break;
           const double yy = (xs[u] * s + ys[u] * c) - ymin;
-          const long long yi = to_int(floor_div(yy, r, rinv));
-          if (yi >= ya && yi < yb) {
+          const int yi = floor_div_in(yy, r, rinv, ya, yb);
+          if (yi >= 0) {
             atomicAdd(&acc_re[yi], qs[u].x);
             atomicAdd(&acc_im[yi], qs[u].y);
           }
@@ -899,8 +899,8 @@ __global__ __launch_bounds__(BLK) void k_rowpass_blk(gx_frames d, int64_t f0, in
     const int v = d.azrow[a];
     if (d.fill_bkg && !(v >= zlo && v < zhi)) continue;
     const double yy = (d.ax[a] * s + d.ay[a] * c) - ymin;
-    const long long yi = to_int(floor_div(yy, r, rinv));
-    if (yi >= ya && yi < yb) {
+    const int yi = floor_div_in(yy, r, rinv, ya, yb);
+    if (yi >= 0) {
       int32_t *acc = reinterpret_cast<int32_t *>(smem_raw + (size_t)(v - z0) * ROWB);
       const int2 q = reinterpret_cast<const int2 *>(d.aq)[a];
       atomicAdd(&acc[yi], q.x);
@@ -1057,8 +1057,8 @@ void k_rowpass_rm(gx_frames d, int64_t f0, int64_t nf) {
       for (int u = 0; u < SCATTER_UNROLL; ++u) {
         if (a + u * GX_WAVE >= a1) break;
         const double yy = (xs[u] * s + ys[u] * c) - ymin;
-        const long long yi = to_int(floor_div(yy, r, rinv));
-        if (yi >= ya && yi < yb) {
+        const int yi = floor_div_in(yy, r, rinv, ya, yb);
+        if (yi >= 0) {
           atomicAdd(&acc_re[yi], qs[u].x);
           atomicAdd(&acc_im[yi], qs[u].y);
         }
@@ -1438,8 +1438,8 @@ void k_rowpass_bs(gx_frames d, int64_t f0) {
       __syncthreads();
       for (int a = a0 + tid; a < a1; a += GX_BLK_FFT) {
         const double yy = (d.ax[a] * s + d.ay[a] * c) - ymin;
-        const long long yi = to_int(floor_div(yy, r, rinv));
-        if (yi >= ya && yi < yb) {
+        const int yi = floor_div_in(yy, r, rinv, ya, yb);
+        if (yi >= 0) {
           atomicAdd(&acc_re[yi], d.aq[2 * (int64_t)a]);
           atomicAdd(&acc_im[yi], d.aq[2 * (int64_t)a + 1]);
         }
