@@ -14,7 +14,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_void_p
 from .build import LIB
 
 GX_OK = 0
-GX_ABI_VERSION = 11  # structure layouts below match this version of the header
+GX_ABI_VERSION = 12  # structure layouts below match this version of the header
 GX_FP_STRIDE = 24
 GX_MOMENTS_WORK = 774
 GX_CROP_NOALLOC_MAX_ATOMS = 134217728
@@ -99,6 +99,28 @@ class GxAdjoint(ctypes.Structure):
     ]
 
 
+# gx_workspace_query / gx_workspace_sizes: every field int64, in the header's order
+WORKSPACE_QUERY_FIELDS = (
+    "N", "n_atoms", "n_elem", "n_frames", "batch", "W", "Z", "c_n", "i_lo", "i_hi", "fill_bkg", "smooth",
+    "fft_f64", "bs_M", "bs_Mc", "store_band", "differentiable", "n_pix", "n_init", "n_orient")
+WORKSPACE_SIZE_FIELDS = (
+    "sx", "sy", "sq", "szrow", "row_off", "row_exp", "work", "szf", "sperm", "selem", "stable_work",
+    "fparams", "twiddle", "twiddle64", "gauss_cw", "bs_chirp", "bs_kernel", "bs_twiddle", "bs_kernel_c",
+    "bs_twiddle_c", "ymin_key", "idx_bounds", "col_nm", "col_sy", "col_sz", "inter", "col_spec", "f_band",
+    "avg_q", "bin_sum", "bin_cnt", "status", "iq", "h_axis", "v_axis", "init_R", "orient_R", "orient_w", "det",
+    "det_out", "adj_row_exp", "ainv", "g_bins", "g_part", "red", "amin", "g_atom", "avg_spec", "avg_part",
+    "g_avg", "e_part", "e_sum", "bs_kernel_adj", "bs_kernel_adj_c", "g_coords", "g_f", "det_work", "g_fx",
+    "fx_scale", "map_counts", "map_scan_work", "map_voxel_off", "map_run_bytes", "map_runs_max", "total")
+
+
+class GxWorkspaceQuery(ctypes.Structure):
+    _fields_ = [(f, c_int64) for f in WORKSPACE_QUERY_FIELDS]
+
+
+class GxWorkspaceSizes(ctypes.Structure):
+    _fields_ = [(f, c_int64) for f in WORKSPACE_SIZE_FIELDS]
+
+
 _P = c_void_p
 _D = c_double
 _I = c_int64
@@ -145,6 +167,7 @@ SIGNATURES = {
     "gx_detector_map_count": [POINTER(GxDetector), _P, _P, _P, _P],
     "gx_detector_map_fill": [POINTER(GxDetector), _P, _P, _P, _P, _P, _I, _P],
     "gx_detector_adjoint_mapped": [POINTER(GxDetector), _P, _I, _P, _P, _P, _P, _I, _P, _P, _P],
+    "gx_workspace_size": [POINTER(GxWorkspaceQuery), POINTER(GxWorkspaceSizes)],
 }
 EXPORTS = ["gx_last_error", "gx_abi_version", "gx_supported_grid"] + list(SIGNATURES)
 
@@ -186,6 +209,19 @@ def call(name: str, *args):
         msg = lib.gx_last_error().decode(errors="replace")
         raise GxError(f"{name} failed ({rc}): {msg}")
     return rc
+
+
+def workspace_size(**query):
+    """gx_workspace_size for a geometry given as keyword arguments (the field names of
+    gx_workspace_query; missing fields are 0): {buffer name: bytes}.  Host-only: needs no
+    device."""
+    unknown = sorted(set(query) - set(WORKSPACE_QUERY_FIELDS))
+    if unknown:
+        raise ValueError(f"unknown gx_workspace_query fields {unknown}")
+    q = GxWorkspaceQuery(**{k: int(v) for k, v in query.items()})
+    out = GxWorkspaceSizes()
+    call("gx_workspace_size", ctypes.byref(q), ctypes.byref(out))
+    return {f: int(getattr(out, f)) for f in WORKSPACE_SIZE_FIELDS}
 
 
 def dbl_array(values):

@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgiwaxsim_amd.so")
-SOURCES = ["gx_frames.hip", "gx_detector.hip", "gx_adjoint.hip", "gx_compare.hip", "gx_slab_build.hip"]
+SOURCES = ["gx_frames.hip", "gx_detector.hip", "gx_adjoint.hip", "gx_compare.hip", "gx_slab_build.hip",
+           "gx_workspace.hip"]
 ARCH = os.environ.get("GX_OFFLOAD_ARCH", "gfx950")
 
 

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GX_ABI_VERSION 11
+#define GX_ABI_VERSION 12
 
 #define GX_OK 0
 #define GX_ERR_ARG 1         /* invalid argument (message says which) */
@@ -542,6 +542,64 @@ int gx_detector_map_fill(const gx_detector *d, const int64_t *voxel_off, int32_t
 int gx_detector_adjoint_mapped(const gx_detector *d, const double *g_out, int64_t flags, double *work,
                                const int32_t *run_voxel, const uint32_t *runs, const uint8_t *run_len, int64_t n_runs,
                                unsigned long long *g_fx, double *fx_scale, gx_stream_t stream);
+
+/* ------------------------------------------------------------------------------------
+ * Workspace sizes (no reference counterpart: the reference allocates inside each call;
+ * for callers that allocate the device memory themselves, e.g. a C, Go or Rust binding
+ * or a loop under graph capture).  The library allocates nothing on the paths above:
+ * every device buffer comes from the caller through the descriptors.  Fill a query with
+ * the geometry; gx_workspace_size writes the byte size of each buffer, named after the
+ * descriptor field that points to it (0: not needed for this geometry, the field may
+ * be NULL).  Per-frame buffers hold `batch` frames (a caller that keeps the statistics
+ * of all frames, as the package's engine does, sizes those with batch = n_frames and
+ * offsets the descriptor pointers per batch).
+ * ---------------------------------------------------------------------------------- */
+typedef struct gx_workspace_query {
+  int64_t N;               /* real-space grid edge (16..2048) */
+  int64_t n_atoms;
+  int64_t n_elem;          /* entries of f_table */
+  int64_t n_frames;        /* phi-frames (fparams) */
+  int64_t batch;           /* frames per call (gx_frames.batch) */
+  int64_t W, Z;            /* intermediate width and row range (gx_frames.W, .Z) */
+  int64_t c_n;             /* cropped grid edge (bins [c][c][c]) */
+  int64_t i_lo, i_hi;      /* kept rows of the column transforms (gx_frames.i_lo, .i_hi) */
+  int64_t fill_bkg, smooth; /* as gx_frames (smooth: sigma in voxels) */
+  int64_t fft_f64;         /* 0: float2 transforms, 1: double2 */
+  int64_t bs_M, bs_Mc;     /* chirp-z transform lengths (0 for power-of-two N) */
+  int64_t store_band;      /* 1: gx_frames.f_band is kept (differentiable path) */
+  int64_t differentiable;  /* 1: also the slab's optional outputs and gx_adjoint's buffers */
+  int64_t n_pix;           /* detector pixels per edge (0: no detector) */
+  int64_t n_init, n_orient; /* detector initial rotations (0..3) and orientations */
+} gx_workspace_query;
+
+typedef struct gx_workspace_sizes {
+  /* gx_slab outputs */
+  int64_t sx, sy, sq, szrow, row_off, row_exp, work;
+  int64_t szf, sperm, selem, stable_work;           /* differentiable only */
+  /* gx_frames */
+  int64_t fparams, twiddle, twiddle64, gauss_cw;
+  int64_t bs_chirp, bs_kernel, bs_twiddle, bs_kernel_c, bs_twiddle_c;
+  int64_t ymin_key, idx_bounds, col_nm, col_sy, col_sz, inter, col_spec, f_band, avg_q;
+  int64_t bin_sum, bin_cnt, status;
+  int64_t iq;              /* gx_bins_finalize's FP64 I(q) [c][c][c] (the detector's input) */
+  /* gx_detector and gx_detector_finalize */
+  int64_t h_axis, v_axis, init_R, orient_R, orient_w, det, det_out;
+  /* gx_adjoint (differentiable only) */
+  int64_t adj_row_exp, ainv, g_bins, g_part, red, amin, g_atom, avg_spec, avg_part, g_avg;
+  int64_t e_part, e_sum, bs_kernel_adj, bs_kernel_adj_c;
+  int64_t g_coords, g_f;   /* gx_adj_finish outputs */
+  /* detector adjoint (differentiable with a detector): work of gx_detector_adjoint_mapped
+     (that of gx_detector_adjoint is smaller), one copy of g_fx, fx_scale, and the
+     transpose map's count step; its run arrays take map_run_bytes per run for the
+     voxel_off[nv] runs the count reports, at most map_runs_max */
+  int64_t det_work, g_fx, fx_scale, map_counts, map_scan_work, map_voxel_off;
+  int64_t map_run_bytes, map_runs_max;
+  int64_t total;           /* sum of every size above except map_run_bytes and map_runs_max */
+} gx_workspace_sizes;
+
+/* Byte sizes for the query's geometry; GX_ERR_ARG for an invalid query (message says
+ * which field).  Host-only arithmetic: no device call. */
+int gx_workspace_size(const gx_workspace_query *q, gx_workspace_sizes *out);
 
 #ifdef __cplusplus
 }

@@ -33,7 +33,7 @@ def test_exports_every_declared_symbol():
 
 def test_no_gpu_calls_needed_for_metadata():
     lib = _lib.load()
-    assert lib.gx_abi_version() == _lib.GX_ABI_VERSION == 11
+    assert lib.gx_abi_version() == _lib.GX_ABI_VERSION == 12
     assert [lib.gx_supported_grid(n) for n in (8, 128, 256, 1048, 2048, 2049, 4096)] == \
         [0, 1, 1, 1, 1, 0, 0]
 
