@@ -23,22 +23,24 @@ extern "C" int gx_workspace_size(const gx_workspace_query *q, gx_workspace_sizes
   GX_REQUIRE(p2 ? (q->bs_M == 0 && q->bs_Mc == 0) : (chirp_len(q->bs_M) && chirp_len(q->bs_Mc)),
              "N=%lld needs bs_M = bs_Mc = 0 (powers of two 256..2048) or chirp-z lengths 1024/2048/4096 "
              "(got %lld, %lld)", (long long)N, (long long)q->bs_M, (long long)q->bs_Mc);
-  GX_REQUIRE(n >= 0, "n_atoms=%lld", (long long)n);
-  GX_REQUIRE(ne >= 1, "n_elem=%lld", (long long)ne);
-  GX_REQUIRE(q->n_frames >= 1 && B >= 1, "n_frames=%lld, batch=%lld", (long long)q->n_frames, (long long)B);
+  // caps that keep every size below 2^62 bytes (no int64 overflow)
+  GX_REQUIRE(n >= 0 && n <= (1ll << 36), "n_atoms=%lld outside [0, 2^36]", (long long)n);
+  GX_REQUIRE(ne >= 1 && ne <= (1 << 16), "n_elem=%lld outside [1, 65536]", (long long)ne);
+  GX_REQUIRE(q->n_frames >= 1 && q->n_frames <= (1ll << 31) && B >= 1 && B <= (1 << 20),
+             "n_frames=%lld (1..2^31), batch=%lld (1..2^20)", (long long)q->n_frames, (long long)B);
   GX_REQUIRE(q->W >= 1 && q->W <= N, "W=%lld outside [1, N]", (long long)q->W);
   GX_REQUIRE(q->Z >= 8 && q->Z <= N && q->Z % 8 == 0, "Z=%lld must be a multiple of 8 in [8, N]",
              (long long)q->Z);
-  GX_REQUIRE(q->c_n >= 1, "c_n=%lld", (long long)q->c_n);
+  GX_REQUIRE(q->c_n >= 1 && q->c_n <= 4 * N, "c_n=%lld outside [1, 4 N]", (long long)q->c_n);
   GX_REQUIRE(q->fill_bkg == 0 || q->fill_bkg == 1, "fill_bkg=%lld", (long long)q->fill_bkg);
-  GX_REQUIRE(q->smooth >= 0, "smooth=%lld", (long long)q->smooth);
+  GX_REQUIRE(q->smooth >= 0 && q->smooth <= N, "smooth=%lld outside [0, N]", (long long)q->smooth);
   GX_REQUIRE(q->fft_f64 == 0 || q->fft_f64 == 1, "fft_f64=%lld", (long long)q->fft_f64);
   GX_REQUIRE(!q->store_band || (p2 && q->i_lo >= 0 && q->i_lo <= q->i_hi && q->i_hi < N),
              "store_band needs a power-of-two N and 0 <= i_lo <= i_hi < N (got %lld..%lld)", (long long)q->i_lo,
              (long long)q->i_hi);
   GX_REQUIRE(q->n_pix >= 0 && q->n_pix <= 65536, "n_pix=%lld", (long long)q->n_pix);
-  GX_REQUIRE(q->n_pix == 0 || (q->n_init >= 0 && q->n_init <= 3 && q->n_orient >= 1),
-             "detector: n_init=%lld (0..3), n_orient=%lld (>= 1)", (long long)q->n_init, (long long)q->n_orient);
+  GX_REQUIRE(q->n_pix == 0 || (q->n_init >= 0 && q->n_init <= 3 && q->n_orient >= 1 && q->n_orient <= (1 << 24)),
+             "detector: n_init=%lld (0..3), n_orient=%lld (1..2^24)", (long long)q->n_init, (long long)q->n_orient);
 
   gx_workspace_sizes s = {};
   const int64_t esz = q->fft_f64 ? 16 : 8;   // one complex transform value
@@ -144,7 +146,8 @@ extern "C" int gx_workspace_size(const gx_workspace_query *q, gx_workspace_sizes
       s.det, s.det_out, s.adj_row_exp, s.ainv, s.g_bins, s.g_part, s.red, s.amin, s.g_atom, s.avg_spec,
       s.avg_part, s.g_avg, s.e_part, s.e_sum, s.bs_kernel_adj, s.bs_kernel_adj_c, s.g_coords, s.g_f,
       s.det_work, s.g_fx, s.fx_scale, s.map_counts, s.map_scan_work, s.map_voxel_off};
-  for (int64_t v : summed) s.total += v;
+  for (int64_t v : summed)
+    GX_REQUIRE(!__builtin_add_overflow(s.total, v, &s.total), "the total workspace exceeds 2^63 bytes");
   *out = s;
   return GX_OK;
 }
