@@ -178,9 +178,12 @@ __global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
 
 // The same sum for detectors with few pixels and many orientations (a fit's 250^2 x 16k):
 // one pixel per lane of 64, COOP_NW waves per block.  The waves gather chunks of
-// COOP_K orientations each into LDS (v * w, the same product), then wave 0 adds the
-// chunk in orientation order, so the result is bitwise the one-thread-per-pixel sum
-// while COOP_NW times more gathers are in flight per pixel.  An orientation is the same
+// COOP_K orientations each into one of two LDS buffers (v * w, the same product); after a
+// barrier wave 0 adds the chunk in orientation order while the other waves fill the other
+// buffer with the next chunk (one barrier per chunk), so the result is bitwise the
+// one-thread-per-pixel sum while COOP_NW times more gathers are in flight per pixel
+// (4 orientations per wave and chunk, double-buffered: 4.62 -> 4.48 ms per image at the
+// fit geometry; 8 per wave single-buffered was the previous form, 8 double-buffered 6.5 ms).  An orientation is the same
 // for the whole wave, so its matrices and weight are scalar loads and their tests scalar
 // compares of bit patterns: rotations that are identities are skipped, and the
 // psi-rotated point is kept from the wave's previous orientation when that one has the
@@ -188,7 +191,8 @@ __global__ __launch_bounds__(BLK) void k_detector(gx_detector d) {
 // are 32-bit (the launch checks cy cx cz < 2^31) and the floor division takes
 // floor_div_q's one-multiply path.
 constexpr int COOP_NW = 8;                  // waves per block
-constexpr int COOP_K = 8;                   // orientations per wave and chunk
+constexpr int COOP_K = 4;                   // orientations per wave and chunk
+constexpr int COOP_MIN_ORIENT = 64;         // fewer orientations: the per-pixel kernel
 constexpr int64_t COOP_MAX_PIX = 1 << 17;   // above this the per-pixel kernel has the occupancy
 // A wave's 64 pixels are a 16 x 4 tile: at the fit geometry (250^2 px, 16k orientations) it read
 // fewer distinct cache lines than a 64-pixel row (4.8 vs 5.1 ms), 8 x 8 tied the row, and taller
@@ -199,7 +203,7 @@ template <typename T>
 __global__ __launch_bounds__(COOP_NW *GX_WAVE) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_detector_coop(gx_detector d) {
   extern __shared__ __align__(16) unsigned char coop_smem[];
-  double *prod = reinterpret_cast<double *>(coop_smem);  // [COOP_NW * COOP_K][64]
+  double *prod = reinterpret_cast<double *>(coop_smem);  // [2][COOP_NW * COOP_K][64]
   const double *__restrict__ oR = d.orient_R;
   const double *__restrict__ ow = d.orient_w;
   const T *__restrict__ iq = reinterpret_cast<const T *>(d.iq);
@@ -222,8 +226,10 @@ void k_detector_coop(gx_detector d) {
   constexpr int CH = COOP_NW * COOP_K;
   double acc = 0.0;
   double xs = 0.0, ys = 0.0, zs = 0.0;  // pixel after the psi rotation of the previous orientation
-  for (int64_t o0 = 0; o0 < n_orient; o0 += CH) {
-#pragma unroll 2  // two orientations' gathers in flight per wave; 49 VGPRs
+  int64_t chunk = 0;
+  for (int64_t o0 = 0; o0 < n_orient; o0 += CH, ++chunk) {
+    double *pb = prod + (chunk & 1) * (CH * GX_WAVE);
+#pragma unroll 2  // two orientations' gathers in flight per wave
     for (int k = 0; k < COOP_K; ++k) {
       const int j = wave * COOP_K + k;
       const int64_t o = o0 + j;
@@ -244,14 +250,15 @@ void k_detector_coop(gx_detector d) {
         const int zi = clip_q(floor_div_q(z - d.qz_min, d.dq, dqinv), hz);
         if (valid) pr = (double)iq[(yi * cx + xi) * cz + zi] * ow[o];
       }
-      prod[j * GX_WAVE + lane] = pr;
+      pb[j * GX_WAVE + lane] = pr;
     }
     __syncthreads();
     if (wave == 0) {
       const int nj = (int)min((int64_t)CH, n_orient - o0);
-      for (int j = 0; j < nj; ++j) acc = acc + prod[j * GX_WAVE + lane];
+      for (int j = 0; j < nj; ++j) acc = acc + pb[j * GX_WAVE + lane];
     }
-    __syncthreads();
+    // no second barrier: the next chunk fills the other buffer, and this one is refilled
+    // only after the next barrier, which wave 0 reaches after summing it
   }
   if (wave == 0 && valid) det[p] = acc;
 }
@@ -864,10 +871,10 @@ int gx_detector_accumulate(const gx_detector *d, gx_stream_t stream) {
                  (d->n_orient == 0 || (d->orient_R && d->orient_w)),
              "a detector buffer is NULL");
   const int64_t npx = d->n_pix * d->n_pix;
-  if (npx <= COOP_MAX_PIX && d->n_orient >= COOP_NW * COOP_K && d->cy * d->cx * d->cz < (1ll << 31)) {
+  if (npx <= COOP_MAX_PIX && d->n_orient >= COOP_MIN_ORIENT && d->cy * d->cx * d->cz < (1ll << 31)) {
     // few pixels, many orientations: COOP_NW waves gather for the same 64 pixels
     const unsigned nb = (unsigned)(((d->n_pix + COOP_TW - 1) / COOP_TW) * ((d->n_pix + COOP_TH - 1) / COOP_TH));
-    const size_t lds = (size_t)COOP_NW * COOP_K * GX_WAVE * sizeof(double);
+    const size_t lds = (size_t)2 * COOP_NW * COOP_K * GX_WAVE * sizeof(double);
     if (d->iq_f64)
       k_detector_coop<double><<<nb, COOP_NW * GX_WAVE, lds, as_stream(stream)>>>(*d);
     else
