@@ -2,9 +2,9 @@
 against the FP64 oracle.  Fed the same FP64 I(q) grid, the GPU image is
 bit-identical: same rotation matrices, the same FP64 operations per pixel
 (no FMA), exact floor-division indices and the same (psi, phi, theta)
-summation order.  The bit-exact cases (<= 65^2 pixels, 222 orientations: three full
-64-orientation chunks and a partial one) run the cooperative gather used for small
-detectors with many orientations; the weights case (16 orientations) and the golden
+summation order.  The bit-exact cases (<= 65^2 pixels, 222 orientations: six full
+32-orientation chunks, in both of its LDS buffers, and a partial one) run the cooperative
+gather used for small detectors with many orientations; the weights case (16 orientations) and the golden
 500^2 image run the one-thread-per-pixel kernel.  A lattice case puts pixel values where the
 one-multiply floor is off by one, for both kernels."""
 
