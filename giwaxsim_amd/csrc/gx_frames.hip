@@ -34,39 +34,70 @@ __global__ void k_slab_bounds_init(unsigned long long *keys, unsigned long long 
 
 constexpr int MAX_ELEM = 128;
 
+// Each thread takes PT atoms per grid stride, all loads issued before the first use;
+// the block reduces its extremes in LDS and adds one atomic per key.
+template <int PT>
 __global__ __launch_bounds__(BLK) void k_slab_bounds(const double *__restrict__ x, const double *__restrict__ y,
                                                      const double *__restrict__ z, const int32_t *__restrict__ el,
                                                      int64_t n, int64_t n_elem, unsigned long long *keys,
                                                      unsigned long long *counts) {
+  constexpr int NW = BLK / GX_WAVE;
   __shared__ unsigned int hist[MAX_ELEM];
+  __shared__ double ext[NW][6];
   for (int e = threadIdx.x; e < MAX_ELEM; e += BLK) hist[e] = 0;
   __syncthreads();
   double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t a = blockIdx.x * (int64_t)BLK + threadIdx.x; a < n; a += (int64_t)gridDim.x * BLK) {
-    double v[3] = {x[a], y[a], z[a]};
+  for (int64_t a0 = blockIdx.x * (int64_t)(BLK * PT) + threadIdx.x; a0 < n; a0 += (int64_t)gridDim.x * (BLK * PT)) {
+    double v[PT][3];
+    int e[PT];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      mn[k] = fmin(mn[k], v[k]);
-      mx[k] = fmax(mx[k], v[k]);
+    for (int k = 0; k < PT; ++k) {
+      const int64_t a = a0 + (int64_t)k * BLK;
+      v[k][0] = v[k][1] = v[k][2] = 0.0;
+      e[k] = -1;
+      if (a < n) {
+        v[k][0] = x[a];
+        v[k][1] = y[a];
+        v[k][2] = z[a];
+        e[k] = el[a];
+      }
     }
-    int e = el[a];
-    if (e >= 0 && e < n_elem) atomicAdd(&hist[e], 1u);
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < n_elem; e += BLK)
-    if (hist[e]) atomicAdd(&counts[e], (unsigned long long)hist[e]);
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double a = mn[k], b = mx[k];
+    for (int k = 0; k < PT; ++k) {
+      if (a0 + (int64_t)k * BLK >= n) break;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        mn[c] = fmin(mn[c], v[k][c]);
+        mx[c] = fmax(mx[c], v[k][c]);
+      }
+      if (e[k] >= 0 && e[k] < n_elem) atomicAdd(&hist[e[k]], 1u);
+    }
+  }
+  const int lane = threadIdx.x & (GX_WAVE - 1), w = threadIdx.x / GX_WAVE;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double a = mn[c], b = mx[c];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       a = fmin(a, __shfl_xor(a, o, GX_WAVE));
       b = fmax(b, __shfl_xor(b, o, GX_WAVE));
     }
-    if ((threadIdx.x & (GX_WAVE - 1)) == 0) {
-      atomicMin(&keys[2 * k], dkey(a));
-      atomicMax(&keys[2 * k + 1], dkey(b));
+    if (lane == 0) {
+      ext[w][2 * c] = a;
+      ext[w][2 * c + 1] = b;
     }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n_elem; e += BLK)
+    if (hist[e]) atomicAdd(&counts[e], (unsigned long long)hist[e]);
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    double v = ext[0][k];
+    for (int i = 1; i < NW; ++i) v = (k & 1) ? fmax(v, ext[i][k]) : fmin(v, ext[i][k]);
+    if (k & 1)
+      atomicMax(&keys[k], dkey(v));
+    else
+      atomicMin(&keys[k], dkey(v));
   }
 }
 
@@ -92,16 +123,17 @@ __device__ __forceinline__ int slab_row_of(const gx_slab &s, int64_t a, double r
   return row;
 }
 
-// Row counting and placement work on chunks of SORT_CHUNK atoms per block: counts
+// Row counting and placement work on chunks of PT * BLK atoms per block: counts
 // go to LDS histograms first and reach the global counters once per (block, row),
 // so a row shared by many atoms costs one global atomic per block, not per atom.
-constexpr int SORT_CHUNK = 2048;
-constexpr int SORT_PER_THREAD = SORT_CHUNK / BLK;
+template <int PT> constexpr int sort_chunk() { return PT * BLK; }
 constexpr int SORT_LDS_INTS = 16384;  // 64 KB of LDS histograms at most
 
 // LDS: [N + 1] row counts | [N * n_elem] per-row element counts when elem_lds
 // (otherwise the element counts go straight to the global counters).
+template <int PT>
 __global__ __launch_bounds__(BLK) void k_slab_rowcount(gx_slab s, int elem_lds) {
+  constexpr int SORT_CHUNK = sort_chunk<PT>();
   extern __shared__ int hist[];
   const int N = (int)s.N, M = N + 1;
   const int ne = (int)s.n_elem;
@@ -210,7 +242,9 @@ __global__ void k_slab_avg(const double *__restrict__ f_table, const int32_t *__
 // counts; the block then reserves one contiguous range per row with a single
 // global atomic on the row's cursor and writes the atoms there.
 // LDS: [N + 1] block counts | [N + 1] block bases.
+template <int PT>
 __global__ __launch_bounds__(BLK) void k_slab_scatter(gx_slab s) {
+  constexpr int SORT_CHUNK = sort_chunk<PT>(), SORT_PER_THREAD = PT;
   extern __shared__ int hist[];
   const int N = (int)s.N, M = N + 1;
   int *cnt = hist, *base = hist + M;
@@ -1867,9 +1901,11 @@ int gx_slab_bounds(const gx_slab *s, double *out_minmax, int64_t *out_elem_count
   k_slab_bounds_init<<<1, 64, 0, st>>>(keys, cnts, s->n_elem);
   GX_REQUIRE(s->n_elem >= 1 && s->n_elem <= MAX_ELEM, "n_elem=%lld outside [1, %d]", (long long)s->n_elem,
              MAX_ELEM);
-  unsigned g = grid1d(s->n_atoms, BLK * 16);
+  // four atoms per thread: 108 blocks for 110k atoms (16 per thread left 27 blocks
+  // waiting on their loads one after another)
+  unsigned g = grid1d(s->n_atoms, BLK * 4);
   if (g > 2048) g = 2048;
-  k_slab_bounds<<<g, BLK, 0, st>>>(s->x, s->y, s->z, s->elem, s->n_atoms, s->n_elem, keys, cnts);
+  k_slab_bounds<4><<<g, BLK, 0, st>>>(s->x, s->y, s->z, s->elem, s->n_atoms, s->n_elem, keys, cnts);
   k_slab_bounds_fin<<<1, 64, 0, st>>>(keys);
   return check_launch("gx_slab_bounds");
 }
@@ -1888,10 +1924,11 @@ int gx_slab_sort_rows(const gx_slab *s, gx_stream_t stream) {
   hipStream_t st = as_stream(stream);
   const int64_t nwork = s->N * s->n_elem + 2 * (s->N + 1);
   k_fill_i32<<<grid1d(nwork, BLK) > 4096 ? 4096 : grid1d(nwork, BLK), BLK, 0, st>>>(s->work, nwork, 0);
-  const unsigned g = grid1d(s->n_atoms, SORT_CHUNK);
+  constexpr int SPT = 2;  // atoms per thread of the row count and placement
+  const unsigned g = grid1d(s->n_atoms, sort_chunk<SPT>());
   const int elem_lds = (s->N + 1) + s->N * s->n_elem <= SORT_LDS_INTS;
   const size_t lds_count = (size_t)((s->N + 1) + (elem_lds ? s->N * s->n_elem : 0)) * sizeof(int);
-  k_slab_rowcount<<<g, BLK, lds_count, st>>>(*s, elem_lds);
+  k_slab_rowcount<SPT><<<g, BLK, lds_count, st>>>(*s, elem_lds);
   k_slab_scan<<<1, 1024, 0, st>>>(*s);
   if (s->sperm && s->stable_work) {
     const int64_t n_chunks = (s->n_atoms + GX_STABLE_CHUNK - 1) / GX_STABLE_CHUNK;
@@ -1900,7 +1937,7 @@ int gx_slab_sort_rows(const gx_slab *s, gx_stream_t stream) {
     k_slab_stable_scan<<<grid1d(s->N + 1, BLK), BLK, 0, st>>>(*s, n_chunks);
     k_slab_stable_scatter<<<(unsigned)n_chunks, BLK, lds, st>>>(*s);
   } else {
-    k_slab_scatter<<<g, BLK, 2 * (size_t)(s->N + 1) * sizeof(int), st>>>(*s);
+    k_slab_scatter<SPT><<<g, BLK, 2 * (size_t)(s->N + 1) * sizeof(int), st>>>(*s);
   }
   return check_launch("gx_slab_sort_rows");
 }
