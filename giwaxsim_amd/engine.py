@@ -1191,8 +1191,9 @@ def tuned_variant(runner: FrameRunner):
     deposit; each in frame-major or XCD-aware tile order) and radix plans for this
     runner's grid edge, precision, band width, atoms per row and atom count (whether the
     atoms fit one XCD's L2).  The shapes are timed with plan 0, then for each of the two
-    best shapes the row pass's plans and then the column pass's, each candidate twice
-    (best of two) on the runner's own first frames (into scratch bins, so the real bins
+    best shapes the row pass's plans and then the column pass's, each candidate five times
+    (best of five: the column plans differ by a few per cent of a batch's time, less than
+    one timing's spread) on the runner's own first frames (into scratch bins, so the real bins
     stay untouched); the fastest is cached.  All shapes give identical bins, the plans agree
     to rounding.  GIWAXSIM_AUTOTUNE=0 keeps shape 0 and plan 0."""
     import os
@@ -1220,7 +1221,7 @@ def tuned_variant(runner: FrameRunner):
         except _lib.GxError:
             return float("inf")  # shape does not fit this grid (LDS)
         best_t = float("inf")
-        for _ in range(2):
+        for _ in range(5):
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
