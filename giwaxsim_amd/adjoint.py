@@ -148,7 +148,9 @@ class DifferentiableSlab:
             t = lambda a: torch.as_tensor(a).to(dev).contiguous()
             _, ch, _, _ = chirpz_tables(N, self.f64)
             (rlo, rk), (clo, ck) = chirpz_bands(plan)
-            M, Kr, twr = chirpz_band_tables(N, rlo, rk, self.f64)
+            # the row band sized to the rows the cloud-in-cell deposit reaches (nodes up
+            # to max_voxels + 1, one row of rounding margin); the kernels check it per frame
+            M, Kr, twr = chirpz_band_tables(N, rlo, rk, self.f64, support=int(plan.max_voxels) + 3)
             Mc, Kc, twc = chirpz_band_tables(N, clo, ck, self.f64)
             self.bs = dict(M=M, Mc=Mc, rlo=rlo, rk=rk, clo=clo, ck=ck, chirp=t(ch), K=t(Kr), tw=t(twr),
                            Kc=t(Kc), twc=t(twc), Kadj=t(chirpz_adjoint_tables(N, rlo, rk, M, self.f64)),

@@ -831,7 +831,8 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_adj_rowpass_bs(gx_frames d, gx_a
   C *fbuf = reinterpret_cast<C *>(smem_raw);  // [M] transform exchange; aliased by the accumulators [2][N]
   AU *acc = reinterpret_cast<AU *>(fbuf);
   const int N = (int)d.N;
-  C *stage = fbuf + M;                                    // [W][BS_RB]
+  static_assert(2 * sizeof(AU) == sizeof(C), "the accumulators of a row take N elements of C");
+  C *stage = fbuf + (M > N ? M : N);                      // [W][BS_RB]
   C *tspec = stage + d.W * BS_RB;                         // [W]
   double *ftab = reinterpret_cast<double *>(tspec + d.W);  // [n_elem][2]
   const int tid = threadIdx.x;
@@ -840,7 +841,11 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_adj_rowpass_bs(gx_frames d, gx_a
   const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
   const int wf = jhi - jlo + 1;
   if (wf <= 0) return;  // block-uniform
-  if (wf > d.W || jlo < d.bs_rlo + N / 2 || jhi > d.bs_rlo + d.bs_rk - 1 + N / 2) {
+  // the band's convolution is sized to the rows the deposit reaches (nodes u0 and
+  // u0 + 1 <= y_max + 1 must stay below bs_M - bs_rk + 1)
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  if (wf > d.W || jlo < d.bs_rlo + N / 2 || jhi > d.bs_rlo + d.bs_rk - 1 + N / 2 ||
+      (bnd[1] != INT_MIN && bnd[1] + 1 > d.bs_M - d.bs_rk)) {
     if (tid == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
     return;
   }
@@ -852,7 +857,6 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_adj_rowpass_bs(gx_frames d, gx_a
   for (int i = tid; i < 2 * g.n_elem; i += GX_BLK_FFT) ftab[i] = g.f_table[i];
   const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
   const double ymin = dkey_inv(d.ymin_key[3 * fb]);
-  const int32_t *bnd = d.idx_bounds + 4 * fb;
   const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
   const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
   const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
@@ -1145,8 +1149,14 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_adj_rowgrad_bs(gx_frames d, gx_a
   const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
   const int wf = max(0, min(jhi - jlo + 1, (int)d.W));
   const int z0 = blockIdx.x * RB;
-  for (int i = tid; i < 2 * g.n_elem; i += GX_BLK_FFT) ftab[i] = g.f_table[i];
   const int32_t *bnd = d.idx_bounds + 4 * fb;
+  // outputs o < bs_M - bs_rk + 1 of the adjoint band transform are exact; the gather
+  // reads nodes u0 and u0 + 1 <= y_max + 1 (block-uniform, before any barrier)
+  if (bnd[1] != INT_MIN && bnd[1] + 1 > d.bs_M - d.bs_rk) {
+    if (tid == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
+    return;
+  }
+  for (int i = tid; i < 2 * g.n_elem; i += GX_BLK_FFT) ftab[i] = g.f_table[i];
   const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
   const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
   const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
@@ -1485,7 +1495,8 @@ int dispatch_n(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64
 template <int M, typename C>
 int launch_adj_bs_t(Op op, const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, hipStream_t st) {
   if (op == OP_ROWPASS) {
-    const size_t lds = (size_t)M * sizeof(C) + (size_t)d->W * (BS_RB + 1) * sizeof(C) + (size_t)g->n_elem * 16;
+    const size_t lds = (size_t)(M > d->N ? M : d->N) * sizeof(C) + (size_t)d->W * (BS_RB + 1) * sizeof(C) +
+                       (size_t)g->n_elem * 16;
     if (lds > 150 * 1024)
       return set_error(GX_ERR_UNSUPPORTED, "chirp-z differentiable row pass needs %zu bytes of LDS", lds);
     static bool attr_set = false;
@@ -1560,7 +1571,7 @@ int check_adj(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, b
     GX_REQUIRE(d->bs_chirp && d->bs_kernel && d->bs_twiddle && d->bs_kernel_c && d->bs_twiddle_c && d->twiddle64,
                "chirp-z grid edge N=%lld needs the band tables of gx_frames", (long long)d->N);
     GX_REQUIRE((d->bs_M == 1024 || d->bs_M == 2048 || d->bs_M == 4096) &&
-                   (d->bs_Mc == 1024 || d->bs_Mc == 2048 || d->bs_Mc == 4096) && d->bs_M >= d->N + d->bs_rk - 1 &&
+                   (d->bs_Mc == 1024 || d->bs_Mc == 2048 || d->bs_Mc == 4096) && d->bs_M >= d->bs_rk &&
                    d->bs_Mc >= d->N + d->bs_ck - 1,
                "chirp-z transform lengths (%lld, %lld) do not fit N=%lld", (long long)d->bs_M, (long long)d->bs_Mc,
                (long long)d->N);

@@ -1408,6 +1408,14 @@ __global__ __launch_bounds__(BLK) void k_frames_spectra_dft(gx_frames d, int64_t
 // Waves per SIMD the FP32 chirp-z row pass is compiled for, up to M = 2048: at 4 (128 VGPRs,
 // a 92-byte spill) a fit evaluation at N = 1048 measured ~10% faster than at 3 (155 VGPRs,
 // no spill); FP64 and M = 4096 keep the compiler's choice.
+// Elements of C at the start of the chirp-z row pass's LDS: the length-M transform
+// exchange, which the row's 2 N int32 accumulators alias (N may exceed M when the
+// band's convolution is sized to the rows the atoms reach).
+template <int M, typename C> __host__ __device__ constexpr int bs_row_fbuf(int N) {
+  return (2 * N * 4 + (int)sizeof(C) - 1) / (int)sizeof(C) > M ? (2 * N * 4 + (int)sizeof(C) - 1) / (int)sizeof(C)
+                                                                 : M;
+}
+
 template <int M, typename C> struct RowBsWaves { static constexpr int value = (sizeof(C) == 8 && M <= 2048) ? 4 : 1; };
 template <int M, typename C>
 __global__ __launch_bounds__(GX_BLK_FFT) __attribute__((amdgpu_waves_per_eu(RowBsWaves<M, C>::value)))
@@ -1422,16 +1430,20 @@ void k_rowpass_bs(gx_frames d, int64_t f0) {
   int32_t *acc_re = reinterpret_cast<int32_t *>(fbuf);
   const int N = (int)d.N;
   int32_t *acc_im = acc_re + N;
-  C *stage = fbuf + M;               // [W][RB]
-  C *tspec = stage + d.W * RB;       // [W]
+  C *stage = fbuf + bs_row_fbuf<M, C>(N);  // [W][RB]
+  C *tspec = stage + d.W * RB;             // [W]
   const int tid = threadIdx.x;
   const int fb = blockIdx.y;
   const double *fp = d.fparams + (f0 + fb) * GX_FP_STRIDE;
   const int jlo = (int)fp[GX_FP_JLO], jhi = (int)fp[GX_FP_JHI];
   const int wf = jhi - jlo + 1;
   if (wf <= 0) return;
-  // the planner sizes W and the chirp-z band to every frame's kept columns
-  if (wf > d.W || jlo < d.bs_rlo + N / 2 || jhi > d.bs_rlo + d.bs_rk - 1 + N / 2) {
+  // the planner sizes W and the chirp-z band to every frame's kept columns, and the
+  // band's convolution length to the y rows the frames' atoms reach: inputs at
+  // y >= bs_M - bs_rk + 1 would alias (and y >= bs_M would not be read at all)
+  const int32_t *bnd = d.idx_bounds + 4 * fb;
+  if (wf > d.W || jlo < d.bs_rlo + N / 2 || jhi > d.bs_rlo + d.bs_rk - 1 + N / 2 ||
+      (bnd[1] != INT_MIN && bnd[1] > d.bs_M - d.bs_rk)) {
     if (tid == 0 && blockIdx.x == 0) atomicAdd(&d.status[1], 1);
     return;
   }
@@ -1442,7 +1454,6 @@ void k_rowpass_bs(gx_frames d, int64_t f0) {
   }
   const double s = fp[GX_FP_SIN], c = fp[GX_FP_COS];
   const double ymin = dkey_inv(d.ymin_key[3 * fb]);
-  const int32_t *bnd = d.idx_bounds + 4 * fb;
   const int ylo = bnd[0], yhi = bnd[1], zlo = bnd[2], zhi = bnd[3];
   const int ya = d.fill_bkg ? (ylo == INT_MAX ? N : ylo + 1) : 0;
   const int yb = d.fill_bkg ? (yhi == INT_MIN ? 0 : yhi) : N;
@@ -1596,7 +1607,7 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_colpass_bs(gx_frames d, int64_t 
 template <int M, typename C>
 int launch_rowpass_bs_t(const gx_frames *d, int64_t f0, int64_t nf, hipStream_t st) {
   constexpr int RB = 8;
-  size_t lds = ((size_t)M + (size_t)d->W * (RB + 1)) * sizeof(C);
+  size_t lds = ((size_t)bs_row_fbuf<M, C>((int)d->N) + (size_t)d->W * (RB + 1)) * sizeof(C);
   if (lds > 160 * 1024)
     return set_error(GX_ERR_UNSUPPORTED, "chirp-z row pass needs %zu bytes of LDS (W=%lld)", lds, (long long)d->W);
   static bool attr_set = false;
@@ -1836,7 +1847,7 @@ int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
              "chirp-z tables missing");
   GX_REQUIRE(!d->bs_M || (d->bs_Mc == 1024 || d->bs_Mc == 2048 || d->bs_Mc == 4096),
              "chirp-z column length %lld", (long long)d->bs_Mc);
-  GX_REQUIRE(!d->bs_M || (d->bs_rk >= 1 && d->bs_rk <= d->N && d->bs_M >= d->N + d->bs_rk - 1 &&
+  GX_REQUIRE(!d->bs_M || (d->bs_rk >= 1 && d->bs_rk <= d->N && d->bs_M >= d->bs_rk &&
                           d->bs_rlo >= -(d->N / 2) && d->bs_rlo + d->bs_rk <= d->N - d->N / 2),
              "chirp-z row band [%lld, +%lld) does not fit M=%lld or the frequencies of N=%lld",
              (long long)d->bs_rlo, (long long)d->bs_rk, (long long)d->bs_M, (long long)d->N);

@@ -435,16 +435,21 @@ def is_pow2_grid(N):
 
 
 @functools.lru_cache(maxsize=32)
-def chirpz_band_tables(N: int, k_lo: int, K: int, f64: bool = False):
+def chirpz_band_tables(N: int, k_lo: int, K: int, f64: bool = False, support: int = None):
     """Zoomed chirp-z of a length-N DFT kept at the signed frequencies k = k_lo + j,
-    j < K: the length M >= N + K - 1 (a power of two, at least 1024), the kernel
+    j < K, for inputs that vanish at n >= ``support`` (default N): the length
+    M >= min(N, support) + K - 1 (a power of two, at least 1024), the kernel
     H = FFT_M(h) / M with h[t] = conj(c[k_lo + t]) = exp(+i pi ((k_lo + t)^2 mod 2N) / N)
-    for the lags t in [-(N-1), K) placed at t mod M (exact integer squares, FP64), and
-    the length-M twiddles."""
-    M = next((m for m in (1024, 2048, 4096) if m >= N + K - 1), None)
+    for the lags t in [-(L-1), K), L = min(N, M - K + 1), placed at t mod M (exact
+    integer squares, FP64), and the length-M twiddles.  The tables depend on (N, k_lo,
+    K, M) only and hold for any input supported on n < L: a row pass that uses them
+    must see no input at n >= M - K + 1."""
+    need = min(N, N if support is None else max(1, int(support)))
+    M = next((m for m in (1024, 2048, 4096) if m >= need + K - 1), None)
     if M is None:
         raise NotImplementedError(f"chirp-z band of {K} frequencies at N={N} needs a transform longer than 4096")
-    t = np.arange(-(N - 1), K, dtype=np.int64)
+    L = min(N, M - K + 1)
+    t = np.arange(-(L - 1), K, dtype=np.int64)
     m = np.abs(k_lo + t)
     ph = np.pi * ((m * m) % (2 * N)).astype(np.float64) / N
     b = np.zeros(M, dtype=np.complex128)
@@ -458,14 +463,16 @@ def chirpz_band_tables(N: int, k_lo: int, K: int, f64: bool = False):
 
 def chirpz_adjoint_tables(N: int, k_lo: int, K: int, M: int, f64: bool = False):
     """Kernel spectrum of the adjoint of the zoomed chirp-z band (the band DFT run from
-    the K band values k = k_lo + j to the N grid points o): with o k = (o^2 + k^2 -
+    the K band values k = k_lo + j to the grid points o): with o k = (o^2 + k^2 -
     (k - o)^2) / 2 the sum over j is a convolution in the lag t = o - j of
     a_j = u_{k_j} c[|k_j|] with h[t] = conj(c[k_lo - t]) = exp(+i pi ((k_lo - t)^2 mod 2N) / N),
-    t in [-(K-1), N), placed at t mod M; H = FFT_M(h) / M (exact integer squares, FP64).
-    M is the forward band's length (M >= N + K - 1)."""
-    if M < N + K - 1:
-        raise ValueError(f"chirp-z adjoint needs M >= N + K - 1 (M={M}, N={N}, K={K})")
-    t = np.arange(-(K - 1), N, dtype=np.int64)
+    t in [-(K-1), L), L = min(N, M - K + 1), placed at t mod M; H = FFT_M(h) / M (exact
+    integer squares, FP64).  M is the forward band's length: the outputs o < L are exact
+    (all N of them when M >= N + K - 1), the rows the forward band's input may reach."""
+    if M < K:
+        raise ValueError(f"chirp-z adjoint needs M >= K (M={M}, K={K})")
+    L = min(N, M - K + 1)
+    t = np.arange(-(K - 1), L, dtype=np.int64)
     m = np.abs(k_lo - t)
     ph = np.pi * ((m * m) % (2 * N)).astype(np.float64) / N
     b = np.zeros(M, dtype=np.complex128)
@@ -710,10 +717,14 @@ class FrameRunner:
     """Runs every phi-frame of one slab into shared bins, in batches."""
 
     def __init__(self, plan: FramePlan, slab: Slab, bins: Bins, fill_bkg, smooth, avg,
-                 batch=None, precision='auto', variant=None, counts=None, volume=None):
+                 batch=None, precision='auto', variant=None, counts=None, volume=None,
+                 bs_support=None):
         """``counts`` (atoms per element, all atoms) and ``volume`` (x_b y_b z_b): the
         fill and smoothing terms then use avg_voxel_f of the f values the fixed-point
-        deposit adds (gx_slab_avg, computed on the device) instead of ``avg``."""
+        deposit adds (gx_slab_avg, computed on the device) instead of ``avg``.
+        ``bs_support``: grid edges that are not powers of two only, the y rows the chirp-z
+        row band is sized for (default: the slab's projected width, max_voxels + 2); a
+        frame whose atoms reach beyond the tables' support stops the run with an error."""
         dev = device()
         N, W, Z = plan.N, plan.W, plan.Z
         self.plan, self.slab, self.bins = plan, slab, bins
@@ -733,12 +744,17 @@ class FrameRunner:
         else:
             _, ch, _, _ = chirpz_tables(N, self.f64)
             (rlo, rk), (clo, ck) = chirpz_bands(plan)
-            M, Kr, twr = chirpz_band_tables(N, rlo, rk, self.f64)
+            # a row's deposit reaches y indices up to (x_b^2 + y_b^2)^(1/2) / r (the
+            # projected width; max_voxels), so the row band's convolution can be shorter
+            # than N + K - 1 (the kernel checks each frame's y bound against the tables);
+            # the column band keeps every z (the filled rows outside the atoms' range)
+            support = int(plan.max_voxels) + 2 if bs_support is None else int(bs_support)
+            M, Kr, twr = chirpz_band_tables(N, rlo, rk, self.f64, support=support)
             Mc, Kc, twc = chirpz_band_tables(N, clo, ck, self.f64)
             self.bs_M = M
             dc = lambda key, a: dev_const(("chirpz", N, self.f64) + key, lambda: a)
-            self.bs = (dc(("c",), ch), dc(("K", rlo, rk), Kr), dc(("tw", M), twr))
-            self.bs_band = (rlo, rk, Mc, clo, ck, dc(("K", clo, ck), Kc), dc(("tw", Mc), twc))
+            self.bs = (dc(("c",), ch), dc(("K", rlo, rk, M), Kr), dc(("tw", M), twr))
+            self.bs_band = (rlo, rk, Mc, clo, ck, dc(("K", clo, ck, Mc), Kc), dc(("tw", Mc), twc))
         B = self.batch
         # per-frame statistics and column data for every frame (small: the statistics,
         # column masks and background spectra of all frames are computed in one launch
@@ -848,7 +864,8 @@ class FrameRunner:
             raise ValueError("zero-size array to reduction operation minimum which has no identity "
                              f"({int(s[0])} phi-frame(s) without any atom inside the grid)")
         if s[1]:
-            raise RuntimeError("internal: a frame's column band exceeded the planned width")
+            raise RuntimeError("internal: a frame's column band exceeded the planned width, or its "
+                               "atoms the y rows of the chirp-z row band")
 
 
 _WORKSPACES: dict = {}

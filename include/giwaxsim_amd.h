@@ -143,13 +143,16 @@ typedef struct gx_frames {
   const double *twiddle64; /* [N] the same as double2 (per-frame background spectra) */
   /* grid edges that are not powers of two: zoomed chirp-z (Bluestein) transforms.  The
      row pass evaluates the DFT only at k = bs_rlo + j, j < bs_rk (signed frequencies
-     covering every frame's kept columns), as a circular convolution of length
-     bs_M >= N + bs_rk - 1; the column pass likewise at k = bs_clo + j, j < bs_ck (the
-     kept rows [i_lo, i_hi]), length bs_Mc >= N + bs_ck - 1 */
+     covering every frame's kept columns), as a circular convolution of length bs_M,
+     exact for rows whose deposits stay below L = min(N, bs_M - bs_rk + 1) in y: bs_M
+     >= N + bs_rk - 1 covers every row; a shorter bs_M (sized to the slab's projected
+     width) is checked per frame (y_max of idx_bounds >= L sets status[1] and the frame
+     is skipped).  The column pass likewise at k = bs_clo + j, j < bs_ck (the kept rows
+     [i_lo, i_hi]), length bs_Mc >= N + bs_ck - 1 */
   int64_t bs_M;            /* 0: N is a power of two (in-wave transforms); else 1024, 2048 or 4096 */
   const void *bs_chirp;    /* [N] c[n] = exp(-i pi (n^2 mod 2N) / N) as float2 / double2 */
   const void *bs_kernel;   /* [bs_M] FFT of h[t] = conj(c[|bs_rlo + t|]), lags t in
-                              [-(N-1), bs_rk) at t mod bs_M, divided by bs_M */
+                              [-(L-1), bs_rk) at t mod bs_M, divided by bs_M */
   const void *bs_twiddle;  /* [bs_M] exp(-2 pi i m / bs_M) */
   int64_t n_frames;        /* frames described by fparams */
   const double *fparams;   /* [n_frames][GX_FP_STRIDE] */
@@ -462,7 +465,9 @@ typedef struct gx_adjoint {
   /* grid edges that are not powers of two (gx_frames.bs_M != 0): kernel spectra of the
      adjoint band transforms (band values -> grid points), same lengths and layouts as
      gx_frames.bs_kernel / bs_kernel_c: FFT_M(h) / M, h[t mod M] = exp(+i pi ((k_lo - t)^2 mod 2N) / N)
-     for t in [-(K-1), N), k_lo / K the row band (bs_rlo, bs_rk) or column band (bs_clo, bs_ck) */
+     for t in [-(K-1), L), L = min(N, M - K + 1), k_lo / K the row band (bs_rlo, bs_rk) or column
+     band (bs_clo, bs_ck); with a row band shorter than N + bs_rk - 1 every frame's cloud-in-cell
+     nodes (y_max + 1) must stay below bs_M - bs_rk + 1 (checked per frame: status[1]) */
   const void *bs_kernel_adj;
   const void *bs_kernel_adj_c;
 } gx_adjoint;

@@ -143,6 +143,77 @@ def test_chirpz_band_tables(N, k_lo, K):
     assert np.allclose(X, np.fft.fft(x)[k % N], rtol=1e-12, atol=1e-10)
 
 
+@pytest.mark.parametrize("N,k_lo,K,support", [(1048, -141, 283, 606), (1048, -95, 190, 190), (1500, -300, 600, 300),
+                                              (700, 5, 40, 700), (1048, -141, 283, 900)])
+def test_chirpz_band_tables_with_support(N, k_lo, K, support):
+    """Row bands sized to the rows the atoms reach: M is the shortest length with
+    M >= min(N, support) + K - 1, and the tables reproduce the band DFT exactly for any
+    input that vanishes at n >= L = min(N, M - K + 1) -- the bound the row pass checks
+    per frame (y_max <= M - K) -- while an input at n = L (when L < N) aliases."""
+    M, Kt, _ = engine.chirpz_band_tables(N, k_lo, K, True, support=support)
+    need = min(N, support) + K - 1
+    assert M in (1024, 2048, 4096) and M >= need and (M == 1024 or M // 2 < need)
+    L = min(N, M - K + 1)
+    assert L >= min(N, support)
+    _, ch, _, _ = engine.chirpz_tables(N, True)
+    c = ch[0::2] + 1j * ch[1::2]
+    H = Kt[0::2] + 1j * Kt[1::2]
+    k = k_lo + np.arange(K)
+    rng = np.random.default_rng(N + K + support)
+
+    def band(x):
+        a = np.zeros(M, complex)
+        n = min(len(x), M)
+        assert not np.any(x[n:])
+        a[:n] = (x * c)[:n]
+        conv = np.conj(np.fft.fft(np.conj(np.fft.fft(a) * H)))
+        return c[np.abs(k)] * conv[:K]
+
+    x = np.zeros(N, complex)
+    x[:L] = rng.normal(size=L) + 1j * rng.normal(size=L)
+    assert np.allclose(band(x), np.fft.fft(x)[k % N], rtol=1e-12, atol=1e-10)
+    if L < N and L < M:
+        y = np.zeros(N, complex)
+        y[L] = 1.0
+        assert not np.allclose(band(y), np.fft.fft(y)[k % N], rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("N,k_lo,K,support", [(1048, -141, 283, 606), (1048, -95, 190, 191), (1500, -300, 600, 300)])
+def test_chirpz_adjoint_tables_with_support(N, k_lo, K, support):
+    """With the row band sized to the rows the deposit reaches (M < N + K - 1), the
+    adjoint band transform is exact at the outputs o < L = min(N, M - K + 1), the rows
+    the forward input may occupy, and the adjoint identity holds for inputs there."""
+    M, Kf, _ = engine.chirpz_band_tables(N, k_lo, K, True, support=support)
+    assert M < N + K - 1
+    L = min(N, M - K + 1)
+    Ha = engine.chirpz_adjoint_tables(N, k_lo, K, M, True)
+    _, ch, _, _ = engine.chirpz_tables(N, True)
+    c = ch[0::2] + 1j * ch[1::2]
+    H = Ha[0::2] + 1j * Ha[1::2]
+    Hf = Kf[0::2] + 1j * Kf[1::2]
+    rng = np.random.default_rng(N * 3 + K)
+    k = k_lo + np.arange(K)
+    u = rng.normal(size=K) + 1j * rng.normal(size=K)
+    a = np.zeros(M, complex)
+    a[:K] = u * c[np.abs(k)]
+    conv = np.conj(np.fft.fft(np.conj(np.fft.fft(a) * H)))
+    o = np.arange(L)
+    y = c[o] * conv[:L]
+    want = np.exp(-2j * np.pi * np.outer(o, k % N) / N) @ u
+    assert np.allclose(y, want, rtol=1e-11, atol=1e-9 * np.abs(want).max())
+    # <S F x, u> = <x, S F^H u> for x on the rows [0, L)
+    x = np.zeros(N, complex)
+    x[:L] = rng.normal(size=L) + 1j * rng.normal(size=L)
+    af = np.zeros(M, complex)
+    af[:L] = x[:L] * c[:L]
+    X = c[np.abs(k)] * np.conj(np.fft.fft(np.conj(np.fft.fft(af) * Hf)))[:K]
+    assert np.allclose(X, np.fft.fft(x)[k % N], rtol=1e-12, atol=1e-10)
+    a2 = np.zeros(M, complex)
+    a2[:K] = np.conj(u) * c[np.abs(k)]
+    xbar = np.conj(c[o] * np.conj(np.fft.fft(np.conj(np.fft.fft(a2) * H)))[:L])
+    assert np.isclose(np.vdot(u, X), np.vdot(xbar, x[:L]), rtol=1e-11)
+
+
 @pytest.mark.parametrize("N,k_lo,K", [(131, -30, 61), (300, -150, 300), (1048, -95, 190),
                                       (1048, -141, 283), (700, 5, 40)])
 def test_chirpz_adjoint_tables(N, k_lo, K):

@@ -102,7 +102,36 @@ def test_full_small(name):
                                          ('polymer1048_fill_smooth', (200, 210))])
 def test_frame_subsets(name, frames):
     _, _, ref, res = _run_both(name, frames)
+    if name == 'polymer1048_fill_smooth':
+        # the row band's convolution sized to the slab's projected width (166 rows):
+        # length 1024 instead of the 2048 that N + K - 1 = 1334 would need
+        assert res['runner'].bs_M == 1024
     _check(ref, res)
+
+
+def test_chirpz_row_band_too_short_is_an_error():
+    """A chirp-z row band shorter than the rows a frame's atoms reach is refused per
+    frame (status, then RuntimeError), never binned with aliased rows: a 250 A long slab
+    at N = 1048 needs length 2048 at phi near 90 degrees; forced to 1024 (support 100
+    rows, so the tables hold rows < 1024 - K + 1) the run stops with the error."""
+    coords, el = graphite_slab(size=(250.0, 10.0, 10.0), seed=11)
+    ids, ftab, _, _ = engine.element_table(el, 12700.0)
+    g = engine.qgrid(0.02, 2.0)
+    mid = len(g.phis) // 2
+    print(f"[gpu-frames] chirp-z support check, {len(coords)} atoms, frames {mid - 4}..{mid + 4}", flush=True)
+    res = engine.run_voxelgrid(coords, el, 0.3, 0.02, 2.0, 12700.0, f_table=ftab, elem_ids=ids,
+                               frames=(mid - 4, mid + 4))
+    plan = res['plan']
+    assert plan.N == 1048 and res['runner'].bs_M == 2048 and plan.max_voxels > 1024 - 287
+    short = engine.FrameRunner(plan, res['slab'], res['bins'], False, 0, res['avg'], bs_support=100)
+    assert short.bs_M == 1024
+    short.run((mid - 4, mid + 4))
+    with pytest.raises(RuntimeError, match="chirp-z row band"):
+        short.check()
+    # frames near phi = 0 project the slab onto its 10 A width and pass
+    short.status.zero_()
+    short.run((0, 4))
+    short.check()
 
 
 def test_frame_stats_bit_exact():
