@@ -408,7 +408,12 @@ class DifferentiableSlab:
 
     def _check_status(self):
         s = self.status.cpu().numpy()
-        if s[1] or s[2]:
+        if s[1]:
+            raise RuntimeError(f"differentiable path: {int(s[1])} frame(s) outside the planned bands (a "
+                               "frame's kept columns, or its atoms' y rows beyond the chirp-z row band "
+                               "sized to the slab the model was built for): rebuild the model for the "
+                               f"new slab extent (status {s.tolist()})")
+        if s[2]:
             raise RuntimeError(f"internal: differentiable path status {s.tolist()}")
 
     def _forward(self, coords, f_re, f_im):

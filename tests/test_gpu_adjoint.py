@@ -368,3 +368,30 @@ def test_production_grids_match_autograd(N, precision, tol_img, tol_grad):
     _say(f"  image {e_img:.2e}  d/dcoords {e_c:.2e}  d/df_re {e_fr:.2e}  d/df_im {e_fi:.2e}")
     assert e_img <= tol_img
     assert e_c <= tol_grad and e_fr <= tol_grad and e_fi <= tol_grad
+
+
+def test_chirpz_row_band_support_is_checked():
+    """The differentiable path's chirp-z row band is sized to the slab the model was built
+    for (here 188 rows of projected width: length 1024 at N = 1048, rows below 738
+    covered).  Called with atoms stretched to ~250 A along x, frames near phi = 90 degrees
+    reach ~830 rows: the kernels skip those frames and the call raises with the reason,
+    instead of returning an image with aliased rows."""
+    from giwaxsim_amd.engine import qgrid
+    from giwaxsim_amd.synth import polymer_slab
+    coords, el = polymer_slab(size=(40.0, 40.0, 30.0), seed=31)
+    det = dict(num_pixels=64, max_q=2.0, psis=[0.0, 45.0], phis=[0.0], thetas=[0.0],
+               angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
+    mid = len(qgrid(0.02, 2.0).phis) // 2
+    frames = (mid - 1, mid + 2)
+    _say(f"[gpu-adjoint] chirp-z row band support check N=1048, frames {frames}")
+    model = DifferentiableSlab(el, coords, 0.3, 0.02, 2.0, det, frames=frames, precision='fp32',
+                               fill_bkg=True, smooth=5)
+    assert model.plan.N == 1048 and model.bs['M'] == 1024
+    dev = torch.device('cuda')
+    fr, fi = model.f_re0.clone(), model.f_im0.clone()
+    img = model(torch.tensor(coords, dtype=torch.float64, device=dev), fr, fi)
+    assert torch.isfinite(img).all()
+    wide = coords.copy()
+    wide[:, 0] = wide[:, 0] * 6.0
+    with pytest.raises(RuntimeError, match="chirp-z row band"):
+        model(torch.tensor(wide, dtype=torch.float64, device=dev), fr, fi)
