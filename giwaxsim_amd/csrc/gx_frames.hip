@@ -1570,16 +1570,22 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_colpass_bs(gx_frames d, int64_t 
   if (!d.fill_bkg && d.smooth) u = reinterpret_cast<const C *>(d.col_spec)[(int64_t)fb * d.W + wcol];
   const double *sz_row = d.col_sz + (int64_t)fb * N;
   const C *chirp = reinterpret_cast<const C *>(d.bs_chirp);
+  // A band shorter than N + bs_ck - 1 takes the rows z < Lc = bs_Mc - bs_ck + 1 (>= Z,
+  // checked on the host): rows >= Z are zero in every column but the k = 0 column with
+  // fill, whose constant A N on the rows [Lc, N) is added below in closed form.
+  const int Lc = (int)min((int64_t)N, d.bs_Mc - d.bs_ck + 1);
   C v[P];
 #pragma unroll
   for (int t = 0; t < P / R; ++t)
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int z = tid + GX_BLK_FFT * t + q * (M / R);
-      v[t * R + q] = (z < N) ? cmul(col_input(d, col, z, k0, u, sz_row), chirp[z]) : C{(T)0, (T)0};
+      v[t * R + q] = (z < Lc) ? cmul(col_input(d, col, z, k0, u, sz_row), chirp[z]) : C{(T)0, (T)0};
     }
   chirpz_block<M, C>(v, fbuf, reinterpret_cast<const C *>(d.bs_kernel_c),
                      reinterpret_cast<const C *>(d.bs_twiddle_c), tid);
+  const bool tail = k0 && d.fill_bkg && Lc < N;
+  const double tre = d.avg_re * d.max_voxels * (double)N, tim = d.avg_im * d.max_voxels * (double)N;
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;
   const double val = d.q_val;
   const int nn = (N - 1) / 2 + 1;  // fftfreq: k < nn are the non-negative frequencies
@@ -1597,7 +1603,27 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_colpass_bs(gx_frames d, int64_t 
       const double qz = (((double)kint * val) * 2.0) * M_PI;
       const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
       if (bz < 0 || bz >= d.c_n) continue;
-      const double I = (double)cnorm(cmul(v[t * R + q], chirp[k < 0 ? -k : k]));
+      C F = cmul(v[t * R + q], chirp[k < 0 ? -k : k]);
+      if (tail) {
+        // sum_{z = Lc}^{N-1} A N exp(-2 pi i z k / N) = A N (N - Lc) at k = 0, else
+        // A N (exp(-2 pi i Lc k / N) - 1) / (1 - exp(-2 pi i k / N)); angles reduced in integers
+        double gre, gim;
+        if (kk == 0) {
+          gre = tre * (double)(N - Lc);
+          gim = tim * (double)(N - Lc);
+        } else {
+          double s1, c1, sL, cL;
+          sincos(-2.0 * M_PI * (double)kk / (double)N, &s1, &c1);
+          sincos(-2.0 * M_PI * (double)(((int64_t)Lc * kk) % N) / (double)N, &sL, &cL);
+          const double nre = cL - 1.0, nim = sL, dre = 1.0 - c1, dim = -s1;
+          const double den = dre * dre + dim * dim;
+          const double qre = (nre * dre + nim * dim) / den, qim = (nim * dre - nre * dim) / den;
+          gre = tre * qre - tim * qim;
+          gim = tre * qim + tim * qre;
+        }
+        F = C{(T)((double)F.x + gre), (T)((double)F.y + gim)};
+      }
+      const double I = (double)cnorm(F);
       const unsigned long long fx = (unsigned long long)llrint(I * d.bin_scale);
       atomicAdd(&d.bin_sum[cbase + bz], fx);
       atomicAdd(&d.bin_cnt[cbase + bz], 1);
@@ -1851,7 +1877,14 @@ int check_frames(const gx_frames *d, int64_t f0, int64_t nf) {
                           d->bs_rlo >= -(d->N / 2) && d->bs_rlo + d->bs_rk <= d->N - d->N / 2),
              "chirp-z row band [%lld, +%lld) does not fit M=%lld or the frequencies of N=%lld",
              (long long)d->bs_rlo, (long long)d->bs_rk, (long long)d->bs_M, (long long)d->N);
-  GX_REQUIRE(!d->bs_M || (d->bs_ck >= 1 && d->bs_ck <= d->N && d->bs_Mc >= d->N + d->bs_ck - 1 &&
+  // a column band shorter than N + bs_ck - 1 must still take every row z < Z, and is exact only
+  // where rows >= Z are zero in every column but the fill constant of the k = 0 column
+  GX_REQUIRE(!d->bs_M || d->bs_Mc >= d->N + d->bs_ck - 1 ||
+                 (d->bs_Mc - d->bs_ck + 1 >= d->Z && (d->fill_bkg || !d->smooth)),
+             "chirp-z column band of length %lld covers rows below %lld, Z=%lld (and needs fill_bkg "
+             "or no smoothing when shorter than N + bs_ck - 1)", (long long)d->bs_Mc,
+             (long long)(d->bs_Mc - d->bs_ck + 1), (long long)d->Z);
+  GX_REQUIRE(!d->bs_M || (d->bs_ck >= 1 && d->bs_ck <= d->N && d->bs_Mc >= d->bs_ck &&
                           d->bs_clo >= -(d->N / 2) && d->bs_clo + d->bs_ck <= d->N - d->N / 2 &&
                           d->bs_clo + d->N / 2 <= d->i_lo && d->bs_clo + d->bs_ck - 1 + d->N / 2 >= d->i_hi),
              "chirp-z column band [%lld, +%lld) does not fit M=%lld or cover rows [%lld, %lld]",

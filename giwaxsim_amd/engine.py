@@ -746,11 +746,14 @@ class FrameRunner:
             (rlo, rk), (clo, ck) = chirpz_bands(plan)
             # a row's deposit reaches y indices up to (x_b^2 + y_b^2)^(1/2) / r (the
             # projected width; max_voxels), so the row band's convolution can be shorter
-            # than N + K - 1 (the kernel checks each frame's y bound against the tables);
-            # the column band keeps every z (the filled rows outside the atoms' range)
+            # than N + K - 1 (the kernel checks each frame's y bound against the tables)
             support = int(plan.max_voxels) + 2 if bs_support is None else int(bs_support)
             M, Kr, twr = chirpz_band_tables(N, rlo, rk, self.f64, support=support)
-            Mc, Kc, twc = chirpz_band_tables(N, clo, ck, self.f64)
+            # along z only the Z rows that hold atoms carry data, unless smoothing without fill
+            # spreads the background over every row; the k = 0 column's fill constant on
+            # the rows beyond the band's support is added in closed form by the column pass
+            Mc, Kc, twc = chirpz_band_tables(N, clo, ck, self.f64,
+                                             support=Z if (fill_bkg or not smooth) else None)
             self.bs_M = M
             dc = lambda key, a: dev_const(("chirpz", N, self.f64) + key, lambda: a)
             self.bs = (dc(("c",), ch), dc(("K", rlo, rk, M), Kr), dc(("tw", M), twr))

@@ -148,7 +148,10 @@ typedef struct gx_frames {
      >= N + bs_rk - 1 covers every row; a shorter bs_M (sized to the slab's projected
      width) is checked per frame (y_max of idx_bounds >= L sets status[1] and the frame
      is skipped).  The column pass likewise at k = bs_clo + j, j < bs_ck (the kept rows
-     [i_lo, i_hi]), length bs_Mc >= N + bs_ck - 1 */
+     [i_lo, i_hi]), length bs_Mc >= N + bs_ck - 1, or shorter when it still takes every
+     row z < Z (bs_Mc - bs_ck + 1 >= Z) and fill_bkg is set or smoothing is off: rows >= Z
+     then carry nothing but the k = 0 column's fill constant, whose transform on the rows
+     the band leaves out is added in closed form */
   int64_t bs_M;            /* 0: N is a power of two (in-wave transforms); else 1024, 2048 or 4096 */
   const void *bs_chirp;    /* [N] c[n] = exp(-i pi (n^2 mod 2N) / N) as float2 / double2 */
   const void *bs_kernel;   /* [bs_M] FFT of h[t] = conj(c[|bs_rlo + t|]), lags t in

@@ -178,6 +178,40 @@ def test_chirpz_band_tables_with_support(N, k_lo, K, support):
         assert not np.allclose(band(y), np.fft.fft(y)[k % N], rtol=1e-6, atol=1e-6)
 
 
+@pytest.mark.parametrize("N,k_lo,K,Z", [(1048, -100, 202, 624), (1048, -100, 202, 136), (1500, -300, 600, 300)])
+def test_chirpz_column_band_with_fill_tail(N, k_lo, K, Z):
+    """The column pass's short band for fill_bkg: the k = 0 column holds data on z < Z and
+    the fill constant A on every row z >= Z; the band takes the rows z < Lc = M - K + 1
+    (>= Z) and the constant on [Lc, N) is added in closed form as the kernel does,
+    A (exp(-2 pi i Lc k / N) - 1) / (1 - exp(-2 pi i k / N)) (A (N - Lc) at k = 0), with
+    the angles reduced in integers.  Equals the full DFT of the column."""
+    M, Kt, _ = engine.chirpz_band_tables(N, k_lo, K, True, support=Z)
+    Lc = min(N, M - K + 1)
+    assert Z <= Lc < N and M < N + K - 1
+    _, ch, _, _ = engine.chirpz_tables(N, True)
+    c = ch[0::2] + 1j * ch[1::2]
+    H = Kt[0::2] + 1j * Kt[1::2]
+    rng = np.random.default_rng(N + Z)
+    A = complex(rng.normal() * 50, rng.normal() * 50)
+    x = np.full(N, A)
+    x[:Z] = rng.normal(size=Z) + 1j * rng.normal(size=Z)
+    a = np.zeros(M, complex)
+    a[:Lc] = x[:Lc] * c[:Lc]
+    k = k_lo + np.arange(K)
+    F = c[np.abs(k)] * np.conj(np.fft.fft(np.conj(np.fft.fft(a) * H)))[:K]
+    kk = np.where(k < 0, k + N, k)
+    s1, c1 = np.sin(-2 * np.pi * kk / N), np.cos(-2 * np.pi * kk / N)
+    m = (Lc * kk) % N
+    sL, cL = np.sin(-2 * np.pi * m / N), np.cos(-2 * np.pi * m / N)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        q = ((cL - 1.0) + 1j * sL) / ((1.0 - c1) - 1j * s1)
+    tail = np.where(kk == 0, A * (N - Lc), A * q)
+    want = np.fft.fft(x)[k % N]
+    assert np.allclose(F + tail, want, rtol=1e-11, atol=1e-9 * np.abs(want).max())
+    # without the tail the short band misses the constant rows (the term is needed)
+    assert not np.allclose(F, want, rtol=1e-6, atol=1e-6 * np.abs(want).max())
+
+
 @pytest.mark.parametrize("N,k_lo,K,support", [(1048, -141, 283, 606), (1048, -95, 190, 191), (1500, -300, 600, 300)])
 def test_chirpz_adjoint_tables_with_support(N, k_lo, K, support):
     """With the row band sized to the rows the deposit reaches (M < N + K - 1), the

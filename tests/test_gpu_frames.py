@@ -104,8 +104,10 @@ def test_frame_subsets(name, frames):
     _, _, ref, res = _run_both(name, frames)
     if name == 'polymer1048_fill_smooth':
         # the row band's convolution sized to the slab's projected width (166 rows):
-        # length 1024 instead of the 2048 that N + K - 1 = 1334 would need
-        assert res['runner'].bs_M == 1024
+        # length 1024 instead of the 2048 that N + K - 1 = 1334 would need; the column
+        # band sized to the Z = 136 rows that hold atoms (the k = 0 column's fill
+        # constant beyond it added in closed form): 1024 instead of 2048
+        assert res['runner'].bs_M == 1024 and res['runner'].desc.bs_Mc == 1024
     _check(ref, res)
 
 
