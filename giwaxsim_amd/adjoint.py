@@ -151,7 +151,10 @@ class DifferentiableSlab:
             # the row band sized to the rows the cloud-in-cell deposit reaches (nodes up
             # to max_voxels + 1, one row of rounding margin); the kernels check it per frame
             M, Kr, twr = chirpz_band_tables(N, rlo, rk, self.f64, support=int(plan.max_voxels) + 3)
-            Mc, Kc, twc = chirpz_band_tables(N, clo, ck, self.f64)
+            # the column band sized to the rows that hold atoms (with the batch buffers' 16
+            # rows of slack) unless smoothing without fill spreads the background below N
+            zs = min(N, self._rows_for(zb) + 16) if (self.fill or not self.smooth) else None
+            Mc, Kc, twc = chirpz_band_tables(N, clo, ck, self.f64, support=zs)
             self.bs = dict(M=M, Mc=Mc, rlo=rlo, rk=rk, clo=clo, ck=ck, chirp=t(ch), K=t(Kr), tw=t(twr),
                            Kc=t(Kc), twc=t(twc), Kadj=t(chirpz_adjoint_tables(N, rlo, rk, M, self.f64)),
                            Kadj_c=t(chirpz_adjoint_tables(N, clo, ck, Mc, self.f64)))

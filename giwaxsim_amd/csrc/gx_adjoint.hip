@@ -989,6 +989,10 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_adj_colpass_bs(gx_frames d, gx_a
   const double *sz_row = d.col_sz + (int64_t)fb * N;
   const C *chirp = reinterpret_cast<const C *>(d.bs_chirp);
   const C *tw = reinterpret_cast<const C *>(d.bs_twiddle_c);
+  // a band shorter than N + bs_ck - 1 takes the rows z < Lc (>= Z) and the k = 0 column's
+  // fill constant beyond them in closed form, as the forward column pass (k_colpass_bs)
+  const int Lc = (int)min((int64_t)N, d.bs_Mc - d.bs_ck + 1);
+  const bool tail = k0 && d.fill_bkg && Lc < N;
   C v[P];
 #pragma unroll
   for (int t = 0; t < P / R; ++t)
@@ -996,7 +1000,7 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_adj_colpass_bs(gx_frames d, gx_a
     for (int q = 0; q < R; ++q) {
       const int z = tid + GX_BLK_FFT * t + q * (M / R);
       C e = C{(T)0, (T)0};
-      if (z < N) e = cmul(col_input(d, (const C *)col, z, k0, u, sz_row), chirp[z]);
+      if (z < Lc) e = cmul(col_input(d, (const C *)col, z, k0, u, sz_row), chirp[z]);
       v[t * R + q] = e;
     }
   chirpz_block<M, C>(v, fbuf, reinterpret_cast<const C *>(d.bs_kernel_c), tw, tid);
@@ -1023,7 +1027,11 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_adj_colpass_bs(gx_frames d, gx_a
           const long long bz = to_int(floor_div(qz - d.qmin, d.dq, dqinv)) - d.c_lo;
           if (bz >= 0 && bz < d.c_n) {
             const double gI = 2.0 * g.g_bins[cbase + bz];
-            const C F = cmul(v[t * R + q], ck);
+            C F = cmul(v[t * R + q], ck);
+            if (tail) {
+              const double2 gt = chirpz_fill_tail(d, kk, Lc);
+              F = C{(T)((double)F.x + gt.x), (T)((double)F.y + gt.y)};
+            }
             gr = gI * (double)F.x;
             gi = gI * (double)F.y;
           }
@@ -1572,7 +1580,8 @@ int check_adj(const gx_frames *d, const gx_adjoint *g, int64_t f0, int64_t nf, b
                "chirp-z grid edge N=%lld needs the band tables of gx_frames", (long long)d->N);
     GX_REQUIRE((d->bs_M == 1024 || d->bs_M == 2048 || d->bs_M == 4096) &&
                    (d->bs_Mc == 1024 || d->bs_Mc == 2048 || d->bs_Mc == 4096) && d->bs_M >= d->bs_rk &&
-                   d->bs_Mc >= d->N + d->bs_ck - 1,
+                   (d->bs_Mc >= d->N + d->bs_ck - 1 ||
+                    (d->bs_Mc - d->bs_ck + 1 >= d->Z && (d->fill_bkg || !d->smooth))),
                "chirp-z transform lengths (%lld, %lld) do not fit N=%lld", (long long)d->bs_M, (long long)d->bs_Mc,
                (long long)d->N);
   } else {

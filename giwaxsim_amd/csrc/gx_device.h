@@ -660,6 +660,23 @@ __device__ __forceinline__ C col_input(const gx_frames &d, const C *col, int z, 
     }                              \
   } while (0)
 
+// Transform of the fill constant A N (A = avg_voxel_f max_voxels) on the rows [Lc, N) of
+// the k = 0 column at natural frequency kk, which a chirp-z column band shorter than
+// N + K - 1 leaves out: A N (N - Lc) at kk = 0, else A N (w^Lc - 1) / (1 - w) with
+// w = exp(-2 pi i kk / N), FP64, the angles reduced in integers.  d.avg_* resolved.
+__device__ __forceinline__ double2 chirpz_fill_tail(const gx_frames &d, int kk, int Lc) {
+  const int N = (int)d.N;
+  const double tre = d.avg_re * d.max_voxels * (double)N, tim = d.avg_im * d.max_voxels * (double)N;
+  if (kk == 0) return make_double2(tre * (double)(N - Lc), tim * (double)(N - Lc));
+  double s1, c1, sL, cL;
+  sincos(-2.0 * M_PI * (double)kk / (double)N, &s1, &c1);
+  sincos(-2.0 * M_PI * (double)(((int64_t)Lc * kk) % N) / (double)N, &sL, &cL);
+  const double nre = cL - 1.0, nim = sL, dre = 1.0 - c1, dim = -s1;
+  const double den = dre * dre + dim * dim;
+  const double qre = (nre * dre + nim * dim) / den, qim = (nim * dre - nre * dim) / den;
+  return make_double2(tre * qre - tim * qim, tre * qim + tim * qre);
+}
+
 // numpy linspace element i of n: i * step + start, the last one exactly stop.
 __device__ __forceinline__ double lin_axis(int64_t i, int64_t n, double start, double step, double stop) {
   return (i == n - 1) ? stop : (double)i * step + start;

@@ -1585,7 +1585,6 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_colpass_bs(gx_frames d, int64_t 
   chirpz_block<M, C>(v, fbuf, reinterpret_cast<const C *>(d.bs_kernel_c),
                      reinterpret_cast<const C *>(d.bs_twiddle_c), tid);
   const bool tail = k0 && d.fill_bkg && Lc < N;
-  const double tre = d.avg_re * d.max_voxels * (double)N, tim = d.avg_im * d.max_voxels * (double)N;
   const int64_t cbase = (by * d.c_n + bx) * d.c_n;
   const double val = d.q_val;
   const int nn = (N - 1) / 2 + 1;  // fftfreq: k < nn are the non-negative frequencies
@@ -1605,23 +1604,8 @@ __global__ __launch_bounds__(GX_BLK_FFT) void k_colpass_bs(gx_frames d, int64_t 
       if (bz < 0 || bz >= d.c_n) continue;
       C F = cmul(v[t * R + q], chirp[k < 0 ? -k : k]);
       if (tail) {
-        // sum_{z = Lc}^{N-1} A N exp(-2 pi i z k / N) = A N (N - Lc) at k = 0, else
-        // A N (exp(-2 pi i Lc k / N) - 1) / (1 - exp(-2 pi i k / N)); angles reduced in integers
-        double gre, gim;
-        if (kk == 0) {
-          gre = tre * (double)(N - Lc);
-          gim = tim * (double)(N - Lc);
-        } else {
-          double s1, c1, sL, cL;
-          sincos(-2.0 * M_PI * (double)kk / (double)N, &s1, &c1);
-          sincos(-2.0 * M_PI * (double)(((int64_t)Lc * kk) % N) / (double)N, &sL, &cL);
-          const double nre = cL - 1.0, nim = sL, dre = 1.0 - c1, dim = -s1;
-          const double den = dre * dre + dim * dim;
-          const double qre = (nre * dre + nim * dim) / den, qim = (nim * dre - nre * dim) / den;
-          gre = tre * qre - tim * qim;
-          gim = tre * qim + tim * qre;
-        }
-        F = C{(T)((double)F.x + gre), (T)((double)F.y + gim)};
+        const double2 gt = chirpz_fill_tail(d, kk, Lc);
+        F = C{(T)((double)F.x + gt.x), (T)((double)F.y + gt.y)};
       }
       const double I = (double)cnorm(F);
       const unsigned long long fx = (unsigned long long)llrint(I * d.bin_scale);

@@ -353,8 +353,10 @@ def test_production_grids_match_autograd(N, precision, tol_img, tol_grad):
                                fill_bkg=True, smooth=smooth)
     assert model.plan.N == N
     if N == 1048:
-        # the row band sized to the slab's projected width (188 rows): 1024, not 2048
-        assert model.bs['M'] == 1024
+        # the row band sized to the slab's projected width (188 rows) and the column band
+        # to its 120 rows (the k = 0 column's fill constant beyond them in closed form):
+        # 1024 each, not 2048
+        assert model.bs['M'] == 1024 and model.bs['Mc'] == 1024
     dev = torch.device('cuda')
     c = torch.tensor(coords, dtype=torch.float64, device=dev, requires_grad=True)
     fr = model.f_re0.clone().requires_grad_(True)
