@@ -470,7 +470,8 @@ typedef struct gx_adjoint {
      gx_frames.bs_kernel / bs_kernel_c: FFT_M(h) / M, h[t mod M] = exp(+i pi ((k_lo - t)^2 mod 2N) / N)
      for t in [-(K-1), L), L = min(N, M - K + 1), k_lo / K the row band (bs_rlo, bs_rk) or column
      band (bs_clo, bs_ck); with a row band shorter than N + bs_rk - 1 every frame's cloud-in-cell
-     nodes (y_max + 1) must stay below bs_M - bs_rk + 1 (checked per frame: status[1]) */
+     nodes (y_max + 1) must stay below bs_M - bs_rk + 1 (checked per frame: status[1]); a shorter
+     column band follows the forward path's rule (rows z < Z covered, fill_bkg or no smoothing) */
   const void *bs_kernel_adj;
   const void *bs_kernel_adj_c;
 } gx_adjoint;
