@@ -11,7 +11,8 @@ import oracle as O
 from giwaxsim_amd import engine
 
 # (r, q_voxel, max_q) -> N, q_num, frames, crop edge, computed independently of
-# this code from the reference's formulas (survey appendix A).
+# this code from the reference's formulas (tools/comparison.py:705-731 for the grid
+# and frames, tools/voxelgrids.py:16-48 for the crop).
 SIZES = [
     ((0.25, 2 * np.pi / (0.25 * 512) * (1 + 1e-9), 2.5), (512, 145, 453, 103)),
     ((0.2, 2 * np.pi / (0.2 * 1024) * (1 + 1e-9), 2.0), (1024, 185, 581, 131)),
