@@ -46,6 +46,12 @@ def _case(name):
     if name == 'polymer1048_fill_smooth':  # the reference's example settings (N = 1048 = 8 x 131)
         c, e = polymer_slab(size=(40.0, 30.0, 40.0), seed=7)
         return c, e, dict(r=0.3, qv=0.02, mq=2.0, fill=True, smooth=5)
+    if name == 'graphite1048':         # N = 1048 without fill or smoothing: FP64, both bands short, no tail
+        c, e = graphite_slab(size=(30.0, 30.0, 20.0), seed=9)
+        return c, e, dict(r=0.3, qv=0.02, mq=2.0, fill=False, smooth=0)
+    if name == 'polymer1048_smooth':   # smoothing without fill: the column band keeps every row
+        c, e = polymer_slab(size=(40.0, 30.0, 40.0), seed=10)
+        return c, e, dict(r=0.3, qv=0.02, mq=2.0, fill=False, smooth=5)
     raise KeyError(name)
 
 
@@ -99,9 +105,17 @@ def test_full_small(name):
                                          ('polymer1024_smooth', (100, 116)),
                                          ('graphite300', (0, 40)),
                                          ('polymer700_fill', (30, 50)),
-                                         ('polymer1048_fill_smooth', (200, 210))])
+                                         ('polymer1048_fill_smooth', (200, 210)),
+                                         ('graphite1048', (300, 306)),
+                                         ('polymer1048_smooth', (500, 506))])
 def test_frame_subsets(name, frames):
     _, _, ref, res = _run_both(name, frames)
+    if name == 'graphite1048':
+        d = res['runner'].desc
+        assert res['runner'].f64 and d.bs_M == 1024 and d.bs_Mc == 1024 and d.bs_Mc < 1048 + d.bs_ck - 1
+    if name == 'polymer1048_smooth':
+        d = res['runner'].desc
+        assert d.bs_M == 1024 and d.bs_Mc == 2048
     if name == 'polymer1048_fill_smooth':
         # the row band's convolution sized to the slab's projected width (166 rows):
         # length 1024 instead of the 2048 that N + K - 1 = 1334 would need; the column
