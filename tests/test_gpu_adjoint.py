@@ -397,3 +397,47 @@ def test_chirpz_row_band_support_is_checked():
     wide[:, 0] = wide[:, 0] * 6.0
     with pytest.raises(RuntimeError, match="chirp-z row band"):
         model(torch.tensor(wide, dtype=torch.float64, device=dev), fr, fi)
+
+
+@pytest.mark.parametrize("smooth", [0, 5])
+def test_chirpz_short_bands_without_fill_match_autograd(smooth):
+    """N = 1048 (the fit geometry's edge) without fill: the row band is sized to the slab's
+    projected width (1024); without smoothing the column band is short too (rows >= Z are
+    empty, no fill tail), with smoothing only it keeps every row (2048).  FP64 transforms
+    against the FP64 autograd reference on three frames, the bars of the production grids."""
+    from giwaxsim_amd.engine import element_table
+    from giwaxsim_amd.synth import polymer_slab
+    coords, el = polymer_slab(size=(30.0, 30.0, 20.0), seed=41)
+    r, qv, mq = 0.3, 0.02, 2.0
+    det = dict(num_pixels=64, max_q=mq, psis=list(np.arange(0.0, 91.0, 15.0)), phis=[0.0],
+               thetas=[0.0], angle_init_vals=(0.15,), angle_init_axs=('psi',), mirror=True)
+    frames = (100, 103)
+    _say(f"[gpu-adjoint] N=1048 short bands, no fill, smooth {smooth}, {len(coords)} atoms, frames {frames}")
+    uniq = sorted(set(el))
+    ids = np.array([uniq.index(e) for e in el])
+    _, ftab, _, _ = element_table(el, 12700.0)
+    ref = CicModel(coords, ids, r, qv, mq, det, frames=frames, fill_bkg=False, smooth=smooth)
+    ct = torch.tensor(coords, dtype=torch.float64, requires_grad=True)
+    frt = torch.tensor(ftab.real.copy(), requires_grad=True)
+    fit = torch.tensor(ftab.imag.copy(), requires_grad=True)
+    G = np.random.default_rng(9).normal(size=(64, 64))
+    want = ref(ct, frt, fit)
+    (want * torch.as_tensor(G)).sum().backward()
+    model = DifferentiableSlab(el, coords, r, qv, mq, det, frames=frames, precision='fp64',
+                               fill_bkg=False, smooth=smooth)
+    assert model.plan.N == 1048 and model.bs['M'] == 1024
+    assert model.bs['Mc'] == (2048 if smooth else 1024)
+    dev = torch.device('cuda')
+    c = torch.tensor(coords, dtype=torch.float64, device=dev, requires_grad=True)
+    fr = model.f_re0.clone().requires_grad_(True)
+    fi = model.f_im0.clone().requires_grad_(True)
+    img = model(c, fr, fi)
+    (img * torch.as_tensor(G, device=dev)).sum().backward()
+    w = want.detach().numpy()
+    e_img = np.max(np.abs(img.detach().cpu().numpy() - w)) / np.max(np.abs(w))
+    e_c = np.max(np.abs(c.grad.cpu().numpy() - ct.grad.numpy())) / np.max(np.abs(ct.grad.numpy()))
+    e_fr = np.max(np.abs(fr.grad.cpu().numpy() - frt.grad.numpy())) / np.max(np.abs(frt.grad.numpy()))
+    e_fi = np.max(np.abs(fi.grad.cpu().numpy() - fit.grad.numpy())) / np.max(np.abs(fit.grad.numpy()))
+    _say(f"  image {e_img:.2e}  d/dcoords {e_c:.2e}  d/df_re {e_fr:.2e}  d/df_im {e_fi:.2e}")
+    assert e_img <= 1e-10
+    assert e_c <= 1e-10 and e_fr <= 1e-10 and e_fi <= 1e-10
