@@ -38,14 +38,27 @@ except ImportError:  # pragma: no cover - torch is part of the platform
 # --------------------------------------------------------------------------------------
 # device plumbing
 # --------------------------------------------------------------------------------------
+_DEVICE_SEEN = False
+# torch's raw current-stream getter (0.3 us against 2.8 us for current_stream(), which
+# builds a Stream object); the public call where a build lacks it
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None) if torch is not None else None
+
+
 def device():
-    """The HIP device; raises if none is visible (there is no CPU path)."""
-    if torch is None or not torch.cuda.is_available():
-        raise RuntimeError("giwaxsim_amd runs on a ROCm GPU (HIP device) and none is visible")
+    """The HIP device; raises if none is visible (there is no CPU path).  Once a device
+    has been seen the check is not repeated (a process does not lose its GPU)."""
+    global _DEVICE_SEEN
+    if not _DEVICE_SEEN:
+        if torch is None or not torch.cuda.is_available():
+            raise RuntimeError("giwaxsim_amd runs on a ROCm GPU (HIP device) and none is visible")
+        _DEVICE_SEEN = True
     return torch.device("cuda")
 
 
 def stream_handle():
+    """torch's current stream on the current device, as the C ABI's gx_stream_t."""
+    if _RAW_STREAM is not None:
+        return ctypes.c_void_p(_RAW_STREAM(torch.cuda.current_device()))
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
@@ -921,8 +934,7 @@ def run_voxelgrid(coords, elements, r_voxel_size, q_voxel_size, max_q, energy, f
     if columns is None:
         columns = device_columns(coords, elem_ids)
     key = (_Ident(plan), int(columns[0].shape[0]), len(f_table), bool(fill_bkg), int(smooth),
-           precision, batch, variant, torch.cuda.current_device(),
-           torch.cuda.current_stream().cuda_stream)
+           precision, batch, variant, torch.cuda.current_device(), stream_handle().value)
     ws = _WORKSPACES.pop(key, None)
     if ws is None:
         bins = Bins(plan.grid, scale)
