@@ -180,11 +180,12 @@ constexpr int AU_UNROLL = 4;
 // Rows per block of the differentiable row pass: RB rows of accumulators in 32 KB of
 // LDS (at most 8), so that three blocks share a CU; at N = 1024 FP32 two rows (16 KB:
 // more blocks per CU outweigh the atoms of row z0 - 1 read by twice as many blocks: the
-// forward of the 110k-atom slab's 581 frames 2.08 -> 1.97 ms; eight rows: 2.42 ms)
+// forward of the 110k-atom slab's 581 frames 2.08 -> 1.97 ms; eight rows: 2.42 ms), at
+// N = 512 FP32 four (the 5.6k-atom graphite forward 0.217 -> 0.210 ms; two: 0.225 ms)
 template <int N, typename C>
 struct RowPassShape {
   static constexpr int FIT = 32768 / (N * (int)sizeof(C));
-  static constexpr int RB = (N == 1024 && sizeof(C) == 8) ? 2 : (FIT < 1 ? 1 : (FIT < 8 ? FIT : 8));
+  static constexpr int RB = (N == 1024 && sizeof(C) == 8) ? 2 : (N == 512 && sizeof(C) == 8) ? 4 : (FIT < 1 ? 1 : (FIT < 8 ? FIT : 8));
 };
 
 // One block = RB consecutive z rows of one frame.  All 256 threads deposit the atoms
